@@ -1,0 +1,215 @@
+"""Headline benchmark: closest-hit rays per second through the BVH on one MI355X.
+
+Workload (closest-hit on the 1.8M-triangle scene): a 512x512 fBm heightfield plus 1,000 subdivision-3
+icospheres (1,804,288 triangles), 4,194,304 pinhole rays at 2048x2048 and 4,194,304 cosine-hemisphere
+bounce rays from their hits. One step traces both ray sets through the BVH (8,388,608 closest-hit rays);
+rays, BVH and hit buffers are resident in HBM before the timed region. The 4,194,304 shadow rays are
+traced with the any-hit kernel and reported beside the headline.
+
+Prints one JSON line (rank 0). Options: --gpus (only 1 is supported), --steps, --warmup,
+--cpu-seconds (budget of the CPU baseline sample), --no-cpu-baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_HBM_GBPS = 8000.0
+CLOSEST_BYTES_PER_RAY = 48   # 32 B ray in + 16 B hit out
+ANY_BYTES_PER_RAY = 33       # 32 B ray in + 1 B flag out
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(tris, rays, budget_s):
+    """the oracle's brute-force closest hit (restatement of src/intersection.cpp:114-129 in a loop), timed
+    on this host on a bounded random subset of the primary + bounce rays, extrapolated to rays/s"""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as orc
+    nthreads = min(16, os.cpu_count() or 1)
+    rng = np.random.default_rng(0)
+    probe = rays[rng.choice(len(rays), 256, replace=False)]
+    t0 = time.perf_counter()
+    orc.brute_closest(tris, probe, nthreads)
+    dt = time.perf_counter() - t0
+    n = int(min(16384, max(256, 256 * budget_s / max(dt, 1e-6))))
+    sample = rays[np.sort(rng.choice(len(rays), n, replace=False))]
+    t0 = time.perf_counter()
+    orc.brute_closest(tris, sample, nthreads)
+    dt = time.perf_counter() - t0
+    return {"value": n / dt / 1e9, "unit": "Grays/s", "cores": nthreads, "kind": "port",
+            "sample": f"{n} random closest-hit rays of the workload, brute force over all {len(tris)} triangles "
+                      f"({n * len(tris):.3e} ray-triangle tests in {dt:.1f} s, {nthreads} threads), "
+                      "extrapolated to rays/s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verify-rays", type=int, default=65536)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (rays shard with no exchange; "
+              "see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+
+    log("generating scene")
+    pos, idx = W.config2_scene()
+    tris = W.flat_triangles(pos, idx)
+    primary = W.pinhole_rays(2048, 2048)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ctx = vk.Context(0, stream=stream.cuda_stream)
+
+    log("building BVH")
+    t0 = time.perf_counter()
+    bvh = ctx.bvh_build(pos, idx)
+    build_wall_ms = (time.perf_counter() - t0) * 1e3
+    info = bvh.info()
+
+    def to_dev(a):
+        return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8)).to(dev)
+
+    d_primary = to_dev(primary).view(torch.float32)
+    d_hits_p = torch.empty(len(primary) * 4, dtype=torch.float32, device=dev)
+    ctx.trace_closest(bvh, d_primary, out=d_hits_p)
+    hits_p = d_hits_p.cpu().numpy().view(vk.HIT_DTYPE)
+    log("generating secondary rays")
+    bounce, shadow = W.secondary_rays(primary, hits_p, tris)
+    d_bounce = to_dev(bounce).view(torch.float32)
+    d_shadow = to_dev(shadow).view(torch.float32)
+    d_hits_b = torch.empty(len(bounce) * 4, dtype=torch.float32, device=dev)
+    d_occ = torch.empty(len(shadow), dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+
+    # parity check of a subset against GPU brute force (not timed)
+    hits_b = None
+    if args.verify_rays:
+        log("verifying a subset against brute force")
+        ctx.trace_closest(bvh, d_bounce, out=d_hits_b)
+        hits_b = d_hits_b.cpu().numpy().view(vk.HIT_DTYPE)
+        sel = np.sort(np.random.default_rng(1).choice(len(primary), args.verify_rays, replace=False))
+        mismatch = 0
+        for rays_np, hits_np in ((primary, hits_p), (bounce, hits_b)):
+            ref = ctx.brute_closest(tris, rays_np[sel])
+            got = hits_np[sel]
+            mismatch += int(np.sum((ref["prim"] != got["prim"]) | (ref["t"].view(np.uint32) != got["t"].view(np.uint32))))
+        verified = {"rays": 2 * args.verify_rays, "mismatches": mismatch}
+    else:
+        verified = None
+
+    flags = vk.ASYNC
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+
+    def step(timed):
+        if timed:
+            ev[0].record(stream)
+        ctx.trace_closest(bvh, d_primary, out=d_hits_p, flags=flags)
+        if timed:
+            ev[1].record(stream)
+        ctx.trace_closest(bvh, d_bounce, out=d_hits_b, flags=flags)
+        if timed:
+            ev[2].record(stream)
+
+    log(f"warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    log(f"timing {args.steps} steps")
+    prim_ms, bounce_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+        ev[2].synchronize()
+        prim_ms.append(ev[0].elapsed_time(ev[1]))
+        bounce_ms.append(ev[1].elapsed_time(ev[2]))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the per-step host sync above only reads events; report the event total as the step time
+    ms_per_step = float(np.mean(np.array(prim_ms) + np.array(bounce_ms)))
+    rays_per_step = len(primary) + len(bounce)
+    value = rays_per_step / (ms_per_step * 1e-3) / 1e9
+
+    # any-hit shadow rays (reported beside the headline)
+    any_ms = []
+    for _ in range(max(3, args.steps // 2)):
+        ev[3].record(stream)
+        ctx.trace_any(bvh, d_shadow, out=d_occ, flags=flags)
+        ev[4].record(stream)
+        ev[4].synchronize()
+        any_ms.append(ev[3].elapsed_time(ev[4]))
+    any_ms = float(np.mean(any_ms))
+
+    launch_ms = float(np.mean(prim_ms + bounce_ms))
+    achieved = CLOSEST_BYTES_PER_RAY * len(primary) / (launch_ms * 1e-3) / 1e9
+    result = {
+        "metric": "Grays/s closest-hit (1 GPU)",
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "closest-hit on the 1.8M-triangle scene: 512x512 fBm heightfield + 1,000 subdiv-3 "
+                        "icospheres; 4,194,304 pinhole (2048x2048) + 4,194,304 cosine bounce rays per step",
+            "n_tri": int(len(tris)),
+            "rays_per_step": int(rays_per_step),
+            "bvh": {"builder": "lbvh", "max_leaf": 4, "nodes": int(info.n_nodes), "depth": int(info.depth),
+                    "sah_cost": round(float(info.sah_cost), 2), "build_ms": round(float(info.build_ms), 2),
+                    "build_wall_ms": round(build_wall_ms, 2)},
+            "primary_ms": round(float(np.mean(prim_ms)), 4),
+            "bounce_ms": round(float(np.mean(bounce_ms)), 4),
+            "primary_grays_per_s": round(len(primary) / (np.mean(prim_ms) * 1e-3) / 1e9, 4),
+            "bounce_grays_per_s": round(len(bounce) / (np.mean(bounce_ms) * 1e-3) / 1e9, 4),
+            "any_hit_shadow_ms": round(any_ms, 4),
+            "any_hit_grays_per_s": round(len(shadow) / (any_ms * 1e-3) / 1e9, 4),
+            "verified_vs_brute_force": verified,
+            "timed_wall_s": round(wall, 3),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_trace_closest",
+            "achieved": round(achieved, 3),
+            "peak": MI355X_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / MI355X_HBM_GBPS, 6),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": CLOSEST_BYTES_PER_RAY * len(primary),
+            "avg_launch_ms": round(launch_ms, 4),
+        },
+        "cpu_baseline": None,
+    }
+    ctx.sync()
+    bvh.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        allrays = np.concatenate([primary, bounce])
+        result["cpu_baseline"] = cpu_baseline(tris, allrays, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,188 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference-compiled helpers in
+oracle/_ref/libvkref.so. Test infrastructure: only tests/, __graft_entry__.smoke() and the cpu_baseline
+leg of bench.py use it, and only as the checker."""
+import ctypes
+import os
+import pathlib
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+ORACLE = ROOT / "oracle" / "liboracle.so"
+REF = ROOT / "oracle" / "_ref" / "libvkref.so"
+
+_vp, _u32, _i32, _f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float
+_fp, _dp = ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_double)
+
+_SIG = {
+    "orc_intersect_tri": (_i32, [_vp, _vp, _vp, _fp, _fp, _fp]),
+    "orc_intersect_tri_f64": (_i32, [_vp, _vp, _vp, _dp, _dp, _dp]),
+    "orc_intersect_obb": (_i32, [_vp, _vp, _vp, _fp]),
+    "orc_intersect_aabb": (_i32, [_vp, _vp, _vp, _fp]),
+    "orc_intersect_sphere": (_i32, [_vp, _vp, _vp, _fp]),
+    "orc_intersect_plane": (_i32, [_vp, _vp, _vp, _fp]),
+    "orc_compute_aabb": (None, [_vp, _u32, _vp]),
+    "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
+    "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
+    "orc_closest_f64": (_u32, [_vp, _u32, _vp, _dp, _dp, _dp]),
+    "orc_icosphere_size": (None, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "orc_icosphere": (None, [_f32, _u32, _vp, _vp]),
+    "orc_plane": (None, [_f32, _f32, _u32, _u32, _vp, _vp]),
+    "orc_xxhash32": (_u32, [_u32, _u32]),
+    "orc_lcg": (_u32, [ctypes.POINTER(_u32)]),
+    "orc_rnd": (_f32, [ctypes.POINTER(_u32)]),
+    "orc_hammersley": (None, [_u32, _u32, _vp]),
+}
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE.exists():
+            raise RuntimeError(f"{ORACLE} missing: run `make -C oracle`")
+        h = ctypes.CDLL(str(ORACLE))
+        for name, (res, args) in _SIG.items():
+            getattr(h, name).restype = res
+            getattr(h, name).argtypes = args
+        _lib = h
+    return _lib
+
+
+def ref():
+    """reference-compiled triangle_intersection.h helpers, or None when oracle/_ref was not built"""
+    global _ref
+    if _ref is None and REF.exists():
+        h = ctypes.CDLL(str(REF))
+        h.ref_tri_box_overlap.restype = _u32
+        h.ref_tri_box_overlap.argtypes = [_vp, _vp]
+        h.ref_tri_tri_overlap.restype = _u32
+        h.ref_tri_tri_overlap.argtypes = [_vp, _vp]
+        _ref = h
+    return _ref
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data
+
+
+def threads():
+    return int(os.environ.get("ORACLE_THREADS", min(16, os.cpu_count() or 1)))
+
+
+def intersect_tri(tri9, o, d):
+    tri9, pt = _f(tri9)
+    o, po = _f(o)
+    d, pd = _f(d)
+    t, u, v = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+    r = lib().orc_intersect_tri(pt, po, pd, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return (r, t.value, u.value, v.value) if r else (0, 0.0, 0.0, 0.0)
+
+
+def intersect_tri_f64(tri9, o, d):
+    tri9, pt = _f(tri9)
+    o, po = _f(o)
+    d, pd = _f(d)
+    t, u, v = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    r = lib().orc_intersect_tri_f64(pt, po, pd, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return (r, t.value, u.value, v.value) if r else (0, 0.0, 0.0, 0.0)
+
+
+def _volume(fn, prim, o, d):
+    prim, pp = _f(prim)
+    o, po = _f(o)
+    d, pd = _f(d)
+    dist = ctypes.c_float()
+    r = fn(pp, po, pd, ctypes.byref(dist))
+    return (r, dist.value) if r else (0, 0.0)
+
+
+def intersect_aabb(box6, o, d):
+    return _volume(lib().orc_intersect_aabb, box6, o, d)
+
+
+def intersect_obb(obb15, o, d):
+    return _volume(lib().orc_intersect_obb, obb15, o, d)
+
+
+def intersect_sphere(s4, o, d):
+    return _volume(lib().orc_intersect_sphere, s4, o, d)
+
+
+def intersect_plane(p4, o, d):
+    return _volume(lib().orc_intersect_plane, p4, o, d)
+
+
+def ray_od(rays):
+    o = np.stack([rays["ox"], rays["oy"], rays["oz"]], axis=1).astype(np.float32)
+    d = np.stack([rays["dx"], rays["dy"], rays["dz"]], axis=1).astype(np.float32)
+    return o, d
+
+
+def compute_aabb(points):
+    points, pp = _f(points)
+    out = np.zeros(6, dtype=np.float32)
+    lib().orc_compute_aabb(pp, len(points) // 3 if points.ndim == 1 else len(points), out.ctypes.data)
+    return out
+
+
+def brute_closest(tris_flat, rays, nthreads=None):
+    from vierkant_amd import HIT_DTYPE
+    tris_flat, pt = _f(tris_flat)
+    rays = np.ascontiguousarray(rays)
+    out = np.zeros(len(rays), dtype=HIT_DTYPE)
+    lib().orc_brute_closest(pt, tris_flat.size // 9, rays.ctypes.data, len(rays), out.ctypes.data,
+                            nthreads or threads())
+    return out
+
+
+def brute_any(tris_flat, rays, nthreads=None):
+    tris_flat, pt = _f(tris_flat)
+    rays = np.ascontiguousarray(rays)
+    out = np.zeros(len(rays), dtype=np.uint8)
+    lib().orc_brute_any(pt, tris_flat.size // 9, rays.ctypes.data, len(rays), out.ctypes.data,
+                        nthreads or threads())
+    return out
+
+
+def closest_f64(tris_flat, ray):
+    tris_flat, pt = _f(tris_flat)
+    ray = np.ascontiguousarray(np.asarray(ray).reshape(1))
+    t, u, v = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    prim = lib().orc_closest_f64(pt, tris_flat.size // 9, ray.ctypes.data, ctypes.byref(t), ctypes.byref(u),
+                                 ctypes.byref(v))
+    return prim, t.value, u.value, v.value
+
+
+def icosphere(radius, subdiv):
+    nv, ni = _u32(), _u32()
+    lib().orc_icosphere_size(subdiv, ctypes.byref(nv), ctypes.byref(ni))
+    pos = np.zeros((nv.value, 3), dtype=np.float32)
+    idx = np.zeros(ni.value, dtype=np.uint32)
+    lib().orc_icosphere(radius, subdiv, pos.ctypes.data, idx.ctypes.data)
+    return pos, idx
+
+
+def plane(w, h, sw, sh):
+    pos = np.zeros(((sw + 1) * (sh + 1), 3), dtype=np.float32)
+    idx = np.zeros(sw * sh * 6, dtype=np.uint32)
+    lib().orc_plane(w, h, sw, sh, pos.ctypes.data, idx.ctypes.data)
+    return pos, idx
+
+
+def xxhash32(a, b):
+    return lib().orc_xxhash32(a, b)
+
+
+def rnd_sequence(state, n):
+    s = _u32(state)
+    return [lib().orc_rnd(ctypes.byref(s)) for _ in range(n)], s.value
+
+
+def hammersley(i, n):
+    out = np.zeros(2, dtype=np.float32)
+    lib().orc_hammersley(i, n, out.ctypes.data)
+    return out

@@ -1,0 +1,143 @@
+"""GPU parity of BVH traversal: closest and any hit through the BVH equal the brute-force loop over the
+reference test (bit-exact records, lowest index on ties), for every builder setting, edge-case meshes,
+refit, and the 1.8M-triangle heightfield scene."""
+import numpy as np
+import pytest
+
+import oracle_binding as orc
+import vierkant_amd as vk
+from parity import assert_hits_equal
+from vierkant_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def config1():
+    pos, idx, rays = W.config1()
+    tris = W.flat_triangles(pos, idx)
+    return pos, idx, tris, rays, orc.brute_closest(tris, rays)
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_bvh_closest_config1_bit_exact(ctx, config1, leaf):
+    pos, idx, tris, rays, want = config1
+    bvh = ctx.bvh_build(pos, idx, max_leaf_size=leaf)
+    info = bvh.info()
+    assert info.n_tri == len(idx) // 3 and 1 < info.depth <= 64
+    got = ctx.trace_closest(bvh, rays)
+    assert_hits_equal(got, want, f"bvh leaf={leaf}")
+
+
+def test_bvh_any_config1(ctx, config1):
+    pos, idx, tris, rays, _ = config1
+    bvh = ctx.bvh_build(pos, idx)
+    r = rays.copy()
+    rng = np.random.default_rng(20)
+    r["tmax"] = rng.uniform(0.0, 6.0, len(r)).astype(np.float32)
+    r["tmin"] = rng.uniform(-0.5, 0.5, len(r)).astype(np.float32)
+    want = orc.brute_any(tris, r)
+    got = ctx.trace_any(bvh, r)
+    assert np.array_equal(got, want)
+    # the same rays, closest hit with the interval [tmin, tmax]
+    assert_hits_equal(ctx.trace_closest(bvh, r), orc.brute_closest(tris, r), "bvh closest windowed")
+
+
+def test_bvh_duplicate_triangles_tie_break(ctx):
+    rng = np.random.default_rng(21)
+    pos = rng.uniform(-1, 1, (300, 3)).astype(np.float32)
+    idx = rng.integers(0, 300, (150, 3)).astype(np.uint32)
+    idx = np.concatenate([idx, idx[::-1], idx]).reshape(-1)      # each triangle three times
+    tris = W.flat_triangles(pos, idx)
+    o = rng.uniform(-3, 3, (5000, 3)).astype(np.float32)
+    d = W.glm_normalize(rng.normal(size=(5000, 3)).astype(np.float32))
+    rays = W.make_rays(o, d)
+    want = orc.brute_closest(tris, rays)
+    assert np.sum(want["prim"] != 0xFFFFFFFF) > 200
+    for leaf in (1, 3, 8):
+        bvh = ctx.bvh_build(pos, idx, max_leaf_size=leaf)
+        assert_hits_equal(ctx.trace_closest(bvh, rays), want, f"dups leaf={leaf}")
+
+
+@pytest.mark.parametrize("n_tri", [0, 1, 2, 3, 7, 9, 100])
+def test_bvh_small_meshes(ctx, n_tri):
+    rng = np.random.default_rng(22 + n_tri)
+    pos = rng.uniform(-1, 1, (max(3 * n_tri, 1), 3)).astype(np.float32)
+    idx = np.arange(3 * n_tri, dtype=np.uint32)
+    tris = W.flat_triangles(pos, idx) if n_tri else np.zeros((0, 9), np.float32)
+    o = rng.uniform(-3, 3, (2000, 3)).astype(np.float32)
+    d = W.glm_normalize(rng.normal(size=(2000, 3)).astype(np.float32))
+    rays = W.make_rays(o, d)
+    bvh = ctx.bvh_build(pos, idx)
+    assert_hits_equal(ctx.trace_closest(bvh, rays), orc.brute_closest(tris, rays), f"n_tri={n_tri}")
+    assert np.array_equal(ctx.trace_any(bvh, rays), orc.brute_any(tris, rays))
+
+
+def test_bvh_degenerate_and_nonfinite(ctx):
+    rng = np.random.default_rng(23)
+    pos = rng.uniform(-1, 1, (600, 3)).astype(np.float32)
+    pos[0:3] = pos[0]                       # a point triangle
+    pos[3:6, 1] = 0.5                       # a flat triangle in y
+    pos[9] = [np.inf, 0, 0]
+    pos[12] = [np.nan, 0, 0]
+    idx = np.arange(600, dtype=np.uint32)
+    tris = W.flat_triangles(pos, idx)
+    o = rng.uniform(-3, 3, (4000, 3)).astype(np.float32)
+    d = W.glm_normalize(rng.normal(size=(4000, 3)).astype(np.float32))
+    d[:10] = [0, 0, 1]                       # axis-aligned rays
+    d[10:20] = [0, 1, 0]
+    rays = W.make_rays(o, d)
+    bvh = ctx.bvh_build(pos, idx, max_leaf_size=2)
+    assert_hits_equal(ctx.trace_closest(bvh, rays), orc.brute_closest(tris, rays), "degenerate")
+
+
+def test_bvh_bad_index_is_an_error(ctx):
+    pos = np.zeros((3, 3), np.float32)
+    idx = np.array([0, 1, 3], np.uint32)
+    with pytest.raises(vk.VkrtError) as e:
+        ctx.bvh_build(pos, idx)
+    assert e.value.code == vk.E_INVALID_ARG
+
+
+def test_bvh_stride(ctx, config1):
+    pos, idx, tris, rays, want = config1
+    padded = np.zeros((len(pos), 8), np.float32)
+    padded[:, :3] = pos
+    bvh = ctx.bvh_build(padded, idx, stride=32)
+    assert_hits_equal(ctx.trace_closest(bvh, rays[:4096]), want[:4096], "stride 32")
+
+
+def test_bvh_refit(ctx, config1):
+    pos, idx, tris, rays, want = config1
+    bvh = ctx.bvh_build(pos, idx)
+    moved = (pos * np.float32(1.3) + np.float32(0.1)).astype(np.float32)
+    bvh.refit(moved)
+    tris2 = W.flat_triangles(moved, idx)
+    want2 = orc.brute_closest(tris2, rays[:8192])
+    assert_hits_equal(ctx.trace_closest(bvh, rays[:8192]), want2, "refit")
+
+
+@pytest.fixture(scope="module")
+def config2():
+    pos, idx = W.config2_scene()
+    return pos, idx, W.flat_triangles(pos, idx)
+
+
+def test_bvh_config2_primary_and_bounce(ctx, config2):
+    """1,804,288 triangles: 65,536 primary and 65,536 bounce rays through the BVH equal GPU brute force
+    (itself bit-exact with the oracle on config 1), and 1,024 of them equal the oracle directly"""
+    pos, idx, tris = config2
+    bvh = ctx.bvh_build(pos, idx)
+    primary_all = W.pinhole_rays(2048, 2048)
+    sel = np.random.default_rng(24).choice(len(primary_all), 65536, replace=False)
+    primary = primary_all[np.sort(sel)]
+    hits = ctx.trace_closest(bvh, primary)
+    assert np.mean(hits["prim"] != 0xFFFFFFFF) > 0.5
+    assert_hits_equal(hits, ctx.brute_closest(tris, primary), "config2 primary")
+    bounce, shadow = W.secondary_rays(primary, hits, tris)
+    bh = ctx.trace_closest(bvh, bounce)
+    assert_hits_equal(bh, ctx.brute_closest(tris, bounce), "config2 bounce")
+    assert np.array_equal(ctx.trace_any(bvh, shadow), ctx.brute_any(tris, shadow))
+    few = np.arange(0, 65536, 64)
+    assert_hits_equal(hits[few], orc.brute_closest(tris, primary[few]), "config2 vs oracle")
+    assert_hits_equal(bh[few], orc.brute_closest(tris, bounce[few]), "config2 bounce vs oracle")

@@ -1,0 +1,294 @@
+"""vierkant_amd - MI355X (gfx950) ray-intersection and path-tracing core behind a C ABI.
+
+This module is the Python binding of ``libvkrt.so`` (declared in ``include/vkrt.h``). It loads the
+in-tree shared library and raises if it is missing: there is no CPU fallback. Arrays are numpy arrays
+(host memory, staged by the library) or torch tensors on the GPU (passed as device pointers).
+"""
+import ctypes
+import pathlib
+
+import numpy as np
+
+PKG = pathlib.Path(__file__).resolve().parent
+LIB_PATH = PKG / "libvkrt.so"
+
+OK, E_INVALID_ARG, E_OOM, E_HIP, E_UNSUPPORTED, E_NO_DEVICE, E_INTERNAL = 0, -1, -2, -3, -4, -5, -6
+HOST_PTRS, DEVICE_PTRS, ASYNC = 0, 1, 2
+REJECT, INTERSECT, INSIDE = 0, 1, 2
+INVALID_PRIM = 0xFFFFFFFF
+BUILDER_LBVH, BUILDER_PLOC = 0, 1
+
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim", "<u4")])
+TRI_RESULT_DTYPE = np.dtype([("type", "<u4"), ("distance", "<f4"), ("u", "<f4"), ("v", "<f4")])
+RESULT_DTYPE = np.dtype([("type", "<u4"), ("distance", "<f4")])
+
+
+class VkrtError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"vkrt error {code}: {message}")
+        self.code = code
+
+
+class BvhParams(ctypes.Structure):
+    _fields_ = [("builder", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 6)]
+
+
+class BvhInfo(ctypes.Structure):
+    _fields_ = [("n_tri", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32),
+                ("depth", ctypes.c_uint32), ("sah_cost", ctypes.c_float), ("build_ms", ctypes.c_float),
+                ("device_bytes", ctypes.c_uint64), ("bounds", ctypes.c_float * 6)]
+
+
+_lib = None
+_vp, _u32, _i32, _f32, _u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_uint64
+
+_SIGNATURES = {
+    "vkrt_version": (_i32, []),
+    "vkrt_device_count": (_i32, []),
+    "vkrt_create": (_i32, [_i32, _vp, ctypes.POINTER(_vp)]),
+    "vkrt_destroy": (None, [_vp]),
+    "vkrt_last_error": (ctypes.c_char_p, [_vp]),
+    "vkrt_sync": (_i32, [_vp]),
+    "vkrt_stream": (_vp, [_vp]),
+    "vkrt_intersect_triangles": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_aabbs": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_obbs": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_spheres": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_planes": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_compute_aabb": (_i32, [_vp, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_brute_closest": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_brute_any": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
+    "vkrt_bvh_build": (_i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _i32, ctypes.POINTER(_vp)]),
+    "vkrt_bvh_refit": (_i32, [_vp, _vp, _vp, _u32, _i32]),
+    "vkrt_bvh_get_info": (_i32, [_vp, ctypes.POINTER(BvhInfo)]),
+    "vkrt_bvh_destroy": (None, [_vp]),
+    "vkrt_trace_closest": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_trace_any": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_icosphere_size": (_i32, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "vkrt_icosphere": (_i32, [_f32, _u32, _vp, _vp]),
+    "vkrt_plane_size": (_i32, [_u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "vkrt_plane": (_i32, [_f32, _f32, _u32, _u32, _vp, _vp]),
+}
+
+
+def lib():
+    """Load libvkrt.so (built by ``python -m vierkant_amd.build``); raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m vierkant_amd.build` "
+                               "(there is no CPU fallback)")
+        handle = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def declared_symbols():
+    return list(_SIGNATURES)
+
+
+def _ptr(a):
+    """(pointer, is_device) of a numpy array or torch tensor."""
+    if a is None:
+        return None, False
+    if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return a.ctypes.data, False
+    if hasattr(a, "data_ptr"):
+        if not a.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return a.data_ptr(), a.is_cuda
+    raise TypeError(f"unsupported array type {type(a)}")
+
+
+def _flags(arrays, extra=0):
+    kinds = {_ptr(a)[1] for a in arrays if a is not None}
+    if len(kinds) > 1:
+        raise ValueError("mixing host and device arrays in one call")
+    return (DEVICE_PTRS if kinds == {True} else HOST_PTRS) | extra
+
+
+def _n(a, per=1):
+    if isinstance(a, np.ndarray):
+        return a.nbytes // (4 * per)
+    return a.numel() * a.element_size() // (4 * per)
+
+
+class Context:
+    """A vkrt context on one HIP device (vkrt_create / vkrt_destroy)."""
+
+    def __init__(self, device=0, stream=None):
+        self._h = _vp()
+        rc = lib().vkrt_create(device, stream, ctypes.byref(self._h))
+        if rc != OK:
+            raise VkrtError(rc, "vkrt_create failed (no gfx950 device?)")
+
+    def close(self):
+        if self._h:
+            lib().vkrt_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def stream(self):
+        return lib().vkrt_stream(self._h)
+
+    def check(self, rc):
+        if rc != OK:
+            raise VkrtError(rc, lib().vkrt_last_error(self._h).decode())
+
+    def sync(self):
+        self.check(lib().vkrt_sync(self._h))
+
+    # -- pairwise tests -------------------------------------------------------------------------
+    def _pairwise(self, fn, prims, per, rays, out_dtype, out=None):
+        n = _n(rays, 8)
+        if out is None:
+            out = np.zeros(n, dtype=out_dtype)
+        self.check(fn(self._h, _ptr(prims)[0], _ptr(rays)[0], n, _ptr(out)[0], _flags([prims, rays, out])))
+        return out
+
+    def intersect_triangles(self, tris, rays, out=None):
+        return self._pairwise(lib().vkrt_intersect_triangles, tris, 9, rays, TRI_RESULT_DTYPE, out)
+
+    def intersect_aabbs(self, boxes, rays, out=None):
+        return self._pairwise(lib().vkrt_intersect_aabbs, boxes, 6, rays, RESULT_DTYPE, out)
+
+    def intersect_obbs(self, obbs, rays, out=None):
+        return self._pairwise(lib().vkrt_intersect_obbs, obbs, 15, rays, RESULT_DTYPE, out)
+
+    def intersect_spheres(self, spheres, rays, out=None):
+        return self._pairwise(lib().vkrt_intersect_spheres, spheres, 4, rays, RESULT_DTYPE, out)
+
+    def intersect_planes(self, planes, rays, out=None):
+        return self._pairwise(lib().vkrt_intersect_planes, planes, 4, rays, RESULT_DTYPE, out)
+
+    def compute_aabb(self, points, out=None):
+        n = _n(points, 3)
+        if out is None:
+            out = np.zeros(6, dtype=np.float32)
+        self.check(lib().vkrt_compute_aabb(self._h, _ptr(points)[0], n, _ptr(out)[0], _flags([points, out])))
+        return out
+
+    # -- brute force ----------------------------------------------------------------------------
+    def brute_closest(self, tris, rays, out=None, flags=0):
+        n_tri, n_ray = _n(tris, 9), _n(rays, 8)
+        if out is None:
+            out = np.zeros(n_ray, dtype=HIT_DTYPE)
+        self.check(lib().vkrt_intersect_brute_closest(self._h, _ptr(tris)[0], n_tri, _ptr(rays)[0], n_ray,
+                                                      _ptr(out)[0], _flags([tris, rays, out], flags)))
+        return out
+
+    def brute_any(self, tris, rays, out=None, flags=0):
+        n_tri, n_ray = _n(tris, 9), _n(rays, 8)
+        if out is None:
+            out = np.zeros(n_ray, dtype=np.uint8)
+        self.check(lib().vkrt_intersect_brute_any(self._h, _ptr(tris)[0], n_tri, _ptr(rays)[0], n_ray,
+                                                  _ptr(out)[0], _flags([tris, rays, out], flags)))
+        return out
+
+    # -- BVH ------------------------------------------------------------------------------------
+    def bvh_build(self, positions, indices, builder=BUILDER_LBVH, max_leaf_size=0, stride=12):
+        params = BvhParams(builder, max_leaf_size)
+        n_vert = _n(positions) * 4 // stride
+        n_tri = _n(indices, 3)
+        h = _vp()
+        self.check(lib().vkrt_bvh_build(self._h, _ptr(positions)[0], n_vert, stride, _ptr(indices)[0], n_tri,
+                                        ctypes.byref(params), _flags([positions, indices]), ctypes.byref(h)))
+        return Bvh(self, h)
+
+    def trace_closest(self, bvh, rays, out=None, flags=0):
+        n = _n(rays, 8)
+        if out is None:
+            out = np.zeros(n, dtype=HIT_DTYPE)
+        self.check(lib().vkrt_trace_closest(self._h, bvh.handle, _ptr(rays)[0], n, _ptr(out)[0],
+                                            _flags([rays, out], flags)))
+        return out
+
+    def trace_any(self, bvh, rays, out=None, flags=0):
+        n = _n(rays, 8)
+        if out is None:
+            out = np.zeros(n, dtype=np.uint8)
+        self.check(lib().vkrt_trace_any(self._h, bvh.handle, _ptr(rays)[0], n, _ptr(out)[0],
+                                        _flags([rays, out], flags)))
+        return out
+
+
+class Bvh:
+    """A BVH built by vkrt_bvh_build; destroyed with the object."""
+
+    def __init__(self, ctx, handle):
+        self.ctx, self._h = ctx, handle
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        info = BvhInfo()
+        self.ctx.check(lib().vkrt_bvh_get_info(self._h, ctypes.byref(info)))
+        return info
+
+    def refit(self, positions, stride=12):
+        self.ctx.check(lib().vkrt_bvh_refit(self.ctx.handle, self._h, _ptr(positions)[0], stride,
+                                            _flags([positions])))
+
+    def close(self):
+        if self._h:
+            lib().vkrt_bvh_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -- procedural geometry (host) ----------------------------------------------------------------
+def icosphere(radius, subdiv):
+    """Geometry::IcoSphere(radius, subdiv): (positions float32 [n,3], indices uint32 [m])."""
+    nv, ni = _u32(), _u32()
+    if lib().vkrt_icosphere_size(subdiv, ctypes.byref(nv), ctypes.byref(ni)) != OK:
+        raise ValueError("bad subdivision count")
+    pos = np.zeros((nv.value, 3), dtype=np.float32)
+    idx = np.zeros(ni.value, dtype=np.uint32)
+    if lib().vkrt_icosphere(ctypes.c_float(radius), subdiv, pos.ctypes.data, idx.ctypes.data) != OK:
+        raise RuntimeError("vkrt_icosphere failed")
+    return pos, idx
+
+
+def plane(width, height, segments_w, segments_h):
+    """Geometry::Plane(width, height, segments_w, segments_h) in the XZ plane."""
+    nv, ni = _u32(), _u32()
+    if lib().vkrt_plane_size(segments_w, segments_h, ctypes.byref(nv), ctypes.byref(ni)) != OK:
+        raise ValueError("bad segment counts")
+    pos = np.zeros((nv.value, 3), dtype=np.float32)
+    idx = np.zeros(ni.value, dtype=np.uint32)
+    if lib().vkrt_plane(ctypes.c_float(width), ctypes.c_float(height), segments_w, segments_h,
+                        pos.ctypes.data, idx.ctypes.data) != OK:
+        raise RuntimeError("vkrt_plane failed")
+    return pos, idx

@@ -1,0 +1,703 @@
+// bvh.hip - mesh -> BVH build (LBVH) and closest/any-hit traversal for gfx950.
+//
+// The reference hands triangles to the Vulkan driver (RayBuilder::create_mesh_structures,
+// src/RayBuilder.cpp:98-303, vkCmdBuildAccelerationStructuresKHR :274) and traces with TraceRay
+// (shaders/slang/raypipeline.slang:182); both are opaque. This file is the MI355X replacement.
+//
+// Build (all on the GPU, one stream):
+//   k_gather    indexed mesh -> per-triangle (v0, e1, e2), exact AABB, centroid-bounds atomics
+//   k_morton    30-bit Morton code of the AABB center in the centroid bounds
+//   radix sort  (code, triangle) pairs, project-written (radix_sort.hip)
+//   k_karras    binary radix tree over the sorted codes (Karras 2012), ties broken by position
+//   k_refit     bottom-up AABB union, one thread per leaf, second arrival at a node continues
+//   k_layout    BVH2 nodes holding both children's boxes (64 B), subtrees of <= max_leaf_size
+//               triangles collapsed into leaves over the contiguous sorted range
+//   k_reorder   triangles in leaf order as 48-B records (v0, e1, e2, original index)
+//
+// Traversal: one ray per lane, front-to-back descent with a per-lane stack. Box tests are padded so
+// that a box is never culled when it holds a triangle the reference test accepts (see slab_interval),
+// and equal-t hits resolve to the lowest original triangle index, so results are identical to the
+// brute-force loop over intersect(Triangle, Ray).
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "context.hpp"
+#include "kernels.hpp"
+#include "radix_sort.hpp"
+#include "vierkant/rt_math.hpp"
+
+using namespace vierkant::rt;
+
+struct vkrt_bvh_node
+{
+    float4 a;// child 0: lo.x hi.x lo.y hi.y
+    float4 b;// child 1: lo.x hi.x lo.y hi.y
+    float4 c;// lo.z hi.z of child 0, lo.z hi.z of child 1
+    int4 d;  // child refs: >= 0 internal node, < 0 leaf ~((first << 3) | (count - 1))
+};
+
+struct vkrt_bvh
+{
+    int device = 0;
+    uint32_t n_tri = 0, n_vert = 0, n_nodes = 0, max_leaf = 4;
+    int32_t root_ref = 0;
+    bool empty = true;
+    vkrt::device_buffer nodes, tris;
+    vkrt::device_buffer indices, order, child, range, parent, nbox, tbox, flags, pre, err;
+    vkrt_bvh_info info{};
+};
+
+namespace vkrt
+{
+
+constexpr int32_t MAX_LEAF = 8;
+constexpr int STACK_SIZE = 64;
+constexpr float BOX_PAD = 0x1p-16f; // relative growth of every stored box coordinate
+constexpr float T_PAD = 0x1p-16f;   // relative growth of every slab interval
+
+__device__ __forceinline__ vec3 ld3(const float *p) { return vec3{p[0], p[1], p[2]}; }
+
+__device__ __forceinline__ bool finite3(const vec3 &v)
+{
+    return __builtin_isfinite(v.x) && __builtin_isfinite(v.y) && __builtin_isfinite(v.z);
+}
+
+__device__ __forceinline__ void atomic_min_f(uint32_t *p, float v) { atomicMin(p, ordered_bits(v + 0.0f)); }
+__device__ __forceinline__ void atomic_max_f(uint32_t *p, float v) { atomicMax(p, ordered_bits(v + 0.0f)); }
+
+__device__ __forceinline__ float from_ordered(uint32_t b)
+{
+    uint32_t r = (b & 0x80000000u) ? (b & 0x7fffffffu) : ~b;
+    return __uint_as_float(r);
+}
+
+// ---------------------------------------------------------------------------------------------
+// build kernels
+// ---------------------------------------------------------------------------------------------
+__global__ void k_gather(const uint8_t *__restrict__ pos, uint32_t stride, const uint32_t *__restrict__ idx,
+                         uint32_t n_tri, uint32_t n_vert, float *__restrict__ pre, float *__restrict__ box,
+                         uint32_t *__restrict__ cbounds, uint32_t *__restrict__ err)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n_tri) return;
+    uint32_t i0 = idx[3 * (size_t)j], i1 = idx[3 * (size_t)j + 1], i2 = idx[3 * (size_t)j + 2];
+    if(i0 >= n_vert || i1 >= n_vert || i2 >= n_vert)
+    {
+        atomicOr(err, 1u);
+        i0 = i1 = i2 = 0;
+    }
+    vec3 v0 = ld3(reinterpret_cast<const float *>(pos + (size_t)i0 * stride));
+    vec3 v1 = ld3(reinterpret_cast<const float *>(pos + (size_t)i1 * stride));
+    vec3 v2 = ld3(reinterpret_cast<const float *>(pos + (size_t)i2 * stride));
+    vec3 e1 = v1 - v0, e2 = v2 - v0;
+    float *p = pre + 9 * (size_t)j;
+    p[0] = v0.x, p[1] = v0.y, p[2] = v0.z, p[3] = e1.x, p[4] = e1.y, p[5] = e1.z, p[6] = e2.x, p[7] = e2.y,
+    p[8] = e2.z;
+    float *b = box + 6 * (size_t)j;
+    if(finite3(v0) && finite3(v1) && finite3(v2))
+    {
+        vec3 lo{fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y), fminf(fminf(v0.z, v1.z), v2.z)};
+        vec3 hi{fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y), fmaxf(fmaxf(v0.z, v1.z), v2.z)};
+        b[0] = lo.x, b[1] = lo.y, b[2] = lo.z, b[3] = hi.x, b[4] = hi.y, b[5] = hi.z;
+        float cx = 0.5f * lo.x + 0.5f * hi.x, cy = 0.5f * lo.y + 0.5f * hi.y, cz = 0.5f * lo.z + 0.5f * hi.z;
+        atomic_min_f(&cbounds[0], cx), atomic_min_f(&cbounds[1], cy), atomic_min_f(&cbounds[2], cz);
+        atomic_max_f(&cbounds[3], cx), atomic_max_f(&cbounds[4], cy), atomic_max_f(&cbounds[5], cz);
+    }
+    else
+    {
+        // a triangle with a non-finite vertex never yields a finite t (the reference test then produces
+        // NaN), so it gets an empty box and is never visited
+        b[0] = b[1] = b[2] = FLT_MAX;
+        b[3] = b[4] = b[5] = -FLT_MAX;
+    }
+}
+
+__device__ __forceinline__ uint32_t spread10(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+__global__ void k_morton(const float *__restrict__ box, uint32_t n, const uint32_t *__restrict__ cbounds,
+                         uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    const float *b = box + 6 * (size_t)j;
+    uint32_t code = 0;
+    if(b[0] <= b[3])
+    {
+        uint32_t q[3];
+        for(int a = 0; a < 3; ++a)
+        {
+            float lo = from_ordered(cbounds[a]), hi = from_ordered(cbounds[3 + a]);
+            float c = 0.5f * b[a] + 0.5f * b[3 + a];
+            float ext = hi - lo;
+            float f = ext > 0.f ? (c - lo) / ext : 0.5f;
+            f = fminf(fmaxf(f, 0.f), 1.f);
+            q[a] = min(1023u, static_cast<uint32_t>(f * 1024.f));
+        }
+        code = (spread10(q[0]) << 2) | (spread10(q[1]) << 1) | spread10(q[2]);
+    }
+    keys[j] = code;
+    vals[j] = j;
+}
+
+__device__ __forceinline__ int delta(const uint32_t *__restrict__ keys, int n, int i, int j)
+{
+    if(j < 0 || j >= n) return -1;
+    uint32_t a = keys[i], b = keys[j];
+    if(a == b) return 32 + __clz(static_cast<uint32_t>(i ^ j));
+    return __clz(a ^ b);
+}
+
+//! Karras 2012 binary radix tree. child refs: >= 0 internal node, < 0 leaf at sorted position ~ref.
+__global__ void k_karras(const uint32_t *__restrict__ keys, int n, int2 *__restrict__ child, uint2 *__restrict__ range,
+                         int32_t *__restrict__ parent_int, int32_t *__restrict__ parent_leaf)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n - 1) return;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while(delta(keys, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for(int t = lmax / 2; t >= 1; t /= 2)
+        if(delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    for(int div = 2;; div *= 2)
+    {
+        int t = (l + div - 1) / div;
+        if(delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+        if(t <= 1) break;
+    }
+    const int gamma = i + s * d + min(d, 0);
+    const int first = min(i, j), last = max(i, j);
+    int left = (first == gamma) ? ~gamma : gamma;
+    int right = (last == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    child[i] = make_int2(left, right);
+    range[i] = make_uint2(first, last);
+    if(left < 0) parent_leaf[~left] = i;
+    else parent_int[left] = i;
+    if(right < 0) parent_leaf[~right] = i;
+    else parent_int[right] = i;
+    if(i == 0) parent_int[0] = -1;
+}
+
+__global__ void k_sorted_boxes(const uint32_t *__restrict__ order, uint32_t n, const float *__restrict__ box,
+                               float *__restrict__ tbox)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    const float *s = box + 6 * (size_t)order[j];
+    float *d = tbox + 6 * (size_t)j;
+    for(int a = 0; a < 6; ++a) d[a] = s[a];
+}
+
+__device__ __forceinline__ void load_box(const float *p, float b[6])
+{
+    for(int a = 0; a < 6; ++a) b[a] = p[a];
+}
+
+__global__ void k_refit(uint32_t n, const int2 *__restrict__ child, const int32_t *__restrict__ parent_int,
+                        const int32_t *__restrict__ parent_leaf, const float *__restrict__ tbox,
+                        float *__restrict__ nbox, uint32_t *__restrict__ flags)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    int32_t p = parent_leaf[j];
+    while(p >= 0)
+    {
+        uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        if(old == 0) return;// the sibling subtree is not finished; its thread continues
+        int2 c = child[p];
+        float a[6], b[6];
+        load_box(c.x < 0 ? tbox + 6 * (size_t)(~c.x) : nbox + 6 * (size_t)c.x, a);
+        load_box(c.y < 0 ? tbox + 6 * (size_t)(~c.y) : nbox + 6 * (size_t)c.y, b);
+        float *o = nbox + 6 * (size_t)p;
+        for(int k = 0; k < 3; ++k) o[k] = fminf(a[k], b[k]);
+        for(int k = 3; k < 6; ++k) o[k] = fmaxf(a[k], b[k]);
+        p = parent_int[p];
+    }
+}
+
+__device__ __forceinline__ float pad_lo(float x) { return x - fabsf(x) * BOX_PAD; }
+__device__ __forceinline__ float pad_hi(float x) { return x + fabsf(x) * BOX_PAD; }
+
+__global__ void k_layout(uint32_t n, const int2 *__restrict__ child, const uint2 *__restrict__ range,
+                         const float *__restrict__ nbox, const float *__restrict__ tbox, uint32_t max_leaf,
+                         vkrt_bvh_node *__restrict__ nodes)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i + 1 >= n) return;
+    int2 c = child[i];
+    float bx[2][6];
+    int ref[2];
+    const int cr[2] = {c.x, c.y};
+    for(int k = 0; k < 2; ++k)
+    {
+        uint32_t first, last;
+        const float *src;
+        if(cr[k] < 0)
+        {
+            first = last = ~cr[k];
+            src = tbox + 6 * (size_t)first;
+        }
+        else
+        {
+            uint2 r = range[cr[k]];
+            first = r.x, last = r.y;
+            src = nbox + 6 * (size_t)cr[k];
+        }
+        uint32_t count = last - first + 1;
+        ref[k] = (cr[k] < 0 || count <= max_leaf) ? ~static_cast<int>((first << 3) | (count - 1)) : cr[k];
+        for(int a = 0; a < 3; ++a) bx[k][a] = pad_lo(src[a]);
+        for(int a = 3; a < 6; ++a) bx[k][a] = pad_hi(src[a]);
+    }
+    vkrt_bvh_node nd;
+    nd.a = make_float4(bx[0][0], bx[0][3], bx[0][1], bx[0][4]);
+    nd.b = make_float4(bx[1][0], bx[1][3], bx[1][1], bx[1][4]);
+    nd.c = make_float4(bx[0][2], bx[0][5], bx[1][2], bx[1][5]);
+    nd.d = make_int4(ref[0], ref[1], 0, 0);
+    nodes[i] = nd;
+}
+
+__global__ void k_reorder(const uint32_t *__restrict__ order, uint32_t n, const float *__restrict__ pre,
+                          float4 *__restrict__ tris)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    uint32_t src = order[j];
+    const float *p = pre + 9 * (size_t)src;
+    tris[3 * (size_t)j + 0] = make_float4(p[0], p[1], p[2], p[3]);
+    tris[3 * (size_t)j + 1] = make_float4(p[4], p[5], p[6], p[7]);
+    tris[3 * (size_t)j + 2] = make_float4(p[8], __uint_as_float(src), 0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------------------------
+// traversal
+// ---------------------------------------------------------------------------------------------
+struct ray_setup
+{
+    vec3 o, d, inv;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ ray_setup setup_ray(const float4 *__restrict__ rays, uint32_t i)
+{
+    ray_setup r;
+    float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
+    r.o = vec3{a.x, a.y, a.z};
+    r.tmin = a.w;
+    r.d = vec3{b.x, b.y, b.z};
+    r.tmax = b.w;
+    // a zero direction component becomes +-1e-20: that axis then gives t = 0 on a slab plane through
+    // the origin and |t| >= 1e20 |b - o| elsewhere, i.e. "inside the slab for all t" or "never"
+    auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
+    r.inv = vec3{1.0f / safe(r.d.x), 1.0f / safe(r.d.y), 1.0f / safe(r.d.z)};
+    return r;
+}
+
+//! Conservative ray/box interval. Each slab value t = RN(RN(b - o) * RN(1/d)) is within 3 eps
+//! (eps = 2^-24) relative of (b - o)/d, so growing the interval by T_PAD |t| (T_PAD = 2^-16) keeps the
+//! exact interval inside it; boxes were grown by BOX_PAD relative at build time to absorb the rounding
+//! of the reference test's hit point.
+__device__ __forceinline__ void slab_interval(const ray_setup &r, float lox, float hix, float loy, float hiy, float loz,
+                                              float hiz, float &tn, float &tf)
+{
+    float x0 = (lox - r.o.x) * r.inv.x, x1 = (hix - r.o.x) * r.inv.x;
+    float y0 = (loy - r.o.y) * r.inv.y, y1 = (hiy - r.o.y) * r.inv.y;
+    float z0 = (loz - r.o.z) * r.inv.z, z1 = (hiz - r.o.z) * r.inv.z;
+    float n_ = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
+    float f_ = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
+    tn = n_ - fabsf(n_) * T_PAD;
+    tf = f_ + fabsf(f_) * T_PAD;
+}
+
+__device__ __forceinline__ void leaf_decode(int ref, uint32_t &first, uint32_t &count)
+{
+    uint32_t v = static_cast<uint32_t>(~ref);
+    first = v >> 3;
+    count = (v & 7u) + 1;
+}
+
+template<bool ANY>
+__device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                         int32_t root, const ray_setup &r, float &best_t, float &best_u,
+                                         float &best_v, uint32_t &best_prim)
+{
+    int32_t stack[STACK_SIZE];
+    int sp = 0;
+    int32_t ref = root;
+    best_t = r.tmax;
+    best_prim = VKRT_INVALID_PRIM;
+    while(true)
+    {
+        if(ref >= 0)
+        {
+            const vkrt_bvh_node &nd = nodes[ref];
+            float4 a = nd.a, b = nd.b, c = nd.c;
+            int4 d = nd.d;
+            float tn0, tf0, tn1, tf1;
+            slab_interval(r, a.x, a.y, a.z, a.w, c.x, c.y, tn0, tf0);
+            slab_interval(r, b.x, b.y, b.z, b.w, c.z, c.w, tn1, tf1);
+            const bool h0 = tn0 <= tf0 && tf0 >= r.tmin && tn0 <= best_t;
+            const bool h1 = tn1 <= tf1 && tf1 >= r.tmin && tn1 <= best_t;
+            if(h0 && h1)
+            {
+                const bool first0 = tn0 <= tn1;
+                stack[sp++] = first0 ? d.y : d.x;
+                ref = first0 ? d.x : d.y;
+                continue;
+            }
+            if(h0 || h1)
+            {
+                ref = h0 ? d.x : d.y;
+                continue;
+            }
+        }
+        else
+        {
+            uint32_t first, count;
+            leaf_decode(ref, first, count);
+            for(uint32_t k = 0; k < count; ++k)
+            {
+                const float4 *tr = tris + 3 * (size_t)(first + k);
+                float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+                float t, u, v;
+                if(mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y}, vec3{p1.z, p1.w, p2.x}, t, u,
+                           v) &&
+                   t > r.tmin)
+                {
+                    const uint32_t prim = __float_as_uint(p2.y);
+                    if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim))
+                    {
+                        best_t = t;
+                        best_u = u;
+                        best_v = v;
+                        best_prim = prim;
+                        if(ANY) return true;
+                    }
+                }
+            }
+        }
+        if(sp == 0) break;
+        ref = stack[--sp];
+    }
+    return best_prim != VKRT_INVALID_PRIM;
+}
+
+__global__ void __launch_bounds__(256) k_trace_closest(const vkrt_bvh_node *__restrict__ nodes,
+                                                       const float4 *__restrict__ tris, int32_t root, bool empty,
+                                                       const float4 *__restrict__ rays, uint32_t n,
+                                                       vkrt_hit *__restrict__ hits)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    vkrt_hit h{__int_as_float(0x7f800000), 0.f, 0.f, VKRT_INVALID_PRIM};
+    if(!empty)
+    {
+        ray_setup r = setup_ray(rays, i);
+        float t, u = 0.f, v = 0.f;
+        uint32_t prim;
+        if(traverse<false>(nodes, tris, root, r, t, u, v, prim)) h = vkrt_hit{t, u, v, prim};
+    }
+    hits[i] = h;
+}
+
+__global__ void __launch_bounds__(256) k_trace_any(const vkrt_bvh_node *__restrict__ nodes,
+                                                   const float4 *__restrict__ tris, int32_t root, bool empty,
+                                                   const float4 *__restrict__ rays, uint32_t n,
+                                                   uint8_t *__restrict__ occluded)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    uint8_t occ = 0;
+    if(!empty)
+    {
+        ray_setup r = setup_ray(rays, i);
+        float t, u, v;
+        uint32_t prim;
+        occ = traverse<true>(nodes, tris, root, r, t, u, v, prim) ? 1 : 0;
+    }
+    occluded[i] = occ;
+}
+
+// ---------------------------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------------------------
+static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
+{
+    // depth and SAH cost from the final nodes (build-time statistics only)
+    vkrt_bvh_info &info = bvh->info;
+    info.depth = 1;
+    info.sah_cost = 0.f;
+    info.n_leaves = bvh->empty ? 0 : 1;
+    if(bvh->empty || bvh->root_ref < 0) return;
+    std::vector<vkrt_bvh_node> nodes(bvh->n_nodes);
+    VKRT_HIP(hipMemcpyAsync(nodes.data(), bvh->nodes.ptr, nodes.size() * sizeof(vkrt_bvh_node), hipMemcpyDeviceToHost,
+                            s));
+    VKRT_HIP(hipStreamSynchronize(s));
+    auto area = [](float lx, float hx, float ly, float hy, float lz, float hz) {
+        double dx = std::max(0.0, (double)hx - lx), dy = std::max(0.0, (double)hy - ly),
+               dz = std::max(0.0, (double)hz - lz);
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    };
+    const vkrt_bvh_node &root = nodes[0];
+    float rb[6] = {std::min(root.a.x, root.b.x), std::max(root.a.y, root.b.y), std::min(root.a.z, root.b.z),
+                   std::max(root.a.w, root.b.w), std::min(root.c.x, root.c.z), std::max(root.c.y, root.c.w)};
+    for(int a = 0; a < 3; ++a)
+    {
+        info.bounds[a] = rb[2 * a];
+        info.bounds[3 + a] = rb[2 * a + 1];
+    }
+    double root_area = std::max(1e-30, area(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5]));
+    double sah = 1.0;// root traversal
+    uint32_t leaves = 0, max_depth = 1;
+    std::vector<std::pair<int32_t, uint32_t>> todo{{0, 1}};
+    while(!todo.empty())
+    {
+        auto [ref, depth] = todo.back();
+        todo.pop_back();
+        max_depth = std::max(max_depth, depth);
+        const vkrt_bvh_node &nd = nodes[ref];
+        const int refs[2] = {nd.d.x, nd.d.y};
+        const double ar[2] = {area(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y),
+                              area(nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w)};
+        for(int k = 0; k < 2; ++k)
+        {
+            if(refs[k] >= 0)
+            {
+                sah += 1.0 * ar[k] / root_area;
+                todo.push_back({refs[k], depth + 1});
+            }
+            else
+            {
+                uint32_t v = static_cast<uint32_t>(~refs[k]);
+                sah += 1.0 * ((v & 7u) + 1) * ar[k] / root_area;
+                ++leaves;
+                max_depth = std::max(max_depth, depth + 1);
+            }
+        }
+    }
+    info.depth = max_depth;
+    info.sah_cost = static_cast<float>(sah);
+    info.n_leaves = leaves;
+    if(max_depth > STACK_SIZE) throw error(VKRT_E_INTERNAL, "BVH deeper than the traversal stack");
+}
+
+static void build_topology(vkrt_ctx *ctx, vkrt_bvh *bvh, const uint8_t *pos, uint32_t stride, const uint32_t *idx)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t n = bvh->n_tri;
+    bvh->pre.reserve((size_t)n * 9 * sizeof(float));
+    bvh->tbox.reserve((size_t)n * 6 * sizeof(float));
+    bvh->err.reserve(64);
+    // scratch: per-triangle boxes in input order + centroid bounds + sort keys/values + sort temp
+    device_buffer &tmp = ctx->scratch[2];
+    const size_t box_bytes = (size_t)n * 6 * sizeof(float);
+    const size_t key_bytes = (size_t)n * sizeof(uint32_t);
+    tmp.reserve(box_bytes + 64 + key_bytes + radix_sort_temp_bytes(n));
+    float *box = tmp.as<float>();
+    uint32_t *cb = reinterpret_cast<uint32_t *>(tmp.as<uint8_t>() + box_bytes);
+    uint32_t *keys = reinterpret_cast<uint32_t *>(tmp.as<uint8_t>() + box_bytes + 64);
+    void *sort_tmp = tmp.as<uint8_t>() + box_bytes + 64 + key_bytes;
+    bvh->order.reserve(key_bytes);
+    uint32_t *order = bvh->order.as<uint32_t>();
+
+    uint32_t init[8] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u, 0u, 0u};
+    VKRT_HIP(hipMemcpyAsync(cb, init, sizeof(init), hipMemcpyHostToDevice, s));
+    VKRT_HIP(hipMemsetAsync(bvh->err.ptr, 0, 4, s));
+    k_gather<<<div_up(n, 256), 256, 0, s>>>(pos, stride, idx, n, bvh->n_vert, bvh->pre.as<float>(), box, cb,
+                                            bvh->err.as<uint32_t>());
+    k_morton<<<div_up(n, 256), 256, 0, s>>>(box, n, cb, keys, order);
+    radix_sort_pairs(s, keys, order, n, 30, sort_tmp);
+    k_sorted_boxes<<<div_up(n, 256), 256, 0, s>>>(order, n, box, bvh->tbox.as<float>());
+    uint32_t err_host = 0;
+    VKRT_HIP(hipMemcpyAsync(&err_host, bvh->err.ptr, 4, hipMemcpyDeviceToHost, s));
+    if(n >= 2)
+    {
+        const uint32_t ni = n - 1;
+        bvh->child.reserve(ni * sizeof(int2));
+        bvh->range.reserve(ni * sizeof(uint2));
+        bvh->parent.reserve(((size_t)ni + n) * sizeof(int32_t));
+        bvh->nbox.reserve((size_t)ni * 6 * sizeof(float));
+        bvh->flags.reserve(ni * sizeof(uint32_t));
+        int32_t *parent_int = bvh->parent.as<int32_t>(), *parent_leaf = parent_int + ni;
+        k_karras<<<div_up(ni, 256), 256, 0, s>>>(keys, (int)n, bvh->child.as<int2>(), bvh->range.as<uint2>(),
+                                                 parent_int, parent_leaf);
+    }
+    VKRT_HIP(hipStreamSynchronize(s));
+    VKRT_HIP(hipGetLastError());
+    if(err_host) throw error(VKRT_E_INVALID_ARG, "triangle index out of range of the vertex count");
+}
+
+static void build_bounds(vkrt_ctx *ctx, vkrt_bvh *bvh)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t n = bvh->n_tri;
+    if(n >= 2)
+    {
+        const uint32_t ni = n - 1;
+        int32_t *parent_int = bvh->parent.as<int32_t>(), *parent_leaf = parent_int + ni;
+        VKRT_HIP(hipMemsetAsync(bvh->flags.ptr, 0, ni * sizeof(uint32_t), s));
+        k_refit<<<div_up(n, 256), 256, 0, s>>>(n, bvh->child.as<int2>(), parent_int, parent_leaf,
+                                               bvh->tbox.as<float>(), bvh->nbox.as<float>(),
+                                               bvh->flags.as<uint32_t>());
+        bvh->n_nodes = ni;
+        bvh->nodes.reserve((size_t)ni * sizeof(vkrt_bvh_node));
+        k_layout<<<div_up(ni, 256), 256, 0, s>>>(n, bvh->child.as<int2>(), bvh->range.as<uint2>(),
+                                                 bvh->nbox.as<float>(), bvh->tbox.as<float>(), bvh->max_leaf,
+                                                 bvh->nodes.as<vkrt_bvh_node>());
+        bvh->root_ref = (n <= bvh->max_leaf) ? ~static_cast<int>(n - 1) : 0;
+    }
+    else
+    {
+        bvh->n_nodes = 0;
+        bvh->root_ref = ~0;// a leaf holding the single triangle
+    }
+    bvh->tris.reserve((size_t)n * 3 * sizeof(float4));
+    k_reorder<<<div_up(n, 256), 256, 0, s>>>(bvh->order.as<uint32_t>(), n, bvh->pre.as<float>(),
+                                             bvh->tris.as<float4>());
+    VKRT_HIP(hipGetLastError());
+}
+
+static void refresh_info(vkrt_bvh *bvh, hipStream_t s)
+{
+    bvh->info.n_tri = bvh->n_tri;
+    bvh->info.n_nodes = bvh->n_nodes;
+    bvh->info.device_bytes = bvh->nodes.bytes + bvh->tris.bytes;
+    compute_stats(bvh, s);
+}
+
+}// namespace vkrt
+
+using namespace vkrt;
+
+extern "C" {
+
+int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint32_t stride_bytes,
+                   const uint32_t *indices, uint32_t n_tri, const vkrt_bvh_params *params, int flags, vkrt_bvh **out)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        VKRT_REQUIRE(stride_bytes >= 12 && stride_bytes % 4 == 0, "stride must be >= 12 and a multiple of 4");
+        VKRT_REQUIRE(n_tri == 0 || (positions && indices && n_vert), "null mesh arrays");
+        VKRT_REQUIRE(n_tri < (1u << 28), "at most 2^28 - 1 triangles per BVH");
+        uint32_t leaf = params && params->max_leaf_size ? params->max_leaf_size : 4;
+        VKRT_REQUIRE(leaf >= 1 && leaf <= (uint32_t)MAX_LEAF, "max_leaf_size must be in 1..8");
+        VKRT_REQUIRE(!params || params->builder == VKRT_BUILDER_LBVH, "builder not supported");
+        auto t_start = std::chrono::steady_clock::now();
+        auto bvh = std::make_unique<vkrt_bvh>();
+        bvh->device = ctx->device;
+        bvh->n_tri = n_tri;
+        bvh->n_vert = n_vert;
+        bvh->max_leaf = leaf;
+        bvh->empty = n_tri == 0;
+        if(n_tri)
+        {
+            const size_t pos_bytes = (size_t)(n_vert - 1) * stride_bytes + 12;
+            const uint8_t *dpos =
+                    input(ctx, 0, reinterpret_cast<const uint8_t *>(positions), pos_bytes, flags);
+            bvh->indices.reserve((size_t)n_tri * 3 * sizeof(uint32_t));
+            const uint32_t *didx = input(ctx, 1, indices, (size_t)n_tri * 3, flags);
+            VKRT_HIP(hipMemcpyAsync(bvh->indices.ptr, didx, (size_t)n_tri * 3 * sizeof(uint32_t),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+            build_topology(ctx, bvh.get(), dpos, stride_bytes, bvh->indices.as<uint32_t>());
+            build_bounds(ctx, bvh.get());
+            VKRT_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        bvh->info.build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+        refresh_info(bvh.get(), ctx->stream);
+        *out = bvh.release();
+    });
+}
+
+int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_t stride_bytes, int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(bvh, "null bvh");
+        VKRT_REQUIRE(stride_bytes >= 12 && stride_bytes % 4 == 0, "stride must be >= 12 and a multiple of 4");
+        if(bvh->empty) return;
+        VKRT_REQUIRE(positions, "null positions");
+        hipStream_t s = ctx->stream;
+        const uint32_t n = bvh->n_tri;
+        const size_t pos_bytes = (size_t)(bvh->n_vert - 1) * stride_bytes + 12;
+        const uint8_t *dpos = input(ctx, 0, reinterpret_cast<const uint8_t *>(positions), pos_bytes, flags);
+        device_buffer &tmp = ctx->scratch[2];
+        tmp.reserve((size_t)n * 6 * sizeof(float) + 64);
+        float *box = tmp.as<float>();
+        uint32_t *cb = reinterpret_cast<uint32_t *>(tmp.as<uint8_t>() + (size_t)n * 6 * sizeof(float));
+        VKRT_HIP(hipMemsetAsync(bvh->err.ptr, 0, 4, s));
+        k_gather<<<div_up(n, 256), 256, 0, s>>>(dpos, stride_bytes, bvh->indices.as<uint32_t>(), n, bvh->n_vert,
+                                                bvh->pre.as<float>(), box, cb, bvh->err.as<uint32_t>());
+        k_sorted_boxes<<<div_up(n, 256), 256, 0, s>>>(bvh->order.as<uint32_t>(), n, box, bvh->tbox.as<float>());
+        build_bounds(ctx, bvh);
+        VKRT_HIP(hipStreamSynchronize(s));
+        refresh_info(bvh, s);
+    });
+}
+
+int vkrt_bvh_get_info(const vkrt_bvh *bvh, vkrt_bvh_info *out)
+{
+    if(!bvh || !out) return VKRT_E_INVALID_ARG;
+    *out = bvh->info;
+    return VKRT_OK;
+}
+
+void vkrt_bvh_destroy(vkrt_bvh *bvh)
+{
+    if(!bvh) return;
+    (void)hipSetDevice(bvh->device);
+    delete bvh;
+}
+
+int vkrt_trace_closest(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uint32_t n, vkrt_hit *out,
+                       int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(bvh, "null bvh");
+        VKRT_REQUIRE(bvh->device == ctx->device, "bvh belongs to another device");
+        VKRT_REQUIRE(n == 0 || (rays && out), "null array argument");
+        if(!n) return;
+        const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
+        vkrt_hit *dh = output(ctx, 2, out, n, flags);
+        k_trace_closest<<<div_up(n, 256), 256, 0, ctx->stream>>>(
+                bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref, bvh->empty,
+                reinterpret_cast<const float4 *>(dr), n, dh);
+        VKRT_HIP(hipGetLastError());
+        finish_output(ctx, 2, out, n, flags);
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_trace_any(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uint32_t n, uint8_t *occluded,
+                   int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(bvh, "null bvh");
+        VKRT_REQUIRE(bvh->device == ctx->device, "bvh belongs to another device");
+        VKRT_REQUIRE(n == 0 || (rays && occluded), "null array argument");
+        if(!n) return;
+        const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
+        uint8_t *docc = output(ctx, 2, occluded, n, flags);
+        k_trace_any<<<div_up(n, 256), 256, 0, ctx->stream>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(),
+                                                             bvh->root_ref, bvh->empty,
+                                                             reinterpret_cast<const float4 *>(dr), n, docc);
+        VKRT_HIP(hipGetLastError());
+        finish_output(ctx, 2, occluded, n, flags);
+        finish_call(ctx, flags);
+    });
+}
+
+}// extern "C"

@@ -1,0 +1,39 @@
+// kernels.hpp - device helpers and launchers shared between the kernel translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "vkrt.h"
+
+struct vkrt_ctx;
+
+namespace vkrt
+{
+
+//! IEEE float -> uint32 whose unsigned order is the float order (-0 sorts below +0; callers that
+//! need -0 == +0 pass t + 0.0f, which maps -0 to +0)
+__host__ __device__ __forceinline__ uint32_t ordered_bits(float f)
+{
+    uint32_t b;
+    __builtin_memcpy(&b, &f, 4);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+//! 64-bit closest-hit key: smaller t first, then the smaller primitive index
+__host__ __device__ __forceinline__ uint64_t hit_key(float t, uint32_t prim)
+{
+    return (static_cast<uint64_t>(ordered_bits(t + 0.0f)) << 32) | prim;
+}
+
+void launch_tri_prep(hipStream_t s, const float *tris, uint32_t n, float *pre);
+void launch_fill_u64(hipStream_t s, uint64_t *p, uint32_t n, uint64_t value);
+void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t value);
+
+//! brute-force closest hit over pre-processed triangles (v0, e1, e2); keys: n_ray scratch words
+void brute_closest(vkrt_ctx *ctx, const float *pre, uint32_t n_tri, const float4 *rays, uint32_t n_ray,
+                   uint64_t *keys, vkrt_hit *hits);
+void brute_any(vkrt_ctx *ctx, const float *pre, uint32_t n_tri, const float4 *rays, uint32_t n_ray, uint8_t *occ);
+
+}// namespace vkrt
