@@ -65,7 +65,6 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
@@ -74,9 +73,8 @@ def main():
     pos, idx = W.config2_scene()
     tris = W.flat_triangles(pos, idx)
     primary = W.pinhole_rays(2048, 2048)
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream(dev)
-    ctx = vk.Context(0, stream=stream.cuda_stream)
+    ctx = vk.Context(0)
+    stream = ctx.stream
 
     log("building BVH")
     t0 = time.perf_counter()
@@ -84,65 +82,60 @@ def main():
     build_wall_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
 
-    def to_dev(a):
-        return torch.from_numpy(np.ascontiguousarray(a).view(np.uint8)).to(dev)
-
-    d_primary = to_dev(primary).view(torch.float32)
-    d_hits_p = torch.empty(len(primary) * 4, dtype=torch.float32, device=dev)
+    d_primary = vk.DeviceArray.from_host(ctx, primary)
+    d_hits_p = vk.DeviceArray(ctx, len(primary), vk.HIT_DTYPE)
     ctx.trace_closest(bvh, d_primary, out=d_hits_p)
-    hits_p = d_hits_p.cpu().numpy().view(vk.HIT_DTYPE)
+    hits_p = d_hits_p.host()
     log("generating secondary rays")
     bounce, shadow = W.secondary_rays(primary, hits_p, tris)
-    d_bounce = to_dev(bounce).view(torch.float32)
-    d_shadow = to_dev(shadow).view(torch.float32)
-    d_hits_b = torch.empty(len(bounce) * 4, dtype=torch.float32, device=dev)
-    d_occ = torch.empty(len(shadow), dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
+    d_bounce = vk.DeviceArray.from_host(ctx, bounce)
+    d_shadow = vk.DeviceArray.from_host(ctx, shadow)
+    d_hits_b = vk.DeviceArray(ctx, len(bounce), vk.HIT_DTYPE)
+    d_occ = vk.DeviceArray(ctx, len(shadow), np.uint8)
 
     # parity check of a subset against GPU brute force (not timed)
-    hits_b = None
+    verified = None
     if args.verify_rays:
         log("verifying a subset against brute force")
         ctx.trace_closest(bvh, d_bounce, out=d_hits_b)
-        hits_b = d_hits_b.cpu().numpy().view(vk.HIT_DTYPE)
+        hits_b = d_hits_b.host()
         sel = np.sort(np.random.default_rng(1).choice(len(primary), args.verify_rays, replace=False))
         mismatch = 0
         for rays_np, hits_np in ((primary, hits_p), (bounce, hits_b)):
             ref = ctx.brute_closest(tris, rays_np[sel])
             got = hits_np[sel]
-            mismatch += int(np.sum((ref["prim"] != got["prim"]) | (ref["t"].view(np.uint32) != got["t"].view(np.uint32))))
+            mismatch += int(np.sum((ref["prim"] != got["prim"]) |
+                                   (ref["t"].view(np.uint32) != got["t"].view(np.uint32))))
         verified = {"rays": 2 * args.verify_rays, "mismatches": mismatch}
-    else:
-        verified = None
 
     flags = vk.ASYNC
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    ev = vk.HipEvents(5)
 
     def step(timed):
         if timed:
-            ev[0].record(stream)
+            ev.record(0, stream)
         ctx.trace_closest(bvh, d_primary, out=d_hits_p, flags=flags)
         if timed:
-            ev[1].record(stream)
+            ev.record(1, stream)
         ctx.trace_closest(bvh, d_bounce, out=d_hits_b, flags=flags)
         if timed:
-            ev[2].record(stream)
+            ev.record(2, stream)
 
     log(f"warmup {args.warmup}")
     for _ in range(args.warmup):
         step(False)
-    torch.cuda.synchronize()
+    ctx.sync()
     log(f"timing {args.steps} steps")
     prim_ms, bounce_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(True)
-        ev[2].synchronize()
-        prim_ms.append(ev[0].elapsed_time(ev[1]))
-        bounce_ms.append(ev[1].elapsed_time(ev[2]))
-    torch.cuda.synchronize()
+        ev.sync(2)
+        prim_ms.append(ev.ms(0, 1))
+        bounce_ms.append(ev.ms(1, 2))
+    ctx.sync()
     wall = time.perf_counter() - t0
-    # the per-step host sync above only reads events; report the event total as the step time
+    # steps are event-timed back to back on the stream; the host only reads the events between steps
     ms_per_step = float(np.mean(np.array(prim_ms) + np.array(bounce_ms)))
     rays_per_step = len(primary) + len(bounce)
     value = rays_per_step / (ms_per_step * 1e-3) / 1e9
@@ -150,11 +143,11 @@ def main():
     # any-hit shadow rays (reported beside the headline)
     any_ms = []
     for _ in range(max(3, args.steps // 2)):
-        ev[3].record(stream)
+        ev.record(3, stream)
         ctx.trace_any(bvh, d_shadow, out=d_occ, flags=flags)
-        ev[4].record(stream)
-        ev[4].synchronize()
-        any_ms.append(ev[3].elapsed_time(ev[4]))
+        ev.record(4, stream)
+        ev.sync(4)
+        any_ms.append(ev.ms(3, 4))
     any_ms = float(np.mean(any_ms))
 
     launch_ms = float(np.mean(prim_ms + bounce_ms))

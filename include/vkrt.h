@@ -120,6 +120,13 @@ int vkrt_version(void);
 /* number of HIP devices visible (0 when none); never initialises a context */
 int vkrt_device_count(void);
 
+/* Device memory helpers for callers without their own HIP allocator (no counterpart in the reference,
+ * whose buffers are Vulkan/VMA allocations). vkrt_copy kinds: 0 host->device, 1 device->host,
+ * 2 device->device; synchronous on the context's stream unless VKRT_ASYNC is in flags. */
+int vkrt_device_alloc(vkrt_ctx *ctx, size_t bytes, void **out);
+int vkrt_device_free(vkrt_ctx *ctx, void *ptr);
+int vkrt_copy(vkrt_ctx *ctx, void *dst, const void *src, size_t bytes, int kind, int flags);
+
 /* ---------------------------------------------------------------------------------------------
  * pairwise intersection tests: element i tests primitive i against ray i (n pairs)
  * ------------------------------------------------------------------------------------------- */

@@ -112,7 +112,7 @@ def test_brute_closest_config1_full_bit_exact(ctx, config1):
     """65,536 rays x 20,480 triangles: every record identical to the oracle"""
     tris, rays, want = config1
     got = ctx.brute_closest(tris, rays)
-    assert np.sum(want["prim"] != 0xFFFFFFFF) > 10000
+    assert np.sum(want["prim"] != 0xFFFFFFFF) > 3000   # about 6% of the random rays hit the sphere
     assert_hits_equal(got, want, "config1 brute")
 
 
@@ -155,12 +155,14 @@ def test_brute_sizes(ctx, n_tri, n_ray):
 
 
 def test_device_pointer_path(ctx, config1):
-    torch = pytest.importorskip("torch")
+    """device-resident inputs and outputs (VKRT_DEVICE_PTRS) give the same records"""
     tris, rays, want = config1
-    dt = torch.from_numpy(tris).cuda()
-    dr = torch.from_numpy(rays.view(np.float32).reshape(-1)).cuda()
-    out = torch.zeros(len(rays) * 4, dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
+    dt = vk.DeviceArray.from_host(ctx, tris)
+    dr = vk.DeviceArray.from_host(ctx, rays)
+    out = vk.DeviceArray(ctx, len(rays), vk.HIT_DTYPE)
     ctx.brute_closest(dt, dr, out=out)
-    got = out.cpu().numpy().view(vk.HIT_DTYPE)
-    assert_hits_equal(got, want, "device pointers")
+    assert_hits_equal(out.host(), want, "device pointers")
+    occ = vk.DeviceArray(ctx, len(rays), np.uint8)
+    ctx.brute_any(dt, dr, out=occ, flags=vk.ASYNC)
+    ctx.sync()
+    assert np.array_equal(occ.host(), (want["prim"] != 0xFFFFFFFF).astype(np.uint8))

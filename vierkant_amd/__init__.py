@@ -2,7 +2,7 @@
 
 This module is the Python binding of ``libvkrt.so`` (declared in ``include/vkrt.h``). It loads the
 in-tree shared library and raises if it is missing: there is no CPU fallback. Arrays are numpy arrays
-(host memory, staged by the library) or torch tensors on the GPU (passed as device pointers).
+(host memory, staged by the library) or DeviceArray buffers in HBM (passed as device pointers).
 """
 import ctypes
 import pathlib
@@ -53,6 +53,9 @@ _SIGNATURES = {
     "vkrt_last_error": (ctypes.c_char_p, [_vp]),
     "vkrt_sync": (_i32, [_vp]),
     "vkrt_stream": (_vp, [_vp]),
+    "vkrt_device_alloc": (_i32, [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "vkrt_device_free": (_i32, [_vp, _vp]),
+    "vkrt_copy": (_i32, [_vp, _vp, _vp, ctypes.c_size_t, _i32, _i32]),
     "vkrt_intersect_triangles": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_aabbs": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_obbs": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
@@ -95,9 +98,11 @@ def declared_symbols():
 
 
 def _ptr(a):
-    """(pointer, is_device) of a numpy array or torch tensor."""
+    """(pointer, is_device) of a numpy array, a DeviceArray or a torch tensor."""
     if a is None:
         return None, False
+    if isinstance(a, DeviceArray):
+        return a.ptr, True
     if isinstance(a, np.ndarray):
         if not a.flags["C_CONTIGUOUS"]:
             raise ValueError("arrays must be C-contiguous")
@@ -117,9 +122,86 @@ def _flags(arrays, extra=0):
 
 
 def _n(a, per=1):
-    if isinstance(a, np.ndarray):
+    if isinstance(a, (np.ndarray, DeviceArray)):
         return a.nbytes // (4 * per)
     return a.numel() * a.element_size() // (4 * per)
+
+
+class DeviceArray:
+    """A device (HBM) buffer owned through the context (vkrt_device_alloc / vkrt_device_free), typed
+    like a numpy array; pass it wherever the binding takes an array to use the device-pointer path."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx, self.dtype = ctx, np.dtype(dtype)
+        self.shape = (shape,) if isinstance(shape, int) else tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        h = _vp()
+        ctx.check(lib().vkrt_device_alloc(ctx.handle, self.nbytes, ctypes.byref(h)))
+        self.ptr = h.value
+
+    @classmethod
+    def from_host(cls, ctx, a):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        ctx.check(lib().vkrt_copy(ctx.handle, d.ptr, a.ctypes.data, a.nbytes, 0, 0))
+        return d
+
+    def host(self):
+        out = np.empty(self.shape, self.dtype)
+        self.ctx.check(lib().vkrt_copy(self.ctx.handle, out.ctypes.data, self.ptr, self.nbytes, 1, 0))
+        return out
+
+    def __len__(self):
+        return self.shape[0]
+
+    def free(self):
+        if self.ptr:
+            lib().vkrt_device_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class HipEvents:
+    """hipEvent pairs recorded on a stream, through the HIP runtime libvkrt.so itself uses."""
+
+    def __init__(self, n):
+        lib()
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(_vp)]
+        self.hip.hipEventRecord.argtypes = [_vp, _vp]
+        self.hip.hipEventSynchronize.argtypes = [_vp]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), _vp, _vp]
+        self.hip.hipEventDestroy.argtypes = [_vp]
+        self.ev = []
+        for _ in range(n):
+            e = _vp()
+            if self.hip.hipEventCreate(ctypes.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            self.ev.append(e)
+
+    def record(self, i, stream):
+        if self.hip.hipEventRecord(self.ev[i], stream) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def sync(self, i):
+        if self.hip.hipEventSynchronize(self.ev[i]) != 0:
+            raise RuntimeError("hipEventSynchronize failed")
+
+    def ms(self, a, b):
+        out = ctypes.c_float()
+        if self.hip.hipEventElapsedTime(ctypes.byref(out), self.ev[a], self.ev[b]) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return out.value
+
+    def close(self):
+        for e in self.ev:
+            self.hip.hipEventDestroy(e)
+        self.ev = []
 
 
 class Context:

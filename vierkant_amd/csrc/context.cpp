@@ -69,4 +69,33 @@ int vkrt_sync(vkrt_ctx *ctx)
 
 void *vkrt_stream(const vkrt_ctx *ctx) { return ctx ? static_cast<void *>(ctx->stream) : nullptr; }
 
+int vkrt_device_alloc(vkrt_ctx *ctx, size_t bytes, void **out)
+{
+    return vkrt::guarded(ctx, [&] {
+        VKRT_REQUIRE(out, "null output pointer");
+        *out = nullptr;
+        VKRT_HIP(hipMalloc(out, bytes ? bytes : 1));
+    });
+}
+
+int vkrt_device_free(vkrt_ctx *ctx, void *ptr)
+{
+    return vkrt::guarded(ctx, [&] {
+        if(ptr) VKRT_HIP(hipFree(ptr));
+    });
+}
+
+int vkrt_copy(vkrt_ctx *ctx, void *dst, const void *src, size_t bytes, int kind, int flags)
+{
+    return vkrt::guarded(ctx, [&] {
+        VKRT_REQUIRE(kind >= 0 && kind <= 2, "copy kind must be 0 (h2d), 1 (d2h) or 2 (d2d)");
+        if(!bytes) return;
+        VKRT_REQUIRE(dst && src, "null copy pointer");
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                          : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        VKRT_HIP(hipMemcpyAsync(dst, src, bytes, k, ctx->stream));
+        if(!(flags & VKRT_ASYNC)) VKRT_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 }// extern "C"
