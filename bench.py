@@ -23,6 +23,22 @@ CLOSEST_BYTES_PER_RAY = 48   # 32 B ray in + 16 B hit out
 ANY_BYTES_PER_RAY = 33       # 32 B ray in + 1 B flag out
 
 
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/trace_closest_config2.json, each counter scaled by its plain-copy calibration), or None"""
+    path = os.path.join(ROOT, "profiles", "trace_closest_config2.json")
+    try:
+        with open(path) as f:
+            data = json.load(f)
+        return data["kernels"][kernel]["bytes_per_launch"], data["source"]
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
+def rays_per_step_of(primary, bounce):
+    return len(primary) + len(bounce)
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -53,11 +69,14 @@ def cpu_baseline(tris, rays, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-rays", type=int, default=65536)
+    ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (rays shard with no exchange; "
@@ -108,8 +127,35 @@ def main():
                                    (ref["t"].view(np.uint32) != got["t"].view(np.uint32))))
         verified = {"rays": 2 * args.verify_rays, "mismatches": mismatch}
 
+    if args.copy_mb:
+        src = vk.DeviceArray(ctx, args.copy_mb << 20, np.uint8)
+        dst = vk.DeviceArray(ctx, args.copy_mb << 20, np.uint8)
+        ctx.check(vk.lib().vkrt_copy(ctx.handle, dst.ptr, src.ptr, args.copy_mb << 20, 2, 0))
+        src.free()
+        dst.free()
+
     flags = vk.ASYNC
     ev = vk.HipEvents(5)
+
+    if args.sweep:
+        for block in (64, 128, 256):
+            for lds in (0, 8, 16, 32):
+                ctx.set_trace_config(block, lds)
+                ctx.trace_closest(bvh, d_primary, out=d_hits_p)
+                times = []
+                for _ in range(5):
+                    ev.record(0, stream)
+                    ctx.trace_closest(bvh, d_primary, out=d_hits_p, flags=flags)
+                    ev.record(1, stream)
+                    ctx.trace_closest(bvh, d_bounce, out=d_hits_b, flags=flags)
+                    ev.record(2, stream)
+                    ev.sync(2)
+                    times.append((ev.ms(0, 1), ev.ms(1, 2)))
+                p_ms, b_ms = np.median([t[0] for t in times]), np.median([t[1] for t in times])
+                print(json.dumps({"block": block, "lds": lds, "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4),
+                                  "grays_per_s": round(rays_per_step_of(primary, bounce) / ((p_ms + b_ms) * 1e-3) / 1e9, 4)}),
+                      flush=True)
+        return
 
     def step(timed):
         if timed:
@@ -151,6 +197,7 @@ def main():
     any_ms = float(np.mean(any_ms))
 
     launch_ms = float(np.mean(prim_ms + bounce_ms))
+    traffic, traffic_source = profiled_traffic("k_trace_closest")
     achieved = CLOSEST_BYTES_PER_RAY * len(primary) / (launch_ms * 1e-3) / 1e9
     result = {
         "metric": "Grays/s closest-hit (1 GPU)",
@@ -189,7 +236,8 @@ def main():
             "peak": MI355X_HBM_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / MI355X_HBM_GBPS, 6),
-            "traffic": None,
+            "traffic": None if traffic is None else round(traffic),
+            "traffic_source": traffic_source,
             "algorithmic_bytes_per_launch": CLOSEST_BYTES_PER_RAY * len(primary),
             "avg_launch_ms": round(launch_ms, 4),
         },

@@ -202,6 +202,19 @@ int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_
 int vkrt_bvh_get_info(const vkrt_bvh *bvh, vkrt_bvh_info *out);
 void vkrt_bvh_destroy(vkrt_bvh *bvh);
 
+/* Launch shape of the traversal kernels: threads per block (64, 128, 256) and traversal-stack entries
+ * kept in LDS per ray (0, 8, 16, 32; deeper entries live in scratch memory). Results do not depend on
+ * it. No reference counterpart: the Vulkan driver owns the traversal there. */
+typedef struct vkrt_trace_config
+{
+    uint32_t block_size;
+    uint32_t lds_stack_depth;
+    uint32_t reserved[2];
+} vkrt_trace_config;
+
+int vkrt_set_trace_config(vkrt_ctx *ctx, const vkrt_trace_config *cfg);
+int vkrt_get_trace_config(const vkrt_ctx *ctx, vkrt_trace_config *cfg);
+
 /* Closest hit through the BVH; results identical to vkrt_intersect_brute_closest. */
 int vkrt_trace_closest(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uint32_t n, vkrt_hit *out,
                        int flags);

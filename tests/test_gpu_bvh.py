@@ -141,3 +141,21 @@ def test_bvh_config2_primary_and_bounce(ctx, config2):
     few = np.arange(0, 65536, 64)
     assert_hits_equal(hits[few], orc.brute_closest(tris, primary[few]), "config2 vs oracle")
     assert_hits_equal(bh[few], orc.brute_closest(tris, bounce[few]), "config2 bounce vs oracle")
+
+
+def test_trace_configs_give_identical_results(ctx, config1):
+    """every launch shape (block size x LDS stack depth) returns the same records"""
+    pos, idx, tris, rays, want = config1
+    bvh = ctx.bvh_build(pos, idx)
+    saved = ctx.get_trace_config()
+    try:
+        for block in (64, 128, 256):
+            for lds in (0, 8, 16, 32):
+                ctx.set_trace_config(block, lds)
+                assert_hits_equal(ctx.trace_closest(bvh, rays), want, f"block={block} lds={lds}")
+                occ = ctx.trace_any(bvh, rays)
+                assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8))
+        with pytest.raises(vk.VkrtError):
+            ctx.set_trace_config(96, 16)
+    finally:
+        ctx.set_trace_config(*saved)

@@ -36,6 +36,11 @@ class BvhParams(ctypes.Structure):
                 ("reserved", ctypes.c_uint32 * 6)]
 
 
+class TraceConfig(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("lds_stack_depth", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
+
+
 class BvhInfo(ctypes.Structure):
     _fields_ = [("n_tri", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32),
                 ("depth", ctypes.c_uint32), ("sah_cost", ctypes.c_float), ("build_ms", ctypes.c_float),
@@ -68,6 +73,8 @@ _SIGNATURES = {
     "vkrt_bvh_refit": (_i32, [_vp, _vp, _vp, _u32, _i32]),
     "vkrt_bvh_get_info": (_i32, [_vp, ctypes.POINTER(BvhInfo)]),
     "vkrt_bvh_destroy": (None, [_vp]),
+    "vkrt_set_trace_config": (_i32, [_vp, ctypes.POINTER(TraceConfig)]),
+    "vkrt_get_trace_config": (_i32, [_vp, ctypes.POINTER(TraceConfig)]),
     "vkrt_trace_closest": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_trace_any": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_icosphere_size": (_i32, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
@@ -301,6 +308,14 @@ class Context:
         self.check(lib().vkrt_bvh_build(self._h, _ptr(positions)[0], n_vert, stride, _ptr(indices)[0], n_tri,
                                         ctypes.byref(params), _flags([positions, indices]), ctypes.byref(h)))
         return Bvh(self, h)
+
+    def set_trace_config(self, block_size, lds_stack_depth):
+        self.check(lib().vkrt_set_trace_config(self._h, ctypes.byref(TraceConfig(block_size, lds_stack_depth))))
+
+    def get_trace_config(self):
+        c = TraceConfig()
+        self.check(lib().vkrt_get_trace_config(self._h, ctypes.byref(c)))
+        return c.block_size, c.lds_stack_depth
 
     def trace_closest(self, bvh, rays, out=None, flags=0):
         n = _n(rays, 8)

@@ -26,37 +26,14 @@
 
 #include "context.hpp"
 #include "kernels.hpp"
+#include "bvh_types.hpp"
 #include "radix_sort.hpp"
 #include "vierkant/rt_math.hpp"
 
 using namespace vierkant::rt;
 
-struct vkrt_bvh_node
-{
-    float4 a;// child 0: lo.x hi.x lo.y hi.y
-    float4 b;// child 1: lo.x hi.x lo.y hi.y
-    float4 c;// lo.z hi.z of child 0, lo.z hi.z of child 1
-    int4 d;  // child refs: >= 0 internal node, < 0 leaf ~((first << 3) | (count - 1))
-};
-
-struct vkrt_bvh
-{
-    int device = 0;
-    uint32_t n_tri = 0, n_vert = 0, n_nodes = 0, max_leaf = 4;
-    int32_t root_ref = 0;
-    bool empty = true;
-    vkrt::device_buffer nodes, tris;
-    vkrt::device_buffer indices, order, child, range, parent, nbox, tbox, flags, pre, err;
-    vkrt_bvh_info info{};
-};
-
 namespace vkrt
 {
-
-constexpr int32_t MAX_LEAF = 8;
-constexpr int STACK_SIZE = 64;
-constexpr float BOX_PAD = 0x1p-16f; // relative growth of every stored box coordinate
-constexpr float T_PAD = 0x1p-16f;   // relative growth of every slab interval
 
 __device__ __forceinline__ vec3 ld3(const float *p) { return vec3{p[0], p[1], p[2]}; }
 
@@ -281,155 +258,6 @@ __global__ void k_reorder(const uint32_t *__restrict__ order, uint32_t n, const 
     tris[3 * (size_t)j + 0] = make_float4(p[0], p[1], p[2], p[3]);
     tris[3 * (size_t)j + 1] = make_float4(p[4], p[5], p[6], p[7]);
     tris[3 * (size_t)j + 2] = make_float4(p[8], __uint_as_float(src), 0.f, 0.f);
-}
-
-// ---------------------------------------------------------------------------------------------
-// traversal
-// ---------------------------------------------------------------------------------------------
-struct ray_setup
-{
-    vec3 o, d, inv;
-    float tmin, tmax;
-};
-
-__device__ __forceinline__ ray_setup setup_ray(const float4 *__restrict__ rays, uint32_t i)
-{
-    ray_setup r;
-    float4 a = rays[2 * (size_t)i], b = rays[2 * (size_t)i + 1];
-    r.o = vec3{a.x, a.y, a.z};
-    r.tmin = a.w;
-    r.d = vec3{b.x, b.y, b.z};
-    r.tmax = b.w;
-    // a zero direction component becomes +-1e-20: that axis then gives t = 0 on a slab plane through
-    // the origin and |t| >= 1e20 |b - o| elsewhere, i.e. "inside the slab for all t" or "never"
-    auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
-    r.inv = vec3{1.0f / safe(r.d.x), 1.0f / safe(r.d.y), 1.0f / safe(r.d.z)};
-    return r;
-}
-
-//! Conservative ray/box interval. Each slab value t = RN(RN(b - o) * RN(1/d)) is within 3 eps
-//! (eps = 2^-24) relative of (b - o)/d, so growing the interval by T_PAD |t| (T_PAD = 2^-16) keeps the
-//! exact interval inside it; boxes were grown by BOX_PAD relative at build time to absorb the rounding
-//! of the reference test's hit point.
-__device__ __forceinline__ void slab_interval(const ray_setup &r, float lox, float hix, float loy, float hiy, float loz,
-                                              float hiz, float &tn, float &tf)
-{
-    float x0 = (lox - r.o.x) * r.inv.x, x1 = (hix - r.o.x) * r.inv.x;
-    float y0 = (loy - r.o.y) * r.inv.y, y1 = (hiy - r.o.y) * r.inv.y;
-    float z0 = (loz - r.o.z) * r.inv.z, z1 = (hiz - r.o.z) * r.inv.z;
-    float n_ = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
-    float f_ = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-    tn = n_ - fabsf(n_) * T_PAD;
-    tf = f_ + fabsf(f_) * T_PAD;
-}
-
-__device__ __forceinline__ void leaf_decode(int ref, uint32_t &first, uint32_t &count)
-{
-    uint32_t v = static_cast<uint32_t>(~ref);
-    first = v >> 3;
-    count = (v & 7u) + 1;
-}
-
-template<bool ANY>
-__device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
-                                         int32_t root, const ray_setup &r, float &best_t, float &best_u,
-                                         float &best_v, uint32_t &best_prim)
-{
-    int32_t stack[STACK_SIZE];
-    int sp = 0;
-    int32_t ref = root;
-    best_t = r.tmax;
-    best_prim = VKRT_INVALID_PRIM;
-    while(true)
-    {
-        if(ref >= 0)
-        {
-            const vkrt_bvh_node &nd = nodes[ref];
-            float4 a = nd.a, b = nd.b, c = nd.c;
-            int4 d = nd.d;
-            float tn0, tf0, tn1, tf1;
-            slab_interval(r, a.x, a.y, a.z, a.w, c.x, c.y, tn0, tf0);
-            slab_interval(r, b.x, b.y, b.z, b.w, c.z, c.w, tn1, tf1);
-            const bool h0 = tn0 <= tf0 && tf0 >= r.tmin && tn0 <= best_t;
-            const bool h1 = tn1 <= tf1 && tf1 >= r.tmin && tn1 <= best_t;
-            if(h0 && h1)
-            {
-                const bool first0 = tn0 <= tn1;
-                stack[sp++] = first0 ? d.y : d.x;
-                ref = first0 ? d.x : d.y;
-                continue;
-            }
-            if(h0 || h1)
-            {
-                ref = h0 ? d.x : d.y;
-                continue;
-            }
-        }
-        else
-        {
-            uint32_t first, count;
-            leaf_decode(ref, first, count);
-            for(uint32_t k = 0; k < count; ++k)
-            {
-                const float4 *tr = tris + 3 * (size_t)(first + k);
-                float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
-                float t, u, v;
-                if(mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y}, vec3{p1.z, p1.w, p2.x}, t, u,
-                           v) &&
-                   t > r.tmin)
-                {
-                    const uint32_t prim = __float_as_uint(p2.y);
-                    if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim))
-                    {
-                        best_t = t;
-                        best_u = u;
-                        best_v = v;
-                        best_prim = prim;
-                        if(ANY) return true;
-                    }
-                }
-            }
-        }
-        if(sp == 0) break;
-        ref = stack[--sp];
-    }
-    return best_prim != VKRT_INVALID_PRIM;
-}
-
-__global__ void __launch_bounds__(256) k_trace_closest(const vkrt_bvh_node *__restrict__ nodes,
-                                                       const float4 *__restrict__ tris, int32_t root, bool empty,
-                                                       const float4 *__restrict__ rays, uint32_t n,
-                                                       vkrt_hit *__restrict__ hits)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    vkrt_hit h{__int_as_float(0x7f800000), 0.f, 0.f, VKRT_INVALID_PRIM};
-    if(!empty)
-    {
-        ray_setup r = setup_ray(rays, i);
-        float t, u = 0.f, v = 0.f;
-        uint32_t prim;
-        if(traverse<false>(nodes, tris, root, r, t, u, v, prim)) h = vkrt_hit{t, u, v, prim};
-    }
-    hits[i] = h;
-}
-
-__global__ void __launch_bounds__(256) k_trace_any(const vkrt_bvh_node *__restrict__ nodes,
-                                                   const float4 *__restrict__ tris, int32_t root, bool empty,
-                                                   const float4 *__restrict__ rays, uint32_t n,
-                                                   uint8_t *__restrict__ occluded)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    uint8_t occ = 0;
-    if(!empty)
-    {
-        ray_setup r = setup_ray(rays, i);
-        float t, u, v;
-        uint32_t prim;
-        occ = traverse<true>(nodes, tris, root, r, t, u, v, prim) ? 1 : 0;
-    }
-    occluded[i] = occ;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -660,44 +488,6 @@ void vkrt_bvh_destroy(vkrt_bvh *bvh)
     if(!bvh) return;
     (void)hipSetDevice(bvh->device);
     delete bvh;
-}
-
-int vkrt_trace_closest(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uint32_t n, vkrt_hit *out,
-                       int flags)
-{
-    return guarded(ctx, [&] {
-        VKRT_REQUIRE(bvh, "null bvh");
-        VKRT_REQUIRE(bvh->device == ctx->device, "bvh belongs to another device");
-        VKRT_REQUIRE(n == 0 || (rays && out), "null array argument");
-        if(!n) return;
-        const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
-        vkrt_hit *dh = output(ctx, 2, out, n, flags);
-        k_trace_closest<<<div_up(n, 256), 256, 0, ctx->stream>>>(
-                bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref, bvh->empty,
-                reinterpret_cast<const float4 *>(dr), n, dh);
-        VKRT_HIP(hipGetLastError());
-        finish_output(ctx, 2, out, n, flags);
-        finish_call(ctx, flags);
-    });
-}
-
-int vkrt_trace_any(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uint32_t n, uint8_t *occluded,
-                   int flags)
-{
-    return guarded(ctx, [&] {
-        VKRT_REQUIRE(bvh, "null bvh");
-        VKRT_REQUIRE(bvh->device == ctx->device, "bvh belongs to another device");
-        VKRT_REQUIRE(n == 0 || (rays && occluded), "null array argument");
-        if(!n) return;
-        const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
-        uint8_t *docc = output(ctx, 2, occluded, n, flags);
-        k_trace_any<<<div_up(n, 256), 256, 0, ctx->stream>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(),
-                                                             bvh->root_ref, bvh->empty,
-                                                             reinterpret_cast<const float4 *>(dr), n, docc);
-        VKRT_HIP(hipGetLastError());
-        finish_output(ctx, 2, occluded, n, flags);
-        finish_call(ctx, flags);
-    });
 }
 
 }// extern "C"

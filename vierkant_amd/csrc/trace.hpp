@@ -1,0 +1,17 @@
+// trace.hpp - launch of the BVH traversal kernels with a given launch shape.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vkrt.h"
+
+namespace vkrt
+{
+
+bool valid_trace_config(const vkrt_trace_config &c);
+
+//! closest (any = false, out = vkrt_hit[n]) or any hit (any = true, out = uint8_t[n]) for n device rays
+void launch_trace(hipStream_t s, const vkrt_trace_config &cfg, bool any, const vkrt_bvh *bvh, const float4 *rays,
+                  uint32_t n, void *out);
+
+}// namespace vkrt
