@@ -88,7 +88,7 @@ struct vkrt_ctx
     bool own_stream = false;
     std::string last_error;
     int cu_count = 256;
-    vkrt_trace_config trace_cfg{128, 16, {0, 0}};
+    vkrt_trace_config trace_cfg{256, 16, {0, 0}};// fastest shape of the block x LDS-depth sweep on MI355X
     //! staging slots for host-pointer calls (inputs/outputs copied through these)
     vkrt::device_buffer staging[8];
     //! scratch for multi-pass kernels (keys, counters)
