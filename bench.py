@@ -66,6 +66,39 @@ def cpu_baseline(tris, rays, budget_s):
                       "extrapolated to rays/s"}
 
 
+def path_tracer_frame(ctx, spp):
+    """one PBRPathTracer batch of the Cornell-style room (12 quads + 200 spheres + the jittered icosphere,
+    276,504 triangles) at 1920x1080, max_trace_depth 8, `spp` samples per pixel"""
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+    scene = W.config3_scene()
+    s = vk.default_settings()
+    s.resolution[0], s.resolution[1] = 1920, 1080
+    s.num_samples, s.max_trace_depth = spp, 8
+    pt = vk.PathTracer(ctx, s, seed=0)
+    pt.set_scene(scene)
+    cam = W.config3_camera()
+    s_warm = vk.default_settings()
+    s_warm.resolution[0], s_warm.resolution[1] = 1920, 1080
+    s_warm.num_samples, s_warm.max_trace_depth = 1, 8
+    pt.settings = s_warm
+    pt.render(cam)                      # warm-up batch (1 spp)
+    pt.reset_accumulator()
+    pt.settings = s
+    pt.render(cam)
+    st = pt.stats()
+    px = pt.pixels()
+    seg_bytes = 332      # per path segment: SoA ray, hit and path state in/out, 3 indices + 3 vertices, material
+    out = {"workload": "1920x1080, max_trace_depth 8, %d spp, 276,504 triangles" % spp,
+           "frame_ms": round(st.raytrace_ms, 3), "samples": int(st.samples), "segments": int(st.segments),
+           "samples_per_s": round(st.samples / (st.raytrace_ms * 1e-3), 1),
+           "segments_per_s": round(st.segments / (st.raytrace_ms * 1e-3), 1),
+           "segment_gbps": round(seg_bytes * st.segments / (st.raytrace_ms * 1e-3) / 1e9, 2),
+           "mean_radiance": float(px["radiance"].mean())}
+    pt.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +108,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-rays", type=int, default=65536)
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
+    ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -245,6 +279,11 @@ def main():
     }
     ctx.sync()
     bvh.close()
+    for a in (d_primary, d_hits_p, d_bounce, d_shadow, d_hits_b, d_occ):
+        a.free()
+    if args.pt_spp:
+        log("path tracer frame")
+        result["config"]["path_tracer"] = path_tracer_frame(ctx, args.pt_spp)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         allrays = np.concatenate([primary, bounce])

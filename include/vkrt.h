@@ -27,6 +27,11 @@
  *   vkrt_trace_closest / _any      <- TraceRay closest-hit / ACCEPT_FIRST_HIT shadow queries
  *                                     (shaders/slang/raypipeline.slang:182, ray/visibility_test.slang:45-87)
  *   vkrt_icosphere / vkrt_plane    <- Geometry::IcoSphere / Geometry::Plane       (src/Geometry.cpp:566-613, :313-371)
+ *   vkrt_pt_*                      <- PBRPathTracer: settings_t (include/vierkant/PBRPathTracer.hpp:42-110),
+ *                                     render_scene / reset_accumulator / current_batch
+ *                                     (src/PBRPathTracer.cpp:184-255, :677-682), the raygen / closest-hit / miss
+ *                                     programs (shaders/slang/raypipeline.slang:91-270, :360-680, :746-761) and
+ *                                     pixel_data_t (shaders/slang/ray/pixel_data.slang:9-21)
  */
 #ifndef VKRT_H
 #define VKRT_H
@@ -221,6 +226,110 @@ int vkrt_trace_closest(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays,
 /* Any hit (shadow query) through the BVH; results identical to vkrt_intersect_brute_any. */
 int vkrt_trace_any(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uint32_t n, uint8_t *occluded,
                    int flags);
+
+/* ---------------------------------------------------------------------------------------------
+ * path tracer (PBRPathTracer)
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct vkrt_pt vkrt_pt;
+
+/* PBRPathTracer::settings_t fields that affect the traced image (PBRPathTracer.hpp:42-110). Defaults
+ * (vkrt_pt_default_settings): 1280x720, max_num_batches 0, num_samples 1, max_trace_depth 6,
+ * max_path_beta 0, suspend_trace_when_done 1, disable_material 0, suppress_reset 0. */
+typedef struct vkrt_pt_settings
+{
+    uint32_t resolution[2];
+    uint32_t max_num_batches;
+    uint32_t num_samples;
+    uint32_t max_trace_depth;
+    float max_path_beta;
+    uint32_t suspend_trace_when_done;
+    uint32_t disable_material;
+    uint32_t suppress_reset;
+    uint32_t reserved[7];
+} vkrt_pt_settings;
+
+/* material_t subset evaluated by the tracer (shaders/slang/ray/types.slang:64-107): base color, emission
+ * (rgb, a = strength), metal/roughness, transmission + ior, clearcoat, sheen, diffuse transmission. */
+typedef struct vkrt_material
+{
+    float color[4];
+    float emission[4];
+    float metalness, roughness, transmission, ior;
+    float clearcoat, clearcoat_roughness, sheen_roughness, diffuse_transmission;
+    float sheen_color[4];
+    float diffuse_transmission_color[4];
+    uint32_t two_sided; /* its triangles are never culled as back faces (VK_GEOMETRY_INSTANCE_TRIANGLE_FACING_CULL_DISABLE) */
+    uint32_t reserved[3];
+} vkrt_material;
+
+/* A flat scene: an indexed triangle mesh with shading normals, split into objects (entities) that each
+ * carry one material. All pointers are host pointers; the tracer keeps device copies. */
+typedef struct vkrt_pt_scene
+{
+    const float *positions; /* 3 floats per vertex */
+    const float *normals;   /* 3 floats per vertex */
+    uint32_t n_vert;
+    const uint32_t *indices; /* 3 per triangle */
+    uint32_t n_tri;
+    const uint32_t *tri_object;      /* object index per triangle */
+    const uint32_t *object_material; /* material index per object */
+    uint32_t n_objects;
+    const vkrt_material *materials;
+    uint32_t n_materials;
+    float environment; /* scale of the procedural sky (environment_white) */
+} vkrt_pt_scene;
+
+/* Pinhole / thin-lens camera (camera_params_t): eye, unit forward and up vectors, vertical field of view
+ * in radians, aperture diameter (0 = pinhole) and focal distance. */
+typedef struct vkrt_camera
+{
+    float eye[3];
+    float fov_y;
+    float forward[3];
+    float aperture;
+    float up[3];
+    float focal_distance;
+} vkrt_camera;
+
+/* pixel_data_t (32 B): accumulated radiance, this batch's coverage, projected depth, octahedral normal,
+ * object id (0xFFFF on a miss), albedo RGBA8 of the first hit. */
+typedef struct vkrt_pixel
+{
+    float radiance[3];
+    float alpha;
+    float depth;
+    uint32_t normal;
+    uint32_t object_id;
+    uint32_t albedo;
+} vkrt_pixel;
+
+typedef struct vkrt_pt_stats
+{
+    float raytrace_ms;       /* GPU time of the last traced batch */
+    uint32_t traced;         /* 1 if the last vkrt_pt_render traced a batch */
+    uint64_t segments;       /* path segments (extension rays) traced in the last batch */
+    uint64_t samples;        /* camera samples in the last batch */
+} vkrt_pt_stats;
+
+void vkrt_pt_default_settings(vkrt_pt_settings *out);
+/* seed: create_info_t::seed; each traced batch draws its random seed from std::default_random_engine(seed) */
+int vkrt_pt_create(vkrt_ctx *ctx, const vkrt_pt_settings *settings, uint32_t seed, vkrt_pt **out);
+void vkrt_pt_destroy(vkrt_pt *pt);
+int vkrt_pt_set_settings(vkrt_pt *pt, const vkrt_pt_settings *settings);
+int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out);
+/* Upload the scene and build its BVH (the RayBuilder step of the reference's render_scene). */
+int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *scene);
+/* One render_scene call: traces one batch of settings.num_samples samples per pixel into the running mean
+ * unless max_num_batches is reached with suspend_trace_when_done set; then advances the batch index as
+ * min(batch + 1, max_num_batches). A resolution change resets the accumulator. */
+int vkrt_pt_render(vkrt_pt *pt, const vkrt_camera *camera);
+/* Reset the batch index to 0 unless settings.suppress_reset is set (PBRPathTracer.cpp:677-680). */
+int vkrt_pt_reset_accumulator(vkrt_pt *pt);
+uint32_t vkrt_pt_current_batch(const vkrt_pt *pt);
+/* Copy the W*H pixel buffer (vkrt_pixel, row-major from the top-left) to `out` (host, or device with flags). */
+int vkrt_pt_read_pixels(vkrt_pt *pt, vkrt_pixel *out, int flags);
+int vkrt_pt_get_stats(const vkrt_pt *pt, vkrt_pt_stats *out);
 
 /* ---------------------------------------------------------------------------------------------
  * procedural geometry (host) used for the benchmark scenes

@@ -1,0 +1,779 @@
+/*
+ * vkpt_oracle.c - CPU reference integrator for the path tracer (TEST INFRASTRUCTURE ONLY).
+ *
+ * A plain-C restatement, pixel by pixel, of the reference's ray-tracing programs for the scenes the
+ * tracer supports (no textures, no lights[] / NEE, no participating media, no iridescence/dispersion):
+ *   raygen trace() + main             shaders/slang/raypipeline.slang:58-270
+ *   closest_hit (surface subset)      shaders/slang/raypipeline.slang:360-680
+ *   miss_default                      shaders/slang/raypipeline.slang:746-761
+ *   sample_disney + lobes             shaders/slang/ray/bsdf_disney.slang:44-296
+ *   bsdf / sampling / random / hash   shaders/slang/utils/{bsdf,sampling,random,hash}.slang
+ *   environment_white                 shaders/slang/utils/procedural_environment.slang:7-11
+ * Closest hits use the oracle's restatement of intersect(Triangle, Ray) (vkoracle.c) with tmin < t < tmax,
+ * lowest index on equal t, and back-face culling (det < 0) for rays in air unless the triangle's material
+ * is two-sided, through a simple median-split BVH whose box tests are done in double precision on boxes
+ * grown by 2^-16 (never culls a triangle the test accepts).
+ * Used only by tests/ as the judge of the GPU image (RMSE at equal samples per pixel).
+ */
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "vkoracle.h"
+#include "vkpt_oracle.h"
+
+typedef struct
+{
+    float x, y, z;
+} v3;
+
+static inline v3 V3(float x, float y, float z) { return (v3){x, y, z}; }
+static inline v3 add(v3 a, v3 b) { return V3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scl(v3 a, float s) { return V3(a.x * s, a.y * s, a.z * s); }
+static inline v3 mul(v3 a, v3 b) { return V3(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline v3 neg(v3 a) { return V3(-a.x, -a.y, -a.z); }
+static inline float dot(v3 a, v3 b)
+{
+    float x = a.x * b.x, y = a.y * b.y, z = a.z * b.z;
+    return (x + y) + z;
+}
+static inline v3 cross(v3 x, v3 y) { return V3(x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y); }
+static inline v3 nrmz(v3 v)
+{
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return scl(v, inv);
+}
+static inline float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+static inline float lerpf(float a, float b, float t) { return a + t * (b - a); }
+static inline v3 lerp3(v3 a, v3 b, float t) { return add(a, scl(sub(b, a), t)); }
+static inline v3 reflect3(v3 i, v3 n) { return sub(i, scl(n, 2.0f * dot(n, i))); }
+static inline float max3(v3 v) { return v.x > v.y ? fmaxf(v.x, v.z) : fmaxf(v.y, v.z); }
+static inline v3 refract3(v3 i, v3 n, float eta)
+{
+    float ndi = dot(n, i);
+    float k = 1.0f - eta * eta * (1.0f - ndi * ndi);
+    if(k < 0.0f) return V3(0, 0, 0);
+    return sub(scl(i, eta), scl(n, eta * ndi + sqrtf(k)));
+}
+
+static const float PI = 3.1415926535897932384626433832795f;
+static const float ONE_OVER_PI = 0.31830988618379067153776752674503f;
+static const float EPS = 1e-4f, PDF_EPS = 1e-3f, DEFAULT_TMAX = 10000.0f;
+
+/* ---- RNG (hash.slang, random.slang, sampling.slang) ------------------------------------------ */
+static inline float rnd(uint32_t *s)
+{
+    *s = 1664525u * *s + 1013904223u;
+    return (float)(*s & 0x00FFFFFFu) / (float)0x01000000;
+}
+
+static inline uint32_t brev(uint32_t x)
+{
+    uint32_t r = 0;
+    for(int i = 0; i < 32; ++i) r |= ((x >> i) & 1u) << (31 - i);
+    return r;
+}
+
+/* ---- sampling -------------------------------------------------------------------------------- */
+typedef struct
+{
+    v3 tx, ty, n;
+} frame;
+
+static frame local_frame(v3 d)
+{
+    float len2 = d.x * d.x + d.y * d.y;
+    float len = sqrtf(len2);
+    frame f;
+    f.tx = len2 > 0.f ? V3(-d.y / len, d.x / len, 0.f / len) : V3(1, 0, 0);
+    f.ty = cross(d, f.tx);
+    f.n = d;
+    return f;
+}
+static inline v3 to_local(frame f, v3 v) { return V3(dot(f.tx, v), dot(f.ty, v), dot(f.n, v)); }
+static inline v3 to_world(frame f, v3 v) { return add(add(scl(f.tx, v.x), scl(f.ty, v.y)), scl(f.n, v.z)); }
+static inline v3 sph(float s, float c, float phi) { return V3(s * cosf(phi), s * sinf(phi), c); }
+
+static v3 hemi_cos(float xi_x, float xi_y)
+{
+    float phi = 2.0f * PI * xi_y;
+    float c = sqrtf(fmaxf(1.0f - xi_x, 0.0f));
+    float s = sqrtf(fmaxf(1.0f - c * c, 0.0f));
+    return sph(s, c, phi);
+}
+
+static v3 ggx_vndf(float xi_x, float xi_y, v3 wi, float a)
+{
+    v3 w = nrmz(V3(wi.x * a, wi.y * a, wi.z));
+    float phi = (2.0f * xi_x - 1.0f) * PI;
+    float z = fmaf(1.0f - xi_y, 1.0f + w.z, -w.z);
+    float st = sqrtf(clampf(1.0f - z * z, 0.0f, 1.0f));
+    v3 c = V3(st * cosf(phi), st * sinf(phi), z);
+    v3 m = add(c, w);
+    return nrmz(V3(m.x * a, m.y * a, m.z));
+}
+
+static v3 gtr1_sample(float xi_x, float xi_y, float r)
+{
+    float a = fmaxf(0.001f, r), a2 = a * a;
+    float phi = xi_y * 2.0f * PI;
+    float c = sqrtf((1.0f - powf(a2, 1.0f - xi_x)) / (1.0f - a2));
+    float s = clampf(sqrtf(1.0f - c * c), 0.0f, 1.0f);
+    return sph(s, c, phi);
+}
+
+/* ---- bsdf.slang ------------------------------------------------------------------------------ */
+static inline float schlick(float u)
+{
+    float m = clampf(1.0f - u, 0.0f, 1.0f);
+    return m * m * m * m * m;
+}
+static float diel_fresnel(float ci, float eta)
+{
+    float s2 = eta * eta * (1.0f - ci * ci);
+    if(s2 >= 1.0f) return 1.0f;
+    float ct = sqrtf(fmaxf(1.0f - s2, 0.0f));
+    float rs = (eta * ct - ci) / (eta * ct + ci);
+    float rp = (eta * ci - ct) / (eta * ci + ct);
+    return 0.5f * (rs * rs + rp * rp);
+}
+static inline float gtr2(float nh, float a)
+{
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * nh * nh;
+    return a2 / (PI * t * t);
+}
+static inline float smith(float nv, float ag)
+{
+    float a = ag * ag, b = nv * nv;
+    return 1.0f / (nv + sqrtf(a + b - a * b));
+}
+
+typedef struct
+{
+    v3 color;
+    v3 emission;
+    float emission_strength;
+    float metal, rough, trans, ior, cc, cc_rough, sheen_rough, dt;
+    v3 sheen, dt_color;
+    int two_sided;
+} mat_t;
+
+typedef struct
+{
+    v3 dir;
+    float pdf;
+    v3 F;
+    int transmission;
+} bs_t;
+
+static v3 ev_diel_refl(float r, v3 color, float eta, v3 V, v3 N, v3 L, v3 H, float *pdf)
+{
+    *pdf = 0.0f;
+    if(dot(N, L) <= 0.0f) return V3(0, 0, 0);
+    float F = diel_fresnel(dot(V, H), eta);
+    float D = gtr2(dot(N, H), r);
+    *pdf = 0.5f * D * smith(fabsf(dot(N, V)), r) * F;
+    float G = smith(fabsf(dot(N, L)), r) * smith(fabsf(dot(N, V)), r);
+    return scl(scl(scl(color, F), D), G);
+}
+
+static v3 ev_diel_refr(float r, v3 color, float eta, v3 V, v3 N, v3 L, v3 H, float *pdf)
+{
+    *pdf = 0.0f;
+    if(dot(N, L) >= 0.0f) return V3(0, 0, 0);
+    float ds = dot(L, H) + dot(V, H) * eta;
+    if(fabsf(ds) < 1e-7f) return V3(0, 0, 0);
+    float F = diel_fresnel(fabsf(dot(V, H)), eta);
+    float D = gtr2(dot(N, H), r);
+    *pdf = 2.0f * smith(fabsf(dot(N, V)), r) * D * fabsf(dot(V, H)) * (1.0f - F) * fabsf(dot(L, H)) * eta * eta /
+           (ds * ds);
+    float G = smith(fabsf(dot(N, L)), r) * smith(fabsf(dot(N, V)), r);
+    v3 c = scl(color, 1.0f - F);
+    c = scl(c, D);
+    c = scl(c, G);
+    c = scl(c, fabsf(dot(V, H)));
+    c = scl(c, fabsf(dot(L, H)));
+    c = scl(c, 4.0f);
+    c = scl(c, eta);
+    c = scl(c, eta);
+    float dd = ds * ds;
+    return V3(c.x / dd, c.y / dd, c.z / dd);
+}
+
+static v3 ev_spec(float r, v3 cs, v3 V, v3 N, v3 L, v3 H, float *pdf)
+{
+    *pdf = 0.0f;
+    if(dot(N, L) <= 0.0f) return V3(0, 0, 0);
+    float D = gtr2(dot(N, H), r);
+    *pdf = 0.5f * D * smith(fabsf(dot(N, V)), r);
+    float FH = schlick(dot(L, H));
+    v3 F = lerp3(cs, V3(1, 1, 1), FH);
+    float G = smith(fabsf(dot(N, L)), r) * smith(fabsf(dot(N, V)), r);
+    return scl(scl(F, D), G);
+}
+
+static v3 ev_cc(float ccr, float cc, v3 V, v3 N, v3 L, v3 H, float *pdf)
+{
+    *pdf = 0.0f;
+    if(dot(N, L) <= 0.0f) return V3(0, 0, 0);
+    float D = gtr2(dot(N, H), ccr);
+    *pdf = D * dot(N, H) / (4.0f * dot(V, H));
+    float FH = schlick(dot(L, H));
+    float F = lerpf(0.04f, 1.0f, FH);
+    float G = smith(dot(N, L), 0.25f) * smith(fabsf(dot(N, V)), 0.25f);
+    float x = 0.25f * cc * F * D * G;
+    return V3(x, x, x);
+}
+
+static v3 ev_diffuse(float r, float sr, v3 color, v3 sheen, float metal, v3 V, v3 N, v3 L, v3 H, float *pdf)
+{
+    *pdf = 0.0f;
+    float NoL = dot(N, L);
+    if(NoL <= 0.0f) return V3(0, 0, 0);
+    *pdf = NoL * ONE_OVER_PI;
+    float LoH = dot(L, H);
+    float FL = schlick(NoL), FV = schlick(dot(N, V)), FH = schlick(LoH);
+    float Fd90 = 0.5f + 2.0f * LoH * LoH * r;
+    float Fd = lerpf(1.0f, Fd90, FL) * lerpf(1.0f, Fd90, FV);
+    v3 Fs = scl(sheen, FH * sr);
+    return scl(scl(add(scl(color, Fd), Fs), ONE_OVER_PI), 1.0f - metal);
+}
+
+static bs_t sample_disney(mat_t m, v3 N, v3 V, float eta, uint32_t *rng)
+{
+    bs_t ret;
+    ret.pdf = 0.0f;
+    ret.F = V3(0, 0, 0);
+    ret.transmission = 0;
+    ret.dir = V3(0, 0, 0);
+    float diffuse_ratio = 0.5f * (1.0f - m.metal);
+    float trans_weight = (1.0f - m.metal) * m.trans;
+    m.rough = fmaxf(0.001f, m.rough * m.rough);
+    m.sheen_rough = fmaxf(0.001f, m.sheen_rough);
+    v3 spec_color = lerp3(V3(0.08f, 0.08f, 0.08f), m.color, m.metal);
+    frame fr = local_frame(N);
+    float xi_x = rnd(rng), xi_y = rnd(rng);
+    v3 H = to_world(fr, ggx_vndf(xi_x, xi_y, to_local(fr, V), m.rough));
+    if(dot(V, H) < 0.0f) H = neg(H);
+    if(rnd(rng) < trans_weight)
+    {
+        v3 R = reflect3(neg(V), H);
+        float F = diel_fresnel(fabsf(dot(R, H)), eta);
+        if(xi_y < F)
+        {
+            ret.dir = nrmz(R);
+            (void)rnd(rng);
+            ret.F = ev_diel_refl(m.rough, m.color, eta, V, N, ret.dir, H, &ret.pdf);
+            ret.pdf *= 1.0f - 0.0f;
+        }
+        else
+        {
+            v3 rf = refract3(neg(V), H, eta);
+            if(dot(rf, rf) < 1e-7f)
+            {
+                ret.dir = nrmz(reflect3(neg(V), H));
+                ret.F = ev_diel_refl(m.rough, m.color, eta, V, N, ret.dir, H, &ret.pdf);
+            }
+            else
+            {
+                ret.dir = nrmz(rf);
+                ret.F = ev_diel_refr(m.rough, m.color, eta, V, N, ret.dir, H, &ret.pdf);
+                ret.transmission = 1;
+            }
+        }
+        ret.F = scl(ret.F, trans_weight);
+        ret.pdf *= trans_weight;
+    }
+    else
+    {
+        if(rnd(rng) < diffuse_ratio)
+        {
+            float dt = m.dt;
+            if(dt > 0.0f && rnd(rng) < dt)
+            {
+                v3 l = hemi_cos(xi_x, xi_y);
+                ret.dir = to_world(fr, V3(l.x, l.y, -l.z));
+                ret.F = scl(m.dt_color, dt * ONE_OVER_PI);
+                ret.pdf = l.z * ONE_OVER_PI * diffuse_ratio * dt;
+                ret.transmission = 1;
+            }
+            else
+            {
+                ret.dir = to_world(fr, hemi_cos(xi_x, xi_y));
+                v3 Hd = nrmz(add(ret.dir, V));
+                ret.F = scl(ev_diffuse(m.rough, m.sheen_rough, m.color, m.sheen, m.metal, V, N, ret.dir, Hd, &ret.pdf),
+                            1.0f - dt);
+                ret.pdf *= diffuse_ratio * (1.0f - dt);
+            }
+        }
+        else
+        {
+            float psr = 1.0f / (1.0f + m.cc);
+            if(rnd(rng) < psr)
+            {
+                ret.dir = nrmz(reflect3(neg(V), H));
+                (void)rnd(rng);
+                ret.F = ev_spec(m.rough, spec_color, V, N, ret.dir, H, &ret.pdf);
+                ret.pdf *= psr * (1.0f - diffuse_ratio) * (1.0f - 0.0f);
+            }
+            else
+            {
+                v3 Hc = to_world(fr, gtr1_sample(xi_x, xi_y, lerpf(0.1f, 0.001f, m.cc)));
+                if(dot(V, Hc) < 0.0f) Hc = neg(Hc);
+                ret.dir = nrmz(reflect3(neg(V), Hc));
+                ret.F = ev_cc(m.cc_rough, m.cc, V, N, ret.dir, Hc, &ret.pdf);
+                ret.pdf *= (1.0f - psr) * (1.0f - diffuse_ratio);
+            }
+        }
+        ret.F = scl(ret.F, 1.0f - trans_weight);
+        ret.pdf *= (1.0f - trans_weight);
+    }
+    if(ret.pdf != ret.pdf) ret.pdf = 0.0f;
+    return ret;
+}
+
+static v3 env_white(v3 d)
+{
+    v3 sky = V3(1.3f, 1.3f, 1.3f), hor = V3(0.6f, 0.6f, 0.6f), gr = V3(0.1f, 0.1f, 0.1f);
+    return d.y > 0.f ? lerp3(hor, sky, d.y) : lerp3(hor, gr, -d.y);
+}
+
+static float round_half(float x)
+{
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    uint32_t sign = b & 0x80000000u, a = b & 0x7fffffffu;
+    if(a >= 0x47800000u || a < 0x38800000u) return x;
+    uint32_t lsb = (a >> 13) & 1u;
+    uint32_t r = ((a + 0xfffu + lsb) & ~0x1fffu) | sign;
+    float out;
+    memcpy(&out, &r, 4);
+    return out;
+}
+
+/* ---- a CPU BVH with exact closest-hit semantics ----------------------------------------------- */
+typedef struct
+{
+    double lo[3], hi[3];
+    int32_t left, right; /* children (node index), or -1 for a leaf */
+    uint32_t first, count;
+} cnode;
+
+typedef struct
+{
+    const float *tris; /* 9 floats per triangle */
+    const uint8_t *two_sided;
+    uint32_t n;
+    cnode *nodes;
+    uint32_t n_nodes;
+    uint32_t *order;
+} cbvh;
+
+static void tri_box(const float *t, double lo[3], double hi[3])
+{
+    for(int a = 0; a < 3; ++a)
+    {
+        double mn = fmin(fmin(t[a], t[3 + a]), t[6 + a]), mx = fmax(fmax(t[a], t[3 + a]), t[6 + a]);
+        double pad = (fabs(mn) > fabs(mx) ? fabs(mn) : fabs(mx)) * (1.0 / 65536.0) + 1e-30;
+        lo[a] = mn - pad;
+        hi[a] = mx + pad;
+    }
+}
+
+static uint32_t build_rec(cbvh *b, uint32_t first, uint32_t count, const double *cent)
+{
+    uint32_t id = b->n_nodes++;
+    cnode *nd = &b->nodes[id];
+    for(int a = 0; a < 3; ++a)
+    {
+        nd->lo[a] = INFINITY;
+        nd->hi[a] = -INFINITY;
+    }
+    for(uint32_t k = first; k < first + count; ++k)
+    {
+        double lo[3], hi[3];
+        tri_box(b->tris + 9 * (size_t)b->order[k], lo, hi);
+        for(int a = 0; a < 3; ++a)
+        {
+            nd->lo[a] = fmin(nd->lo[a], lo[a]);
+            nd->hi[a] = fmax(nd->hi[a], hi[a]);
+        }
+    }
+    if(count <= 4)
+    {
+        nd->left = nd->right = -1;
+        nd->first = first;
+        nd->count = count;
+        return id;
+    }
+    double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(uint32_t k = first; k < first + count; ++k)
+        for(int a = 0; a < 3; ++a)
+        {
+            double c = cent[3 * (size_t)b->order[k] + a];
+            clo[a] = fmin(clo[a], c);
+            chi[a] = fmax(chi[a], c);
+        }
+    int axis = 0;
+    for(int a = 1; a < 3; ++a)
+        if(chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+    double split = 0.5 * (clo[axis] + chi[axis]);
+    uint32_t i = first, j = first + count;
+    while(i < j)
+    {
+        if(cent[3 * (size_t)b->order[i] + axis] < split) ++i;
+        else
+        {
+            --j;
+            uint32_t t = b->order[i];
+            b->order[i] = b->order[j];
+            b->order[j] = t;
+        }
+    }
+    uint32_t mid = i;
+    if(mid == first || mid == first + count) mid = first + count / 2;
+    int32_t l = (int32_t)build_rec(b, first, mid - first, cent);
+    int32_t r = (int32_t)build_rec(b, mid, first + count - mid, cent);
+    nd = &b->nodes[id];
+    nd->left = l;
+    nd->right = r;
+    nd->first = nd->count = 0;
+    return id;
+}
+
+static int box_hit(const cnode *nd, const double o[3], const double inv[3], double tmin, double tmax)
+{
+    double t0 = -INFINITY, t1 = INFINITY;
+    for(int a = 0; a < 3; ++a)
+    {
+        double ta = (nd->lo[a] - o[a]) * inv[a], tb = (nd->hi[a] - o[a]) * inv[a];
+        if(ta != ta) ta = -INFINITY; /* 0 * inf on a slab plane: inside */
+        if(tb != tb) tb = INFINITY;
+        double lo = fmin(ta, tb), hi = fmax(ta, tb);
+        t0 = fmax(t0, lo);
+        t1 = fmin(t1, hi);
+    }
+    t0 -= fabs(t0) * 1e-6;
+    t1 += fabs(t1) * 1e-6;
+    return t0 <= t1 && t1 >= tmin && t0 <= tmax;
+}
+
+/* closest hit with tmin < t < tmax, lowest index on ties, back faces culled when `cull` */
+static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int cull, float *bt, float *bu, float *bv)
+{
+    double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    uint32_t best = 0xFFFFFFFFu;
+    float best_t = tmax;
+    uint32_t stack[128];
+    int sp = 0;
+    if(b->n_nodes == 0) return best;
+    stack[sp++] = 0;
+    while(sp)
+    {
+        const cnode *nd = &b->nodes[stack[--sp]];
+        if(!box_hit(nd, od, inv, tmin, best_t)) continue;
+        if(nd->left < 0)
+        {
+            for(uint32_t k = nd->first; k < nd->first + nd->count; ++k)
+            {
+                uint32_t prim = b->order[k];
+                const float *tr = b->tris + 9 * (size_t)prim;
+                float t, u, v;
+                if(!orc_intersect_tri(tr, &o.x, &d.x, &t, &u, &v)) continue;
+                if(cull && !b->two_sided[prim])
+                {
+                    v3 e1 = V3(tr[3] - tr[0], tr[4] - tr[1], tr[5] - tr[2]);
+                    v3 e2 = V3(tr[6] - tr[0], tr[7] - tr[1], tr[8] - tr[2]);
+                    if(dot(e1, cross(d, e2)) < 0.f) continue;
+                }
+                if(!(t > tmin)) continue;
+                if(t < best_t || (t == best_t && best != 0xFFFFFFFFu && prim < best))
+                {
+                    best_t = t;
+                    *bu = u;
+                    *bv = v;
+                    best = prim;
+                }
+            }
+        }
+        else
+        {
+            stack[sp++] = (uint32_t)nd->left;
+            stack[sp++] = (uint32_t)nd->right;
+        }
+    }
+    *bt = best_t;
+    return best;
+}
+
+/* ---- the integrator ------------------------------------------------------------------------- */
+typedef struct
+{
+    const orc_pt_scene *sc;
+    const orc_pt_params *pp;
+    cbvh bvh;
+    mat_t *mats;
+    float *obj_tmax;
+    float *out; /* 8 floats per pixel: r g b alpha depth normal(bits) object(bits) albedo(bits) */
+    uint32_t y0, y1;
+} job_t;
+
+static void render_pixel(job_t *j, uint32_t x, uint32_t y)
+{
+    const orc_pt_scene *sc = j->sc;
+    const orc_pt_params *pp = j->pp;
+    const uint32_t W = pp->width, H = pp->height;
+    uint32_t rng = orc_xxhash32(pp->seed, W * y + x);
+    v3 result = V3(0, 0, 0);
+    float coverage = 0.f;
+    float offset = 2.0f * rnd(&rng) - 1.0f;
+    v3 eye = V3(pp->eye[0], pp->eye[1], pp->eye[2]);
+    v3 fwd = V3(pp->forward[0], pp->forward[1], pp->forward[2]);
+    v3 right = V3(pp->right[0], pp->right[1], pp->right[2]);
+    v3 up = V3(pp->up[0], pp->up[1], pp->up[2]);
+    float zdepth = 0.f;
+    uint32_t first_obj = 0xFFFFu;
+    v3 first_n = V3(0, 0, 0), first_alb = V3(0, 0, 0);
+    for(uint32_t s = 0; s < pp->num_samples; ++s)
+    {
+        float hx = (float)s / (float)pp->num_samples;
+        float hy = (float)brev(s) * 2.3283064365386963e-10f;
+        float fx = hx + offset, fy = hy + offset;
+        float cx = (float)x + (fx - floorf(fx));
+        float cy = (float)y + (fy - floorf(fy));
+        float xi_x = rnd(&rng), xi_y = rnd(&rng);
+        /* camera ray */
+        float u = cx / (float)W, v = cy / (float)H;
+        float dx = u * 2.0f - 1.0f, dy = v * 2.0f - 1.0f;
+        float r = sqrtf(xi_x), th = 2.0f * PI * xi_y;
+        float lx = pp->aperture / 2.0f * (r * cosf(th)), ly = pp->aperture / 2.0f * (r * sinf(th));
+        v3 org = add(add(eye, scl(right, lx)), scl(up, ly));
+        v3 tg = nrmz(V3(dx * pp->tan_half_x, -dy * pp->tan_half_y, -1.0f));
+        v3 fp = scl(tg, pp->focal_distance / fabsf(tg.z));
+        v3 dv = sub(fp, V3(lx, ly, 0.f));
+        v3 dir = nrmz(sub(add(scl(right, dv.x), scl(up, dv.y)), scl(fwd, dv.z)));
+        float tmax = DEFAULT_TMAX;
+        uint32_t media_index = 0;
+        float iors[4] = {1.f, 1.f, 1.f, 1.f};
+        int regularize = 0, caustic = 0, first_miss = 1;
+        v3 beta = V3(1, 1, 1), rad = V3(0, 0, 0);
+        for(uint32_t i = 0; i < pp->max_depth; ++i)
+        {
+            rng = orc_xxhash32(rng, i);
+            uint32_t lr = rng;
+            float t = 0.f, bu = 0.f, bv = 0.f;
+            uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, &t, &bu, &bv);
+            int stop = 0, media_op = 0;
+            float next_ior = 1.f;
+            uint32_t obj = 0xFFFFu;
+            v3 pos = V3(0, 0, 0), nrm = V3(0, 0, 0), alb = V3(0, 0, 0);
+            if(prim == 0xFFFFFFFFu)
+            {
+                stop = 1;
+                rad = add(rad, mul(scl(beta, sc->environment), env_white(dir)));
+            }
+            else
+            {
+                const uint32_t *ix = sc->indices + 3 * (size_t)prim;
+                const float *P = sc->positions, *N = sc->normals;
+                v3 v0 = V3(P[3 * ix[0]], P[3 * ix[0] + 1], P[3 * ix[0] + 2]);
+                v3 v1 = V3(P[3 * ix[1]], P[3 * ix[1] + 1], P[3 * ix[1] + 2]);
+                v3 v2 = V3(P[3 * ix[2]], P[3 * ix[2] + 1], P[3 * ix[2] + 2]);
+                v3 e1 = sub(v1, v0), e2 = sub(v2, v0);
+                int backface = dot(e1, cross(dir, e2)) < 0.f;
+                float bx = 1.0f - bu - bv, by = bu, bz = bv;
+                pos = add(add(scl(v0, bx), scl(v1, by)), scl(v2, bz));
+                v3 n = add(add(scl(V3(N[3 * ix[0]], N[3 * ix[0] + 1], N[3 * ix[0] + 2]), bx),
+                               scl(V3(N[3 * ix[1]], N[3 * ix[1] + 1], N[3 * ix[1] + 2]), by)),
+                           scl(V3(N[3 * ix[2]], N[3 * ix[2] + 1], N[3 * ix[2] + 2]), bz));
+                nrm = nrmz(n);
+                if(backface) nrm = neg(nrm);
+                obj = sc->tri_object[prim];
+                mat_t m = j->mats[sc->object_material[obj]];
+                if(pp->disable_material) m.color = V3(0.8f, 0.8f, 0.8f);
+                v3 Vw = neg(dir);
+                org = pos;
+                float eta = backface ? m.ior / iors[media_index > 0 ? media_index - 1 : 0] : iors[media_index] / m.ior;
+                eta += EPS;
+                next_ior = m.ior;
+                media_op = !(m.trans > 0.0f) ? 0 : (backface ? 2 : 1);
+                if(i == 0) alb = m.color;
+                v3 ff = dot(dir, nrm) < 0.f ? nrm : neg(nrm);
+                v3 em = scl(m.emission, dot(nrm, ff) > 0.f ? m.emission_strength : 0.0f);
+                rad = add(rad, mul(beta, em));
+                if(regularize) m.rough = fmaxf(m.rough, 0.25f);
+                bs_t b = sample_disney(m, nrm, Vw, eta, &lr);
+                if(!(b.pdf > 0.0f)) stop = 1;
+                else
+                {
+                    dir = b.dir;
+                    int specular = b.pdf < 1e3f;
+                    regularize = regularize || specular;
+                    caustic = specular && (caustic || b.transmission);
+                    float ct = fabsf(dot(nrm, b.dir));
+                    float den = fmaxf(b.pdf, PDF_EPS);
+                    v3 thr = scl(b.F, ct);
+                    thr = V3(thr.x / den, thr.y / den, thr.z / den);
+                    if(i >= 1 && pp->max_path_beta > 0.0f)
+                        thr = V3(fminf(thr.x, pp->max_path_beta), fminf(thr.y, pp->max_path_beta),
+                                 fminf(thr.z, pp->max_path_beta));
+                    beta = mul(beta, thr);
+                    if(b.transmission) tmax = media_op == 1 ? j->obj_tmax[obj] : DEFAULT_TMAX;
+                    media_op = b.transmission ? media_op : 0;
+                    if(max3(beta) <= 0.05f * (float)i)
+                    {
+                        float q = 1.0f - max3(beta);
+                        if(rnd(&lr) < q) stop = 1;
+                        else
+                        {
+                            float keep = 1.0f - q;
+                            beta = V3(beta.x / keep, beta.y / keep, beta.z / keep);
+                        }
+                    }
+                }
+            }
+            if(media_op == 1)
+            {
+                media_index = media_index + 1 < 3 ? media_index + 1 : 3;
+                iors[media_index] = round_half(next_ior);
+            }
+            else if(media_op == 2)
+                media_index = media_index > 0 ? media_index - 1 : 0;
+            if(i == 0)
+            {
+                first_miss = obj == 0xFFFFu;
+                first_obj = obj;
+                first_n = nrm;
+                first_alb = alb;
+                zdepth = first_miss ? 0.f : dot(sub(pos, eye), fwd);
+            }
+            if(stop) break;
+        }
+        result = add(result, rad);
+        coverage = fmaxf(coverage, first_miss ? 0.f : 1.f);
+    }
+    float ns = (float)pp->num_samples;
+    float *o = j->out + 8 * ((size_t)y * W + x);
+    o[0] = result.x / ns;
+    o[1] = result.y / ns;
+    o[2] = result.z / ns;
+    o[3] = coverage;
+    o[4] = zdepth;
+    (void)first_n;
+    (void)first_alb;
+    memcpy(&o[6], &first_obj, 4);
+    o[5] = 0.f;
+    o[7] = 0.f;
+}
+
+static void *render_rows(void *arg)
+{
+    job_t *j = (job_t *)arg;
+    for(uint32_t y = j->y0; y < j->y1; ++y)
+        for(uint32_t x = 0; x < j->pp->width; ++x) render_pixel(j, x, y);
+    return NULL;
+}
+
+void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads)
+{
+    const uint32_t n = sc->n_tri;
+    cbvh b;
+    memset(&b, 0, sizeof(b));
+    float *tris = (float *)malloc(sizeof(float) * 9 * (size_t)(n ? n : 1));
+    uint8_t *two = (uint8_t *)calloc(n ? n : 1, 1);
+    double *cent = (double *)malloc(sizeof(double) * 3 * (size_t)(n ? n : 1));
+    mat_t *mats = (mat_t *)calloc(sc->n_materials ? sc->n_materials : 1, sizeof(mat_t));
+    for(uint32_t m = 0; m < sc->n_materials; ++m)
+    {
+        const float *x = sc->materials + 28 * (size_t)m; /* vkrt_material layout, 28 words */
+        mats[m].color = V3(x[0], x[1], x[2]);
+        mats[m].emission = V3(x[4], x[5], x[6]);
+        mats[m].emission_strength = x[7];
+        mats[m].metal = x[8];
+        mats[m].rough = x[9];
+        mats[m].trans = x[10];
+        mats[m].ior = x[11];
+        mats[m].cc = x[12];
+        mats[m].cc_rough = x[13];
+        mats[m].sheen_rough = x[14];
+        mats[m].dt = x[15];
+        mats[m].sheen = V3(x[16], x[17], x[18]);
+        mats[m].dt_color = V3(x[20], x[21], x[22]);
+        uint32_t ts;
+        memcpy(&ts, &x[24], 4);
+        mats[m].two_sided = ts != 0;
+    }
+    for(uint32_t t = 0; t < n; ++t)
+    {
+        for(int k = 0; k < 3; ++k)
+            for(int a = 0; a < 3; ++a) tris[9 * (size_t)t + 3 * k + a] = sc->positions[3 * (size_t)sc->indices[3 * (size_t)t + k] + a];
+        two[t] = (uint8_t)mats[sc->object_material[sc->tri_object[t]]].two_sided;
+        for(int a = 0; a < 3; ++a)
+            cent[3 * (size_t)t + a] = ((double)tris[9 * t + a] + tris[9 * t + 3 + a] + tris[9 * t + 6 + a]) / 3.0;
+    }
+    b.tris = tris;
+    b.two_sided = two;
+    b.n = n;
+    b.order = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for(uint32_t t = 0; t < n; ++t) b.order[t] = t;
+    b.nodes = (cnode *)malloc(sizeof(cnode) * (2 * (size_t)(n ? n : 1)));
+    if(n) build_rec(&b, 0, n, cent);
+    /* per-object tmax inside a volume: 2 * length(aabb half extents) */
+    float *obj_tmax = (float *)calloc(sc->n_objects ? sc->n_objects : 1, sizeof(float));
+    float *lo = (float *)malloc(sizeof(float) * 3 * (sc->n_objects ? sc->n_objects : 1));
+    float *hi = (float *)malloc(sizeof(float) * 3 * (sc->n_objects ? sc->n_objects : 1));
+    for(uint32_t o = 0; o < 3 * sc->n_objects; ++o)
+    {
+        lo[o] = FLT_MAX;
+        hi[o] = -FLT_MAX;
+    }
+    for(uint32_t t = 0; t < n; ++t)
+    {
+        uint32_t o = sc->tri_object[t];
+        for(int k = 0; k < 3; ++k)
+            for(int a = 0; a < 3; ++a)
+            {
+                float c = tris[9 * (size_t)t + 3 * k + a];
+                lo[3 * o + a] = c < lo[3 * o + a] ? c : lo[3 * o + a];
+                hi[3 * o + a] = hi[3 * o + a] < c ? c : hi[3 * o + a];
+            }
+    }
+    for(uint32_t o = 0; o < sc->n_objects; ++o)
+    {
+        float he0 = (hi[3 * o] - lo[3 * o]) / 2.f, he1 = (hi[3 * o + 1] - lo[3 * o + 1]) / 2.f,
+              he2 = (hi[3 * o + 2] - lo[3 * o + 2]) / 2.f;
+        obj_tmax[o] = hi[3 * o] < lo[3 * o] ? 0.f : 2.f * sqrtf((he0 * he0 + he1 * he1) + he2 * he2);
+    }
+    if(nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
+    job_t *jobs = (job_t *)calloc((size_t)nthreads, sizeof(job_t));
+    for(int k = 0; k < nthreads; ++k)
+    {
+        jobs[k].sc = sc;
+        jobs[k].pp = pp;
+        jobs[k].bvh = b;
+        jobs[k].mats = mats;
+        jobs[k].obj_tmax = obj_tmax;
+        jobs[k].out = out8;
+        jobs[k].y0 = (uint32_t)((uint64_t)pp->height * k / nthreads);
+        jobs[k].y1 = (uint32_t)((uint64_t)pp->height * (k + 1) / nthreads);
+        pthread_create(&th[k], NULL, render_rows, &jobs[k]);
+    }
+    for(int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
+    free(th);
+    free(jobs);
+    free(lo);
+    free(hi);
+    free(obj_tmax);
+    free(b.nodes);
+    free(b.order);
+    free(mats);
+    free(cent);
+    free(two);
+    free(tris);
+}

@@ -1,0 +1,44 @@
+/*
+ * vkpt_oracle.h - CPU reference integrator (TEST INFRASTRUCTURE ONLY; see vkpt_oracle.c).
+ */
+#ifndef VKPT_ORACLE_H
+#define VKPT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_pt_scene
+{
+    const float *positions; /* 3 per vertex */
+    const float *normals;   /* 3 per vertex */
+    uint32_t n_vert;
+    const uint32_t *indices;
+    uint32_t n_tri;
+    const uint32_t *tri_object;
+    const uint32_t *object_material;
+    uint32_t n_objects;
+    const float *materials; /* vkrt_material records, 28 words each */
+    uint32_t n_materials;
+    float environment;
+} orc_pt_scene;
+
+typedef struct orc_pt_params
+{
+    uint32_t width, height, num_samples, max_depth, seed, disable_material;
+    float max_path_beta;
+    float eye[3], forward[3], right[3], up[3];
+    float tan_half_x, tan_half_y, aperture, focal_distance;
+} orc_pt_params;
+
+/* One batch: out8 = 8 floats per pixel (radiance rgb / num_samples, coverage, linear depth, -, object id
+ * bits, -), row-major from the top-left. */
+void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

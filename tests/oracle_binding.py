@@ -186,3 +186,41 @@ def hammersley(i, n):
     out = np.zeros(2, dtype=np.float32)
     lib().orc_hammersley(i, n, out.ctypes.data)
     return out
+
+
+class _PtScene(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("n_vert", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32), ("tri_object", ctypes.c_void_p),
+                ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
+                ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float)]
+
+
+class _PtParams(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("num_samples", ctypes.c_uint32),
+                ("max_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32), ("disable_material", ctypes.c_uint32),
+                ("max_path_beta", ctypes.c_float), ("eye", ctypes.c_float * 3), ("forward", ctypes.c_float * 3),
+                ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("tan_half_x", ctypes.c_float),
+                ("tan_half_y", ctypes.c_float), ("aperture", ctypes.c_float), ("focal_distance", ctypes.c_float)]
+
+
+def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_path_beta=0.0, disable_material=0,
+              nthreads=None):
+    """one batch of the CPU reference integrator: (H, W, 8) float32 (rgb, coverage, depth, -, object bits, -)"""
+    from vierkant_amd import workloads as W
+    h = lib()
+    h.orc_pt_render.restype = None
+    h.orc_pt_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    arr = {k: np.ascontiguousarray(scene[k]) for k in
+           ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
+    sc = _PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
+                  arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
+                  arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,
+                  len(arr["materials"]), float(scene.get("environment", 1.0)))
+    eye, fw, rt, up, thx, thy = W.camera_basis(camera, width, height)
+    pp = _PtParams(width, height, num_samples, max_depth, seed, disable_material, max_path_beta,
+                   (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*fw), (ctypes.c_float * 3)(*rt),
+                   (ctypes.c_float * 3)(*up), thx, thy, float(camera.aperture),
+                   float(camera.focal_distance) if camera.focal_distance > 0 else 1.0)
+    out = np.zeros((height, width, 8), np.float32)
+    h.orc_pt_render(ctypes.byref(sc), ctypes.byref(pp), out.ctypes.data, nthreads or threads())
+    return out

@@ -41,6 +41,52 @@ class TraceConfig(ctypes.Structure):
                 ("reserved", ctypes.c_uint32 * 2)]
 
 
+class PtSettings(ctypes.Structure):
+    """PBRPathTracer::settings_t subset (vkrt_pt_settings)"""
+    _fields_ = [("resolution", ctypes.c_uint32 * 2), ("max_num_batches", ctypes.c_uint32),
+                ("num_samples", ctypes.c_uint32), ("max_trace_depth", ctypes.c_uint32),
+                ("max_path_beta", ctypes.c_float), ("suspend_trace_when_done", ctypes.c_uint32),
+                ("disable_material", ctypes.c_uint32), ("suppress_reset", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 7)]
+
+
+class PtScene(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("n_vert", ctypes.c_uint32),
+                ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32), ("tri_object", ctypes.c_void_p),
+                ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
+                ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("fov_y", ctypes.c_float), ("forward", ctypes.c_float * 3),
+                ("aperture", ctypes.c_float), ("up", ctypes.c_float * 3), ("focal_distance", ctypes.c_float)]
+
+
+class PtStats(ctypes.Structure):
+    _fields_ = [("raytrace_ms", ctypes.c_float), ("traced", ctypes.c_uint32), ("segments", ctypes.c_uint64),
+                ("samples", ctypes.c_uint64)]
+
+
+MATERIAL_DTYPE = np.dtype([("color", "<f4", 4), ("emission", "<f4", 4), ("metalness", "<f4"), ("roughness", "<f4"),
+                           ("transmission", "<f4"), ("ior", "<f4"), ("clearcoat", "<f4"),
+                           ("clearcoat_roughness", "<f4"), ("sheen_roughness", "<f4"),
+                           ("diffuse_transmission", "<f4"), ("sheen_color", "<f4", 4),
+                           ("diffuse_transmission_color", "<f4", 4), ("two_sided", "<u4"), ("reserved", "<u4", 3)])
+PIXEL_DTYPE = np.dtype([("radiance", "<f4", 3), ("alpha", "<f4"), ("depth", "<f4"), ("normal", "<u4"),
+                        ("object_id", "<u4"), ("albedo", "<u4")])
+
+
+def material(color=(0.8, 0.8, 0.8), emission=(0.0, 0.0, 0.0), emission_strength=0.0, metalness=0.0, roughness=1.0,
+             transmission=0.0, ior=1.5, two_sided=False):
+    """one vkrt_material record (material_t defaults: alpha 1, no clearcoat/sheen/diffuse transmission)"""
+    m = np.zeros((), dtype=MATERIAL_DTYPE)
+    m["color"] = (*color, 1.0)
+    m["emission"] = (*emission, emission_strength)
+    m["metalness"], m["roughness"], m["transmission"], m["ior"] = metalness, roughness, transmission, ior
+    m["two_sided"] = 1 if two_sided else 0
+    return m
+
+
 class BvhInfo(ctypes.Structure):
     _fields_ = [("n_tri", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32),
                 ("depth", ctypes.c_uint32), ("sah_cost", ctypes.c_float), ("build_ms", ctypes.c_float),
@@ -77,6 +123,17 @@ _SIGNATURES = {
     "vkrt_get_trace_config": (_i32, [_vp, ctypes.POINTER(TraceConfig)]),
     "vkrt_trace_closest": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_trace_any": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_pt_default_settings": (None, [ctypes.POINTER(PtSettings)]),
+    "vkrt_pt_create": (_i32, [_vp, ctypes.POINTER(PtSettings), _u32, ctypes.POINTER(_vp)]),
+    "vkrt_pt_destroy": (None, [_vp]),
+    "vkrt_pt_set_settings": (_i32, [_vp, ctypes.POINTER(PtSettings)]),
+    "vkrt_pt_get_settings": (_i32, [_vp, ctypes.POINTER(PtSettings)]),
+    "vkrt_pt_set_scene": (_i32, [_vp, ctypes.POINTER(PtScene)]),
+    "vkrt_pt_render": (_i32, [_vp, ctypes.POINTER(Camera)]),
+    "vkrt_pt_reset_accumulator": (_i32, [_vp]),
+    "vkrt_pt_current_batch": (_u32, [_vp]),
+    "vkrt_pt_read_pixels": (_i32, [_vp, _vp, _i32]),
+    "vkrt_pt_get_stats": (_i32, [_vp, ctypes.POINTER(PtStats)]),
     "vkrt_icosphere_size": (_i32, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "vkrt_icosphere": (_i32, [_f32, _u32, _vp, _vp]),
     "vkrt_plane_size": (_i32, [_u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
@@ -356,6 +413,81 @@ class Bvh:
     def close(self):
         if self._h:
             lib().vkrt_bvh_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_settings():
+    s = PtSettings()
+    lib().vkrt_pt_default_settings(ctypes.byref(s))
+    return s
+
+
+def make_camera(eye, forward, up=(0.0, 1.0, 0.0), fov_y=0.7853981633974483, aperture=0.0, focal_distance=1.0):
+    return Camera((ctypes.c_float * 3)(*eye), fov_y, (ctypes.c_float * 3)(*forward), aperture,
+                  (ctypes.c_float * 3)(*up), focal_distance)
+
+
+class PathTracer:
+    """PBRPathTracer on the GPU: `settings` (settings_t subset), render(camera) = render_scene,
+    current_batch(), reset_accumulator(), pixels() = the pixel_data_t buffer."""
+
+    def __init__(self, ctx, settings=None, seed=0):
+        self.ctx = ctx
+        self._h = _vp()
+        s = settings if settings is not None else default_settings()
+        ctx.check(lib().vkrt_pt_create(ctx.handle, ctypes.byref(s), seed, ctypes.byref(self._h)))
+        self._keep = None
+
+    @property
+    def settings(self):
+        s = PtSettings()
+        self.ctx.check(lib().vkrt_pt_get_settings(self._h, ctypes.byref(s)))
+        return s
+
+    @settings.setter
+    def settings(self, s):
+        self.ctx.check(lib().vkrt_pt_set_settings(self._h, ctypes.byref(s)))
+
+    def set_scene(self, scene):
+        """scene: dict with positions [n,3] f32, normals [n,3] f32, indices u32, tri_object u32,
+        object_material u32, materials (MATERIAL_DTYPE array), environment float"""
+        arr = {k: np.ascontiguousarray(scene[k]) for k in
+               ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
+        sc = PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
+                     arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
+                     arr["object_material"].ctypes.data, len(arr["object_material"]),
+                     arr["materials"].ctypes.data, len(arr["materials"]), float(scene.get("environment", 1.0)))
+        self.ctx.check(lib().vkrt_pt_set_scene(self._h, ctypes.byref(sc)))
+
+    def render(self, camera):
+        self.ctx.check(lib().vkrt_pt_render(self._h, ctypes.byref(camera)))
+
+    def reset_accumulator(self):
+        self.ctx.check(lib().vkrt_pt_reset_accumulator(self._h))
+
+    def current_batch(self):
+        return lib().vkrt_pt_current_batch(self._h)
+
+    def pixels(self):
+        s = self.settings
+        out = np.zeros(s.resolution[0] * s.resolution[1], dtype=PIXEL_DTYPE)
+        self.ctx.check(lib().vkrt_pt_read_pixels(self._h, out.ctypes.data, 0))
+        return out.reshape(s.resolution[1], s.resolution[0])
+
+    def stats(self):
+        st = PtStats()
+        self.ctx.check(lib().vkrt_pt_get_stats(self._h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if self._h:
+            lib().vkrt_pt_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

@@ -35,5 +35,6 @@ constexpr int32_t MAX_LEAF = 8;
 constexpr int STACK_SIZE = 64;      // traversal stack entries per ray; the build rejects deeper trees
 constexpr float BOX_PAD = 0x1p-16f; // relative growth of every stored box coordinate
 constexpr float T_PAD = 0x1p-16f;   // relative growth of every slab interval
+constexpr uint32_t TRI_TWO_SIDED = 1u;// flag bit in a leaf triangle record: never culled as a back face
 
 }// namespace vkrt

@@ -1,0 +1,759 @@
+// pt.hip - wavefront path tracer with PBRPathTracer's settings and accumulation semantics.
+//
+// Reference: the raygen/closest-hit/miss programs of shaders/slang/raypipeline.slang (:91-270, :360-680,
+// :746-761) run by the Vulkan RT pipeline, and the host state machine of src/PBRPathTracer.cpp
+// (render_scene :184-255, reset_accumulator :677-680, per-frame seed :588). On gfx950 the per-pixel
+// sample x bounce loop becomes a wavefront over all pixels of one sample:
+//
+//   k_pt_begin   per pixel: first sample draws the pixel's RNG state and sample offset; every sample draws
+//                the lens sample and writes the camera ray into the SoA ray queue
+//   k_pt_trace   per queued path: closest hit through the BVH, back faces culled while the path is in air
+//   k_pt_shade   per queued path: the bounce's RNG hash, miss (sky) or closest-hit shading (emission,
+//                sample_disney, throughput, media stack, Russian roulette); survivors are appended to the
+//                next queue with one wave ballot + one atomic per wave; finished paths add their radiance
+//                to the pixel's sample sum
+//   k_pt_resolve per pixel: mean over the batch's samples, running mean over batches, pixel_data_t
+//
+// All per-path state is SoA in HBM (float4 / uint4 per path) so every kernel reads and writes it
+// coalesced; queues hold pixel indices.
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "bvh_types.hpp"
+#include "context.hpp"
+#include "trace.hpp"
+#include "traverse.hpp"
+#include "vierkant/pt_math.hpp"
+
+using namespace vierkant::pt;
+using vierkant::rt::vec3;
+
+namespace vkrt
+{
+
+constexpr uint32_t MEDIA_STACK = 4;          // MAX_MEDIA_STACK_SIZE (raypipeline.slang:21)
+constexpr uint32_t NO_OBJECT = 0xFFFFu;      // utils::MAX_UINT16
+constexpr uint32_t F_MEDIA_MASK = 3u, F_REGULARIZE = 4u, F_CAUSTIC = 8u, F_FIRST_MISS = 16u;
+constexpr int PT_BLOCK = 256;
+constexpr int PT_LDS = 16;
+
+//! f32tof16 / f16tof32 round trip (round to nearest even) of the media stack's packed ior
+__host__ __device__ inline float round_half(float x)
+{
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    const uint32_t sign = b & 0x80000000u;
+    const uint32_t absb = b & 0x7fffffffu;
+    if(absb >= 0x47800000u) return x;// |x| >= 65536: out of the ior range the tracer uses
+    if(absb < 0x38800000u)          // below the smallest normal half: keep as is (never an ior)
+        return x;
+    // keep 10 mantissa bits, round to nearest even
+    uint32_t lsb = (absb >> 13) & 1u;
+    uint32_t r = absb + 0xfffu + lsb;
+    r &= ~0x1fffu;
+    r |= sign;
+    float out;
+    __builtin_memcpy(&out, &r, 4);
+    return out;
+}
+
+struct gpu_material
+{
+    float4 color;    // rgb, alpha
+    float4 emission; // rgb, strength
+    float4 mrti;     // metalness, roughness, transmission, ior
+    float4 cc;       // clearcoat, clearcoat_roughness, sheen_roughness, diffuse_transmission
+    float4 sheen;    // sheen color rgb, two_sided flag
+    float4 dtc;      // diffuse transmission color rgb, -
+};
+
+struct pt_frame
+{
+    uint32_t width, height, n_pix;
+    uint32_t num_samples, max_depth;
+    uint32_t seed, batch_index, disable_material;
+    float max_path_beta, environment;
+    vec3 eye, right, up, forward;
+    float tan_half_x, tan_half_y, aperture, focal_distance;
+};
+
+struct pt_buffers
+{
+    float4 *ray_o, *ray_d;  // origin + tmin, direction + tmax
+    uint4 *hit;             // t, u, v, prim | backface << 31
+    float4 *beta, *rad;     // throughput, path radiance
+    uint4 *state;           // raygen rng, flags, sample offset bits, -
+    float4 *iors;           // media stack iors
+    float4 *acc;            // sum of the batch's sample radiance (rgb), coverage
+    uint4 *info;            // first-hit depth bits, normal, object id, albedo
+    uint32_t *queue[2];
+    uint32_t *counter;      // [0], [1]: queue sizes; [2..3]: segment count (64-bit)
+    vkrt_pixel *pixels;
+};
+
+struct pt_scene_dev
+{
+    const float *pos, *nrm;
+    const uint32_t *idx, *tri_object, *object_material;
+    const float *object_tmax;
+    const gpu_material *mats;
+};
+
+__device__ __forceinline__ vec3 ld3f(const float *p, uint32_t i) { return vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ __forceinline__ vec3 xyz(const float4 &f) { return vec3{f.x, f.y, f.z}; }
+__device__ __forceinline__ float frac(float x) { return x - floorf(x); }
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b, uint32_t sample)
+{
+    const uint32_t p = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(p == 0) b.counter[0] = f.n_pix;
+    if(p >= f.n_pix) return;
+    const uint32_t x = p % f.width, y = p / f.width;
+    uint4 st = b.state[p];
+    float offset;
+    if(sample == 0)
+    {
+        // rng_state = xxhash32(seed, W * y + x); sample_offset = 2 rnd - 1 (raypipeline.slang:95-100)
+        uint32_t rng = xxhash32(f.seed, f.width * y + x);
+        offset = 2.0f * rnd(rng) - 1.0f;
+        st.x = rng;
+        st.z = __float_as_uint(offset);
+        b.acc[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    else
+        offset = __uint_as_float(st.z);
+    uint32_t rng = st.x;
+    vec2 h = hammersley(sample, f.num_samples);
+    float cx = static_cast<float>(x) + frac(h.x + offset);
+    float cy = static_cast<float>(y) + frac(h.y + offset);
+    vec2 xi;
+    xi.x = rnd(rng);
+    xi.y = rnd(rng);
+    st.x = rng;
+    st.y = 0u;// media index 0, no regularization, no caustic marker
+    b.state[p] = st;
+
+    // camera_ray (raypipeline.slang:58-89), perspective branch, in the camera's own basis
+    float u = cx / static_cast<float>(f.width), v = cy / static_cast<float>(f.height);
+    float dx = u * 2.0f - 1.0f, dy = v * 2.0f - 1.0f;
+    vec2 disc = sample_unit_disc(xi);
+    vec3 lens{f.aperture / 2.0f * disc.x, f.aperture / 2.0f * disc.y, 0.f};
+    vec3 origin = f.eye + f.right * lens.x + f.up * lens.y;
+    vec3 target = normalize3(vec3{dx * f.tan_half_x, -dy * f.tan_half_y, -1.0f});
+    vec3 fpoint = target * (f.focal_distance / fabsf(target.z));
+    vec3 dv = fpoint - lens;
+    vec3 dir = normalize3(f.right * dv.x + f.up * dv.y - f.forward * dv.z);
+    b.ray_o[p] = make_float4(origin.x, origin.y, origin.z, EPS);
+    b.ray_d[p] = make_float4(dir.x, dir.y, dir.z, DEFAULT_TMAX);
+    b.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+    b.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+    b.iors[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+    b.queue[0][p] = p;
+}
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_zero(uint32_t *c) { c[threadIdx.x] = 0u; }
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
+                                                       const float4 *__restrict__ tris, int32_t root, bool empty,
+                                                       pt_buffers b, int in)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(q >= b.counter[in]) return;
+    const uint32_t p = b.queue[in][q];
+    float t = 0.f, u = 0.f, v = 0.f;
+    uint32_t prim = VKRT_INVALID_PRIM;
+    if(!empty)
+    {
+        ray_setup r = make_ray_setup(b.ray_o[p], b.ray_d[p]);
+        const bool in_air = (b.state[p].y & F_MEDIA_MASK) == 0u;
+        // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
+        traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim, in_air);
+    }
+    b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
+}
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b, pt_scene_dev s, uint32_t depth,
+                                                       int in)
+{
+    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
+    const uint32_t count = b.counter[in];
+    const bool active = q < count;
+    bool alive = false;
+    uint32_t p = 0;
+    if(active)
+    {
+        p = b.queue[in][q];
+        uint4 st = b.state[p];
+        // rng_state = xxhash32(rng_state, i) in the raygen chain (raypipeline.slang:150); the hit program
+        // draws from a copy
+        st.x = xxhash32(st.x, depth);
+        uint32_t rng = st.x;
+        uint32_t flags = st.y;
+        float4 ro = b.ray_o[p], rd = b.ray_d[p];
+        vec3 dir = xyz(rd);
+        vec3 beta = xyz(b.beta[p]);
+        vec3 radiance = xyz(b.rad[p]);
+        float4 iors = b.iors[p];
+        float ior_stack[MEDIA_STACK] = {iors.x, iors.y, iors.z, iors.w};
+        uint32_t media_index = flags & F_MEDIA_MASK;
+        uint4 h = b.hit[p];
+        bool stop = false;
+        uint32_t object = NO_OBJECT;
+        vec3 position{0.f, 0.f, 0.f}, normal{0.f, 0.f, 0.f}, albedo{0.f, 0.f, 0.f};
+        uint32_t media_op = 0;// 0 NoOp, 1 Enter, 2 Leave
+        float next_ior = 1.f;
+
+        if(h.w == VKRT_INVALID_PRIM)
+        {
+            // miss_default (raypipeline.slang:746-761)
+            stop = true;
+            vec3 col = environment_white(dir);
+            radiance = radiance + mul3(beta * f.environment, col);
+        }
+        else
+        {
+            const uint32_t prim = h.w;
+            const float u = __uint_as_float(h.y), v = __uint_as_float(h.z);
+            const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+            const vec3 v0 = ld3f(s.pos, i0), v1 = ld3f(s.pos, i1), v2 = ld3f(s.pos, i2);
+            const vec3 e1 = v1 - v0, e2 = v2 - v0;
+            const bool backface = dot(e1, vierkant::rt::cross(dir, e2)) < 0.f;
+            // interpolate_vertex (raypipeline.slang:336-358), identity instance transform
+            const float bx = 1.0f - u - v, by = u, bz = v;
+            position = (v0 * bx + v1 * by) + v2 * bz;
+            vec3 n = (ld3f(s.nrm, i0) * bx + ld3f(s.nrm, i1) * by) + ld3f(s.nrm, i2) * bz;
+            normal = normalize3(n);
+            if(backface) normal = -normal;
+            object = s.tri_object[prim];
+            const gpu_material gm = s.mats[s.object_material[object]];
+            const float tmax_inside = s.object_tmax[object];
+            const vec3 V = -dir;
+            ro = make_float4(position.x, position.y, position.z, ro.w);
+
+            material m;
+            m.color = f.disable_material ? splat(0.8f) : xyz(gm.color);
+            m.alpha = f.disable_material ? 1.0f : gm.color.w;
+            m.emission = xyz(gm.emission);
+            m.emission_strength = gm.emission.w;
+            m.metalness = gm.mrti.x;
+            m.roughness = gm.mrti.y;
+            m.transmission = gm.mrti.z;
+            m.ior = gm.mrti.w;
+            m.clearcoat = gm.cc.x;
+            m.clearcoat_roughness = gm.cc.y;
+            m.sheen_roughness = gm.cc.z;
+            m.diffuse_transmission = gm.cc.w;
+            m.sheen_color = xyz(gm.sheen);
+            m.diffuse_transmission_color = xyz(gm.dtc);
+
+            const float media_ior = ior_stack[media_index];
+            const float last_ior = ior_stack[media_index > 0u ? media_index - 1u : 0u];
+            float eta = backface ? m.ior / last_ior : media_ior / m.ior;
+            eta += EPS;
+            next_ior = m.ior;
+            const bool media_boundary = m.transmission > 0.0f;
+            media_op = !media_boundary ? 0u : (backface ? 2u : 1u);
+            if(depth == 0) albedo = m.color;
+            const vec3 sn = normal;
+            const vec3 ff = dot(dir, sn) < 0.f ? sn : -sn;// faceforward(sn, ray dir, sn)
+            vec3 emission = m.emission * (dot(sn, ff) > 0.f ? m.emission_strength : 0.0f);
+            radiance = radiance + mul3(beta, emission);
+            if(flags & F_REGULARIZE) m.roughness = fmaxf(m.roughness, REGULARIZE_MIN_ROUGHNESS);
+            bsdf_sample bs = sample_disney(m, sn, V, eta, rng);
+            if(!(bs.pdf > 0.0f)) { stop = true; }
+            else
+            {
+                dir = bs.direction;
+                const bool specular = bs.pdf < REGULARIZE_PDF_THRESHOLD;
+                if(specular) flags |= F_REGULARIZE;
+                const bool caustic = specular && ((flags & F_CAUSTIC) || bs.transmission);
+                flags = caustic ? (flags | F_CAUSTIC) : (flags & ~F_CAUSTIC);
+                const float cos_theta = fabsf(dot(sn, bs.direction));
+                const float denom = fmaxf(bs.pdf, PDF_EPS);
+                vec3 thr = bs.F * cos_theta;
+                thr = vec3{thr.x / denom, thr.y / denom, thr.z / denom};
+                if(depth >= 1 && f.max_path_beta > 0.0f)
+                    thr = vec3{fminf(thr.x, f.max_path_beta), fminf(thr.y, f.max_path_beta),
+                               fminf(thr.z, f.max_path_beta)};
+                beta = mul3(beta, thr);
+                if(bs.transmission) rd.w = media_op == 1u ? tmax_inside : DEFAULT_TMAX;
+                media_op = bs.transmission ? media_op : 0u;
+                // Russian roulette (raypipeline.slang:667-677)
+                if(max3(beta) <= 0.05f * static_cast<float>(depth))
+                {
+                    float qq = 1.0f - max3(beta);
+                    if(rnd(rng) < qq) stop = true;
+                    else
+                    {
+                        const float keep = 1.0f - qq;
+                        beta = vec3{beta.x / keep, beta.y / keep, beta.z / keep};
+                    }
+                }
+            }
+        }
+
+        // media stack update in the raygen loop (raypipeline.slang:195-200)
+        if(media_op == 1u)
+        {
+            media_index = min(media_index + 1u, MEDIA_STACK - 1u);
+            ior_stack[media_index] = round_half(next_ior);
+        }
+        else if(media_op == 2u)
+            media_index = media_index > 0u ? media_index - 1u : 0u;
+        flags = (flags & ~F_MEDIA_MASK) | media_index;
+
+        if(depth == 0)
+        {
+            const bool miss = object == NO_OBJECT;
+            if(miss) flags |= F_FIRST_MISS;
+            else flags &= ~F_FIRST_MISS;
+            const float zdepth = miss ? 0.0f : dot(position - f.eye, f.forward);
+            b.info[p] = make_uint4(__float_as_uint(zdepth), pack_normal_oct(normal), object, pack_rgba8(albedo, 1.0f));
+        }
+        st.y = flags;
+        b.state[p] = st;
+        alive = !stop && depth + 1 < f.max_depth;
+        if(alive)
+        {
+            b.ray_o[p] = ro;
+            b.ray_d[p] = make_float4(dir.x, dir.y, dir.z, rd.w);
+            b.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+            b.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.f);
+            b.iors[p] = make_float4(ior_stack[0], ior_stack[1], ior_stack[2], ior_stack[3]);
+        }
+        else
+        {
+            // result.radiance += payload.radiance; coverage = max(coverage, first hit ? 1 : 0)
+            float4 a = b.acc[p];
+            const float cov = (flags & F_FIRST_MISS) ? 0.0f : 1.0f;
+            b.acc[p] = make_float4(a.x + radiance.x, a.y + radiance.y, a.z + radiance.z, fmaxf(a.w, cov));
+        }
+    }
+    // append survivors to the next queue: one ballot and one atomic per wave
+    const uint64_t mask = __ballot(alive);
+    const uint32_t lane = __lane_id();
+    const uint64_t n_active = __ballot(active);
+    uint32_t base = 0;
+    if(lane == 0)
+    {
+        if(mask) base = atomicAdd(&b.counter[in ^ 1], static_cast<uint32_t>(__popcll(mask)));
+        if(n_active) atomicAdd(reinterpret_cast<unsigned long long *>(&b.counter[2]), (unsigned long long)__popcll(n_active));
+    }
+    base = __shfl(base, 0);
+    if(alive)
+    {
+        const uint64_t below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
+        b.queue[in ^ 1][base + __popcll(below)] = p;
+    }
+}
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_resolve(pt_frame f, pt_buffers b)
+{
+    const uint32_t p = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(p >= f.n_pix) return;
+    float4 a = b.acc[p];
+    const float ns = static_cast<float>(f.num_samples);
+    vec3 r{a.x / ns, a.y / ns, a.z / ns};
+    const float coverage = a.w;
+    vkrt_pixel out;
+    vec3 accum = r;
+    if(f.batch_index != 0)
+    {
+        // running mean over batches (raypipeline.slang:258-266)
+        const float bs = static_cast<float>(f.batch_index);
+        const vkrt_pixel old = b.pixels[p];
+        vec3 sum{accum.x + bs * old.radiance[0], accum.y + bs * old.radiance[1], accum.z + bs * old.radiance[2]};
+        const float div = static_cast<float>(f.batch_index + 1);
+        accum = vec3{sum.x / div, sum.y / div, sum.z / div};
+    }
+    const uint4 info = b.info[p];
+    out.radiance[0] = accum.x;
+    out.radiance[1] = accum.y;
+    out.radiance[2] = accum.z;
+    out.alpha = coverage;// the stored alpha is this batch's coverage (raypipeline.slang:268-269)
+    out.depth = __uint_as_float(info.x);
+    out.normal = info.y;
+    out.object_id = info.z;
+    out.albedo = info.w;
+    b.pixels[p] = out;
+}
+
+__global__ void k_pt_tag(float4 *__restrict__ tris, uint32_t n, const uint32_t *__restrict__ tri_object,
+                         const uint32_t *__restrict__ object_material, const gpu_material *__restrict__ mats)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    float4 r = tris[3 * (size_t)j + 2];
+    const uint32_t prim = __float_as_uint(r.y);
+    const uint32_t mat = object_material[tri_object[prim]];
+    r.z = __uint_as_float(mat);
+    r.w = __uint_as_float(mats[mat].sheen.w != 0.f ? TRI_TWO_SIDED : 0u);
+    tris[3 * (size_t)j + 2] = r;
+}
+
+}// namespace vkrt
+
+// =============================================================================================
+// the tracer object
+// =============================================================================================
+struct vkrt_pt
+{
+    vkrt_ctx *ctx = nullptr;
+    vkrt_pt_settings settings{};
+    std::minstd_rand0 random_engine;// std::default_random_engine of libstdc++ (PBRPathTracer.cpp:32)
+    uint32_t batch_index = 0;
+    vkrt_bvh *bvh = nullptr;
+    uint32_t n_tri = 0, n_vert = 0, n_objects = 0, n_materials = 0;
+    float environment = 1.f;
+    vkrt::device_buffer pos, nrm, idx, tri_object, object_material, object_tmax, mats;
+    uint32_t width = 0, height = 0;
+    vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    vkrt_pt_stats stats{};
+};
+
+namespace vkrt
+{
+
+static void default_settings(vkrt_pt_settings &s)
+{
+    s = {};
+    s.resolution[0] = 1280;
+    s.resolution[1] = 720;
+    s.max_num_batches = 0;
+    s.num_samples = 1;
+    s.max_trace_depth = 6;
+    s.max_path_beta = 0.f;
+    s.suspend_trace_when_done = 1;
+    s.disable_material = 0;
+    s.suppress_reset = 0;
+}
+
+static void check_settings(const vkrt_pt_settings &s)
+{
+    VKRT_REQUIRE(s.resolution[0] > 0 && s.resolution[1] > 0, "resolution must be positive");
+    VKRT_REQUIRE((uint64_t)s.resolution[0] * s.resolution[1] <= (1u << 26), "at most 2^26 pixels");
+    VKRT_REQUIRE(s.num_samples > 0 && s.num_samples <= 65536, "num_samples must be in 1..65536");
+    VKRT_REQUIRE(s.max_trace_depth > 0 && s.max_trace_depth <= 64, "max_trace_depth must be in 1..64");
+}
+
+//! resize_storage (PBRPathTracer.cpp:684-830): new pixel storage resets the batch index
+static void resize_storage(vkrt_pt *pt, uint32_t w, uint32_t h)
+{
+    if(w == pt->width && h == pt->height) return;
+    const size_t n = (size_t)w * h;
+    pt->ray_o.reserve(n * 16);
+    pt->ray_d.reserve(n * 16);
+    pt->hit.reserve(n * 16);
+    pt->beta.reserve(n * 16);
+    pt->rad.reserve(n * 16);
+    pt->state.reserve(n * 16);
+    pt->iors.reserve(n * 16);
+    pt->acc.reserve(n * 16);
+    pt->info.reserve(n * 16);
+    pt->queue0.reserve(n * 4);
+    pt->queue1.reserve(n * 4);
+    pt->counter.reserve(256);
+    pt->pixels.reserve(n * sizeof(vkrt_pixel));
+    VKRT_HIP(hipMemsetAsync(pt->pixels.ptr, 0, n * sizeof(vkrt_pixel), pt->ctx->stream));
+    pt->width = w;
+    pt->height = h;
+    pt->batch_index = 0;
+}
+
+static pt_buffers buffers(vkrt_pt *pt)
+{
+    pt_buffers b;
+    b.ray_o = pt->ray_o.as<float4>();
+    b.ray_d = pt->ray_d.as<float4>();
+    b.hit = pt->hit.as<uint4>();
+    b.beta = pt->beta.as<float4>();
+    b.rad = pt->rad.as<float4>();
+    b.state = pt->state.as<uint4>();
+    b.iors = pt->iors.as<float4>();
+    b.acc = pt->acc.as<float4>();
+    b.info = pt->info.as<uint4>();
+    b.queue[0] = pt->queue0.as<uint32_t>();
+    b.queue[1] = pt->queue1.as<uint32_t>();
+    b.counter = pt->counter.as<uint32_t>();
+    b.pixels = pt->pixels.as<vkrt_pixel>();
+    return b;
+}
+
+static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
+{
+    pt_frame f;
+    f.width = pt->width;
+    f.height = pt->height;
+    f.n_pix = pt->width * pt->height;
+    f.num_samples = pt->settings.num_samples;
+    f.max_depth = pt->settings.max_trace_depth;
+    f.seed = seed;
+    f.batch_index = pt->batch_index;
+    f.disable_material = pt->settings.disable_material;
+    f.max_path_beta = pt->settings.max_path_beta;
+    f.environment = pt->environment;
+    // camera basis in double precision, rounded once to float: forward, right = forward x up, up = right x forward
+    auto nrm = [](const double v[3], double out[3]) {
+        const double l = std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+        for(int k = 0; k < 3; ++k) out[k] = v[k] / l;
+    };
+    auto crs = [](const double a[3], const double b[3], double out[3]) {
+        out[0] = a[1] * b[2] - a[2] * b[1];
+        out[1] = a[2] * b[0] - a[0] * b[2];
+        out[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    const double fw0[3] = {cam.forward[0], cam.forward[1], cam.forward[2]};
+    const double up0[3] = {cam.up[0], cam.up[1], cam.up[2]};
+    double fw[3], rt0[3], rt[3], up[3];
+    nrm(fw0, fw);
+    crs(fw, up0, rt0);
+    nrm(rt0, rt);
+    crs(rt, fw, up);
+    f.eye = vec3{cam.eye[0], cam.eye[1], cam.eye[2]};
+    f.forward = vec3{(float)fw[0], (float)fw[1], (float)fw[2]};
+    f.right = vec3{(float)rt[0], (float)rt[1], (float)rt[2]};
+    f.up = vec3{(float)up[0], (float)up[1], (float)up[2]};
+    const double th = std::tan(0.5 * (double)cam.fov_y);
+    f.tan_half_y = (float)th;
+    f.tan_half_x = (float)(th * (double)pt->width / (double)pt->height);
+    f.aperture = cam.aperture;
+    f.focal_distance = cam.focal_distance > 0.f ? cam.focal_distance : 1.f;
+    return f;
+}
+
+static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
+{
+    hipStream_t s = pt->ctx->stream;
+    pt_frame f = make_frame(pt, cam, seed);
+    pt_buffers b = buffers(pt);
+    pt_scene_dev sc{pt->pos.as<float>(), pt->nrm.as<float>(), pt->idx.as<uint32_t>(), pt->tri_object.as<uint32_t>(),
+                    pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(), pt->mats.as<gpu_material>()};
+    const uint32_t blocks = div_up(f.n_pix, PT_BLOCK);
+    const vkrt_bvh *bvh = pt->bvh;
+    VKRT_HIP(hipEventRecord(pt->ev0, s));
+    k_pt_zero<<<1, 4, 0, s>>>(b.counter);
+    for(uint32_t sample = 0; sample < f.num_samples; ++sample)
+    {
+        k_pt_begin<<<blocks, PT_BLOCK, 0, s>>>(f, b, sample);
+        int in = 0;
+        for(uint32_t depth = 0; depth < f.max_depth; ++depth)
+        {
+            k_pt_zero<<<1, 1, 0, s>>>(b.counter + (in ^ 1));
+            k_pt_trace<<<blocks, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref,
+                                                   bvh->empty, b, in);
+            k_pt_shade<<<blocks, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
+            in ^= 1;
+        }
+    }
+    k_pt_resolve<<<blocks, PT_BLOCK, 0, s>>>(f, b);
+    VKRT_HIP(hipEventRecord(pt->ev1, s));
+    VKRT_HIP(hipGetLastError());
+    uint64_t seg = 0;
+    VKRT_HIP(hipMemcpyAsync(&seg, b.counter + 2, 8, hipMemcpyDeviceToHost, s));
+    VKRT_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    VKRT_HIP(hipEventElapsedTime(&ms, pt->ev0, pt->ev1));
+    pt->stats.raytrace_ms = ms;
+    pt->stats.traced = 1;
+    pt->stats.segments = seg;
+    pt->stats.samples = (uint64_t)f.n_pix * f.num_samples;
+}
+
+}// namespace vkrt
+
+using namespace vkrt;
+
+extern "C" {
+
+void vkrt_pt_default_settings(vkrt_pt_settings *out)
+{
+    if(out) default_settings(*out);
+}
+
+int vkrt_pt_create(vkrt_ctx *ctx, const vkrt_pt_settings *settings, uint32_t seed, vkrt_pt **out)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        auto pt = std::make_unique<vkrt_pt>();
+        pt->ctx = ctx;
+        if(settings) pt->settings = *settings;
+        else default_settings(pt->settings);
+        check_settings(pt->settings);
+        pt->random_engine.seed(seed);
+        VKRT_HIP(hipEventCreate(&pt->ev0));
+        VKRT_HIP(hipEventCreate(&pt->ev1));
+        *out = pt.release();
+    });
+}
+
+void vkrt_pt_destroy(vkrt_pt *pt)
+{
+    if(!pt) return;
+    (void)hipSetDevice(pt->ctx->device);
+    (void)hipStreamSynchronize(pt->ctx->stream);
+    if(pt->bvh) vkrt_bvh_destroy(pt->bvh);
+    if(pt->ev0) (void)hipEventDestroy(pt->ev0);
+    if(pt->ev1) (void)hipEventDestroy(pt->ev1);
+    delete pt;
+}
+
+int vkrt_pt_set_settings(vkrt_pt *pt, const vkrt_pt_settings *settings)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(settings, "null settings");
+        check_settings(*settings);
+        pt->settings = *settings;
+    });
+}
+
+int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out)
+{
+    if(!pt || !out) return VKRT_E_INVALID_ARG;
+    *out = pt->settings;
+    return VKRT_OK;
+}
+
+int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(sc, "null scene");
+        VKRT_REQUIRE(sc->n_tri == 0 || (sc->positions && sc->normals && sc->indices && sc->tri_object),
+                     "null scene arrays");
+        VKRT_REQUIRE(sc->n_objects > 0 && sc->object_material && sc->n_materials > 0 && sc->materials,
+                     "a scene needs objects and materials");
+        hipStream_t s = pt->ctx->stream;
+        for(uint32_t o = 0; o < sc->n_objects; ++o)
+            VKRT_REQUIRE(sc->object_material[o] < sc->n_materials, "object material index out of range");
+        std::vector<float> lo(3 * (size_t)sc->n_objects, 3.40282347e+38f), hi(3 * (size_t)sc->n_objects, -3.40282347e+38f);
+        for(uint32_t t = 0; t < sc->n_tri; ++t)
+        {
+            const uint32_t o = sc->tri_object[t];
+            VKRT_REQUIRE(o < sc->n_objects, "triangle object index out of range");
+            for(int k = 0; k < 3; ++k)
+            {
+                const uint32_t vi = sc->indices[3 * (size_t)t + k];
+                VKRT_REQUIRE(vi < sc->n_vert, "vertex index out of range");
+                for(int a = 0; a < 3; ++a)
+                {
+                    const float c = sc->positions[3 * (size_t)vi + a];
+                    lo[3 * o + a] = c < lo[3 * o + a] ? c : lo[3 * o + a];
+                    hi[3 * o + a] = hi[3 * o + a] < c ? c : hi[3 * o + a];
+                }
+            }
+        }
+        // tmax_inside_volume = 2 * length(scale * aabb.half_extents()) (raypipeline.slang:370), unit scale
+        std::vector<float> tmax(sc->n_objects);
+        for(uint32_t o = 0; o < sc->n_objects; ++o)
+        {
+            float he[3];
+            for(int a = 0; a < 3; ++a) he[a] = (hi[3 * o + a] - lo[3 * o + a]) / 2.f;
+            tmax[o] = hi[3 * o] < lo[3 * o] ? 0.f : 2.f * std::sqrt((he[0] * he[0] + he[1] * he[1]) + he[2] * he[2]);
+        }
+        std::vector<gpu_material> mats(sc->n_materials);
+        for(uint32_t m = 0; m < sc->n_materials; ++m)
+        {
+            const vkrt_material &x = sc->materials[m];
+            mats[m].color = make_float4(x.color[0], x.color[1], x.color[2], x.color[3]);
+            mats[m].emission = make_float4(x.emission[0], x.emission[1], x.emission[2], x.emission[3]);
+            mats[m].mrti = make_float4(x.metalness, x.roughness, x.transmission, x.ior);
+            mats[m].cc = make_float4(x.clearcoat, x.clearcoat_roughness, x.sheen_roughness, x.diffuse_transmission);
+            mats[m].sheen = make_float4(x.sheen_color[0], x.sheen_color[1], x.sheen_color[2], x.two_sided ? 1.f : 0.f);
+            mats[m].dtc = make_float4(x.diffuse_transmission_color[0], x.diffuse_transmission_color[1],
+                                      x.diffuse_transmission_color[2], 0.f);
+        }
+        auto upload = [&](device_buffer &buf, const void *src, size_t bytes) {
+            buf.reserve(bytes);
+            if(bytes) VKRT_HIP(hipMemcpyAsync(buf.ptr, src, bytes, hipMemcpyHostToDevice, s));
+        };
+        upload(pt->pos, sc->positions, (size_t)sc->n_vert * 12);
+        upload(pt->nrm, sc->normals, (size_t)sc->n_vert * 12);
+        upload(pt->idx, sc->indices, (size_t)sc->n_tri * 12);
+        upload(pt->tri_object, sc->tri_object, (size_t)sc->n_tri * 4);
+        upload(pt->object_material, sc->object_material, (size_t)sc->n_objects * 4);
+        upload(pt->object_tmax, tmax.data(), tmax.size() * 4);
+        upload(pt->mats, mats.data(), mats.size() * sizeof(gpu_material));
+        VKRT_HIP(hipStreamSynchronize(s));
+        if(pt->bvh)
+        {
+            vkrt_bvh_destroy(pt->bvh);
+            pt->bvh = nullptr;
+        }
+        vkrt_bvh_params params{};
+        params.builder = VKRT_BUILDER_LBVH;
+        params.max_leaf_size = 4;
+        vkrt_bvh *bvh = nullptr;
+        int rc = vkrt_bvh_build(pt->ctx, pt->pos.as<float>(), sc->n_vert, 12, pt->idx.as<uint32_t>(), sc->n_tri,
+                                &params, VKRT_DEVICE_PTRS, &bvh);
+        if(rc != VKRT_OK) throw error(rc, "BVH build failed: " + pt->ctx->last_error);
+        pt->bvh = bvh;
+        if(sc->n_tri)
+            k_pt_tag<<<div_up(sc->n_tri, 256), 256, 0, s>>>(bvh->tris.as<float4>(), sc->n_tri,
+                                                              pt->tri_object.as<uint32_t>(),
+                                                              pt->object_material.as<uint32_t>(),
+                                                              pt->mats.as<gpu_material>());
+        VKRT_HIP(hipStreamSynchronize(s));
+        pt->n_tri = sc->n_tri;
+        pt->n_vert = sc->n_vert;
+        pt->n_objects = sc->n_objects;
+        pt->n_materials = sc->n_materials;
+        pt->environment = sc->environment;
+    });
+}
+
+int vkrt_pt_render(vkrt_pt *pt, const vkrt_camera *camera)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(camera, "null camera");
+        VKRT_REQUIRE(pt->bvh, "vkrt_pt_set_scene first");
+        const vkrt_pt_settings settings = pt->settings;// settings are copied per frame (PBRPathTracer.cpp:198)
+        resize_storage(pt, settings.resolution[0], settings.resolution[1]);
+        pt->stats.traced = 0;
+        if(!settings.max_num_batches || !settings.suspend_trace_when_done || pt->batch_index < settings.max_num_batches)
+        {
+            const uint32_t seed = static_cast<uint32_t>(pt->random_engine());
+            trace_batch(pt, *camera, seed);
+            pt->batch_index = std::min<uint32_t>(pt->batch_index + 1, settings.max_num_batches);
+        }
+    });
+}
+
+int vkrt_pt_reset_accumulator(vkrt_pt *pt)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    if(!pt->settings.suppress_reset) pt->batch_index = 0;
+    return VKRT_OK;
+}
+
+uint32_t vkrt_pt_current_batch(const vkrt_pt *pt) { return pt ? pt->batch_index : 0u; }
+
+int vkrt_pt_read_pixels(vkrt_pt *pt, vkrt_pixel *out, int flags)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(out, "null output");
+        VKRT_REQUIRE(pt->width > 0, "nothing rendered yet");
+        const size_t bytes = (size_t)pt->width * pt->height * sizeof(vkrt_pixel);
+        VKRT_HIP(hipMemcpyAsync(out, pt->pixels.ptr, bytes,
+                                (flags & VKRT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                pt->ctx->stream));
+        VKRT_HIP(hipStreamSynchronize(pt->ctx->stream));
+    });
+}
+
+int vkrt_pt_get_stats(const vkrt_pt *pt, vkrt_pt_stats *out)
+{
+    if(!pt || !out) return VKRT_E_INVALID_ARG;
+    *out = pt->stats;
+    return VKRT_OK;
+}
+
+}// extern "C"

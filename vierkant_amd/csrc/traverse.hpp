@@ -65,10 +65,12 @@ __device__ __forceinline__ void leaf_decode(int ref, uint32_t &first, uint32_t &
 
 //! closest (ANY = false) or any (ANY = true) hit with tmin < t < tmax; ties on t go to the lowest
 //! original triangle index. `lds` points at this lane's column of a [LDS_DEPTH][BLOCK] int array.
+//! With `cull_back`, triangles hit from behind are skipped unless their record is flagged two-sided
+//! (bit 0 of the fourth word of the triangle's last float4).
 template<bool ANY, int BLOCK, int LDS_DEPTH>
 __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                          int32_t root, const ray_setup &r, int32_t *lds, float &best_t,
-                                         float &best_u, float &best_v, uint32_t &best_prim)
+                                         float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false)
 {
     constexpr int SPILL = STACK_SIZE - LDS_DEPTH > 0 ? STACK_SIZE - LDS_DEPTH : 1;
     int32_t spill[SPILL];
@@ -113,8 +115,9 @@ __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes
                 const float4 *tr = tris + 3 * (size_t)(first + k);
                 float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
                 float t, u, v;
+                const bool cull = cull_back && !(__float_as_uint(p2.w) & TRI_TWO_SIDED);
                 if(vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
-                                         vec3{p1.z, p1.w, p2.x}, t, u, v) &&
+                                         vec3{p1.z, p1.w, p2.x}, t, u, v, cull) &&
                    t > r.tmin)
                 {
                     const uint32_t prim = __float_as_uint(p2.y);

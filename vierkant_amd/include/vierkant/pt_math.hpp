@@ -1,0 +1,443 @@
+// pt_math.hpp - the path tracer's per-vertex arithmetic (RNG, sampling, Disney BSDF sampling, sky),
+// restated from vierkant's ray-tracing shaders for the gfx950 wavefront kernels. Host/device inline.
+//
+// Sources (reference checkout):
+//   shaders/slang/utils/hash.slang:9-18         xxhash32
+//   shaders/slang/utils/random.slang:27-36      lcg / rnd
+//   shaders/slang/utils/sampling.slang          Hammersley :9-13, local_frame :16-22, spherical_direction
+//                                               :25-28, sample_unit_disc :31-40, sample_hemisphere_cosine
+//                                               :79-85, sample_GGX_VNDF :98-118, sample_GTR1 :120-129
+//   shaders/slang/utils/bsdf.slang              SchlickFresnel :9-13, DielectricFresnel :24-33, GTR1/GTR2
+//                                               :143-156, SmithGGX :158-163
+//   shaders/slang/ray/bsdf_disney.slang         Eval* lobes :44-160, sample_disney :162-296
+//   shaders/slang/utils/procedural_environment.slang:7-11   environment_white
+//   shaders/slang/utils/octahedral_map.slang:9-19, :49-55   octahedral normal packing
+//   shaders/slang/utils/color.slang:14-18       color_cast (RGBA8)
+// Iridescence and dispersion are not part of this material model (their strengths are zero in the
+// benchmark scenes); the rnd() draws the reference makes for them are kept, so the RNG stream matches.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+
+#include "rt_math.hpp"
+
+namespace vierkant::pt
+{
+
+using rt::vec3;
+using rt::cross;
+using rt::dot;
+
+constexpr float PI = 3.1415926535897932384626433832795f;
+constexpr float ONE_OVER_PI = 0.31830988618379067153776752674503f;
+constexpr float EPS = 1e-4f;          // ray_common.slang:12
+constexpr float PDF_EPS = 1e-3f;      // ray_common.slang:13
+constexpr float DEFAULT_TMAX = 10000.0f;
+constexpr float REGULARIZE_PDF_THRESHOLD = 1e3f;// raypipeline.slang:24
+constexpr float REGULARIZE_MIN_ROUGHNESS = 0.25f;
+
+struct vec2
+{
+    float x, y;
+};
+
+VK_HD vec3 mul3(const vec3 &a, const vec3 &b) { return vec3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+VK_HD vec3 splat(float s) { return vec3{s, s, s}; }
+VK_HD float length3(const vec3 &v) { return sqrtf(dot(v, v)); }
+VK_HD vec3 normalize3(const vec3 &v)
+{
+    float inv = 1.0f / sqrtf(dot(v, v));
+    return v * inv;
+}
+VK_HD float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+VK_HD float lerpf(float a, float b, float t) { return a + t * (b - a); }
+VK_HD vec3 lerp3(const vec3 &a, const vec3 &b, float t) { return a + (b - a) * t; }
+VK_HD vec3 reflect3(const vec3 &i, const vec3 &n) { return i - n * (2.0f * dot(n, i)); }
+VK_HD float max3(const vec3 &v) { return v.x > v.y ? fmaxf(v.x, v.z) : fmaxf(v.y, v.z); }
+
+//! HLSL refract: zero vector on total internal reflection
+VK_HD vec3 refract3(const vec3 &i, const vec3 &n, float eta)
+{
+    float ndi = dot(n, i);
+    float k = 1.0f - eta * eta * (1.0f - ndi * ndi);
+    if(k < 0.0f) return vec3{0.f, 0.f, 0.f};
+    return i * eta - n * (eta * ndi + sqrtf(k));
+}
+
+// ------------------------------------------------------------------------------------------------
+// RNG
+// ------------------------------------------------------------------------------------------------
+VK_HD uint32_t xxhash32(uint32_t lhs, uint32_t rhs)
+{
+    const uint32_t P2 = 2246822519U, P3 = 3266489917U, P4 = 668265263U, P5 = 374761393U;
+    uint32_t h = lhs + P5 + rhs * P3;
+    h = P4 * ((h << 17) | (h >> 15));
+    h = P2 * (h ^ (h >> 15));
+    h = P3 * (h ^ (h >> 13));
+    return h ^ (h >> 16);
+}
+
+VK_HD float rnd(uint32_t &state)
+{
+    state = 1664525u * state + 1013904223u;
+    return static_cast<float>(state & 0x00FFFFFFu) / static_cast<float>(0x01000000);
+}
+
+VK_HD uint32_t reverse_bits(uint32_t x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __brev(x);
+#else
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0F0F0F0Fu) | ((x & 0x0F0F0F0Fu) << 4);
+    x = ((x >> 8) & 0x00FF00FFu) | ((x & 0x00FF00FFu) << 8);
+    return (x >> 16) | (x << 16);
+#endif
+}
+
+VK_HD vec2 hammersley(uint32_t i, uint32_t n)
+{
+    return vec2{static_cast<float>(i) / static_cast<float>(n),
+                static_cast<float>(reverse_bits(i)) * 2.3283064365386963e-10f};
+}
+
+// ------------------------------------------------------------------------------------------------
+// sampling
+// ------------------------------------------------------------------------------------------------
+struct frame3
+{
+    vec3 tx, ty, n;// rows
+};
+
+VK_HD frame3 local_frame(const vec3 &direction)
+{
+    float len2 = direction.x * direction.x + direction.y * direction.y;
+    float len = sqrtf(len2);
+    vec3 tx = len2 > 0.f ? vec3{-direction.y / len, direction.x / len, 0.f / len} : vec3{1.f, 0.f, 0.f};
+    vec3 ty = cross(direction, tx);
+    return frame3{tx, ty, direction};
+}
+
+//! mul(frame, v): world -> local
+VK_HD vec3 to_local(const frame3 &f, const vec3 &v) { return vec3{dot(f.tx, v), dot(f.ty, v), dot(f.n, v)}; }
+//! mul(v, frame): local -> world
+VK_HD vec3 to_world(const frame3 &f, const vec3 &v) { return (f.tx * v.x + f.ty * v.y) + f.n * v.z; }
+
+VK_HD vec3 spherical_direction(float sin_theta, float cos_theta, float phi)
+{
+    return vec3{sin_theta * cosf(phi), sin_theta * sinf(phi), cos_theta};
+}
+
+VK_HD vec2 sample_unit_disc(vec2 xi)
+{
+    float r = sqrtf(xi.x);
+    float theta = 2.0f * PI * xi.y;
+    return vec2{r * cosf(theta), r * sinf(theta)};
+}
+
+VK_HD vec3 sample_hemisphere_cosine(vec2 xi)
+{
+    float phi = 2.0f * PI * xi.y;
+    float cos_theta = sqrtf(fmaxf(1.0f - xi.x, 0.0f));
+    float sin_theta = sqrtf(fmaxf(1.0f - cos_theta * cos_theta, 0.0f));
+    return spherical_direction(sin_theta, cos_theta, phi);
+}
+
+VK_HD vec3 sample_ggx_vndf(vec2 xi, const vec3 &wi, float alpha)
+{
+    vec3 wi_std = normalize3(vec3{wi.x * alpha, wi.y * alpha, wi.z});
+    float phi = (2.0f * xi.x - 1.0f) * PI;
+    float z = fmaf(1.0f - xi.y, 1.0f + wi_std.z, -wi_std.z);
+    float sin_theta = sqrtf(clampf(1.0f - z * z, 0.0f, 1.0f));
+    vec3 c{sin_theta * cosf(phi), sin_theta * sinf(phi), z};
+    vec3 wm_std = c + wi_std;
+    return normalize3(vec3{wm_std.x * alpha, wm_std.y * alpha, wm_std.z});
+}
+
+VK_HD vec3 sample_gtr1(vec2 xi, float roughness)
+{
+    float a = fmaxf(0.001f, roughness);
+    float a2 = a * a;
+    float phi = xi.y * 2.0f * PI;
+    float cos_theta = sqrtf((1.0f - powf(a2, 1.0f - xi.x)) / (1.0f - a2));
+    float sin_theta = clampf(sqrtf(1.0f - cos_theta * cos_theta), 0.0f, 1.0f);
+    return spherical_direction(sin_theta, cos_theta, phi);
+}
+
+// ------------------------------------------------------------------------------------------------
+// BSDF building blocks
+// ------------------------------------------------------------------------------------------------
+VK_HD float schlick_fresnel(float u)
+{
+    float m = clampf(1.0f - u, 0.0f, 1.0f);
+    return m * m * m * m * m;
+}
+
+VK_HD float dielectric_fresnel(float cos_theta_i, float eta)
+{
+    float sin_t_sq = eta * eta * (1.0f - cos_theta_i * cos_theta_i);
+    if(sin_t_sq >= 1.0f) return 1.0f;
+    float cos_theta_t = sqrtf(fmaxf(1.0f - sin_t_sq, 0.0f));
+    float rs = (eta * cos_theta_t - cos_theta_i) / (eta * cos_theta_t + cos_theta_i);
+    float rp = (eta * cos_theta_i - cos_theta_t) / (eta * cos_theta_i + cos_theta_t);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+VK_HD float gtr1(float ndoth, float a)
+{
+    if(a >= 1.0f) return 1.0f / PI;
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * ndoth * ndoth;
+    return (a2 - 1.0f) / (PI * logf(a2) * t);
+}
+
+VK_HD float gtr2(float ndoth, float a)
+{
+    float a2 = a * a;
+    float t = 1.0f + (a2 - 1.0f) * ndoth * ndoth;
+    return a2 / (PI * t * t);
+}
+
+VK_HD float smith_ggx(float ndotv, float alpha_g)
+{
+    float a = alpha_g * alpha_g;
+    float b = ndotv * ndotv;
+    return 1.0f / (ndotv + sqrtf(a + b - a * b));
+}
+
+//! the material fields the wavefront tracer evaluates (ray/types.slang:64-107 subset)
+struct material
+{
+    vec3 color;
+    float alpha;
+    vec3 emission;
+    float emission_strength;
+    float metalness, roughness, transmission, ior;
+    float clearcoat, clearcoat_roughness, sheen_roughness, diffuse_transmission;
+    vec3 sheen_color;
+    vec3 diffuse_transmission_color;
+};
+
+struct bsdf_sample
+{
+    vec3 direction;
+    float pdf;
+    vec3 F;
+    bool transmission;
+};
+
+VK_HD vec3 eval_dielectric_reflection(float r, const vec3 &color, float eta, const vec3 &V, const vec3 &N,
+                                      const vec3 &L, const vec3 &H, float &pdf)
+{
+    pdf = 0.0f;
+    if(dot(N, L) <= 0.0f) return splat(0.f);
+    float F = dielectric_fresnel(dot(V, H), eta);
+    float D = gtr2(dot(N, H), r);
+    pdf = 0.5f * D * smith_ggx(fabsf(dot(N, V)), r) * F;
+    float G = smith_ggx(fabsf(dot(N, L)), r) * smith_ggx(fabsf(dot(N, V)), r);
+    return color * F * D * G;
+}
+
+VK_HD vec3 eval_dielectric_refraction(float r, const vec3 &color, float eta, const vec3 &V, const vec3 &N,
+                                      const vec3 &L, const vec3 &H, float &pdf)
+{
+    pdf = 0.0f;
+    if(dot(N, L) >= 0.0f) return splat(0.f);
+    float denom_sqrt = dot(L, H) + dot(V, H) * eta;
+    if(fabsf(denom_sqrt) < 1e-7f) return splat(0.f);
+    float F = dielectric_fresnel(fabsf(dot(V, H)), eta);
+    float D = gtr2(dot(N, H), r);
+    pdf = 2.0f * smith_ggx(fabsf(dot(N, V)), r) * D * fabsf(dot(V, H)) * (1.0f - F) * fabsf(dot(L, H)) * eta * eta /
+          (denom_sqrt * denom_sqrt);
+    float G = smith_ggx(fabsf(dot(N, L)), r) * smith_ggx(fabsf(dot(N, V)), r);
+    // left to right as the shader writes it: color * (1 - F) * D * G * |VoH| * |LoH| * 4 * eta * eta / d^2
+    vec3 c = color * (1.0f - F);
+    c = c * D;
+    c = c * G;
+    c = c * fabsf(dot(V, H));
+    c = c * fabsf(dot(L, H));
+    c = c * 4.0f;
+    c = c * eta;
+    c = c * eta;
+    float dd = denom_sqrt * denom_sqrt;
+    return vec3{c.x / dd, c.y / dd, c.z / dd};
+}
+
+VK_HD vec3 eval_specular(float r, const vec3 &cspec0, const vec3 &V, const vec3 &N, const vec3 &L, const vec3 &H,
+                         float &pdf)
+{
+    pdf = 0.0f;
+    if(dot(N, L) <= 0.0f) return splat(0.f);
+    float D = gtr2(dot(N, H), r);
+    pdf = 0.5f * D * smith_ggx(fabsf(dot(N, V)), r);
+    float FH = schlick_fresnel(dot(L, H));
+    vec3 F = lerp3(cspec0, splat(1.0f), FH);
+    float G = smith_ggx(fabsf(dot(N, L)), r) * smith_ggx(fabsf(dot(N, V)), r);
+    return F * D * G;
+}
+
+VK_HD vec3 eval_clearcoat(float cc_roughness, float clearcoat, const vec3 &V, const vec3 &N, const vec3 &L,
+                          const vec3 &H, float &pdf)
+{
+    pdf = 0.0f;
+    if(dot(N, L) <= 0.0f) return splat(0.f);
+    float D = gtr2(dot(N, H), cc_roughness);
+    pdf = D * dot(N, H) / (4.0f * dot(V, H));
+    float FH = schlick_fresnel(dot(L, H));
+    float F = lerpf(0.04f, 1.0f, FH);
+    float G = smith_ggx(dot(N, L), 0.25f) * smith_ggx(fabsf(dot(N, V)), 0.25f);
+    return splat(0.25f * clearcoat * F * D * G);
+}
+
+VK_HD vec3 eval_diffuse(float r, float sheen_r, const vec3 &color, const vec3 &sheen_color, float metalness,
+                        const vec3 &V, const vec3 &N, const vec3 &L, const vec3 &H, float &pdf)
+{
+    pdf = 0.0f;
+    float NoL = dot(N, L);
+    if(NoL <= 0.0f) return splat(0.f);
+    pdf = NoL * ONE_OVER_PI;
+    float LoH = dot(L, H);
+    float FL = schlick_fresnel(NoL);
+    float FV = schlick_fresnel(dot(N, V));
+    float FH = schlick_fresnel(LoH);
+    float Fd90 = 0.5f + 2.0f * LoH * LoH * r;
+    float Fd = lerpf(1.0f, Fd90, FL) * lerpf(1.0f, Fd90, FV);
+    vec3 Fsheen = sheen_color * (FH * sheen_r);
+    return (color * Fd + Fsheen) * ONE_OVER_PI * (1.0f - metalness);
+}
+
+//! sample_disney (ray/bsdf_disney.slang:162-296) with iridescence strength 0
+VK_HD bsdf_sample sample_disney(material m, const vec3 &N, const vec3 &V, float eta, uint32_t &rng)
+{
+    bsdf_sample ret;
+    ret.pdf = 0.0f;
+    ret.F = splat(0.f);
+    ret.transmission = false;
+    ret.direction = splat(0.f);
+
+    float diffuse_ratio = 0.5f * (1.0f - m.metalness);
+    float trans_weight = (1.0f - m.metalness) * m.transmission;
+    m.roughness = fmaxf(0.001f, m.roughness * m.roughness);
+    m.sheen_roughness = fmaxf(0.001f, m.sheen_roughness);
+    vec3 spec_color = lerp3(splat(0.08f), m.color, m.metalness);
+    frame3 frame = local_frame(N);
+
+    vec2 xi;
+    xi.x = rnd(rng);
+    xi.y = rnd(rng);
+    vec3 H = to_world(frame, sample_ggx_vndf(xi, to_local(frame, V), m.roughness));
+    if(dot(V, H) < 0.0f) H = -H;
+
+    if(rnd(rng) < trans_weight)
+    {
+        vec3 R = reflect3(-V, H);
+        float F = dielectric_fresnel(fabsf(dot(R, H)), eta);
+        if(xi.y < F)
+        {
+            ret.direction = normalize3(R);
+            (void)rnd(rng);// iridescence lobe selection (strength 0)
+            ret.F = eval_dielectric_reflection(m.roughness, m.color, eta, V, N, ret.direction, H, ret.pdf);
+            ret.pdf *= 1.0f - 0.0f;
+        }
+        else
+        {
+            vec3 refracted = refract3(-V, H, eta);
+            if(dot(refracted, refracted) < 1e-7f)
+            {
+                ret.direction = normalize3(reflect3(-V, H));
+                ret.F = eval_dielectric_reflection(m.roughness, m.color, eta, V, N, ret.direction, H, ret.pdf);
+            }
+            else
+            {
+                ret.direction = normalize3(refracted);
+                ret.F = eval_dielectric_refraction(m.roughness, m.color, eta, V, N, ret.direction, H, ret.pdf);
+                ret.transmission = true;
+            }
+        }
+        ret.F = ret.F * trans_weight;
+        ret.pdf *= trans_weight;
+    }
+    else
+    {
+        if(rnd(rng) < diffuse_ratio)
+        {
+            float dt = m.diffuse_transmission;
+            if(dt > 0.0f && rnd(rng) < dt)
+            {
+                vec3 local = sample_hemisphere_cosine(xi);
+                ret.direction = to_world(frame, vec3{local.x, local.y, -local.z});
+                ret.F = m.diffuse_transmission_color * (dt * ONE_OVER_PI);
+                ret.pdf = local.z * ONE_OVER_PI * diffuse_ratio * dt;
+                ret.transmission = true;
+            }
+            else
+            {
+                ret.direction = to_world(frame, sample_hemisphere_cosine(xi));
+                vec3 Hd = normalize3(ret.direction + V);
+                ret.F = eval_diffuse(m.roughness, m.sheen_roughness, m.color, m.sheen_color, m.metalness, V, N,
+                                     ret.direction, Hd, ret.pdf) *
+                        (1.0f - dt);
+                ret.pdf *= diffuse_ratio * (1.0f - dt);
+            }
+        }
+        else
+        {
+            float primary_spec_ratio = 1.0f / (1.0f + m.clearcoat);
+            if(rnd(rng) < primary_spec_ratio)
+            {
+                ret.direction = normalize3(reflect3(-V, H));
+                (void)rnd(rng);// iridescence lobe selection (strength 0)
+                ret.F = eval_specular(m.roughness, spec_color, V, N, ret.direction, H, ret.pdf);
+                ret.pdf *= primary_spec_ratio * (1.0f - diffuse_ratio) * (1.0f - 0.0f);
+            }
+            else
+            {
+                vec3 Hc = to_world(frame, sample_gtr1(xi, lerpf(0.1f, 0.001f, m.clearcoat)));
+                if(dot(V, Hc) < 0.0f) Hc = -Hc;
+                ret.direction = normalize3(reflect3(-V, Hc));
+                ret.F = eval_clearcoat(m.clearcoat_roughness, m.clearcoat, V, N, ret.direction, Hc, ret.pdf);
+                ret.pdf *= (1.0f - primary_spec_ratio) * (1.0f - diffuse_ratio);
+            }
+        }
+        ret.F = ret.F * (1.0f - trans_weight);
+        ret.pdf *= (1.0f - trans_weight);
+    }
+    ret.pdf = (ret.pdf != ret.pdf) ? 0.0f : ret.pdf;
+    return ret;
+}
+
+//! environment_white (utils/procedural_environment.slang:7-11)
+VK_HD vec3 environment_white(const vec3 &d)
+{
+    const vec3 sky = splat(1.3f), horizon = splat(0.6f), ground = splat(0.1f);
+    return d.y > 0.f ? lerp3(horizon, sky, d.y) : lerp3(horizon, ground, -d.y);
+}
+
+//! pack_snorm_2x16(normalized_vector_to_octahedral_mapping(n)) (utils/octahedral_map.slang)
+VK_HD uint32_t pack_normal_oct(const vec3 &n)
+{
+    float s = 1.0f / (fabsf(n.x) + fabsf(n.y) + fabsf(n.z));
+    float px = n.x * s, py = n.y * s;
+    if(n.z < 0.0f)
+    {
+        float qx = (1.0f - fabsf(py)) * (px >= 0.0f ? 1.0f : -1.0f);
+        float qy = (1.0f - fabsf(px)) * (py >= 0.0f ? 1.0f : -1.0f);
+        px = qx, py = qy;
+    }
+    if(px != px || py != py) px = py = 0.f;
+    px = clampf(px, -1.f, 1.f);
+    py = clampf(py, -1.f, 1.f);
+    int ix = static_cast<int>(rintf(px * 32767.0f)), iy = static_cast<int>(rintf(py * 32767.0f));
+    return (static_cast<uint32_t>(ix) & 0xffffu) | (static_cast<uint32_t>(iy) << 16);
+}
+
+//! color_cast(float4) -> RGBA8 (utils/color.slang:14-18)
+VK_HD uint32_t pack_rgba8(const vec3 &c, float a)
+{
+    auto q = [](float x) { return static_cast<uint32_t>(clampf(x, 0.f, 1.f) * 255.0f); };
+    return q(c.x) | (q(c.y) << 8) | (q(c.z) << 16) | (q(a) << 24);
+}
+
+}// namespace vierkant::pt

@@ -81,13 +81,16 @@ VK_HD bool mt_reference(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3
 //!   RN(A+B) >= RN(D*(1+2^-16)) (given A, B > -D*2^-80) => RN(u+v) >= 1+2^-18 > 1     => REJECT
 //! (u and v each carry at most two roundings, relative error < 2^-23.) NaN or infinite
 //! intermediates fail every bound and fall through to the reference sequence.
+//! With `cull_back` set, a triangle whose det is negative (its counter-clockwise normal faces away from
+//! the ray) is rejected as well; that is the back-face culling of a ray traced in air.
 VK_HD bool mt_fast(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1, const vec3 &e2, float &t, float &u,
-                   float &v)
+                   float &v, bool cull_back = false)
 {
     vec3 pvec = cross(d, e2);
     float det = dot(e1, pvec);
     constexpr float epsilon = 10.0e-10f;
     if(det > -epsilon && det < epsilon) return false;
+    if(cull_back && det < 0.f) return false;
     vec3 tvec = o - v0;
     float a = dot(tvec, pvec);
     const bool neg = det < 0.f;

@@ -219,3 +219,151 @@ def secondary_rays(primary, hits, tris_flat, seed=0, light=(0.3, 1.0, 0.2), eps=
     shadow = make_rays(s_o, np.broadcast_to(L, (n, 3)).astype(np.float32), tmin=0.0)
     shadow["tmin"] = np.where(hit, np.float32(eps), np.float32(0.0))
     return bounce, shadow
+
+
+# ------------------------------------------------------------------------------------------------
+# configuration 3: Cornell-style room + 200 spheres + the jittered icosphere (path tracer)
+# ------------------------------------------------------------------------------------------------
+class _SceneBuilder:
+    def __init__(self):
+        self.pos, self.nrm, self.idx, self.obj, self.obj_mat, self.mats = [], [], [], [], [], []
+        self.n_vert = 0
+
+    def material(self, m):
+        self.mats.append(m)
+        return len(self.mats) - 1
+
+    def add(self, pos, nrm, idx, mat):
+        o = len(self.obj_mat)
+        self.obj_mat.append(mat)
+        self.pos.append(np.asarray(pos, np.float32))
+        self.nrm.append(np.asarray(nrm, np.float32))
+        self.idx.append(np.asarray(idx, np.uint32) + np.uint32(self.n_vert))
+        self.obj.append(np.full(len(idx) // 3, o, np.uint32))
+        self.n_vert += len(pos)
+        return o
+
+    def quads(self, quads, mat):
+        """quads: list of (4 corners, outward normal); each becomes 2 counter-clockwise triangles facing n"""
+        pos, nrm, idx = [], [], []
+        for corners, n in quads:
+            c = np.asarray(corners, np.float64)
+            n = np.asarray(n, np.float64)
+            if np.dot(np.cross(c[1] - c[0], c[2] - c[0]), n) < 0:
+                c = c[::-1]
+            b = len(pos)
+            pos.extend(c)
+            nrm.extend([n] * 4)
+            idx.extend([b, b + 1, b + 2, b, b + 2, b + 3])
+        return self.add(pos, nrm, idx, mat)
+
+    def scene(self, environment=1.0):
+        import vierkant_amd as vk
+        return {"positions": np.ascontiguousarray(np.concatenate(self.pos)),
+                "normals": np.ascontiguousarray(np.concatenate(self.nrm)),
+                "indices": np.ascontiguousarray(np.concatenate(self.idx)),
+                "tri_object": np.ascontiguousarray(np.concatenate(self.obj)),
+                "object_material": np.asarray(self.obj_mat, np.uint32),
+                "materials": np.array(self.mats, dtype=vk.MATERIAL_DTYPE), "environment": environment}
+
+
+def _box_quads(lo, hi, inward):
+    x0, y0, z0 = lo
+    x1, y1, z1 = hi
+    s = -1.0 if inward else 1.0
+    return [([(x0, y0, z0), (x1, y0, z0), (x1, y0, z1), (x0, y0, z1)], (0, -s, 0)),
+            ([(x0, y1, z0), (x1, y1, z0), (x1, y1, z1), (x0, y1, z1)], (0, s, 0)),
+            ([(x0, y0, z0), (x1, y0, z0), (x1, y1, z0), (x0, y1, z0)], (0, 0, -s)),
+            ([(x0, y0, z1), (x1, y0, z1), (x1, y1, z1), (x0, y1, z1)], (0, 0, s)),
+            ([(x0, y0, z0), (x0, y1, z0), (x0, y1, z1), (x0, y0, z1)], (-s, 0, 0)),
+            ([(x1, y0, z0), (x1, y1, z0), (x1, y1, z1), (x1, y0, z1)], (s, 0, 0))]
+
+
+def _random_material(u):
+    """roughness U(0.05, 1), metallic Bernoulli(0.3), 10% transmissive (transmission 1, IOR 1.5); u = 6 uniforms"""
+    import vierkant_amd as vk
+    color = tuple(float(0.2 + 0.7 * x) for x in u[:3])
+    rough = float(0.05 + 0.95 * u[3])
+    metal = 1.0 if u[4] < 0.3 else 0.0
+    trans = 1.0 if u[5] < 0.1 else 0.0
+    return vk.material(color=color, roughness=rough, metalness=metal, transmission=trans, ior=1.5)
+
+
+def config3_scene(n_spheres=200, seed=0):
+    """Room x in [-3, 3], y in [0, 6], z in [-3, 3] with the front (z = +3) open to the sky: floor, ceiling,
+    back, red left and green right walls, an emissive ceiling quad and a box (12 quads); 200 IcoSphere(r, 3)
+    with r ~ U[0.05, 0.3] inside the room; the jittered subdiv-5 icosphere (radius 1, sigma 0.01).
+    Sphere s uses uniform draws 10 s .. 10 s + 9 of stream seed + 3 (x, y, z, r, then 6 for its material);
+    the big icosphere and the box take the next 12."""
+    import vierkant_amd as vk
+    b = _SceneBuilder()
+    white = b.material(vk.material((0.8, 0.8, 0.8), roughness=1.0))
+    red = b.material(vk.material((0.8, 0.1, 0.1), roughness=1.0))
+    green = b.material(vk.material((0.1, 0.8, 0.1), roughness=1.0))
+    light = b.material(vk.material((0.0, 0.0, 0.0), emission=(1.0, 0.9, 0.8), emission_strength=8.0))
+    room = _box_quads((-3, 0, -3), (3, 6, 3), inward=True)
+    b.quads([room[0], room[1], room[2]], white)       # floor, ceiling, back
+    b.quads([room[4]], red)
+    b.quads([room[5]], green)
+    b.quads([([(-1, 5.99, -1), (1, 5.99, -1), (1, 5.99, 1), (-1, 5.99, 1)], (0, -1, 0))], light)
+    u = uniform(seed + 3, 0, 10 * n_spheres + 12).reshape(-1).astype(np.float64)
+    for s in range(n_spheres):
+        x = u[10 * s: 10 * s + 10]
+        r = 0.05 + 0.25 * x[3]
+        c = np.array([-2.7 + 5.4 * x[0], 0.3 + 4.2 * x[1], -2.7 + 5.4 * x[2]])
+        p, i = vk.icosphere(float(r), 3)
+        n = p.astype(np.float64)
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        b.add((p + c).astype(np.float32), n.astype(np.float32), i, b.material(_random_material(x[4:10])))
+    x = u[10 * n_spheres:]
+    p0, i = vk.icosphere(1.0, 5)
+    n = p0.astype(np.float64)
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    pj, _ = jittered_icosphere(5, seed=seed)
+    b.add((pj + np.array([-1.2, 1.2, 0.3], np.float32)).astype(np.float32), n.astype(np.float32), i,
+          b.material(_random_material(x[0:6])))
+    b.quads(_box_quads((0.4, 0.0, -1.8), (1.8, 1.4, -0.4), inward=False), b.material(_random_material(x[6:12])))
+    return b.scene(environment=1.0)
+
+
+def config3_camera():
+    """eye (0, 3, 10.5) looking down -z (slightly down), 45 degree vertical field of view"""
+    import vierkant_amd as vk
+    return vk.make_camera(eye=(0.0, 3.0, 10.5), forward=(0.0, -0.05, -1.0), up=(0.0, 1.0, 0.0),
+                          fov_y=float(np.radians(45.0)))
+
+
+def camera_basis(camera, width, height):
+    """the tracer's camera basis, computed like the library does (double precision, rounded once):
+    returns (eye, forward, right, up, tan_half_x, tan_half_y) as float32"""
+    import math
+    fw0 = [float(camera.forward[k]) for k in range(3)]
+    up0 = [float(camera.up[k]) for k in range(3)]
+
+    def nrm(v):
+        l = math.sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2])
+        return [v[0] / l, v[1] / l, v[2] / l]
+
+    def crs(a, b):
+        return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+    fw = nrm(fw0)
+    rt = nrm(crs(fw, up0))
+    up = crs(rt, fw)
+    th = math.tan(0.5 * float(camera.fov_y))
+    f32 = lambda v: np.asarray(v, np.float32)
+    return (f32([camera.eye[k] for k in range(3)]), f32(fw), f32(rt), f32(up),
+            np.float32(th * width / height), np.float32(th))
+
+
+def minstd_rand0(seed, n):
+    """std::default_random_engine (libstdc++: minstd_rand0) outputs: the per-batch random seeds the
+    reference path tracer draws (src/PBRPathTracer.cpp:32, :588)"""
+    x = seed % 2147483647
+    if x == 0:
+        x = 1
+    out = []
+    for _ in range(n):
+        x = (16807 * x) % 2147483647
+        out.append(x)
+    return out
