@@ -19,6 +19,8 @@
  *   vkrt_intersect_obbs            <- intersect(const OBB&, const Ray&)           (src/intersection.cpp:152-184)
  *   vkrt_intersect_spheres         <- intersect(const Sphere&, const Ray&)        (src/intersection.cpp:133-148)
  *   vkrt_intersect_planes          <- intersect(const Plane&, const Ray&)         (src/intersection.cpp:10-21)
+ *   vkrt_tri_box_overlap           <- intersect(const Triangle&, const AABB&)     (src/intersection.cpp:364-370)
+ *   vkrt_tri_tri_overlap           <- intersect(const Triangle&, const Triangle&) (src/intersection.cpp:374-378)
  *   vkrt_compute_aabb              <- compute_aabb(const std::vector<vec3>&)      (src/intersection.cpp:96-106)
  *   vkrt_intersect_brute_*         <- a closest/any-hit loop over intersect(Triangle, Ray); the reference
  *                                     has no such loop, its callers test one triangle at a time
@@ -151,6 +153,12 @@ int vkrt_intersect_spheres(vkrt_ctx *ctx, const float *spheres, const vkrt_ray *
 /* planes: 4 floats per plane (coefficients a, b, c, d with a unit normal) */
 int vkrt_intersect_planes(vkrt_ctx *ctx, const float *planes, const vkrt_ray *rays, uint32_t n, vkrt_result *out,
                           int flags);
+/* triangle (9 floats) vs AABB (6 floats: min.xyz, max.xyz) overlap; out[i] in {0, 1} */
+int vkrt_tri_box_overlap(vkrt_ctx *ctx, const float *tris, const float *boxes, uint32_t n, uint32_t *out,
+                         int flags);
+/* triangle vs triangle overlap (boundaries count); out[i] in {0, 1} */
+int vkrt_tri_tri_overlap(vkrt_ctx *ctx, const float *tris_a, const float *tris_b, uint32_t n, uint32_t *out,
+                         int flags);
 /* AABB of n points (3 floats each); out: 6 floats (min.xyz, max.xyz). n == 0 gives the
  * reference's empty box (min = FLT_MAX, max = -FLT_MAX). */
 int vkrt_compute_aabb(vkrt_ctx *ctx, const float *points, uint32_t n, float *out6, int flags);

@@ -112,6 +112,8 @@ _SIGNATURES = {
     "vkrt_intersect_obbs": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_spheres": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_planes": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_tri_box_overlap": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_tri_tri_overlap": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_compute_aabb": (_i32, [_vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_brute_closest": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_brute_any": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
@@ -331,6 +333,22 @@ class Context:
 
     def intersect_planes(self, planes, rays, out=None):
         return self._pairwise(lib().vkrt_intersect_planes, planes, 4, rays, RESULT_DTYPE, out)
+
+    def tri_box_overlap(self, tris, boxes, out=None):
+        n = _n(tris, 9)
+        if out is None:
+            out = np.zeros(n, dtype=np.uint32)
+        self.check(lib().vkrt_tri_box_overlap(self._h, _ptr(tris)[0], _ptr(boxes)[0], n, _ptr(out)[0],
+                                              _flags([tris, boxes, out])))
+        return out
+
+    def tri_tri_overlap(self, tris_a, tris_b, out=None):
+        n = _n(tris_a, 9)
+        if out is None:
+            out = np.zeros(n, dtype=np.uint32)
+        self.check(lib().vkrt_tri_tri_overlap(self._h, _ptr(tris_a)[0], _ptr(tris_b)[0], n, _ptr(out)[0],
+                                              _flags([tris_a, tris_b, out])))
+        return out
 
     def compute_aabb(self, points, out=None):
         n = _n(points, 3)

@@ -74,8 +74,22 @@ def build_oracle():
         raise RuntimeError("oracle build failed:\n" + res.stdout + res.stderr)
 
 
+def build_test_helpers():
+    """tests/cpp/librt_math_host.so: the intersection header compiled for the host (CPU tests)"""
+    src = ROOT / "tests" / "cpp" / "rt_math_host.cpp"
+    if not src.exists():
+        return
+    out = src.parent / "librt_math_host.so"
+    cmd = ["g++", "-O3", "-std=c++20", "-fPIC", "-shared", "-ffp-contract=off", f"-I{PKG / 'include'}",
+           str(src), "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("host build of rt_math.hpp failed:\n" + res.stderr)
+
+
 def build_all(verbose=False):
     build_oracle()
+    build_test_helpers()
     return build_lib(verbose=verbose)
 
 

@@ -105,6 +105,27 @@ __global__ void k_pair_plane(const float *__restrict__ planes, const float4 *__r
     out[i] = vkrt_result{type, type ? dist : 0.f};
 }
 
+__global__ void k_pair_tri_box(const float *__restrict__ tris, const float *__restrict__ boxes, uint32_t n,
+                               uint32_t *__restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    float t[9], b[6];
+    for(int k = 0; k < 9; ++k) t[k] = tris[9 * (size_t)i + k];
+    for(int k = 0; k < 6; ++k) b[k] = boxes[6 * (size_t)i + k];
+    out[i] = tri_box_overlap(t, b, b + 3);
+}
+
+__global__ void k_pair_tri_tri(const float *__restrict__ ta, const float *__restrict__ tb, uint32_t n,
+                               uint32_t *__restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    float a[9], b[9];
+    for(int k = 0; k < 9; ++k) a[k] = ta[9 * (size_t)i + k], b[k] = tb[9 * (size_t)i + k];
+    out[i] = tri_tri_overlap(a, b);
+}
+
 // ---------------------------------------------------------------------------------------------
 // shared helpers
 // ---------------------------------------------------------------------------------------------
@@ -385,6 +406,37 @@ int vkrt_intersect_planes(vkrt_ctx *ctx, const float *planes, const vkrt_ray *ra
                           int flags)
 {
     return pairwise(ctx, planes, 4, rays, n, out, flags, k_pair_plane);
+}
+
+int vkrt_tri_box_overlap(vkrt_ctx *ctx, const float *tris, const float *boxes, uint32_t n, uint32_t *out, int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(n == 0 || (tris && boxes && out), "null array argument");
+        if(n == 0) return;
+        const float *dt = input(ctx, 0, tris, (size_t)n * 9, flags);
+        const float *db = input(ctx, 1, boxes, (size_t)n * 6, flags);
+        uint32_t *dout = output(ctx, 2, out, n, flags);
+        k_pair_tri_box<<<div_up(n, 256), 256, 0, ctx->stream>>>(dt, db, n, dout);
+        VKRT_HIP(hipGetLastError());
+        finish_output(ctx, 2, out, n, flags);
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_tri_tri_overlap(vkrt_ctx *ctx, const float *tris_a, const float *tris_b, uint32_t n, uint32_t *out,
+                         int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(n == 0 || (tris_a && tris_b && out), "null array argument");
+        if(n == 0) return;
+        const float *da = input(ctx, 0, tris_a, (size_t)n * 9, flags);
+        const float *db = input(ctx, 1, tris_b, (size_t)n * 9, flags);
+        uint32_t *dout = output(ctx, 2, out, n, flags);
+        k_pair_tri_tri<<<div_up(n, 256), 256, 0, ctx->stream>>>(da, db, n, dout);
+        VKRT_HIP(hipGetLastError());
+        finish_output(ctx, 2, out, n, flags);
+        finish_call(ctx, flags);
+    });
 }
 
 int vkrt_compute_aabb(vkrt_ctx *ctx, const float *points, uint32_t n, float *out6, int flags)

@@ -195,4 +195,317 @@ VK_HD uint32_t plane_reference(const float coeff[4], const vec3 &o, const vec3 &
     return REJECT;
 }
 
+// ------------------------------------------------------------------------------------------------
+// triangle / AABB overlap: separating-axis test (Akenine-Moller 2001) as intersect(const Triangle&,
+// const AABB&) evaluates it (src/intersection.cpp:364-370 -> src/triangle_intersection.h:753-843):
+// 9 edge x axis tests, the 3 box axes, then the triangle plane. Each edge test projects the two vertices
+// the reference projects (the third projects like one of them in exact arithmetic).
+// ------------------------------------------------------------------------------------------------
+namespace detail
+{
+struct v3a
+{
+    float c[3];
+};
+
+//! one edge x axis separating test: projections p_a, p_b of two vertices on axis (a, b) in the plane
+//! (i, j), radius fa h[i] + fb h[j]; `second_first` swaps the comparison order used for min/max
+VK_HD bool axis_separates(float a, float b, float fa, float fb, const float *va, const float *vb, int i, int j,
+                          bool neg_first, bool second_first, const float h[3])
+{
+    float pa, pb;
+    if(!neg_first)
+    {
+        pa = a * va[i] - b * va[j];
+        pb = a * vb[i] - b * vb[j];
+    }
+    else
+    {
+        pa = -a * va[i] + b * va[j];
+        pb = -a * vb[i] + b * vb[j];
+    }
+    float mn, mx;
+    if(!second_first)
+    {
+        if(pa < pb) mn = pa, mx = pb;
+        else mn = pb, mx = pa;
+    }
+    else
+    {
+        if(pb < pa) mn = pb, mx = pa;
+        else mn = pa, mx = pb;
+    }
+    float rad = fa * h[i] + fb * h[j];
+    return mn > rad || mx < -rad;
+}
+
+VK_HD void minmax3(float x0, float x1, float x2, float &mn, float &mx)
+{
+    mn = mx = x0;
+    if(x1 < mn) mn = x1;
+    if(x1 > mx) mx = x1;
+    if(x2 < mn) mn = x2;
+    if(x2 > mx) mx = x2;
+}
+}// namespace detail
+
+VK_HD uint32_t tri_box_overlap(const float tri[9], const float box_min[3], const float box_max[3])
+{
+    float center[3], h[3], v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+    for(int k = 0; k < 3; ++k)
+    {
+        center[k] = (box_max[k] + box_min[k]) / 2.f;// AABB::center()
+        h[k] = (box_max[k] - box_min[k]) / 2.f;     // AABB::half_extents()
+    }
+    for(int k = 0; k < 3; ++k)
+    {
+        v0[k] = tri[k] - center[k];
+        v1[k] = tri[3 + k] - center[k];
+        v2[k] = tri[6 + k] - center[k];
+    }
+    for(int k = 0; k < 3; ++k)
+    {
+        e0[k] = v1[k] - v0[k];
+        e1[k] = v2[k] - v1[k];
+        e2[k] = v0[k] - v2[k];
+    }
+    using detail::axis_separates;
+    // edge e0: x-axis (v0, v2), y-axis (v0, v2), z-axis (v1, v2 compared second-first)
+    float fx = fabs_(e0[0]), fy = fabs_(e0[1]), fz = fabs_(e0[2]);
+    if(axis_separates(e0[2], e0[1], fz, fy, v0, v2, 1, 2, false, false, h)) return 0;
+    if(axis_separates(e0[2], e0[0], fz, fx, v0, v2, 0, 2, true, false, h)) return 0;
+    if(axis_separates(e0[1], e0[0], fy, fx, v1, v2, 0, 1, false, true, h)) return 0;
+    // edge e1: x (v0, v2), y (v0, v2), z (v0, v1)
+    fx = fabs_(e1[0]), fy = fabs_(e1[1]), fz = fabs_(e1[2]);
+    if(axis_separates(e1[2], e1[1], fz, fy, v0, v2, 1, 2, false, false, h)) return 0;
+    if(axis_separates(e1[2], e1[0], fz, fx, v0, v2, 0, 2, true, false, h)) return 0;
+    if(axis_separates(e1[1], e1[0], fy, fx, v0, v1, 0, 1, false, false, h)) return 0;
+    // edge e2: x (v0, v1), y (v0, v1), z (v1, v2 compared second-first)
+    fx = fabs_(e2[0]), fy = fabs_(e2[1]), fz = fabs_(e2[2]);
+    if(axis_separates(e2[2], e2[1], fz, fy, v0, v1, 1, 2, false, false, h)) return 0;
+    if(axis_separates(e2[2], e2[0], fz, fx, v0, v1, 0, 2, true, false, h)) return 0;
+    if(axis_separates(e2[1], e2[0], fy, fx, v1, v2, 0, 1, false, true, h)) return 0;
+    // the three box axes
+    for(int k = 0; k < 3; ++k)
+    {
+        float mn, mx;
+        detail::minmax3(v0[k], v1[k], v2[k], mn, mx);
+        if(mn > h[k] || mx < -h[k]) return 0;
+    }
+    // the triangle's plane against the box (planeBoxOverlap)
+    float n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+    float vmin[3], vmax[3];
+    for(int q = 0; q < 3; ++q)
+    {
+        float v = v0[q];
+        if(n[q] > 0.0f)
+        {
+            vmin[q] = -h[q] - v;
+            vmax[q] = h[q] - v;
+        }
+        else
+        {
+            vmin[q] = h[q] - v;
+            vmax[q] = -h[q] - v;
+        }
+    }
+    if((n[0] * vmin[0] + n[1] * vmin[1]) + n[2] * vmin[2] > 0.0f) return 0;
+    if((n[0] * vmax[0] + n[1] * vmax[1]) + n[2] * vmax[2] >= 0.0f) return 1;
+    return 0;
+}
+
+// ------------------------------------------------------------------------------------------------
+// triangle / triangle overlap: orientation-predicate test of Guigue and Devillers (JGT 2003) as
+// intersect(const Triangle&, const Triangle&) evaluates it (src/intersection.cpp:374-378 ->
+// src/triangle_intersection.h:145-297, 2D :580-636): plane-side signs with a 1e-12 zero band, the
+// canonical vertex permutations, two interval checks with ">= -eps", and the 2D test for coplanar pairs
+// on the projection of largest area. Boundaries count as overlap.
+// ------------------------------------------------------------------------------------------------
+namespace gd
+{
+struct p2
+{
+    float x, y;
+};
+constexpr float EPS = 1e-12f;
+
+VK_HD float orient(const p2 &a, const p2 &b, const p2 &c) { return (a.x - c.x) * (b.y - c.y) - (a.y - c.y) * (b.x - c.x); }
+
+//! the "vertex" configuration of the 2D test (P1 is a vertex of T1 outside T2's vertex region)
+VK_HD bool vertex_case(p2 P1, p2 Q1, p2 R1, p2 P2, p2 Q2, p2 R2)
+{
+    if(orient(R2, P2, Q1) >= 0.f)
+    {
+        if(orient(R2, Q2, Q1) <= 0.f)
+        {
+            if(orient(P1, P2, Q1) > 0.f) return orient(P1, Q2, Q1) <= 0.f;
+            if(orient(P1, P2, R1) >= 0.f) return orient(Q1, R1, P2) >= 0.f;
+            return false;
+        }
+        if(orient(P1, Q2, Q1) <= 0.f && orient(R2, Q2, R1) <= 0.f) return orient(Q1, R1, Q2) >= 0.f;
+        return false;
+    }
+    if(orient(R2, P2, R1) >= 0.f)
+    {
+        if(orient(Q1, R1, R2) >= 0.f) return orient(P1, P2, R1) >= 0.f;
+        if(orient(Q1, R1, Q2) >= 0.f) return orient(R2, R1, Q2) >= 0.f;
+    }
+    return false;
+}
+
+//! the "edge" configuration of the 2D test
+VK_HD bool edge_case(p2 P1, p2 Q1, p2 R1, p2 P2, p2 Q2, p2 R2)
+{
+    (void)Q2;
+    if(orient(R2, P2, Q1) >= 0.f)
+    {
+        if(orient(P1, P2, Q1) >= 0.f) return orient(P1, Q1, R2) >= 0.f;
+        if(orient(Q1, R1, P2) >= 0.f) return orient(R1, P1, P2) >= 0.f;
+        return false;
+    }
+    if(orient(R2, P2, R1) >= 0.f && orient(P1, P2, R1) >= 0.f)
+    {
+        if(orient(P1, R1, R2) >= 0.f) return true;
+        return orient(Q1, R1, R2) >= 0.f;
+    }
+    return false;
+}
+
+//! both triangles counter-clockwise
+VK_HD bool ccw_2d(p2 p1, p2 q1, p2 r1, p2 p2_, p2 q2, p2 r2)
+{
+    if(orient(p2_, q2, p1) >= 0.f)
+    {
+        if(orient(q2, r2, p1) >= 0.f)
+        {
+            if(orient(r2, p2_, p1) >= 0.f) return true;
+            return edge_case(p1, q1, r1, p2_, q2, r2);
+        }
+        if(orient(r2, p2_, p1) >= 0.f) return edge_case(p1, q1, r1, r2, p2_, q2);
+        return vertex_case(p1, q1, r1, p2_, q2, r2);
+    }
+    if(orient(q2, r2, p1) >= 0.f)
+    {
+        if(orient(r2, p2_, p1) >= 0.f) return edge_case(p1, q1, r1, q2, r2, p2_);
+        return vertex_case(p1, q1, r1, q2, r2, p2_);
+    }
+    return vertex_case(p1, q1, r1, r2, p2_, q2);
+}
+
+VK_HD bool overlap_2d(p2 p1, p2 q1, p2 r1, p2 p2_, p2 q2, p2 r2)
+{
+    const bool cw1 = orient(p1, q1, r1) < 0.f, cw2 = orient(p2_, q2, r2) < 0.f;
+    if(cw1) return cw2 ? ccw_2d(p1, r1, q1, p2_, r2, q2) : ccw_2d(p1, r1, q1, p2_, q2, r2);
+    return cw2 ? ccw_2d(p1, q1, r1, p2_, r2, q2) : ccw_2d(p1, q1, r1, p2_, q2, r2);
+}
+
+VK_HD vec3 ld(const float *p) { return vec3{p[0], p[1], p[2]}; }
+
+//! coplanar triangles: project on the axis-aligned plane where N1 has its largest component
+VK_HD bool coplanar(const vec3 &p1, const vec3 &q1, const vec3 &r1, const vec3 &p2_, const vec3 &q2, const vec3 &r2,
+                    const vec3 &n1)
+{
+    const float nx = n1.x < 0.f ? -n1.x : n1.x, ny = n1.y < 0.f ? -n1.y : n1.y, nz = n1.z < 0.f ? -n1.z : n1.z;
+    auto proj = [&](const vec3 &v, int plane) {
+        return plane == 0 ? p2{v.z, v.y} : (plane == 1 ? p2{v.x, v.z} : p2{v.x, v.y});
+    };
+    int plane = ((nx > nz) && (nx >= ny)) ? 0 : (((ny > nz) && (ny >= nx)) ? 1 : 2);
+    if(plane == 2)
+        return overlap_2d(proj(p1, 2), proj(q1, 2), proj(r1, 2), proj(p2_, 2), proj(q2, 2), proj(r2, 2));
+    // the YZ and XZ projections swap the first two vertices of each triangle
+    return overlap_2d(proj(q1, plane), proj(p1, plane), proj(r1, plane), proj(q2, plane), proj(p2_, plane),
+                      proj(r2, plane));
+}
+
+//! the two interval checks once T1 = (p1, q1, r1) has p1 alone on its side and T2 is permuted to match
+VK_HD bool check_min_max(const vec3 &p1, const vec3 &q1, const vec3 &r1, const vec3 &p2_, const vec3 &q2,
+                         const vec3 &r2)
+{
+    vec3 n = cross(p2_ - q1, p1 - q1);
+    if(dot(q2 - q1, n) >= -EPS) return false;
+    n = cross(p2_ - p1, r1 - p1);
+    if(dot(r2 - p1, n) >= -EPS) return false;
+    return true;
+}
+
+//! canonical permutation of T2 by the signs of its distances to T1's plane
+VK_HD bool permute_t2(const vec3 &p1, const vec3 &q1, const vec3 &r1, const vec3 &p2_, const vec3 &q2,
+                      const vec3 &r2, float dp2, float dq2, float dr2, const vec3 &n1)
+{
+    if(dp2 > 0.f)
+    {
+        if(dq2 > 0.f) return check_min_max(p1, r1, q1, r2, p2_, q2);
+        if(dr2 > 0.f) return check_min_max(p1, r1, q1, q2, r2, p2_);
+        return check_min_max(p1, q1, r1, p2_, q2, r2);
+    }
+    if(dp2 < 0.f)
+    {
+        if(dq2 < 0.f) return check_min_max(p1, q1, r1, r2, p2_, q2);
+        if(dr2 < 0.f) return check_min_max(p1, q1, r1, q2, r2, p2_);
+        return check_min_max(p1, r1, q1, p2_, q2, r2);
+    }
+    if(dq2 < 0.f)
+    {
+        if(dr2 >= 0.f) return check_min_max(p1, r1, q1, q2, r2, p2_);
+        return check_min_max(p1, q1, r1, p2_, q2, r2);
+    }
+    if(dq2 > 0.f)
+    {
+        if(dr2 > 0.f) return check_min_max(p1, r1, q1, p2_, q2, r2);
+        return check_min_max(p1, q1, r1, q2, r2, p2_);
+    }
+    if(dr2 > 0.f) return check_min_max(p1, q1, r1, r2, p2_, q2);
+    if(dr2 < 0.f) return check_min_max(p1, r1, q1, r2, p2_, q2);
+    return coplanar(p1, q1, r1, p2_, q2, r2, n1);
+}
+}// namespace gd
+
+VK_HD uint32_t tri_tri_overlap(const float a[9], const float b[9])
+{
+    using namespace gd;
+    const vec3 p1 = ld(a), q1 = ld(a + 3), r1 = ld(a + 6), P2 = ld(b), Q2 = ld(b + 3), R2 = ld(b + 6);
+    // signed distances of T1's vertices to T2's plane
+    const vec3 n2 = cross(P2 - R2, Q2 - R2);
+    float dp1 = dot(p1 - R2, n2), dq1 = dot(q1 - R2, n2), dr1 = dot(r1 - R2, n2);
+    if(((dp1 * dq1) > 0.f) && ((dp1 * dr1) > 0.f)) return 0;
+    // signed distances of T2's vertices to T1's plane
+    const vec3 n1 = cross(q1 - p1, r1 - p1);
+    float dp2 = dot(P2 - r1, n1), dq2 = dot(Q2 - r1, n1), dr2 = dot(R2 - r1, n1);
+    if(((dp2 * dq2) > 0.f) && ((dp2 * dr2) > 0.f)) return 0;
+    auto snap = [](float &d) {
+        if(fabs_(d) <= EPS) d = 0.f;
+    };
+    snap(dp1), snap(dq1), snap(dr1), snap(dp2), snap(dq2), snap(dr2);
+    bool r;
+    if(dp1 > 0.f)
+    {
+        if(dq1 > 0.f) r = permute_t2(r1, p1, q1, P2, R2, Q2, dp2, dr2, dq2, n1);
+        else if(dr1 > 0.f) r = permute_t2(q1, r1, p1, P2, R2, Q2, dp2, dr2, dq2, n1);
+        else r = permute_t2(p1, q1, r1, P2, Q2, R2, dp2, dq2, dr2, n1);
+    }
+    else if(dp1 < 0.f)
+    {
+        if(dq1 < 0.f) r = permute_t2(r1, p1, q1, P2, Q2, R2, dp2, dq2, dr2, n1);
+        else if(dr1 < 0.f) r = permute_t2(q1, r1, p1, P2, Q2, R2, dp2, dq2, dr2, n1);
+        else r = permute_t2(p1, q1, r1, P2, R2, Q2, dp2, dr2, dq2, n1);
+    }
+    else if(dq1 < 0.f)
+    {
+        if(dr1 >= 0.f) r = permute_t2(q1, r1, p1, P2, R2, Q2, dp2, dr2, dq2, n1);
+        else r = permute_t2(p1, q1, r1, P2, Q2, R2, dp2, dq2, dr2, n1);
+    }
+    else if(dq1 > 0.f)
+    {
+        if(dr1 > 0.f) r = permute_t2(p1, q1, r1, P2, R2, Q2, dp2, dr2, dq2, n1);
+        else r = permute_t2(q1, r1, p1, P2, Q2, R2, dp2, dq2, dr2, n1);
+    }
+    else
+    {
+        if(dr1 > 0.f) r = permute_t2(r1, p1, q1, P2, Q2, R2, dp2, dq2, dr2, n1);
+        else if(dr1 < 0.f) r = permute_t2(r1, p1, q1, P2, R2, Q2, dp2, dr2, dq2, n1);
+        else r = coplanar(p1, q1, r1, P2, Q2, R2, n1);
+    }
+    return r ? 1u : 0u;
+}
+
 }// namespace vierkant::rt
