@@ -99,6 +99,59 @@ def path_tracer_frame(ctx, spp):
     return out
 
 
+def batched_frame(ctx, n_scenes, verify_rays):
+    """256 independent scenes (log-uniform 1k-500k triangles, 16k-2M rays each) traced by one persistent
+    launch replayed as a hipGraph, after the launch-shape autotuner picked block size, LDS stack depth,
+    tile and blocks per CU"""
+    import numpy as np
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+    bvhs, rays, hits, host = [], [], [], []
+    n_tri = 0
+    t_build = 0.0
+    for pos, idx, r in W.config4(n_scenes=n_scenes):
+        t0 = time.perf_counter()
+        bvhs.append(ctx.bvh_build(pos, idx))
+        t_build += time.perf_counter() - t0
+        rays.append(vk.DeviceArray.from_host(ctx, r))
+        hits.append(vk.DeviceArray(ctx, len(r), vk.HIT_DTYPE))
+        n_tri += len(idx) // 3
+        sub = np.random.default_rng(len(host)).choice(len(r), min(verify_rays, len(r)), replace=False)
+        host.append((W.flat_triangles(pos, idx), r[np.sort(sub)], np.sort(sub)))
+    batch = vk.Batch(ctx, list(zip(bvhs, rays, hits)))
+    total = batch.total_rays()
+    default_cfg = batch.config
+    batch.launch()
+    ev = vk.HipEvents(2)
+    def timed(k):
+        ev.record(0, ctx.stream)
+        for _ in range(k):
+            batch.launch(flags=vk.ASYNC)
+        ev.record(1, ctx.stream)
+        ev.sync(1)
+        return ev.ms(0, 1) / k
+    default_ms = timed(3)
+    best, tune_ms = batch.autotune(repeats=2)
+    tuned_ms = timed(5)
+    mismatches = 0
+    for (flat, sub_rays, sub), h in zip(host, hits):
+        got = h.host()[sub]
+        ref = ctx.brute_closest(flat, sub_rays)
+        mismatches += int(np.sum((got["prim"] != ref["prim"]) | (got["t"].view(np.uint32) != ref["t"].view(np.uint32))))
+    for a in rays + hits:
+        a.free()
+    batch.close()
+    for b in bvhs:
+        b.close()
+    return {"workload": f"{n_scenes} scenes, {n_tri} triangles, {total} closest-hit rays in one graph launch",
+            "grays_per_s": round(total / (tuned_ms * 1e-3) / 1e9, 4), "launch_ms": round(tuned_ms, 3),
+            "default_config": list(default_cfg), "default_launch_ms": round(default_ms, 3),
+            "autotuned_config": {"block_size": best[0], "lds_stack_depth": best[1], "tile": best[2],
+                                 "blocks_per_cu": best[3]},
+            "bvh_build_s": round(t_build, 3),
+            "verified_vs_brute_force": {"rays": int(sum(len(x[2]) for x in host)), "mismatches": mismatches}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +162,7 @@ def main():
     ap.add_argument("--verify-rays", type=int, default=65536)
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
     ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
+    ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -284,6 +338,9 @@ def main():
     if args.pt_spp:
         log("path tracer frame")
         result["config"]["path_tracer"] = path_tracer_frame(ctx, args.pt_spp)
+    if args.batch_scenes:
+        log("batched scenes")
+        result["config"]["batched"] = batched_frame(ctx, args.batch_scenes, 4096)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         allrays = np.concatenate([primary, bounce])

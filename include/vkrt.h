@@ -236,6 +236,49 @@ int vkrt_trace_any(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uin
                    int flags);
 
 /* ---------------------------------------------------------------------------------------------
+ * batched closest hit over many scenes: one persistent launch, replayed as a captured hipGraph
+ * (no reference counterpart: vierkant traces one scene per vkCmdTraceRaysKHR, src/RayTracer.cpp:128-129)
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct vkrt_batch vkrt_batch;
+
+/* one scene's work: its BVH, and device buffers of n_rays rays in and hit records out */
+typedef struct vkrt_batch_item
+{
+    const vkrt_bvh *bvh;
+    const vkrt_ray *rays;
+    vkrt_hit *hits;
+    uint32_t n_rays;
+    uint32_t reserved;
+} vkrt_batch_item;
+
+/* launch shape: threads per block (64/128/256), traversal-stack entries in LDS (0/8/16/32), rays per tile
+ * (a multiple of 64) and persistent blocks per compute unit (1..64) */
+typedef struct vkrt_batch_config
+{
+    uint32_t block_size;
+    uint32_t lds_stack_depth;
+    uint32_t tile;
+    uint32_t blocks_per_cu;
+    uint32_t reserved[4];
+} vkrt_batch_config;
+
+/* Capture the batch (items are copied; their buffers must stay valid) into a graph with the default shape. */
+int vkrt_batch_create(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_items, vkrt_batch **out);
+void vkrt_batch_destroy(vkrt_batch *batch);
+/* Change the launch shape (re-captures the graph). Results do not depend on it. */
+int vkrt_batch_set_config(vkrt_batch *batch, const vkrt_batch_config *cfg);
+int vkrt_batch_get_config(const vkrt_batch *batch, vkrt_batch_config *out);
+/* Replay the graph: closest hits of every item's rays into its hit buffer. */
+int vkrt_batch_launch(vkrt_batch *batch, int flags);
+/* Time every launch shape (block x LDS depth x tile x blocks per CU), `repeats` replays each, and keep the
+ * fastest; returns it and its time per launch. */
+int vkrt_batch_autotune(vkrt_batch *batch, uint32_t repeats, vkrt_batch_config *best, float *best_ms);
+uint64_t vkrt_batch_total_rays(const vkrt_batch *batch);
+/* One-shot: create, launch once, destroy. */
+int vkrt_batch_closest(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_items, int flags);
+
+/* ---------------------------------------------------------------------------------------------
  * path tracer (PBRPathTracer)
  * ------------------------------------------------------------------------------------------- */
 

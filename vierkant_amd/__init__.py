@@ -87,6 +87,16 @@ def material(color=(0.8, 0.8, 0.8), emission=(0.0, 0.0, 0.0), emission_strength=
     return m
 
 
+class BatchItem(ctypes.Structure):
+    _fields_ = [("bvh", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("hits", ctypes.c_void_p),
+                ("n_rays", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_uint32), ("lds_stack_depth", ctypes.c_uint32), ("tile", ctypes.c_uint32),
+                ("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
 class BvhInfo(ctypes.Structure):
     _fields_ = [("n_tri", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32),
                 ("depth", ctypes.c_uint32), ("sah_cost", ctypes.c_float), ("build_ms", ctypes.c_float),
@@ -125,6 +135,14 @@ _SIGNATURES = {
     "vkrt_get_trace_config": (_i32, [_vp, ctypes.POINTER(TraceConfig)]),
     "vkrt_trace_closest": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_trace_any": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_batch_create": (_i32, [_vp, _vp, _u32, ctypes.POINTER(_vp)]),
+    "vkrt_batch_destroy": (None, [_vp]),
+    "vkrt_batch_set_config": (_i32, [_vp, ctypes.POINTER(BatchConfig)]),
+    "vkrt_batch_get_config": (_i32, [_vp, ctypes.POINTER(BatchConfig)]),
+    "vkrt_batch_launch": (_i32, [_vp, _i32]),
+    "vkrt_batch_autotune": (_i32, [_vp, _u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_float)]),
+    "vkrt_batch_total_rays": (_u64, [_vp]),
+    "vkrt_batch_closest": (_i32, [_vp, _vp, _u32, _i32]),
     "vkrt_pt_default_settings": (None, [ctypes.POINTER(PtSettings)]),
     "vkrt_pt_create": (_i32, [_vp, ctypes.POINTER(PtSettings), _u32, ctypes.POINTER(_vp)]),
     "vkrt_pt_destroy": (None, [_vp]),
@@ -431,6 +449,49 @@ class Bvh:
     def close(self):
         if self._h:
             lib().vkrt_bvh_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """many (bvh, device rays, device hits) items traced by one persistent launch, replayed as a hipGraph"""
+
+    def __init__(self, ctx, items):
+        """items: list of (Bvh, DeviceArray rays, DeviceArray hits)"""
+        self.ctx, self._items = ctx, items
+        arr = (BatchItem * len(items))(*[BatchItem(b.handle, r.ptr, h.ptr, len(r), 0) for b, r, h in items])
+        self._h = _vp()
+        ctx.check(lib().vkrt_batch_create(ctx.handle, arr, len(items), ctypes.byref(self._h)))
+
+    def launch(self, flags=0):
+        self.ctx.check(lib().vkrt_batch_launch(self._h, flags))
+
+    @property
+    def config(self):
+        c = BatchConfig()
+        self.ctx.check(lib().vkrt_batch_get_config(self._h, ctypes.byref(c)))
+        return c.block_size, c.lds_stack_depth, c.tile, c.blocks_per_cu
+
+    @config.setter
+    def config(self, cfg):
+        self.ctx.check(lib().vkrt_batch_set_config(self._h, ctypes.byref(BatchConfig(*cfg))))
+
+    def autotune(self, repeats=2):
+        best, ms = BatchConfig(), ctypes.c_float()
+        self.ctx.check(lib().vkrt_batch_autotune(self._h, repeats, ctypes.byref(best), ctypes.byref(ms)))
+        return (best.block_size, best.lds_stack_depth, best.tile, best.blocks_per_cu), ms.value
+
+    def total_rays(self):
+        return lib().vkrt_batch_total_rays(self._h)
+
+    def close(self):
+        if self._h:
+            lib().vkrt_batch_destroy(self._h)
             self._h = _vp()
 
     def __del__(self):

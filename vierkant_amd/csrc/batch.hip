@@ -1,0 +1,305 @@
+// batch.hip - many scenes and ray batches of uneven size traced by one persistent launch, replayed as a
+// captured hipGraph, with a launch-shape autotuner.
+//
+// The reference traces one scene per vkCmdTraceRaysKHR (src/RayTracer.cpp:128-129); this batches the work
+// of many independent scenes. Every scene's rays are cut into tiles of `tile` rays; a device table holds
+// each item's BVH and buffers and the prefix of its tile counts. Blocks of k_batch_trace claim tiles with
+// one atomic each and leave once the counter passes the number of tiles, so every wave reaches the exit.
+// The graph is [k_batch_reset (zeroes the counter with a kernel), k_batch_trace]; sizes and pointers live
+// in device memory, so the captured graph replays without re-capture until the launch shape changes.
+#include <algorithm>
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "bvh_types.hpp"
+#include "context.hpp"
+#include "traverse.hpp"
+
+namespace vkrt
+{
+
+struct batch_item_dev
+{
+    const vkrt_bvh_node *nodes;
+    const float4 *tris;
+    const float4 *rays;
+    vkrt_hit *hits;
+    int32_t root;
+    uint32_t empty;
+    uint32_t n_rays;
+    uint32_t pad;
+};
+
+__global__ void k_batch_reset(uint32_t *counter) { counter[threadIdx.x] = 0u; }
+
+template<int BLOCK, int LDS>
+__global__ void __launch_bounds__(BLOCK) k_batch_trace(const batch_item_dev *__restrict__ items, uint32_t n_items,
+                                                       const uint32_t *__restrict__ tile_prefix, uint32_t tile,
+                                                       uint32_t *__restrict__ counter)
+{
+    __shared__ int32_t smem[LDS > 0 ? LDS * BLOCK : 1];
+    __shared__ uint32_t s_tile;
+    const uint32_t total = tile_prefix[n_items];
+    while(true)
+    {
+        if(threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t t = s_tile;
+        __syncthreads();
+        if(t >= total) break;
+        // item owning tile t: the last item whose prefix is <= t (binary search, block-uniform)
+        uint32_t lo = 0, hi = n_items;
+        while(hi - lo > 1)
+        {
+            const uint32_t mid = (lo + hi) >> 1;
+            if(tile_prefix[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const batch_item_dev it = items[lo];
+        const uint32_t first = (t - tile_prefix[lo]) * tile;
+        const uint32_t count = min(tile, it.n_rays - first);
+        for(uint32_t k = threadIdx.x; k < count; k += BLOCK)
+        {
+            const uint32_t r = first + k;
+            float bt = 0.f, bu = 0.f, bv = 0.f;
+            uint32_t prim = VKRT_INVALID_PRIM;
+            bool hit = false;
+            if(!it.empty)
+            {
+                ray_setup rs = setup_ray(it.rays, r);
+                hit = traverse<false, BLOCK, LDS>(it.nodes, it.tris, it.root, rs, smem + threadIdx.x, bt, bu, bv, prim);
+            }
+            it.hits[r] = hit ? vkrt_hit{bt, bu, bv, prim}
+                             : vkrt_hit{__int_as_float(0x7f800000), 0.f, 0.f, VKRT_INVALID_PRIM};
+        }
+    }
+}
+
+template<int BLOCK, int LDS>
+static void launch_batch(hipStream_t s, uint32_t grid, const batch_item_dev *items, uint32_t n_items,
+                         const uint32_t *prefix, uint32_t tile, uint32_t *counter)
+{
+    k_batch_trace<BLOCK, LDS><<<grid, BLOCK, 0, s>>>(items, n_items, prefix, tile, counter);
+}
+
+template<int BLOCK>
+static void launch_batch_lds(hipStream_t s, uint32_t lds, uint32_t grid, const batch_item_dev *items,
+                             uint32_t n_items, const uint32_t *prefix, uint32_t tile, uint32_t *counter)
+{
+    switch(lds)
+    {
+        case 0: launch_batch<BLOCK, 0>(s, grid, items, n_items, prefix, tile, counter); break;
+        case 8: launch_batch<BLOCK, 8>(s, grid, items, n_items, prefix, tile, counter); break;
+        case 16: launch_batch<BLOCK, 16>(s, grid, items, n_items, prefix, tile, counter); break;
+        default: launch_batch<BLOCK, 32>(s, grid, items, n_items, prefix, tile, counter); break;
+    }
+}
+
+static bool valid_batch_config(const vkrt_batch_config &c)
+{
+    const bool block = c.block_size == 64 || c.block_size == 128 || c.block_size == 256;
+    const bool lds = c.lds_stack_depth == 0 || c.lds_stack_depth == 8 || c.lds_stack_depth == 16 ||
+                     c.lds_stack_depth == 32;
+    const bool tile = c.tile >= 64 && c.tile <= (1u << 20) && (c.tile % 64) == 0;
+    return block && lds && tile && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 64;
+}
+
+}// namespace vkrt
+
+struct vkrt_batch
+{
+    vkrt_ctx *ctx = nullptr;
+    std::vector<vkrt::batch_item_dev> items;
+    std::vector<uint32_t> n_rays;
+    vkrt::device_buffer d_items, d_prefix, d_counter;
+    vkrt_batch_config cfg{256, 16, 1024, 8, {0, 0, 0, 0}};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t total_rays = 0;
+};
+
+namespace vkrt
+{
+
+static void release_graph(vkrt_batch *b)
+{
+    if(b->exec) (void)hipGraphExecDestroy(b->exec);
+    if(b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+//! tile prefix for the current tile size, uploaded, then the graph captured
+static void prepare(vkrt_batch *b)
+{
+    hipStream_t s = b->ctx->stream;
+    const uint32_t n = static_cast<uint32_t>(b->items.size());
+    std::vector<uint32_t> prefix(n + 1, 0);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const uint64_t tiles = (b->n_rays[i] + (uint64_t)b->cfg.tile - 1) / b->cfg.tile;
+        VKRT_REQUIRE(prefix[i] + tiles < 0xffffffffull, "too many tiles");
+        prefix[i + 1] = prefix[i] + static_cast<uint32_t>(tiles);
+    }
+    b->d_prefix.reserve(prefix.size() * 4);
+    VKRT_HIP(hipMemcpyAsync(b->d_prefix.ptr, prefix.data(), prefix.size() * 4, hipMemcpyHostToDevice, s));
+    VKRT_HIP(hipStreamSynchronize(s));
+    release_graph(b);
+    const uint32_t grid = std::max<uint32_t>(1, b->cfg.blocks_per_cu * b->ctx->cu_count);
+    VKRT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    k_batch_reset<<<1, 4, 0, s>>>(b->d_counter.as<uint32_t>());
+    const batch_item_dev *items = b->d_items.as<batch_item_dev>();
+    const uint32_t *pre = b->d_prefix.as<uint32_t>();
+    uint32_t *counter = b->d_counter.as<uint32_t>();
+    switch(b->cfg.block_size)
+    {
+        case 64: launch_batch_lds<64>(s, b->cfg.lds_stack_depth, grid, items, n, pre, b->cfg.tile, counter); break;
+        case 128: launch_batch_lds<128>(s, b->cfg.lds_stack_depth, grid, items, n, pre, b->cfg.tile, counter); break;
+        default: launch_batch_lds<256>(s, b->cfg.lds_stack_depth, grid, items, n, pre, b->cfg.tile, counter); break;
+    }
+    hipError_t launch_err = hipGetLastError();
+    hipError_t end_err = hipStreamEndCapture(s, &b->graph);
+    VKRT_HIP(launch_err);
+    VKRT_HIP(end_err);
+    VKRT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+}
+
+static float timed_launch(vkrt_batch *b, int repeats)
+{
+    hipStream_t s = b->ctx->stream;
+    hipEvent_t e0, e1;
+    VKRT_HIP(hipEventCreate(&e0));
+    VKRT_HIP(hipEventCreate(&e1));
+    VKRT_HIP(hipGraphLaunch(b->exec, s));// warm-up
+    VKRT_HIP(hipEventRecord(e0, s));
+    for(int k = 0; k < repeats; ++k) VKRT_HIP(hipGraphLaunch(b->exec, s));
+    VKRT_HIP(hipEventRecord(e1, s));
+    VKRT_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    VKRT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms / repeats;
+}
+
+}// namespace vkrt
+
+using namespace vkrt;
+
+extern "C" {
+
+int vkrt_batch_create(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_items, vkrt_batch **out)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(out, "null output handle");
+        *out = nullptr;
+        VKRT_REQUIRE(n_items > 0 && items, "a batch needs at least one item");
+        auto b = std::make_unique<vkrt_batch>();
+        b->ctx = ctx;
+        for(uint32_t i = 0; i < n_items; ++i)
+        {
+            const vkrt_batch_item &it = items[i];
+            VKRT_REQUIRE(it.bvh, "null bvh in batch item");
+            VKRT_REQUIRE(it.bvh->device == ctx->device, "bvh belongs to another device");
+            VKRT_REQUIRE(it.n_rays == 0 || (it.rays && it.hits), "null ray or hit buffer in batch item");
+            batch_item_dev d{it.bvh->nodes.as<vkrt_bvh_node>(), it.bvh->tris.as<float4>(),
+                             reinterpret_cast<const float4 *>(it.rays), it.hits, it.bvh->root_ref,
+                             it.bvh->empty ? 1u : 0u, it.n_rays, 0u};
+            b->items.push_back(d);
+            b->n_rays.push_back(it.n_rays);
+            b->total_rays += it.n_rays;
+        }
+        b->d_items.reserve(b->items.size() * sizeof(batch_item_dev));
+        VKRT_HIP(hipMemcpyAsync(b->d_items.ptr, b->items.data(), b->items.size() * sizeof(batch_item_dev),
+                                hipMemcpyHostToDevice, ctx->stream));
+        b->d_counter.reserve(64);
+        prepare(b.get());
+        *out = b.release();
+    });
+}
+
+void vkrt_batch_destroy(vkrt_batch *b)
+{
+    if(!b) return;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    release_graph(b);
+    delete b;
+}
+
+int vkrt_batch_set_config(vkrt_batch *b, const vkrt_batch_config *cfg)
+{
+    if(!b) return VKRT_E_INVALID_ARG;
+    return guarded(b->ctx, [&] {
+        VKRT_REQUIRE(cfg && valid_batch_config(*cfg),
+                     "block_size 64/128/256, lds_stack_depth 0/8/16/32, tile a multiple of 64, blocks_per_cu 1..64");
+        b->cfg = *cfg;
+        prepare(b);
+    });
+}
+
+int vkrt_batch_get_config(const vkrt_batch *b, vkrt_batch_config *out)
+{
+    if(!b || !out) return VKRT_E_INVALID_ARG;
+    *out = b->cfg;
+    return VKRT_OK;
+}
+
+int vkrt_batch_launch(vkrt_batch *b, int flags)
+{
+    if(!b) return VKRT_E_INVALID_ARG;
+    return guarded(b->ctx, [&] {
+        VKRT_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+        if(!(flags & VKRT_ASYNC)) VKRT_HIP(hipStreamSynchronize(b->ctx->stream));
+    });
+}
+
+int vkrt_batch_autotune(vkrt_batch *b, uint32_t repeats, vkrt_batch_config *best, float *best_ms)
+{
+    if(!b) return VKRT_E_INVALID_ARG;
+    return guarded(b->ctx, [&] {
+        const uint32_t blocks[] = {64, 128, 256};
+        const uint32_t lds[] = {0, 8, 16, 32};
+        const uint32_t tiles[] = {256, 1024, 4096};
+        const uint32_t occupancy[] = {4, 8};
+        vkrt_batch_config win = b->cfg;
+        float win_ms = 3.4e38f;
+        const int reps = repeats ? static_cast<int>(repeats) : 2;
+        for(uint32_t bs: blocks)
+            for(uint32_t ld: lds)
+                for(uint32_t tl: tiles)
+                    for(uint32_t oc: occupancy)
+                    {
+                        // LDS of one block must leave room for at least two blocks per CU (160 KiB)
+                        if((size_t)ld * bs * 4 > 64 * 1024) continue;
+                        if(bs * oc > 4096) continue;
+                        b->cfg = vkrt_batch_config{bs, ld, tl, oc, {0, 0, 0, 0}};
+                        prepare(b);
+                        const float ms = timed_launch(b, reps);
+                        if(ms < win_ms)
+                        {
+                            win_ms = ms;
+                            win = b->cfg;
+                        }
+                    }
+        b->cfg = win;
+        prepare(b);
+        if(best) *best = win;
+        if(best_ms) *best_ms = win_ms;
+    });
+}
+
+uint64_t vkrt_batch_total_rays(const vkrt_batch *b) { return b ? b->total_rays : 0u; }
+
+int vkrt_batch_closest(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_items, int flags)
+{
+    vkrt_batch *b = nullptr;
+    int rc = vkrt_batch_create(ctx, items, n_items, &b);
+    if(rc != VKRT_OK) return rc;
+    rc = vkrt_batch_launch(b, flags & ~VKRT_ASYNC);
+    vkrt_batch_destroy(b);
+    return rc;
+}
+
+}// extern "C"

@@ -367,3 +367,40 @@ def minstd_rand0(seed, n):
         x = (16807 * x) % 2147483647
         out.append(x)
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# configuration 4: many independent scenes for the batched launch
+# ------------------------------------------------------------------------------------------------
+def config4(n_scenes=256, seed=0, tri_range=(1000, 500000), ray_range=(16384, 2097152)):
+    """Yields (positions, indices, rays) for `n_scenes` scenes. Scene s draws (from
+    numpy.random.default_rng(seed), in order) its triangle count T and ray count R log-uniform in the given
+    ranges; it is a 4 x 4 fBm heightfield patch with about T/2 triangles plus round(T / 2560) IcoSpheres of
+    subdivision 3 (scaled copies of the unit one, radius U[0.05, 0.3], each vertex copy jittered by
+    N(0, 0.01 r)), and R rays with origins uniform in 1.5x the scene box and directions uniform on S^2."""
+    rng = np.random.default_rng(seed)
+    unit_pos, unit_idx = vk.icosphere(1.0, 3)
+    lt, lr = np.log(tri_range), np.log(ray_range)
+    for s in range(n_scenes):
+        n_tri = int(np.exp(rng.uniform(lt[0], lt[1])))
+        n_ray = int(np.exp(rng.uniform(lr[0], lr[1])))
+        g = max(1, int(round(np.sqrt(n_tri / 4.0))))
+        pos, idx = vk.plane(4.0, 4.0, g, g)
+        pos[:, 1] = (0.2 * fbm(pos[:, 0], pos[:, 2], seed=s)).astype(np.float32)
+        k = max(1, int(round(n_tri / 2560.0)))
+        r = rng.uniform(0.05, 0.3, k)
+        c = np.stack([rng.uniform(-2, 2, k), rng.uniform(0.1, 1.0, k), rng.uniform(-2, 2, k)], axis=1)
+        sp = unit_pos[None, :, :] * r[:, None, None] + c[:, None, :]
+        sp = sp + rng.normal(0.0, 1.0, sp.shape) * (0.01 * r)[:, None, None]
+        base = len(pos) + np.arange(k, dtype=np.uint32)[:, None] * np.uint32(len(unit_pos))
+        si = (unit_idx[None, :] + base).reshape(-1)
+        positions = np.ascontiguousarray(np.concatenate([pos, sp.reshape(-1, 3).astype(np.float32)]))
+        indices = np.ascontiguousarray(np.concatenate([idx, si.astype(np.uint32)]))
+        lo, hi = positions.min(0), positions.max(0)
+        center, half = 0.5 * (lo + hi), 0.75 * (hi - lo)
+        o = (center + (rng.uniform(-1.0, 1.0, (n_ray, 3)) * half)).astype(np.float32)
+        z = rng.uniform(-1.0, 1.0, n_ray)
+        phi = rng.uniform(0.0, 2.0 * np.pi, n_ray)
+        rxy = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+        d = glm_normalize(np.stack([rxy * np.cos(phi), rxy * np.sin(phi), z], axis=1).astype(np.float32))
+        yield positions, indices, make_rays(o, d)
