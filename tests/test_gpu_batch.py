@@ -45,12 +45,13 @@ def test_batch_graph_replay_and_autotune(ctx, small_batch):
     scenes, bvhs, rays, hits, flats = small_batch
     batch = vk.Batch(ctx, list(zip(bvhs, rays, hits)))
     want = per_scene(ctx, small_batch)
+    poison = np.full(max(h.nbytes for h in hits), 0xAB, np.uint8)
     for cfg in [(64, 0, 64, 1), (128, 8, 256, 4), (256, 32, 4096, 8), (256, 16, 1024, 2)]:
         batch.config = cfg
         assert batch.config == cfg
         for _ in range(2):                          # replays of the same captured graph
             for h in hits:                          # poison outputs between replays
-                ctx.check(vk.lib().vkrt_copy(ctx.handle, h.ptr, rays[0].ptr, 0, 2, 0))
+                ctx.check(vk.lib().vkrt_copy(ctx.handle, h.ptr, poison.ctypes.data, h.nbytes, 0, 0))
             batch.launch()
             for s, (h, w) in enumerate(zip(hits, want)):
                 assert_hits_equal(h.host(), w, f"cfg {cfg} scene {s}")
