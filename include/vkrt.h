@@ -118,6 +118,7 @@ typedef struct vkrt_result
 /* Create a context on HIP device `device`. `hip_stream` is an existing hipStream_t to run on,
  * or NULL to let the context create its own. */
 int vkrt_create(int device, void *hip_stream, vkrt_ctx **out);
+/* Destroy batches and path tracers made on a context before the context: they use its stream. */
 void vkrt_destroy(vkrt_ctx *ctx);
 const char *vkrt_last_error(const vkrt_ctx *ctx);
 int vkrt_sync(vkrt_ctx *ctx);
@@ -265,6 +266,7 @@ typedef struct vkrt_batch_config
 
 /* Capture the batch (items are copied; their buffers must stay valid) into a graph with the default shape. */
 int vkrt_batch_create(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_items, vkrt_batch **out);
+/* Before the context and before the BVHs and buffers its items name. */
 void vkrt_batch_destroy(vkrt_batch *batch);
 /* Change the launch shape (re-captures the graph). Results do not depend on it. */
 int vkrt_batch_set_config(vkrt_batch *batch, const vkrt_batch_config *cfg);
@@ -366,6 +368,7 @@ typedef struct vkrt_pt_stats
 void vkrt_pt_default_settings(vkrt_pt_settings *out);
 /* seed: create_info_t::seed; each traced batch draws its random seed from std::default_random_engine(seed) */
 int vkrt_pt_create(vkrt_ctx *ctx, const vkrt_pt_settings *settings, uint32_t seed, vkrt_pt **out);
+/* Before the context. */
 void vkrt_pt_destroy(vkrt_pt *pt);
 int vkrt_pt_set_settings(vkrt_pt *pt, const vkrt_pt_settings *settings);
 int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out);

@@ -1,0 +1,129 @@
+// test_host_api.cpp - the C++ host layer end to end on a GPU: RayBuilder's BVH queries equal the scalar
+// vierkant::intersect(Triangle, Ray) brute-force loop bit for bit, and PBRPathTracer keeps the reference's
+// batch semantics. Prints "host api ok" and exits 0 on success.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "vierkant/PBRPathTracer.hpp"
+#include "vierkant/RayBuilder.hpp"
+#include "vierkant/intersection.hpp"
+
+using namespace vierkant;
+
+static int fail(const char *what)
+{
+    std::printf("FAIL: %s\n", what);
+    return 1;
+}
+
+static uint32_t bits(float f)
+{
+    uint32_t b;
+    std::memcpy(&b, &f, 4);
+    return b;
+}
+
+int main()
+{
+    auto ctx = std::make_shared<gpu_context>(0);
+
+    // a subdivision-3 icosphere from the library's Geometry::IcoSphere
+    uint32_t nv = 0, ni = 0;
+    vkrt_icosphere_size(3, &nv, &ni);
+    std::vector<float> pos(3 * (size_t)nv);
+    std::vector<uint32_t> idx(ni);
+    vkrt_icosphere(1.0f, 3, pos.data(), idx.data());
+
+    RayBuilder builder(ctx);
+    RayBuilder::mesh_t mesh;
+    mesh.positions = pos.data();
+    mesh.num_vertices = nv;
+    mesh.indices = idx.data();
+    mesh.num_indices = ni;
+    auto as = builder.create_mesh_structure(mesh);
+    if(as->info().n_tri != ni / 3) return fail("triangle count");
+
+    // deterministic rays (LCG), normalized by the Ray constructor
+    uint32_t s = 12345u;
+    auto rnd = [&]() {
+        s = 1664525u * s + 1013904223u;
+        return (float)(s >> 8) / 16777216.0f;
+    };
+    std::vector<vkrt_ray> rays;
+    std::vector<Ray> host_rays;
+    for(int i = 0; i < 4096; ++i)
+    {
+        vec3 o{rnd() * 6.f - 3.f, rnd() * 6.f - 3.f, rnd() * 6.f - 3.f};
+        vec3 d{rnd() * 2.f - 1.f, rnd() * 2.f - 1.f, rnd() * 2.f - 1.f};
+        Ray r(o, d);
+        host_rays.push_back(r);
+        rays.push_back(vkrt_ray{r.origin.x, r.origin.y, r.origin.z, 0.f, r.direction.x, r.direction.y,
+                                r.direction.z, INFINITY});
+    }
+    auto hits = builder.trace_closest(*as, rays);
+    auto occ = builder.trace_any(*as, rays);
+    int n_hit = 0;
+    for(size_t i = 0; i < rays.size(); ++i)
+    {
+        float best = INFINITY;
+        uint32_t best_prim = VKRT_INVALID_PRIM;
+        ray_triangle_intersection best_hit;
+        for(uint32_t t = 0; t < ni / 3; ++t)
+        {
+            Triangle tri{{pos[3 * idx[3 * t]], pos[3 * idx[3 * t] + 1], pos[3 * idx[3 * t] + 2]},
+                         {pos[3 * idx[3 * t + 1]], pos[3 * idx[3 * t + 1] + 1], pos[3 * idx[3 * t + 1] + 2]},
+                         {pos[3 * idx[3 * t + 2]], pos[3 * idx[3 * t + 2] + 1], pos[3 * idx[3 * t + 2] + 2]}};
+            auto h = intersect(host_rays[i], tri);
+            if(h.type == INTERSECT && h.distance > 0.f && h.distance < best)
+            {
+                best = h.distance;
+                best_prim = t;
+                best_hit = h;
+            }
+        }
+        if(hits[i].prim != best_prim) return fail("closest primitive differs from the scalar intersect loop");
+        if(best_prim != VKRT_INVALID_PRIM)
+        {
+            ++n_hit;
+            if(bits(hits[i].t) != bits(best_hit.distance) || bits(hits[i].u) != bits(best_hit.u) ||
+               bits(hits[i].v) != bits(best_hit.v))
+                return fail("t/u/v bits differ from the scalar intersect");
+        }
+        if(occ[i] != (best_prim != VKRT_INVALID_PRIM ? 1 : 0)) return fail("any-hit flag");
+    }
+    if(n_hit < 100) return fail("too few hits");
+
+    // scalar volume tests of the API surface
+    AABB box({0.f, 0.f, 0.f}, {1.f, 1.f, 1.f});
+    if(box.intersect(Ray({-1.f, 0.5f, 0.5f}, {1.f, 0.f, 0.f})).distance != 1.0f) return fail("AABB entry distance");
+    if(intersect(Sphere{{0.f, 0.f, 5.f}, 1.f}, Ray({0.f, 0.f, 0.f}, {0.f, 0.f, 1.f})).distance != 4.0f)
+        return fail("sphere distance");
+
+    // PBRPathTracer: an empty scene, two batches, then suspended
+    PBRPathTracer::create_info_t info;
+    info.seed = 3;
+    info.settings.resolution[0] = 32;
+    info.settings.resolution[1] = 16;
+    info.settings.max_num_batches = 2;
+    auto pt = PBRPathTracer::create(ctx, info);
+    float zero3[3] = {0.f, 0.f, 0.f};
+    uint32_t obj_mat = 0;
+    vkrt_material mat{};
+    mat.color[0] = mat.color[1] = mat.color[2] = mat.color[3] = 1.f;
+    mat.ior = 1.5f;
+    mat.roughness = 1.f;
+    PBRPathTracer::scene_t scene{zero3, zero3, 0, nullptr, 0, nullptr, &obj_mat, 1, &mat, 1, 1.0f};
+    pt->set_scene(scene);
+    PBRPathTracer::camera_t cam{{0.f, 0.f, 0.f}, 1.0f, {0.f, 0.f, -1.f}, 0.f, {0.f, 1.f, 0.f}, 1.f};
+    for(int f = 0; f < 3; ++f) pt->render_scene(cam);
+    if(pt->current_batch() != 2) return fail("batch index");
+    if(pt->statistics().size() != 2) return fail("statistics history");
+    auto px = pt->pixels();
+    if(px.size() != 32 * 16 || !(px[0].radiance[0] > 0.f)) return fail("pixels");
+    pt->reset_accumulator();
+    if(pt->current_batch() != 0) return fail("reset");
+    std::printf("host api ok (%d hits of %zu rays)\n", n_hit, rays.size());
+    return 0;
+}

@@ -4,8 +4,10 @@ This module is the Python binding of ``libvkrt.so`` (declared in ``include/vkrt.
 in-tree shared library and raises if it is missing: there is no CPU fallback. Arrays are numpy arrays
 (host memory, staged by the library) or DeviceArray buffers in HBM (passed as device pointers).
 """
+import atexit
 import ctypes
 import pathlib
+import weakref
 
 import numpy as np
 
@@ -222,6 +224,7 @@ class DeviceArray:
         h = _vp()
         ctx.check(lib().vkrt_device_alloc(ctx.handle, self.nbytes, ctypes.byref(h)))
         self.ptr = h.value
+        ctx._adopt(self)
 
     @classmethod
     def from_host(cls, ctx, a):
@@ -242,6 +245,8 @@ class DeviceArray:
         if self.ptr:
             lib().vkrt_device_free(self.ctx.handle, self.ptr)
             self.ptr = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -293,12 +298,26 @@ class Context:
 
     def __init__(self, device=0, stream=None):
         self._h = _vp()
+        self._children = []
         rc = lib().vkrt_create(device, stream, ctypes.byref(self._h))
         if rc != OK:
             raise VkrtError(rc, "vkrt_create failed (no gfx950 device?)")
+        _live_contexts.add(self)
+
+    def _adopt(self, obj):
+        """Track an object that holds native state of this context, so that close() can release it first."""
+        self._children = [r for r in self._children if r() is not None]
+        self._children.append(weakref.ref(obj))
 
     def close(self):
+        """Release every batch, path tracer, BVH and device array made on this context, newest first (a
+        batch refers to its BVHs and arrays, every object to the context), then the context itself."""
         if self._h:
+            for r in reversed(self._children):
+                obj = r()
+                if obj is not None:
+                    obj.close()
+            self._children = []
             lib().vkrt_destroy(self._h)
             self._h = _vp()
 
@@ -427,11 +446,23 @@ class Context:
         return out
 
 
+_live_contexts = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live_contexts():
+    """At interpreter exit the finalizer can release a context before the objects that use it; close every
+    live context (and with it its dependents, newest first) while the module is still intact."""
+    for c in list(_live_contexts):
+        c.close()
+
+
 class Bvh:
     """A BVH built by vkrt_bvh_build; destroyed with the object."""
 
     def __init__(self, ctx, handle):
         self.ctx, self._h = ctx, handle
+        ctx._adopt(self)
 
     @property
     def handle(self):
@@ -467,6 +498,7 @@ class Batch:
         arr = (BatchItem * len(items))(*[BatchItem(b.handle, r.ptr, h.ptr, len(r), 0) for b, r, h in items])
         self._h = _vp()
         ctx.check(lib().vkrt_batch_create(ctx.handle, arr, len(items), ctypes.byref(self._h)))
+        ctx._adopt(self)
 
     def launch(self, flags=0):
         self.ctx.check(lib().vkrt_batch_launch(self._h, flags))
@@ -522,6 +554,7 @@ class PathTracer:
         s = settings if settings is not None else default_settings()
         ctx.check(lib().vkrt_pt_create(ctx.handle, ctypes.byref(s), seed, ctypes.byref(self._h)))
         self._keep = None
+        ctx._adopt(self)
 
     @property
     def settings(self):

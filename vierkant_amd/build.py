@@ -87,10 +87,25 @@ def build_test_helpers():
         raise RuntimeError("host build of rt_math.hpp failed:\n" + res.stderr)
 
 
+def build_host_api_test():
+    """tests/cpp/test_host_api: the C++ host layer (vierkant/*.hpp) linked against libvkrt.so"""
+    src = ROOT / "tests" / "cpp" / "test_host_api.cpp"
+    if not src.exists():
+        return
+    out = src.parent / "test_host_api"
+    cmd = ["g++", "-O2", "-std=c++20", "-ffp-contract=off", f"-I{ROOT / 'include'}", f"-I{PKG / 'include'}",
+           str(src), "-o", str(out), f"-L{PKG}", "-lvkrt", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../vierkant_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("build of the C++ host-layer test failed:\n" + res.stderr)
+
+
 def build_all(verbose=False):
     build_oracle()
     build_test_helpers()
-    return build_lib(verbose=verbose)
+    lib = build_lib(verbose=verbose)
+    build_host_api_test()
+    return lib
 
 
 if __name__ == "__main__":

@@ -1,0 +1,139 @@
+// PBRPathTracer.hpp - the path tracer for C++ callers, with the reference's settings and accumulation
+// semantics (include/vierkant/PBRPathTracer.hpp:42-142, src/PBRPathTracer.cpp:184-255, :677-682).
+//
+// settings_t keeps the reference's field names and defaults for everything that changes the traced image;
+// display-side fields (denoising, tonemap, bloom, gamma, exposure, ...) are kept for source compatibility and
+// have no effect here. render_scene traces one batch unless max_num_batches is reached with
+// suspend_trace_when_done set, then advances the batch index as min(batch + 1, max_num_batches).
+#pragma once
+
+#include <chrono>
+#include <deque>
+#include <memory>
+#include <vector>
+
+#include "vierkant/RayBuilder.hpp"
+
+namespace vierkant
+{
+
+class PBRPathTracer
+{
+public:
+    struct settings_t
+    {
+        uint32_t resolution[2] = {1280, 720};
+        uint32_t max_num_batches = 0;
+        uint32_t num_samples = 1;
+        uint32_t max_trace_depth = 6;
+        float max_path_beta = 0.f;
+        bool suspend_trace_when_done = true;
+        bool disable_material = false;
+        uint32_t mis_mode = 0;
+        bool suppress_refractive_caustics = false;
+        bool draw_skybox = true;
+        bool compaction = true;
+        bool denoising = false;
+        bool tonemap = true;
+        bool bloom = true;
+        float gamma = 1.0f;
+        float exposure = 2.0f;
+        bool depth_of_field = false;
+        bool suppress_reset = false;
+        uint32_t timing_history_size = 300;
+    };
+
+    struct create_info_t
+    {
+        uint32_t seed = 0;
+        settings_t settings = {};
+    };
+
+    struct statistics_t
+    {
+        std::chrono::steady_clock::time_point timestamp;
+        double raytrace_ms = 0.0;
+        uint64_t segments = 0;
+    };
+
+    using scene_t = vkrt_pt_scene;
+    using camera_t = vkrt_camera;
+
+    static std::shared_ptr<PBRPathTracer> create(gpu_context_ptr ctx, const create_info_t &info)
+    {
+        return std::shared_ptr<PBRPathTracer>(new PBRPathTracer(std::move(ctx), info));
+    }
+
+    PBRPathTracer(const PBRPathTracer &) = delete;
+    PBRPathTracer &operator=(const PBRPathTracer &) = delete;
+    ~PBRPathTracer() { vkrt_pt_destroy(m_pt); }
+
+    //! upload a scene and build its acceleration structure
+    void set_scene(const scene_t &scene) { m_ctx->check(vkrt_pt_set_scene(m_pt, &scene)); }
+
+    //! one frame: settings are copied, a batch is traced unless suspended
+    void render_scene(const camera_t &camera)
+    {
+        vkrt_pt_settings s = to_c(settings);
+        m_ctx->check(vkrt_pt_set_settings(m_pt, &s));
+        m_ctx->check(vkrt_pt_render(m_pt, &camera));
+        vkrt_pt_stats st{};
+        vkrt_pt_get_stats(m_pt, &st);
+        if(st.traced)
+        {
+            m_statistics.push_back({std::chrono::steady_clock::now(), st.raytrace_ms, st.segments});
+            while(m_statistics.size() > settings.timing_history_size) m_statistics.pop_front();
+        }
+    }
+
+    [[nodiscard]] size_t current_batch() const { return vkrt_pt_current_batch(m_pt); }
+
+    void reset_accumulator()
+    {
+        vkrt_pt_settings s = to_c(settings);
+        m_ctx->check(vkrt_pt_set_settings(m_pt, &s));
+        m_ctx->check(vkrt_pt_reset_accumulator(m_pt));
+    }
+
+    //! the pixel_data_t buffer, row-major from the top-left
+    [[nodiscard]] std::vector<vkrt_pixel> pixels() const
+    {
+        vkrt_pt_settings s{};
+        vkrt_pt_get_settings(m_pt, &s);
+        std::vector<vkrt_pixel> out((size_t)s.resolution[0] * s.resolution[1]);
+        m_ctx->check(vkrt_pt_read_pixels(m_pt, out.data(), 0));
+        return out;
+    }
+
+    [[nodiscard]] const std::deque<statistics_t> &statistics() const { return m_statistics; }
+
+    settings_t settings;
+
+private:
+    PBRPathTracer(gpu_context_ptr ctx, const create_info_t &info) : settings(info.settings), m_ctx(std::move(ctx))
+    {
+        vkrt_pt_settings s = to_c(settings);
+        m_ctx->check(vkrt_pt_create(m_ctx->handle(), &s, info.seed, &m_pt));
+    }
+
+    static vkrt_pt_settings to_c(const settings_t &s)
+    {
+        vkrt_pt_settings c{};
+        c.resolution[0] = s.resolution[0];
+        c.resolution[1] = s.resolution[1];
+        c.max_num_batches = s.max_num_batches;
+        c.num_samples = s.num_samples;
+        c.max_trace_depth = s.max_trace_depth;
+        c.max_path_beta = s.max_path_beta;
+        c.suspend_trace_when_done = s.suspend_trace_when_done ? 1u : 0u;
+        c.disable_material = s.disable_material ? 1u : 0u;
+        c.suppress_reset = s.suppress_reset ? 1u : 0u;
+        return c;
+    }
+
+    gpu_context_ptr m_ctx;
+    vkrt_pt *m_pt = nullptr;
+    std::deque<statistics_t> m_statistics;
+};
+
+}// namespace vierkant
