@@ -386,6 +386,54 @@ int vkrt_pt_read_pixels(vkrt_pt *pt, vkrt_pixel *out, int flags);
 int vkrt_pt_get_stats(const vkrt_pt *pt, vkrt_pt_stats *out);
 
 /* ---------------------------------------------------------------------------------------------
+ * radiance cache: multiresolution hash grid (16 levels x 2^log2_table entries x 2 features, fp16) +
+ * MLP 64 -> 5 x 64 ReLU -> 3 (bf16 matrix inputs, fp32 accumulation), trained online with Adam on a
+ * relative L2 loss. No counterpart in the reference (a new component, not a drop-in).
+ *
+ * Record (16 floats per sample): pos.xyz, roughness | wo.xyz (unit, towards the viewer), metalness |
+ * shading normal.xyz, transmission | albedo.rgb, unused. Targets: 4 floats (rgb radiance, unused).
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct vkrt_rc vkrt_rc;
+
+typedef struct
+{
+    uint32_t log2_table;      /* hash table entries per level = 2^log2_table, 10..24 (default 19) */
+    uint32_t base_resolution; /* coarsest grid resolution (16) */
+    uint32_t max_resolution;  /* finest grid resolution (2048) */
+    float learning_rate;      /* Adam (1e-2) */
+    float beta1, beta2;       /* (0.9, 0.99) */
+    float epsilon;            /* (1e-15) */
+    uint32_t reserved[9];
+} vkrt_rc_params;
+
+/* vkrt_rc_train flag: compute gradients only (readable with vkrt_rc_get_gradients), no Adam step */
+#define VKRT_RC_GRAD_ONLY 4
+
+void vkrt_rc_default_params(vkrt_rc_params *out);
+/* Parameters: grid U(-1e-4, 1e-4), Glorot-uniform MLP, both from `seed`. Bounds default to the unit cube. */
+int vkrt_rc_create(vkrt_ctx *ctx, const vkrt_rc_params *params, uint32_t seed, vkrt_rc **out);
+/* Before the context. */
+void vkrt_rc_destroy(vkrt_rc *rc);
+/* parameters, MLP weight count (21,504) and grid parameter count */
+int vkrt_rc_get_config(const vkrt_rc *rc, vkrt_rc_params *params, uint32_t *n_mlp, uint64_t *n_grid);
+/* World-space box mapped to the grid's unit cube (positions outside are clamped). */
+int vkrt_rc_set_bounds(vkrt_rc *rc, const float bmin[3], const float bmax[3]);
+/* out[4 i .. 4 i + 2] = max(prediction, 0) for record i; out[4 i + 3] = 0. */
+int vkrt_rc_query(vkrt_rc *rc, const float *records, uint32_t n, float *out, int flags);
+/* One optimisation step on n (record, target) pairs; *loss (nullable) = mean relative L2 before the step. */
+int vkrt_rc_train(vkrt_rc *rc, const float *records, const float *targets, uint32_t n, int flags, float *loss);
+/* fp32 master parameters (host arrays of n_mlp and n_grid floats; either may be NULL). MLP layout: W1..W5
+ * [64 out][64 in], W6 [16 out][64 in] (outputs 3..15 are kept at zero); grid: [level][entry][feature].
+ * set_params also resets the Adam moments and step count. */
+int vkrt_rc_get_params(vkrt_rc *rc, float *mlp, float *grid);
+int vkrt_rc_set_params(vkrt_rc *rc, const float *mlp, const float *grid);
+/* Gradients of the last vkrt_rc_train call made with VKRT_RC_GRAD_ONLY (same layouts as the parameters). */
+int vkrt_rc_get_gradients(vkrt_rc *rc, float *mlp, float *grid);
+/* Adam steps taken */
+uint32_t vkrt_rc_step(const vkrt_rc *rc);
+
+/* ---------------------------------------------------------------------------------------------
  * procedural geometry (host) used for the benchmark scenes
  * ------------------------------------------------------------------------------------------- */
 

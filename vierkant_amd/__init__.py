@@ -105,6 +105,14 @@ class BvhInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_uint64), ("bounds", ctypes.c_float * 6)]
 
 
+class RcParams(ctypes.Structure):
+    _fields_ = [("log2_table", ctypes.c_uint32), ("base_resolution", ctypes.c_uint32),
+                ("max_resolution", ctypes.c_uint32), ("learning_rate", ctypes.c_float), ("beta1", ctypes.c_float),
+                ("beta2", ctypes.c_float), ("epsilon", ctypes.c_float), ("reserved", ctypes.c_uint32 * 9)]
+
+
+RC_GRAD_ONLY = 4
+
 _lib = None
 _vp, _u32, _i32, _f32, _u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_uint64
 
@@ -156,6 +164,17 @@ _SIGNATURES = {
     "vkrt_pt_current_batch": (_u32, [_vp]),
     "vkrt_pt_read_pixels": (_i32, [_vp, _vp, _i32]),
     "vkrt_pt_get_stats": (_i32, [_vp, ctypes.POINTER(PtStats)]),
+    "vkrt_rc_default_params": (None, [ctypes.POINTER(RcParams)]),
+    "vkrt_rc_create": (_i32, [_vp, ctypes.POINTER(RcParams), _u32, ctypes.POINTER(_vp)]),
+    "vkrt_rc_destroy": (None, [_vp]),
+    "vkrt_rc_get_config": (_i32, [_vp, ctypes.POINTER(RcParams), ctypes.POINTER(_u32), ctypes.POINTER(_u64)]),
+    "vkrt_rc_set_bounds": (_i32, [_vp, _vp, _vp]),
+    "vkrt_rc_query": (_i32, [_vp, _vp, _u32, _vp, _i32]),
+    "vkrt_rc_train": (_i32, [_vp, _vp, _vp, _u32, _i32, ctypes.POINTER(ctypes.c_float)]),
+    "vkrt_rc_get_params": (_i32, [_vp, _vp, _vp]),
+    "vkrt_rc_set_params": (_i32, [_vp, _vp, _vp]),
+    "vkrt_rc_get_gradients": (_i32, [_vp, _vp, _vp]),
+    "vkrt_rc_step": (_u32, [_vp]),
     "vkrt_icosphere_size": (_i32, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "vkrt_icosphere": (_i32, [_f32, _u32, _vp, _vp]),
     "vkrt_plane_size": (_i32, [_u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
@@ -607,6 +626,87 @@ class PathTracer:
             self.close()
         except Exception:
             pass
+
+
+class RadianceCache:
+    """Neural radiance cache (hash grid + MLP on the matrix cores): query, train, parameters."""
+
+    RECORD_FLOATS = 16
+
+    def __init__(self, ctx, params=None, seed=0):
+        self.ctx = ctx
+        if params is None:
+            params = default_rc_params()
+        self._h = _vp()
+        ctx.check(lib().vkrt_rc_create(ctx.handle, ctypes.byref(params), seed, ctypes.byref(self._h)))
+        ctx._adopt(self)
+        p, nm, ng = RcParams(), _u32(), _u64()
+        ctx.check(lib().vkrt_rc_get_config(self._h, ctypes.byref(p), ctypes.byref(nm), ctypes.byref(ng)))
+        self.params, self.n_mlp, self.n_grid = p, nm.value, ng.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_bounds(self, bmin, bmax):
+        lo = np.ascontiguousarray(bmin, np.float32)
+        hi = np.ascontiguousarray(bmax, np.float32)
+        self.ctx.check(lib().vkrt_rc_set_bounds(self._h, lo.ctypes.data, hi.ctypes.data))
+
+    def query(self, records):
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        out = np.zeros((len(rec), 4), np.float32)
+        self.ctx.check(lib().vkrt_rc_query(self._h, rec.ctypes.data, len(rec), out.ctypes.data, 0))
+        return out[:, :3]
+
+    def train(self, records, targets, grad_only=False):
+        rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
+        tg = np.zeros((len(rec), 4), np.float32)
+        tg[:, :3] = np.asarray(targets, np.float32).reshape(len(rec), -1)[:, :3]
+        loss = ctypes.c_float()
+        self.ctx.check(lib().vkrt_rc_train(self._h, rec.ctypes.data, tg.ctypes.data, len(rec),
+                                           RC_GRAD_ONLY if grad_only else 0, ctypes.byref(loss)))
+        return loss.value
+
+    def get_params(self):
+        mlp = np.zeros(self.n_mlp, np.float32)
+        grid = np.zeros(self.n_grid, np.float32)
+        self.ctx.check(lib().vkrt_rc_get_params(self._h, mlp.ctypes.data, grid.ctypes.data))
+        return mlp, grid.reshape(16, -1, 2)
+
+    def set_params(self, mlp=None, grid=None):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        self.ctx.check(lib().vkrt_rc_set_params(self._h, None if m is None else m.ctypes.data,
+                                                None if g is None else g.ctypes.data))
+
+    def get_gradients(self):
+        mlp = np.zeros(self.n_mlp, np.float32)
+        grid = np.zeros(self.n_grid, np.float32)
+        self.ctx.check(lib().vkrt_rc_get_gradients(self._h, mlp.ctypes.data, grid.ctypes.data))
+        return mlp, grid.reshape(16, -1, 2)
+
+    def step(self):
+        return lib().vkrt_rc_step(self._h)
+
+    def close(self):
+        if self._h:
+            lib().vkrt_rc_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def default_rc_params(**kw):
+    p = RcParams()
+    lib().vkrt_rc_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
 
 
 # -- procedural geometry (host) ----------------------------------------------------------------
