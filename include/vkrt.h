@@ -433,6 +433,22 @@ int vkrt_rc_get_gradients(vkrt_rc *rc, float *mlp, float *grid);
 /* Adam steps taken */
 uint32_t vkrt_rc_step(const vkrt_rc *rc);
 
+/* Path tracing with the radiance cache (no counterpart in the reference). */
+typedef struct
+{
+    uint32_t cache_depth;       /* paths stop at this hit depth and add beta * cache (2: after two bounces) */
+    uint32_t train_stride;      /* one full-length training path per train_stride pixels per sample (32) */
+    uint32_t max_train_records; /* training records (path vertices) per sample, at most (65536) */
+    uint32_t reserved[5];
+} vkrt_pt_cache_config;
+
+void vkrt_pt_default_cache_config(vkrt_pt_cache_config *out);
+/* Attach `rc` (not owned; NULL detaches) to the tracer: every sample then queries the cache at
+ * cache_depth and trains it once on the sample's training paths. The cache's bounds are set to the
+ * scene's, padded by 1%. Results are not bitwise reproducible across runs (training records are appended
+ * in arrival order). */
+int vkrt_pt_set_radiance_cache(vkrt_pt *pt, vkrt_rc *rc, const vkrt_pt_cache_config *cfg);
+
 /* ---------------------------------------------------------------------------------------------
  * procedural geometry (host) used for the benchmark scenes
  * ------------------------------------------------------------------------------------------- */

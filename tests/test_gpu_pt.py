@@ -160,3 +160,41 @@ def test_pt_deterministic(ctx, scene3):
         pt.render(W.config3_camera())
     assert a.pixels().tobytes() == b.pixels().tobytes()
     assert a.stats().segments > 64 * 36 * 2
+
+
+def test_pt_cache_inactive_below_its_depth_is_plain_pt(ctx, scene3):
+    """with the query depth at max_trace_depth no path reaches the cache: recording training vertices and
+    training must leave every pixel bit-identical to plain path tracing"""
+    w, h = 96, 54
+    plain = vk.PathTracer(ctx, settings(w, h, 2), seed=4)
+    cached = vk.PathTracer(ctx, settings(w, h, 2), seed=4)
+    cache = vk.RadianceCache(ctx, vk.default_rc_params(log2_table=16), seed=1)
+    for pt in (plain, cached):
+        pt.set_scene(scene3)
+    cached.set_radiance_cache(cache, cache_depth=8, train_stride=4)
+    for pt in (plain, cached):
+        pt.render(W.config3_camera())
+    assert plain.pixels().tobytes() == cached.pixels().tobytes()
+    assert cache.step() == 2                        # one training step per sample
+    cache.close()
+
+
+def test_pt_cache_converges_near_reference(ctx, scene3):
+    """terminating into the trained cache after two bounces: the accumulated image's mean radiance stays
+    within a few percent of plain path tracing with many samples (the cache's bias), and is finite"""
+    w, h = 128, 72
+    ref = vk.PathTracer(ctx, settings(w, h, 256), seed=8)
+    ref.set_scene(scene3)
+    ref.render(W.config3_camera())
+    want = ref.pixels()["radiance"]
+    pt = vk.PathTracer(ctx, settings(w, h, 1, batches=256), seed=9)
+    pt.set_scene(scene3)
+    cache = vk.RadianceCache(ctx, seed=3)
+    pt.set_radiance_cache(cache, cache_depth=2, train_stride=8)
+    for _ in range(192):
+        pt.render(W.config3_camera())
+    got = pt.pixels()["radiance"]
+    assert np.isfinite(got).all() and (got >= 0).all()
+    rel = abs(float(got.mean()) - float(want.mean())) / float(want.mean())
+    assert rel < 0.05, rel
+    cache.close()

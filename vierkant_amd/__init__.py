@@ -113,6 +113,11 @@ class RcParams(ctypes.Structure):
 
 RC_GRAD_ONLY = 4
 
+
+class PtCacheConfig(ctypes.Structure):
+    _fields_ = [("cache_depth", ctypes.c_uint32), ("train_stride", ctypes.c_uint32),
+                ("max_train_records", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 5)]
+
 _lib = None
 _vp, _u32, _i32, _f32, _u64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float, ctypes.c_uint64
 
@@ -175,6 +180,8 @@ _SIGNATURES = {
     "vkrt_rc_set_params": (_i32, [_vp, _vp, _vp]),
     "vkrt_rc_get_gradients": (_i32, [_vp, _vp, _vp]),
     "vkrt_rc_step": (_u32, [_vp]),
+    "vkrt_pt_default_cache_config": (None, [ctypes.POINTER(PtCacheConfig)]),
+    "vkrt_pt_set_radiance_cache": (_i32, [_vp, _vp, ctypes.POINTER(PtCacheConfig)]),
     "vkrt_icosphere_size": (_i32, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "vkrt_icosphere": (_i32, [_f32, _u32, _vp, _vp]),
     "vkrt_plane_size": (_i32, [_u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
@@ -610,6 +617,16 @@ class PathTracer:
         out = np.zeros(s.resolution[0] * s.resolution[1], dtype=PIXEL_DTYPE)
         self.ctx.check(lib().vkrt_pt_read_pixels(self._h, out.ctypes.data, 0))
         return out.reshape(s.resolution[1], s.resolution[0])
+
+    def set_radiance_cache(self, cache, **cfg):
+        """attach a RadianceCache (None detaches); cfg: cache_depth, train_stride, max_train_records"""
+        c = PtCacheConfig()
+        lib().vkrt_pt_default_cache_config(ctypes.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.ctx.check(lib().vkrt_pt_set_radiance_cache(self._h, None if cache is None else cache.handle,
+                                                        ctypes.byref(c)))
+        self._cache = cache
 
     def stats(self):
         st = PtStats()

@@ -24,6 +24,7 @@
 
 #include "bvh_types.hpp"
 #include "context.hpp"
+#include "radiance_cache.hpp"
 #include "trace.hpp"
 #include "traverse.hpp"
 #include "vierkant/pt_math.hpp"
@@ -36,7 +37,9 @@ namespace vkrt
 
 constexpr uint32_t MEDIA_STACK = 4;          // MAX_MEDIA_STACK_SIZE (raypipeline.slang:21)
 constexpr uint32_t NO_OBJECT = 0xFFFFu;      // utils::MAX_UINT16
-constexpr uint32_t F_MEDIA_MASK = 3u, F_REGULARIZE = 4u, F_CAUSTIC = 8u, F_FIRST_MISS = 16u;
+constexpr uint32_t F_MEDIA_MASK = 3u, F_REGULARIZE = 4u, F_CAUSTIC = 8u, F_FIRST_MISS = 16u, F_TRAIN = 32u;
+constexpr uint32_t QUERY_F4 = 5;   // cache query record: 4 float4 of features + (pixel, beta)
+constexpr uint32_t VERTEX_F4 = 6;  // training vertex: 4 float4 of features + (emission, kind) + (weight, -)
 constexpr int PT_BLOCK = 256;
 constexpr int PT_LDS = 16;
 
@@ -76,6 +79,9 @@ struct pt_frame
     uint32_t num_samples, max_depth;
     uint32_t seed, batch_index, disable_material;
     float max_path_beta, environment;
+    // radiance cache: paths that are not training paths stop at hit depth cache_depth and query the cache;
+    // every train_stride-th pixel (from train_offset) traces a full training path
+    uint32_t cache_on, cache_depth, train_stride, train_offset, n_train_paths, train_max;
     vec3 eye, right, up, forward;
     float tan_half_x, tan_half_y, aperture, focal_distance;
 };
@@ -90,8 +96,12 @@ struct pt_buffers
     float4 *acc;            // sum of the batch's sample radiance (rgb), coverage
     uint4 *info;            // first-hit depth bits, normal, object id, albedo
     uint32_t *queue[2];
-    uint32_t *counter;      // [0], [1]: queue sizes; [2..3]: segment count (64-bit)
+    uint32_t *counter;      // [0], [1]: queue sizes; [2..3]: segment count (64-bit); [4]: cache queries;
+                            // [5]: training records
     vkrt_pixel *pixels;
+    float4 *qrec;           // cache queries (QUERY_F4 float4 each)
+    float4 *tvtx;           // training paths' vertices [path][depth] (VERTEX_F4 float4 each)
+    float4 *trec, *ttgt;    // training records (4 float4) and targets
 };
 
 struct pt_scene_dev
@@ -109,7 +119,12 @@ __device__ __forceinline__ float frac(float x) { return x - floorf(x); }
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b, uint32_t sample)
 {
     const uint32_t p = blockIdx.x * PT_BLOCK + threadIdx.x;
-    if(p == 0) b.counter[0] = f.n_pix;
+    if(p == 0)
+    {
+        b.counter[0] = f.n_pix;
+        b.counter[4] = 0u;
+        b.counter[5] = 0u;
+    }
     if(p >= f.n_pix) return;
     const uint32_t x = p % f.width, y = p / f.width;
     uint4 st = b.state[p];
@@ -134,6 +149,16 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
     xi.y = rnd(rng);
     st.x = rng;
     st.y = 0u;// media index 0, no regularization, no caustic marker
+    if(f.cache_on && p % f.train_stride == f.train_offset)
+    {
+        st.y = F_TRAIN;
+        float4 *tv = b.tvtx + (size_t)(p / f.train_stride) * f.max_depth * VERTEX_F4;
+        for(uint32_t d = 0; d < f.max_depth; ++d)
+        {
+            tv[d * VERTEX_F4 + 4] = make_float4(0.f, 0.f, 0.f, 0.f);
+            tv[d * VERTEX_F4 + 5] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
     b.state[p] = st;
 
     // camera_ray (raypipeline.slang:58-89), perspective branch, in the camera's own basis
@@ -183,6 +208,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
     const uint32_t count = b.counter[in];
     const bool active = q < count;
     bool alive = false;
+    bool to_cache = false;
+    float4 qfeat[QUERY_F4];
     uint32_t p = 0;
     if(active)
     {
@@ -202,10 +229,16 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
         uint32_t media_index = flags & F_MEDIA_MASK;
         uint4 h = b.hit[p];
         bool stop = false;
+        to_cache = false;
         uint32_t object = NO_OBJECT;
         vec3 position{0.f, 0.f, 0.f}, normal{0.f, 0.f, 0.f}, albedo{0.f, 0.f, 0.f};
         uint32_t media_op = 0;// 0 NoOp, 1 Enter, 2 Leave
         float next_ior = 1.f;
+        const bool train = (flags & F_TRAIN) != 0u;
+        // training vertex: kind 1 = surface (features, emission, throughput weight), 2 = miss (sky radiance)
+        uint32_t vkind = 0;
+        vec3 v_e{0.f, 0.f, 0.f}, v_w{0.f, 0.f, 0.f};
+        float4 feat[4];
 
         if(h.w == VKRT_INVALID_PRIM)
         {
@@ -213,6 +246,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             stop = true;
             vec3 col = environment_white(dir);
             radiance = radiance + mul3(beta * f.environment, col);
+            vkind = 2;
+            v_e = col * f.environment;
         }
         else
         {
@@ -250,6 +285,18 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             m.sheen_color = xyz(gm.sheen);
             m.diffuse_transmission_color = xyz(gm.dtc);
 
+            feat[0] = make_float4(position.x, position.y, position.z, m.roughness);
+            feat[1] = make_float4(V.x, V.y, V.z, m.metalness);
+            feat[2] = make_float4(normal.x, normal.y, normal.z, m.transmission);
+            feat[3] = make_float4(m.color.x, m.color.y, m.color.z, 0.f);
+            if(f.cache_on && !train && depth == f.cache_depth)
+            {
+                // terminate into the cache: the query kernel adds beta * L_o(x, wo), emission included
+                stop = true;
+                to_cache = true;
+            }
+            else
+            {
             const float media_ior = ior_stack[media_index];
             const float last_ior = ior_stack[media_index > 0u ? media_index - 1u : 0u];
             float eta = backface ? m.ior / last_ior : media_ior / m.ior;
@@ -262,6 +309,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             const vec3 ff = dot(dir, sn) < 0.f ? sn : -sn;// faceforward(sn, ray dir, sn)
             vec3 emission = m.emission * (dot(sn, ff) > 0.f ? m.emission_strength : 0.0f);
             radiance = radiance + mul3(beta, emission);
+            vkind = 1;
+            v_e = emission;
             if(flags & F_REGULARIZE) m.roughness = fmaxf(m.roughness, REGULARIZE_MIN_ROUGHNESS);
             bsdf_sample bs = sample_disney(m, sn, V, eta, rng);
             if(!(bs.pdf > 0.0f)) { stop = true; }
@@ -280,20 +329,48 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
                     thr = vec3{fminf(thr.x, f.max_path_beta), fminf(thr.y, f.max_path_beta),
                                fminf(thr.z, f.max_path_beta)};
                 beta = mul3(beta, thr);
+                v_w = thr;
                 if(bs.transmission) rd.w = media_op == 1u ? tmax_inside : DEFAULT_TMAX;
                 media_op = bs.transmission ? media_op : 0u;
                 // Russian roulette (raypipeline.slang:667-677)
                 if(max3(beta) <= 0.05f * static_cast<float>(depth))
                 {
                     float qq = 1.0f - max3(beta);
-                    if(rnd(rng) < qq) stop = true;
+                    if(rnd(rng) < qq)
+                    {
+                        stop = true;
+                        v_w = vec3{0.f, 0.f, 0.f};
+                    }
                     else
                     {
                         const float keep = 1.0f - qq;
                         beta = vec3{beta.x / keep, beta.y / keep, beta.z / keep};
+                        v_w = vec3{v_w.x / keep, v_w.y / keep, v_w.z / keep};
                     }
                 }
             }
+            }
+        }
+        if(train && vkind != 0u)
+        {
+            float4 *tv = b.tvtx + ((size_t)(p / f.train_stride) * f.max_depth + depth) * VERTEX_F4;
+            if(vkind == 1u)
+            {
+                tv[0] = feat[0];
+                tv[1] = feat[1];
+                tv[2] = feat[2];
+                tv[3] = feat[3];
+            }
+            tv[4] = make_float4(v_e.x, v_e.y, v_e.z, __uint_as_float(vkind));
+            tv[5] = make_float4(v_w.x, v_w.y, v_w.z, 0.f);
+        }
+        if(to_cache)
+        {
+            qfeat[0] = feat[0];
+            qfeat[1] = feat[1];
+            qfeat[2] = feat[2];
+            qfeat[3] = feat[3];
+            qfeat[4] = make_float4(__uint_as_float(p), beta.x, beta.y, beta.z);
         }
 
         // media stack update in the raygen loop (raypipeline.slang:195-200)
@@ -348,6 +425,47 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
     {
         const uint64_t below = lane == 0 ? 0ull : (mask & (~0ull >> (64 - lane)));
         b.queue[in ^ 1][base + __popcll(below)] = p;
+    }
+    // cache queries: one ballot and one atomic per wave
+    const uint64_t qmask = __ballot(to_cache);
+    if(qmask)
+    {
+        uint32_t qbase = 0;
+        if(lane == 0) qbase = atomicAdd(&b.counter[4], static_cast<uint32_t>(__popcll(qmask)));
+        qbase = __shfl(qbase, 0);
+        if(to_cache)
+        {
+            const uint64_t below = lane == 0 ? 0ull : (qmask & (~0ull >> (64 - lane)));
+            float4 *dst = b.qrec + (size_t)(qbase + __popcll(below)) * QUERY_F4;
+            for(uint32_t k = 0; k < QUERY_F4; ++k) dst[k] = qfeat[k];
+        }
+    }
+}
+
+//! training targets: L_d = e_d + w_d * L_{d+1} from the last vertex back (a miss slot holds the sky's
+//! radiance); every surface vertex becomes a training record, at most train_max per sample
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_train_targets(pt_frame f, pt_buffers b)
+{
+    const uint32_t t = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(t >= f.n_train_paths) return;
+    if((size_t)t * f.train_stride + f.train_offset >= f.n_pix) return;
+    const float4 *tv = b.tvtx + (size_t)t * f.max_depth * VERTEX_F4;
+    vec3 L{0.f, 0.f, 0.f};
+    for(int d = (int)f.max_depth - 1; d >= 0; --d)
+    {
+        const float4 e = tv[d * VERTEX_F4 + 4], w = tv[d * VERTEX_F4 + 5];
+        const uint32_t kind = __float_as_uint(e.w);
+        if(kind == 0u) continue;
+        L = vec3{e.x + w.x * L.x, e.y + w.y * L.y, e.z + w.z * L.z};
+        if(kind != 1u) continue;
+        const uint32_t k = atomicAdd(&b.counter[5], 1u);
+        if(k >= f.train_max) continue;
+        float4 *r = b.trec + (size_t)k * 4;
+        r[0] = tv[d * VERTEX_F4 + 0];
+        r[1] = tv[d * VERTEX_F4 + 1];
+        r[2] = tv[d * VERTEX_F4 + 2];
+        r[3] = tv[d * VERTEX_F4 + 3];
+        b.ttgt[k] = make_float4(L.x, L.y, L.z, 0.f);
     }
 }
 
@@ -414,6 +532,10 @@ struct vkrt_pt
     vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     vkrt_pt_stats stats{};
+    // radiance cache (not owned) and its per-sample buffers
+    vkrt_rc *rc = nullptr;
+    vkrt_pt_cache_config cache_cfg{};
+    vkrt::device_buffer qrec, tvtx, trec, ttgt;
 };
 
 namespace vkrt
@@ -523,7 +645,31 @@ static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     f.tan_half_x = (float)(th * (double)pt->width / (double)pt->height);
     f.aperture = cam.aperture;
     f.focal_distance = cam.focal_distance > 0.f ? cam.focal_distance : 1.f;
+    f.cache_on = pt->rc ? 1u : 0u;
+    f.cache_depth = pt->cache_cfg.cache_depth;
+    f.train_stride = std::max<uint32_t>(1u, pt->cache_cfg.train_stride);
+    f.train_offset = 0;
+    f.n_train_paths = div_up(f.n_pix, f.train_stride);
+    f.train_max = pt->cache_cfg.max_train_records;
     return f;
+}
+
+//! the cache's grid covers the scene bounds, padded by 1% per side
+static void fit_cache_bounds(vkrt_pt *pt)
+{
+    if(!pt->rc || !pt->bvh) return;
+    const float *bb = pt->bvh->info.bounds;
+    float lo[3], hi[3];
+    for(int a = 0; a < 3; ++a)
+    {
+        const bool valid = bb[a + 3] >= bb[a];
+        const float e = valid ? bb[a + 3] - bb[a] : 0.f;
+        const float pad = std::max(0.01f * e, 1e-3f);
+        lo[a] = valid ? bb[a] - pad : -1.f;
+        hi[a] = valid ? bb[a + 3] + pad : 1.f;
+    }
+    const int rc = vkrt_rc_set_bounds(pt->rc, lo, hi);
+    if(rc != VKRT_OK) throw error(rc, "radiance cache bounds: " + pt->ctx->last_error);
 }
 
 static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
@@ -535,10 +681,27 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
                     pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(), pt->mats.as<gpu_material>()};
     const uint32_t blocks = div_up(f.n_pix, PT_BLOCK);
     const vkrt_bvh *bvh = pt->bvh;
+    if(f.cache_on)
+    {
+        pt->qrec.reserve((size_t)f.n_pix * QUERY_F4 * 16);
+        pt->tvtx.reserve((size_t)f.n_train_paths * f.max_depth * VERTEX_F4 * 16);
+        pt->trec.reserve((size_t)std::max(1u, f.train_max) * 64);
+        pt->ttgt.reserve((size_t)std::max(1u, f.train_max) * 16);
+        b.qrec = pt->qrec.as<float4>();
+        b.tvtx = pt->tvtx.as<float4>();
+        b.trec = pt->trec.as<float4>();
+        b.ttgt = pt->ttgt.as<float4>();
+    }
     VKRT_HIP(hipEventRecord(pt->ev0, s));
     k_pt_zero<<<1, 4, 0, s>>>(b.counter);
     for(uint32_t sample = 0; sample < f.num_samples; ++sample)
     {
+        if(f.cache_on)
+        {
+            // training pixels move every sample: offset from the batch seed and the sample index
+            f.train_offset = xxhash32(seed, sample) % f.train_stride;
+            f.n_train_paths = f.train_offset < f.n_pix ? div_up(f.n_pix - f.train_offset, f.train_stride) : 0u;
+        }
         k_pt_begin<<<blocks, PT_BLOCK, 0, s>>>(f, b, sample);
         int in = 0;
         for(uint32_t depth = 0; depth < f.max_depth; ++depth)
@@ -548,6 +711,15 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
                                                    bvh->empty, b, in);
             k_pt_shade<<<blocks, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
             in ^= 1;
+        }
+        if(f.cache_on)
+        {
+            rc::query(pt->rc, b.qrec, QUERY_F4, f.n_pix, nullptr, b.acc, b.counter + 4);
+            if(f.n_train_paths && f.train_max)
+            {
+                k_pt_train_targets<<<div_up(f.n_train_paths, PT_BLOCK), PT_BLOCK, 0, s>>>(f, b);
+                rc::train(pt->rc, b.trec, 4, b.ttgt, f.train_max, false, b.counter + 5);
+            }
         }
     }
     k_pt_resolve<<<blocks, PT_BLOCK, 0, s>>>(f, b);
@@ -705,6 +877,33 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         pt->n_objects = sc->n_objects;
         pt->n_materials = sc->n_materials;
         pt->environment = sc->environment;
+        fit_cache_bounds(pt);
+    });
+}
+
+void vkrt_pt_default_cache_config(vkrt_pt_cache_config *out)
+{
+    if(!out) return;
+    *out = {};
+    out->cache_depth = 2;
+    out->train_stride = 32;
+    out->max_train_records = 65536;
+}
+
+int vkrt_pt_set_radiance_cache(vkrt_pt *pt, vkrt_rc *rc, const vkrt_pt_cache_config *cfg)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        vkrt_pt_cache_config c{};
+        if(cfg) c = *cfg;
+        else vkrt_pt_default_cache_config(&c);
+        VKRT_REQUIRE(c.cache_depth >= 1 && c.cache_depth <= 64, "cache_depth must be in 1..64");
+        VKRT_REQUIRE(c.train_stride >= 1 && c.train_stride <= (1u << 20), "train_stride must be in 1..2^20");
+        VKRT_REQUIRE(c.max_train_records <= (1u << 22), "max_train_records must be at most 2^22");
+        VKRT_REQUIRE(!rc || rc->ctx == pt->ctx, "the cache belongs to another context");
+        pt->rc = rc;
+        pt->cache_cfg = c;
+        fit_cache_bounds(pt);
     });
 }
 

@@ -85,9 +85,12 @@ __global__ void __launch_bounds__(QUERY_BLOCK) k_rc_query(cache_dev c, const flo
 
 __global__ void __launch_bounds__(TRAIN_BLOCK) k_rc_train(cache_dev c, const float4 *__restrict__ records,
                                                           uint32_t stride_f4, const float4 *__restrict__ targets,
-                                                          uint32_t n, float *__restrict__ grid_grad,
-                                                          float *__restrict__ partials, float *__restrict__ loss_sum)
+                                                          uint32_t n_cap, const uint32_t *__restrict__ n_dev,
+                                                          float *__restrict__ grid_grad, float *__restrict__ partials,
+                                                          float *__restrict__ loss_sum)
 {
+    // sample count: n_cap, or the device counter clamped to n_cap
+    const uint32_t n = n_dev ? min(*n_dev, n_cap) : n_cap;
     __shared__ __bf16 act[N_LAYERS][WIDTH][LDS_ROW];// a_0 = x0, a_i = h_i (i = 1..5), [feature][sample]
     __shared__ __bf16 dzs[WIDTH][LDS_ROW];          // the current layer's output gradient, [feature][sample]
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, group = lane >> 4, col = lane & 15u;
@@ -373,7 +376,8 @@ static uint32_t train_blocks(vkrt_rc *c, uint32_t n)
     return std::max<uint32_t>(1u, std::min<uint32_t>(iterations, 2u * (uint32_t)c->ctx->cu_count));
 }
 
-void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only)
+void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only,
+           const uint32_t *n_dev)
 {
     hipStream_t s = c->ctx->stream;
     const uint32_t nb = train_blocks(c, n);
@@ -381,8 +385,8 @@ void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *
     fill(s, c->grid_grad.as<float>(), c->n_grid, 0.0f);
     fill(s, c->loss.as<float>(), 1, 0.0f);
     if(n)
-        k_rc_train<<<nb, TRAIN_BLOCK, 0, s>>>(device_view(c), records, stride_f4, targets, n, c->grid_grad.as<float>(),
-                                              c->partials.as<float>(), c->loss.as<float>());
+        k_rc_train<<<nb, TRAIN_BLOCK, 0, s>>>(device_view(c), records, stride_f4, targets, n, n_dev,
+                                              c->grid_grad.as<float>(), c->partials.as<float>(), c->loss.as<float>());
     else
         fill(s, c->partials.as<float>(), (uint64_t)nb * N_MLP, 0.0f);
     adam_consts a{};
@@ -601,7 +605,7 @@ int vkrt_rc_train(vkrt_rc *c, const float *records, const float *targets, uint32
         VKRT_REQUIRE(n == 0 || (records && targets), "null records or targets");
         const float4 *rec = reinterpret_cast<const float4 *>(input(ctx, 0, records, (size_t)n * 16, flags));
         const float4 *tg = reinterpret_cast<const float4 *>(input(ctx, 1, targets, (size_t)n * 4, flags));
-        rc::train(c, rec, RECORD_F4, tg, n, (flags & VKRT_RC_GRAD_ONLY) != 0);
+        rc::train(c, rec, RECORD_F4, tg, n, (flags & VKRT_RC_GRAD_ONLY) != 0, nullptr);
         if(loss)
         {
             float sum = 0.f;

@@ -290,8 +290,10 @@ inline cache_dev device_view(const vkrt_rc *c)
 }
 
 //! one training step over `n` device records (RECORD_F4 float4 each, `stride_f4` apart) with targets
-//! (float4 rgb, -): gradients, then Adam unless grad_only
-void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only);
+//! (float4 rgb, -): gradients, then Adam unless grad_only. With n_dev the count is min(*n_dev, n), read on
+//! the device (no host synchronization).
+void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only,
+           const uint32_t *n_dev);
 
 //! predictions max(y_rgb, 0): written to out[i] (float4), or, when pixel_acc is set, added as
 //! beta * prediction to pixel_acc[pixel] with pixel and beta taken from the record's fifth float4
