@@ -679,7 +679,8 @@ class RadianceCache:
     def train(self, records, targets, grad_only=False):
         rec = np.ascontiguousarray(records, np.float32).reshape(-1, 16)
         tg = np.zeros((len(rec), 4), np.float32)
-        tg[:, :3] = np.asarray(targets, np.float32).reshape(len(rec), -1)[:, :3]
+        if len(rec):
+            tg[:, :3] = np.asarray(targets, np.float32).reshape(len(rec), -1)[:, :3]
         loss = ctypes.c_float()
         self.ctx.check(lib().vkrt_rc_train(self._h, rec.ctypes.data, tg.ctypes.data, len(rec),
                                            RC_GRAD_ONLY if grad_only else 0, ctypes.byref(loss)))
