@@ -99,6 +99,85 @@ def path_tracer_frame(ctx, spp):
     return out
 
 
+def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64, max_train_records=16384):
+    """time to equal RMSE on the room scene at 1920x1080, max_trace_depth 8. The target is plain path
+    tracing at 64 spp, timed as one 64-spp batch (its fastest form). The reference image is plain path
+    tracing at ref_spp. Cache frames trace 1 spp: paths that are not training paths stop at hit depth
+    `cache_depth` (after two bounces) and add beta * cache; one pixel in `train_stride` traces a
+    full-length training path, and the cache trains once per frame on at most `max_train_records` of their
+    vertices. Frame times are GPU event times
+    (tracing, cache queries and training); image read-back and RMSE are outside them."""
+    import numpy as np
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+    scene = W.config3_scene()
+    cam = W.config3_camera()
+
+    def settings(spp, batches):
+        st = vk.default_settings()
+        st.resolution[0], st.resolution[1] = 1920, 1080
+        st.num_samples, st.max_trace_depth, st.max_num_batches = spp, 8, batches
+        return st
+
+    t0 = time.perf_counter()
+    ref_pt = vk.PathTracer(ctx, settings(64, max(1, ref_spp // 64)), seed=1000)
+    ref_pt.set_scene(scene)
+    for _ in range(max(1, ref_spp // 64)):
+        ref_pt.render(cam)
+    ref = ref_pt.pixels()["radiance"].astype(np.float64)
+    ref_pt.close()
+    ref_s = time.perf_counter() - t0
+
+    def rmse(img):
+        return float(np.sqrt(np.mean((img.astype(np.float64) - ref) ** 2)))
+
+    plain = vk.PathTracer(ctx, settings(64, 0), seed=0)
+    plain.set_scene(scene)
+    plain.render(cam)
+    plain.render(cam)
+    target = rmse(plain.pixels()["radiance"])
+    plain_ms = float(plain.stats().raytrace_ms)
+    plain.close()
+
+    # code objects of the cache kernels load on their first launch: warm them on a throwaway tracer
+    warm = vk.PathTracer(ctx, settings(1, 0), seed=0)
+    warm.set_scene(scene)
+    warm_cache = vk.RadianceCache(ctx, seed=7)
+    warm.set_radiance_cache(warm_cache, cache_depth=cache_depth, train_stride=train_stride,
+                            max_train_records=max_train_records)
+    warm.render(cam)
+    warm.close()
+    warm_cache.close()
+
+    pt = vk.PathTracer(ctx, settings(1, 1 << 20), seed=0)
+    pt.set_scene(scene)
+    cache = vk.RadianceCache(ctx, seed=1)
+    pt.set_radiance_cache(cache, cache_depth=cache_depth, train_stride=train_stride,
+                          max_train_records=max_train_records)
+    t_ms, hit, curve, frame_ms = 0.0, None, [], []
+    for k in range(1, max_frames + 1):
+        pt.render(cam)
+        frame_ms.append(float(pt.stats().raytrace_ms))
+        t_ms += frame_ms[-1]
+        r = rmse(pt.pixels()["radiance"])
+        if k in (1, 2, 4, 8, 16, 32, 64, 128) or (hit is None and r <= target):
+            curve.append([k, round(t_ms, 2), round(r, 5)])
+        if hit is None and r <= target:
+            hit = (k, t_ms)
+            break
+    pt.close()
+    cache.close()
+    return {"workload": "room scene (276,504 triangles), 1920x1080, max_trace_depth 8; target = RMSE of plain "
+                        f"path tracing at 64 spp against a {ref_spp}-spp plain reference",
+            "target_rmse": round(target, 5), "plain_64spp_ms": round(plain_ms, 3),
+            "cache_depth": cache_depth, "train_stride": train_stride, "max_train_records": max_train_records,
+            "cache_frames_to_target": None if hit is None else hit[0],
+            "cache_ms_to_target": None if hit is None else round(hit[1], 3),
+            "speedup": None if hit is None else round(plain_ms / hit[1], 3),
+            "cache_frame_ms_mean": round(float(np.mean(frame_ms)), 3),
+            "rmse_curve": curve, "reference_s": round(ref_s, 2)}
+
+
 def batched_frame(ctx, n_scenes, verify_rays):
     """256 independent scenes (log-uniform 1k-500k triangles, 16k-2M rays each) traced by one persistent
     launch replayed as a hipGraph, after the launch-shape autotuner picked block size, LDS stack depth,
@@ -163,6 +242,9 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
     ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
+    ap.add_argument("--rc-ref-spp", type=int, default=4096,
+                    help="samples per pixel of the radiance-cache leg's reference image (0: skip the leg)")
+    ap.add_argument("--rc-frames", type=int, default=128, help="most cache frames the radiance-cache leg traces")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -288,7 +370,7 @@ def main():
     traffic, traffic_source = profiled_traffic("k_trace_closest")
     achieved = CLOSEST_BYTES_PER_RAY * len(primary) / (launch_ms * 1e-3) / 1e9
     result = {
-        "metric": "Grays/s closest-hit (1 GPU)",
+        "metric": "Grays/s closest-hit (1 GPU); time-to-equal-RMSE: radiance-cache vs plain PT",
         "value": round(value, 4),
         "unit": "Grays/s",
         "n_gpus": 1,
@@ -341,6 +423,9 @@ def main():
     if args.batch_scenes:
         log("batched scenes")
         result["config"]["batched"] = batched_frame(ctx, args.batch_scenes, 4096)
+    if args.rc_ref_spp:
+        log("radiance cache: time to equal RMSE")
+        result["config"]["radiance_cache"] = radiance_cache_run(ctx, args.rc_ref_spp, args.rc_frames)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         allrays = np.concatenate([primary, bounce])

@@ -437,8 +437,8 @@ uint32_t vkrt_rc_step(const vkrt_rc *rc);
 typedef struct
 {
     uint32_t cache_depth;       /* paths stop at this hit depth and add beta * cache (2: after two bounces) */
-    uint32_t train_stride;      /* one full-length training path per train_stride pixels per sample (32) */
-    uint32_t max_train_records; /* training records (path vertices) per sample, at most (65536) */
+    uint32_t train_stride;      /* one full-length training path per train_stride pixels per sample (64) */
+    uint32_t max_train_records; /* training records (path vertices) per sample, at most (16384) */
     uint32_t reserved[5];
 } vkrt_pt_cache_config;
 

@@ -84,9 +84,11 @@ def test_gradients_match_oracle(cache):
         assert np.abs(a - b).max() <= 2e-2 * s, f"layer at {off}"
         off += o * i
     assert np.all(R.split_weights(g_mlp)[5][3:] == 0)
-    # the same grid entries are touched; values agree up to the accumulated rounding
-    assert np.array_equal(g_grid != 0, w_grid != 0)
+    # grid gradients accumulate in fp16 (loss-scaled): only entries the oracle touches are non-zero, every
+    # entry with a gradient above fp16 resolution is touched, and values agree to fp16 accumulation error
     s = float(np.abs(w_grid).max())
+    assert not np.any((g_grid != 0) & (w_grid == 0))
+    assert np.all(g_grid[np.abs(w_grid) > 1e-3 * s] != 0)
     assert np.abs(g_grid - w_grid).max() <= 2e-2 * s
 
 

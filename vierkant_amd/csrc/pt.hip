@@ -706,10 +706,13 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         int in = 0;
         for(uint32_t depth = 0; depth < f.max_depth; ++depth)
         {
+            // past the cache depth only training paths are alive: at most n_train_paths queue entries
+            const uint32_t grid = (f.cache_on && depth > f.cache_depth) ? div_up(std::max(1u, f.n_train_paths), PT_BLOCK)
+                                                                          : blocks;
             k_pt_zero<<<1, 1, 0, s>>>(b.counter + (in ^ 1));
-            k_pt_trace<<<blocks, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref,
-                                                   bvh->empty, b, in);
-            k_pt_shade<<<blocks, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
+            k_pt_trace<<<grid, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref,
+                                                 bvh->empty, b, in);
+            k_pt_shade<<<grid, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
             in ^= 1;
         }
         if(f.cache_on)
@@ -886,8 +889,8 @@ void vkrt_pt_default_cache_config(vkrt_pt_cache_config *out)
     if(!out) return;
     *out = {};
     out->cache_depth = 2;
-    out->train_stride = 32;
-    out->max_train_records = 65536;
+    out->train_stride = 64;
+    out->max_train_records = 16384;
 }
 
 int vkrt_pt_set_radiance_cache(vkrt_pt *pt, vkrt_rc *rc, const vkrt_pt_cache_config *cfg)

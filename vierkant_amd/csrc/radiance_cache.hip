@@ -3,7 +3,8 @@
 //   k_rc_query    per wave 4 x 16 samples: hash-grid + SH encoding in registers, 6 MFMA layers, output
 //   k_rc_train    per block 64 samples per iteration: forward (activations to LDS), relative L2 loss,
 //                 back-propagation, weight gradients contracted over the samples in LDS, hash-grid gradient
-//                 scattered with float atomics; per-block weight gradients written to a partial buffer
+//                 scattered with packed fp16x2 atomics (one per corner, loss-scaled by a power of two);
+//                 per-block weight gradients written to a partial buffer
 //   k_rc_adam_mlp per weight: sum of the block partials in block order (deterministic), Adam, bf16 swizzles
 //   k_rc_adam_grid per grid parameter: Adam where the gradient is non-zero, fp16 copy for the encoder
 #include <algorithm>
@@ -86,8 +87,8 @@ __global__ void __launch_bounds__(QUERY_BLOCK) k_rc_query(cache_dev c, const flo
 __global__ void __launch_bounds__(TRAIN_BLOCK) k_rc_train(cache_dev c, const float4 *__restrict__ records,
                                                           uint32_t stride_f4, const float4 *__restrict__ targets,
                                                           uint32_t n_cap, const uint32_t *__restrict__ n_dev,
-                                                          float *__restrict__ grid_grad, float *__restrict__ partials,
-                                                          float *__restrict__ loss_sum)
+                                                          half2v *__restrict__ grid_grad, float grad_scale,
+                                                          float *__restrict__ partials, float *__restrict__ loss_sum)
 {
     // sample count: n_cap, or the device counter clamped to n_cap
     const uint32_t n = n_dev ? min(*n_dev, n_cap) : n_cap;
@@ -246,12 +247,12 @@ __global__ void __launch_bounds__(TRAIN_BLOCK) k_rc_train(cache_dev c, const flo
                     uint32_t idx[8];
                     float w[8];
                     level_corners(c.g, level, p, idx, w);
+                    const float s0 = g0 * grad_scale, s1 = g1 * grad_scale;
 #pragma unroll
                     for(int k = 0; k < 8; ++k)
                     {
-                        float *dst = grid_grad + 2 * (level * table + idx[k]);
-                        atomicAdd(dst, w[k] * g0);
-                        atomicAdd(dst + 1, w[k] * g1);
+                        const half2v val = {static_cast<_Float16>(w[k] * s0), static_cast<_Float16>(w[k] * s1)};
+                        __builtin_amdgcn_global_atomic_fadd_v2f16(grid_grad + (level * table + idx[k]), val);
                     }
                 }
         }
@@ -307,8 +308,16 @@ __global__ void k_rc_adam_mlp(float *__restrict__ master, float *__restrict__ mo
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if(k >= N_MLP) return;
-    float g = 0.0f;
-    for(uint32_t b = 0; b < n_blocks; ++b) g = g + partials[(size_t)b * N_MLP + k];
+    // eight independent partial sums (blocks b = j mod 8), combined in a fixed order: deterministic
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    uint32_t b = 0;
+    for(; b + 8 <= n_blocks; b += 8)
+#pragma unroll
+        for(int j = 0; j < 8; ++j) acc[j] = acc[j] + partials[(size_t)(b + j) * N_MLP + k];
+#pragma unroll
+    for(int j = 0; j < 8; ++j)
+        if(b + j < n_blocks) acc[j] = acc[j] + partials[(size_t)(b + j) * N_MLP + k];
+    const float g = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     grad_out[k] = g;
     if(!apply || g == 0.0f) return;
     float p = master[k], m = mom[k], v = vel[k];
@@ -326,12 +335,14 @@ __global__ void k_rc_publish_mlp(const float *__restrict__ master, __bf16 *__res
 }
 
 __global__ void k_rc_adam_grid(float *__restrict__ master, float *__restrict__ mom, float *__restrict__ vel,
-                               const float *__restrict__ grad, __half *__restrict__ stored, uint64_t n, adam_consts a)
+                               const _Float16 *__restrict__ grad, float inv_scale, __half *__restrict__ stored,
+                               uint64_t n, adam_consts a)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(k >= n) return;
-    const float g = grad[k];
-    if(g == 0.0f) return;// untouched entries keep their parameters and moments
+    const float g = static_cast<float>(grad[k]) * inv_scale;
+    // untouched entries keep their parameters and moments; a non-finite (overflowed) gradient is dropped
+    if(g == 0.0f || !isfinite(g)) return;
     float p = master[k], m = mom[k], v = vel[k];
     adam_update(p, m, v, g, a);
     master[k] = p;
@@ -372,8 +383,23 @@ static void fill(hipStream_t s, float *p, uint64_t n, float value)
 
 static uint32_t train_blocks(vkrt_rc *c, uint32_t n)
 {
+    // one 4-wave block per CU (the kernel runs one wave per SIMD)
     const uint32_t iterations = div_up(n, TRAIN_SAMPLES);
-    return std::max<uint32_t>(1u, std::min<uint32_t>(iterations, 2u * (uint32_t)c->ctx->cu_count));
+    return std::max<uint32_t>(1u, std::min<uint32_t>(iterations, (uint32_t)c->ctx->cu_count));
+}
+
+//! power-of-two loss scale of the fp16 grid gradient: per-sample gradients scale as 1 / n, so the scaled
+//! contributions stay near 2^-6 (inside fp16's normal range, far from its 65504 limit)
+float grid_grad_scale(uint32_t n)
+{
+    const int e = n ? 31 - __builtin_clz(n) : 0;
+    return std::ldexp(1.0f, std::min(12, std::max(-6, e - 6)));
+}
+
+__global__ void k_rc_grid_grad_f32(const _Float16 *__restrict__ g, float inv_scale, float *__restrict__ out, uint64_t n)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < n) out[k] = static_cast<float>(g[k]) * inv_scale;
 }
 
 void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only,
@@ -382,11 +408,13 @@ void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *
     hipStream_t s = c->ctx->stream;
     const uint32_t nb = train_blocks(c, n);
     c->partials.reserve((size_t)nb * N_MLP * 4);
-    fill(s, c->grid_grad.as<float>(), c->n_grid, 0.0f);
+    c->grad_scale = grid_grad_scale(n);
+    fill(s, c->grid_grad.as<float>(), c->n_grid / 2, 0.0f);// n_grid fp16 values
     fill(s, c->loss.as<float>(), 1, 0.0f);
     if(n)
         k_rc_train<<<nb, TRAIN_BLOCK, 0, s>>>(device_view(c), records, stride_f4, targets, n, n_dev,
-                                              c->grid_grad.as<float>(), c->partials.as<float>(), c->loss.as<float>());
+                                              c->grid_grad.as<half2v>(), c->grad_scale, c->partials.as<float>(),
+                                              c->loss.as<float>());
     else
         fill(s, c->partials.as<float>(), (uint64_t)nb * N_MLP, 0.0f);
     adam_consts a{};
@@ -404,8 +432,8 @@ void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *
                                                        c->w_fwd.as<__bf16>(), c->w_bwd.as<__bf16>());
     if(!grad_only)
         k_rc_adam_grid<<<(uint32_t)((c->n_grid + 255) / 256), 256, 0, s>>>(
-            c->grid_master.as<float>(), c->grid_m.as<float>(), c->grid_v.as<float>(), c->grid_grad.as<float>(),
-            c->grid_half.as<__half>(), c->n_grid, a);
+            c->grid_master.as<float>(), c->grid_m.as<float>(), c->grid_v.as<float>(), c->grid_grad.as<_Float16>(),
+            1.0f / c->grad_scale, c->grid_half.as<__half>(), c->n_grid, a);
     VKRT_HIP(hipGetLastError());
 }
 
@@ -526,7 +554,8 @@ int vkrt_rc_create(vkrt_ctx *ctx, const vkrt_rc_params *params, uint32_t seed, v
         c->n_grid = (uint64_t)LEVELS * (1ull << p.log2_table) * 2ull;
         c->grid_master.reserve(c->n_grid * 4);
         c->grid_half.reserve(c->n_grid * 2);
-        c->grid_grad.reserve(c->n_grid * 4);
+        c->grid_grad.reserve(c->n_grid * 2);
+        c->grid_grad_f32.reserve(c->n_grid * 4);
         c->grid_m.reserve(c->n_grid * 4);
         c->grid_v.reserve(c->n_grid * 4);
         c->mlp_master.reserve(N_MLP * 4);
@@ -654,7 +683,13 @@ int vkrt_rc_get_gradients(vkrt_rc *c, float *mlp, float *grid)
     return guarded(c->ctx, [&] {
         hipStream_t s = c->ctx->stream;
         if(mlp) VKRT_HIP(hipMemcpyAsync(mlp, c->mlp_grad.ptr, N_MLP * 4, hipMemcpyDeviceToHost, s));
-        if(grid) VKRT_HIP(hipMemcpyAsync(grid, c->grid_grad.ptr, c->n_grid * 4, hipMemcpyDeviceToHost, s));
+        if(grid)
+        {
+            k_rc_grid_grad_f32<<<(uint32_t)((c->n_grid + 255) / 256), 256, 0, s>>>(
+                c->grid_grad.as<_Float16>(), 1.0f / c->grad_scale, c->grid_grad_f32.as<float>(), c->n_grid);
+            VKRT_HIP(hipGetLastError());
+            VKRT_HIP(hipMemcpyAsync(grid, c->grid_grad_f32.ptr, c->n_grid * 4, hipMemcpyDeviceToHost, s));
+        }
         VKRT_HIP(hipStreamSynchronize(s));
     });
 }

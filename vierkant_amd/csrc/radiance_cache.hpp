@@ -28,6 +28,7 @@ namespace vkrt::rc
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
 constexpr uint32_t LEVELS = 16;        // 4 lane groups x 4 levels
 constexpr uint32_t WIDTH = 64;
@@ -277,7 +278,8 @@ struct vkrt_rc
     vkrt::rc::grid_desc desc{};
     uint64_t n_grid = 0;          // LEVELS * T * 2 parameters
     uint32_t step = 0;            // Adam steps taken
-    vkrt::device_buffer grid_master, grid_half, grid_grad, grid_m, grid_v;
+    float grad_scale = 1.0f;      // loss scale of the last step's fp16 grid gradient
+    vkrt::device_buffer grid_master, grid_half, grid_grad, grid_grad_f32, grid_m, grid_v;
     vkrt::device_buffer mlp_master, mlp_grad, mlp_m, mlp_v, w_fwd, w_bwd, partials, loss;
 };
 
