@@ -25,7 +25,7 @@ ANY_BYTES_PER_RAY = 33       # 32 B ray in + 1 B flag out
 
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (profiles/trace_closest_config2.json, each counter scaled by its plain-copy calibration), or None"""
+    (profiles/k_trace_heightfield_spheres.json, each counter scaled by its plain-copy calibration), or None"""
     path = os.path.join(ROOT, "profiles", "trace_closest_config2.json")
     try:
         with open(path) as f:
