@@ -26,7 +26,7 @@ ANY_BYTES_PER_RAY = 33       # 32 B ray in + 1 B flag out
 def profiled_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
     (profiles/k_trace_heightfield_spheres.json, each counter scaled by its plain-copy calibration), or None"""
-    path = os.path.join(ROOT, "profiles", "trace_closest_config2.json")
+    path = os.path.join(ROOT, "profiles", "k_trace_heightfield_spheres.json")
     try:
         with open(path) as f:
             data = json.load(f)
@@ -71,13 +71,13 @@ def path_tracer_frame(ctx, spp):
     276,504 triangles) at 1920x1080, max_trace_depth 8, `spp` samples per pixel"""
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
-    scene = W.config3_scene()
+    scene = W.room_scene()
     s = vk.default_settings()
     s.resolution[0], s.resolution[1] = 1920, 1080
     s.num_samples, s.max_trace_depth = spp, 8
     pt = vk.PathTracer(ctx, s, seed=0)
     pt.set_scene(scene)
-    cam = W.config3_camera()
+    cam = W.room_camera()
     s_warm = vk.default_settings()
     s_warm.resolution[0], s_warm.resolution[1] = 1920, 1080
     s_warm.num_samples, s_warm.max_trace_depth = 1, 8
@@ -110,8 +110,8 @@ def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64,
     import numpy as np
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
-    scene = W.config3_scene()
-    cam = W.config3_camera()
+    scene = W.room_scene()
+    cam = W.room_camera()
 
     def settings(spp, batches):
         st = vk.default_settings()
@@ -188,7 +188,7 @@ def batched_frame(ctx, n_scenes, verify_rays):
     bvhs, rays, hits, host = [], [], [], []
     n_tri = 0
     t_build = 0.0
-    for pos, idx, r in W.config4(n_scenes=n_scenes):
+    for pos, idx, r in W.many_scenes(n_scenes=n_scenes):
         t0 = time.perf_counter()
         bvhs.append(ctx.bvh_build(pos, idx))
         t_build += time.perf_counter() - t0
@@ -259,7 +259,7 @@ def main():
     from vierkant_amd import workloads as W
 
     log("generating scene")
-    pos, idx = W.config2_scene()
+    pos, idx = W.heightfield_spheres_scene()
     tris = W.flat_triangles(pos, idx)
     primary = W.pinhole_rays(2048, 2048)
     ctx = vk.Context(0)

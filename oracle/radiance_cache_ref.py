@@ -1,7 +1,7 @@
 """Radiance cache, CPU restatement (numpy) - TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module, as the
-checker of the GPU radiance cache; no product path calls it.
+reference the GPU radiance cache is compared with; no product path calls it.
 
 The reference has no radiance cache: this component is new. It follows the published methods it is
 built from, each restated here from the mathematics:

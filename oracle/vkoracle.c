@@ -1,7 +1,7 @@
 /*
  * vkoracle.c - CPU oracle for the ray-intersection path (TEST INFRASTRUCTURE ONLY).
  *
- * A plain-C restatement of the reference's arithmetic, used as the checker by tests/,
+ * A plain-C restatement of the reference's arithmetic, used as the comparison reference by tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py. The product never links it.
  *
  * glm semantics restated (g-truc/glm, the reference's math library; its submodule is empty in the

@@ -2,7 +2,7 @@
  * vkoracle.h - CPU oracle for the ray-intersection path.
  *
  * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's algorithms, used
- * as the checker by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.
+ * as the comparison reference by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py.
  * Nothing in the product (libvkrt.so, the C++ host layer) links, loads or calls it.
  *
  * Built with -O3 -ffp-contract=off on x86-64 (SSE2 scalar float, no FMA, no FTZ/DAZ), which is

@@ -1,6 +1,6 @@
 """ctypes binding of the CPU oracle (oracle/liboracle.so) and of the reference-compiled helpers in
 oracle/_ref/libvkref.so. Test infrastructure: only tests/, __graft_entry__.smoke() and the cpu_baseline
-leg of bench.py use it, and only as the checker."""
+leg of bench.py use it, and only as the comparison reference."""
 import ctypes
 import os
 import pathlib

@@ -62,8 +62,8 @@ def test_plane_bitwise_equals_oracle():
     assert len(idx) // 3 == 524288
 
 
-def test_config2_scene_counts():
+def test_heightfield_spheres_scene_counts():
     from vierkant_amd import workloads as W
-    pos, idx = W.config2_scene(grid=64, n_spheres=20)
+    pos, idx = W.heightfield_spheres_scene(grid=64, n_spheres=20)
     assert len(idx) // 3 == 64 * 64 * 2 + 20 * 1280
     assert idx.max() < len(pos)

@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def small_batch(ctx):
-    scenes = list(W.config4(n_scenes=12, seed=3, tri_range=(200, 20000), ray_range=(500, 9000)))
+    scenes = list(W.many_scenes(n_scenes=12, seed=3, tri_range=(200, 20000), ray_range=(500, 9000)))
     bvhs, rays, hits, flats = [], [], [], []
     for pos, idx, r in scenes:
         bvhs.append(ctx.bvh_build(pos, idx))

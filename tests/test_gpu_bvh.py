@@ -13,15 +13,15 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
-def config1():
-    pos, idx, rays = W.config1()
+def icosphere_case():
+    pos, idx, rays = W.icosphere_rays()
     tris = W.flat_triangles(pos, idx)
     return pos, idx, tris, rays, orc.brute_closest(tris, rays)
 
 
 @pytest.mark.parametrize("leaf", [1, 4, 8])
-def test_bvh_closest_config1_bit_exact(ctx, config1, leaf):
-    pos, idx, tris, rays, want = config1
+def test_bvh_closest_icosphere_bit_exact(ctx, icosphere_case, leaf):
+    pos, idx, tris, rays, want = icosphere_case
     bvh = ctx.bvh_build(pos, idx, max_leaf_size=leaf)
     info = bvh.info()
     assert info.n_tri == len(idx) // 3 and 1 < info.depth <= 64
@@ -29,8 +29,8 @@ def test_bvh_closest_config1_bit_exact(ctx, config1, leaf):
     assert_hits_equal(got, want, f"bvh leaf={leaf}")
 
 
-def test_bvh_any_config1(ctx, config1):
-    pos, idx, tris, rays, _ = config1
+def test_bvh_any_icosphere(ctx, icosphere_case):
+    pos, idx, tris, rays, _ = icosphere_case
     bvh = ctx.bvh_build(pos, idx)
     r = rays.copy()
     rng = np.random.default_rng(20)
@@ -99,16 +99,16 @@ def test_bvh_bad_index_is_an_error(ctx):
     assert e.value.code == vk.E_INVALID_ARG
 
 
-def test_bvh_stride(ctx, config1):
-    pos, idx, tris, rays, want = config1
+def test_bvh_stride(ctx, icosphere_case):
+    pos, idx, tris, rays, want = icosphere_case
     padded = np.zeros((len(pos), 8), np.float32)
     padded[:, :3] = pos
     bvh = ctx.bvh_build(padded, idx, stride=32)
     assert_hits_equal(ctx.trace_closest(bvh, rays[:4096]), want[:4096], "stride 32")
 
 
-def test_bvh_refit(ctx, config1):
-    pos, idx, tris, rays, want = config1
+def test_bvh_refit(ctx, icosphere_case):
+    pos, idx, tris, rays, want = icosphere_case
     bvh = ctx.bvh_build(pos, idx)
     moved = (pos * np.float32(1.3) + np.float32(0.1)).astype(np.float32)
     bvh.refit(moved)
@@ -118,34 +118,34 @@ def test_bvh_refit(ctx, config1):
 
 
 @pytest.fixture(scope="module")
-def config2():
-    pos, idx = W.config2_scene()
+def heightfield_case():
+    pos, idx = W.heightfield_spheres_scene()
     return pos, idx, W.flat_triangles(pos, idx)
 
 
-def test_bvh_config2_primary_and_bounce(ctx, config2):
+def test_bvh_heightfield_primary_and_bounce(ctx, heightfield_case):
     """1,804,288 triangles: 65,536 primary and 65,536 bounce rays through the BVH equal GPU brute force
-    (itself bit-exact with the oracle on config 1), and 1,024 of them equal the oracle directly"""
-    pos, idx, tris = config2
+    (itself bit-exact with the oracle on the 20,480-triangle icosphere), and 1,024 of them equal the oracle directly"""
+    pos, idx, tris = heightfield_case
     bvh = ctx.bvh_build(pos, idx)
     primary_all = W.pinhole_rays(2048, 2048)
     sel = np.random.default_rng(24).choice(len(primary_all), 65536, replace=False)
     primary = primary_all[np.sort(sel)]
     hits = ctx.trace_closest(bvh, primary)
     assert np.mean(hits["prim"] != 0xFFFFFFFF) > 0.5
-    assert_hits_equal(hits, ctx.brute_closest(tris, primary), "config2 primary")
+    assert_hits_equal(hits, ctx.brute_closest(tris, primary), "heightfield_case primary")
     bounce, shadow = W.secondary_rays(primary, hits, tris)
     bh = ctx.trace_closest(bvh, bounce)
-    assert_hits_equal(bh, ctx.brute_closest(tris, bounce), "config2 bounce")
+    assert_hits_equal(bh, ctx.brute_closest(tris, bounce), "heightfield_case bounce")
     assert np.array_equal(ctx.trace_any(bvh, shadow), ctx.brute_any(tris, shadow))
     few = np.arange(0, 65536, 64)
-    assert_hits_equal(hits[few], orc.brute_closest(tris, primary[few]), "config2 vs oracle")
-    assert_hits_equal(bh[few], orc.brute_closest(tris, bounce[few]), "config2 bounce vs oracle")
+    assert_hits_equal(hits[few], orc.brute_closest(tris, primary[few]), "heightfield_case vs oracle")
+    assert_hits_equal(bh[few], orc.brute_closest(tris, bounce[few]), "heightfield_case bounce vs oracle")
 
 
-def test_trace_configs_give_identical_results(ctx, config1):
+def test_trace_configs_give_identical_results(ctx, icosphere_case):
     """every launch shape (block size x LDS stack depth) returns the same records"""
-    pos, idx, tris, rays, want = config1
+    pos, idx, tris, rays, want = icosphere_case
     bvh = ctx.bvh_build(pos, idx)
     saved = ctx.get_trace_config()
     try:

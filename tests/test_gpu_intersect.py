@@ -102,22 +102,22 @@ def test_compute_aabb_bit_exact(ctx):
 
 
 @pytest.fixture(scope="module")
-def config1():
-    pos, idx, rays = W.config1()
+def icosphere_case():
+    pos, idx, rays = W.icosphere_rays()
     tris = W.flat_triangles(pos, idx)
     return tris, rays, orc.brute_closest(tris, rays)
 
 
-def test_brute_closest_config1_full_bit_exact(ctx, config1):
+def test_brute_closest_icosphere_full_bit_exact(ctx, icosphere_case):
     """65,536 rays x 20,480 triangles: every record identical to the oracle"""
-    tris, rays, want = config1
+    tris, rays, want = icosphere_case
     got = ctx.brute_closest(tris, rays)
     assert np.sum(want["prim"] != 0xFFFFFFFF) > 3000   # about 6% of the random rays hit the sphere
-    assert_hits_equal(got, want, "config1 brute")
+    assert_hits_equal(got, want, "icosphere_case brute")
 
 
-def test_brute_any_matches_oracle(ctx, config1):
-    tris, rays, _ = config1
+def test_brute_any_matches_oracle(ctx, icosphere_case):
+    tris, rays, _ = icosphere_case
     r = rays[:8192].copy()
     r["tmax"] = np.random.default_rng(13).uniform(0.0, 6.0, len(r)).astype(np.float32)
     r["tmin"] = np.random.default_rng(14).uniform(-1.0, 1.0, len(r)).astype(np.float32)
@@ -154,9 +154,9 @@ def test_brute_sizes(ctx, n_tri, n_ray):
     assert np.array_equal(ctx.brute_any(tris, rays), orc.brute_any(tris, rays) if n_ray else np.zeros(0, np.uint8))
 
 
-def test_device_pointer_path(ctx, config1):
+def test_device_pointer_path(ctx, icosphere_case):
     """device-resident inputs and outputs (VKRT_DEVICE_PTRS) give the same records"""
-    tris, rays, want = config1
+    tris, rays, want = icosphere_case
     dt = vk.DeviceArray.from_host(ctx, tris)
     dr = vk.DeviceArray.from_host(ctx, rays)
     out = vk.DeviceArray(ctx, len(rays), vk.HIT_DTYPE)

@@ -21,7 +21,7 @@ def settings(w, h, spp=1, depth=8, batches=0, suspend=1, suppress_reset=0, max_p
 
 @pytest.fixture(scope="module")
 def scene3():
-    return W.config3_scene()
+    return W.room_scene()
 
 
 def sphere_scene(material, radius=2.0):
@@ -40,12 +40,12 @@ def empty_scene():
 
 
 def test_pt_matches_cpu_integrator(ctx, scene3):
-    """config-3 scene, 160x90, 4 spp, depth 8: per-pixel radiance equals the CPU reference integrator up to
+    """room scene, 160x90, 4 spp, depth 8: per-pixel radiance equals the CPU reference integrator up to
     transcendental-function ulps (ocml vs glibc cos/sin/pow), which can redirect a handful of paths"""
     w, h, spp = 160, 90, 4
     pt = vk.PathTracer(ctx, settings(w, h, spp), seed=0)
     pt.set_scene(scene3)
-    cam = W.config3_camera()
+    cam = W.room_camera()
     pt.render(cam)
     got = pt.pixels()
     seed = W.minstd_rand0(0, 1)[0]
@@ -104,7 +104,7 @@ def test_pt_batches_running_mean(ctx, scene3):
     w, h = 96, 54
     pt = vk.PathTracer(ctx, settings(w, h, 1, batches=8), seed=5)
     pt.set_scene(scene3)
-    cam = W.config3_camera()
+    cam = W.room_camera()
     seeds = W.minstd_rand0(5, 3)
     acc = None
     for b, sd in enumerate(seeds):
@@ -157,7 +157,7 @@ def test_pt_deterministic(ctx, scene3):
     b = vk.PathTracer(ctx, settings(64, 36, 2), seed=11)
     for pt in (a, b):
         pt.set_scene(scene3)
-        pt.render(W.config3_camera())
+        pt.render(W.room_camera())
     assert a.pixels().tobytes() == b.pixels().tobytes()
     assert a.stats().segments > 64 * 36 * 2
 
@@ -173,7 +173,7 @@ def test_pt_cache_inactive_below_its_depth_is_plain_pt(ctx, scene3):
         pt.set_scene(scene3)
     cached.set_radiance_cache(cache, cache_depth=8, train_stride=4)
     for pt in (plain, cached):
-        pt.render(W.config3_camera())
+        pt.render(W.room_camera())
     assert plain.pixels().tobytes() == cached.pixels().tobytes()
     assert cache.step() == 2                        # one training step per sample
     cache.close()
@@ -185,14 +185,14 @@ def test_pt_cache_converges_near_reference(ctx, scene3):
     w, h = 128, 72
     ref = vk.PathTracer(ctx, settings(w, h, 256), seed=8)
     ref.set_scene(scene3)
-    ref.render(W.config3_camera())
+    ref.render(W.room_camera())
     want = ref.pixels()["radiance"]
     pt = vk.PathTracer(ctx, settings(w, h, 1, batches=256), seed=9)
     pt.set_scene(scene3)
     cache = vk.RadianceCache(ctx, seed=3)
     pt.set_radiance_cache(cache, cache_depth=2, train_stride=8)
     for _ in range(192):
-        pt.render(W.config3_camera())
+        pt.render(W.room_camera())
     got = pt.pixels()["radiance"]
     assert np.isfinite(got).all() and (got >= 0).all()
     rel = abs(float(got.mean()) - float(want.mean())) / float(want.mean())

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/vkoracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/vkoracle.c) before results are compared with it.
 
 Each check is chosen so that a plausible slip in the restatement (a dropped term, a swapped cross-product
 operand, a wrong sign or index) fails it: exact known answers computed by hand for representable inputs,

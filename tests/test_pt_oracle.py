@@ -52,8 +52,8 @@ def test_emission_inside_black_sphere_closed_form():
 
 
 def test_integrator_deterministic_and_bounded():
-    sc = W.config3_scene(n_spheres=20)
-    cam = W.config3_camera()
+    sc = W.room_scene(n_spheres=20)
+    cam = W.room_camera()
     a = orc.pt_render(sc, cam, 48, 27, 2, 6, 99, nthreads=2)
     b = orc.pt_render(sc, cam, 48, 27, 2, 6, 99, nthreads=3)
     assert a.tobytes() == b.tobytes()

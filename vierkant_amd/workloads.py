@@ -85,7 +85,7 @@ def jittered_icosphere(subdiv=5, radius=1.0, sigma=0.01, seed=0):
     return (pos + jit).astype(np.float32), idx
 
 
-def config1(n_rays=65536, subdiv=5, seed=0):
+def icosphere_rays(n_rays=65536, subdiv=5, seed=0):
     """65,536 rays (origins U[-3,3]^3, directions uniform on S^2) vs the jittered subdiv-5 icosphere.
     Ray r uses uniform draws K0 + 5 r .. K0 + 5 r + 4 with K0 = 6 n_vert (after the jitter)."""
     pos, idx = jittered_icosphere(subdiv, seed=seed)
@@ -137,7 +137,7 @@ def fbm(x, z, octaves=6, base_freq=0.4, lacunarity=2.0, gain=0.5, seed=0):
     return total
 
 
-def config2_scene(grid=512, n_spheres=1000, seed=0, amplitude=0.2, size=10.0):
+def heightfield_spheres_scene(grid=512, n_spheres=1000, seed=0, amplitude=0.2, size=10.0):
     """Heightfield Plane(10, 10, 512, 512) displaced by 0.2 fBm (524,288 tris) followed by 1,000
     IcoSphere(r, 3) (1,280 tris each) with r = 0.01 * 10^U, centers x, z ~ U[-5, 5] and y = surface
     height + U[0, 1]; each sphere takes 4 uniform draws (r, x, z, y) from stream `seed`.
@@ -289,7 +289,7 @@ def _random_material(u):
     return vk.material(color=color, roughness=rough, metalness=metal, transmission=trans, ior=1.5)
 
 
-def config3_scene(n_spheres=200, seed=0):
+def room_scene(n_spheres=200, seed=0):
     """Room x in [-3, 3], y in [0, 6], z in [-3, 3] with the front (z = +3) open to the sky: floor, ceiling,
     back, red left and green right walls, an emissive ceiling quad and a box (12 quads); 200 IcoSphere(r, 3)
     with r ~ U[0.05, 0.3] inside the room; the jittered subdiv-5 icosphere (radius 1, sigma 0.01).
@@ -326,7 +326,7 @@ def config3_scene(n_spheres=200, seed=0):
     return b.scene(environment=1.0)
 
 
-def config3_camera():
+def room_camera():
     """eye (0, 3, 10.5) looking down -z (slightly down), 45 degree vertical field of view"""
     import vierkant_amd as vk
     return vk.make_camera(eye=(0.0, 3.0, 10.5), forward=(0.0, -0.05, -1.0), up=(0.0, 1.0, 0.0),
@@ -372,7 +372,7 @@ def minstd_rand0(seed, n):
 # ------------------------------------------------------------------------------------------------
 # configuration 4: many independent scenes for the batched launch
 # ------------------------------------------------------------------------------------------------
-def config4(n_scenes=256, seed=0, tri_range=(1000, 500000), ray_range=(16384, 2097152)):
+def many_scenes(n_scenes=256, seed=0, tri_range=(1000, 500000), ray_range=(16384, 2097152)):
     """Yields (positions, indices, rays) for `n_scenes` scenes. Scene s draws (from
     numpy.random.default_rng(seed), in order) its triangle count T and ray count R log-uniform in the given
     ranges; it is a 4 x 4 fBm heightfield patch with about T/2 triangles plus round(T / 2560) IcoSpheres of
