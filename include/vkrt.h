@@ -22,6 +22,7 @@
  *   vkrt_tri_box_overlap           <- intersect(const Triangle&, const AABB&)     (src/intersection.cpp:364-370)
  *   vkrt_tri_tri_overlap           <- intersect(const Triangle&, const Triangle&) (src/intersection.cpp:374-378)
  *   vkrt_compute_aabb              <- compute_aabb(const std::vector<vec3>&)      (src/intersection.cpp:96-106)
+ *   vkrt_compute_bounding_sphere   <- compute_bounding_sphere(const std::vector<vec3>&) (src/intersection.cpp:25-92)
  *   vkrt_intersect_brute_*         <- a closest/any-hit loop over intersect(Triangle, Ray); the reference
  *                                     has no such loop, its callers test one triangle at a time
  *   vkrt_bvh_build / _refit        <- RayBuilder::create_mesh_structures / build_scene_acceleration
@@ -163,6 +164,9 @@ int vkrt_tri_tri_overlap(vkrt_ctx *ctx, const float *tris_a, const float *tris_b
 /* AABB of n points (3 floats each); out: 6 floats (min.xyz, max.xyz). n == 0 gives the
  * reference's empty box (min = FLT_MAX, max = -FLT_MAX). */
 int vkrt_compute_aabb(vkrt_ctx *ctx, const float *points, uint32_t n, float *out6, int flags);
+/* compute_bounding_sphere (src/intersection.cpp:25-92): Ritter's sphere with the reference's extremum
+ * choice and growth order, bit-identical. out4 = center xyz, radius. n == 0 gives zeros. */
+int vkrt_compute_bounding_sphere(vkrt_ctx *ctx, const float *points, uint32_t n, float *out4, int flags);
 
 /* ---------------------------------------------------------------------------------------------
  * brute force: every ray against every triangle

@@ -230,6 +230,73 @@ void orc_compute_aabb(const float *pts, uint32_t n, float *out6)
     }
 }
 
+/* compute_bounding_sphere (src/intersection.cpp:25-92, the `#if 1` branch): Ritter's sphere. Extremum
+ * points per axis (strict comparisons: the first occurrence wins, and a NaN coordinate never replaces the
+ * current candidate), the axis with the largest squared extent (strict >, from 0) seeds the sphere, then one
+ * pass grows it to each point outside: k = 0.5 + (r / d) / 2, c = mix(p, c, k) = p (1 - k) + c k,
+ * r = (r + d) / 2. glm::length2(v) = (x x + y y) + z z. out4 = center xyz, radius. n == 0 gives zeros
+ * (the reference reads vertices[0] there). */
+void orc_bounding_sphere(const float *pts, uint32_t n, float *out4)
+{
+    if(n == 0)
+    {
+        out4[0] = out4[1] = out4[2] = out4[3] = 0.f;
+        return;
+    }
+    uint32_t pmin[3] = {0, 0, 0}, pmax[3] = {0, 0, 0};
+    for(uint32_t i = 0; i < n; ++i)
+        for(int a = 0; a < 3; ++a)
+        {
+            const float c = pts[3 * i + a];
+            pmin[a] = (c < pts[3 * pmin[a] + a]) ? i : pmin[a];
+            pmax[a] = (c > pts[3 * pmax[a] + a]) ? i : pmax[a];
+        }
+    float best = 0.f;
+    int axis = 0;
+    for(int a = 0; a < 3; ++a)
+    {
+        const float *p1 = pts + 3 * pmin[a], *p2 = pts + 3 * pmax[a];
+        const float dx = p2[0] - p1[0], dy = p2[1] - p1[1], dz = p2[2] - p1[2];
+        const float d2 = (dx * dx + dy * dy) + dz * dz;
+        if(d2 > best)
+        {
+            best = d2;
+            axis = a;
+        }
+    }
+    const float *p1 = pts + 3 * pmin[axis], *p2 = pts + 3 * pmax[axis];
+    float c[3] = {(p1[0] + p2[0]) / 2.f, (p1[1] + p2[1]) / 2.f, (p1[2] + p2[2]) / 2.f};
+    float r = sqrtf(best) / 2.f;
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const float *p = pts + 3 * i;
+        const float vx = p[0] - c[0], vy = p[1] - c[1], vz = p[2] - c[2];
+        const float d2 = (vx * vx + vy * vy) + vz * vz;
+        if(d2 > r * r)
+        {
+            const float d = sqrtf(d2);
+            const float k = 0.5f + (r / d) / 2;
+            for(int a = 0; a < 3; ++a) c[a] = p[a] * (1.f - k) + c[a] * k;
+            r = (r + d) / 2;
+        }
+    }
+    out4[0] = c[0];
+    out4[1] = c[1];
+    out4[2] = c[2];
+    out4[3] = r;
+}
+
+/* compute_centroid (src/intersection.cpp:110) returns crocore::mean(vertices); crocore is an empty
+ * submodule in the reference checkout, so its summation order is unknown: this is the sequential mean
+ * (sum from zero in index order, divided by n as float). Parity unpinned. */
+void orc_centroid(const float *pts, uint32_t n, float *out3)
+{
+    float s[3] = {0.f, 0.f, 0.f};
+    for(uint32_t i = 0; i < n; ++i)
+        for(int a = 0; a < 3; ++a) s[a] = s[a] + pts[3 * i + a];
+    for(int a = 0; a < 3; ++a) out3[a] = n ? s[a] / (float)n : 0.f;
+}
+
 /* ------------------------------------------------------------------------------------------------
  * brute-force closest / any hit (the harness the reference lacks; acceptance tmin < t < tmax,
  * ties on t resolve to the lowest triangle index)

@@ -36,6 +36,8 @@ int orc_intersect_sphere(const float *sphere4, const float *o, const float *d, f
 int orc_intersect_plane(const float *plane4, const float *o, const float *d, float *dist);
 /* compute_aabb, src/intersection.cpp:96-106 */
 void orc_compute_aabb(const float *points, uint32_t n, float *out6);
+void orc_bounding_sphere(const float *points, uint32_t n, float *out4);
+void orc_centroid(const float *points, uint32_t n, float *out3);
 
 /* brute-force closest / any hit over flat triangles (9 floats each), split over nthreads */
 void orc_brute_closest(const float *tris, uint32_t n_tri, const float *rays8, uint32_t n_ray, void *hits16,

@@ -101,6 +101,25 @@ int main()
     if(intersect(Sphere{{0.f, 0.f, 5.f}, 1.f}, Ray({0.f, 0.f, 0.f}, {0.f, 0.f, 1.f})).distance != 4.0f)
         return fail("sphere distance");
 
+    // point sets: the scalar compute_bounding_sphere equals the GPU entry point bit for bit
+    std::vector<vec3> cloud;
+    for(uint32_t i = 0; i < nv; ++i) cloud.push_back(vec3{pos[3 * i] * 3.f, pos[3 * i + 1], pos[3 * i + 2] * 0.5f});
+    const Sphere hs = compute_bounding_sphere(cloud);
+    float gs[4];
+    ctx->check(vkrt_compute_bounding_sphere(ctx->handle(), pos.data(), 0, gs, 0));
+    std::vector<float> flat;
+    for(const vec3 &p: cloud) flat.insert(flat.end(), {p.x, p.y, p.z});
+    ctx->check(vkrt_compute_bounding_sphere(ctx->handle(), flat.data(), nv, gs, 0));
+    if(bits(gs[0]) != bits(hs.center.x) || bits(gs[1]) != bits(hs.center.y) || bits(gs[2]) != bits(hs.center.z) ||
+       bits(gs[3]) != bits(hs.radius))
+        return fail("bounding sphere differs between host and GPU");
+    const AABB hb = compute_aabb(cloud);
+    float gb[6];
+    ctx->check(vkrt_compute_aabb(ctx->handle(), flat.data(), nv, gb, 0));
+    if(gb[0] != hb.min.x || gb[4] != hb.max.y || gb[5] != hb.max.z) return fail("aabb differs between host and GPU");
+    const vec3 cen = compute_centroid(std::vector<vec3>{{1.f, 2.f, 3.f}, {3.f, 2.f, 1.f}});
+    if(cen.x != 2.f || cen.y != 2.f || cen.z != 2.f) return fail("centroid");
+
     // PBRPathTracer: an empty scene, two batches, then suspended
     PBRPathTracer::create_info_t info;
     info.seed = 3;

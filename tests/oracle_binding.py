@@ -22,6 +22,8 @@ _SIG = {
     "orc_intersect_sphere": (_i32, [_vp, _vp, _vp, _fp]),
     "orc_intersect_plane": (_i32, [_vp, _vp, _vp, _fp]),
     "orc_compute_aabb": (None, [_vp, _u32, _vp]),
+    "orc_bounding_sphere": (None, [_vp, _u32, _vp]),
+    "orc_centroid": (None, [_vp, _u32, _vp]),
     "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_closest_f64": (_u32, [_vp, _u32, _vp, _dp, _dp, _dp]),
@@ -126,6 +128,20 @@ def compute_aabb(points):
     points, pp = _f(points)
     out = np.zeros(6, dtype=np.float32)
     lib().orc_compute_aabb(pp, len(points) // 3 if points.ndim == 1 else len(points), out.ctypes.data)
+    return out
+
+
+def bounding_sphere(points):
+    points, pp = _f(points)
+    out = np.zeros(4, dtype=np.float32)
+    lib().orc_bounding_sphere(pp, len(points) // 3 if points.ndim == 1 else len(points), out.ctypes.data)
+    return out
+
+
+def centroid(points):
+    points, pp = _f(points)
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_centroid(pp, len(points) // 3 if points.ndim == 1 else len(points), out.ctypes.data)
     return out
 
 

@@ -197,3 +197,23 @@ def test_overlap_kernels_match_reference_build_random(ctx):
     assert 0 < want_tt.sum() < n and 0 < want_tb.sum() < n
     assert np.array_equal(ctx.tri_tri_overlap(a, b), want_tt)
     assert np.array_equal(ctx.tri_box_overlap(a, boxes), want_tb)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 1000, 1025, 300000])
+def test_bounding_sphere_bit_exact(ctx, n):
+    """compute_bounding_sphere: the GPU's extremum choice and in-order growth equal the sequential oracle
+    bit for bit; coordinates are quantized so that ties between extremum candidates are common"""
+    rng = np.random.default_rng(n)
+    pts = (np.round(rng.normal(size=(n, 3)) * rng.uniform(0.5, 5.0, size=3) * 8) / 8).astype(np.float32)
+    got = ctx.compute_bounding_sphere(pts)
+    want = orc.bounding_sphere(pts)
+    assert got.view(np.uint32).tolist() == want.view(np.uint32).tolist()
+
+
+def test_bounding_sphere_special_inputs(ctx):
+    assert ctx.compute_bounding_sphere(np.zeros((0, 3), np.float32)).tolist() == [0.0, 0.0, 0.0, 0.0]
+    pts = np.array([[0, 0, 0], [np.nan, 0, 0], [4, 0, 0]], np.float32)
+    assert ctx.compute_bounding_sphere(pts).tolist() == orc.bounding_sphere(pts).tolist()
+    pts = np.array([[np.nan, 0, 0], [1, 0, 0], [5, 0, 0]], np.float32)
+    got, want = ctx.compute_bounding_sphere(pts), orc.bounding_sphere(pts)
+    assert np.isnan(got[0]) and got[1:].tolist() == want[1:].tolist()

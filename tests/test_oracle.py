@@ -243,3 +243,48 @@ def test_hammersley_closed_form():
         assert orc.hammersley(2 ** k, 1)[1] == 2.0 ** -(k + 1)
     assert tuple(orc.hammersley(3, 4)) == (0.75, 0.75)
     assert tuple(orc.hammersley(0, 4)) == (0.0, 0.0)
+
+
+# ---- compute_bounding_sphere / compute_centroid (src/intersection.cpp:25-110) ------------------------------
+
+def test_bounding_sphere_two_points_exact():
+    """two points: the seed diameter is the segment, no point lies outside: center = midpoint, r = |p2-p1|/2"""
+    s = orc.bounding_sphere(np.array([[1.0, 2.0, 3.0], [5.0, 2.0, 3.0]], np.float32))
+    assert s.tolist() == [3.0, 2.0, 3.0, 2.0]
+
+
+def test_bounding_sphere_hand_computed_growth():
+    """x extremes (-2,0,0),(2,0,0) (squared extent 16 beats y's 13) seed c = 0, r = 2; (0,3,0) lies outside
+    with d = 3: k = 0.5 + (2/3)/2, c = (0,3,0)(1 - k) + 0 k, r = (2 + 3)/2 = 2.5"""
+    pts = np.array([[-2, 0, 0], [2, 0, 0], [0, 3, 0]], np.float32)
+    s = orc.bounding_sphere(pts)
+    k = np.float32(0.5) + (np.float32(2.0) / np.float32(3.0)) / np.float32(2.0)
+    assert s[0] == 0.0 and s[2] == 0.0 and s[3] == np.float32(2.5)
+    assert s[1] == np.float32(3.0) * (np.float32(1.0) - k)
+
+
+def test_bounding_sphere_contains_all_points():
+    rng = np.random.default_rng(8)
+    for n in (1, 3, 17, 1000, 20000):
+        pts = (rng.normal(size=(n, 3)) * rng.uniform(0.1, 10, size=3)).astype(np.float32)
+        s = orc.bounding_sphere(pts)
+        d = np.linalg.norm(pts.astype(np.float64) - s[:3].astype(np.float64), axis=1)
+        assert (d <= s[3] * (1 + 1e-5) + 1e-6).all()
+        lo, hi = pts.min(0), pts.max(0)
+        assert s[3] <= np.linalg.norm(hi - lo)            # never above the box diagonal
+
+
+def test_bounding_sphere_nan_never_wins():
+    """a NaN coordinate fails every strict comparison: it is never an extremum and never grows the sphere"""
+    pts = np.array([[0, 0, 0], [np.nan, 0, 0], [4, 0, 0]], np.float32)
+    assert orc.bounding_sphere(pts).tolist() == [2.0, 0.0, 0.0, 2.0]
+    # a NaN at index 0 stays the candidate of that axis; every seed distance is then NaN, so axis 0 with
+    # squared extent 0 seeds c = (p0 + p0) / 2 and r = 0, and no NaN distance exceeds r^2
+    s = orc.bounding_sphere(np.array([[np.nan, 0, 0], [1, 0, 0], [5, 0, 0]], np.float32))
+    assert np.isnan(s[0]) and s[1:].tolist() == [0.0, 0.0, 0.0]
+
+
+def test_centroid_sequential_mean():
+    pts = np.array([[1, 2, 3], [3, 2, 1], [2, 2, 2]], np.float32)
+    assert orc.centroid(pts).tolist() == [2.0, 2.0, 2.0]
+    assert orc.centroid(np.zeros((0, 3), np.float32)).tolist() == [0.0, 0.0, 0.0]

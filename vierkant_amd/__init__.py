@@ -140,6 +140,7 @@ _SIGNATURES = {
     "vkrt_tri_box_overlap": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_tri_tri_overlap": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
     "vkrt_compute_aabb": (_i32, [_vp, _vp, _u32, _vp, _i32]),
+    "vkrt_compute_bounding_sphere": (_i32, [_vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_brute_closest": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_brute_any": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
     "vkrt_bvh_build": (_i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _i32, ctypes.POINTER(_vp)]),
@@ -418,6 +419,15 @@ class Context:
         if out is None:
             out = np.zeros(6, dtype=np.float32)
         self.check(lib().vkrt_compute_aabb(self._h, _ptr(points)[0], n, _ptr(out)[0], _flags([points, out])))
+        return out
+
+    def compute_bounding_sphere(self, points, out=None):
+        """(center.x, center.y, center.z, radius): compute_bounding_sphere of the reference, bit-identical"""
+        n = _n(points, 3)
+        if out is None:
+            out = np.zeros(4, dtype=np.float32)
+        self.check(lib().vkrt_compute_bounding_sphere(self._h, _ptr(points)[0], n, _ptr(out)[0],
+                                                      _flags([points, out])))
         return out
 
     # -- brute force ----------------------------------------------------------------------------

@@ -296,6 +296,96 @@ __global__ void k_aabb_finalize(const float *__restrict__ pts, const uint64_t *_
     out6[a] = r;
 }
 
+//! compute_bounding_sphere (src/intersection.cpp:25-92): the extremum indices come from k_aabb_reduce's
+//! keys (first occurrence of the strict minimum / maximum, NaN never wins), then one workgroup seeds the
+//! sphere from the axis with the largest squared extent and grows it in index order. Per chunk of 1024
+//! points the block finds the first point outside the current sphere, that lane applies the reference's
+//! update, and the scan resumes after it: the exact sequential result, in parallel within each chunk.
+constexpr int SPHERE_BLOCK = 1024;
+
+__global__ void __launch_bounds__(SPHERE_BLOCK) k_sphere_grow(const float *__restrict__ pts, uint32_t n,
+                                                             const uint64_t *__restrict__ keys,
+                                                             float *__restrict__ out4)
+{
+    __shared__ float sph[4];
+    __shared__ uint32_t first;
+    const uint32_t tid = threadIdx.x;
+    if(tid == 0)
+    {
+        uint32_t pmin[3], pmax[3];
+        for(int a = 0; a < 3; ++a)
+        {
+            const float v0 = pts[a];
+            const bool nan0 = v0 != v0;// a NaN first coordinate is never replaced
+            pmin[a] = (nan0 || keys[a] == ~0ull) ? 0u : static_cast<uint32_t>(keys[a] & 0xffffffffu);
+            pmax[a] = (nan0 || keys[3 + a] == 0ull) ? 0u : 0xffffffffu - static_cast<uint32_t>(keys[3 + a] & 0xffffffffu);
+        }
+        float best = 0.f;
+        int axis = 0;
+        for(int a = 0; a < 3; ++a)
+        {
+            const float *p1 = pts + 3 * (size_t)pmin[a], *p2 = pts + 3 * (size_t)pmax[a];
+            const float dx = p2[0] - p1[0], dy = p2[1] - p1[1], dz = p2[2] - p1[2];
+            const float d2 = (dx * dx + dy * dy) + dz * dz;
+            if(d2 > best)
+            {
+                best = d2;
+                axis = a;
+            }
+        }
+        const float *p1 = pts + 3 * (size_t)pmin[axis], *p2 = pts + 3 * (size_t)pmax[axis];
+        sph[0] = (p1[0] + p2[0]) / 2.f;
+        sph[1] = (p1[1] + p2[1]) / 2.f;
+        sph[2] = (p1[2] + p2[2]) / 2.f;
+        sph[3] = sqrtf(best) / 2.f;
+    }
+    __syncthreads();
+    for(uint32_t base = 0; base < n; base += SPHERE_BLOCK)
+    {
+        const uint32_t i = base + tid;
+        float px = 0.f, py = 0.f, pz = 0.f;
+        if(i < n)
+        {
+            px = pts[3 * (size_t)i];
+            py = pts[3 * (size_t)i + 1];
+            pz = pts[3 * (size_t)i + 2];
+        }
+        uint32_t start = 0;
+        while(true)
+        {
+            const float cx = sph[0], cy = sph[1], cz = sph[2], r = sph[3];
+            const float vx = px - cx, vy = py - cy, vz = pz - cz;
+            const float d2 = (vx * vx + vy * vy) + vz * vz;
+            const bool outside = i < n && tid >= start && d2 > r * r;
+            if(tid == 0) first = 0xffffffffu;
+            __syncthreads();
+            if(outside) atomicMin(&first, tid);
+            __syncthreads();
+            const uint32_t f = first;
+            if(f == 0xffffffffu) break;
+            if(tid == f)
+            {
+                const float d = sqrtf(d2);
+                const float k = 0.5f + (r / d) / 2;
+                sph[0] = px * (1.f - k) + cx * k;
+                sph[1] = py * (1.f - k) + cy * k;
+                sph[2] = pz * (1.f - k) + cz * k;
+                sph[3] = (r + d) / 2;
+            }
+            __syncthreads();
+            start = f + 1;
+        }
+        __syncthreads();
+    }
+    if(tid == 0)
+    {
+        out4[0] = sph[0];
+        out4[1] = sph[1];
+        out4[2] = sph[2];
+        out4[3] = sph[3];
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------------------------
@@ -458,6 +548,36 @@ int vkrt_compute_aabb(vkrt_ctx *ctx, const float *points, uint32_t n, float *out
         k_aabb_finalize<<<1, 64, 0, ctx->stream>>>(dp, keys, dout);
         VKRT_HIP(hipGetLastError());
         finish_output(ctx, 2, out6, 6, flags);
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_compute_bounding_sphere(vkrt_ctx *ctx, const float *points, uint32_t n, float *out4, int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(out4 && (n == 0 || points), "null array argument");
+        float *dout = output(ctx, 2, out4, 4, flags);
+        if(n == 0)
+        {
+            // the reference reads vertices[0] of an empty array; here the sphere is all zeros
+            const float zero[4] = {0.f, 0.f, 0.f, 0.f};
+            VKRT_HIP(hipMemcpyAsync(dout, zero, sizeof zero, hipMemcpyHostToDevice, ctx->stream));
+            VKRT_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        else
+        {
+            const float *dp = input(ctx, 0, points, (size_t)n * 3, flags);
+            ctx->scratch[0].reserve(6 * sizeof(uint64_t));
+            uint64_t *keys = ctx->scratch[0].as<uint64_t>();
+            launch_fill_u64(ctx->stream, keys, 3, ~0ull);
+            launch_fill_u64(ctx->stream, keys + 3, 3, 0ull);
+            uint32_t blocks = div_up(n, 256);
+            if(blocks > 4 * (uint32_t)ctx->cu_count) blocks = 4 * ctx->cu_count;
+            k_aabb_reduce<<<blocks, 256, 0, ctx->stream>>>(dp, n, keys);
+            k_sphere_grow<<<1, SPHERE_BLOCK, 0, ctx->stream>>>(dp, n, keys, dout);
+            VKRT_HIP(hipGetLastError());
+        }
+        finish_output(ctx, 2, out4, 4, flags);
         finish_call(ctx, flags);
     });
 }

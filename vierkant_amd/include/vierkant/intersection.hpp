@@ -6,7 +6,9 @@
 // through the C ABI (vkrt.h) via vierkant::RayBuilder and the vkrt_intersect_* entry points.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
+#include <vector>
 
 #include "vierkant/rt_math.hpp"
 
@@ -160,6 +162,75 @@ inline uint32_t intersect(const Triangle &a, const Triangle &b)
     const float ta[9] = {a.v0.x, a.v0.y, a.v0.z, a.v1.x, a.v1.y, a.v1.z, a.v2.x, a.v2.y, a.v2.z};
     const float tb[9] = {b.v0.x, b.v0.y, b.v0.z, b.v1.x, b.v1.y, b.v1.z, b.v2.x, b.v2.y, b.v2.z};
     return rt::tri_tri_overlap(ta, tb);
+}
+
+// ---- point sets (src/intersection.cpp:25-110) -----------------------------------------------------
+
+//! compute_aabb (src/intersection.cpp:96-106): glm::min / glm::max fold from the empty box
+inline AABB compute_aabb(const std::vector<vec3> &vertices)
+{
+    AABB ret;
+    for(const vec3 &p: vertices)
+    {
+        ret.min = vec3{p.x < ret.min.x ? p.x : ret.min.x, p.y < ret.min.y ? p.y : ret.min.y, p.z < ret.min.z ? p.z : ret.min.z};
+        ret.max = vec3{ret.max.x < p.x ? p.x : ret.max.x, ret.max.y < p.y ? p.y : ret.max.y, ret.max.z < p.z ? p.z : ret.max.z};
+    }
+    return ret;
+}
+
+//! compute_bounding_sphere (src/intersection.cpp:25-92): Ritter's sphere; vkrt_compute_bounding_sphere is
+//! the same computation on the GPU. An empty array gives a zero sphere (the reference reads vertices[0]).
+inline Sphere compute_bounding_sphere(const std::vector<vec3> &vertices)
+{
+    Sphere ret;
+    if(vertices.empty()) return ret;
+    auto at = [&](size_t i, int a) { return a == 0 ? vertices[i].x : (a == 1 ? vertices[i].y : vertices[i].z); };
+    size_t pmin[3] = {0, 0, 0}, pmax[3] = {0, 0, 0};
+    for(size_t i = 0; i < vertices.size(); ++i)
+        for(int a = 0; a < 3; ++a)
+        {
+            pmin[a] = at(i, a) < at(pmin[a], a) ? i : pmin[a];
+            pmax[a] = at(i, a) > at(pmax[a], a) ? i : pmax[a];
+        }
+    float best = 0.f;
+    int axis = 0;
+    for(int a = 0; a < 3; ++a)
+    {
+        const vec3 d = vertices[pmax[a]] - vertices[pmin[a]];
+        const float d2 = rt::dot(d, d);
+        if(d2 > best)
+        {
+            best = d2;
+            axis = a;
+        }
+    }
+    const vec3 &p1 = vertices[pmin[axis]], &p2 = vertices[pmax[axis]];
+    ret.center = vec3{(p1.x + p2.x) / 2.f, (p1.y + p2.y) / 2.f, (p1.z + p2.z) / 2.f};
+    ret.radius = std::sqrt(best) / 2.f;
+    for(const vec3 &p: vertices)
+    {
+        const vec3 v = p - ret.center;
+        const float d2 = rt::dot(v, v);
+        if(d2 > ret.radius * ret.radius)
+        {
+            const float d = std::sqrt(d2);
+            const float k = 0.5f + (ret.radius / d) / 2;
+            ret.center = vec3{p.x * (1.f - k) + ret.center.x * k, p.y * (1.f - k) + ret.center.y * k,
+                              p.z * (1.f - k) + ret.center.z * k};
+            ret.radius = (ret.radius + d) / 2;
+        }
+    }
+    return ret;
+}
+
+//! compute_centroid (src/intersection.cpp:110) returns crocore::mean, whose source is not in the reference
+//! checkout (empty submodule): here the sequential mean, sum in index order divided by the count
+inline vec3 compute_centroid(const std::vector<vec3> &vertices)
+{
+    vec3 s{0.f, 0.f, 0.f};
+    for(const vec3 &p: vertices) s = vec3{s.x + p.x, s.y + p.y, s.z + p.z};
+    const float n = static_cast<float>(vertices.size());
+    return vertices.empty() ? s : vec3{s.x / n, s.y / n, s.z / n};
 }
 
 }// namespace vierkant
