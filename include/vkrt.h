@@ -194,7 +194,9 @@ typedef struct vkrt_bvh_params
 {
     uint32_t builder;       /* VKRT_BUILDER_* */
     uint32_t max_leaf_size; /* triangles per leaf, 1..8 (0 = default) */
-    uint32_t reserved[6];
+    uint32_t width;         /* traversal layout: 0 = 4-wide collapse when its stack bound fits (default),
+                             * 2 = the binary tree only. Results do not depend on it. */
+    uint32_t reserved[5];
 } vkrt_bvh_params;
 
 typedef struct vkrt_bvh_info
@@ -207,6 +209,8 @@ typedef struct vkrt_bvh_info
     float build_ms;
     uint64_t device_bytes;
     float bounds[6];
+    uint32_t n_wide_nodes;  /* nodes of the 4-wide collapse (0 when traversal uses the binary tree) */
+    uint32_t wide_stack;    /* worst-case traversal stack depth of the 4-wide collapse */
 } vkrt_bvh_info;
 
 /* Build over an indexed triangle mesh: positions with a byte stride (>= 12) between vertices,

@@ -144,18 +144,43 @@ def test_bvh_heightfield_primary_and_bounce(ctx, heightfield_case):
 
 
 def test_trace_configs_give_identical_results(ctx, icosphere_case):
-    """every launch shape (block size x LDS stack depth) returns the same records"""
+    """every launch shape (block size x LDS stack depth) and both traversal layouts return the same records"""
     pos, idx, tris, rays, want = icosphere_case
-    bvh = ctx.bvh_build(pos, idx)
     saved = ctx.get_trace_config()
     try:
-        for block in (64, 128, 256):
-            for lds in (0, 8, 16, 32):
-                ctx.set_trace_config(block, lds)
-                assert_hits_equal(ctx.trace_closest(bvh, rays), want, f"block={block} lds={lds}")
-                occ = ctx.trace_any(bvh, rays)
-                assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8))
+        for width in (0, 2):
+            bvh = ctx.bvh_build(pos, idx, width=width)
+            for block in (64, 128, 256):
+                for lds in (0, 8, 16, 32):
+                    ctx.set_trace_config(block, lds)
+                    assert_hits_equal(ctx.trace_closest(bvh, rays), want, f"width={width} block={block} lds={lds}")
+                    occ = ctx.trace_any(bvh, rays)
+                    assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8))
+            bvh.close()
         with pytest.raises(vk.VkrtError):
             ctx.set_trace_config(96, 16)
     finally:
         ctx.set_trace_config(*saved)
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 8])
+def test_wide_collapse_equals_binary_tree(ctx, heightfield_case, leaf):
+    """the 4-wide collapse (default layout) and the binary tree give identical closest and any hits on
+    primary and bounce rays of the heightfield + spheres scene; the collapse has about a third as many nodes"""
+    pos, idx, tris = heightfield_case
+    primary = W.pinhole_rays(256, 256)
+    wide = ctx.bvh_build(pos, idx, max_leaf_size=leaf)
+    binary = ctx.bvh_build(pos, idx, max_leaf_size=leaf, width=2)
+    wi, bi = wide.info(), binary.info()
+    assert wi.n_wide_nodes > 0 and bi.n_wide_nodes == 0
+    assert wi.n_wide_nodes < 0.5 * wi.n_nodes and 0 < wi.wide_stack <= 64
+    hw = ctx.trace_closest(wide, primary)
+    hb = ctx.trace_closest(binary, primary)
+    assert_hits_equal(hw, hb, "wide vs binary primary")
+    bounce, shadow = W.secondary_rays(primary, hb, tris)
+    assert_hits_equal(ctx.trace_closest(wide, bounce), ctx.trace_closest(binary, bounce), "wide vs binary bounce")
+    assert np.array_equal(ctx.trace_any(wide, shadow), ctx.trace_any(binary, shadow))
+    sel = np.arange(0, len(bounce), 97)
+    assert_hits_equal(ctx.trace_closest(wide, bounce[sel]), ctx.brute_closest(tris, bounce[sel]), "wide vs brute")
+    wide.close()
+    binary.close()

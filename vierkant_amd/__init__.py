@@ -34,8 +34,8 @@ class VkrtError(RuntimeError):
 
 
 class BvhParams(ctypes.Structure):
-    _fields_ = [("builder", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 6)]
+    _fields_ = [("builder", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
 
 
 class TraceConfig(ctypes.Structure):
@@ -102,7 +102,8 @@ class BatchConfig(ctypes.Structure):
 class BvhInfo(ctypes.Structure):
     _fields_ = [("n_tri", ctypes.c_uint32), ("n_nodes", ctypes.c_uint32), ("n_leaves", ctypes.c_uint32),
                 ("depth", ctypes.c_uint32), ("sah_cost", ctypes.c_float), ("build_ms", ctypes.c_float),
-                ("device_bytes", ctypes.c_uint64), ("bounds", ctypes.c_float * 6)]
+                ("device_bytes", ctypes.c_uint64), ("bounds", ctypes.c_float * 6),
+                ("n_wide_nodes", ctypes.c_uint32), ("wide_stack", ctypes.c_uint32)]
 
 
 class RcParams(ctypes.Structure):
@@ -448,8 +449,9 @@ class Context:
         return out
 
     # -- BVH ------------------------------------------------------------------------------------
-    def bvh_build(self, positions, indices, builder=BUILDER_LBVH, max_leaf_size=0, stride=12):
-        params = BvhParams(builder, max_leaf_size)
+    def bvh_build(self, positions, indices, builder=BUILDER_LBVH, max_leaf_size=0, stride=12, width=0):
+        """width: 0 = traverse the 4-wide collapse when its stack bound fits, 2 = the binary tree only"""
+        params = BvhParams(builder, max_leaf_size, width)
         n_vert = _n(positions) * 4 // stride
         n_tri = _n(indices, 3)
         h = _vp()

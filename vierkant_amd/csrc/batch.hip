@@ -25,10 +25,13 @@ struct batch_item_dev
     const float4 *tris;
     const float4 *rays;
     vkrt_hit *hits;
+    const vkrt_bvh4_node *nodes4;
     int32_t root;
     uint32_t empty;
     uint32_t n_rays;
-    uint32_t pad;
+    int32_t root4;
+    uint32_t wide;
+    uint32_t pad[3];
 };
 
 __global__ void k_batch_reset(uint32_t *counter) { counter[threadIdx.x] = 0u; }
@@ -68,7 +71,10 @@ __global__ void __launch_bounds__(BLOCK) k_batch_trace(const batch_item_dev *__r
             if(!it.empty)
             {
                 ray_setup rs = setup_ray(it.rays, r);
-                hit = traverse<false, BLOCK, LDS>(it.nodes, it.tris, it.root, rs, smem + threadIdx.x, bt, bu, bv, prim);
+                hit = it.wide ? traverse4<false, BLOCK, LDS>(it.nodes4, it.tris, it.root4, rs, smem + threadIdx.x, bt,
+                                                             bu, bv, prim)
+                              : traverse<false, BLOCK, LDS>(it.nodes, it.tris, it.root, rs, smem + threadIdx.x, bt, bu,
+                                                            bv, prim);
             }
             it.hits[r] = hit ? vkrt_hit{bt, bu, bv, prim}
                              : vkrt_hit{__int_as_float(0x7f800000), 0.f, 0.f, VKRT_INVALID_PRIM};
@@ -203,9 +209,17 @@ int vkrt_batch_create(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_it
             VKRT_REQUIRE(it.bvh, "null bvh in batch item");
             VKRT_REQUIRE(it.bvh->device == ctx->device, "bvh belongs to another device");
             VKRT_REQUIRE(it.n_rays == 0 || (it.rays && it.hits), "null ray or hit buffer in batch item");
-            batch_item_dev d{it.bvh->nodes.as<vkrt_bvh_node>(), it.bvh->tris.as<float4>(),
-                             reinterpret_cast<const float4 *>(it.rays), it.hits, it.bvh->root_ref,
-                             it.bvh->empty ? 1u : 0u, it.n_rays, 0u};
+            batch_item_dev d{};
+            d.nodes = it.bvh->nodes.as<vkrt_bvh_node>();
+            d.tris = it.bvh->tris.as<float4>();
+            d.rays = reinterpret_cast<const float4 *>(it.rays);
+            d.hits = it.hits;
+            d.nodes4 = it.bvh->nodes4.as<vkrt_bvh4_node>();
+            d.root = it.bvh->root_ref;
+            d.empty = it.bvh->empty ? 1u : 0u;
+            d.n_rays = it.n_rays;
+            d.root4 = it.bvh->root4;
+            d.wide = it.bvh->wide ? 1u : 0u;
             b->items.push_back(d);
             b->n_rays.push_back(it.n_rays);
             b->total_rays += it.n_rays;

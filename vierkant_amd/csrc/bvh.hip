@@ -399,11 +399,165 @@ static void build_bounds(vkrt_ctx *ctx, vkrt_bvh *bvh)
     VKRT_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------------------------
+// 4-wide collapse: top-down, one BVH4 node per work item. A work item starts from a BVH2 node's two
+// children and repeatedly opens the internal child with the largest surface area until it has four
+// children or only leaves; each internal child becomes a new work item. The boxes are the BVH2's (already
+// padded), so traversal stays conservative. Every item also carries the traversal stack depth on entry
+// (the sum of pushes, children - 1, over its ancestors); the maximum decides whether the collapse fits the
+// fixed stack.
+// ---------------------------------------------------------------------------------------------
+struct collapse_child
+{
+    int32_t ref;
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ collapse_child child_of(const vkrt_bvh_node &nd, int k)
+{
+    collapse_child c;
+    const float4 xy = k == 0 ? nd.a : nd.b;
+    c.ref = k == 0 ? nd.d.x : nd.d.y;
+    c.lo[0] = xy.x;
+    c.hi[0] = xy.y;
+    c.lo[1] = xy.z;
+    c.hi[1] = xy.w;
+    c.lo[2] = k == 0 ? nd.c.x : nd.c.z;
+    c.hi[2] = k == 0 ? nd.c.y : nd.c.w;
+    return c;
+}
+
+__device__ __forceinline__ float half_area(const collapse_child &c)
+{
+    const float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+//! queue entries: (BVH2 node, BVH4 slot, stack depth on entry, -); counters: [0] BVH4 nodes allocated,
+//! [1] next queue size, [2] maximum stack depth
+__global__ void __launch_bounds__(256) k_collapse(const vkrt_bvh_node *__restrict__ n2, const int4 *__restrict__ in_q,
+                                                  uint32_t n_in, vkrt_bvh4_node *__restrict__ n4,
+                                                  int4 *__restrict__ out_q, uint32_t *__restrict__ counters)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if(t >= n_in) return;
+    const int4 item = in_q[t];
+    collapse_child ch[4];
+    const vkrt_bvh_node root = n2[item.x];
+    ch[0] = child_of(root, 0);
+    ch[1] = child_of(root, 1);
+    int cnt = 2;
+    while(cnt < 4)
+    {
+        int best = -1;
+        float best_a = -1.0f;
+        for(int j = 0; j < cnt; ++j)
+        {
+            if(ch[j].ref < 0) continue;
+            const float a = half_area(ch[j]);
+            if(a > best_a)
+            {
+                best_a = a;
+                best = j;
+            }
+        }
+        if(best < 0) break;
+        const vkrt_bvh_node nd = n2[ch[best].ref];
+        ch[best] = child_of(nd, 0);
+        ch[cnt] = child_of(nd, 1);
+        ++cnt;
+    }
+    const int32_t depth_here = item.z + (cnt - 1);
+    atomicMax(&counters[2], static_cast<uint32_t>(depth_here));
+    vkrt_bvh4_node out;
+    float lo[3][4], hi[3][4];
+    int32_t refs[4];
+    for(int j = 0; j < 4; ++j)
+    {
+        for(int a = 0; a < 3; ++a)
+        {
+            lo[a][j] = j < cnt ? ch[j].lo[a] : 0.0f;
+            hi[a][j] = j < cnt ? ch[j].hi[a] : 0.0f;
+        }
+        refs[j] = EMPTY_CHILD;
+        if(j >= cnt) continue;
+        if(ch[j].ref < 0) refs[j] = ch[j].ref;
+        else
+        {
+            const uint32_t slot = atomicAdd(&counters[0], 1u);
+            refs[j] = static_cast<int32_t>(slot);
+            const uint32_t q = atomicAdd(&counters[1], 1u);
+            out_q[q] = make_int4(ch[j].ref, static_cast<int32_t>(slot), depth_here, 0);
+        }
+    }
+    out.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    out.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    out.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    out.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    out.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    out.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    out.child = make_int4(refs[0], refs[1], refs[2], refs[3]);
+    out.pad = make_int4(0, 0, 0, 0);
+    n4[item.y] = out;
+}
+
+//! collapse the BVH2 into the BVH4 used for traversal; keeps the BVH2 path when the worst-case stack
+//! depth of the collapse exceeds STACK_SIZE
+static void build_wide(vkrt_ctx *ctx, vkrt_bvh *bvh)
+{
+    hipStream_t s = ctx->stream;
+    bvh->wide = false;
+    bvh->n_nodes4 = 0;
+    bvh->stack4 = 0;
+    if(bvh->empty || !bvh->allow_wide) return;
+    if(bvh->root_ref < 0)
+    {
+        bvh->root4 = bvh->root_ref;// a single leaf
+        bvh->wide = true;
+        return;
+    }
+    const size_t cap = std::max<uint32_t>(1u, bvh->n_nodes);
+    bvh->nodes4.reserve(cap * sizeof(vkrt_bvh4_node));
+    bvh->collapse_q[0].reserve(cap * sizeof(int4));
+    bvh->collapse_q[1].reserve(cap * sizeof(int4));
+    bvh->collapse_cnt.reserve(64);
+    uint32_t *cnt = bvh->collapse_cnt.as<uint32_t>();
+    const uint32_t init[4] = {1u, 0u, 0u, 0u};
+    VKRT_HIP(hipMemcpyAsync(cnt, init, sizeof init, hipMemcpyHostToDevice, s));
+    const int4 root_item = make_int4(bvh->root_ref, 0, 0, 0);
+    VKRT_HIP(hipMemcpyAsync(bvh->collapse_q[0].ptr, &root_item, sizeof root_item, hipMemcpyHostToDevice, s));
+    uint32_t n_in = 1;
+    int cur = 0;
+    while(n_in)
+    {
+        VKRT_HIP(hipMemsetAsync(cnt + 1, 0, 4, s));
+        k_collapse<<<div_up(n_in, 256), 256, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->collapse_q[cur].as<int4>(),
+                                                     n_in, bvh->nodes4.as<vkrt_bvh4_node>(),
+                                                     bvh->collapse_q[cur ^ 1].as<int4>(), cnt);
+        VKRT_HIP(hipGetLastError());
+        uint32_t next = 0;
+        VKRT_HIP(hipMemcpyAsync(&next, cnt + 1, 4, hipMemcpyDeviceToHost, s));
+        VKRT_HIP(hipStreamSynchronize(s));
+        n_in = next;
+        cur ^= 1;
+    }
+    uint32_t host_cnt[4];
+    VKRT_HIP(hipMemcpyAsync(host_cnt, cnt, sizeof host_cnt, hipMemcpyDeviceToHost, s));
+    VKRT_HIP(hipStreamSynchronize(s));
+    bvh->n_nodes4 = host_cnt[0];
+    bvh->stack4 = host_cnt[2];
+    bvh->root4 = 0;
+    VKRT_REQUIRE(bvh->n_nodes4 <= cap, "BVH4 collapse overflow");
+    bvh->wide = bvh->stack4 <= (uint32_t)STACK_SIZE;
+}
+
 static void refresh_info(vkrt_bvh *bvh, hipStream_t s)
 {
     bvh->info.n_tri = bvh->n_tri;
     bvh->info.n_nodes = bvh->n_nodes;
-    bvh->info.device_bytes = bvh->nodes.bytes + bvh->tris.bytes;
+    bvh->info.device_bytes = bvh->nodes.bytes + bvh->tris.bytes + bvh->nodes4.bytes;
+    bvh->info.n_wide_nodes = bvh->wide ? bvh->n_nodes4 : 0;
+    bvh->info.wide_stack = bvh->stack4;
     compute_stats(bvh, s);
 }
 
@@ -425,6 +579,8 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
         uint32_t leaf = params && params->max_leaf_size ? params->max_leaf_size : 4;
         VKRT_REQUIRE(leaf >= 1 && leaf <= (uint32_t)MAX_LEAF, "max_leaf_size must be in 1..8");
         VKRT_REQUIRE(!params || params->builder == VKRT_BUILDER_LBVH, "builder not supported");
+        VKRT_REQUIRE(!params || params->width == 0 || params->width == 2 || params->width == 4,
+                     "width must be 0 (auto), 2 or 4");
         auto t_start = std::chrono::steady_clock::now();
         auto bvh = std::make_unique<vkrt_bvh>();
         bvh->device = ctx->device;
@@ -432,6 +588,7 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
         bvh->n_vert = n_vert;
         bvh->max_leaf = leaf;
         bvh->empty = n_tri == 0;
+        bvh->allow_wide = !params || params->width != 2;
         if(n_tri)
         {
             const size_t pos_bytes = (size_t)(n_vert - 1) * stride_bytes + 12;
@@ -443,6 +600,7 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
                                     hipMemcpyDeviceToDevice, ctx->stream));
             build_topology(ctx, bvh.get(), dpos, stride_bytes, bvh->indices.as<uint32_t>());
             build_bounds(ctx, bvh.get());
+            build_wide(ctx, bvh.get());
             VKRT_HIP(hipStreamSynchronize(ctx->stream));
         }
         bvh->info.build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
@@ -471,6 +629,7 @@ int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_
                                                 bvh->pre.as<float>(), box, cb, bvh->err.as<uint32_t>());
         k_sorted_boxes<<<div_up(n, 256), 256, 0, s>>>(bvh->order.as<uint32_t>(), n, box, bvh->tbox.as<float>());
         build_bounds(ctx, bvh);
+        build_wide(ctx, bvh);
         VKRT_HIP(hipStreamSynchronize(s));
         refresh_info(bvh, s);
     });

@@ -16,6 +16,15 @@ struct vkrt_bvh_node
     int4 d;  // child refs: >= 0 internal node, < 0 leaf ~((first << 3) | (count - 1))
 };
 
+//! 4-wide node collapsed from the BVH2 (128 B): children's boxes by plane, so one node visit is seven
+//! 16-B loads and four slab tests; an unused slot has child ref EMPTY_CHILD (its box is never read)
+struct vkrt_bvh4_node
+{
+    float4 lox, hix, loy, hiy, loz, hiz;
+    int4 child;// >= 0 BVH4 node, < 0 leaf (the BVH2 leaf encoding), EMPTY_CHILD unused
+    int4 pad;
+};
+
 struct vkrt_bvh
 {
     int device = 0;
@@ -23,6 +32,11 @@ struct vkrt_bvh
     int32_t root_ref = 0;
     bool empty = true;
     vkrt::device_buffer nodes, tris;
+    // the 4-wide collapse used by traversal when its worst-case stack fits (wide == true)
+    vkrt::device_buffer nodes4, collapse_q[2], collapse_cnt;
+    int32_t root4 = 0;
+    uint32_t n_nodes4 = 0, stack4 = 0;
+    bool wide = false, allow_wide = true;
     vkrt::device_buffer indices, order, child, range, parent, nbox, tbox, flags, pre, err;
     vkrt_bvh_info info{};
 };
@@ -36,5 +50,6 @@ constexpr int STACK_SIZE = 64;      // traversal stack entries per ray; the buil
 constexpr float BOX_PAD = 0x1p-16f; // relative growth of every stored box coordinate
 constexpr float T_PAD = 0x1p-16f;   // relative growth of every slab interval
 constexpr uint32_t TRI_TWO_SIDED = 1u;// flag bit in a leaf triangle record: never culled as a back face
+constexpr int32_t EMPTY_CHILD = 0x7fffffff;// unused BVH4 child slot (no node index reaches it)
 
 }// namespace vkrt

@@ -183,7 +183,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_zero(uint32_t *c) { c[threadIdx
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                        const float4 *__restrict__ tris, int32_t root, bool empty,
-                                                       pt_buffers b, int in)
+                                                       const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
+                                                       bool wide, pt_buffers b, int in)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
@@ -196,7 +197,10 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__re
         ray_setup r = make_ray_setup(b.ray_o[p], b.ray_d[p]);
         const bool in_air = (b.state[p].y & F_MEDIA_MASK) == 0u;
         // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
-        traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim, in_air);
+        if(wide)
+            traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem + threadIdx.x, t, u, v, prim, in_air);
+        else
+            traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim, in_air);
     }
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
 }
@@ -711,7 +715,8 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
                                                                           : blocks;
             k_pt_zero<<<1, 1, 0, s>>>(b.counter + (in ^ 1));
             k_pt_trace<<<grid, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref,
-                                                 bvh->empty, b, in);
+                                                 bvh->empty, bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide,
+                                                 b, in);
             k_pt_shade<<<grid, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
             in ^= 1;
         }

@@ -18,6 +18,7 @@ namespace vkrt
 template<bool ANY, int BLOCK, int LDS_DEPTH>
 __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                  const float4 *__restrict__ tris, int32_t root, bool empty,
+                                                 const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide,
                                                  const float4 *__restrict__ rays, uint32_t n, void *__restrict__ out)
 {
     __shared__ int32_t smem[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
@@ -29,7 +30,8 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict
     if(!empty)
     {
         ray_setup r = setup_ray(rays, i);
-        hit = traverse<ANY, BLOCK, LDS_DEPTH>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim);
+        hit = wide ? traverse4<ANY, BLOCK, LDS_DEPTH>(nodes4, tris, root4, r, smem + threadIdx.x, t, u, v, prim)
+                   : traverse<ANY, BLOCK, LDS_DEPTH>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim);
     }
     if(ANY) static_cast<uint8_t *>(out)[i] = hit ? 1 : 0;
     else
@@ -42,7 +44,8 @@ static void launch_one(hipStream_t s, const vkrt_bvh *bvh, const float4 *rays, u
 {
     k_trace<ANY, BLOCK, LDS><<<div_up(n, BLOCK), BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(),
                                                                  bvh->tris.as<float4>(), bvh->root_ref,
-                                                                 bvh->empty, rays, n, out);
+                                                                 bvh->empty, bvh->nodes4.as<vkrt_bvh4_node>(),
+                                                                 bvh->root4, bvh->wide, rays, n, out);
 }
 
 template<bool ANY, int BLOCK>

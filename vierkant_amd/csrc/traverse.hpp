@@ -1,9 +1,13 @@
 // traverse.hpp - device-side BVH traversal shared by the trace, batched and path-tracer kernels.
 //
-// One ray per lane, front-to-back descent of the BVH2 (both children's boxes live in the parent, so a
-// node visit is four 16-B loads and two slab tests). The per-lane stack keeps its top LDS_DEPTH entries
-// in LDS, slot-major with a stride of the block size (lane i of a wave touches bank i: conflict-free),
-// and only deeper entries in a private array, which the compiler places in scratch memory.
+// One ray per lane, front-to-back descent. traverse() walks the BVH2 (both children's boxes live in the
+// parent: a node visit is four 16-B loads and two slab tests); traverse4() walks the 4-wide collapse
+// (seven 16-B loads and four slab tests per visit, half the levels), pushing the hit children farthest
+// first. Stack entries are child refs only: storing entry distances for pop-time culling doubled the LDS
+// stack and lost more to occupancy than the culling saved (3.79 vs 3.90 Grays/s, heightfield scene).
+// The per-lane stack keeps its top LDS_DEPTH entries in LDS, slot-major with a stride of the block size
+// (lane i of a wave touches bank i: conflict-free), and only deeper entries in a private array, which the
+// compiler places in scratch memory.
 #pragma once
 
 #include "bvh_types.hpp"
@@ -132,6 +136,113 @@ __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes
                 }
             }
         }
+        if(sp == 0) break;
+        --sp;
+        ref = (LDS_DEPTH > 0 && sp < LDS_DEPTH) ? lds[sp * BLOCK] : spill[sp - LDS_DEPTH];
+    }
+    return best_prim != VKRT_INVALID_PRIM;
+}
+
+//! one leaf's triangles (shared by both traversals); returns true when ANY and a hit was found
+template<bool ANY>
+__device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32_t ref, const ray_setup &r,
+                                          float &best_t, float &best_u, float &best_v, uint32_t &best_prim,
+                                          bool cull_back)
+{
+    uint32_t first, count;
+    leaf_decode(ref, first, count);
+    for(uint32_t k = 0; k < count; ++k)
+    {
+        const float4 *tr = tris + 3 * (size_t)(first + k);
+        float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+        float t, u, v;
+        const bool cull = cull_back && !(__float_as_uint(p2.w) & TRI_TWO_SIDED);
+        if(vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y}, vec3{p1.z, p1.w, p2.x}, t,
+                                 u, v, cull) &&
+           t > r.tmin)
+        {
+            const uint32_t prim = __float_as_uint(p2.y);
+            if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim))
+            {
+                best_t = t;
+                best_u = u;
+                best_v = v;
+                best_prim = prim;
+                if(ANY) return true;
+            }
+        }
+    }
+    return false;
+}
+
+//! the same query through the 4-wide collapse. `lds` points at this lane's column of a [LDS_DEPTH][BLOCK]
+//! array. A child is entered iff its padded interval meets [tmin, best_t] exactly as in traverse(), so the
+//! result is the same (the closest-hit rule is independent of the visiting order).
+template<bool ANY, int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ bool traverse4(const vkrt_bvh4_node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                          int32_t root, const ray_setup &r, int32_t *lds, float &best_t,
+                                          float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false)
+{
+    constexpr int SPILL = STACK_SIZE - LDS_DEPTH > 0 ? STACK_SIZE - LDS_DEPTH : 1;
+    int32_t spill[SPILL];
+    int sp = 0;
+    int32_t ref = root;
+    best_t = r.tmax;
+    best_prim = VKRT_INVALID_PRIM;
+    while(true)
+    {
+        if(ref >= 0)
+        {
+            const vkrt_bvh4_node &nd = nodes[ref];
+            const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
+            const int4 cr = nd.child;
+            float k0, k1, k2, k3, f;
+            slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
+            bool v0 = cr.x != EMPTY_CHILD && k0 <= f && f >= r.tmin && k0 <= best_t;
+            slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
+            bool v1 = cr.y != EMPTY_CHILD && k1 <= f && f >= r.tmin && k1 <= best_t;
+            slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
+            bool v2 = cr.z != EMPTY_CHILD && k2 <= f && f >= r.tmin && k2 <= best_t;
+            slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
+            bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= best_t;
+            int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+            // sort (valid first, then nearest first): 5 compare-exchanges
+            auto cx = [](bool &va, float &ka, int32_t &ca, bool &vb, float &kb, int32_t &cb) {
+                const bool sw = vb && (!va || kb < ka);
+                if(sw)
+                {
+                    const bool tv = va;
+                    va = vb;
+                    vb = tv;
+                    const float tk = ka;
+                    ka = kb;
+                    kb = tk;
+                    const int32_t tc = ca;
+                    ca = cb;
+                    cb = tc;
+                }
+            };
+            cx(v0, k0, c0, v1, k1, c1);
+            cx(v2, k2, c2, v3, k3, c3);
+            cx(v0, k0, c0, v2, k2, c2);
+            cx(v1, k1, c1, v3, k3, c3);
+            cx(v1, k1, c1, v2, k2, c2);
+            if(v0)
+            {
+                auto push = [&](int32_t c) {
+                    if(LDS_DEPTH > 0 && sp < LDS_DEPTH) lds[sp * BLOCK] = c;
+                    else spill[sp - LDS_DEPTH] = c;
+                    ++sp;
+                };
+                if(v3) push(c3);
+                if(v2) push(c2);
+                if(v1) push(c1);
+                ref = c0;
+                continue;
+            }
+        }
+        else if(test_leaf<ANY>(tris, ref, r, best_t, best_u, best_v, best_prim, cull_back))
+            return true;
         if(sp == 0) break;
         --sp;
         ref = (LDS_DEPTH > 0 && sp < LDS_DEPTH) ? lds[sp * BLOCK] : spill[sp - LDS_DEPTH];

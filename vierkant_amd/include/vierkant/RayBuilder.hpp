@@ -61,6 +61,7 @@ public:
     {
         uint32_t builder = VKRT_BUILDER_LBVH;
         uint32_t max_leaf_size = 4;
+        uint32_t width = 0;// 0: 4-wide traversal layout when it fits, 2: binary tree only
     };
 
     struct timings_t
@@ -101,6 +102,7 @@ public:
         vkrt_bvh_params p{};
         p.builder = params.builder;
         p.max_leaf_size = params.max_leaf_size;
+        p.width = params.width;
         vkrt_bvh *h = nullptr;
         m_ctx->check(vkrt_bvh_build(m_ctx->handle(), mesh.positions, mesh.num_vertices, mesh.stride, mesh.indices,
                                     mesh.num_indices / 3, &p, mesh.device_pointers ? VKRT_DEVICE_PTRS : 0, &h));
