@@ -388,6 +388,7 @@ def main():
             "n_tri": int(len(tris)),
             "rays_per_step": int(rays_per_step),
             "bvh": {"builder": "lbvh", "max_leaf": 4, "nodes": int(info.n_nodes), "depth": int(info.depth),
+                    "wide_nodes": int(info.n_wide_nodes), "wide_stack_bound": int(info.wide_stack),
                     "sah_cost": round(float(info.sah_cost), 2), "build_ms": round(float(info.build_ms), 2),
                     "build_wall_ms": round(build_wall_ms, 2)},
             "primary_ms": round(float(np.mean(prim_ms)), 4),
