@@ -308,23 +308,22 @@ def main():
     ev = vk.HipEvents(5)
 
     if args.sweep:
-        for block in (64, 128, 256):
-            for lds in (0, 8, 16, 32):
-                ctx.set_trace_config(block, lds)
-                ctx.trace_closest(bvh, d_primary, out=d_hits_p)
-                times = []
-                for _ in range(5):
-                    ev.record(0, stream)
-                    ctx.trace_closest(bvh, d_primary, out=d_hits_p, flags=flags)
-                    ev.record(1, stream)
-                    ctx.trace_closest(bvh, d_bounce, out=d_hits_b, flags=flags)
-                    ev.record(2, stream)
-                    ev.sync(2)
-                    times.append((ev.ms(0, 1), ev.ms(1, 2)))
-                p_ms, b_ms = np.median([t[0] for t in times]), np.median([t[1] for t in times])
-                print(json.dumps({"block": block, "lds": lds, "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4),
-                                  "grays_per_s": round(rays_per_step_of(primary, bounce) / ((p_ms + b_ms) * 1e-3) / 1e9, 4)}),
-                      flush=True)
+        for block, lds in [(b, l) for b in (64, 128, 256) for l in (0, 8, 16, 32)]:
+            ctx.set_trace_config(block, lds)
+            ctx.trace_closest(bvh, d_primary, out=d_hits_p)
+            times = []
+            for _ in range(5):
+                ev.record(0, stream)
+                ctx.trace_closest(bvh, d_primary, out=d_hits_p, flags=flags)
+                ev.record(1, stream)
+                ctx.trace_closest(bvh, d_bounce, out=d_hits_b, flags=flags)
+                ev.record(2, stream)
+                ev.sync(2)
+                times.append((ev.ms(0, 1), ev.ms(1, 2)))
+            p_ms, b_ms = np.median([t[0] for t in times]), np.median([t[1] for t in times])
+            print(json.dumps({"block": block, "lds": lds, "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4),
+                              "grays_per_s": round(rays_per_step_of(primary, bounce) / ((p_ms + b_ms) * 1e-3) / 1e9, 4)}),
+                  flush=True)
         return
 
     def step(timed):

@@ -153,9 +153,10 @@ def test_trace_configs_give_identical_results(ctx, icosphere_case):
             for block in (64, 128, 256):
                 for lds in (0, 8, 16, 32):
                     ctx.set_trace_config(block, lds)
-                    assert_hits_equal(ctx.trace_closest(bvh, rays), want, f"width={width} block={block} lds={lds}")
+                    tag = f"width={width} block={block} lds={lds}"
+                    assert_hits_equal(ctx.trace_closest(bvh, rays), want, tag)
                     occ = ctx.trace_any(bvh, rays)
-                    assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8))
+                    assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8)), tag
             bvh.close()
         with pytest.raises(vk.VkrtError):
             ctx.set_trace_config(96, 16)

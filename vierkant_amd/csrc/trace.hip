@@ -49,14 +49,15 @@ static void launch_one(hipStream_t s, const vkrt_bvh *bvh, const float4 *rays, u
 }
 
 template<bool ANY, int BLOCK>
-static void launch_block(hipStream_t s, uint32_t lds, const vkrt_bvh *bvh, const float4 *rays, uint32_t n, void *out)
+static void launch_block(vkrt_ctx *ctx, const vkrt_trace_config &cfg, const vkrt_bvh *bvh, const float4 *rays,
+                         uint32_t n, void *out)
 {
-    switch(lds)
+    switch(cfg.lds_stack_depth)
     {
-        case 0: launch_one<ANY, BLOCK, 0>(s, bvh, rays, n, out); break;
-        case 8: launch_one<ANY, BLOCK, 8>(s, bvh, rays, n, out); break;
-        case 16: launch_one<ANY, BLOCK, 16>(s, bvh, rays, n, out); break;
-        case 32: launch_one<ANY, BLOCK, 32>(s, bvh, rays, n, out); break;
+        case 0: launch_one<ANY, BLOCK, 0>(ctx->stream, bvh, rays, n, out); break;
+        case 8: launch_one<ANY, BLOCK, 8>(ctx->stream, bvh, rays, n, out); break;
+        case 16: launch_one<ANY, BLOCK, 16>(ctx->stream, bvh, rays, n, out); break;
+        case 32: launch_one<ANY, BLOCK, 32>(ctx->stream, bvh, rays, n, out); break;
         default: throw error(VKRT_E_INVALID_ARG, "lds_stack_depth must be 0, 8, 16 or 32");
     }
 }
@@ -68,19 +69,19 @@ bool valid_trace_config(const vkrt_trace_config &c)
             c.lds_stack_depth == 32);
 }
 
-void launch_trace(hipStream_t s, const vkrt_trace_config &cfg, bool any, const vkrt_bvh *bvh, const float4 *rays,
+void launch_trace(vkrt_ctx *ctx, const vkrt_trace_config &cfg, bool any, const vkrt_bvh *bvh, const float4 *rays,
                   uint32_t n, void *out)
 {
     if(!n) return;
     VKRT_REQUIRE(valid_trace_config(cfg), "unsupported trace configuration");
     switch(cfg.block_size)
     {
-        case 64: any ? launch_block<true, 64>(s, cfg.lds_stack_depth, bvh, rays, n, out)
-                     : launch_block<false, 64>(s, cfg.lds_stack_depth, bvh, rays, n, out); break;
-        case 128: any ? launch_block<true, 128>(s, cfg.lds_stack_depth, bvh, rays, n, out)
-                      : launch_block<false, 128>(s, cfg.lds_stack_depth, bvh, rays, n, out); break;
-        default: any ? launch_block<true, 256>(s, cfg.lds_stack_depth, bvh, rays, n, out)
-                     : launch_block<false, 256>(s, cfg.lds_stack_depth, bvh, rays, n, out); break;
+        case 64: any ? launch_block<true, 64>(ctx, cfg, bvh, rays, n, out)
+                     : launch_block<false, 64>(ctx, cfg, bvh, rays, n, out); break;
+        case 128: any ? launch_block<true, 128>(ctx, cfg, bvh, rays, n, out)
+                      : launch_block<false, 128>(ctx, cfg, bvh, rays, n, out); break;
+        default: any ? launch_block<true, 256>(ctx, cfg, bvh, rays, n, out)
+                     : launch_block<false, 256>(ctx, cfg, bvh, rays, n, out); break;
     }
     VKRT_HIP(hipGetLastError());
 }
@@ -101,7 +102,7 @@ int vkrt_trace_closest(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays,
         if(!n) return;
         const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
         vkrt_hit *dh = output(ctx, 2, out, n, flags);
-        launch_trace(ctx->stream, ctx->trace_cfg, false, bvh, reinterpret_cast<const float4 *>(dr), n, dh);
+        launch_trace(ctx, ctx->trace_cfg, false, bvh, reinterpret_cast<const float4 *>(dr), n, dh);
         finish_output(ctx, 2, out, n, flags);
         finish_call(ctx, flags);
     });
@@ -117,7 +118,7 @@ int vkrt_trace_any(vkrt_ctx *ctx, const vkrt_bvh *bvh, const vkrt_ray *rays, uin
         if(!n) return;
         const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
         uint8_t *docc = output(ctx, 2, occluded, n, flags);
-        launch_trace(ctx->stream, ctx->trace_cfg, true, bvh, reinterpret_cast<const float4 *>(dr), n, docc);
+        launch_trace(ctx, ctx->trace_cfg, true, bvh, reinterpret_cast<const float4 *>(dr), n, docc);
         finish_output(ctx, 2, occluded, n, flags);
         finish_call(ctx, flags);
     });

@@ -175,78 +175,111 @@ __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32
     return false;
 }
 
-//! the same query through the 4-wide collapse. `lds` points at this lane's column of a [LDS_DEPTH][BLOCK]
-//! array. A child is entered iff its padded interval meets [tmin, best_t] exactly as in traverse(), so the
-//! result is the same (the closest-hit rule is independent of the visiting order).
+//! per-ray traversal state of the 4-wide walk (the stack lives in LDS + a private spill array)
+struct trav_state
+{
+    ray_setup r;
+    float best_t, best_u, best_v;
+    uint32_t best_prim;
+    int32_t ref;
+    int sp;
+};
+
+__device__ __forceinline__ void trav_init(trav_state &s, const ray_setup &r, int32_t root)
+{
+    s.r = r;
+    s.best_t = r.tmax;
+    s.best_u = 0.f;
+    s.best_v = 0.f;
+    s.best_prim = VKRT_INVALID_PRIM;
+    s.ref = root;
+    s.sp = 0;
+}
+
+//! one visit of the 4-wide walk: a node (children whose padded interval meets [tmin, best_t] are entered
+//! nearest first, the others pushed farthest first) or a leaf, then a pop. Returns true when the ray is
+//! finished (stack empty, or the first hit when ANY). `lds` is this lane's column of a [LDS_DEPTH][BLOCK]
+//! array; `spill` holds entries past LDS_DEPTH. A child is entered under the same condition as in
+//! traverse(), and the closest-hit rule does not depend on the visiting order, so the result is the same.
+template<bool ANY, int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                      trav_state &s, int32_t *lds, int32_t *spill, bool cull_back)
+{
+    const ray_setup &r = s.r;
+    if(s.ref >= 0)
+    {
+        const vkrt_bvh4_node &nd = nodes[s.ref];
+        const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
+        const int4 cr = nd.child;
+        float k0, k1, k2, k3, f;
+        slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
+        bool v0 = cr.x != EMPTY_CHILD && k0 <= f && f >= r.tmin && k0 <= s.best_t;
+        slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
+        bool v1 = cr.y != EMPTY_CHILD && k1 <= f && f >= r.tmin && k1 <= s.best_t;
+        slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
+        bool v2 = cr.z != EMPTY_CHILD && k2 <= f && f >= r.tmin && k2 <= s.best_t;
+        slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
+        bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= s.best_t;
+        int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+        // sort (valid first, then nearest first): 5 compare-exchanges
+        auto cx = [](bool &va, float &ka, int32_t &ca, bool &vb, float &kb, int32_t &cb) {
+            const bool sw = vb && (!va || kb < ka);
+            if(sw)
+            {
+                const bool tv = va;
+                va = vb;
+                vb = tv;
+                const float tk = ka;
+                ka = kb;
+                kb = tk;
+                const int32_t tc = ca;
+                ca = cb;
+                cb = tc;
+            }
+        };
+        cx(v0, k0, c0, v1, k1, c1);
+        cx(v2, k2, c2, v3, k3, c3);
+        cx(v0, k0, c0, v2, k2, c2);
+        cx(v1, k1, c1, v3, k3, c3);
+        cx(v1, k1, c1, v2, k2, c2);
+        if(v0)
+        {
+            auto push = [&](int32_t c) {
+                if(LDS_DEPTH > 0 && s.sp < LDS_DEPTH) lds[s.sp * BLOCK] = c;
+                else spill[s.sp - LDS_DEPTH] = c;
+                ++s.sp;
+            };
+            if(v3) push(c3);
+            if(v2) push(c2);
+            if(v1) push(c1);
+            s.ref = c0;
+            return false;
+        }
+    }
+    else if(test_leaf<ANY>(tris, s.ref, r, s.best_t, s.best_u, s.best_v, s.best_prim, cull_back))
+        return true;
+    if(s.sp == 0) return true;
+    --s.sp;
+    s.ref = (LDS_DEPTH > 0 && s.sp < LDS_DEPTH) ? lds[s.sp * BLOCK] : spill[s.sp - LDS_DEPTH];
+    return false;
+}
+
+constexpr int spill_entries(int lds_depth) { return STACK_SIZE - lds_depth > 0 ? STACK_SIZE - lds_depth : 1; }
+
+//! the whole 4-wide walk for one ray (see step4)
 template<bool ANY, int BLOCK, int LDS_DEPTH>
 __device__ __forceinline__ bool traverse4(const vkrt_bvh4_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                           int32_t root, const ray_setup &r, int32_t *lds, float &best_t,
                                           float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false)
 {
-    constexpr int SPILL = STACK_SIZE - LDS_DEPTH > 0 ? STACK_SIZE - LDS_DEPTH : 1;
-    int32_t spill[SPILL];
-    int sp = 0;
-    int32_t ref = root;
-    best_t = r.tmax;
-    best_prim = VKRT_INVALID_PRIM;
-    while(true)
-    {
-        if(ref >= 0)
-        {
-            const vkrt_bvh4_node &nd = nodes[ref];
-            const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
-            const int4 cr = nd.child;
-            float k0, k1, k2, k3, f;
-            slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
-            bool v0 = cr.x != EMPTY_CHILD && k0 <= f && f >= r.tmin && k0 <= best_t;
-            slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
-            bool v1 = cr.y != EMPTY_CHILD && k1 <= f && f >= r.tmin && k1 <= best_t;
-            slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
-            bool v2 = cr.z != EMPTY_CHILD && k2 <= f && f >= r.tmin && k2 <= best_t;
-            slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
-            bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= best_t;
-            int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
-            // sort (valid first, then nearest first): 5 compare-exchanges
-            auto cx = [](bool &va, float &ka, int32_t &ca, bool &vb, float &kb, int32_t &cb) {
-                const bool sw = vb && (!va || kb < ka);
-                if(sw)
-                {
-                    const bool tv = va;
-                    va = vb;
-                    vb = tv;
-                    const float tk = ka;
-                    ka = kb;
-                    kb = tk;
-                    const int32_t tc = ca;
-                    ca = cb;
-                    cb = tc;
-                }
-            };
-            cx(v0, k0, c0, v1, k1, c1);
-            cx(v2, k2, c2, v3, k3, c3);
-            cx(v0, k0, c0, v2, k2, c2);
-            cx(v1, k1, c1, v3, k3, c3);
-            cx(v1, k1, c1, v2, k2, c2);
-            if(v0)
-            {
-                auto push = [&](int32_t c) {
-                    if(LDS_DEPTH > 0 && sp < LDS_DEPTH) lds[sp * BLOCK] = c;
-                    else spill[sp - LDS_DEPTH] = c;
-                    ++sp;
-                };
-                if(v3) push(c3);
-                if(v2) push(c2);
-                if(v1) push(c1);
-                ref = c0;
-                continue;
-            }
-        }
-        else if(test_leaf<ANY>(tris, ref, r, best_t, best_u, best_v, best_prim, cull_back))
-            return true;
-        if(sp == 0) break;
-        --sp;
-        ref = (LDS_DEPTH > 0 && sp < LDS_DEPTH) ? lds[sp * BLOCK] : spill[sp - LDS_DEPTH];
-    }
+    int32_t spill[spill_entries(LDS_DEPTH)];
+    trav_state s;
+    trav_init(s, r, root);
+    while(!step4<ANY, BLOCK, LDS_DEPTH>(nodes, tris, s, lds, spill, cull_back)) {}
+    best_t = s.best_t;
+    best_u = s.best_u;
+    best_v = s.best_v;
+    best_prim = s.best_prim;
     return best_prim != VKRT_INVALID_PRIM;
 }
 
