@@ -20,6 +20,7 @@ sys.path.insert(0, ROOT)
 
 MI355X_HBM_GBPS = 8000.0
 CLOSEST_BYTES_PER_RAY = 48   # 32 B ray in + 16 B hit out
+MAX_LEAF = 2                 # triangles per BVH leaf (the library default; fastest with the 4-wide layout)
 ANY_BYTES_PER_RAY = 33       # 32 B ray in + 1 B flag out
 
 
@@ -267,7 +268,7 @@ def main():
 
     log("building BVH")
     t0 = time.perf_counter()
-    bvh = ctx.bvh_build(pos, idx)
+    bvh = ctx.bvh_build(pos, idx, max_leaf_size=MAX_LEAF)
     build_wall_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
 
@@ -386,7 +387,7 @@ def main():
                         "icospheres; 4,194,304 pinhole (2048x2048) + 4,194,304 cosine bounce rays per step",
             "n_tri": int(len(tris)),
             "rays_per_step": int(rays_per_step),
-            "bvh": {"builder": "lbvh", "max_leaf": 4, "nodes": int(info.n_nodes), "depth": int(info.depth),
+            "bvh": {"builder": "lbvh", "max_leaf": MAX_LEAF, "nodes": int(info.n_nodes), "depth": int(info.depth),
                     "wide_nodes": int(info.n_wide_nodes), "wide_stack_bound": int(info.wide_stack),
                     "sah_cost": round(float(info.sah_cost), 2), "build_ms": round(float(info.build_ms), 2),
                     "build_wall_ms": round(build_wall_ms, 2)},

@@ -193,7 +193,7 @@ enum
 typedef struct vkrt_bvh_params
 {
     uint32_t builder;       /* VKRT_BUILDER_* */
-    uint32_t max_leaf_size; /* triangles per leaf, 1..8 (0 = default) */
+    uint32_t max_leaf_size; /* triangles per leaf, 1..8 (0 = default: 2, fastest with the 4-wide layout) */
     uint32_t width;         /* traversal layout: 0 = 4-wide collapse when its stack bound fits (default),
                              * 2 = the binary tree only. Results do not depend on it. */
     uint32_t reserved[5];

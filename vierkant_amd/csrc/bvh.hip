@@ -576,7 +576,7 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
         VKRT_REQUIRE(stride_bytes >= 12 && stride_bytes % 4 == 0, "stride must be >= 12 and a multiple of 4");
         VKRT_REQUIRE(n_tri == 0 || (positions && indices && n_vert), "null mesh arrays");
         VKRT_REQUIRE(n_tri < (1u << 28), "at most 2^28 - 1 triangles per BVH");
-        uint32_t leaf = params && params->max_leaf_size ? params->max_leaf_size : 4;
+        uint32_t leaf = params && params->max_leaf_size ? params->max_leaf_size : 2;
         VKRT_REQUIRE(leaf >= 1 && leaf <= (uint32_t)MAX_LEAF, "max_leaf_size must be in 1..8");
         VKRT_REQUIRE(!params || params->builder == VKRT_BUILDER_LBVH, "builder not supported");
         VKRT_REQUIRE(!params || params->width == 0 || params->width == 2 || params->width == 4,

@@ -868,7 +868,7 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         }
         vkrt_bvh_params params{};
         params.builder = VKRT_BUILDER_LBVH;
-        params.max_leaf_size = 4;
+        params.max_leaf_size = 2;
         vkrt_bvh *bvh = nullptr;
         int rc = vkrt_bvh_build(pt->ctx, pt->pos.as<float>(), sc->n_vert, 12, pt->idx.as<uint32_t>(), sc->n_tri,
                                 &params, VKRT_DEVICE_PTRS, &bvh);
