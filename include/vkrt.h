@@ -224,6 +224,51 @@ int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_
 int vkrt_bvh_get_info(const vkrt_bvh *bvh, vkrt_bvh_info *out);
 void vkrt_bvh_destroy(vkrt_bvh *bvh);
 
+/* ---------------------------------------------------------------------------------------------
+ * two-level scenes: instances of mesh BVHs with per-instance transforms (the top-level structure of
+ * RayBuilder::build_scene_acceleration, src/RayBuilder.cpp:386-781; entry_t::transform / inv_transform,
+ * include/vierkant/RayBuilder.hpp:24-38)
+ * ------------------------------------------------------------------------------------------- */
+
+typedef struct vkrt_scene vkrt_scene;
+
+typedef struct
+{
+    float object_to_world[12]; /* 3x4 row-major: world = M (p, 1) */
+    float world_to_object[12]; /* its inverse (entry_t::inv_transform); traversal transforms rays with it */
+    uint32_t blas;             /* index into the scene's mesh list */
+    uint32_t custom_index;     /* reported with the hit */
+    uint32_t reserved[2];
+} vkrt_instance;
+
+/* Closest hit of a scene (32 B): t is shared by world and object space (rays are transformed without
+ * renormalizing); prim indexes the instance's mesh; instance = VKRT_INVALID_PRIM on a miss. */
+typedef struct
+{
+    float t, u, v;
+    uint32_t prim;
+    uint32_t instance;
+    uint32_t custom_index;
+    uint32_t reserved[2];
+} vkrt_instance_hit;
+
+/* Fill object_to_world and its inverse (computed in double, rounded once) of an instance. */
+void vkrt_instance_set_transform(vkrt_instance *inst, const float object_to_world[12]);
+/* Build the top level over `instances` of `meshes` (4-wide mesh BVHs). Ties: smallest t, then the lowest
+ * instance, then the lowest triangle. The meshes must outlive the scene; after a mesh refit or a
+ * transform change, call vkrt_scene_set_instances (the top level is rebuilt on the host). */
+int vkrt_scene_create(vkrt_ctx *ctx, vkrt_bvh *const *meshes, uint32_t n_meshes, const vkrt_instance *instances,
+                      uint32_t n_instances, vkrt_scene **out);
+int vkrt_scene_set_instances(vkrt_scene *scene, const vkrt_instance *instances, uint32_t n_instances);
+/* Before the context and its meshes. */
+void vkrt_scene_destroy(vkrt_scene *scene);
+int vkrt_scene_trace_closest(vkrt_ctx *ctx, vkrt_scene *scene, const vkrt_ray *rays, uint32_t n,
+                             vkrt_instance_hit *out, int flags);
+int vkrt_scene_trace_any(vkrt_ctx *ctx, vkrt_scene *scene, const vkrt_ray *rays, uint32_t n, uint8_t *occluded,
+                         int flags);
+/* instance count, top-level nodes, worst-case traversal stack (top level + deepest mesh, at most 96) */
+int vkrt_scene_get_info(const vkrt_scene *scene, uint32_t *n_instances, uint32_t *n_nodes, uint32_t *stack_bound);
+
 /* Launch shape of the traversal kernels: threads per block (64, 128, 256) and traversal-stack entries
  * kept in LDS per ray (0, 8, 16, 32; deeper entries live in scratch memory). Results do not depend on
  * it. No reference counterpart: the Vulkan driver owns the traversal there. */

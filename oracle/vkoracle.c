@@ -396,6 +396,58 @@ void orc_brute_any(const float *tris, uint32_t n_tri, const float *rays8, uint32
     run_jobs(any_job, tris, n_tri, rays8, n_ray, NULL, occluded, nthreads);
 }
 
+/* Two-level scene (the instance level of RayBuilder's top-level structure, src/RayBuilder.cpp:386-781):
+ * each instance has a row-major 3x4 world-to-object matrix W (the reference's entry_t::inv_transform) and a
+ * mesh. The object-space ray is o' = W (o, 1), d' = W (d, 0), each row as ((w0 x + w1 y) + w2 z) + w3,
+ * with d' not normalized, so t is the same parameter in both spaces. A candidate counts iff tmin < t < tmax;
+ * the smallest t wins, then the lowest instance, then the lowest triangle of that instance's mesh.
+ * tris: all meshes' flat triangles (9 floats each) concatenated; tri_offset[m] .. tri_offset[m + 1] is mesh
+ * m. out: 8 words per ray (t, u, v, prim, instance, custom index, 0, 0). */
+void orc_scene_closest(const float *tris, const uint32_t *tri_offset, const float *w2o, const uint32_t *inst_mesh,
+                       const uint32_t *inst_custom, uint32_t n_inst, const float *rays8, uint32_t n_ray, uint32_t *out8)
+{
+    for(uint32_t r = 0; r < n_ray; ++r)
+    {
+        const float *ray = rays8 + 8 * (size_t)r;
+        float best = INFINITY, bu = 0.f, bv = 0.f;
+        uint32_t bp = 0xFFFFFFFFu, bi = 0xFFFFFFFFu;
+        for(uint32_t k = 0; k < n_inst; ++k)
+        {
+            const float *w = w2o + 12 * (size_t)k;
+            float o[3], d[3];
+            for(int a = 0; a < 3; ++a)
+            {
+                const float *row = w + 4 * a;
+                o[a] = ((row[0] * ray[0] + row[1] * ray[1]) + row[2] * ray[2]) + row[3];
+                d[a] = (row[0] * ray[4] + row[1] * ray[5]) + row[2] * ray[6];
+            }
+            const uint32_t m = inst_mesh[k];
+            for(uint32_t i = tri_offset[m]; i < tri_offset[m + 1]; ++i)
+            {
+                float t, u, v;
+                if(orc_intersect_tri(tris + 9 * (size_t)i, o, d, &t, &u, &v) && t > ray[3] && t < ray[7] &&
+                   (bp == 0xFFFFFFFFu || t < best))
+                {
+                    best = t;
+                    bu = u;
+                    bv = v;
+                    bp = i - tri_offset[m];
+                    bi = k;
+                }
+            }
+        }
+        uint32_t *h = out8 + 8 * (size_t)r;
+        const float tt = bp == 0xFFFFFFFFu ? INFINITY : best;
+        memcpy(h, &tt, 4);
+        memcpy(h + 1, &bu, 4);
+        memcpy(h + 2, &bv, 4);
+        h[3] = bp;
+        h[4] = bi;
+        h[5] = bi == 0xFFFFFFFFu ? 0u : inst_custom[bi];
+        h[6] = h[7] = 0u;
+    }
+}
+
 uint32_t orc_closest_f64(const float *tris, uint32_t n_tri, const float *r, double *bt, double *bu, double *bv)
 {
     double best = INFINITY;

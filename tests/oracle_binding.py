@@ -24,6 +24,7 @@ _SIG = {
     "orc_compute_aabb": (None, [_vp, _u32, _vp]),
     "orc_bounding_sphere": (None, [_vp, _u32, _vp]),
     "orc_centroid": (None, [_vp, _u32, _vp]),
+    "orc_scene_closest": (None, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp]),
     "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_closest_f64": (_u32, [_vp, _u32, _vp, _dp, _dp, _dp]),
@@ -128,6 +129,24 @@ def compute_aabb(points):
     points, pp = _f(points)
     out = np.zeros(6, dtype=np.float32)
     lib().orc_compute_aabb(pp, len(points) // 3 if points.ndim == 1 else len(points), out.ctypes.data)
+    return out
+
+
+def scene_closest(mesh_tris, instances, rays):
+    """mesh_tris: list of flat (n, 9) triangle arrays; instances: INSTANCE_DTYPE records; returns
+    INSTANCE_HIT_DTYPE records"""
+    from vierkant_amd import INSTANCE_HIT_DTYPE
+    tris = np.ascontiguousarray(np.concatenate([np.asarray(t, np.float32).reshape(-1, 9) for t in mesh_tris]) if mesh_tris
+                                else np.zeros((0, 9), np.float32))
+    off = np.zeros(len(mesh_tris) + 1, np.uint32)
+    off[1:] = np.cumsum([len(np.asarray(t).reshape(-1, 9)) for t in mesh_tris])
+    w2o = np.ascontiguousarray(instances["world_to_object"], np.float32)
+    mesh = np.ascontiguousarray(instances["blas"], np.uint32)
+    custom = np.ascontiguousarray(instances["custom_index"], np.uint32)
+    r = np.ascontiguousarray(rays)
+    out = np.zeros(len(r), INSTANCE_HIT_DTYPE)
+    lib().orc_scene_closest(tris.ctypes.data, off.ctypes.data, w2o.ctypes.data, mesh.ctypes.data, custom.ctypes.data,
+                            len(instances), r.ctypes.data, len(r), out.ctypes.data)
     return out
 
 

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 import oracle_binding as orc
+from vierkant_amd import workloads as W
 
 TRI = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], dtype=np.float32)
 
@@ -288,3 +289,72 @@ def test_centroid_sequential_mean():
     pts = np.array([[1, 2, 3], [3, 2, 1], [2, 2, 2]], np.float32)
     assert orc.centroid(pts).tolist() == [2.0, 2.0, 2.0]
     assert orc.centroid(np.zeros((0, 3), np.float32)).tolist() == [0.0, 0.0, 0.0]
+
+
+# ---- two-level scene (instances with world-to-object transforms) ------------------------------------------
+
+def _instances(mats, meshes, customs=None):
+    from vierkant_amd import INSTANCE_DTYPE
+    inst = np.zeros(len(mats), INSTANCE_DTYPE)
+    for k, (o2w, w2o) in enumerate(mats):
+        inst[k]["object_to_world"] = np.asarray(o2w, np.float32).reshape(12)
+        inst[k]["world_to_object"] = np.asarray(w2o, np.float32).reshape(12)
+        inst[k]["blas"] = meshes[k]
+        inst[k]["custom_index"] = 100 + k if customs is None else customs[k]
+    return inst
+
+
+def _affine(scale=1.0, t=(0.0, 0.0, 0.0)):
+    m = np.zeros((3, 4), np.float64)
+    m[:, :3] = np.eye(3) * scale
+    m[:, 3] = t
+    w = np.zeros((3, 4), np.float64)
+    w[:, :3] = np.eye(3) / scale
+    w[:, 3] = -np.asarray(t) / scale
+    return m, w
+
+
+def _small_mesh(seed):
+    rng = np.random.default_rng(seed)
+    return (np.round(rng.uniform(-1, 1, (40, 9)) * 8) / 8).astype(np.float32)
+
+
+def _rays(n, seed):
+    rng = np.random.default_rng(seed)
+    o = (np.round(rng.uniform(-3, 3, (n, 3)) * 8) / 8).astype(np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    return W.make_rays(o, W.glm_normalize(d))
+
+
+def test_scene_identity_instance_equals_brute_force():
+    tris = _small_mesh(1)
+    rays = _rays(500, 2)
+    got = orc.scene_closest([tris], _instances([_affine()], [0]), rays)
+    want = orc.brute_closest(tris.reshape(-1), rays)
+    assert got["prim"].tolist() == want["prim"].tolist()
+    assert got["t"].view(np.uint32).tolist() == want["t"].view(np.uint32).tolist()
+    hit = want["prim"] != 0xFFFFFFFF
+    assert (got["instance"][hit] == 0).all() and (got["instance"][~hit] == 0xFFFFFFFF).all()
+    assert (got["custom_index"][hit] == 100).all()
+
+
+def test_scene_ties_go_to_the_lower_instance():
+    tris = _small_mesh(3)
+    rays = _rays(300, 4)
+    got = orc.scene_closest([tris], _instances([_affine(), _affine()], [0, 0]), rays)
+    hit = got["prim"] != 0xFFFFFFFF
+    assert hit.any() and (got["instance"][hit] == 0).all()
+
+
+def test_scene_power_of_two_scale_and_grid_translation_are_exact():
+    """object->world = 2 x + (1, -2, 0.5): world rays map to object rays exactly, so t, u, v equal the
+    brute force over the world-space triangles (whose coordinates are exact too)"""
+    tris = _small_mesh(5)
+    m, w = _affine(2.0, (1.0, -2.0, 0.5))
+    world = tris.reshape(-1, 3, 3) * 2 + np.array([1.0, -2.0, 0.5], np.float32)
+    rays = _rays(600, 6)
+    got = orc.scene_closest([tris], _instances([(m, w)], [0]), rays)
+    want = orc.brute_closest(world.astype(np.float32).reshape(-1), rays)
+    assert got["prim"].tolist() == want["prim"].tolist()
+    assert got["t"].view(np.uint32).tolist() == want["t"].view(np.uint32).tolist()
+    assert got["u"].view(np.uint32).tolist() == want["u"].view(np.uint32).tolist()

@@ -184,6 +184,13 @@ _SIGNATURES = {
     "vkrt_rc_step": (_u32, [_vp]),
     "vkrt_pt_default_cache_config": (None, [ctypes.POINTER(PtCacheConfig)]),
     "vkrt_pt_set_radiance_cache": (_i32, [_vp, _vp, ctypes.POINTER(PtCacheConfig)]),
+    "vkrt_instance_set_transform": (None, [_vp, _vp]),
+    "vkrt_scene_create": (_i32, [_vp, _vp, _u32, _vp, _u32, ctypes.POINTER(_vp)]),
+    "vkrt_scene_set_instances": (_i32, [_vp, _vp, _u32]),
+    "vkrt_scene_destroy": (None, [_vp]),
+    "vkrt_scene_trace_closest": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_scene_trace_any": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
+    "vkrt_scene_get_info": (_i32, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "vkrt_icosphere_size": (_i32, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "vkrt_icosphere": (_i32, [_f32, _u32, _vp, _vp]),
     "vkrt_plane_size": (_i32, [_u32, _u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
@@ -737,6 +744,70 @@ def default_rc_params(**kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+INSTANCE_DTYPE = np.dtype([("object_to_world", np.float32, 12), ("world_to_object", np.float32, 12),
+                           ("blas", np.uint32), ("custom_index", np.uint32), ("reserved", np.uint32, 2)])
+INSTANCE_HIT_DTYPE = np.dtype([("t", np.float32), ("u", np.float32), ("v", np.float32), ("prim", np.uint32),
+                               ("instance", np.uint32), ("custom_index", np.uint32), ("reserved", np.uint32, 2)])
+
+
+def instance(object_to_world, blas, custom_index=0):
+    """one vkrt_instance record: the 3x4 object-to-world matrix and its inverse (vkrt_instance_set_transform)"""
+    rec = np.zeros(1, INSTANCE_DTYPE)
+    m = np.ascontiguousarray(np.asarray(object_to_world, np.float32).reshape(12))
+    lib().vkrt_instance_set_transform(rec.ctypes.data, m.ctypes.data)
+    rec["blas"], rec["custom_index"] = blas, custom_index
+    return rec[0]
+
+
+class Scene:
+    """two-level scene: instances (INSTANCE_DTYPE records) of mesh BVHs"""
+
+    def __init__(self, ctx, meshes, instances):
+        self.ctx, self._meshes = ctx, list(meshes)
+        arr = (_vp * max(1, len(self._meshes)))(*[m.handle for m in self._meshes])
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        self._h = _vp()
+        ctx.check(lib().vkrt_scene_create(ctx.handle, arr, len(self._meshes), inst.ctypes.data, len(inst),
+                                          ctypes.byref(self._h)))
+        ctx._adopt(self)
+
+    def set_instances(self, instances):
+        inst = np.ascontiguousarray(instances, INSTANCE_DTYPE)
+        self.ctx.check(lib().vkrt_scene_set_instances(self._h, inst.ctypes.data, len(inst)))
+
+    def info(self):
+        a, b, c = _u32(), _u32(), _u32()
+        self.ctx.check(lib().vkrt_scene_get_info(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def trace_closest(self, rays, out=None, flags=0):
+        n = _n(rays, 8)
+        if out is None:
+            out = np.zeros(n, INSTANCE_HIT_DTYPE)
+        self.ctx.check(lib().vkrt_scene_trace_closest(self.ctx.handle, self._h, _ptr(rays)[0], n, _ptr(out)[0],
+                                                      _flags([rays, out], flags)))
+        return out
+
+    def trace_any(self, rays, out=None, flags=0):
+        n = _n(rays, 8)
+        if out is None:
+            out = np.zeros(n, np.uint8)
+        self.ctx.check(lib().vkrt_scene_trace_any(self.ctx.handle, self._h, _ptr(rays)[0], n, _ptr(out)[0],
+                                                  _flags([rays, out], flags)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().vkrt_scene_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # -- procedural geometry (host) ----------------------------------------------------------------
