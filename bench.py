@@ -179,6 +179,59 @@ def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64,
             "rmse_curve": curve, "reference_s": round(ref_s, 2)}
 
 
+def instanced_frame(ctx, primary, bounce, verify_rays, reps=5):
+    """the heightfield + spheres scene as a two-level scene (1 heightfield instance + 1,000 instances of
+    one unit IcoSphere(1, 3)): the same primary and bounce rays through the instance-level walk"""
+    import numpy as np
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    meshes, mats, mesh_of = W.heightfield_spheres_instanced()
+    bvhs = [ctx.bvh_build(p, i, max_leaf_size=MAX_LEAF) for p, i in meshes]
+    inst = np.array([vk.instance(m, k, custom_index=j) for j, (m, k) in enumerate(zip(mats, mesh_of))],
+                    dtype=vk.INSTANCE_DTYPE)
+    sc = vk.Scene(ctx, bvhs, inst)
+    ev = vk.HipEvents(3)
+    dp, db = vk.DeviceArray.from_host(ctx, primary), vk.DeviceArray.from_host(ctx, bounce)
+    hp = vk.DeviceArray(ctx, len(primary), vk.INSTANCE_HIT_DTYPE)
+    hb = vk.DeviceArray(ctx, len(bounce), vk.INSTANCE_HIT_DTYPE)
+    sc.trace_closest(dp, out=hp)
+    ts = []
+    for _ in range(reps):
+        ev.record(0, ctx.stream)
+        sc.trace_closest(dp, out=hp, flags=vk.ASYNC)
+        ev.record(1, ctx.stream)
+        sc.trace_closest(db, out=hb, flags=vk.ASYNC)
+        ev.record(2, ctx.stream)
+        ev.sync(2)
+        ts.append((ev.ms(0, 1), ev.ms(1, 2)))
+    p_ms, b_ms = float(np.median([t[0] for t in ts])), float(np.median([t[1] for t in ts]))
+    mismatches = None
+    if verify_rays:
+        import oracle_binding as orc
+        sel = np.sort(np.random.default_rng(3).choice(len(primary), verify_rays, replace=False))
+        tris = [W.flat_triangles(p, i) for p, i in meshes]
+        mismatches = 0
+        for rays, dh in ((primary, hp), (bounce, hb)):
+            got = dh.host()[sel]
+            want = orc.scene_closest(tris, inst, rays[sel])
+            mismatches += int(np.sum((got["prim"] != want["prim"]) | (got["instance"] != want["instance"]) |
+                                     (got["t"].view(np.uint32) != want["t"].view(np.uint32))))
+    n_inst, n_nodes, stack = sc.info()
+    for a in (dp, db, hp, hb):
+        a.free()
+    sc.close()
+    for b in bvhs:
+        b.close()
+    return {"workload": f"heightfield + 1,000 spheres as {n_inst} instances of 2 meshes "
+                        f"({sum(len(i) // 3 for _, i in meshes)} mesh triangles); same primary and bounce rays",
+            "grays_per_s": round((len(primary) + len(bounce)) / ((p_ms + b_ms) * 1e-3) / 1e9, 4),
+            "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4), "top_level_nodes": n_nodes,
+            "stack_bound": stack,
+            "verified_vs_instance_brute_force": None if mismatches is None else
+            {"rays": 2 * verify_rays, "mismatches": mismatches}}
+
+
 def batched_frame(ctx, n_scenes, verify_rays):
     """256 independent scenes (log-uniform 1k-500k triangles, 16k-2M rays each) traced by one persistent
     launch replayed as a hipGraph, after the launch-shape autotuner picked block size, LDS stack depth,
@@ -243,6 +296,7 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
     ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
+    ap.add_argument("--instanced", type=int, default=1, help="trace the two-level version of the scene (0: skip)")
     ap.add_argument("--rc-ref-spp", type=int, default=4096,
                     help="samples per pixel of the radiance-cache leg's reference image (0: skip the leg)")
     ap.add_argument("--rc-frames", type=int, default=128, help="most cache frames the radiance-cache leg traces")
@@ -418,6 +472,9 @@ def main():
     bvh.close()
     for a in (d_primary, d_hits_p, d_bounce, d_shadow, d_hits_b, d_occ):
         a.free()
+    if args.instanced:
+        log("two-level scene")
+        result["config"]["instanced"] = instanced_frame(ctx, primary, bounce, 128 if args.verify_rays else 0)
     if args.pt_spp:
         log("path tracer frame")
         result["config"]["path_tracer"] = path_tracer_frame(ctx, args.pt_spp)

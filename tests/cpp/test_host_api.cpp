@@ -95,6 +95,27 @@ int main()
     }
     if(n_hit < 100) return fail("too few hits");
 
+    // a two-level scene: two translated copies of the icosphere; every hit equals the plain BVH's hit of the
+    // ray moved into the instance's space (translations by integers are exact)
+    {
+        const float m0[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+        const float m1[12] = {1, 0, 0, 4, 0, 1, 0, 0, 0, 0, 1, 0};
+        std::vector<vkrt_instance> inst{RayBuilder::make_instance(m0, 0, 7), RayBuilder::make_instance(m1, 0, 8)};
+        RayBuilder::scene sc(ctx, {as.get()}, inst);
+        std::vector<vkrt_ray> r2 = rays;
+        auto sh = sc.trace_closest(r2);
+        int n_inst_hits = 0;
+        for(size_t i = 0; i < rays.size(); ++i)
+        {
+            if(sh[i].prim == VKRT_INVALID_PRIM) continue;
+            if(sh[i].instance == 0 && (sh[i].prim != hits[i].prim || bits(sh[i].t) != bits(hits[i].t)))
+                return fail("scene instance 0 disagrees with the mesh BVH");
+            n_inst_hits += sh[i].custom_index == 7 + sh[i].instance;
+        }
+        if(n_inst_hits < 100) return fail("scene hits");
+        sc.update(inst);
+    }
+
     // scalar volume tests of the API surface
     AABB box({0.f, 0.f, 0.f}, {1.f, 1.f, 1.f});
     if(box.intersect(Ray({-1.f, 0.5f, 0.5f}, {1.f, 0.f, 0.f})).distance != 1.0f) return fail("AABB entry distance");

@@ -137,6 +137,53 @@ public:
         return occ;
     }
 
+    //! a two-level scene: instances (transform + mesh index) of acceleration structures, the role of the
+    //! reference's top-level structure (src/RayBuilder.cpp:386-781)
+    class scene
+    {
+    public:
+        scene(gpu_context_ptr ctx, const std::vector<const acceleration_structure *> &meshes,
+              const std::vector<vkrt_instance> &instances)
+            : m_ctx(std::move(ctx))
+        {
+            std::vector<vkrt_bvh *> h;
+            for(const auto *m: meshes) h.push_back(m->handle());
+            m_ctx->check(vkrt_scene_create(m_ctx->handle(), h.data(), static_cast<uint32_t>(h.size()), instances.data(),
+                                           static_cast<uint32_t>(instances.size()), &m_scene));
+        }
+        scene(const scene &) = delete;
+        scene &operator=(const scene &) = delete;
+        ~scene() { vkrt_scene_destroy(m_scene); }
+
+        //! new transforms or refitted meshes: the top level is rebuilt
+        void update(const std::vector<vkrt_instance> &instances)
+        {
+            m_ctx->check(vkrt_scene_set_instances(m_scene, instances.data(), static_cast<uint32_t>(instances.size())));
+        }
+
+        std::vector<vkrt_instance_hit> trace_closest(const std::vector<vkrt_ray> &rays) const
+        {
+            std::vector<vkrt_instance_hit> hits(rays.size());
+            m_ctx->check(vkrt_scene_trace_closest(m_ctx->handle(), m_scene, rays.data(),
+                                                  static_cast<uint32_t>(rays.size()), hits.data(), 0));
+            return hits;
+        }
+
+    private:
+        gpu_context_ptr m_ctx;
+        vkrt_scene *m_scene = nullptr;
+    };
+
+    //! an instance of mesh `mesh` with the given 3x4 row-major object-to-world matrix
+    static vkrt_instance make_instance(const float object_to_world[12], uint32_t mesh, uint32_t custom_index = 0)
+    {
+        vkrt_instance in{};
+        vkrt_instance_set_transform(&in, object_to_world);
+        in.blas = mesh;
+        in.custom_index = custom_index;
+        return in;
+    }
+
     [[nodiscard]] timings_t timings() const { return m_timings; }
 
 private:

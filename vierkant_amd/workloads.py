@@ -160,6 +160,31 @@ def heightfield_spheres_scene(grid=512, n_spheres=1000, seed=0, amplitude=0.2, s
     return np.ascontiguousarray(np.concatenate(all_pos)), np.ascontiguousarray(np.concatenate(all_idx))
 
 
+def heightfield_spheres_instanced(grid=512, n_spheres=1000, seed=0, amplitude=0.2, size=10.0):
+    """the heightfield + spheres scene as a two-level scene: the displaced plane as one mesh, the spheres as
+    instances of one IcoSphere(1, 3) scaled by r and translated to their centers (the same draws as
+    heightfield_spheres_scene). Returns (meshes [(pos, idx), (pos, idx)], object-to-world 3x4 matrices,
+    mesh index per instance)."""
+    pos, idx = vk.plane(size, size, grid, grid)
+    pos[:, 1] = (amplitude * fbm(pos[:, 0], pos[:, 2], seed=seed)).astype(np.float32)
+    u = uniform(seed, 0, 4 * n_spheres).reshape(n_spheres, 4).astype(np.float64)
+    radii = (0.01 * 10.0 ** u[:, 0]).astype(np.float32)
+    cx = (-0.5 * size + size * u[:, 1])
+    cz = (-0.5 * size + size * u[:, 2])
+    cy = amplitude * fbm(cx, cz, seed=seed) + u[:, 3]
+    centers = np.stack([cx, cy, cz], axis=1).astype(np.float32)
+    unit = vk.icosphere(1.0, 3)
+    mats = [np.hstack([np.eye(3), np.zeros((3, 1))])]
+    mesh = [0]
+    for s in range(n_spheres):
+        m = np.zeros((3, 4), np.float64)
+        m[:, :3] = np.eye(3) * float(radii[s])
+        m[:, 3] = centers[s]
+        mats.append(m)
+        mesh.append(1)
+    return [(pos, idx), unit], mats, mesh
+
+
 def pinhole_rays(width=2048, height=2048, eye=(0.0, 3.0, 7.0), target=(0.0, 0.0, 0.0), fov_y_deg=50.0):
     """one ray through each pixel center, row-major from the top-left, tmin 0, tmax inf"""
     eye = np.asarray(eye, dtype=np.float64)
