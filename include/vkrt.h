@@ -66,7 +66,8 @@ enum
 {
     VKRT_HOST_PTRS = 0,
     VKRT_DEVICE_PTRS = 1,
-    VKRT_ASYNC = 2
+    VKRT_ASYNC = 2,
+    VKRT_CULL_BACK = 8 /* transmittance rays: skip back faces of one-sided materials */
 };
 
 /* intersection_type (include/vierkant/intersection.hpp:18-23) */
@@ -437,6 +438,13 @@ uint32_t vkrt_pt_current_batch(const vkrt_pt *pt);
 /* Copy the W*H pixel buffer (vkrt_pixel, row-major from the top-left) to `out` (host, or device with flags). */
 int vkrt_pt_read_pixels(vkrt_pt *pt, vkrt_pixel *out, int flags);
 int vkrt_pt_get_stats(const vkrt_pt *pt, vkrt_pt_stats *out);
+/* transmittance_test (shaders/slang/ray/visibility_test.slang:94-165) through the tracer's scene: out4 =
+ * (rgb transmittance, 0) per ray; 0 when an opaque triangle (transmission == 0) is hit in (tmin, tmax),
+ * else the product of transmission * color over every transmissive hit, taken in ascending (t, triangle)
+ * order (the reference's order is the driver's). VKRT_CULL_BACK in flags culls back faces of one-sided
+ * materials (the RAY_FLAG_CULL_BACK_FACING_TRIANGLES call, direct_lighting.slang:354). Materials have no
+ * interior absorption here (attenuation_distance = infinity). */
+int vkrt_pt_trace_transmittance(vkrt_pt *pt, const vkrt_ray *rays, uint32_t n, float *out4, int flags);
 
 /* ---------------------------------------------------------------------------------------------
  * radiance cache: multiresolution hash grid (16 levels x 2^log2_table entries x 2 features, fp16) +

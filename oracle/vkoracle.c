@@ -448,6 +448,74 @@ void orc_scene_closest(const float *tris, const uint32_t *tri_offset, const floa
     }
 }
 
+/* transmittance_test (shaders/slang/ray/visibility_test.slang:94-165) by brute force: 0 if a triangle with
+ * transmission == 0 is hit in (tmin, tmax); otherwise the product of transmission * color (per channel) over
+ * the transmissive hits, multiplied in ascending (t, triangle) order. With `cull`, triangles whose
+ * det = dot(e1, cross(d, e2)) is negative are skipped unless two-sided. Per triangle: trans[i], color[3i..],
+ * two_sided[i]. out4: rgb, 0 per ray. */
+typedef struct
+{
+    float t;
+    uint32_t prim;
+} cand_t;
+
+static int cand_cmp(const void *a, const void *b)
+{
+    const cand_t *x = (const cand_t *)a, *y = (const cand_t *)b;
+    if(x->t < y->t) return -1;
+    if(x->t > y->t) return 1;
+    return x->prim < y->prim ? -1 : (x->prim > y->prim ? 1 : 0);
+}
+
+void orc_transmittance(const float *tris, const float *trans, const float *color, const uint8_t *two_sided,
+                       uint32_t n_tri, const float *rays8, uint32_t n_ray, int cull, float *out4)
+{
+    cand_t *c = (cand_t *)malloc(sizeof(cand_t) * (n_tri ? n_tri : 1));
+    for(uint32_t r = 0; r < n_ray; ++r)
+    {
+        const float *ray = rays8 + 8 * (size_t)r;
+        uint32_t nc = 0;
+        int blocked = 0;
+        for(uint32_t i = 0; i < n_tri && !blocked; ++i)
+        {
+            const float *tr = tris + 9 * (size_t)i;
+            float t, u, v;
+            if(!orc_intersect_tri(tr, ray, ray + 4, &t, &u, &v) || !(t > ray[3] && t < ray[7])) continue;
+            if(cull && !two_sided[i])
+            {
+                const float e1[3] = {tr[3] - tr[0], tr[4] - tr[1], tr[5] - tr[2]};
+                const float e2[3] = {tr[6] - tr[0], tr[7] - tr[1], tr[8] - tr[2]};
+                const float *d = ray + 4;
+                const float p[3] = {d[1] * e2[2] - e2[1] * d[2], d[2] * e2[0] - e2[2] * d[0], d[0] * e2[1] - e2[0] * d[1]};
+                const float det = (e1[0] * p[0] + e1[1] * p[1]) + e1[2] * p[2];
+                if(det < 0.f) continue;
+            }
+            if(!(trans[i] > 0.f))
+            {
+                blocked = 1;
+                break;
+            }
+            c[nc].t = t;
+            c[nc].prim = i;
+            ++nc;
+        }
+        float T[3] = {1.f, 1.f, 1.f};
+        if(blocked) T[0] = T[1] = T[2] = 0.f;
+        else
+        {
+            qsort(c, nc, sizeof(cand_t), cand_cmp);
+            for(uint32_t k = 0; k < nc; ++k)
+                for(int a = 0; a < 3; ++a) T[a] = T[a] * (trans[c[k].prim] * color[3 * (size_t)c[k].prim + a]);
+        }
+        float *o = out4 + 4 * (size_t)r;
+        o[0] = T[0];
+        o[1] = T[1];
+        o[2] = T[2];
+        o[3] = 0.f;
+    }
+    free(c);
+}
+
 uint32_t orc_closest_f64(const float *tris, uint32_t n_tri, const float *r, double *bt, double *bu, double *bv)
 {
     double best = INFINITY;

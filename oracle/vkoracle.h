@@ -38,6 +38,8 @@ int orc_intersect_plane(const float *plane4, const float *o, const float *d, flo
 void orc_compute_aabb(const float *points, uint32_t n, float *out6);
 void orc_bounding_sphere(const float *points, uint32_t n, float *out4);
 void orc_centroid(const float *points, uint32_t n, float *out3);
+void orc_transmittance(const float *tris, const float *trans, const float *color, const uint8_t *two_sided,
+                       uint32_t n_tri, const float *rays8, uint32_t n_ray, int cull, float *out4);
 void orc_scene_closest(const float *tris, const uint32_t *tri_offset, const float *w2o, const uint32_t *inst_mesh,
                        const uint32_t *inst_custom, uint32_t n_inst, const float *rays8, uint32_t n_ray,
                        uint32_t *out8);

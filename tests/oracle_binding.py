@@ -25,6 +25,7 @@ _SIG = {
     "orc_bounding_sphere": (None, [_vp, _u32, _vp]),
     "orc_centroid": (None, [_vp, _u32, _vp]),
     "orc_scene_closest": (None, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp]),
+    "orc_transmittance": (None, [_vp, _vp, _vp, _vp, _u32, _vp, _u32, ctypes.c_int, _vp]),
     "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_closest_f64": (_u32, [_vp, _u32, _vp, _dp, _dp, _dp]),
@@ -147,6 +148,22 @@ def scene_closest(mesh_tris, instances, rays):
     out = np.zeros(len(r), INSTANCE_HIT_DTYPE)
     lib().orc_scene_closest(tris.ctypes.data, off.ctypes.data, w2o.ctypes.data, mesh.ctypes.data, custom.ctypes.data,
                             len(instances), r.ctypes.data, len(r), out.ctypes.data)
+    return out
+
+
+def transmittance(scene, rays, cull=False):
+    """brute-force transmittance_test over a path-tracer scene dict (positions, indices, tri_object,
+    object_material, materials)"""
+    tris = np.ascontiguousarray(
+        scene["positions"][scene["indices"].reshape(-1, 3)].reshape(-1, 9).astype(np.float32))
+    mats = scene["materials"][scene["object_material"][scene["tri_object"]]]
+    trans = np.ascontiguousarray(mats["transmission"], np.float32)
+    color = np.ascontiguousarray(mats["color"][:, :3], np.float32)
+    two = np.ascontiguousarray(mats["two_sided"] != 0, np.uint8)
+    r = np.ascontiguousarray(rays)
+    out = np.zeros((len(r), 4), np.float32)
+    lib().orc_transmittance(tris.ctypes.data, trans.ctypes.data, color.ctypes.data, two.ctypes.data, len(tris),
+                            r.ctypes.data, len(r), int(cull), out.ctypes.data)
     return out
 
 

@@ -198,3 +198,45 @@ def test_pt_cache_converges_near_reference(ctx, scene3):
     rel = abs(float(got.mean()) - float(want.mean())) / float(want.mean())
     assert rel < 0.05, rel
     cache.close()
+
+
+def test_transmittance_matches_oracle(ctx, scene3):
+    """transmittance_test through the room scene (10% of the spheres transmissive): every record identical to
+    the brute-force oracle, with and without back-face culling"""
+    import oracle_binding as orc2
+    rng = np.random.default_rng(21)
+    n = 1500
+    o = np.column_stack([rng.uniform(-2.8, 2.8, n), rng.uniform(0.2, 5.8, n), rng.uniform(-2.8, 2.8, n)])
+    p = np.column_stack([rng.uniform(-2.8, 2.8, n), rng.uniform(0.2, 5.8, n), rng.uniform(-2.8, 2.8, n)])
+    d = p - o
+    dist = np.linalg.norm(d, axis=1)
+    rays = W.make_rays(o.astype(np.float32), W.glm_normalize((d / dist[:, None]).astype(np.float32)), tmin=1e-4,
+                       tmax=dist.astype(np.float32))
+    pt = vk.PathTracer(ctx, settings(8, 8, 1), seed=0)
+    pt.set_scene(scene3)
+    for cull in (False, True):
+        got = pt.trace_transmittance(rays, cull_back=cull)
+        want = orc2.transmittance(scene3, rays, cull=cull)[:, :3]
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"cull={cull}"
+        assert (got == 0).any(1).sum() > 100 and ((got > 0) & (got < 1)).all(1).sum() > 10
+    pt.close()
+
+
+def test_transmittance_many_layers_in_order(ctx):
+    """20 transmissive plates along the ray: more than one 8-candidate pass, product in ascending t"""
+    import oracle_binding as orc2
+    from test_oracle import _quad_scene, _down_rays
+    zs = [0.1 * k for k in range(20)]
+    rng = np.random.default_rng(5)
+    sc = _quad_scene(zs, list(rng.uniform(0.6, 1.0, 20)), (0.9, 0.7, 0.95))
+    mats = sc["materials"].copy()
+    mats["color"][:, :3] = rng.uniform(0.5, 1.0, (20, 3))
+    sc["materials"] = mats
+    pt = vk.PathTracer(ctx, settings(8, 8, 1), seed=0)
+    pt.set_scene(sc)
+    rays = _down_rays(64)
+    got = pt.trace_transmittance(rays)
+    want = orc2.transmittance(sc, rays)[:, :3]
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert (got > 0).all() and (got < 0.05).all()
+    pt.close()

@@ -358,3 +358,50 @@ def test_scene_power_of_two_scale_and_grid_translation_are_exact():
     assert got["prim"].tolist() == want["prim"].tolist()
     assert got["t"].view(np.uint32).tolist() == want["t"].view(np.uint32).tolist()
     assert got["u"].view(np.uint32).tolist() == want["u"].view(np.uint32).tolist()
+
+
+# ---- transmittance shadow rays (visibility_test.slang:94-165) ----------------------------------------------
+
+def _quad_scene(zs, trans, color, two_sided=True):
+    """unit quads in planes z = zs[k] (facing +z), each with its own material"""
+    import vierkant_amd as vk
+    pos, idx, tri_obj = [], [], []
+    for k, z in enumerate(zs):
+        base = len(pos)
+        pos += [(-1, -1, z), (1, -1, z), (1, 1, z), (-1, 1, z)]
+        idx += [base, base + 1, base + 2, base, base + 2, base + 3]
+        tri_obj += [k, k]
+    mats = np.array([vk.material(color, transmission=t, two_sided=two_sided) for t in trans], dtype=vk.MATERIAL_DTYPE)
+    pos = np.array(pos, np.float32)
+    return {"positions": pos, "normals": np.tile(np.array([0, 0, 1], np.float32), (len(pos), 1)),
+            "indices": np.array(idx, np.uint32), "tri_object": np.array(tri_obj, np.uint32),
+            "object_material": np.arange(len(zs), dtype=np.uint32), "materials": mats, "environment": 1.0}
+
+
+def _down_rays(n=4, tmax=np.inf):
+    o = np.tile(np.array([[0.25, 0.125, 5.0]], np.float32), (n, 1))
+    d = np.tile(np.array([[0.0, 0.0, -1.0]], np.float32), (n, 1))
+    return W.make_rays(o, d, tmin=0.0, tmax=tmax)
+
+
+def test_transmittance_closed_forms():
+    sc = _quad_scene([1.0, 2.0], [0.5, 0.5], (1.0, 0.5, 0.25))
+    t = orc.transmittance(sc, _down_rays())
+    assert np.array_equal(t[:, :3], np.tile([0.25, 0.0625, 0.015625], (4, 1)))
+    # an opaque plate in the interval blocks; beyond tmax it does not count
+    sc = _quad_scene([1.0, 2.0, -1.0], [0.5, 0.5, 0.0], (1.0, 0.5, 0.25))
+    assert (orc.transmittance(sc, _down_rays())[:, :3] == 0).all()
+    assert np.array_equal(orc.transmittance(sc, _down_rays(tmax=5.5))[:, :3], np.tile([0.25, 0.0625, 0.015625], (4, 1)))
+    # nothing hit: fully visible
+    assert (orc.transmittance(sc, W.make_rays(np.array([[5, 5, 5]], np.float32),
+                                              np.array([[0, 0, -1]], np.float32)))[:, :3] == 1).all()
+
+
+def test_transmittance_back_face_culling():
+    """the plates face +z: a ray going +z meets their back faces, culled unless two-sided"""
+    up = W.make_rays(np.array([[0.25, 0.125, -5.0]], np.float32), np.array([[0, 0, 1.0]], np.float32))
+    one = _quad_scene([1.0, 2.0], [0.0, 0.5], (1.0, 1.0, 1.0), two_sided=False)
+    assert (orc.transmittance(one, up)[:, :3] == 0).all()
+    assert (orc.transmittance(one, up, cull=True)[:, :3] == 1).all()
+    two = _quad_scene([1.0, 2.0], [0.0, 0.5], (1.0, 1.0, 1.0), two_sided=True)
+    assert (orc.transmittance(two, up, cull=True)[:, :3] == 0).all()

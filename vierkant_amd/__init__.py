@@ -16,6 +16,7 @@ LIB_PATH = PKG / "libvkrt.so"
 
 OK, E_INVALID_ARG, E_OOM, E_HIP, E_UNSUPPORTED, E_NO_DEVICE, E_INTERNAL = 0, -1, -2, -3, -4, -5, -6
 HOST_PTRS, DEVICE_PTRS, ASYNC = 0, 1, 2
+CULL_BACK = 8
 REJECT, INTERSECT, INSIDE = 0, 1, 2
 INVALID_PRIM = 0xFFFFFFFF
 BUILDER_LBVH, BUILDER_PLOC = 0, 1
@@ -183,6 +184,7 @@ _SIGNATURES = {
     "vkrt_rc_get_gradients": (_i32, [_vp, _vp, _vp]),
     "vkrt_rc_step": (_u32, [_vp]),
     "vkrt_pt_default_cache_config": (None, [ctypes.POINTER(PtCacheConfig)]),
+    "vkrt_pt_trace_transmittance": (_i32, [_vp, _vp, _u32, _vp, _i32]),
     "vkrt_pt_set_radiance_cache": (_i32, [_vp, _vp, ctypes.POINTER(PtCacheConfig)]),
     "vkrt_instance_set_transform": (None, [_vp, _vp]),
     "vkrt_scene_create": (_i32, [_vp, _vp, _u32, _vp, _u32, ctypes.POINTER(_vp)]),
@@ -636,6 +638,14 @@ class PathTracer:
         out = np.zeros(s.resolution[0] * s.resolution[1], dtype=PIXEL_DTYPE)
         self.ctx.check(lib().vkrt_pt_read_pixels(self._h, out.ctypes.data, 0))
         return out.reshape(s.resolution[1], s.resolution[0])
+
+    def trace_transmittance(self, rays, cull_back=False):
+        """transmittance_test through the scene: (n, 3) rgb transmittance per ray"""
+        r = np.ascontiguousarray(rays)
+        out = np.zeros((len(r), 4), np.float32)
+        self.ctx.check(lib().vkrt_pt_trace_transmittance(self._h, r.ctypes.data, len(r), out.ctypes.data,
+                                                         CULL_BACK if cull_back else 0))
+        return out[:, :3]
 
     def set_radiance_cache(self, cache, **cfg):
         """attach a RadianceCache (None detaches); cfg: cache_depth, train_stride, max_train_records"""

@@ -504,6 +504,150 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_resolve(pt_frame f, pt_buffers 
     b.pixels[p] = out;
 }
 
+// ---------------------------------------------------------------------------------------------
+// transmittance shadow rays (transmittance_test, shaders/slang/ray/visibility_test.slang:94-165): any hit of
+// an opaque triangle (transmission == 0; RayBuilder.cpp:212-214 marks only those geometries opaque) in
+// (tmin, tmax) gives 0; otherwise every transmissive hit multiplies the transmittance by transmission * color.
+// The reference multiplies in the driver's candidate order, whose rounding is unspecified; here the factors are
+// multiplied in ascending (t, triangle) order, so the result is defined bit for bit: each pass walks all leaves
+// whose boxes meet [tmin, tmax], keeps the TK nearest candidates above the previous pass's last one in a
+// sorted register list, and another pass runs only when more remained.
+// ---------------------------------------------------------------------------------------------
+constexpr int TK = 8;
+constexpr int TR_STACK = 64;
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_node *__restrict__ nodes2,
+                                                               int32_t root2, const vkrt_bvh4_node *__restrict__ nodes4,
+                                                               int32_t root4, bool wide, bool empty,
+                                                               const float4 *__restrict__ tris,
+                                                               const gpu_material *__restrict__ mats,
+                                                               const float4 *__restrict__ rays, uint32_t n, bool cull,
+                                                               float4 *__restrict__ out)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    int32_t spill[TR_STACK - PT_LDS];
+    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(i >= n) return;
+    lane_stack<PT_BLOCK, PT_LDS> st{smem + threadIdx.x, spill};
+    const ray_setup r = setup_ray(rays, i);
+    float T0 = 1.f, T1 = 1.f, T2 = 1.f;
+    bool blocked = false, more = !empty, first_pass = true;
+    float lb_t = 0.f;
+    uint32_t lb_p = 0;
+    while(more && !blocked)
+    {
+        float kt[TK];
+        uint32_t kp[TK], km[TK];
+#pragma unroll
+        for(int j = 0; j < TK; ++j)
+        {
+            kt[j] = __int_as_float(0x7f800000);
+            kp[j] = 0xffffffffu;
+            km[j] = 0u;
+        }
+        int cnt = 0;
+        more = false;
+        int sp = 0;
+        int32_t ref = wide ? root4 : root2;
+        while(true)
+        {
+            if(ref >= 0)
+            {
+                int32_t c[4];
+                int nc;
+                if(wide) nc = visit4(nodes4[ref], r, r.tmax, c);
+                else
+                {
+                    const vkrt_bvh_node nd = nodes2[ref];
+                    float tn0, tf0, tn1, tf1;
+                    slab_interval(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y, tn0, tf0);
+                    slab_interval(r, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w, tn1, tf1);
+                    const bool h0 = tn0 <= tf0 && tf0 >= r.tmin && tn0 <= r.tmax;
+                    const bool h1 = tn1 <= tf1 && tf1 >= r.tmin && tn1 <= r.tmax;
+                    nc = 0;
+                    if(h0) c[nc++] = nd.d.x;
+                    if(h1) c[nc++] = nd.d.y;
+                }
+                if(nc)
+                {
+                    for(int j = nc - 1; j >= 1; --j) st.put(sp++, c[j]);
+                    ref = c[0];
+                    continue;
+                }
+            }
+            else
+            {
+                uint32_t first, count;
+                leaf_decode(ref, first, count);
+                for(uint32_t k = 0; k < count; ++k)
+                {
+                    const float4 *tr = tris + 3 * (size_t)(first + k);
+                    const float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+                    const bool two_sided = (__float_as_uint(p2.w) & TRI_TWO_SIDED) != 0u;
+                    float t, u, v;
+                    if(!vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
+                                              vec3{p1.z, p1.w, p2.x}, t, u, v, cull && !two_sided) ||
+                       !(t > r.tmin && t < r.tmax))
+                        continue;
+                    const uint32_t mid = __float_as_uint(p2.z);
+                    const gpu_material m = mats[mid];
+                    if(!(m.mrti.z > 0.0f))
+                    {
+                        blocked = true;
+                        break;
+                    }
+                    const uint32_t prim = __float_as_uint(p2.y);
+                    if(!first_pass && (t < lb_t || (t == lb_t && prim <= lb_p))) continue;
+                    if(cnt == TK && (t > kt[TK - 1] || (t == kt[TK - 1] && prim > kp[TK - 1])))
+                    {
+                        more = true;
+                        continue;
+                    }
+                    if(cnt == TK) more = true;// the largest kept candidate is pushed out
+                    // insert into the sorted list (ascending t, then triangle)
+                    float ct = t;
+                    uint32_t cp = prim, cm = mid;
+#pragma unroll
+                    for(int j = 0; j < TK; ++j)
+                    {
+                        const bool less = ct < kt[j] || (ct == kt[j] && cp < kp[j]);
+                        const float tt = kt[j];
+                        const uint32_t tp = kp[j], tm = km[j];
+                        kt[j] = less ? ct : tt;
+                        kp[j] = less ? cp : tp;
+                        km[j] = less ? cm : tm;
+                        ct = less ? tt : ct;
+                        cp = less ? tp : cp;
+                        cm = less ? tm : cm;
+                    }
+                    cnt = cnt < TK ? cnt + 1 : TK;
+                }
+                if(blocked) break;
+            }
+            if(sp == 0) break;
+            --sp;
+            ref = st.get(sp);
+        }
+        if(blocked) break;
+        // multiply in ascending (t, triangle) order; the triangle's material comes from its leaf record
+#pragma unroll
+        for(int j = 0; j < TK; ++j)
+        {
+            if(j < cnt)
+            {
+                const gpu_material m = mats[km[j]];
+                T0 = T0 * (m.mrti.z * m.color.x);
+                T1 = T1 * (m.mrti.z * m.color.y);
+                T2 = T2 * (m.mrti.z * m.color.z);
+                lb_t = kt[j];
+                lb_p = kp[j];
+            }
+        }
+        first_pass = false;
+    }
+    out[i] = blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
+}
+
 __global__ void k_pt_tag(float4 *__restrict__ tris, uint32_t n, const uint32_t *__restrict__ tri_object,
                          const uint32_t *__restrict__ object_material, const gpu_material *__restrict__ mats)
 {
@@ -886,6 +1030,27 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         pt->n_materials = sc->n_materials;
         pt->environment = sc->environment;
         fit_cache_bounds(pt);
+    });
+}
+
+int vkrt_pt_trace_transmittance(vkrt_pt *pt, const vkrt_ray *rays, uint32_t n, float *out4, int flags)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    vkrt_ctx *ctx = pt->ctx;
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(pt->bvh, "vkrt_pt_set_scene first");
+        VKRT_REQUIRE(n == 0 || (rays && out4), "null array argument");
+        if(!n) return;
+        const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
+        float4 *dout = reinterpret_cast<float4 *>(output(ctx, 2, out4, (size_t)n * 4, flags));
+        const vkrt_bvh *b = pt->bvh;
+        k_pt_transmittance<<<div_up(n, PT_BLOCK), PT_BLOCK, 0, ctx->stream>>>(
+                b->nodes.as<vkrt_bvh_node>(), b->root_ref, b->nodes4.as<vkrt_bvh4_node>(), b->root4, b->wide, b->empty,
+                b->tris.as<float4>(), pt->mats.as<gpu_material>(), reinterpret_cast<const float4 *>(dr), n,
+                (flags & VKRT_CULL_BACK) != 0, dout);
+        VKRT_HIP(hipGetLastError());
+        finish_output(ctx, 2, out4, (size_t)n * 4, flags);
+        finish_call(ctx, flags);
     });
 }
 

@@ -42,64 +42,6 @@ struct blas_dev
     uint32_t empty;
 };
 
-//! children of a 4-wide node whose padded interval meets [tmin, best_t], nearest first; returns the count
-__device__ __forceinline__ int visit4(const vkrt_bvh4_node &nd, const ray_setup &r, float best_t, int32_t c[4])
-{
-    const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
-    const int4 cr = nd.child;
-    float k[4], f;
-    bool v[4];
-    slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k[0], f);
-    v[0] = cr.x != EMPTY_CHILD && k[0] <= f && f >= r.tmin && k[0] <= best_t;
-    slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k[1], f);
-    v[1] = cr.y != EMPTY_CHILD && k[1] <= f && f >= r.tmin && k[1] <= best_t;
-    slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k[2], f);
-    v[2] = cr.z != EMPTY_CHILD && k[2] <= f && f >= r.tmin && k[2] <= best_t;
-    slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k[3], f);
-    v[3] = cr.w != EMPTY_CHILD && k[3] <= f && f >= r.tmin && k[3] <= best_t;
-    c[0] = cr.x;
-    c[1] = cr.y;
-    c[2] = cr.z;
-    c[3] = cr.w;
-    auto cx = [&](int a, int b) {
-        const bool sw = v[b] && (!v[a] || k[b] < k[a]);
-        if(sw)
-        {
-            const bool tv = v[a];
-            v[a] = v[b];
-            v[b] = tv;
-            const float tk = k[a];
-            k[a] = k[b];
-            k[b] = tk;
-            const int32_t tc = c[a];
-            c[a] = c[b];
-            c[b] = tc;
-        }
-    };
-    cx(0, 1);
-    cx(2, 3);
-    cx(0, 2);
-    cx(1, 3);
-    cx(1, 2);
-    return (v[0] ? 1 : 0) + (v[1] ? 1 : 0) + (v[2] ? 1 : 0) + (v[3] ? 1 : 0);
-}
-
-template<int BLOCK, int LDS_DEPTH>
-struct lane_stack
-{
-    int32_t *lds;
-    int32_t *spill;
-    __device__ __forceinline__ void put(int i, int32_t v)
-    {
-        if(LDS_DEPTH > 0 && i < LDS_DEPTH) lds[i * BLOCK] = v;
-        else spill[i - LDS_DEPTH] = v;
-    }
-    __device__ __forceinline__ int32_t get(int i) const
-    {
-        return (LDS_DEPTH > 0 && i < LDS_DEPTH) ? lds[i * BLOCK] : spill[i - LDS_DEPTH];
-    }
-};
-
 template<bool ANY, int BLOCK, int LDS_DEPTH>
 __global__ void __launch_bounds__(BLOCK) k_scene_trace(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
                                                        uint32_t empty_scene, const uint32_t *__restrict__ order,
