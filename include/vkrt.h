@@ -353,7 +353,8 @@ typedef struct vkrt_pt_settings
     uint32_t suspend_trace_when_done;
     uint32_t disable_material;
     uint32_t suppress_reset;
-    uint32_t reserved[7];
+    uint32_t denoising;     /* settings_t::denoising: vkrt_pt_read_image applies the a-trous denoiser */
+    uint32_t reserved[6];
 } vkrt_pt_settings;
 
 /* material_t subset evaluated by the tracer (shaders/slang/ray/types.slang:64-107): base color, emission
@@ -437,6 +438,13 @@ int vkrt_pt_reset_accumulator(vkrt_pt *pt);
 uint32_t vkrt_pt_current_batch(const vkrt_pt *pt);
 /* Copy the W*H pixel buffer (vkrt_pixel, row-major from the top-left) to `out` (host, or device with flags). */
 int vkrt_pt_read_pixels(vkrt_pt *pt, vkrt_pixel *out, int flags);
+/* The tracer's output image (W*H float4, row-major from the top-left): with settings.denoising the
+ * edge-avoiding a-trous wavelet of denoise.slang over the pixel buffer (5 passes, steps 1..16, rgb, alpha 0;
+ * PBRPathTracer::denoise_pass, src/PBRPathTracer.cpp:346-400), otherwise radiance and alpha. */
+int vkrt_pt_read_image(vkrt_pt *pt, float *out_rgba, int flags);
+/* The same denoise pass over any pixel buffer (width * height vkrt_pixel). */
+int vkrt_denoise(vkrt_ctx *ctx, const vkrt_pixel *pixels, uint32_t width, uint32_t height, uint32_t denoise,
+                 float *out_rgba, int flags);
 int vkrt_pt_get_stats(const vkrt_pt *pt, vkrt_pt_stats *out);
 /* transmittance_test (shaders/slang/ray/visibility_test.slang:94-165) through the tracer's scene: out4 =
  * (rgb transmittance, 0) per ray; 0 when an opaque triangle (transmission == 0) is hit in (tmin, tmax),

@@ -777,3 +777,119 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     free(two);
     free(tris);
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * denoise.slang (:32-126) and PBRPathTracer::denoise_pass (src/PBRPathTracer.cpp:346-400): the
+ * edge-avoiding a-trous wavelet over the pixel buffer, steps 1, 2, 4, 8, 16 (:26), ping-ponging between two
+ * images; without denoising the image is radiance and alpha. px8: 8 words per pixel (radiance rgb, alpha,
+ * depth, normal as two snorm16 octahedral coordinates, object id, albedo rgba8). out4: rgba per pixel.
+ * ---------------------------------------------------------------------------------------------- */
+static void dn_unpack_normal(uint32_t packed, float n[3])
+{
+    /* unpack_snorm_2x16 (utils/octahedral_map.slang:40-45), then the octahedral map (:25-35) */
+    int16_t lo = (int16_t)(packed & 0xffffu), hi = (int16_t)(packed >> 16);
+    float px = (float)lo / 32767.0f, py = (float)hi / 32767.0f;
+    if(px < -1.0f) px = -1.0f;
+    if(py < -1.0f) py = -1.0f;
+    float x = px, y = py, z = 1.0f - fabsf(px) - fabsf(py);
+    if(z < 0.0f)
+    {
+        float nx = (1.0f - fabsf(y)) * (x >= 0.0f ? 1.0f : -1.0f);
+        float ny = (1.0f - fabsf(x)) * (y >= 0.0f ? 1.0f : -1.0f);
+        x = nx;
+        y = ny;
+    }
+    float inv = 1.0f / sqrtf((x * x + y * y) + z * z);
+    n[0] = x * inv;
+    n[1] = y * inv;
+    n[2] = z * inv;
+}
+
+static void dn_albedo(uint32_t c, float a[3])
+{
+    a[0] = (float)(c & 0xffu) / 255.0f;
+    a[1] = (float)((c >> 8) & 0xffu) / 255.0f;
+    a[2] = (float)((c >> 16) & 0xffu) / 255.0f;
+}
+
+void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4)
+{
+    const size_t n = (size_t)w * h;
+    if(!denoise)
+    {
+        for(size_t i = 0; i < n; ++i) memcpy(out4 + 4 * i, px8 + 8 * i, 16);
+        return;
+    }
+    static const float kern[25] = {1.0f / 256.0f, 1.0f / 64.0f, 3.0f / 128.0f, 1.0f / 64.0f, 1.0f / 256.0f,
+                                   1.0f / 64.0f,  1.0f / 16.0f, 3.0f / 32.0f,  1.0f / 16.0f, 1.0f / 64.0f,
+                                   3.0f / 128.0f, 3.0f / 32.0f, 9.0f / 64.0f,  3.0f / 32.0f, 3.0f / 128.0f,
+                                   1.0f / 64.0f,  1.0f / 16.0f, 3.0f / 32.0f,  1.0f / 16.0f, 1.0f / 64.0f,
+                                   1.0f / 256.0f, 1.0f / 64.0f, 3.0f / 128.0f, 1.0f / 64.0f, 1.0f / 256.0f};
+    const float lum[3] = {0.298999994993f, 0.587000012398f, 0.114000000060f};
+    const float color_phi = 1.0f / 0.9f, normal_phi = 1.0f / 0.3f, position_phi = 1.0f;
+    const int steps[5] = {1, 2, 4, 8, 16};
+    float *img[2];
+    img[0] = (float *)malloc(n * 16);
+    img[1] = (float *)malloc(n * 16);
+    for(int pass = 0; pass < 5; ++pass)
+    {
+        const float *prev = pass ? img[(pass - 1) % 2] : NULL;
+        float *dst = img[pass % 2];
+        for(uint32_t y = 0; y < h; ++y)
+            for(uint32_t x = 0; x < w; ++x)
+            {
+                const size_t c = (size_t)y * w + x;
+                float cv[3], nc[3], ac[3], eff[3], cd[3];
+                for(int k = 0; k < 3; ++k)
+                {
+                    float f;
+                    memcpy(&f, px8 + 8 * c + k, 4);
+                    cv[k] = prev ? prev[4 * c + k] : f;
+                }
+                float depth_c;
+                memcpy(&depth_c, px8 + 8 * c + 4, 4);
+                dn_unpack_normal(px8[8 * c + 5], nc);
+                dn_albedo(px8[8 * c + 7], ac);
+                for(int k = 0; k < 3; ++k)
+                {
+                    eff[k] = ac[k] > 0.01f ? ac[k] : 0.01f;
+                    cd[k] = cv[k] / eff[k];
+                }
+                const float lum_c = (cd[0] * lum[0] + cd[1] * lum[1]) + cd[2] * lum[2];
+                float sum[3] = {0.f, 0.f, 0.f}, cum = 0.f;
+                for(int t = 0; t < 25; ++t)
+                {
+                    int ux = (int)x + (t % 5 - 2) * steps[pass], uy = (int)y + (t / 5 - 2) * steps[pass];
+                    ux = ux < 0 ? 0 : (ux > (int)w - 1 ? (int)w - 1 : ux);
+                    uy = uy < 0 ? 0 : (uy > (int)h - 1 ? (int)h - 1 : uy);
+                    const size_t j = (size_t)uy * w + (size_t)ux;
+                    float ct[3], at[3], nt[3], td[3], dj;
+                    for(int k = 0; k < 3; ++k)
+                    {
+                        float f;
+                        memcpy(&f, px8 + 8 * j + k, 4);
+                        ct[k] = prev ? prev[4 * j + k] : f;
+                    }
+                    dn_albedo(px8[8 * j + 7], at);
+                    for(int k = 0; k < 3; ++k) td[k] = ct[k] / (at[k] > 0.01f ? at[k] : 0.01f);
+                    const float ld = lum_c - ((td[0] * lum[0] + td[1] * lum[1]) + td[2] * lum[2]);
+                    float cw = expf(-(ld * ld) * color_phi);
+                    cw = cw < 1.0f ? cw : 1.0f;
+                    dn_unpack_normal(px8[8 * j + 5], nt);
+                    const float dx = nc[0] - nt[0], dy = nc[1] - nt[1], dz = nc[2] - nt[2];
+                    float nw = expf(-((dx * dx + dy * dy) + dz * dz) * normal_phi);
+                    nw = nw < 1.0f ? nw : 1.0f;
+                    memcpy(&dj, px8 + 8 * j + 4, 4);
+                    const float pw = expf(-fabsf(depth_c - dj) * position_phi);
+                    const float wt = ((cw * nw) * pw) * kern[t];
+                    for(int k = 0; k < 3; ++k) sum[k] = sum[k] + td[k] * wt;
+                    cum = cum + wt;
+                }
+                for(int k = 0; k < 3; ++k) dst[4 * c + k] = (sum[k] / cum) * eff[k];
+                dst[4 * c + 3] = 0.f;
+            }
+    }
+    memcpy(out4, img[0], n * 16);/* five passes end in image 0 */
+    free(img[0]);
+    free(img[1]);
+}

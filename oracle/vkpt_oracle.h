@@ -36,6 +36,7 @@ typedef struct orc_pt_params
 /* One batch: out8 = 8 floats per pixel (radiance rgb / num_samples, coverage, linear depth, -, object id
  * bits, -), row-major from the top-left. */
 void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads);
+void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4);
 
 #ifdef __cplusplus
 }

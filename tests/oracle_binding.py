@@ -25,6 +25,7 @@ _SIG = {
     "orc_bounding_sphere": (None, [_vp, _u32, _vp]),
     "orc_centroid": (None, [_vp, _u32, _vp]),
     "orc_scene_closest": (None, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp]),
+    "orc_denoise": (None, [_vp, _u32, _u32, ctypes.c_int, _vp]),
     "orc_transmittance": (None, [_vp, _vp, _vp, _vp, _u32, _vp, _u32, ctypes.c_int, _vp]),
     "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
@@ -164,6 +165,15 @@ def transmittance(scene, rays, cull=False):
     out = np.zeros((len(r), 4), np.float32)
     lib().orc_transmittance(tris.ctypes.data, trans.ctypes.data, color.ctypes.data, two.ctypes.data, len(tris),
                             r.ctypes.data, len(r), int(cull), out.ctypes.data)
+    return out
+
+
+def denoise(pixels, denoise=True):
+    """the a-trous denoise pass over a (H, W) PIXEL_DTYPE buffer"""
+    px = np.ascontiguousarray(pixels)
+    h, w = px.shape
+    out = np.zeros((h, w, 4), np.float32)
+    lib().orc_denoise(px.ctypes.data, w, h, int(denoise), out.ctypes.data)
     return out
 
 

@@ -240,3 +240,32 @@ def test_transmittance_many_layers_in_order(ctx):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     assert (got > 0).all() and (got < 0.05).all()
     pt.close()
+
+
+def test_denoiser_matches_oracle(ctx, scene3):
+    """the a-trous denoiser over a rendered pixel buffer equals the C restatement up to exp() ulps (ocml vs
+    glibc): at most 1e-4 relative per channel; raw mode is bit-exact; the tracer's image() is the same pass"""
+    import oracle_binding as orc2
+    w, h = 96, 54
+    s = settings(w, h, 4)
+    s.denoising = 1
+    pt = vk.PathTracer(ctx, s, seed=2)
+    pt.set_scene(scene3)
+    pt.render(W.room_camera())
+    px = pt.pixels()
+    got = ctx.denoise(px)
+    want = orc2.denoise(px)
+    scale = np.maximum(np.abs(want[..., :3]), 1e-3)
+    assert (np.abs(got[..., :3] - want[..., :3]) / scale).max() < 1e-4
+    assert np.array_equal(got[..., 3], want[..., 3])
+    assert np.array_equal(pt.image(), got)
+    raw = ctx.denoise(px, denoise=False)
+    assert np.array_equal(raw, orc2.denoise(px, denoise=False))
+    # denoising lowers the error against many samples
+    ref = vk.PathTracer(ctx, settings(w, h, 256), seed=3)
+    ref.set_scene(scene3)
+    ref.render(W.room_camera())
+    r = ref.pixels()["radiance"]
+    assert np.sqrt(np.mean((got[..., :3] - r) ** 2)) < 0.8 * np.sqrt(np.mean((px["radiance"] - r) ** 2))
+    pt.close()
+    ref.close()

@@ -58,3 +58,60 @@ def test_integrator_deterministic_and_bounded():
     b = orc.pt_render(sc, cam, 48, 27, 2, 6, 99, nthreads=3)
     assert a.tobytes() == b.tobytes()
     assert np.isfinite(a).all() and (a[..., :3] >= 0).all()
+
+
+# ---- a-trous denoiser (denoise.slang) ------------------------------------------------------------------
+
+def _pixels(h, w, radiance, normal=(0.0, 0.0, 1.0), albedo=(0.5, 0.5, 0.5), depth=2.0, alpha=1.0):
+    import vierkant_amd as vk
+    px = np.zeros((h, w), vk.PIXEL_DTYPE)
+    px["radiance"] = radiance
+    px["alpha"] = alpha
+    px["depth"] = depth
+    px["normal"] = _pack_oct(normal)
+    px["albedo"] = _pack_rgba8(albedo)
+    return px
+
+
+def _pack_oct(n):
+    """pack_normal_oct (utils/octahedral_map.slang, the inverse of the denoiser's decode)"""
+    n = np.asarray(n, np.float64)
+    n = n / np.abs(n).sum()
+    x, y = n[0], n[1]
+    if n[2] < 0:
+        x, y = (1 - abs(n[1])) * (1 if n[0] >= 0 else -1), (1 - abs(n[0])) * (1 if n[1] >= 0 else -1)
+    ix, iy = int(np.rint(x * 32767)), int(np.rint(y * 32767))
+    return (ix & 0xFFFF) | ((iy & 0xFFFF) << 16)
+
+
+def _pack_rgba8(c):
+    q = [int(min(max(v, 0.0), 1.0) * 255.0) for v in c]
+    return q[0] | (q[1] << 8) | (q[2] << 16) | (255 << 24)
+
+
+def test_denoise_raw_mode_is_radiance_and_alpha():
+    rng = np.random.default_rng(1)
+    px = _pixels(9, 13, rng.uniform(0, 3, (9, 13, 3)).astype(np.float32), alpha=0.5)
+    out = orc.denoise(px, denoise=False)
+    assert np.array_equal(out[..., :3], px["radiance"]) and (out[..., 3] == 0.5).all()
+
+
+def test_denoise_constant_image_is_a_fixed_point():
+    """every weight sees the same demodulated value, so each pass returns it (up to rounding)"""
+    px = _pixels(20, 24, np.array([0.7, 0.3, 1.9], np.float32))
+    out = orc.denoise(px)
+    assert np.allclose(out[..., :3], [0.7, 0.3, 1.9], rtol=2e-6) and (out[..., 3] == 0).all()
+
+
+def test_denoise_smooths_noise_and_keeps_normal_edges():
+    """noise on two flat regions whose normals are perpendicular: each side is smoothed toward its own
+    mean, and the edge leaks at most exp(-2 / 0.3) per tap"""
+    rng = np.random.default_rng(2)
+    h, w = 32, 32
+    base = np.where(np.arange(w)[None, :, None] < 16, 0.2, 2.0) * np.ones((h, w, 3))
+    noisy = (base * rng.uniform(0.8, 1.2, (h, w, 3))).astype(np.float32)
+    px = _pixels(h, w, noisy)
+    px["normal"][:, 16:] = _pack_oct((1.0, 0.0, 0.0))
+    out = orc.denoise(px)[..., :3]
+    assert np.std(out[:, 4:12]) < 0.5 * np.std(noisy[:, 4:12])
+    assert np.abs(out[:, 12:15].mean() - 0.2) < 0.03 and np.abs(out[:, 17:20].mean() - 2.0) < 0.1

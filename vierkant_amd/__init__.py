@@ -50,7 +50,7 @@ class PtSettings(ctypes.Structure):
                 ("num_samples", ctypes.c_uint32), ("max_trace_depth", ctypes.c_uint32),
                 ("max_path_beta", ctypes.c_float), ("suspend_trace_when_done", ctypes.c_uint32),
                 ("disable_material", ctypes.c_uint32), ("suppress_reset", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 7)]
+                ("denoising", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
 
 
 class PtScene(ctypes.Structure):
@@ -172,6 +172,8 @@ _SIGNATURES = {
     "vkrt_pt_current_batch": (_u32, [_vp]),
     "vkrt_pt_read_pixels": (_i32, [_vp, _vp, _i32]),
     "vkrt_pt_get_stats": (_i32, [_vp, ctypes.POINTER(PtStats)]),
+    "vkrt_pt_read_image": (_i32, [_vp, _vp, _i32]),
+    "vkrt_denoise": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _i32]),
     "vkrt_rc_default_params": (None, [ctypes.POINTER(RcParams)]),
     "vkrt_rc_create": (_i32, [_vp, ctypes.POINTER(RcParams), _u32, ctypes.POINTER(_vp)]),
     "vkrt_rc_destroy": (None, [_vp]),
@@ -431,6 +433,14 @@ class Context:
         self.check(lib().vkrt_compute_aabb(self._h, _ptr(points)[0], n, _ptr(out)[0], _flags([points, out])))
         return out
 
+    def denoise(self, pixels, denoise=True):
+        """the a-trous denoise pass over a (H, W) PIXEL_DTYPE buffer: (H, W, 4) float32"""
+        px = np.ascontiguousarray(pixels, PIXEL_DTYPE)
+        h, w = px.shape
+        out = np.zeros((h, w, 4), np.float32)
+        self.check(lib().vkrt_denoise(self._h, px.ctypes.data, w, h, 1 if denoise else 0, out.ctypes.data, 0))
+        return out
+
     def compute_bounding_sphere(self, points, out=None):
         """(center.x, center.y, center.z, radius): compute_bounding_sphere of the reference, bit-identical"""
         n = _n(points, 3)
@@ -638,6 +648,13 @@ class PathTracer:
         out = np.zeros(s.resolution[0] * s.resolution[1], dtype=PIXEL_DTYPE)
         self.ctx.check(lib().vkrt_pt_read_pixels(self._h, out.ctypes.data, 0))
         return out.reshape(s.resolution[1], s.resolution[0])
+
+    def image(self):
+        """the output image (H, W, 4): denoised rgb when settings.denoising, else radiance and alpha"""
+        s = self.settings
+        out = np.zeros((s.resolution[1], s.resolution[0], 4), np.float32)
+        self.ctx.check(lib().vkrt_pt_read_image(self._h, out.ctypes.data, 0))
+        return out
 
     def trace_transmittance(self, rays, cull_back=False):
         """transmittance_test through the scene: (n, 3) rgb transmittance per ray"""

@@ -648,6 +648,44 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_no
     out[i] = blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
 }
 
+// ---------------------------------------------------------------------------------------------
+// denoise pass (shaders/slang/denoise.slang, PBRPathTracer::denoise_pass src/PBRPathTracer.cpp:346-400):
+// edge-avoiding a-trous wavelet over the pixel_data_t buffer, steps 1, 2, 4, 8, 16 ping-ponging between two
+// float4 images; without denoising the image is the raw radiance and alpha
+// ---------------------------------------------------------------------------------------------
+constexpr uint32_t ATROUS_STEPS[5] = {1, 2, 4, 8, 16};
+
+__global__ void __launch_bounds__(256) k_denoise(const vkrt_pixel *__restrict__ px, const float4 *__restrict__ prev,
+                                                 float4 *__restrict__ out, uint32_t w, uint32_t h, int step,
+                                                 int denoise)
+{
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x % 16, y = blockIdx.y * 16 + threadIdx.x / 16;
+    if(x >= w || y >= h) return;
+    const uint32_t i = y * w + x;
+    if(!denoise)
+    {
+        out[i] = make_float4(px[i].radiance[0], px[i].radiance[1], px[i].radiance[2], px[i].alpha);
+        return;
+    }
+    const vec3 c = atrous_pixel(px, reinterpret_cast<const float *>(prev), w, h, static_cast<int>(x),
+                                static_cast<int>(y), step);
+    out[i] = make_float4(c.x, c.y, c.z, 0.f);
+}
+
+//! the image of a pixel buffer: five a-trous passes when denoising; returns the buffer holding the result
+static float4 *run_denoise(hipStream_t s, const vkrt_pixel *px, uint32_t w, uint32_t h, bool denoise, float4 *ping,
+                           float4 *pong)
+{
+    const dim3 grid(div_up(w, 16), div_up(h, 16));
+    float4 *bufs[2] = {ping, pong};
+    const uint32_t passes = denoise ? 5u : 1u;
+    for(uint32_t p = 0; p < passes; ++p)
+        k_denoise<<<grid, 256, 0, s>>>(px, p ? bufs[(p - 1) % 2] : nullptr, bufs[p % 2], w, h,
+                                       static_cast<int>(ATROUS_STEPS[p]), denoise ? 1 : 0);
+    VKRT_HIP(hipGetLastError());
+    return bufs[(passes - 1) % 2];
+}
+
 __global__ void k_pt_tag(float4 *__restrict__ tris, uint32_t n, const uint32_t *__restrict__ tri_object,
                          const uint32_t *__restrict__ object_material, const gpu_material *__restrict__ mats)
 {
@@ -684,6 +722,7 @@ struct vkrt_pt
     vkrt_rc *rc = nullptr;
     vkrt_pt_cache_config cache_cfg{};
     vkrt::device_buffer qrec, tvtx, trec, ttgt;
+    vkrt::device_buffer image;// denoise ping-pong images
 };
 
 namespace vkrt
@@ -1050,6 +1089,43 @@ int vkrt_pt_trace_transmittance(vkrt_pt *pt, const vkrt_ray *rays, uint32_t n, f
                 (flags & VKRT_CULL_BACK) != 0, dout);
         VKRT_HIP(hipGetLastError());
         finish_output(ctx, 2, out4, (size_t)n * 4, flags);
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_denoise(vkrt_ctx *ctx, const vkrt_pixel *pixels, uint32_t width, uint32_t height, uint32_t denoise,
+                 float *out_rgba, int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(width > 0 && height > 0 && (uint64_t)width * height <= (1u << 26), "bad image size");
+        VKRT_REQUIRE(pixels && out_rgba, "null array argument");
+        const size_t n = (size_t)width * height;
+        const vkrt_pixel *dp = input(ctx, 0, pixels, n, flags);
+        ctx->scratch[2].reserve(2 * n * sizeof(float4));
+        float4 *ping = ctx->scratch[2].as<float4>(), *pong = ping + n;
+        float4 *res = run_denoise(ctx->stream, dp, width, height, denoise != 0, ping, pong);
+        VKRT_HIP(hipMemcpyAsync(out_rgba, res, n * sizeof(float4),
+                                (flags & VKRT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                ctx->stream));
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_pt_read_image(vkrt_pt *pt, float *out_rgba, int flags)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    vkrt_ctx *ctx = pt->ctx;
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(out_rgba, "null output");
+        VKRT_REQUIRE(pt->width > 0, "nothing rendered yet");
+        const size_t n = (size_t)pt->width * pt->height;
+        pt->image.reserve(2 * n * sizeof(float4));
+        float4 *ping = pt->image.as<float4>(), *pong = ping + n;
+        float4 *res = run_denoise(ctx->stream, pt->pixels.as<vkrt_pixel>(), pt->width, pt->height,
+                                  pt->settings.denoising != 0, ping, pong);
+        VKRT_HIP(hipMemcpyAsync(out_rgba, res, n * sizeof(float4),
+                                (flags & VKRT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                ctx->stream));
         finish_call(ctx, flags);
     });
 }

@@ -2,8 +2,8 @@
 // semantics (include/vierkant/PBRPathTracer.hpp:42-142, src/PBRPathTracer.cpp:184-255, :677-682).
 //
 // settings_t keeps the reference's field names and defaults for everything that changes the traced image;
-// display-side fields (denoising, tonemap, bloom, gamma, exposure, ...) are kept for source compatibility and
-// have no effect here. render_scene traces one batch unless max_num_batches is reached with
+// denoising selects the a-trous denoiser for image(); the other display-side fields (tonemap, bloom, gamma,
+// exposure, ...) are kept for source compatibility and have no effect here. render_scene traces one batch unless max_num_batches is reached with
 // suspend_trace_when_done set, then advances the batch index as min(batch + 1, max_num_batches).
 #pragma once
 
@@ -105,6 +105,16 @@ public:
         return out;
     }
 
+    //! the output image (float4 per pixel): denoised when settings.denoising (PBRPathTracer::denoise_pass)
+    [[nodiscard]] std::vector<float> image() const
+    {
+        vkrt_pt_settings s{};
+        vkrt_pt_get_settings(m_pt, &s);
+        std::vector<float> out(4 * (size_t)s.resolution[0] * s.resolution[1]);
+        m_ctx->check(vkrt_pt_read_image(m_pt, out.data(), 0));
+        return out;
+    }
+
     [[nodiscard]] const std::deque<statistics_t> &statistics() const { return m_statistics; }
 
     settings_t settings;
@@ -128,6 +138,7 @@ private:
         c.suspend_trace_when_done = s.suspend_trace_when_done ? 1u : 0u;
         c.disable_material = s.disable_material ? 1u : 0u;
         c.suppress_reset = s.suppress_reset ? 1u : 0u;
+        c.denoising = s.denoising ? 1u : 0u;
         return c;
     }
 

@@ -21,6 +21,7 @@
 #include <cstdint>
 
 #include "rt_math.hpp"
+#include "vkrt.h"
 
 namespace vierkant::pt
 {
@@ -438,6 +439,77 @@ VK_HD uint32_t pack_rgba8(const vec3 &c, float a)
 {
     auto q = [](float x) { return static_cast<uint32_t>(clampf(x, 0.f, 1.f) * 255.0f); };
     return q(c.x) | (q(c.y) << 8) | (q(c.z) << 16) | (q(a) << 24);
+}
+
+//! color_cast(uint) rgb (utils/color.slang:6-12)
+VK_HD vec3 unpack_rgb8(uint32_t c)
+{
+    return vec3{static_cast<float>(c & 0xffu) / 255.0f, static_cast<float>((c >> 8) & 0xffu) / 255.0f,
+                static_cast<float>((c >> 16) & 0xffu) / 255.0f};
+}
+
+//! unpack_snorm_2x16 + octahedral_mapping_to_normalized_vector (utils/octahedral_map.slang:25-45)
+VK_HD vec3 unpack_normal_oct(uint32_t packed)
+{
+    const int bx = static_cast<int>(packed << 16) >> 16, by = static_cast<int>(packed) >> 16;
+    const float px = fmaxf(static_cast<float>(bx) / 32767.0f, -1.0f), py = fmaxf(static_cast<float>(by) / 32767.0f, -1.0f);
+    vec3 n{px, py, 1.0f - fabsf(px) - fabsf(py)};
+    if(n.z < 0.0f)
+    {
+        const float qx = (1.0f - fabsf(n.y)) * (n.x >= 0.0f ? 1.0f : -1.0f);
+        const float qy = (1.0f - fabsf(n.x)) * (n.y >= 0.0f ? 1.0f : -1.0f);
+        n.x = qx;
+        n.y = qy;
+    }
+    return normalize3(n);
+}
+
+//! the edge-avoiding a-trous step of denoise.slang:32-106 for pixel (x, y): `prev` is null on pass 0
+//! (radiance from the pixel buffer), else the previous pass's rgb
+VK_HD vec3 atrous_pixel(const vkrt_pixel *px, const float *prev, uint32_t w, uint32_t h, int x, int y, int step)
+{
+    const float kernel[25] = {1.0f / 256.0f, 1.0f / 64.0f, 3.0f / 128.0f, 1.0f / 64.0f, 1.0f / 256.0f,
+                              1.0f / 64.0f,  1.0f / 16.0f, 3.0f / 32.0f,  1.0f / 16.0f, 1.0f / 64.0f,
+                              3.0f / 128.0f, 3.0f / 32.0f, 9.0f / 64.0f,  3.0f / 32.0f, 3.0f / 128.0f,
+                              1.0f / 64.0f,  1.0f / 16.0f, 3.0f / 32.0f,  1.0f / 16.0f, 1.0f / 64.0f,
+                              1.0f / 256.0f, 1.0f / 64.0f, 3.0f / 128.0f, 1.0f / 64.0f, 1.0f / 256.0f};
+    const float color_phi = 1.0f / 0.9f, normal_phi = 1.0f / 0.3f, position_phi = 1.0f;
+    const vec3 lum{0.298999994993f, 0.587000012398f, 0.114000000060f};
+    auto rgb = [&](uint32_t i) {
+        return prev ? vec3{prev[4 * i], prev[4 * i + 1], prev[4 * i + 2]}
+                    : vec3{px[i].radiance[0], px[i].radiance[1], px[i].radiance[2]};
+    };
+    const uint32_t c = static_cast<uint32_t>(y) * w + static_cast<uint32_t>(x);
+    const vec3 cval = rgb(c);
+    const vec3 nval = unpack_normal_oct(px[c].normal);
+    const float depth_c = px[c].depth;
+    const vec3 a = unpack_rgb8(px[c].albedo);
+    const vec3 eff{fmaxf(a.x, 0.01f), fmaxf(a.y, 0.01f), fmaxf(a.z, 0.01f)};
+    const vec3 cd{cval.x / eff.x, cval.y / eff.y, cval.z / eff.z};
+    const float lum_c = dot(cd, lum);
+    vec3 sum{0.f, 0.f, 0.f};
+    float cum_w = 0.f;
+    for(int k = 0; k < 25; ++k)
+    {
+        const int ox = k % 5 - 2, oy = k / 5 - 2;
+        int ux = x + ox * step, uy = y + oy * step;
+        ux = ux < 0 ? 0 : (ux > static_cast<int>(w) - 1 ? static_cast<int>(w) - 1 : ux);
+        uy = uy < 0 ? 0 : (uy > static_cast<int>(h) - 1 ? static_cast<int>(h) - 1 : uy);
+        const uint32_t j = static_cast<uint32_t>(uy) * w + static_cast<uint32_t>(ux);
+        const vec3 ct = rgb(j);
+        const vec3 at = unpack_rgb8(px[j].albedo);
+        const vec3 td{ct.x / fmaxf(at.x, 0.01f), ct.y / fmaxf(at.y, 0.01f), ct.z / fmaxf(at.z, 0.01f)};
+        const float ld = lum_c - dot(td, lum);
+        const float c_w = fminf(expf(-(ld * ld) * color_phi), 1.0f);
+        const vec3 nt = unpack_normal_oct(px[j].normal);
+        const vec3 dn = nval - nt;
+        const float n_w = fminf(expf(-dot(dn, dn) * normal_phi), 1.0f);
+        const float p_w = expf(-fabsf(depth_c - px[j].depth) * position_phi);
+        const float weight = ((c_w * n_w) * p_w) * kernel[k];
+        sum = vec3{sum.x + td.x * weight, sum.y + td.y * weight, sum.z + td.z * weight};
+        cum_w += weight;
+    }
+    return vec3{(sum.x / cum_w) * eff.x, (sum.y / cum_w) * eff.y, (sum.z / cum_w) * eff.z};
 }
 
 }// namespace vierkant::pt
