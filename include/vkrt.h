@@ -342,7 +342,9 @@ typedef struct vkrt_pt vkrt_pt;
 
 /* PBRPathTracer::settings_t fields that affect the traced image (PBRPathTracer.hpp:42-110). Defaults
  * (vkrt_pt_default_settings): 1280x720, max_num_batches 0, num_samples 1, max_trace_depth 6,
- * max_path_beta 0, suspend_trace_when_done 1, disable_material 0, suppress_reset 0. */
+ * max_path_beta 0, suspend_trace_when_done 1, disable_material 0, suppress_reset 0, denoising 0, tonemap 1,
+ * gamma 1, exposure 2 (include/vierkant/PBRPathTracer.hpp:42-90). Bloom is not provided: the composited
+ * image is the reference's with settings_t::bloom false. */
 typedef struct vkrt_pt_settings
 {
     uint32_t resolution[2];
@@ -354,7 +356,10 @@ typedef struct vkrt_pt_settings
     uint32_t disable_material;
     uint32_t suppress_reset;
     uint32_t denoising;     /* settings_t::denoising: vkrt_pt_read_image applies the a-trous denoiser */
-    uint32_t reserved[6];
+    uint32_t tonemap;       /* settings_t::tonemap: vkrt_pt_read_image composites as bloom_composition.slang */
+    float gamma;            /* settings_t::gamma (> 0 with tonemap) */
+    float exposure;         /* settings_t::exposure */
+    uint32_t reserved[3];
 } vkrt_pt_settings;
 
 /* material_t subset evaluated by the tracer (shaders/slang/ray/types.slang:64-107): base color, emission
@@ -440,8 +445,15 @@ uint32_t vkrt_pt_current_batch(const vkrt_pt *pt);
 int vkrt_pt_read_pixels(vkrt_pt *pt, vkrt_pixel *out, int flags);
 /* The tracer's output image (W*H float4, row-major from the top-left): with settings.denoising the
  * edge-avoiding a-trous wavelet of denoise.slang over the pixel buffer (5 passes, steps 1..16, rgb, alpha 0;
- * PBRPathTracer::denoise_pass, src/PBRPathTracer.cpp:346-400), otherwise radiance and alpha. */
+ * PBRPathTracer::denoise_pass, src/PBRPathTracer.cpp:346-400), otherwise radiance and alpha. With
+ * settings.tonemap that image is then composited as PBRPathTracer::post_fx_pass (src/PBRPathTracer.cpp:405-465,
+ * shaders/slang/fullscreen/bloom_composition.slang:34-45, no bloom, no motion): rgb = pow(1 - exp(-rgb *
+ * exposure), 1 / gamma), alpha kept, each channel rounded to half precision as the R16G16B16A16_SFLOAT target
+ * stores it (src/PBRPathTracer.cpp:819). */
 int vkrt_pt_read_image(vkrt_pt *pt, float *out_rgba, int flags);
+/* That composite alone over n float4 texels (in and out may alias). */
+int vkrt_tonemap(vkrt_ctx *ctx, const float *in_rgba, uint32_t n, float exposure, float gamma, float *out_rgba,
+                 int flags);
 /* The same denoise pass over any pixel buffer (width * height vkrt_pixel). */
 int vkrt_denoise(vkrt_ctx *ctx, const vkrt_pixel *pixels, uint32_t width, uint32_t height, uint32_t denoise,
                  float *out_rgba, int flags);

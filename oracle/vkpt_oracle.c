@@ -893,3 +893,35 @@ void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float
     free(img[0]);
     free(img[1]);
 }
+
+/* The value of x as IEEE binary16 stores it: round to nearest, ties to even, overflow to infinity. The
+ * quotient by the spacing of halves at x's binade is exact in double, so nearbyint (default rounding mode)
+ * does the rounding; subnormal halves have spacing 2^-24. */
+float orc_half_value(float x)
+{
+    if(!(fabsf(x) < INFINITY) || x == 0.f) return x;
+    int k;
+    frexp(x, &k);
+    const int q = (k - 1 >= -14) ? (k - 1) - 10 : -24;
+    const double r = ldexp(nearbyint(ldexp((double)x, -q)), q);
+    if(fabs(r) > 65504.0) return copysignf(INFINITY, x);
+    return (float)r;
+}
+
+/* bloom_composition.slang:34-45 with neither bloom nor motion blur (PBRPathTracer.cpp:405-465):
+ * utils/tonemap.slang tonemap_exposure, pow(., 1 / gamma) with the single-precision quotient, alpha passed on,
+ * every channel stored in the R16G16B16A16_SFLOAT target (PBRPathTracer.cpp:819) */
+void orc_tonemap(const float *in4, size_t n, float exposure, float gamma, float *out4)
+{
+    const float inv_gamma = 1.0f / gamma;
+    for(size_t i = 0; i < n; ++i)
+    {
+        for(int k = 0; k < 3; ++k)
+        {
+            float r = 1.0f - expf(-in4[4 * i + k] * exposure);
+            if(inv_gamma != 1.0f) r = powf(r, inv_gamma);
+            out4[4 * i + k] = orc_half_value(r);
+        }
+        out4[4 * i + 3] = orc_half_value(in4[4 * i + 3]);
+    }
+}

@@ -4,6 +4,7 @@
 #ifndef VKPT_ORACLE_H
 #define VKPT_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -37,6 +38,9 @@ typedef struct orc_pt_params
  * bits, -), row-major from the top-left. */
 void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads);
 void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4);
+/* binary16 rounding (nearest, ties to even) and the exposure/gamma composite of the output image */
+float orc_half_value(float x);
+void orc_tonemap(const float *in4, size_t n, float exposure, float gamma, float *out4);
 
 #ifdef __cplusplus
 }

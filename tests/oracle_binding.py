@@ -26,6 +26,8 @@ _SIG = {
     "orc_centroid": (None, [_vp, _u32, _vp]),
     "orc_scene_closest": (None, [_vp, _vp, _vp, _vp, _vp, _u32, _vp, _u32, _vp]),
     "orc_denoise": (None, [_vp, _u32, _u32, ctypes.c_int, _vp]),
+    "orc_half_value": (ctypes.c_float, [ctypes.c_float]),
+    "orc_tonemap": (None, [_vp, ctypes.c_size_t, ctypes.c_float, ctypes.c_float, _vp]),
     "orc_transmittance": (None, [_vp, _vp, _vp, _vp, _u32, _vp, _u32, ctypes.c_int, _vp]),
     "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
@@ -174,6 +176,20 @@ def denoise(pixels, denoise=True):
     h, w = px.shape
     out = np.zeros((h, w, 4), np.float32)
     lib().orc_denoise(px.ctypes.data, w, h, int(denoise), out.ctypes.data)
+    return out
+
+
+def half_value(x):
+    """float32 array rounded to binary16 values (nearest, ties to even) by the C restatement"""
+    h = lib()
+    return np.array([h.orc_half_value(float(v)) for v in np.asarray(x, np.float32).ravel()], np.float32)
+
+
+def tonemap(rgba, exposure=2.0, gamma=1.0):
+    """the exposure/gamma composite over (..., 4) float32 texels"""
+    a = np.ascontiguousarray(rgba, np.float32)
+    out = np.empty_like(a)
+    lib().orc_tonemap(a.ctypes.data, a.size // 4, exposure, gamma, out.ctypes.data)
     return out
 
 

@@ -248,7 +248,7 @@ def test_denoiser_matches_oracle(ctx, scene3):
     import oracle_binding as orc2
     w, h = 96, 54
     s = settings(w, h, 4)
-    s.denoising = 1
+    s.denoising, s.tonemap = 1, 0
     pt = vk.PathTracer(ctx, s, seed=2)
     pt.set_scene(scene3)
     pt.render(W.room_camera())
@@ -271,3 +271,39 @@ def test_denoiser_matches_oracle(ctx, scene3):
     assert rmse(got[..., :3]) < 0.8 * rmse(px["radiance"])
     pt.close()
     ref.close()
+
+
+def _within_half_ulp(got, want):
+    """equal, or one binary16 step apart (expf of ocml and glibc differ in the last float32 ulp, which can
+    move a value across a rounding boundary of the half-precision target)"""
+    step = np.spacing(np.maximum(np.abs(got), np.abs(want)).astype(np.float16)).astype(np.float32)
+    return (np.abs(got - want) <= step).all() and np.mean(got == want) > 0.99
+
+
+def test_tonemap_matches_oracle(ctx, scene3):
+    """the composited image (bloom_composition.slang without bloom) of the denoised and of the raw image
+    against the C restatement; the tracer's image() is the same fused pass as vkrt_tonemap after
+    vkrt_denoise, bit for bit"""
+    import oracle_binding as orc2
+    w, h = 80, 45
+    for denoising, gamma, exposure in ((1, 1.0, 2.0), (0, 2.2, 1.0), (1, 2.2, 0.5)):
+        s = settings(w, h, 2)
+        s.denoising, s.tonemap, s.gamma, s.exposure = denoising, 1, gamma, exposure
+        pt = vk.PathTracer(ctx, s, seed=4)
+        pt.set_scene(scene3)
+        pt.render(W.room_camera())
+        px = pt.pixels()
+        img = pt.image()
+        lin = ctx.denoise(px, denoise=bool(denoising))
+        assert np.array_equal(img, ctx.tonemap(lin, exposure, gamma))
+        want = orc2.tonemap(orc2.denoise(px, denoise=bool(denoising)), exposure, gamma)
+        assert _within_half_ulp(img, want), (denoising, gamma, exposure)
+        assert np.array_equal(img, img.astype(np.float16).astype(np.float32))
+        assert (img[..., :3] >= 0).all() and (img[..., :3] <= 1).all()
+        pt.close()
+    rng = np.random.default_rng(6)
+    tex = np.concatenate([rng.exponential(1.0, (5000, 4)), np.zeros((1, 4)), np.full((1, 4), 1e4)]).astype(np.float32)
+    for gamma, exposure in ((1.0, 2.0), (2.2, 1.0), (0.7, 3.5)):
+        assert _within_half_ulp(ctx.tonemap(tex, exposure, gamma), orc2.tonemap(tex, exposure, gamma))
+    with pytest.raises(vk.VkrtError):
+        ctx.tonemap(tex, 2.0, 0.0)

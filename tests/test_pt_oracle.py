@@ -115,3 +115,43 @@ def test_denoise_smooths_noise_and_keeps_normal_edges():
     out = orc.denoise(px)[..., :3]
     assert np.std(out[:, 4:12]) < 0.5 * np.std(noisy[:, 4:12])
     assert np.abs(out[:, 12:15].mean() - 0.2) < 0.03 and np.abs(out[:, 17:20].mean() - 2.0) < 0.1
+
+
+def test_half_value_matches_numpy_float16():
+    """the C binary16 rounding against numpy's own float16 conversion: ties to even, subnormal halves,
+    the overflow threshold 65520, signed zeros"""
+    rng = np.random.default_rng(3)
+    special = np.array([0.0, -0.0, 1.0, 1 + 2.0 ** -11, 1 + 3 * 2.0 ** -11, 1 + 2.0 ** -11 + 2.0 ** -20,
+                        2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26, 2.0 ** -26, 6.0e-5, 65504.0, 65519.99,
+                        65520.0, 1e6, -1e6, -3.1416, 1 / 3], np.float32)
+    x = np.concatenate([special, rng.standard_normal(4000).astype(np.float32) * 10.0 ** rng.integers(-8, 5, 4000)])
+    x = x.astype(np.float32)
+    got = orc.half_value(x)
+    with np.errstate(over="ignore"):
+        want = x.astype(np.float16).astype(np.float32)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_tonemap_closed_forms():
+    """0 maps to 0; x = ln 2 / exposure maps to 0.5; large x to 1; gamma 2 is the square root of the gamma-1
+    value; alpha passes through rounded; elsewhere within one half ulp of the float64 curve"""
+    e = 2.0
+    x = np.array([[0, 0, 0, 0.25], [np.log(2) / e] * 3 + [1.0], [50.0, 80.0, 1e30, 0.1]], np.float32)
+    out = orc.tonemap(x, e, 1.0)
+    assert (out[0, :3] == 0).all() and out[0, 3] == 0.25
+    assert (out[1, :3] == 0.5).all() and out[1, 3] == 1.0
+    assert (out[2, :3] == 1.0).all() and out[2, 3] == np.float32(np.float16(0.1))
+    rng = np.random.default_rng(4)
+    v = np.concatenate([rng.exponential(0.5, (2000, 4)), rng.uniform(0, 1e-3, (200, 4))]).astype(np.float32)
+    for gamma in (1.0, 2.0, 2.2):
+        got = orc.tonemap(v, e, gamma)[:, :3].astype(np.float64)
+        r = 1.0 - np.exp(-v[:, :3].astype(np.float64) * e)
+        exact = r ** (1.0 / gamma)
+        # one half ulp at the larger binade, plus 1 - expf's float32 cancellation carried through the power
+        bound = np.spacing(np.maximum(got, exact).astype(np.float16)).astype(np.float64)
+        bound += r ** (1.0 / gamma - 1.0) / gamma * 2.0 ** -23
+        assert (np.abs(got - exact) <= bound).all()
+        assert np.mean(got == exact.astype(np.float16).astype(np.float64)) > 0.98
+    g1 = orc.tonemap(v, e, 1.0)[:, :3].astype(np.float64)
+    g2 = orc.tonemap(v, e, 2.0)[:, :3].astype(np.float64)
+    assert (np.abs(g2 - np.sqrt(g1)) <= 2 * np.spacing(np.float16(g2))).all()

@@ -50,7 +50,8 @@ class PtSettings(ctypes.Structure):
                 ("num_samples", ctypes.c_uint32), ("max_trace_depth", ctypes.c_uint32),
                 ("max_path_beta", ctypes.c_float), ("suspend_trace_when_done", ctypes.c_uint32),
                 ("disable_material", ctypes.c_uint32), ("suppress_reset", ctypes.c_uint32),
-                ("denoising", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
+                ("denoising", ctypes.c_uint32), ("tonemap", ctypes.c_uint32), ("gamma", ctypes.c_float),
+                ("exposure", ctypes.c_float), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class PtScene(ctypes.Structure):
@@ -174,6 +175,7 @@ _SIGNATURES = {
     "vkrt_pt_get_stats": (_i32, [_vp, ctypes.POINTER(PtStats)]),
     "vkrt_pt_read_image": (_i32, [_vp, _vp, _i32]),
     "vkrt_denoise": (_i32, [_vp, _vp, _u32, _u32, _u32, _vp, _i32]),
+    "vkrt_tonemap": (_i32, [_vp, _vp, _u32, ctypes.c_float, ctypes.c_float, _vp, _i32]),
     "vkrt_rc_default_params": (None, [ctypes.POINTER(RcParams)]),
     "vkrt_rc_create": (_i32, [_vp, ctypes.POINTER(RcParams), _u32, ctypes.POINTER(_vp)]),
     "vkrt_rc_destroy": (None, [_vp]),
@@ -441,6 +443,14 @@ class Context:
         self.check(lib().vkrt_denoise(self._h, px.ctypes.data, w, h, 1 if denoise else 0, out.ctypes.data, 0))
         return out
 
+    def tonemap(self, rgba, exposure=2.0, gamma=1.0):
+        """the exposure/gamma composite over (..., 4) float32 texels, each channel rounded to half"""
+        a = np.ascontiguousarray(rgba, np.float32)
+        assert a.shape[-1] == 4
+        out = np.empty_like(a)
+        self.check(lib().vkrt_tonemap(self._h, a.ctypes.data, a.size // 4, exposure, gamma, out.ctypes.data, 0))
+        return out
+
     def compute_bounding_sphere(self, points, out=None):
         """(center.x, center.y, center.z, radius): compute_bounding_sphere of the reference, bit-identical"""
         n = _n(points, 3)
@@ -650,7 +660,8 @@ class PathTracer:
         return out.reshape(s.resolution[1], s.resolution[0])
 
     def image(self):
-        """the output image (H, W, 4): denoised rgb when settings.denoising, else radiance and alpha"""
+        """the output image (H, W, 4): denoised rgb when settings.denoising, else radiance and alpha; then the
+        exposure/gamma composite (half-rounded) when settings.tonemap"""
         s = self.settings
         out = np.zeros((s.resolution[1], s.resolution[0], 4), np.float32)
         self.ctx.check(lib().vkrt_pt_read_image(self._h, out.ctypes.data, 0))

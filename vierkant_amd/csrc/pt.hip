@@ -655,33 +655,72 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_no
 // ---------------------------------------------------------------------------------------------
 constexpr uint32_t ATROUS_STEPS[5] = {1, 2, 4, 8, 16};
 
+//! exposure/gamma parameters of the tonemap stage; `on` is 0 when the image stays linear
+struct tonemap_params
+{
+    int on;
+    float exposure, inv_gamma;
+};
+
+//! the stored value of one R16G16B16A16_SFLOAT channel: round to nearest half, ties to even
+__device__ inline float to_half_value(float v) { return static_cast<float>(static_cast<_Float16>(v)); }
+
+__device__ inline float4 tonemap_texel(float4 c, const tonemap_params &tm)
+{
+    if(!tm.on) return c;
+    return make_float4(to_half_value(tonemap_channel(c.x, tm.exposure, tm.inv_gamma)),
+                       to_half_value(tonemap_channel(c.y, tm.exposure, tm.inv_gamma)),
+                       to_half_value(tonemap_channel(c.z, tm.exposure, tm.inv_gamma)), to_half_value(c.w));
+}
+
+//! one pass of denoise.slang (16 x 16 pixels per workgroup, as its numthreads); the last pass of a chain
+//! also applies the tonemap stage, so the composited image costs no extra read and write
 __global__ void __launch_bounds__(256) k_denoise(const vkrt_pixel *__restrict__ px, const float4 *__restrict__ prev,
                                                  float4 *__restrict__ out, uint32_t w, uint32_t h, int step,
-                                                 int denoise)
+                                                 int denoise, tonemap_params tm)
 {
     const uint32_t x = blockIdx.x * 16 + threadIdx.x % 16, y = blockIdx.y * 16 + threadIdx.x / 16;
     if(x >= w || y >= h) return;
     const uint32_t i = y * w + x;
     if(!denoise)
     {
-        out[i] = make_float4(px[i].radiance[0], px[i].radiance[1], px[i].radiance[2], px[i].alpha);
+        out[i] = tonemap_texel(make_float4(px[i].radiance[0], px[i].radiance[1], px[i].radiance[2], px[i].alpha),
+                               tm);
         return;
     }
     const vec3 c = atrous_pixel(px, reinterpret_cast<const float *>(prev), w, h, static_cast<int>(x),
                                 static_cast<int>(y), step);
-    out[i] = make_float4(c.x, c.y, c.z, 0.f);
+    out[i] = tonemap_texel(make_float4(c.x, c.y, c.z, 0.f), tm);
 }
 
-//! the image of a pixel buffer: five a-trous passes when denoising; returns the buffer holding the result
-static float4 *run_denoise(hipStream_t s, const vkrt_pixel *px, uint32_t w, uint32_t h, bool denoise, float4 *ping,
-                           float4 *pong)
+__global__ void __launch_bounds__(256) k_tonemap(const float4 *in, float4 *out, size_t n,
+                                                 tonemap_params tm)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if(i < n) out[i] = tonemap_texel(in[i], tm);
+}
+
+static tonemap_params make_tonemap(bool on, float exposure, float gamma)
+{
+    VKRT_REQUIRE(!on || (gamma > 0.f && std::isfinite(gamma)), "gamma must be positive and finite");
+    VKRT_REQUIRE(!on || std::isfinite(exposure), "exposure must be finite");
+    // bloom_composition.slang:44 raises to float3(1.0 / gamma), a single-precision quotient
+    return tonemap_params{on ? 1 : 0, exposure, on ? 1.0f / gamma : 1.0f};
+}
+
+//! the image of a pixel buffer: five a-trous passes when denoising, the last one tonemapping when asked;
+//! returns the buffer holding the result
+static float4 *run_denoise(hipStream_t s, const vkrt_pixel *px, uint32_t w, uint32_t h, bool denoise,
+                           const tonemap_params &tm, float4 *ping, float4 *pong)
 {
     const dim3 grid(div_up(w, 16), div_up(h, 16));
     float4 *bufs[2] = {ping, pong};
     const uint32_t passes = denoise ? 5u : 1u;
+    const tonemap_params off{0, 0.f, 1.f};
     for(uint32_t p = 0; p < passes; ++p)
         k_denoise<<<grid, 256, 0, s>>>(px, p ? bufs[(p - 1) % 2] : nullptr, bufs[p % 2], w, h,
-                                       static_cast<int>(ATROUS_STEPS[p]), denoise ? 1 : 0);
+                                       static_cast<int>(ATROUS_STEPS[p]), denoise ? 1 : 0,
+                                       p + 1 == passes ? tm : off);
     VKRT_HIP(hipGetLastError());
     return bufs[(passes - 1) % 2];
 }
@@ -740,6 +779,10 @@ static void default_settings(vkrt_pt_settings &s)
     s.suspend_trace_when_done = 1;
     s.disable_material = 0;
     s.suppress_reset = 0;
+    s.denoising = 0;
+    s.tonemap = 1;// settings_t defaults (PBRPathTracer.hpp:77-90)
+    s.gamma = 1.0f;
+    s.exposure = 2.0f;
 }
 
 static void check_settings(const vkrt_pt_settings &s)
@@ -748,6 +791,8 @@ static void check_settings(const vkrt_pt_settings &s)
     VKRT_REQUIRE((uint64_t)s.resolution[0] * s.resolution[1] <= (1u << 26), "at most 2^26 pixels");
     VKRT_REQUIRE(s.num_samples > 0 && s.num_samples <= 65536, "num_samples must be in 1..65536");
     VKRT_REQUIRE(s.max_trace_depth > 0 && s.max_trace_depth <= 64, "max_trace_depth must be in 1..64");
+    VKRT_REQUIRE(!s.tonemap || (s.gamma > 0.f && std::isfinite(s.gamma) && std::isfinite(s.exposure)),
+                 "tonemap needs a positive finite gamma and a finite exposure");
 }
 
 //! resize_storage (PBRPathTracer.cpp:684-830): new pixel storage resets the batch index
@@ -1103,10 +1148,28 @@ int vkrt_denoise(vkrt_ctx *ctx, const vkrt_pixel *pixels, uint32_t width, uint32
         const vkrt_pixel *dp = input(ctx, 0, pixels, n, flags);
         ctx->scratch[2].reserve(2 * n * sizeof(float4));
         float4 *ping = ctx->scratch[2].as<float4>(), *pong = ping + n;
-        float4 *res = run_denoise(ctx->stream, dp, width, height, denoise != 0, ping, pong);
+        float4 *res = run_denoise(ctx->stream, dp, width, height, denoise != 0, make_tonemap(false, 0.f, 1.f), ping,
+                                  pong);
         VKRT_HIP(hipMemcpyAsync(out_rgba, res, n * sizeof(float4),
                                 (flags & VKRT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                 ctx->stream));
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_tonemap(vkrt_ctx *ctx, const float *in_rgba, uint32_t n, float exposure, float gamma, float *out_rgba,
+                 int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(n <= (1u << 26), "too many texels");
+        VKRT_REQUIRE(n == 0 || (in_rgba && out_rgba), "null array argument");
+        const tonemap_params tm = make_tonemap(true, exposure, gamma);
+        if(n == 0) return;
+        const float4 *din = input(ctx, 0, reinterpret_cast<const float4 *>(in_rgba), n, flags);
+        float4 *dout = output<float4>(ctx, 1, reinterpret_cast<float4 *>(out_rgba), n, flags);
+        k_tonemap<<<div_up(n, 256u), 256, 0, ctx->stream>>>(din, dout, n, tm);
+        VKRT_HIP(hipGetLastError());
+        finish_output(ctx, 1, reinterpret_cast<float4 *>(out_rgba), n, flags);
         finish_call(ctx, flags);
     });
 }
@@ -1121,8 +1184,9 @@ int vkrt_pt_read_image(vkrt_pt *pt, float *out_rgba, int flags)
         const size_t n = (size_t)pt->width * pt->height;
         pt->image.reserve(2 * n * sizeof(float4));
         float4 *ping = pt->image.as<float4>(), *pong = ping + n;
+        const tonemap_params tm = make_tonemap(pt->settings.tonemap != 0, pt->settings.exposure, pt->settings.gamma);
         float4 *res = run_denoise(ctx->stream, pt->pixels.as<vkrt_pixel>(), pt->width, pt->height,
-                                  pt->settings.denoising != 0, ping, pong);
+                                  pt->settings.denoising != 0, tm, ping, pong);
         VKRT_HIP(hipMemcpyAsync(out_rgba, res, n * sizeof(float4),
                                 (flags & VKRT_DEVICE_PTRS) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                 ctx->stream));

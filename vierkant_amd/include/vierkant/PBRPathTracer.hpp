@@ -1,10 +1,13 @@
 // PBRPathTracer.hpp - the path tracer for C++ callers, with the reference's settings and accumulation
 // semantics (include/vierkant/PBRPathTracer.hpp:42-142, src/PBRPathTracer.cpp:184-255, :677-682).
 //
-// settings_t keeps the reference's field names and defaults for everything that changes the traced image;
-// denoising selects the a-trous denoiser for image(); the other display-side fields (tonemap, bloom, gamma,
-// exposure, ...) are kept for source compatibility and have no effect here. render_scene traces one batch unless max_num_batches is reached with
-// suspend_trace_when_done set, then advances the batch index as min(batch + 1, max_num_batches).
+// settings_t keeps the reference's field names and defaults for everything that changes the traced image.
+// image() follows the reference's display side: denoising selects the a-trous denoiser, and tonemap, gamma
+// and exposure the composite of bloom_composition.slang. Bloom is not provided (the image is the reference's
+// with bloom off); bloom and the other display-only fields are kept for source compatibility. Settings are
+// copied at render_scene, as the reference copies them into its frame context. render_scene traces one batch
+// unless max_num_batches is reached with suspend_trace_when_done set, then advances the batch index as
+// min(batch + 1, max_num_batches).
 #pragma once
 
 #include <chrono>
@@ -105,7 +108,8 @@ public:
         return out;
     }
 
-    //! the output image (float4 per pixel): denoised when settings.denoising (PBRPathTracer::denoise_pass)
+    //! the output image (float4 per pixel) with the settings of the last render_scene: denoised when
+    //! denoising (PBRPathTracer::denoise_pass), then composited when tonemap (PBRPathTracer::post_fx_pass)
     [[nodiscard]] std::vector<float> image() const
     {
         vkrt_pt_settings s{};
@@ -139,6 +143,9 @@ private:
         c.disable_material = s.disable_material ? 1u : 0u;
         c.suppress_reset = s.suppress_reset ? 1u : 0u;
         c.denoising = s.denoising ? 1u : 0u;
+        c.tonemap = s.tonemap ? 1u : 0u;
+        c.gamma = s.gamma;
+        c.exposure = s.exposure;
         return c;
     }
 

@@ -512,4 +512,12 @@ VK_HD vec3 atrous_pixel(const vkrt_pixel *px, const float *prev, uint32_t w, uin
     return vec3{(sum.x / cum_w) * eff.x, (sum.y / cum_w) * eff.y, (sum.z / cum_w) * eff.z};
 }
 
+//! one channel of bloom_composition.slang:34-45 without bloom or motion blur: utils/tonemap.slang's
+//! tonemap_exposure, 1 - exp(-x * exposure), then pow(., 1 / gamma) (pow(r, 1) is r)
+VK_HD float tonemap_channel(float x, float exposure, float inv_gamma)
+{
+    const float r = 1.0f - expf(-x * exposure);
+    return inv_gamma == 1.0f ? r : powf(r, inv_gamma);
+}
+
 }// namespace vierkant::pt
