@@ -343,7 +343,7 @@ typedef struct vkrt_pt vkrt_pt;
 /* PBRPathTracer::settings_t fields that affect the traced image (PBRPathTracer.hpp:42-110). Defaults
  * (vkrt_pt_default_settings): 1280x720, max_num_batches 0, num_samples 1, max_trace_depth 6,
  * max_path_beta 0, suspend_trace_when_done 1, disable_material 0, suppress_reset 0, denoising 0, tonemap 1,
- * gamma 1, exposure 2 (include/vierkant/PBRPathTracer.hpp:42-90). Bloom is not provided: the composited
+ * gamma 1, exposure 2, mis_mode 0, suppress_refractive_caustics 0 (include/vierkant/PBRPathTracer.hpp:42-90). Bloom is not provided: the composited
  * image is the reference's with settings_t::bloom false. */
 typedef struct vkrt_pt_settings
 {
@@ -359,8 +359,50 @@ typedef struct vkrt_pt_settings
     uint32_t tonemap;       /* settings_t::tonemap: vkrt_pt_read_image composites as bloom_composition.slang */
     float gamma;            /* settings_t::gamma (> 0 with tonemap) */
     float exposure;         /* settings_t::exposure */
-    uint32_t reserved[3];
+    uint32_t mis_mode;      /* settings_t::mis_mode: VKRT_MIS_FULL, _NEE_ONLY, _BSDF_ONLY (ray_common.slang:289-294) */
+    uint32_t suppress_refractive_caustics; /* settings_t::suppress_refractive_caustics */
+    uint32_t reserved[1];
 } vkrt_pt_settings;
+
+/* Direct-light estimator (MisMode, shaders/slang/ray/ray_common.slang:289-294). */
+enum
+{
+    VKRT_MIS_FULL = 0,      /* next-event estimation and BSDF sampling, power heuristic */
+    VKRT_MIS_NEE_ONLY = 1,  /* light bodies reached by BSDF rays add nothing */
+    VKRT_MIS_BSDF_ONLY = 2  /* next-event samples add nothing for lights BSDF rays can reach */
+};
+
+/* LightType (include/vierkant/punctual_light.hpp:20-33, shaders/slang/ray/ray_common.slang:92-107). */
+enum
+{
+    VKRT_LIGHT_OMNI = 0,
+    VKRT_LIGHT_SPOT = 1,
+    VKRT_LIGHT_DIRECTIONAL = 2, /* angular_size > 0: a sun disc of that apex angle */
+    VKRT_LIGHT_AREA = 3,        /* reserved by the reference for emissive triangles; never gathered */
+    VKRT_LIGHT_RECT = 4,
+    VKRT_LIGHT_SPHERE = 5,
+    VKRT_LIGHT_TUBE = 6,
+    VKRT_LIGHT_DISK = 7
+};
+
+/* light_t, the packed light the shaders read (include/vierkant/punctual_light.hpp:64-89; 80 bytes).
+ * direction is the propagation direction; color * intensity is the emitted radiance of light bodies and the
+ * intensity of punctual lights; size_x is the radius (sphere, disk, tube) or half-width (rect), size_y the
+ * half-height (rect) or half-length (tube); tangent is the rect's in-plane x axis. */
+typedef struct vkrt_light
+{
+    float position[3];
+    uint32_t type;
+    float color[3];
+    float intensity;
+    float direction[3];
+    float range;
+    float spot_angle_scale, spot_angle_offset;
+    float angular_size;
+    float size_y;
+    float tangent[3];
+    float size_x;
+} vkrt_light;
 
 /* material_t subset evaluated by the tracer (shaders/slang/ray/types.slang:64-107): base color, emission
  * (rgb, a = strength), metal/roughness, transmission + ior, clearcoat, sheen, diffuse transmission. */
@@ -434,6 +476,12 @@ int vkrt_pt_set_settings(vkrt_pt *pt, const vkrt_pt_settings *settings);
 int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out);
 /* Upload the scene and build its BVH (the RayBuilder step of the reference's render_scene). */
 int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *scene);
+/* The frame's light list (PBRPathTracer::update_trace_descriptors, src/PBRPathTracer.cpp:607-641): lights of
+ * positive intensity, other than VKRT_LIGHT_AREA, are kept in the given order (a sun first, as the reference
+ * folds it in). With n > 0 after that filter every surface vertex adds next-event estimation with a
+ * transmittance shadow ray (ray/direct_lighting.slang:32-366), and light bodies met by BSDF rays add their
+ * MIS-weighted radiance (raypipeline.slang:201-222, :728-761). At most 65536 lights; n = 0 clears the list. */
+int vkrt_pt_set_lights(vkrt_pt *pt, const vkrt_light *lights, uint32_t n);
 /* One render_scene call: traces one batch of settings.num_samples samples per pixel into the running mean
  * unless max_num_batches is reached with suspend_trace_when_done set; then advances the batch index as
  * min(batch + 1, max_num_batches). A resolution change resets the accumulator. */

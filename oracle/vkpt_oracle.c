@@ -2,11 +2,13 @@
  * vkpt_oracle.c - CPU reference integrator for the path tracer (TEST INFRASTRUCTURE ONLY).
  *
  * A plain-C restatement, pixel by pixel, of the reference's ray-tracing programs for the scenes the
- * tracer supports (no textures, no lights[] / NEE, no participating media, no iridescence/dispersion):
- *   raygen trace() + main             shaders/slang/raypipeline.slang:58-270
- *   closest_hit (surface subset)      shaders/slang/raypipeline.slang:360-680
- *   miss_default                      shaders/slang/raypipeline.slang:746-761
- *   sample_disney + lobes             shaders/slang/ray/bsdf_disney.slang:44-296
+ * tracer supports (no textures, no participating media, no iridescence/dispersion):
+ *   raygen trace() + main, ghost-light pass    shaders/slang/raypipeline.slang:58-270 (:201-222)
+ *   closest_hit (surface subset), NEE call     shaders/slang/raypipeline.slang:360-680 (:615-619)
+ *   miss_default, eval_distant_lights          shaders/slang/raypipeline.slang:728-761
+ *   sample_disney, eval_disney + lobes         shaders/slang/ray/bsdf_disney.slang:44-381
+ *   analytic lights, MIS, nee_surface          shaders/slang/ray/direct_lighting.slang:32-336
+ *   transmittance_test (shadow rays)           shaders/slang/ray/visibility_test.slang:94-165
  *   bsdf / sampling / random / hash   shaders/slang/utils/{bsdf,sampling,random,hash}.slang
  *   environment_white                 shaders/slang/utils/procedural_environment.slang:7-11
  * Closest hits use the oracle's restatement of intersect(Triangle, Ray) (vkoracle.c) with tmin < t < tmax,
@@ -511,6 +513,378 @@ static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int c
     return best;
 }
 
+/* transmittance_test (visibility_test.slang:94-165) without culling: 0 when an opaque triangle lies in
+ * (tmin, tmax), else the product of transmission * color over the transmissive hits in ascending
+ * (t, triangle) order; mat_of_prim maps a triangle to its material */
+#define ORC_MAX_TRANSMISSIVE 4096
+typedef struct
+{
+    float t;
+    uint32_t prim;
+} thit_t;
+
+static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim, v3 o, v3 d, float tmin, float tmax)
+{
+    double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+    static __thread thit_t hits[ORC_MAX_TRANSMISSIVE];
+    uint32_t nh = 0, stack[128];
+    int sp = 0;
+    if(b->n_nodes) stack[sp++] = 0;
+    while(sp)
+    {
+        const cnode *nd = &b->nodes[stack[--sp]];
+        if(!box_hit(nd, od, inv, tmin, tmax)) continue;
+        if(nd->left >= 0)
+        {
+            stack[sp++] = (uint32_t)nd->left;
+            stack[sp++] = (uint32_t)nd->right;
+            continue;
+        }
+        for(uint32_t k = nd->first; k < nd->first + nd->count; ++k)
+        {
+            uint32_t prim = b->order[k];
+            float t, u, v;
+            if(!orc_intersect_tri(b->tris + 9 * (size_t)prim, &o.x, &d.x, &t, &u, &v)) continue;
+            if(!(t > tmin && t < tmax)) continue;
+            if(!(mats[mat_of_prim[prim]].trans > 0.0f)) return V3(0, 0, 0);
+            if(nh == ORC_MAX_TRANSMISSIVE) abort();
+            hits[nh].t = t;
+            hits[nh].prim = prim;
+            ++nh;
+        }
+    }
+    for(uint32_t a = 1; a < nh; ++a)
+    {
+        thit_t x = hits[a];
+        uint32_t c = a;
+        while(c > 0 && (hits[c - 1].t > x.t || (hits[c - 1].t == x.t && hits[c - 1].prim > x.prim)))
+        {
+            hits[c] = hits[c - 1];
+            --c;
+        }
+        hits[c] = x;
+    }
+    v3 T = V3(1, 1, 1);
+    for(uint32_t a = 0; a < nh; ++a)
+    {
+        const mat_t *m = &mats[mat_of_prim[hits[a].prim]];
+        T = V3(T.x * (m->trans * m->color.x), T.y * (m->trans * m->color.y), T.z * (m->trans * m->color.z));
+    }
+    return T;
+}
+
+/* eval_disney (bsdf_disney.slang:298-381), iridescence strength 0 */
+static v3 eval_disney(mat_t m, v3 L, v3 N, v3 V, float eta, float *pdf)
+{
+    int refl = dot(N, L) > 0.0f;
+    v3 LpV = refl ? add(L, V) : add(L, scl(V, eta));
+    float len = sqrtf(dot(LpV, LpV));
+    if(len < 1e-7f)
+    {
+        *pdf = 0.0f;
+        return V3(0, 0, 0);
+    }
+    v3 H = V3(LpV.x / len, LpV.y / len, LpV.z / len);
+    if(dot(V, H) < 0.0f) H = neg(H);
+    if(refl && dot(N, H) <= 0.0f)
+    {
+        *pdf = 0.0f;
+        return V3(0, 0, 0);
+    }
+    float diffuse_ratio = 0.5f * (1.0f - m.metal);
+    float psr = 1.0f / (1.0f + m.cc);
+    float tw = (1.0f - m.metal) * m.trans;
+    m.rough = fmaxf(0.001f, m.rough * m.rough);
+    m.sheen_rough = fmaxf(0.001f, m.sheen_rough);
+    v3 spec_color = lerp3(V3(0.08f, 0.08f, 0.08f), m.color, m.metal);
+    v3 brdf = V3(0, 0, 0), bsdf = V3(0, 0, 0);
+    float brdf_pdf = 0.0f, bsdf_pdf = 0.0f, mp = 0.0f;
+    if(tw > 0.0f)
+    {
+        if(refl)
+        {
+            bsdf = add(bsdf, scl(ev_diel_refl(m.rough, m.color, eta, V, N, L, H, &mp), 1.0f - 0.0f));
+            bsdf_pdf += mp * (1.0f - 0.0f);
+        }
+        else
+            bsdf = ev_diel_refr(m.rough, m.color, eta, V, N, L, H, &bsdf_pdf);
+    }
+    if(tw < 1.0f)
+    {
+        float dt = m.dt;
+        brdf = add(brdf, scl(ev_diffuse(m.rough, m.sheen_rough, m.color, m.sheen, m.metal, V, N, L, H, &mp), 1.0f - dt));
+        brdf_pdf += mp * diffuse_ratio * (1.0f - dt);
+        if(dt > 0.0f && !refl)
+        {
+            float cos_t = -dot(N, L);
+            brdf = add(brdf, scl(scl(m.dt_color, dt), ONE_OVER_PI));
+            brdf_pdf += cos_t * ONE_OVER_PI * diffuse_ratio * dt;
+        }
+        brdf = add(brdf, scl(ev_spec(m.rough, spec_color, V, N, L, H, &mp), 1.0f - 0.0f));
+        brdf_pdf += mp * psr * (1.0f - diffuse_ratio) * (1.0f - 0.0f);
+        brdf = add(brdf, ev_cc(m.cc_rough, m.cc, V, N, L, H, &mp));
+        brdf_pdf += mp * (1.0f - psr) * (1.0f - diffuse_ratio);
+    }
+    float p = lerpf(brdf_pdf, bsdf_pdf, tw);
+    v3 F = lerp3(brdf, bsdf, tw);
+    int valid = p > 0.0f;
+    *pdf = valid ? p : 0.0f;
+    return valid ? F : V3(0, 0, 0);
+}
+
+/* ---- analytic lights: light_t records of 20 words (include/vierkant/punctual_light.hpp:64-89) ---- */
+enum { L_OMNI = 0, L_SPOT = 1, L_DIRECTIONAL = 2, L_AREA = 3, L_RECT = 4, L_SPHERE = 5, L_TUBE = 6, L_DISK = 7 };
+enum { MIS_FULL = 0, MIS_NEE_ONLY = 1, MIS_BSDF_ONLY = 2 };
+
+typedef struct
+{
+    v3 position;
+    uint32_t type;
+    v3 color;
+    float intensity;
+    v3 direction;
+    float range, spot_scale, spot_offset, angular_size, size_y;
+    v3 tangent;
+    float size_x;
+} light_t;
+
+static light_t read_light(const float *w)
+{
+    light_t l;
+    l.position = V3(w[0], w[1], w[2]);
+    memcpy(&l.type, &w[3], 4);
+    l.color = V3(w[4], w[5], w[6]);
+    l.intensity = w[7];
+    l.direction = V3(w[8], w[9], w[10]);
+    l.range = w[11];
+    l.spot_scale = w[12];
+    l.spot_offset = w[13];
+    l.angular_size = w[14];
+    l.size_y = w[15];
+    l.tangent = V3(w[16], w[17], w[18]);
+    l.size_x = w[19];
+    return l;
+}
+
+typedef struct
+{
+    v3 dir;
+    float dist, pdf;
+    v3 radiance;
+} lsample_t;
+
+/* sample_unit_sphere_cap (utils/sampling.slang:55-64) */
+static v3 sphere_cap(float xi_x, float xi_y, float sigma)
+{
+    float phi = 2.0f * PI * xi_y;
+    float cs = cosf(sigma);
+    float ct = xi_x * (1.0f - cs) + cs;
+    float st = sqrtf(1.0f - ct * ct);
+    return sph(st, ct, phi);
+}
+
+/* sample_light (direct_lighting.slang:32-136) */
+static lsample_t sample_light(const light_t *l, v3 P, float xi_x, float xi_y)
+{
+    lsample_t r;
+    r.pdf = 1.0f;
+    r.radiance = scl(l->color, l->intensity);
+    r.dir = V3(0, 0, 1);
+    r.dist = EPS;
+    if(l->type == L_DIRECTIONAL)
+    {
+        r.dir = neg(l->direction);
+        r.dist = DEFAULT_TMAX;
+        if(l->angular_size > 0.0f)
+        {
+            r.dir = to_world(local_frame(r.dir), sphere_cap(xi_x, xi_y, l->angular_size));
+            r.pdf = 1.0f / (2.0f * PI * (1.0f - cosf(l->angular_size)));
+        }
+    }
+    else if(l->type == L_SPHERE)
+    {
+        v3 tc = sub(l->position, P);
+        float d2 = dot(tc, tc), r2 = l->size_x * l->size_x;
+        if(d2 <= r2)
+        {
+            r.radiance = V3(0, 0, 0);
+            r.dir = V3(0, 0, 1);
+            r.dist = EPS;
+        }
+        else
+        {
+            float s2 = r2 / d2;
+            float cs = sqrtf(1.0f - s2);
+            r.dir = to_world(local_frame(scl(tc, 1.0f / sqrtf(d2))), sphere_cap(xi_x, xi_y, asinf(sqrtf(s2))));
+            r.pdf = (1.0f + cs) / (2.0f * PI * s2);
+            float b = dot(r.dir, tc);
+            r.dist = b - sqrtf(fmaxf(b * b - d2 + r2, 0.0f));
+        }
+    }
+    else if(l->type == L_RECT || l->type == L_DISK || l->type == L_TUBE)
+    {
+        v3 Q, n;
+        if(l->type == L_RECT)
+        {
+            v3 bt = cross(l->direction, l->tangent);
+            Q = add(add(l->position, scl(l->tangent, (2.0f * xi_x - 1.0f) * l->size_x)),
+                    scl(bt, (2.0f * xi_y - 1.0f) * l->size_y));
+            n = l->direction;
+            r.pdf = 1.0f / (4.0f * l->size_x * l->size_y);
+        }
+        else if(l->type == L_DISK)
+        {
+            float rr = sqrtf(xi_x), th = 2.0f * PI * xi_y;
+            float dx = rr * cosf(th), dy = rr * sinf(th);
+            Q = add(l->position, to_world(local_frame(l->direction), V3(l->size_x * dx, l->size_x * dy, 0.0f)));
+            n = l->direction;
+            r.pdf = 1.0f / (PI * l->size_x * l->size_x);
+        }
+        else
+        {
+            float phi = 2.0f * PI * xi_x;
+            n = to_world(local_frame(l->direction), V3(cosf(phi), sinf(phi), 0.0f));
+            Q = add(add(l->position, scl(l->direction, (2.0f * xi_y - 1.0f) * l->size_y)), scl(n, l->size_x));
+            r.pdf = 1.0f / (4.0f * PI * l->size_x * l->size_y);
+        }
+        v3 tq = sub(Q, P);
+        float d2 = dot(tq, tq);
+        r.dist = fmaxf(sqrtf(d2), EPS);
+        r.dir = V3(tq.x / r.dist, tq.y / r.dist, tq.z / r.dist);
+        float cl = dot(n, neg(r.dir));
+        r.pdf *= d2 / fmaxf(cl, EPS);
+        if(cl <= 0.0f) r.radiance = V3(0, 0, 0);
+    }
+    else
+    {
+        v3 tl = sub(l->position, P);
+        r.dist = fmaxf(sqrtf(dot(tl, tl)), EPS);
+        r.dir = V3(tl.x / r.dist, tl.y / r.dist, tl.z / r.dist);
+        float w = r.dist / l->range;
+        w *= w;
+        float att = clampf(1.0f - w * w, 0.0f, 1.0f) / (r.dist * r.dist);
+        if(l->type == L_SPOT)
+        {
+            float cd = dot(l->direction, neg(r.dir));
+            float a = clampf(cd * l->spot_scale + l->spot_offset, 0.0f, 1.0f);
+            att *= a * a;
+        }
+        r.radiance = scl(r.radiance, att);
+    }
+    return r;
+}
+
+static float power_heuristic(float pa, float pb)
+{
+    float r = pb / pa;
+    return 1.0f / (1.0f + r * r);
+}
+
+static int bsdf_reachable(const light_t *l)
+{
+    return (l->type == L_DIRECTIONAL && l->angular_size > 0.0f) || l->type == L_RECT || l->type == L_SPHERE ||
+           l->type == L_TUBE || l->type == L_DISK;
+}
+
+/* intersect_light (direct_lighting.slang:166-236): front faces of light bodies; t <= 0 is a miss */
+static float intersect_light(const light_t *l, v3 P, v3 d, float *cos_light)
+{
+    *cos_light = 0.0f;
+    if(l->type == L_SPHERE)
+    {
+        v3 tc = sub(l->position, P);
+        float d2 = dot(tc, tc), r2 = l->size_x * l->size_x;
+        if(d2 <= r2) return 0.0f;
+        float b = dot(d, tc);
+        float disc = b * b - d2 + r2;
+        if(disc <= 0.0f) return 0.0f;
+        float t = b - sqrtf(disc);
+        if(t <= 0.0f) return 0.0f;
+        v3 q = sub(add(P, scl(d, t)), l->position);
+        *cos_light = dot(V3(q.x / l->size_x, q.y / l->size_x, q.z / l->size_x), neg(d));
+        return t;
+    }
+    if(l->type == L_RECT || l->type == L_DISK)
+    {
+        float cl = dot(l->direction, neg(d));
+        if(cl <= 0.0f) return 0.0f;
+        float t = dot(sub(l->position, P), l->direction) / -cl;
+        if(t <= 0.0f) return 0.0f;
+        v3 q = sub(add(P, scl(d, t)), l->position);
+        if(l->type == L_RECT)
+        {
+            v3 bt = cross(l->direction, l->tangent);
+            if(fabsf(dot(q, l->tangent)) > l->size_x || fabsf(dot(q, bt)) > l->size_y) return 0.0f;
+        }
+        else if(dot(q, q) > l->size_x * l->size_x)
+            return 0.0f;
+        *cos_light = cl;
+        return t;
+    }
+    if(l->type == L_TUBE)
+    {
+        v3 o = sub(P, l->position);
+        v3 dp = sub(d, scl(l->direction, dot(d, l->direction)));
+        v3 op = sub(o, scl(l->direction, dot(o, l->direction)));
+        float A = dot(dp, dp), B = dot(op, dp), C = dot(op, op) - l->size_x * l->size_x;
+        float disc = B * B - A * C;
+        if(A <= EPS || disc <= 0.0f || C <= 0.0f) return 0.0f;
+        float t = (-B - sqrtf(disc)) / A;
+        if(t <= 0.0f || fabsf(dot(add(o, scl(d, t)), l->direction)) > l->size_y) return 0.0f;
+        v3 nq = add(op, scl(dp, t));
+        float cl = dot(V3(nq.x / l->size_x, nq.y / l->size_x, nq.z / l->size_x), neg(d));
+        if(cl <= 0.0f) return 0.0f;
+        *cos_light = cl;
+        return t;
+    }
+    return 0.0f;
+}
+
+static float mis_weight_nee(uint32_t mode, float p_nee, float p_bsdf)
+{
+    if(mode == MIS_NEE_ONLY) return 1.0f;
+    if(mode == MIS_BSDF_ONLY) return 0.0f;
+    return power_heuristic(p_nee, p_bsdf);
+}
+
+static float mis_weight_hit(uint32_t mode, float prev_pdf, float p_nee)
+{
+    if(mode == MIS_NEE_ONLY) return 0.0f;
+    if(mode == MIS_BSDF_ONLY || prev_pdf <= 0.0f) return 1.0f;
+    return power_heuristic(prev_pdf, p_nee);
+}
+
+/* light_pdf (direct_lighting.slang:254-292) */
+static float light_pdf(const light_t *l, v3 P, v3 d)
+{
+    if(l->type == L_DIRECTIONAL)
+    {
+        if(l->angular_size <= 0.0f) return 0.0f;
+        float cs = cosf(l->angular_size);
+        return dot(d, neg(l->direction)) < cs ? 0.0f : 1.0f / (2.0f * PI * (1.0f - cs));
+    }
+    if(l->type == L_SPHERE)
+    {
+        v3 tc = sub(l->position, P);
+        float d2 = dot(tc, tc), r2 = l->size_x * l->size_x;
+        if(d2 <= r2) return 0.0f;
+        float s2 = r2 / d2;
+        float cs = sqrtf(1.0f - s2);
+        return dot(d, scl(tc, 1.0f / sqrtf(d2))) < cs ? 0.0f : (1.0f + cs) / (2.0f * PI * s2);
+    }
+    if(l->type == L_RECT || l->type == L_DISK || l->type == L_TUBE)
+    {
+        float cl;
+        float t = intersect_light(l, P, d, &cl);
+        if(t <= 0.0f) return 0.0f;
+        float area = l->type == L_RECT ? 4.0f * l->size_x * l->size_y
+                     : l->type == L_DISK ? PI * l->size_x * l->size_x
+                                         : 4.0f * PI * l->size_x * l->size_y;
+        return t * t / (area * cl);
+    }
+    return 0.0f;
+}
+
 /* ---- the integrator ------------------------------------------------------------------------- */
 typedef struct
 {
@@ -518,6 +892,8 @@ typedef struct
     const orc_pt_params *pp;
     cbvh bvh;
     mat_t *mats;
+    const uint32_t *mat_of_prim;
+    const light_t *lights;
     float *obj_tmax;
     float *out; /* 8 floats per pixel: r g b alpha depth normal(bits) object(bits) albedo(bits) */
     uint32_t y0, y1;
@@ -562,10 +938,15 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
         float iors[4] = {1.f, 1.f, 1.f, 1.f};
         int regularize = 0, caustic = 0, first_miss = 1;
         v3 beta = V3(1, 1, 1), rad = V3(0, 0, 0);
+        float prev_pdf = 0.0f; /* payload.prev_bsdf_pdf */
+        const uint32_t nl = pp->n_lights;
         for(uint32_t i = 0; i < pp->max_depth; ++i)
         {
             rng = orc_xxhash32(rng, i);
             uint32_t lr = rng;
+            const v3 seg_o = org, seg_d = dir, seg_beta = beta;
+            const float seg_tmax = tmax, seg_prev = prev_pdf;
+            const int seg_caustic = caustic;
             float t = 0.f, bu = 0.f, bv = 0.f;
             uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, &t, &bu, &bv);
             int stop = 0, media_op = 0;
@@ -576,6 +957,21 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
             {
                 stop = 1;
                 rad = add(rad, mul(scl(beta, sc->environment), env_white(dir)));
+                if(nl && !(pp->suppress_refractive_caustics && caustic))
+                {
+                    /* eval_distant_lights (raypipeline.slang:728-744) */
+                    v3 dl = V3(0, 0, 0);
+                    for(uint32_t li = 0; li < nl; ++li)
+                    {
+                        const light_t *l = &j->lights[li];
+                        if(l->type != L_DIRECTIONAL) continue;
+                        float pdf = light_pdf(l, V3(0, 0, 0), dir);
+                        if(pdf <= 0.0f) continue;
+                        float w = mis_weight_hit(pp->mis_mode, prev_pdf, pdf / (float)nl);
+                        dl = add(dl, scl(scl(l->color, w), l->intensity));
+                    }
+                    rad = add(rad, mul(beta, dl));
+                }
             }
             else
             {
@@ -608,9 +1004,42 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                 rad = add(rad, mul(beta, em));
                 if(regularize) m.rough = fmaxf(m.rough, 0.25f);
                 bs_t b = sample_disney(m, nrm, Vw, eta, &lr);
+                if(nl)
+                {
+                    /* nee_surface (direct_lighting.slang:296-336) */
+                    uint32_t index = 0;
+                    if(nl > 1)
+                    {
+                        uint32_t k = (uint32_t)(rnd(&lr) * (float)nl);
+                        index = k < nl - 1 ? k : nl - 1;
+                    }
+                    float xi_x = rnd(&lr), xi_y = rnd(&lr);
+                    const light_t *l = &j->lights[index];
+                    lsample_t ls = sample_light(l, pos, xi_x, xi_y);
+                    if(ls.radiance.x > 0.0f || ls.radiance.y > 0.0f || ls.radiance.z > 0.0f)
+                    {
+                        v3 so = add(pos, scl(nrm, EPS));
+                        v3 T = transmit(&j->bvh, j->mats, j->mat_of_prim, so, ls.dir, 0.0f, ls.dist);
+                        if(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)
+                        {
+                            float bpdf = 0.0f;
+                            float ct = fmaxf(dot(nrm, ls.dir), 0.0f);
+                            v3 F = eval_disney(m, ls.dir, nrm, Vw, eta, &bpdf);
+                            float w = bsdf_reachable(l) ? mis_weight_nee(pp->mis_mode, ls.pdf / (float)nl, bpdf) : 1.0f;
+                            v3 nee = scl(T, w);
+                            nee = scl(nee, (float)nl);
+                            nee = mul(nee, ls.radiance);
+                            nee = mul(nee, F);
+                            nee = scl(nee, ct);
+                            nee = V3(nee.x / ls.pdf, nee.y / ls.pdf, nee.z / ls.pdf);
+                            rad = add(rad, mul(beta, nee));
+                        }
+                    }
+                }
                 if(!(b.pdf > 0.0f)) stop = 1;
                 else
                 {
+                    prev_pdf = nl ? b.pdf : 0.0f;
                     dir = b.dir;
                     int specular = b.pdf < 1e3f;
                     regularize = regularize || specular;
@@ -644,6 +1073,23 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
             }
             else if(media_op == 2)
                 media_index = media_index > 0 ? media_index - 1 : 0;
+            if(nl)
+            {
+                /* ghost-light pass over the traced segment (raypipeline.slang:201-222) */
+                v3 dv = sub(pos, seg_o);
+                float t_end = prim == 0xFFFFFFFFu ? seg_tmax : sqrtf(dot(dv, dv));
+                for(uint32_t li = 0; li < nl; ++li)
+                {
+                    const light_t *l = &j->lights[li];
+                    float cl;
+                    float th = intersect_light(l, seg_o, seg_d, &cl);
+                    if(th <= EPS || th >= t_end) continue;
+                    if(pp->suppress_refractive_caustics && seg_caustic) continue;
+                    float p_nee = light_pdf(l, seg_o, seg_d) / (float)nl;
+                    float w = mis_weight_hit(pp->mis_mode, seg_prev, p_nee);
+                    rad = add(rad, scl(mul(scl(seg_beta, w), l->color), l->intensity));
+                }
+            }
             if(i == 0)
             {
                 first_miss = obj == 0xFFFFu;
@@ -749,6 +1195,10 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
               he2 = (hi[3 * o + 2] - lo[3 * o + 2]) / 2.f;
         obj_tmax[o] = hi[3 * o] < lo[3 * o] ? 0.f : 2.f * sqrtf((he0 * he0 + he1 * he1) + he2 * he2);
     }
+    uint32_t *mat_of_prim = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for(uint32_t t = 0; t < n; ++t) mat_of_prim[t] = sc->object_material[sc->tri_object[t]];
+    light_t *lights = (light_t *)calloc(pp->n_lights ? pp->n_lights : 1, sizeof(light_t));
+    for(uint32_t li = 0; li < pp->n_lights; ++li) lights[li] = read_light(pp->lights + 20 * (size_t)li);
     if(nthreads < 1) nthreads = 1;
     pthread_t *th = (pthread_t *)calloc((size_t)nthreads, sizeof(pthread_t));
     job_t *jobs = (job_t *)calloc((size_t)nthreads, sizeof(job_t));
@@ -758,6 +1208,8 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         jobs[k].pp = pp;
         jobs[k].bvh = b;
         jobs[k].mats = mats;
+        jobs[k].mat_of_prim = mat_of_prim;
+        jobs[k].lights = lights;
         jobs[k].obj_tmax = obj_tmax;
         jobs[k].out = out8;
         jobs[k].y0 = (uint32_t)((uint64_t)pp->height * k / nthreads);
@@ -767,6 +1219,8 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     for(int k = 0; k < nthreads; ++k) pthread_join(th[k], NULL);
     free(th);
     free(jobs);
+    free(lights);
+    free(mat_of_prim);
     free(lo);
     free(hi);
     free(obj_tmax);

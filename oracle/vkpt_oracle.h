@@ -32,6 +32,8 @@ typedef struct orc_pt_params
     float max_path_beta;
     float eye[3], forward[3], right[3], up[3];
     float tan_half_x, tan_half_y, aperture, focal_distance;
+    const float *lights; /* light_t records, 20 words each, already gathered (positive intensity, no Area) */
+    uint32_t n_lights, mis_mode, suppress_refractive_caustics;
 } orc_pt_params;
 
 /* One batch: out8 = 8 floats per pixel (radiance rgb / num_samples, coverage, linear depth, -, object id

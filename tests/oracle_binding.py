@@ -278,12 +278,15 @@ class _PtParams(ctypes.Structure):
                 ("max_depth", ctypes.c_uint32), ("seed", ctypes.c_uint32), ("disable_material", ctypes.c_uint32),
                 ("max_path_beta", ctypes.c_float), ("eye", ctypes.c_float * 3), ("forward", ctypes.c_float * 3),
                 ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("tan_half_x", ctypes.c_float),
-                ("tan_half_y", ctypes.c_float), ("aperture", ctypes.c_float), ("focal_distance", ctypes.c_float)]
+                ("tan_half_y", ctypes.c_float), ("aperture", ctypes.c_float), ("focal_distance", ctypes.c_float),
+                ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("mis_mode", ctypes.c_uint32),
+                ("suppress_refractive_caustics", ctypes.c_uint32)]
 
 
 def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_path_beta=0.0, disable_material=0,
-              nthreads=None):
+              nthreads=None, lights=None, mis_mode=0, suppress_refractive_caustics=0):
     """one batch of the CPU reference integrator: (H, W, 8) float32 (rgb, coverage, depth, -, object bits, -)"""
+    import vierkant_amd as vk
     from vierkant_amd import workloads as W
     h = lib()
     h.orc_pt_render.restype = None
@@ -295,10 +298,14 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
                   arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,
                   len(arr["materials"]), float(scene.get("environment", 1.0)))
     eye, fw, rt, up, thx, thy = W.camera_basis(camera, width, height)
+    # the tracer's gather: positive intensity, no Area lights, order kept
+    la = np.zeros(0, vk.LIGHT_DTYPE) if lights is None else np.asarray(lights, vk.LIGHT_DTYPE).reshape(-1)
+    la = np.ascontiguousarray(la[(la["intensity"] > 0) & (la["type"] != vk.LIGHT_AREA)])
     pp = _PtParams(width, height, num_samples, max_depth, seed, disable_material, max_path_beta,
                    (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*fw), (ctypes.c_float * 3)(*rt),
                    (ctypes.c_float * 3)(*up), thx, thy, float(camera.aperture),
-                   float(camera.focal_distance) if camera.focal_distance > 0 else 1.0)
+                   float(camera.focal_distance) if camera.focal_distance > 0 else 1.0,
+                   la.ctypes.data if len(la) else None, len(la), mis_mode, suppress_refractive_caustics)
     out = np.zeros((height, width, 8), np.float32)
     h.orc_pt_render(ctypes.byref(sc), ctypes.byref(pp), out.ctypes.data, nthreads or threads())
     return out

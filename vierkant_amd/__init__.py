@@ -51,7 +51,8 @@ class PtSettings(ctypes.Structure):
                 ("max_path_beta", ctypes.c_float), ("suspend_trace_when_done", ctypes.c_uint32),
                 ("disable_material", ctypes.c_uint32), ("suppress_reset", ctypes.c_uint32),
                 ("denoising", ctypes.c_uint32), ("tonemap", ctypes.c_uint32), ("gamma", ctypes.c_float),
-                ("exposure", ctypes.c_float), ("reserved", ctypes.c_uint32 * 3)]
+                ("exposure", ctypes.c_float), ("mis_mode", ctypes.c_uint32),
+                ("suppress_refractive_caustics", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
 
 
 class PtScene(ctypes.Structure):
@@ -89,6 +90,33 @@ def material(color=(0.8, 0.8, 0.8), emission=(0.0, 0.0, 0.0), emission_strength=
     m["metalness"], m["roughness"], m["transmission"], m["ior"] = metalness, roughness, transmission, ior
     m["two_sided"] = 1 if two_sided else 0
     return m
+
+
+# light_t (vkrt_light): 80 bytes, the layout the shaders read
+LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("type", "<u4"), ("color", "<f4", 3), ("intensity", "<f4"),
+                        ("direction", "<f4", 3), ("range", "<f4"), ("spot_angle_scale", "<f4"),
+                        ("spot_angle_offset", "<f4"), ("angular_size", "<f4"), ("size_y", "<f4"),
+                        ("tangent", "<f4", 3), ("size_x", "<f4")])
+assert LIGHT_DTYPE.itemsize == 80
+LIGHT_OMNI, LIGHT_SPOT, LIGHT_DIRECTIONAL, LIGHT_AREA, LIGHT_RECT, LIGHT_SPHERE, LIGHT_TUBE, LIGHT_DISK = range(8)
+MIS_FULL, MIS_NEE_ONLY, MIS_BSDF_ONLY = 0, 1, 2
+
+
+def light(type, position=(0.0, 0.0, 0.0), direction=(0.0, 0.0, -1.0), color=(1.0, 1.0, 1.0), intensity=1.0,
+          size=(1.0, 1.0), tangent=(1.0, 0.0, 0.0), angular_size=0.0, range=np.inf, inner_cone_angle=0.0,
+          outer_cone_angle=np.pi / 4):
+    """one light_t record as convert_light fills it (include/vierkant/punctual_light.hpp:91-111): spot angle
+    scale/offset from the cone angles, size = (size_x, size_y)"""
+    l = np.zeros((), dtype=LIGHT_DTYPE)
+    l["type"], l["position"], l["direction"], l["color"] = type, position, direction, color
+    l["intensity"], l["tangent"], l["angular_size"] = intensity, tangent, angular_size
+    l["range"] = range if range > 0 else np.inf
+    scale = np.float32(1.0) / max(np.float32(0.001), np.float32(np.cos(np.float32(inner_cone_angle))) -
+                                  np.float32(np.cos(np.float32(outer_cone_angle))))
+    l["spot_angle_scale"] = scale
+    l["spot_angle_offset"] = -np.float32(np.cos(np.float32(outer_cone_angle))) * np.float32(scale)
+    l["size_x"], l["size_y"] = size
+    return l
 
 
 class BatchItem(ctypes.Structure):
@@ -168,6 +196,7 @@ _SIGNATURES = {
     "vkrt_pt_set_settings": (_i32, [_vp, ctypes.POINTER(PtSettings)]),
     "vkrt_pt_get_settings": (_i32, [_vp, ctypes.POINTER(PtSettings)]),
     "vkrt_pt_set_scene": (_i32, [_vp, ctypes.POINTER(PtScene)]),
+    "vkrt_pt_set_lights": (_i32, [_vp, _vp, _u32]),
     "vkrt_pt_render": (_i32, [_vp, ctypes.POINTER(Camera)]),
     "vkrt_pt_reset_accumulator": (_i32, [_vp]),
     "vkrt_pt_current_batch": (_u32, [_vp]),
@@ -643,6 +672,11 @@ class PathTracer:
                      arr["object_material"].ctypes.data, len(arr["object_material"]),
                      arr["materials"].ctypes.data, len(arr["materials"]), float(scene.get("environment", 1.0)))
         self.ctx.check(lib().vkrt_pt_set_scene(self._h, ctypes.byref(sc)))
+
+    def set_lights(self, lights):
+        """the frame's light list (LIGHT_DTYPE records; empty clears it)"""
+        arr = np.ascontiguousarray(np.asarray(lights, dtype=LIGHT_DTYPE).reshape(-1))
+        self.ctx.check(lib().vkrt_pt_set_lights(self._h, arr.ctypes.data if len(arr) else None, len(arr)))
 
     def render(self, camera):
         self.ctx.check(lib().vkrt_pt_render(self._h, ctypes.byref(camera)))

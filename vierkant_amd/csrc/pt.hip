@@ -39,7 +39,11 @@ constexpr uint32_t MEDIA_STACK = 4;          // MAX_MEDIA_STACK_SIZE (raypipelin
 constexpr uint32_t NO_OBJECT = 0xFFFFu;      // utils::MAX_UINT16
 constexpr uint32_t F_MEDIA_MASK = 3u, F_REGULARIZE = 4u, F_CAUSTIC = 8u, F_FIRST_MISS = 16u, F_TRAIN = 32u;
 constexpr uint32_t QUERY_F4 = 5;   // cache query record: 4 float4 of features + (pixel, beta)
-constexpr uint32_t VERTEX_F4 = 6;  // training vertex: 4 float4 of features + (emission, kind) + (weight, -)
+constexpr uint32_t VERTEX_F4 = 7;  // training vertex: 4 float4 of features + (emission, kind) + (weight, -)
+                                   // + (light bodies met on the segment arriving here, -)
+constexpr uint32_t SHADOW_F4 = 6;  // shadow record: ray o, d; (radiance, mis weight); (F, cos); (beta, pdf);
+                                   // (pixel, training vertex, -, -)
+constexpr uint32_t NO_VERTEX = 0xffffffffu;
 constexpr int PT_BLOCK = 256;
 constexpr int PT_LDS = 16;
 
@@ -82,6 +86,8 @@ struct pt_frame
     // radiance cache: paths that are not training paths stop at hit depth cache_depth and query the cache;
     // every train_stride-th pixel (from train_offset) traces a full training path
     uint32_t cache_on, cache_depth, train_stride, train_offset, n_train_paths, train_max;
+    // analytic lights: next-event estimation and the light bodies BSDF rays meet
+    uint32_t n_lights, mis_mode, suppress_caustics;
     vec3 eye, right, up, forward;
     float tan_half_x, tan_half_y, aperture, focal_distance;
 };
@@ -102,6 +108,7 @@ struct pt_buffers
     float4 *qrec;           // cache queries (QUERY_F4 float4 each)
     float4 *tvtx;           // training paths' vertices [path][depth] (VERTEX_F4 float4 each)
     float4 *trec, *ttgt;    // training records (4 float4) and targets
+    float4 *shadow;         // next-event shadow rays of one depth (SHADOW_F4 float4 each), count in counter[6]
 };
 
 struct pt_scene_dev
@@ -110,6 +117,7 @@ struct pt_scene_dev
     const uint32_t *idx, *tri_object, *object_material;
     const float *object_tmax;
     const gpu_material *mats;
+    const vkrt_light *lights;
 };
 
 __device__ __forceinline__ vec3 ld3f(const float *p, uint32_t i) { return vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
@@ -157,6 +165,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
         {
             tv[d * VERTEX_F4 + 4] = make_float4(0.f, 0.f, 0.f, 0.f);
             tv[d * VERTEX_F4 + 5] = make_float4(0.f, 0.f, 0.f, 0.f);
+            tv[d * VERTEX_F4 + 6] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
     b.state[p] = st;
@@ -180,6 +189,13 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
 }
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_zero(uint32_t *c) { c[threadIdx.x] = 0u; }
+
+//! before a depth: the next queue and the shadow-ray count start at zero
+__global__ void k_pt_reset_depth(uint32_t *c, int next)
+{
+    c[next] = 0u;
+    c[6] = 0u;
+}
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                        const float4 *__restrict__ tris, int32_t root, bool empty,
@@ -205,6 +221,45 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__re
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
 }
 
+//! eval_distant_lights (raypipeline.slang:728-744): sun discs seen by a ray that left the scene
+__device__ vec3 eval_distant_lights(const vkrt_light *__restrict__ lights, uint32_t n, uint32_t mode, const vec3 &dir,
+                                    float prev_bsdf_pdf)
+{
+    vec3 radiance{0.f, 0.f, 0.f};
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        const vkrt_light l = lights[i];
+        if(l.type != VKRT_LIGHT_DIRECTIONAL) continue;
+        const float pdf = light_pdf(l, vec3{0.f, 0.f, 0.f}, dir);
+        if(pdf <= 0.0f) continue;
+        const float w = mis_weight_hit(mode, prev_bsdf_pdf, pdf / static_cast<float>(n));
+        radiance = radiance + (light_vec(l.color) * w) * l.intensity;
+    }
+    return radiance;
+}
+
+//! the ghost-light pass of raygen (raypipeline.slang:201-222): light bodies met inside the traced segment
+//! (tmin, t_end) add their MIS-weighted radiance, scaled by the segment's throughput, to the path in light
+//! order; `sum` receives the unscaled total for training records
+__device__ vec3 ghost_lights(const vkrt_light *__restrict__ lights, const pt_frame &f, const vec3 &o, const vec3 &d,
+                             float tmin, float t_end, const vec3 &seg_beta, float prev_pdf, bool caustic,
+                             vec3 radiance, vec3 &sum)
+{
+    sum = vec3{0.f, 0.f, 0.f};
+    for(uint32_t li = 0; li < f.n_lights; ++li)
+    {
+        const vkrt_light l = lights[li];
+        const light_hit hit = intersect_light(l, o, d);
+        if(hit.t <= tmin || hit.t >= t_end) continue;
+        if(f.suppress_caustics && caustic) continue;
+        const float p_nee = light_pdf(l, o, d) / static_cast<float>(f.n_lights);
+        const float w = mis_weight_hit(f.mis_mode, prev_pdf, p_nee);
+        radiance = radiance + mul3(seg_beta * w, light_vec(l.color)) * l.intensity;
+        sum = sum + (light_vec(l.color) * w) * l.intensity;
+    }
+    return radiance;
+}
+
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b, pt_scene_dev s, uint32_t depth,
                                                        int in)
 {
@@ -213,6 +268,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
     const bool active = q < count;
     bool alive = false;
     bool to_cache = false;
+    bool need_shadow = false;
+    float4 shadow_rec[SHADOW_F4];
     float4 qfeat[QUERY_F4];
     uint32_t p = 0;
     if(active)
@@ -226,7 +283,13 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
         uint32_t flags = st.y;
         float4 ro = b.ray_o[p], rd = b.ray_d[p];
         vec3 dir = xyz(rd);
-        vec3 beta = xyz(b.beta[p]);
+        const float4 beta4 = b.beta[p];
+        vec3 beta = xyz(beta4);
+        float prev_pdf = beta4.w;// payload.prev_bsdf_pdf
+        // the traced segment, for the ghost-light pass after shading
+        const vec3 seg_o = xyz(ro), seg_d = dir, seg_beta = beta;
+        const float seg_tmin = ro.w, seg_tmax = rd.w, seg_prev_pdf = prev_pdf;
+        const bool seg_caustic = (flags & F_CAUSTIC) != 0u;
         vec3 radiance = xyz(b.rad[p]);
         float4 iors = b.iors[p];
         float ior_stack[MEDIA_STACK] = {iors.x, iors.y, iors.z, iors.w};
@@ -252,6 +315,12 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             radiance = radiance + mul3(beta * f.environment, col);
             vkind = 2;
             v_e = col * f.environment;
+            if(f.n_lights && !(f.suppress_caustics && seg_caustic))
+            {
+                const vec3 dl = eval_distant_lights(s.lights, f.n_lights, f.mis_mode, dir, prev_pdf);
+                radiance = radiance + mul3(beta, dl);
+                v_e = v_e + dl;
+            }
         }
         else
         {
@@ -317,9 +386,41 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             v_e = emission;
             if(flags & F_REGULARIZE) m.roughness = fmaxf(m.roughness, REGULARIZE_MIN_ROUGHNESS);
             bsdf_sample bs = sample_disney(m, sn, V, eta, rng);
+            if(f.n_lights)
+            {
+                // nee_surface (direct_lighting.slang:296-336): one light picked uniformly, one sample of it; the
+                // shadow ray's transmittance is applied by k_pt_shadow
+                const uint32_t nl = f.n_lights;
+                const uint32_t index = nl > 1u ? min(static_cast<uint32_t>(rnd(rng) * static_cast<float>(nl)), nl - 1u)
+                                                : 0u;
+                vec2 xi;
+                xi.x = rnd(rng);
+                xi.y = rnd(rng);
+                const vkrt_light l = s.lights[index];
+                const light_sample ls = sample_light(l, position, xi);
+                if(ls.radiance.x > 0.0f || ls.radiance.y > 0.0f || ls.radiance.z > 0.0f)
+                {
+                    float bsdf_pdf = 0.0f;
+                    const float cos_theta = fmaxf(dot(sn, ls.direction), 0.0f);
+                    const vec3 F = eval_disney(m, ls.direction, sn, V, eta, bsdf_pdf);
+                    const float weight = bsdf_reachable(l)
+                                                 ? mis_weight_nee(f.mis_mode, ls.pdf / static_cast<float>(nl), bsdf_pdf)
+                                                 : 1.0f;
+                    const vec3 org = position + sn * EPS;
+                    shadow_rec[0] = make_float4(org.x, org.y, org.z, 0.0f);
+                    shadow_rec[1] = make_float4(ls.direction.x, ls.direction.y, ls.direction.z, ls.dist);
+                    shadow_rec[2] = make_float4(ls.radiance.x, ls.radiance.y, ls.radiance.z, weight);
+                    shadow_rec[3] = make_float4(F.x, F.y, F.z, cos_theta);
+                    shadow_rec[4] = make_float4(beta.x, beta.y, beta.z, ls.pdf);
+                    const uint32_t vtx = train ? (p / f.train_stride) * f.max_depth + depth : NO_VERTEX;
+                    shadow_rec[5] = make_float4(__uint_as_float(p), __uint_as_float(vtx), 0.f, 0.f);
+                    need_shadow = true;
+                }
+            }
             if(!(bs.pdf > 0.0f)) { stop = true; }
             else
             {
+                prev_pdf = f.n_lights ? bs.pdf : 0.0f;
                 dir = bs.direction;
                 const bool specular = bs.pdf < REGULARIZE_PDF_THRESHOLD;
                 if(specular) flags |= F_REGULARIZE;
@@ -355,9 +456,18 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             }
             }
         }
+        vec3 ghost{0.f, 0.f, 0.f};
+        if(f.n_lights)
+        {
+            // segment end: the hit point, or the traced tmax on a miss
+            const float t_end = h.w == VKRT_INVALID_PRIM ? seg_tmax : length3(position - seg_o);
+            radiance = ghost_lights(s.lights, f, seg_o, seg_d, seg_tmin, t_end, seg_beta, seg_prev_pdf, seg_caustic,
+                                    radiance, ghost);
+        }
         if(train && vkind != 0u)
         {
             float4 *tv = b.tvtx + ((size_t)(p / f.train_stride) * f.max_depth + depth) * VERTEX_F4;
+            tv[6] = make_float4(ghost.x, ghost.y, ghost.z, 0.f);
             if(vkind == 1u)
             {
                 tv[0] = feat[0];
@@ -402,7 +512,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
         {
             b.ray_o[p] = ro;
             b.ray_d[p] = make_float4(dir.x, dir.y, dir.z, rd.w);
-            b.beta[p] = make_float4(beta.x, beta.y, beta.z, 0.f);
+            b.beta[p] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
             b.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.f);
             b.iors[p] = make_float4(ior_stack[0], ior_stack[1], ior_stack[2], ior_stack[3]);
         }
@@ -444,23 +554,39 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             for(uint32_t k = 0; k < QUERY_F4; ++k) dst[k] = qfeat[k];
         }
     }
+    // next-event shadow rays: one ballot and one atomic per wave
+    const uint64_t smask = __ballot(need_shadow);
+    if(smask)
+    {
+        uint32_t sbase = 0;
+        if(lane == 0) sbase = atomicAdd(&b.counter[6], static_cast<uint32_t>(__popcll(smask)));
+        sbase = __shfl(sbase, 0);
+        if(need_shadow)
+        {
+            const uint64_t below = lane == 0 ? 0ull : (smask & (~0ull >> (64 - lane)));
+            float4 *dst = b.shadow + (size_t)(sbase + __popcll(below)) * SHADOW_F4;
+            for(uint32_t k = 0; k < SHADOW_F4; ++k) dst[k] = shadow_rec[k];
+        }
+    }
 }
 
-//! training targets: L_d = e_d + w_d * L_{d+1} from the last vertex back (a miss slot holds the sky's
-//! radiance); every surface vertex becomes a training record, at most train_max per sample
+//! training targets: L_d = e_d + w_d * (g_{d+1} + L_{d+1}) from the last vertex back, where e_d holds the
+//! vertex's emission and next-event estimate (a miss slot: the sky and sun discs) and g_{d+1} the light bodies
+//! met on the segment leaving it; every surface vertex becomes a training record, at most train_max per sample
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_targets(pt_frame f, pt_buffers b)
 {
     const uint32_t t = blockIdx.x * PT_BLOCK + threadIdx.x;
     if(t >= f.n_train_paths) return;
     if((size_t)t * f.train_stride + f.train_offset >= f.n_pix) return;
     const float4 *tv = b.tvtx + (size_t)t * f.max_depth * VERTEX_F4;
-    vec3 L{0.f, 0.f, 0.f};
+    vec3 Lin{0.f, 0.f, 0.f};
     for(int d = (int)f.max_depth - 1; d >= 0; --d)
     {
-        const float4 e = tv[d * VERTEX_F4 + 4], w = tv[d * VERTEX_F4 + 5];
+        const float4 e = tv[d * VERTEX_F4 + 4], w = tv[d * VERTEX_F4 + 5], g = tv[d * VERTEX_F4 + 6];
         const uint32_t kind = __float_as_uint(e.w);
         if(kind == 0u) continue;
-        L = vec3{e.x + w.x * L.x, e.y + w.y * L.y, e.z + w.z * L.z};
+        const vec3 L{e.x + w.x * Lin.x, e.y + w.y * Lin.y, e.z + w.z * Lin.z};
+        Lin = vec3{g.x + L.x, g.y + L.y, g.z + L.z};
         if(kind != 1u) continue;
         const uint32_t k = atomicAdd(&b.counter[5], 1u);
         if(k >= f.train_max) continue;
@@ -516,20 +642,15 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_resolve(pt_frame f, pt_buffers 
 constexpr int TK = 8;
 constexpr int TR_STACK = 64;
 
-__global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_node *__restrict__ nodes2,
-                                                               int32_t root2, const vkrt_bvh4_node *__restrict__ nodes4,
-                                                               int32_t root4, bool wide, bool empty,
-                                                               const float4 *__restrict__ tris,
-                                                               const gpu_material *__restrict__ mats,
-                                                               const float4 *__restrict__ rays, uint32_t n, bool cull,
-                                                               float4 *__restrict__ out)
+//! the transmittance of one ray through the scene: zero when an opaque triangle lies in (tmin, tmax),
+//! else the product of transmission * color over the transmissive hits in ascending (t, triangle) order
+__device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
+                                     const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide, bool empty,
+                                     const float4 *__restrict__ tris, const gpu_material *__restrict__ mats,
+                                     const ray_setup &r, bool cull, int32_t *lds)
 {
-    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     int32_t spill[TR_STACK - PT_LDS];
-    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
-    if(i >= n) return;
-    lane_stack<PT_BLOCK, PT_LDS> st{smem + threadIdx.x, spill};
-    const ray_setup r = setup_ray(rays, i);
+    lane_stack<PT_BLOCK, PT_LDS> st{lds, spill};
     float T0 = 1.f, T1 = 1.f, T2 = 1.f;
     bool blocked = false, more = !empty, first_pass = true;
     float lb_t = 0.f;
@@ -645,7 +766,58 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_no
         }
         first_pass = false;
     }
-    out[i] = blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
+    return blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
+}
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_node *__restrict__ nodes2,
+                                                               int32_t root2, const vkrt_bvh4_node *__restrict__ nodes4,
+                                                               int32_t root4, bool wide, bool empty,
+                                                               const float4 *__restrict__ tris,
+                                                               const gpu_material *__restrict__ mats,
+                                                               const float4 *__restrict__ rays, uint32_t n, bool cull,
+                                                               float4 *__restrict__ out)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(i >= n) return;
+    out[i] = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, setup_ray(rays, i), cull,
+                                smem + threadIdx.x);
+}
+
+//! the shadow half of nee_surface (direct_lighting.slang:311-333): the unculled transmittance toward the
+//! sampled light, then transmittance * weight * n_lights * radiance * F * cos / pdf scaled by the vertex's
+//! throughput joins the pixel's sample sum; a training vertex also records the unscaled estimate
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
+                                                        const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
+                                                        bool wide, bool empty, const float4 *__restrict__ tris,
+                                                        const gpu_material *__restrict__ mats, pt_frame f,
+                                                        pt_buffers b)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(i >= b.counter[6]) return;
+    const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
+    const float4 o = rec[0], d = rec[1];
+    const float4 T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d),
+                                        false, smem + threadIdx.x);
+    if(!(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)) return;
+    const float4 lr = rec[2], fc = rec[3], bp = rec[4], id = rec[5];
+    vec3 nee = vec3{T.x, T.y, T.z} * lr.w;
+    nee = nee * static_cast<float>(f.n_lights);
+    nee = mul3(nee, xyz(lr));
+    nee = mul3(nee, xyz(fc));
+    nee = nee * fc.w;
+    nee = vec3{nee.x / bp.w, nee.y / bp.w, nee.z / bp.w};
+    const vec3 c = mul3(xyz(bp), nee);
+    const uint32_t p = __float_as_uint(id.x), vtx = __float_as_uint(id.y);
+    float4 a = b.acc[p];
+    b.acc[p] = make_float4(a.x + c.x, a.y + c.y, a.z + c.z, a.w);
+    if(vtx != NO_VERTEX)
+    {
+        float4 *e = b.tvtx + (size_t)vtx * VERTEX_F4 + 4;
+        const float4 ev = *e;
+        *e = make_float4(ev.x + nee.x, ev.y + nee.y, ev.z + nee.z, ev.w);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -761,6 +933,9 @@ struct vkrt_pt
     vkrt_rc *rc = nullptr;
     vkrt_pt_cache_config cache_cfg{};
     vkrt::device_buffer qrec, tvtx, trec, ttgt;
+    // analytic lights (the frame's light list) and the shadow records of one depth
+    vkrt::device_buffer lights, shadow;
+    uint32_t n_lights = 0;
     vkrt::device_buffer image;// denoise ping-pong images
 };
 
@@ -793,6 +968,7 @@ static void check_settings(const vkrt_pt_settings &s)
     VKRT_REQUIRE(s.max_trace_depth > 0 && s.max_trace_depth <= 64, "max_trace_depth must be in 1..64");
     VKRT_REQUIRE(!s.tonemap || (s.gamma > 0.f && std::isfinite(s.gamma) && std::isfinite(s.exposure)),
                  "tonemap needs a positive finite gamma and a finite exposure");
+    VKRT_REQUIRE(s.mis_mode <= VKRT_MIS_BSDF_ONLY, "mis_mode must be 0, 1 or 2");
 }
 
 //! resize_storage (PBRPathTracer.cpp:684-830): new pixel storage resets the batch index
@@ -835,6 +1011,7 @@ static pt_buffers buffers(vkrt_pt *pt)
     b.queue[1] = pt->queue1.as<uint32_t>();
     b.counter = pt->counter.as<uint32_t>();
     b.pixels = pt->pixels.as<vkrt_pixel>();
+    b.qrec = b.tvtx = b.trec = b.ttgt = b.shadow = nullptr;
     return b;
 }
 
@@ -883,6 +1060,9 @@ static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     f.train_offset = 0;
     f.n_train_paths = div_up(f.n_pix, f.train_stride);
     f.train_max = pt->cache_cfg.max_train_records;
+    f.n_lights = pt->n_lights;
+    f.mis_mode = pt->settings.mis_mode;
+    f.suppress_caustics = pt->settings.suppress_refractive_caustics ? 1u : 0u;
     return f;
 }
 
@@ -910,7 +1090,8 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     pt_frame f = make_frame(pt, cam, seed);
     pt_buffers b = buffers(pt);
     pt_scene_dev sc{pt->pos.as<float>(), pt->nrm.as<float>(), pt->idx.as<uint32_t>(), pt->tri_object.as<uint32_t>(),
-                    pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(), pt->mats.as<gpu_material>()};
+                    pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(), pt->mats.as<gpu_material>(),
+                    pt->lights.as<vkrt_light>()};
     const uint32_t blocks = div_up(f.n_pix, PT_BLOCK);
     const vkrt_bvh *bvh = pt->bvh;
     if(f.cache_on)
@@ -923,6 +1104,11 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         b.tvtx = pt->tvtx.as<float4>();
         b.trec = pt->trec.as<float4>();
         b.ttgt = pt->ttgt.as<float4>();
+    }
+    if(f.n_lights)
+    {
+        pt->shadow.reserve((size_t)f.n_pix * SHADOW_F4 * 16);
+        b.shadow = pt->shadow.as<float4>();
     }
     VKRT_HIP(hipEventRecord(pt->ev0, s));
     k_pt_zero<<<1, 4, 0, s>>>(b.counter);
@@ -941,11 +1127,15 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
             // past the cache depth only training paths are alive: at most n_train_paths queue entries
             const uint32_t grid = (f.cache_on && depth > f.cache_depth) ? div_up(std::max(1u, f.n_train_paths), PT_BLOCK)
                                                                           : blocks;
-            k_pt_zero<<<1, 1, 0, s>>>(b.counter + (in ^ 1));
+            k_pt_reset_depth<<<1, 1, 0, s>>>(b.counter, in ^ 1);
             k_pt_trace<<<grid, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref,
                                                  bvh->empty, bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide,
                                                  b, in);
             k_pt_shade<<<grid, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
+            if(f.n_lights)
+                k_pt_shadow<<<grid, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->root_ref,
+                                                      bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide,
+                                                      bvh->empty, bvh->tris.as<float4>(), sc.mats, f, b);
             in ^= 1;
         }
         if(f.cache_on)
@@ -1026,6 +1216,47 @@ int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out)
     if(!pt || !out) return VKRT_E_INVALID_ARG;
     *out = pt->settings;
     return VKRT_OK;
+}
+
+static bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int vkrt_pt_set_lights(vkrt_pt *pt, const vkrt_light *lights, uint32_t n)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(n == 0 || lights, "null light array");
+        VKRT_REQUIRE(n <= 65536u, "at most 65536 lights");
+        // the gather of update_trace_descriptors: positive intensity, Area reserved (PBRPathTracer.cpp:630-637)
+        std::vector<vkrt_light> kept;
+        kept.reserve(n);
+        for(uint32_t i = 0; i < n; ++i)
+        {
+            const vkrt_light &l = lights[i];
+            VKRT_REQUIRE(l.type <= VKRT_LIGHT_DISK, "unknown light type");
+            if(!(l.intensity > 0.f) || l.type == VKRT_LIGHT_AREA) continue;
+            VKRT_REQUIRE(std::isfinite(l.intensity) && finite3(l.color) && finite3(l.position) && finite3(l.direction),
+                         "light with a non-finite field");
+            const bool body = l.type == VKRT_LIGHT_RECT || l.type == VKRT_LIGHT_SPHERE || l.type == VKRT_LIGHT_TUBE ||
+                              l.type == VKRT_LIGHT_DISK;
+            VKRT_REQUIRE(!body || (l.size_x > 0.f && std::isfinite(l.size_x)), "light body needs a positive size_x");
+            VKRT_REQUIRE(!(l.type == VKRT_LIGHT_RECT || l.type == VKRT_LIGHT_TUBE) ||
+                                 (l.size_y > 0.f && std::isfinite(l.size_y)),
+                         "rect and tube lights need a positive size_y");
+            VKRT_REQUIRE(l.type != VKRT_LIGHT_RECT || finite3(l.tangent), "rect light needs a finite tangent");
+            VKRT_REQUIRE(l.type > VKRT_LIGHT_SPOT || l.range > 0.f,
+                         "omni and spot lights need range > 0 (infinity: no range window)");
+            kept.push_back(l);
+        }
+        pt->n_lights = 0;
+        if(!kept.empty())
+        {
+            pt->lights.reserve(kept.size() * sizeof(vkrt_light));
+            VKRT_HIP(hipMemcpyAsync(pt->lights.ptr, kept.data(), kept.size() * sizeof(vkrt_light),
+                                    hipMemcpyHostToDevice, pt->ctx->stream));
+            VKRT_HIP(hipStreamSynchronize(pt->ctx->stream));
+        }
+        pt->n_lights = static_cast<uint32_t>(kept.size());
+    });
 }
 
 int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)

@@ -9,7 +9,11 @@
 //                                               :79-85, sample_GGX_VNDF :98-118, sample_GTR1 :120-129
 //   shaders/slang/utils/bsdf.slang              SchlickFresnel :9-13, DielectricFresnel :24-33, GTR1/GTR2
 //                                               :143-156, SmithGGX :158-163
-//   shaders/slang/ray/bsdf_disney.slang         Eval* lobes :44-160, sample_disney :162-296
+//   shaders/slang/ray/bsdf_disney.slang         Eval* lobes :44-160, sample_disney :162-296, eval_disney
+//                                               :298-381
+//   shaders/slang/ray/direct_lighting.slang     sample_light :32-136, power_heuristic :139-144, bsdf_reachable
+//                                               :148-152, intersect_light :166-236, MIS weights :238-252,
+//                                               light_pdf :254-292
 //   shaders/slang/utils/procedural_environment.slang:7-11   environment_white
 //   shaders/slang/utils/octahedral_map.slang:9-19, :49-55   octahedral normal packing
 //   shaders/slang/utils/color.slang:14-18       color_cast (RGBA8)
@@ -407,6 +411,311 @@ VK_HD bsdf_sample sample_disney(material m, const vec3 &N, const vec3 &V, float 
     }
     ret.pdf = (ret.pdf != ret.pdf) ? 0.0f : ret.pdf;
     return ret;
+}
+
+//! eval_disney (ray/bsdf_disney.slang:298-381) with iridescence strength 0: the lobes sample_disney draws
+//! from, weighted by their selection probabilities; F is zero whenever the pdf is not positive
+VK_HD vec3 eval_disney(material m, const vec3 &L, const vec3 &N, const vec3 &V, float eta, float &pdf)
+{
+    const bool refl = dot(N, L) > 0.0f;
+    const vec3 LpV = refl ? L + V : L + V * eta;
+    const float LpV_len = length3(LpV);
+    if(LpV_len < 1e-7f)
+    {
+        pdf = 0.0f;
+        return splat(0.f);
+    }
+    vec3 H = vec3{LpV.x / LpV_len, LpV.y / LpV_len, LpV.z / LpV_len};
+    if(dot(V, H) < 0.0f) H = -H;
+    if(refl && dot(N, H) <= 0.0f)
+    {
+        pdf = 0.0f;
+        return splat(0.f);
+    }
+    const float diffuse_ratio = 0.5f * (1.0f - m.metalness);
+    const float primary_spec_ratio = 1.0f / (1.0f + m.clearcoat);
+    const float trans_weight = (1.0f - m.metalness) * m.transmission;
+    m.roughness = fmaxf(0.001f, m.roughness * m.roughness);
+    m.sheen_roughness = fmaxf(0.001f, m.sheen_roughness);
+    const vec3 spec_color = lerp3(splat(0.08f), m.color, m.metalness);
+
+    vec3 brdf = splat(0.f), bsdf = splat(0.f);
+    float brdf_pdf = 0.0f, bsdf_pdf = 0.0f, m_pdf = 0.0f;
+    if(trans_weight > 0.0f)
+    {
+        if(refl)
+        {
+            bsdf = bsdf + eval_dielectric_reflection(m.roughness, m.color, eta, V, N, L, H, m_pdf) * (1.0f - 0.0f);
+            bsdf_pdf += m_pdf * (1.0f - 0.0f);
+        }
+        else
+            bsdf = eval_dielectric_refraction(m.roughness, m.color, eta, V, N, L, H, bsdf_pdf);
+    }
+    if(trans_weight < 1.0f)
+    {
+        const float dt = m.diffuse_transmission;
+        brdf = brdf + eval_diffuse(m.roughness, m.sheen_roughness, m.color, m.sheen_color, m.metalness, V, N, L, H,
+                                   m_pdf) * (1.0f - dt);
+        brdf_pdf += m_pdf * diffuse_ratio * (1.0f - dt);
+        if(dt > 0.0f && !refl)
+        {
+            const float cos_t = -dot(N, L);
+            brdf = brdf + m.diffuse_transmission_color * dt * ONE_OVER_PI;
+            brdf_pdf += cos_t * ONE_OVER_PI * diffuse_ratio * dt;
+        }
+        brdf = brdf + eval_specular(m.roughness, spec_color, V, N, L, H, m_pdf) * (1.0f - 0.0f);
+        brdf_pdf += m_pdf * primary_spec_ratio * (1.0f - diffuse_ratio) * (1.0f - 0.0f);
+        brdf = brdf + eval_clearcoat(m.clearcoat_roughness, m.clearcoat, V, N, L, H, m_pdf);
+        brdf_pdf += m_pdf * (1.0f - primary_spec_ratio) * (1.0f - diffuse_ratio);
+    }
+    pdf = lerpf(brdf_pdf, bsdf_pdf, trans_weight);
+    const vec3 F = lerp3(brdf, bsdf, trans_weight);
+    const bool valid = pdf > 0.0f;
+    pdf = valid ? pdf : 0.0f;
+    return valid ? F : splat(0.f);
+}
+
+// ------------------------------------------------------------------------------------------------
+// analytic lights (ray/direct_lighting.slang)
+// ------------------------------------------------------------------------------------------------
+VK_HD vec3 light_vec(const float *p) { return vec3{p[0], p[1], p[2]}; }
+
+//! sample_unit_sphere_cap (utils/sampling.slang:55-64): uniform over the cap of apex half-angle sigma
+VK_HD vec3 sample_unit_sphere_cap(vec2 xi, float sigma)
+{
+    const float phi = 2.0f * PI * xi.y;
+    const float cos_sigma = cosf(sigma);
+    const float cos_theta = xi.x * (1.0f - cos_sigma) + cos_sigma;
+    const float sin_theta = sqrtf(1.0f - cos_theta * cos_theta);
+    return spherical_direction(sin_theta, cos_theta, phi);
+}
+
+struct light_sample
+{
+    vec3 direction;//!< toward the light
+    float dist;    //!< shadow-ray tmax
+    float pdf;     //!< solid angle (1 for delta lights)
+    vec3 radiance; //!< unshadowed incident radiance
+};
+
+//! sample_light (direct_lighting.slang:32-136)
+VK_HD light_sample sample_light(const vkrt_light &l, const vec3 &P, vec2 xi)
+{
+    light_sample ret;
+    ret.pdf = 1.0f;
+    ret.radiance = light_vec(l.color) * l.intensity;
+    ret.direction = vec3{0.f, 0.f, 1.f};
+    ret.dist = EPS;
+    if(l.type == VKRT_LIGHT_DIRECTIONAL)
+    {
+        ret.direction = -light_vec(l.direction);
+        ret.dist = DEFAULT_TMAX;
+        if(l.angular_size > 0.0f)
+        {
+            ret.direction = to_world(local_frame(ret.direction), sample_unit_sphere_cap(xi, l.angular_size));
+            ret.pdf = 1.0f / (2.0f * PI * (1.0f - cosf(l.angular_size)));
+        }
+    }
+    else if(l.type == VKRT_LIGHT_SPHERE)
+    {
+        const vec3 to_center = light_vec(l.position) - P;
+        const float d2 = dot(to_center, to_center);
+        const float r2 = l.size_x * l.size_x;
+        if(d2 <= r2)
+        {
+            ret.radiance = splat(0.f);
+            ret.direction = vec3{0.f, 0.f, 1.f};
+            ret.dist = EPS;
+        }
+        else
+        {
+            const float sin2_sigma = r2 / d2;
+            const float cos_sigma = sqrtf(1.0f - sin2_sigma);
+            ret.direction = to_world(local_frame(to_center * (1.0f / sqrtf(d2))),
+                                     sample_unit_sphere_cap(xi, asinf(sqrtf(sin2_sigma))));
+            ret.pdf = (1.0f + cos_sigma) / (2.0f * PI * sin2_sigma);
+            const float b = dot(ret.direction, to_center);
+            ret.dist = b - sqrtf(fmaxf(b * b - d2 + r2, 0.0f));
+        }
+    }
+    else if(l.type == VKRT_LIGHT_RECT || l.type == VKRT_LIGHT_DISK || l.type == VKRT_LIGHT_TUBE)
+    {
+        vec3 Q, n;
+        const vec3 ldir = light_vec(l.direction), lpos = light_vec(l.position);
+        if(l.type == VKRT_LIGHT_RECT)
+        {
+            const vec3 tangent = light_vec(l.tangent);
+            const vec3 bitangent = cross(ldir, tangent);
+            Q = (lpos + tangent * ((2.0f * xi.x - 1.0f) * l.size_x)) + bitangent * ((2.0f * xi.y - 1.0f) * l.size_y);
+            n = ldir;
+            ret.pdf = 1.0f / (4.0f * l.size_x * l.size_y);
+        }
+        else if(l.type == VKRT_LIGHT_DISK)
+        {
+            const vec2 d = sample_unit_disc(xi);
+            Q = lpos + to_world(local_frame(ldir), vec3{l.size_x * d.x, l.size_x * d.y, 0.0f});
+            n = ldir;
+            ret.pdf = 1.0f / (PI * l.size_x * l.size_x);
+        }
+        else
+        {
+            const float phi = 2.0f * PI * xi.x;
+            n = to_world(local_frame(ldir), vec3{cosf(phi), sinf(phi), 0.0f});
+            Q = (lpos + ldir * ((2.0f * xi.y - 1.0f) * l.size_y)) + n * l.size_x;
+            ret.pdf = 1.0f / (4.0f * PI * l.size_x * l.size_y);
+        }
+        const vec3 to_q = Q - P;
+        const float d2 = dot(to_q, to_q);
+        ret.dist = fmaxf(sqrtf(d2), EPS);
+        ret.direction = vec3{to_q.x / ret.dist, to_q.y / ret.dist, to_q.z / ret.dist};
+        const float cos_light = dot(n, -ret.direction);
+        ret.pdf *= d2 / fmaxf(cos_light, EPS);
+        if(cos_light <= 0.0f) ret.radiance = splat(0.f);
+    }
+    else// omni, spot
+    {
+        const vec3 to_light = light_vec(l.position) - P;
+        ret.dist = fmaxf(length3(to_light), EPS);
+        ret.direction = vec3{to_light.x / ret.dist, to_light.y / ret.dist, to_light.z / ret.dist};
+        float w = ret.dist / l.range;
+        w *= w;
+        float attenuation = clampf(1.0f - w * w, 0.0f, 1.0f) / (ret.dist * ret.dist);
+        if(l.type == VKRT_LIGHT_SPOT)
+        {
+            const float cd = dot(light_vec(l.direction), -ret.direction);
+            const float ang = clampf(cd * l.spot_angle_scale + l.spot_angle_offset, 0.0f, 1.0f);
+            attenuation *= ang * ang;
+        }
+        ret.radiance = ret.radiance * attenuation;
+    }
+    return ret;
+}
+
+//! power heuristic with beta 2, overflow-safe (direct_lighting.slang:139-144)
+VK_HD float power_heuristic(float p_a, float p_b)
+{
+    const float r = p_b / p_a;
+    return 1.0f / (1.0f + r * r);
+}
+
+//! lights that BSDF-sampled rays can reach (direct_lighting.slang:148-152)
+VK_HD bool bsdf_reachable(const vkrt_light &l)
+{
+    return (l.type == VKRT_LIGHT_DIRECTIONAL && l.angular_size > 0.0f) || l.type == VKRT_LIGHT_RECT ||
+           l.type == VKRT_LIGHT_SPHERE || l.type == VKRT_LIGHT_TUBE || l.type == VKRT_LIGHT_DISK;
+}
+
+struct light_hit
+{
+    float t;        //!< <= 0: miss
+    float cos_light;//!< at the light surface, facing the ray origin
+};
+
+//! front-facing intersection with a light body (direct_lighting.slang:166-236)
+VK_HD light_hit intersect_light(const vkrt_light &l, const vec3 &P, const vec3 &dir)
+{
+    light_hit ret{0.0f, 0.0f};
+    const vec3 lpos = light_vec(l.position), ldir = light_vec(l.direction);
+    if(l.type == VKRT_LIGHT_SPHERE)
+    {
+        const vec3 to_center = lpos - P;
+        const float d2 = dot(to_center, to_center);
+        const float r2 = l.size_x * l.size_x;
+        if(d2 <= r2) return ret;
+        const float b = dot(dir, to_center);
+        const float disc = b * b - d2 + r2;
+        if(disc <= 0.0f) return ret;
+        const float t = b - sqrtf(disc);
+        if(t <= 0.0f) return ret;
+        ret.t = t;
+        const vec3 q = (P + dir * t) - lpos;
+        ret.cos_light = dot(vec3{q.x / l.size_x, q.y / l.size_x, q.z / l.size_x}, -dir);
+    }
+    else if(l.type == VKRT_LIGHT_RECT || l.type == VKRT_LIGHT_DISK)
+    {
+        const float cos_light = dot(ldir, -dir);
+        if(cos_light <= 0.0f) return ret;
+        const float t = dot(lpos - P, ldir) / -cos_light;
+        if(t <= 0.0f) return ret;
+        const vec3 q = (P + dir * t) - lpos;
+        if(l.type == VKRT_LIGHT_RECT)
+        {
+            const vec3 tangent = light_vec(l.tangent);
+            const vec3 bitangent = cross(ldir, tangent);
+            if(fabsf(dot(q, tangent)) > l.size_x || fabsf(dot(q, bitangent)) > l.size_y) return ret;
+        }
+        else if(dot(q, q) > l.size_x * l.size_x)
+            return ret;
+        ret.t = t;
+        ret.cos_light = cos_light;
+    }
+    else if(l.type == VKRT_LIGHT_TUBE)
+    {
+        const vec3 o = P - lpos;
+        const vec3 d_perp = dir - ldir * dot(dir, ldir);
+        const vec3 o_perp = o - ldir * dot(o, ldir);
+        const float A = dot(d_perp, d_perp);
+        const float B = dot(o_perp, d_perp);
+        const float C = dot(o_perp, o_perp) - l.size_x * l.size_x;
+        const float disc = B * B - A * C;
+        if(A <= EPS || disc <= 0.0f || C <= 0.0f) return ret;
+        const float t = (-B - sqrtf(disc)) / A;
+        if(t <= 0.0f || fabsf(dot(o + dir * t, ldir)) > l.size_y) return ret;
+        const vec3 nq = o_perp + d_perp * t;
+        const float cos_light = dot(vec3{nq.x / l.size_x, nq.y / l.size_x, nq.z / l.size_x}, -dir);
+        if(cos_light <= 0.0f) return ret;
+        ret.t = t;
+        ret.cos_light = cos_light;
+    }
+    return ret;
+}
+
+//! NEE-side MIS weight (direct_lighting.slang:238-244)
+VK_HD float mis_weight_nee(uint32_t mode, float p_nee, float p_bsdf)
+{
+    if(mode == VKRT_MIS_NEE_ONLY) return 1.0f;
+    if(mode == VKRT_MIS_BSDF_ONLY) return 0.0f;
+    return power_heuristic(p_nee, p_bsdf);
+}
+
+//! hit-side MIS weight (direct_lighting.slang:246-252); prev_bsdf_pdf <= 0: no NEE competitor
+VK_HD float mis_weight_hit(uint32_t mode, float prev_bsdf_pdf, float p_nee)
+{
+    if(mode == VKRT_MIS_NEE_ONLY) return 0.0f;
+    if(mode == VKRT_MIS_BSDF_ONLY || prev_bsdf_pdf <= 0.0f) return 1.0f;
+    return power_heuristic(prev_bsdf_pdf, p_nee);
+}
+
+//! solid-angle density of sample_light producing `dir` from P (direct_lighting.slang:254-292)
+VK_HD float light_pdf(const vkrt_light &l, const vec3 &P, const vec3 &dir)
+{
+    if(l.type == VKRT_LIGHT_DIRECTIONAL)
+    {
+        if(l.angular_size <= 0.0f) return 0.0f;
+        const float cos_sigma = cosf(l.angular_size);
+        return dot(dir, -light_vec(l.direction)) < cos_sigma ? 0.0f : 1.0f / (2.0f * PI * (1.0f - cos_sigma));
+    }
+    if(l.type == VKRT_LIGHT_SPHERE)
+    {
+        const vec3 to_center = light_vec(l.position) - P;
+        const float d2 = dot(to_center, to_center);
+        const float r2 = l.size_x * l.size_x;
+        if(d2 <= r2) return 0.0f;
+        const float sin2_sigma = r2 / d2;
+        const float cos_sigma = sqrtf(1.0f - sin2_sigma);
+        return dot(dir, to_center * (1.0f / sqrtf(d2))) < cos_sigma ? 0.0f
+                                                                   : (1.0f + cos_sigma) / (2.0f * PI * sin2_sigma);
+    }
+    if(l.type == VKRT_LIGHT_RECT || l.type == VKRT_LIGHT_DISK || l.type == VKRT_LIGHT_TUBE)
+    {
+        const light_hit hit = intersect_light(l, P, dir);
+        if(hit.t <= 0.0f) return 0.0f;
+        const float area = l.type == VKRT_LIGHT_RECT   ? 4.0f * l.size_x * l.size_y
+                           : l.type == VKRT_LIGHT_DISK ? PI * l.size_x * l.size_x
+                                                       : 4.0f * PI * l.size_x * l.size_y;
+        return hit.t * hit.t / (area * hit.cos_light);
+    }
+    return 0.0f;
 }
 
 //! environment_white (utils/procedural_environment.slang:7-11)

@@ -358,6 +358,29 @@ def room_camera():
                           fov_y=float(np.radians(45.0)))
 
 
+def room_lights():
+    """analytic lights for the room scene, one of each kind the tracer shades: a rect panel and a sphere near
+    the ceiling, a spot aimed at the back wall, a tube along the back wall, an omni near the floor and a sun
+    disc shining in through the open front"""
+    import vierkant_amd as vk
+
+    def unit(v):
+        v = np.asarray(v, np.float64)
+        return tuple((v / np.linalg.norm(v)).astype(np.float32))
+
+    return np.array([
+        vk.light(vk.LIGHT_RECT, position=(1.5, 5.9, 0.0), direction=(0.0, -1.0, 0.0), tangent=(1.0, 0.0, 0.0),
+                 color=(1.0, 0.95, 0.9), intensity=12.0, size=(0.5, 0.4)),
+        vk.light(vk.LIGHT_SPHERE, position=(-1.5, 4.5, 1.0), color=(1.0, 0.8, 0.6), intensity=30.0, size=(0.2, 0.0)),
+        vk.light(vk.LIGHT_SPOT, position=(0.0, 5.5, 2.5), direction=unit((0.0, -0.6, -1.0)), color=(0.6, 0.8, 1.0),
+                 intensity=40.0, inner_cone_angle=0.2, outer_cone_angle=0.5),
+        vk.light(vk.LIGHT_TUBE, position=(0.0, 3.0, -2.8), direction=(1.0, 0.0, 0.0), color=(0.9, 1.0, 0.9),
+                 intensity=20.0, size=(0.05, 1.0)),
+        vk.light(vk.LIGHT_OMNI, position=(2.0, 1.0, 2.0), intensity=5.0),
+        vk.light(vk.LIGHT_DIRECTIONAL, direction=unit((0.2, -0.5, -0.85)), intensity=3.0, angular_size=0.05),
+    ], dtype=vk.LIGHT_DTYPE)
+
+
 def camera_basis(camera, width, height):
     """the tracer's camera basis, computed like the library does (double precision, rounded once):
     returns (eye, forward, right, up, tan_half_x, tan_half_y) as float32"""
