@@ -67,9 +67,10 @@ def cpu_baseline(tris, rays, budget_s):
                       "extrapolated to rays/s"}
 
 
-def path_tracer_frame(ctx, spp):
+def path_tracer_frame(ctx, spp, lit=False):
     """one PBRPathTracer batch of the Cornell-style room (12 quads + 200 spheres + the jittered icosphere,
-    276,504 triangles) at 1920x1080, max_trace_depth 8, `spp` samples per pixel"""
+    276,504 triangles) at 1920x1080, max_trace_depth 8, `spp` samples per pixel; `lit` adds the room's six
+    analytic lights (next-event estimation with a transmittance shadow ray per surface vertex, MIS)"""
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
     scene = W.room_scene()
@@ -78,6 +79,8 @@ def path_tracer_frame(ctx, spp):
     s.num_samples, s.max_trace_depth = spp, 8
     pt = vk.PathTracer(ctx, s, seed=0)
     pt.set_scene(scene)
+    if lit:
+        pt.set_lights(W.room_lights())
     cam = W.room_camera()
     s_warm = vk.default_settings()
     s_warm.resolution[0], s_warm.resolution[1] = 1920, 1080
@@ -90,7 +93,8 @@ def path_tracer_frame(ctx, spp):
     st = pt.stats()
     px = pt.pixels()
     seg_bytes = 332      # per path segment: SoA ray, hit and path state in/out, 3 indices + 3 vertices, material
-    out = {"workload": "1920x1080, max_trace_depth 8, %d spp, 276,504 triangles" % spp,
+    out = {"workload": "1920x1080, max_trace_depth 8, %d spp, 276,504 triangles%s"
+                       % (spp, ", 6 analytic lights (NEE + MIS)" if lit else ""),
            "frame_ms": round(st.raytrace_ms, 3), "samples": int(st.samples), "segments": int(st.segments),
            "samples_per_s": round(st.samples / (st.raytrace_ms * 1e-3), 1),
            "segments_per_s": round(st.segments / (st.raytrace_ms * 1e-3), 1),
@@ -134,8 +138,11 @@ def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64,
 
     plain = vk.PathTracer(ctx, settings(64, 0), seed=0)
     plain.set_scene(scene)
+    # max_num_batches 0 keeps the batch index at 0 (min(index + 1, 0), PBRPathTracer.cpp:211), so every
+    # render replaces the image: the first warms the kernels, the second is the timed 64-spp image
     plain.render(cam)
     plain.render(cam)
+    assert plain.current_batch() == 0
     target = rmse(plain.pixels()["radiance"])
     plain_ms = float(plain.stats().raytrace_ms)
     plain.close()
@@ -295,6 +302,8 @@ def main():
     ap.add_argument("--verify-rays", type=int, default=65536)
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
     ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
+    ap.add_argument("--pt-lit-spp", type=int, default=16,
+                    help="samples per pixel of the lit path-tracer frame (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
     ap.add_argument("--instanced", type=int, default=1, help="trace the two-level version of the scene (0: skip)")
     ap.add_argument("--rc-ref-spp", type=int, default=4096,
@@ -478,6 +487,9 @@ def main():
     if args.pt_spp:
         log("path tracer frame")
         result["config"]["path_tracer"] = path_tracer_frame(ctx, args.pt_spp)
+    if args.pt_lit_spp:
+        log("lit path tracer frame")
+        result["config"]["path_tracer_lit"] = path_tracer_frame(ctx, args.pt_lit_spp, lit=True)
     if args.batch_scenes:
         log("batched scenes")
         result["config"]["batched"] = batched_frame(ctx, args.batch_scenes, 4096)

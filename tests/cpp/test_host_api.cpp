@@ -164,6 +164,46 @@ int main()
     if(px.size() != 32 * 16 || !(px[0].radiance[0] > 0.f)) return fail("pixels");
     pt->reset_accumulator();
     if(pt->current_batch() != 0) return fail("reset");
+
+    // the sun of settings_t: direction quat(elevation, azimuth, 0) * (0, 0, 1)
+    sunlight_params_t sp;
+    sp.spherical_coords[0] = 0.3f;
+    sp.spherical_coords[1] = 0.7f;
+    const light_t sl = sun_light(sp);
+    const double want_dir[3] = {std::cos(0.3) * std::sin(0.7), -std::sin(0.3), std::cos(0.3) * std::cos(0.7)};
+    for(int k = 0; k < 3; ++k)
+        if(std::fabs(sl.direction[k] - want_dir[k]) > 1e-6) return fail("sun direction");
+
+    // a rough plane (rho 0.5) lit straight down by a delta sun of intensity 2, seen from above: the
+    // next-event estimate is f * 2 with f = (0.5 + 0.02) / pi at normal incidence
+    float qpos[12] = {-50.f, 0.f, -50.f, 50.f, 0.f, -50.f, 50.f, 0.f, 50.f, -50.f, 0.f, 50.f};
+    float qnrm[12] = {0.f, 1.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f, 0.f, 1.f, 0.f};
+    uint32_t qidx[6] = {0, 2, 1, 0, 3, 2}, qobj[2] = {0, 0};
+    vkrt_material pm{};
+    pm.color[0] = pm.color[1] = pm.color[2] = 0.5f;
+    pm.color[3] = 1.f;
+    pm.roughness = 1.f;
+    pm.ior = 1.5f;
+    PBRPathTracer::scene_t plane{qpos, qnrm, 4, qidx, 2, qobj, &obj_mat, 1, &pm, 1, 0.0f};
+    PBRPathTracer::create_info_t lit_info;
+    lit_info.settings.resolution[0] = lit_info.settings.resolution[1] = 8;
+    lit_info.settings.num_samples = 16;
+    lit_info.settings.max_trace_depth = 2;
+    sunlight_params_t sun;
+    sun.color[0] = sun.color[1] = sun.color[2] = 1.f;
+    sun.intensity = 2.f;
+    sun.spherical_coords[0] = 1.5707963267948966f;
+    sun.spherical_coords[1] = 0.f;
+    lit_info.settings.sunlight_params = sun;
+    auto lit = PBRPathTracer::create(ctx, lit_info);
+    lit->set_scene(plane);
+    PBRPathTracer::camera_t down{{0.f, 5.f, 0.f}, 0.0698131701f, {0.f, -1.f, 0.f}, 0.f, {0.f, 0.f, -1.f}, 1.f};
+    lit->render_scene(down);
+    double mean = 0.0;
+    for(const auto &p: lit->pixels()) mean += p.radiance[1];
+    mean /= 64.0;
+    const double want_lit = 0.52 / 3.14159265358979 * 2.0;
+    if(std::fabs(mean / want_lit - 1.0) > 0.01) return fail("sun-lit plane");
     std::printf("host api ok (%d hits of %zu rays)\n", n_hit, rays.size());
     return 0;
 }

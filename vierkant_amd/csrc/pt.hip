@@ -44,6 +44,7 @@ constexpr uint32_t VERTEX_F4 = 7;  // training vertex: 4 float4 of features + (e
 constexpr uint32_t SHADOW_F4 = 6;  // shadow record: ray o, d; (radiance, mis weight); (F, cos); (beta, pdf);
                                    // (pixel, training vertex, -, -)
 constexpr uint32_t NO_VERTEX = 0xffffffffu;
+constexpr uint32_t TRAIN_COUNTERS = 8;// offset of the training stream's counters
 constexpr int PT_BLOCK = 256;
 constexpr int PT_LDS = 16;
 
@@ -103,7 +104,9 @@ struct pt_buffers
     uint4 *info;            // first-hit depth bits, normal, object id, albedo
     uint32_t *queue[2];
     uint32_t *counter;      // [0], [1]: queue sizes; [2..3]: segment count (64-bit); [4]: cache queries;
-                            // [5]: training records
+                            // [5]: training records; [6]: shadow rays. Training paths of the cache mode run on
+                            // their own stream with the same layout at counter + TRAIN_COUNTERS
+    uint32_t *train_queue;  // the training paths' first queue (cache mode)
     vkrt_pixel *pixels;
     float4 *qrec;           // cache queries (QUERY_F4 float4 each)
     float4 *tvtx;           // training paths' vertices [path][depth] (VERTEX_F4 float4 each)
@@ -127,11 +130,15 @@ __device__ __forceinline__ float frac(float x) { return x - floorf(x); }
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b, uint32_t sample)
 {
     const uint32_t p = blockIdx.x * PT_BLOCK + threadIdx.x;
+    // in the cache mode every train_stride-th pixel (from train_offset) traces a training path, queued on
+    // its own; the others keep their order in the main queue
+    const uint32_t n_train = f.cache_on ? f.n_train_paths : 0u;
     if(p == 0)
     {
-        b.counter[0] = f.n_pix;
+        b.counter[0] = f.n_pix - n_train;
         b.counter[4] = 0u;
         b.counter[5] = 0u;
+        b.counter[TRAIN_COUNTERS] = n_train;
     }
     if(p >= f.n_pix) return;
     const uint32_t x = p % f.width, y = p / f.width;
@@ -185,7 +192,13 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
     b.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     b.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     b.iors[p] = make_float4(1.f, 1.f, 1.f, 1.f);
-    b.queue[0][p] = p;
+    if(n_train == 0u) b.queue[0][p] = p;
+    else if(st.y & F_TRAIN) b.train_queue[(p - f.train_offset) / f.train_stride] = p;
+    else
+    {
+        const uint32_t below = p > f.train_offset ? (p - f.train_offset - 1u) / f.train_stride + 1u : 0u;
+        b.queue[0][p - below] = p;
+    }
 }
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_zero(uint32_t *c) { c[threadIdx.x] = 0u; }
@@ -197,15 +210,11 @@ __global__ void k_pt_reset_depth(uint32_t *c, int next)
     c[6] = 0u;
 }
 
-__global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
-                                                       const float4 *__restrict__ tris, int32_t root, bool empty,
-                                                       const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
-                                                       bool wide, pt_buffers b, int in)
+//! the closest hit of path p's current ray; back faces are culled while the path is in air
+__device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
+                                           int32_t root, bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
+                                           int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem)
 {
-    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
-    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
-    if(q >= b.counter[in]) return;
-    const uint32_t p = b.queue[in][q];
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
     if(!empty)
@@ -214,11 +223,22 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__re
         const bool in_air = (b.state[p].y & F_MEDIA_MASK) == 0u;
         // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
         if(wide)
-            traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem + threadIdx.x, t, u, v, prim, in_air);
+            traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air);
         else
-            traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim, in_air);
+            traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem, t, u, v, prim, in_air);
     }
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
+}
+
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
+                                                       const float4 *__restrict__ tris, int32_t root, bool empty,
+                                                       const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
+                                                       bool wide, pt_buffers b, int in)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(q >= b.counter[in]) return;
+    trace_path(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x);
 }
 
 //! eval_distant_lights (raypipeline.slang:728-744): sun discs seen by a ray that left the scene
@@ -260,6 +280,268 @@ __device__ vec3 ghost_lights(const vkrt_light *__restrict__ lights, const pt_fra
     return radiance;
 }
 
+//! one path's vertex at `depth` (closest hit or miss): emission, next-event sample, BSDF sample, Russian
+//! roulette, light bodies met on the segment; writes the path state back and reports what the path needs
+//! next. LIGHTS (the frame has analytic lights) compiles next-event estimation and the light-body passes in,
+//! so the unlit kernels keep their smaller register footprint
+template<bool LIGHTS>
+__device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &b, const pt_scene_dev &s,
+                                           uint32_t depth, uint32_t p, bool &alive, bool &to_cache, bool &need_shadow,
+                                           float4 (&shadow_rec)[SHADOW_F4], float4 (&qfeat)[QUERY_F4])
+{
+    uint4 st = b.state[p];
+    // rng_state = xxhash32(rng_state, i) in the raygen chain (raypipeline.slang:150); the hit program
+    // draws from a copy
+    st.x = xxhash32(st.x, depth);
+    uint32_t rng = st.x;
+    uint32_t flags = st.y;
+    float4 ro = b.ray_o[p], rd = b.ray_d[p];
+    vec3 dir = xyz(rd);
+    const float4 beta4 = b.beta[p];
+    vec3 beta = xyz(beta4);
+    float prev_pdf = beta4.w;// payload.prev_bsdf_pdf
+    // the traced segment, for the ghost-light pass after shading
+    const vec3 seg_o = xyz(ro), seg_d = dir, seg_beta = beta;
+    const float seg_tmin = ro.w, seg_tmax = rd.w, seg_prev_pdf = prev_pdf;
+    const bool seg_caustic = (flags & F_CAUSTIC) != 0u;
+    vec3 radiance = xyz(b.rad[p]);
+    float4 iors = b.iors[p];
+    float ior_stack[MEDIA_STACK] = {iors.x, iors.y, iors.z, iors.w};
+    uint32_t media_index = flags & F_MEDIA_MASK;
+    uint4 h = b.hit[p];
+    bool stop = false;
+    to_cache = false;
+    uint32_t object = NO_OBJECT;
+    vec3 position{0.f, 0.f, 0.f}, normal{0.f, 0.f, 0.f}, albedo{0.f, 0.f, 0.f};
+    uint32_t media_op = 0;// 0 NoOp, 1 Enter, 2 Leave
+    float next_ior = 1.f;
+    const bool train = (flags & F_TRAIN) != 0u;
+    // training vertex: kind 1 = surface (features, emission, throughput weight), 2 = miss (sky radiance)
+    uint32_t vkind = 0;
+    vec3 v_e{0.f, 0.f, 0.f}, v_w{0.f, 0.f, 0.f};
+    float4 feat[4];
+
+    if(h.w == VKRT_INVALID_PRIM)
+    {
+        // miss_default (raypipeline.slang:746-761)
+        stop = true;
+        vec3 col = environment_white(dir);
+        radiance = radiance + mul3(beta * f.environment, col);
+        vkind = 2;
+        v_e = col * f.environment;
+        if(LIGHTS && !(f.suppress_caustics && seg_caustic))
+        {
+            const vec3 dl = eval_distant_lights(s.lights, f.n_lights, f.mis_mode, dir, prev_pdf);
+            radiance = radiance + mul3(beta, dl);
+            v_e = v_e + dl;
+        }
+    }
+    else
+    {
+        const uint32_t prim = h.w;
+        const float u = __uint_as_float(h.y), v = __uint_as_float(h.z);
+        const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
+        const vec3 v0 = ld3f(s.pos, i0), v1 = ld3f(s.pos, i1), v2 = ld3f(s.pos, i2);
+        const vec3 e1 = v1 - v0, e2 = v2 - v0;
+        const bool backface = dot(e1, vierkant::rt::cross(dir, e2)) < 0.f;
+        // interpolate_vertex (raypipeline.slang:336-358), identity instance transform
+        const float bx = 1.0f - u - v, by = u, bz = v;
+        position = (v0 * bx + v1 * by) + v2 * bz;
+        vec3 n = (ld3f(s.nrm, i0) * bx + ld3f(s.nrm, i1) * by) + ld3f(s.nrm, i2) * bz;
+        normal = normalize3(n);
+        if(backface) normal = -normal;
+        object = s.tri_object[prim];
+        const gpu_material gm = s.mats[s.object_material[object]];
+        const float tmax_inside = s.object_tmax[object];
+        const vec3 V = -dir;
+        ro = make_float4(position.x, position.y, position.z, ro.w);
+
+        material m;
+        m.color = f.disable_material ? splat(0.8f) : xyz(gm.color);
+        m.alpha = f.disable_material ? 1.0f : gm.color.w;
+        m.emission = xyz(gm.emission);
+        m.emission_strength = gm.emission.w;
+        m.metalness = gm.mrti.x;
+        m.roughness = gm.mrti.y;
+        m.transmission = gm.mrti.z;
+        m.ior = gm.mrti.w;
+        m.clearcoat = gm.cc.x;
+        m.clearcoat_roughness = gm.cc.y;
+        m.sheen_roughness = gm.cc.z;
+        m.diffuse_transmission = gm.cc.w;
+        m.sheen_color = xyz(gm.sheen);
+        m.diffuse_transmission_color = xyz(gm.dtc);
+
+        feat[0] = make_float4(position.x, position.y, position.z, m.roughness);
+        feat[1] = make_float4(V.x, V.y, V.z, m.metalness);
+        feat[2] = make_float4(normal.x, normal.y, normal.z, m.transmission);
+        feat[3] = make_float4(m.color.x, m.color.y, m.color.z, 0.f);
+        if(f.cache_on && !train && depth == f.cache_depth)
+        {
+            // terminate into the cache: the query kernel adds beta * L_o(x, wo), emission included
+            stop = true;
+            to_cache = true;
+        }
+        else
+        {
+        const float media_ior = ior_stack[media_index];
+        const float last_ior = ior_stack[media_index > 0u ? media_index - 1u : 0u];
+        float eta = backface ? m.ior / last_ior : media_ior / m.ior;
+        eta += EPS;
+        next_ior = m.ior;
+        const bool media_boundary = m.transmission > 0.0f;
+        media_op = !media_boundary ? 0u : (backface ? 2u : 1u);
+        if(depth == 0) albedo = m.color;
+        const vec3 sn = normal;
+        const vec3 ff = dot(dir, sn) < 0.f ? sn : -sn;// faceforward(sn, ray dir, sn)
+        vec3 emission = m.emission * (dot(sn, ff) > 0.f ? m.emission_strength : 0.0f);
+        radiance = radiance + mul3(beta, emission);
+        vkind = 1;
+        v_e = emission;
+        if(flags & F_REGULARIZE) m.roughness = fmaxf(m.roughness, REGULARIZE_MIN_ROUGHNESS);
+        bsdf_sample bs = sample_disney(m, sn, V, eta, rng);
+        if(LIGHTS)
+        {
+            // nee_surface (direct_lighting.slang:296-336): one light picked uniformly, one sample of it; the
+            // shadow ray's transmittance is applied by k_pt_shadow
+            const uint32_t nl = f.n_lights;
+            const uint32_t index = nl > 1u ? min(static_cast<uint32_t>(rnd(rng) * static_cast<float>(nl)), nl - 1u)
+                                            : 0u;
+            vec2 xi;
+            xi.x = rnd(rng);
+            xi.y = rnd(rng);
+            const vkrt_light l = s.lights[index];
+            const light_sample ls = sample_light(l, position, xi);
+            if(ls.radiance.x > 0.0f || ls.radiance.y > 0.0f || ls.radiance.z > 0.0f)
+            {
+                float bsdf_pdf = 0.0f;
+                const float cos_theta = fmaxf(dot(sn, ls.direction), 0.0f);
+                const vec3 F = eval_disney(m, ls.direction, sn, V, eta, bsdf_pdf);
+                const float weight = bsdf_reachable(l)
+                                             ? mis_weight_nee(f.mis_mode, ls.pdf / static_cast<float>(nl), bsdf_pdf)
+                                             : 1.0f;
+                const vec3 org = position + sn * EPS;
+                shadow_rec[0] = make_float4(org.x, org.y, org.z, 0.0f);
+                shadow_rec[1] = make_float4(ls.direction.x, ls.direction.y, ls.direction.z, ls.dist);
+                shadow_rec[2] = make_float4(ls.radiance.x, ls.radiance.y, ls.radiance.z, weight);
+                shadow_rec[3] = make_float4(F.x, F.y, F.z, cos_theta);
+                shadow_rec[4] = make_float4(beta.x, beta.y, beta.z, ls.pdf);
+                const uint32_t vtx = train ? (p / f.train_stride) * f.max_depth + depth : NO_VERTEX;
+                shadow_rec[5] = make_float4(__uint_as_float(p), __uint_as_float(vtx), 0.f, 0.f);
+                need_shadow = true;
+            }
+        }
+        if(!(bs.pdf > 0.0f)) { stop = true; }
+        else
+        {
+            prev_pdf = LIGHTS ? bs.pdf : 0.0f;
+            dir = bs.direction;
+            const bool specular = bs.pdf < REGULARIZE_PDF_THRESHOLD;
+            if(specular) flags |= F_REGULARIZE;
+            const bool caustic = specular && ((flags & F_CAUSTIC) || bs.transmission);
+            flags = caustic ? (flags | F_CAUSTIC) : (flags & ~F_CAUSTIC);
+            const float cos_theta = fabsf(dot(sn, bs.direction));
+            const float denom = fmaxf(bs.pdf, PDF_EPS);
+            vec3 thr = bs.F * cos_theta;
+            thr = vec3{thr.x / denom, thr.y / denom, thr.z / denom};
+            if(depth >= 1 && f.max_path_beta > 0.0f)
+                thr = vec3{fminf(thr.x, f.max_path_beta), fminf(thr.y, f.max_path_beta),
+                           fminf(thr.z, f.max_path_beta)};
+            beta = mul3(beta, thr);
+            v_w = thr;
+            if(bs.transmission) rd.w = media_op == 1u ? tmax_inside : DEFAULT_TMAX;
+            media_op = bs.transmission ? media_op : 0u;
+            // Russian roulette (raypipeline.slang:667-677)
+            if(max3(beta) <= 0.05f * static_cast<float>(depth))
+            {
+                float qq = 1.0f - max3(beta);
+                if(rnd(rng) < qq)
+                {
+                    stop = true;
+                    v_w = vec3{0.f, 0.f, 0.f};
+                }
+                else
+                {
+                    const float keep = 1.0f - qq;
+                    beta = vec3{beta.x / keep, beta.y / keep, beta.z / keep};
+                    v_w = vec3{v_w.x / keep, v_w.y / keep, v_w.z / keep};
+                }
+            }
+        }
+        }
+    }
+    vec3 ghost{0.f, 0.f, 0.f};
+    if(LIGHTS)
+    {
+        // segment end: the hit point, or the traced tmax on a miss
+        const float t_end = h.w == VKRT_INVALID_PRIM ? seg_tmax : length3(position - seg_o);
+        radiance = ghost_lights(s.lights, f, seg_o, seg_d, seg_tmin, t_end, seg_beta, seg_prev_pdf, seg_caustic,
+                                radiance, ghost);
+    }
+    if(train && vkind != 0u)
+    {
+        float4 *tv = b.tvtx + ((size_t)(p / f.train_stride) * f.max_depth + depth) * VERTEX_F4;
+        tv[6] = make_float4(ghost.x, ghost.y, ghost.z, 0.f);
+        if(vkind == 1u)
+        {
+            tv[0] = feat[0];
+            tv[1] = feat[1];
+            tv[2] = feat[2];
+            tv[3] = feat[3];
+        }
+        tv[4] = make_float4(v_e.x, v_e.y, v_e.z, __uint_as_float(vkind));
+        tv[5] = make_float4(v_w.x, v_w.y, v_w.z, 0.f);
+    }
+    if(to_cache)
+    {
+        qfeat[0] = feat[0];
+        qfeat[1] = feat[1];
+        qfeat[2] = feat[2];
+        qfeat[3] = feat[3];
+        qfeat[4] = make_float4(__uint_as_float(p), beta.x, beta.y, beta.z);
+    }
+
+    // media stack update in the raygen loop (raypipeline.slang:195-200)
+    if(media_op == 1u)
+    {
+        media_index = min(media_index + 1u, MEDIA_STACK - 1u);
+        ior_stack[media_index] = round_half(next_ior);
+    }
+    else if(media_op == 2u)
+        media_index = media_index > 0u ? media_index - 1u : 0u;
+    flags = (flags & ~F_MEDIA_MASK) | media_index;
+
+    if(depth == 0)
+    {
+        const bool miss = object == NO_OBJECT;
+        if(miss) flags |= F_FIRST_MISS;
+        else flags &= ~F_FIRST_MISS;
+        const float zdepth = miss ? 0.0f : dot(position - f.eye, f.forward);
+        b.info[p] = make_uint4(__float_as_uint(zdepth), pack_normal_oct(normal), object, pack_rgba8(albedo, 1.0f));
+    }
+    st.y = flags;
+    b.state[p] = st;
+    alive = !stop && depth + 1 < f.max_depth;
+    if(alive)
+    {
+        b.ray_o[p] = ro;
+        b.ray_d[p] = make_float4(dir.x, dir.y, dir.z, rd.w);
+        b.beta[p] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
+        b.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.f);
+        b.iors[p] = make_float4(ior_stack[0], ior_stack[1], ior_stack[2], ior_stack[3]);
+    }
+    else
+    {
+        // result.radiance += payload.radiance; coverage = max(coverage, first hit ? 1 : 0)
+        float4 a = b.acc[p];
+        const float cov = (flags & F_FIRST_MISS) ? 0.0f : 1.0f;
+        b.acc[p] = make_float4(a.x + radiance.x, a.y + radiance.y, a.z + radiance.z, fmaxf(a.w, cov));
+    }
+}
+
+//! one depth of shading over a compacted queue; survivors, cache queries and shadow rays are appended with
+//! one ballot and one atomic per wave each
+template<bool LIGHTS>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b, pt_scene_dev s, uint32_t depth,
                                                        int in)
 {
@@ -275,254 +557,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
     if(active)
     {
         p = b.queue[in][q];
-        uint4 st = b.state[p];
-        // rng_state = xxhash32(rng_state, i) in the raygen chain (raypipeline.slang:150); the hit program
-        // draws from a copy
-        st.x = xxhash32(st.x, depth);
-        uint32_t rng = st.x;
-        uint32_t flags = st.y;
-        float4 ro = b.ray_o[p], rd = b.ray_d[p];
-        vec3 dir = xyz(rd);
-        const float4 beta4 = b.beta[p];
-        vec3 beta = xyz(beta4);
-        float prev_pdf = beta4.w;// payload.prev_bsdf_pdf
-        // the traced segment, for the ghost-light pass after shading
-        const vec3 seg_o = xyz(ro), seg_d = dir, seg_beta = beta;
-        const float seg_tmin = ro.w, seg_tmax = rd.w, seg_prev_pdf = prev_pdf;
-        const bool seg_caustic = (flags & F_CAUSTIC) != 0u;
-        vec3 radiance = xyz(b.rad[p]);
-        float4 iors = b.iors[p];
-        float ior_stack[MEDIA_STACK] = {iors.x, iors.y, iors.z, iors.w};
-        uint32_t media_index = flags & F_MEDIA_MASK;
-        uint4 h = b.hit[p];
-        bool stop = false;
-        to_cache = false;
-        uint32_t object = NO_OBJECT;
-        vec3 position{0.f, 0.f, 0.f}, normal{0.f, 0.f, 0.f}, albedo{0.f, 0.f, 0.f};
-        uint32_t media_op = 0;// 0 NoOp, 1 Enter, 2 Leave
-        float next_ior = 1.f;
-        const bool train = (flags & F_TRAIN) != 0u;
-        // training vertex: kind 1 = surface (features, emission, throughput weight), 2 = miss (sky radiance)
-        uint32_t vkind = 0;
-        vec3 v_e{0.f, 0.f, 0.f}, v_w{0.f, 0.f, 0.f};
-        float4 feat[4];
-
-        if(h.w == VKRT_INVALID_PRIM)
-        {
-            // miss_default (raypipeline.slang:746-761)
-            stop = true;
-            vec3 col = environment_white(dir);
-            radiance = radiance + mul3(beta * f.environment, col);
-            vkind = 2;
-            v_e = col * f.environment;
-            if(f.n_lights && !(f.suppress_caustics && seg_caustic))
-            {
-                const vec3 dl = eval_distant_lights(s.lights, f.n_lights, f.mis_mode, dir, prev_pdf);
-                radiance = radiance + mul3(beta, dl);
-                v_e = v_e + dl;
-            }
-        }
-        else
-        {
-            const uint32_t prim = h.w;
-            const float u = __uint_as_float(h.y), v = __uint_as_float(h.z);
-            const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
-            const vec3 v0 = ld3f(s.pos, i0), v1 = ld3f(s.pos, i1), v2 = ld3f(s.pos, i2);
-            const vec3 e1 = v1 - v0, e2 = v2 - v0;
-            const bool backface = dot(e1, vierkant::rt::cross(dir, e2)) < 0.f;
-            // interpolate_vertex (raypipeline.slang:336-358), identity instance transform
-            const float bx = 1.0f - u - v, by = u, bz = v;
-            position = (v0 * bx + v1 * by) + v2 * bz;
-            vec3 n = (ld3f(s.nrm, i0) * bx + ld3f(s.nrm, i1) * by) + ld3f(s.nrm, i2) * bz;
-            normal = normalize3(n);
-            if(backface) normal = -normal;
-            object = s.tri_object[prim];
-            const gpu_material gm = s.mats[s.object_material[object]];
-            const float tmax_inside = s.object_tmax[object];
-            const vec3 V = -dir;
-            ro = make_float4(position.x, position.y, position.z, ro.w);
-
-            material m;
-            m.color = f.disable_material ? splat(0.8f) : xyz(gm.color);
-            m.alpha = f.disable_material ? 1.0f : gm.color.w;
-            m.emission = xyz(gm.emission);
-            m.emission_strength = gm.emission.w;
-            m.metalness = gm.mrti.x;
-            m.roughness = gm.mrti.y;
-            m.transmission = gm.mrti.z;
-            m.ior = gm.mrti.w;
-            m.clearcoat = gm.cc.x;
-            m.clearcoat_roughness = gm.cc.y;
-            m.sheen_roughness = gm.cc.z;
-            m.diffuse_transmission = gm.cc.w;
-            m.sheen_color = xyz(gm.sheen);
-            m.diffuse_transmission_color = xyz(gm.dtc);
-
-            feat[0] = make_float4(position.x, position.y, position.z, m.roughness);
-            feat[1] = make_float4(V.x, V.y, V.z, m.metalness);
-            feat[2] = make_float4(normal.x, normal.y, normal.z, m.transmission);
-            feat[3] = make_float4(m.color.x, m.color.y, m.color.z, 0.f);
-            if(f.cache_on && !train && depth == f.cache_depth)
-            {
-                // terminate into the cache: the query kernel adds beta * L_o(x, wo), emission included
-                stop = true;
-                to_cache = true;
-            }
-            else
-            {
-            const float media_ior = ior_stack[media_index];
-            const float last_ior = ior_stack[media_index > 0u ? media_index - 1u : 0u];
-            float eta = backface ? m.ior / last_ior : media_ior / m.ior;
-            eta += EPS;
-            next_ior = m.ior;
-            const bool media_boundary = m.transmission > 0.0f;
-            media_op = !media_boundary ? 0u : (backface ? 2u : 1u);
-            if(depth == 0) albedo = m.color;
-            const vec3 sn = normal;
-            const vec3 ff = dot(dir, sn) < 0.f ? sn : -sn;// faceforward(sn, ray dir, sn)
-            vec3 emission = m.emission * (dot(sn, ff) > 0.f ? m.emission_strength : 0.0f);
-            radiance = radiance + mul3(beta, emission);
-            vkind = 1;
-            v_e = emission;
-            if(flags & F_REGULARIZE) m.roughness = fmaxf(m.roughness, REGULARIZE_MIN_ROUGHNESS);
-            bsdf_sample bs = sample_disney(m, sn, V, eta, rng);
-            if(f.n_lights)
-            {
-                // nee_surface (direct_lighting.slang:296-336): one light picked uniformly, one sample of it; the
-                // shadow ray's transmittance is applied by k_pt_shadow
-                const uint32_t nl = f.n_lights;
-                const uint32_t index = nl > 1u ? min(static_cast<uint32_t>(rnd(rng) * static_cast<float>(nl)), nl - 1u)
-                                                : 0u;
-                vec2 xi;
-                xi.x = rnd(rng);
-                xi.y = rnd(rng);
-                const vkrt_light l = s.lights[index];
-                const light_sample ls = sample_light(l, position, xi);
-                if(ls.radiance.x > 0.0f || ls.radiance.y > 0.0f || ls.radiance.z > 0.0f)
-                {
-                    float bsdf_pdf = 0.0f;
-                    const float cos_theta = fmaxf(dot(sn, ls.direction), 0.0f);
-                    const vec3 F = eval_disney(m, ls.direction, sn, V, eta, bsdf_pdf);
-                    const float weight = bsdf_reachable(l)
-                                                 ? mis_weight_nee(f.mis_mode, ls.pdf / static_cast<float>(nl), bsdf_pdf)
-                                                 : 1.0f;
-                    const vec3 org = position + sn * EPS;
-                    shadow_rec[0] = make_float4(org.x, org.y, org.z, 0.0f);
-                    shadow_rec[1] = make_float4(ls.direction.x, ls.direction.y, ls.direction.z, ls.dist);
-                    shadow_rec[2] = make_float4(ls.radiance.x, ls.radiance.y, ls.radiance.z, weight);
-                    shadow_rec[3] = make_float4(F.x, F.y, F.z, cos_theta);
-                    shadow_rec[4] = make_float4(beta.x, beta.y, beta.z, ls.pdf);
-                    const uint32_t vtx = train ? (p / f.train_stride) * f.max_depth + depth : NO_VERTEX;
-                    shadow_rec[5] = make_float4(__uint_as_float(p), __uint_as_float(vtx), 0.f, 0.f);
-                    need_shadow = true;
-                }
-            }
-            if(!(bs.pdf > 0.0f)) { stop = true; }
-            else
-            {
-                prev_pdf = f.n_lights ? bs.pdf : 0.0f;
-                dir = bs.direction;
-                const bool specular = bs.pdf < REGULARIZE_PDF_THRESHOLD;
-                if(specular) flags |= F_REGULARIZE;
-                const bool caustic = specular && ((flags & F_CAUSTIC) || bs.transmission);
-                flags = caustic ? (flags | F_CAUSTIC) : (flags & ~F_CAUSTIC);
-                const float cos_theta = fabsf(dot(sn, bs.direction));
-                const float denom = fmaxf(bs.pdf, PDF_EPS);
-                vec3 thr = bs.F * cos_theta;
-                thr = vec3{thr.x / denom, thr.y / denom, thr.z / denom};
-                if(depth >= 1 && f.max_path_beta > 0.0f)
-                    thr = vec3{fminf(thr.x, f.max_path_beta), fminf(thr.y, f.max_path_beta),
-                               fminf(thr.z, f.max_path_beta)};
-                beta = mul3(beta, thr);
-                v_w = thr;
-                if(bs.transmission) rd.w = media_op == 1u ? tmax_inside : DEFAULT_TMAX;
-                media_op = bs.transmission ? media_op : 0u;
-                // Russian roulette (raypipeline.slang:667-677)
-                if(max3(beta) <= 0.05f * static_cast<float>(depth))
-                {
-                    float qq = 1.0f - max3(beta);
-                    if(rnd(rng) < qq)
-                    {
-                        stop = true;
-                        v_w = vec3{0.f, 0.f, 0.f};
-                    }
-                    else
-                    {
-                        const float keep = 1.0f - qq;
-                        beta = vec3{beta.x / keep, beta.y / keep, beta.z / keep};
-                        v_w = vec3{v_w.x / keep, v_w.y / keep, v_w.z / keep};
-                    }
-                }
-            }
-            }
-        }
-        vec3 ghost{0.f, 0.f, 0.f};
-        if(f.n_lights)
-        {
-            // segment end: the hit point, or the traced tmax on a miss
-            const float t_end = h.w == VKRT_INVALID_PRIM ? seg_tmax : length3(position - seg_o);
-            radiance = ghost_lights(s.lights, f, seg_o, seg_d, seg_tmin, t_end, seg_beta, seg_prev_pdf, seg_caustic,
-                                    radiance, ghost);
-        }
-        if(train && vkind != 0u)
-        {
-            float4 *tv = b.tvtx + ((size_t)(p / f.train_stride) * f.max_depth + depth) * VERTEX_F4;
-            tv[6] = make_float4(ghost.x, ghost.y, ghost.z, 0.f);
-            if(vkind == 1u)
-            {
-                tv[0] = feat[0];
-                tv[1] = feat[1];
-                tv[2] = feat[2];
-                tv[3] = feat[3];
-            }
-            tv[4] = make_float4(v_e.x, v_e.y, v_e.z, __uint_as_float(vkind));
-            tv[5] = make_float4(v_w.x, v_w.y, v_w.z, 0.f);
-        }
-        if(to_cache)
-        {
-            qfeat[0] = feat[0];
-            qfeat[1] = feat[1];
-            qfeat[2] = feat[2];
-            qfeat[3] = feat[3];
-            qfeat[4] = make_float4(__uint_as_float(p), beta.x, beta.y, beta.z);
-        }
-
-        // media stack update in the raygen loop (raypipeline.slang:195-200)
-        if(media_op == 1u)
-        {
-            media_index = min(media_index + 1u, MEDIA_STACK - 1u);
-            ior_stack[media_index] = round_half(next_ior);
-        }
-        else if(media_op == 2u)
-            media_index = media_index > 0u ? media_index - 1u : 0u;
-        flags = (flags & ~F_MEDIA_MASK) | media_index;
-
-        if(depth == 0)
-        {
-            const bool miss = object == NO_OBJECT;
-            if(miss) flags |= F_FIRST_MISS;
-            else flags &= ~F_FIRST_MISS;
-            const float zdepth = miss ? 0.0f : dot(position - f.eye, f.forward);
-            b.info[p] = make_uint4(__float_as_uint(zdepth), pack_normal_oct(normal), object, pack_rgba8(albedo, 1.0f));
-        }
-        st.y = flags;
-        b.state[p] = st;
-        alive = !stop && depth + 1 < f.max_depth;
-        if(alive)
-        {
-            b.ray_o[p] = ro;
-            b.ray_d[p] = make_float4(dir.x, dir.y, dir.z, rd.w);
-            b.beta[p] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
-            b.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.f);
-            b.iors[p] = make_float4(ior_stack[0], ior_stack[1], ior_stack[2], ior_stack[3]);
-        }
-        else
-        {
-            // result.radiance += payload.radiance; coverage = max(coverage, first hit ? 1 : 0)
-            float4 a = b.acc[p];
-            const float cov = (flags & F_FIRST_MISS) ? 0.0f : 1.0f;
-            b.acc[p] = make_float4(a.x + radiance.x, a.y + radiance.y, a.z + radiance.z, fmaxf(a.w, cov));
-        }
+        shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
     }
     // append survivors to the next queue: one ballot and one atomic per wave
     const uint64_t mask = __ballot(alive);
@@ -784,22 +819,16 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_no
                                 smem + threadIdx.x);
 }
 
-//! the shadow half of nee_surface (direct_lighting.slang:311-333): the unculled transmittance toward the
-//! sampled light, then transmittance * weight * n_lights * radiance * F * cos / pdf scaled by the vertex's
-//! throughput joins the pixel's sample sum; a training vertex also records the unscaled estimate
-__global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
-                                                        const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
-                                                        bool wide, bool empty, const float4 *__restrict__ tris,
-                                                        const gpu_material *__restrict__ mats, pt_frame f,
-                                                        pt_buffers b)
+//! trace one shadow record and add its next-event estimate to the pixel (and to its training vertex)
+__device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
+                                            const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide,
+                                            bool empty, const float4 *__restrict__ tris,
+                                            const gpu_material *__restrict__ mats, const pt_frame &f,
+                                            const pt_buffers &b, const float4 (&rec)[SHADOW_F4], int32_t *smem)
 {
-    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
-    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
-    if(i >= b.counter[6]) return;
-    const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
     const float4 o = rec[0], d = rec[1];
     const float4 T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d),
-                                        false, smem + threadIdx.x);
+                                        false, smem);
     if(!(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)) return;
     const float4 lr = rec[2], fc = rec[3], bp = rec[4], id = rec[5];
     vec3 nee = vec3{T.x, T.y, T.z} * lr.w;
@@ -818,6 +847,55 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__r
         const float4 ev = *e;
         *e = make_float4(ev.x + nee.x, ev.y + nee.y, ev.z + nee.z, ev.w);
     }
+}
+
+//! the shadow half of nee_surface (direct_lighting.slang:311-333): the unculled transmittance toward the
+//! sampled light, then transmittance * weight * n_lights * radiance * F * cos / pdf scaled by the vertex's
+//! throughput joins the pixel's sample sum; a training vertex also records the unscaled estimate
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
+                                                        const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
+                                                        bool wide, bool empty, const float4 *__restrict__ tris,
+                                                        const gpu_material *__restrict__ mats, pt_frame f,
+                                                        pt_buffers b)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(i >= b.counter[6]) return;
+    const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
+    float4 r[SHADOW_F4];
+    for(uint32_t k = 0; k < SHADOW_F4; ++k) r[k] = rec[k];
+    shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x);
+}
+
+//! the cache mode's training paths, one lane per path through every depth: trace, shade and the shadow ray
+//! of the next-event estimate, with no kernel boundary between depths. Launched on the second stream at the
+//! start of a sample, its few waves stay resident beside the main paths' full-width launches, so the chain of
+//! dependent traversals does not wait for free compute units at each depth
+template<bool LIGHTS>
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node *__restrict__ nodes2,
+                                                             const float4 *__restrict__ tris, int32_t root2,
+                                                             bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
+                                                             int32_t root4, bool wide, pt_frame f, pt_buffers b,
+                                                             pt_scene_dev s)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
+    if(q >= b.counter[0]) return;
+    const uint32_t p = b.queue[0][q];
+    uint32_t segments = 0;
+    for(uint32_t depth = 0; depth < f.max_depth; ++depth)
+    {
+        trace_path(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x);
+        bool alive = false, to_cache = false, need_shadow = false;
+        float4 shadow_rec[SHADOW_F4];
+        float4 qfeat[QUERY_F4];
+        shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
+        ++segments;
+        if(LIGHTS && need_shadow)
+            shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec, smem + threadIdx.x);
+        if(!alive) break;
+    }
+    atomicAdd(reinterpret_cast<unsigned long long *>(&b.counter[2]), (unsigned long long)segments);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -928,6 +1006,10 @@ struct vkrt_pt
     uint32_t width = 0, height = 0;
     vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // the cache mode's training paths: their own queues, traced on a second stream beside the main paths
+    hipStream_t train_stream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    vkrt::device_buffer train_queue0, train_queue1;
     vkrt_pt_stats stats{};
     // radiance cache (not owned) and its per-sample buffers
     vkrt_rc *rc = nullptr;
@@ -1012,6 +1094,7 @@ static pt_buffers buffers(vkrt_pt *pt)
     b.counter = pt->counter.as<uint32_t>();
     b.pixels = pt->pixels.as<vkrt_pixel>();
     b.qrec = b.tvtx = b.trec = b.ttgt = b.shadow = nullptr;
+    b.train_queue = nullptr;
     return b;
 }
 
@@ -1105,42 +1188,85 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         b.trec = pt->trec.as<float4>();
         b.ttgt = pt->ttgt.as<float4>();
     }
-    if(f.n_lights)
+    if(f.n_lights && !f.cache_on)
     {
         pt->shadow.reserve((size_t)f.n_pix * SHADOW_F4 * 16);
         b.shadow = pt->shadow.as<float4>();
     }
+    // the training stream's view: its own queues and counters, its own half of the shadow records
+    pt_buffers bt = b;
+    if(f.cache_on)
+    {
+        const size_t n_train_max = div_up(f.n_pix, f.train_stride);
+        pt->train_queue0.reserve(n_train_max * 4);
+        pt->train_queue1.reserve(n_train_max * 4);
+        b.train_queue = pt->train_queue0.as<uint32_t>();
+        bt.queue[0] = pt->train_queue0.as<uint32_t>();
+        bt.queue[1] = pt->train_queue1.as<uint32_t>();
+        bt.counter = b.counter + TRAIN_COUNTERS;
+        if(f.n_lights)
+        {
+            pt->shadow.reserve(((size_t)f.n_pix + n_train_max) * SHADOW_F4 * 16);
+            b.shadow = pt->shadow.as<float4>();
+            bt.shadow = b.shadow + (size_t)f.n_pix * SHADOW_F4;
+        }
+    }
+    const vkrt_bvh_node *nodes2 = bvh->nodes.as<vkrt_bvh_node>();
+    const vkrt_bvh4_node *nodes4 = bvh->nodes4.as<vkrt_bvh4_node>();
+    const float4 *tris = bvh->tris.as<float4>();
+    // one depth of a queue of paths: trace, shade, and the shadow rays of next-event estimation
+    auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t grid, uint32_t depth, int in) {
+        k_pt_reset_depth<<<1, 1, 0, st>>>(bb.counter, in ^ 1);
+        k_pt_trace<<<grid, PT_BLOCK, 0, st>>>(nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide,
+                                              bb, in);
+        if(f.n_lights)
+        {
+            k_pt_shade<true><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+            k_pt_shadow<<<grid, PT_BLOCK, 0, st>>>(nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty,
+                                                   tris, sc.mats, f, bb);
+        }
+        else
+            k_pt_shade<false><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+    };
     VKRT_HIP(hipEventRecord(pt->ev0, s));
-    k_pt_zero<<<1, 4, 0, s>>>(b.counter);
+    k_pt_zero<<<1, 2 * TRAIN_COUNTERS, 0, s>>>(b.counter);
     for(uint32_t sample = 0; sample < f.num_samples; ++sample)
     {
+        uint32_t main_depths = f.max_depth;
         if(f.cache_on)
         {
             // training pixels move every sample: offset from the batch seed and the sample index
             f.train_offset = xxhash32(seed, sample) % f.train_stride;
             f.n_train_paths = f.train_offset < f.n_pix ? div_up(f.n_pix - f.train_offset, f.train_stride) : 0u;
+            // paths other than training paths end at the cache depth
+            main_depths = std::min(f.max_depth, f.cache_depth + 1u);
         }
         k_pt_begin<<<blocks, PT_BLOCK, 0, s>>>(f, b, sample);
-        int in = 0;
-        for(uint32_t depth = 0; depth < f.max_depth; ++depth)
+        const bool fork = f.cache_on && f.n_train_paths;
+        if(fork)
         {
-            // past the cache depth only training paths are alive: at most n_train_paths queue entries
-            const uint32_t grid = (f.cache_on && depth > f.cache_depth) ? div_up(std::max(1u, f.n_train_paths), PT_BLOCK)
-                                                                          : blocks;
-            k_pt_reset_depth<<<1, 1, 0, s>>>(b.counter, in ^ 1);
-            k_pt_trace<<<grid, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref,
-                                                 bvh->empty, bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide,
-                                                 b, in);
-            k_pt_shade<<<grid, PT_BLOCK, 0, s>>>(f, b, sc, depth, in);
-            if(f.n_lights)
-                k_pt_shadow<<<grid, PT_BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->root_ref,
-                                                      bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide,
-                                                      bvh->empty, bvh->tris.as<float4>(), sc.mats, f, b);
-            in ^= 1;
+            // the training paths' long, narrow tail is latency-bound: it runs on the second stream while the
+            // main stream traces the full-width depths and queries the cache
+            VKRT_HIP(hipEventRecord(pt->ev_fork, s));
+            VKRT_HIP(hipStreamWaitEvent(pt->train_stream, pt->ev_fork, 0));
         }
+        if(fork)
+        {
+            const uint32_t tgrid = div_up(f.n_train_paths, PT_BLOCK);
+            if(f.n_lights)
+                k_pt_train_paths<true><<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(
+                        nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, bt, sc);
+            else
+                k_pt_train_paths<false><<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(
+                        nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, bt, sc);
+            VKRT_HIP(hipGetLastError());
+            VKRT_HIP(hipEventRecord(pt->ev_join, pt->train_stream));
+        }
+        for(uint32_t depth = 0; depth < main_depths; ++depth) depth_step(s, b, blocks, depth, static_cast<int>(depth & 1u));
         if(f.cache_on)
         {
             rc::query(pt->rc, b.qrec, QUERY_F4, f.n_pix, nullptr, b.acc, b.counter + 4);
+            if(fork) VKRT_HIP(hipStreamWaitEvent(s, pt->ev_join, 0));
             if(f.n_train_paths && f.train_max)
             {
                 k_pt_train_targets<<<div_up(f.n_train_paths, PT_BLOCK), PT_BLOCK, 0, s>>>(f, b);
@@ -1151,14 +1277,15 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     k_pt_resolve<<<blocks, PT_BLOCK, 0, s>>>(f, b);
     VKRT_HIP(hipEventRecord(pt->ev1, s));
     VKRT_HIP(hipGetLastError());
-    uint64_t seg = 0;
-    VKRT_HIP(hipMemcpyAsync(&seg, b.counter + 2, 8, hipMemcpyDeviceToHost, s));
+    uint64_t seg[2] = {0, 0};
+    VKRT_HIP(hipMemcpyAsync(&seg[0], b.counter + 2, 8, hipMemcpyDeviceToHost, s));
+    VKRT_HIP(hipMemcpyAsync(&seg[1], b.counter + TRAIN_COUNTERS + 2, 8, hipMemcpyDeviceToHost, s));
     VKRT_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     VKRT_HIP(hipEventElapsedTime(&ms, pt->ev0, pt->ev1));
     pt->stats.raytrace_ms = ms;
     pt->stats.traced = 1;
-    pt->stats.segments = seg;
+    pt->stats.segments = seg[0] + seg[1];
     pt->stats.samples = (uint64_t)f.n_pix * f.num_samples;
 }
 
@@ -1186,6 +1313,9 @@ int vkrt_pt_create(vkrt_ctx *ctx, const vkrt_pt_settings *settings, uint32_t see
         pt->random_engine.seed(seed);
         VKRT_HIP(hipEventCreate(&pt->ev0));
         VKRT_HIP(hipEventCreate(&pt->ev1));
+        VKRT_HIP(hipEventCreateWithFlags(&pt->ev_fork, hipEventDisableTiming));
+        VKRT_HIP(hipEventCreateWithFlags(&pt->ev_join, hipEventDisableTiming));
+        VKRT_HIP(hipStreamCreateWithFlags(&pt->train_stream, hipStreamNonBlocking));
         *out = pt.release();
     });
 }
@@ -1195,9 +1325,13 @@ void vkrt_pt_destroy(vkrt_pt *pt)
     if(!pt) return;
     (void)hipSetDevice(pt->ctx->device);
     (void)hipStreamSynchronize(pt->ctx->stream);
+    if(pt->train_stream) (void)hipStreamSynchronize(pt->train_stream);
     if(pt->bvh) vkrt_bvh_destroy(pt->bvh);
     if(pt->ev0) (void)hipEventDestroy(pt->ev0);
     if(pt->ev1) (void)hipEventDestroy(pt->ev1);
+    if(pt->ev_fork) (void)hipEventDestroy(pt->ev_fork);
+    if(pt->ev_join) (void)hipEventDestroy(pt->ev_join);
+    if(pt->train_stream) (void)hipStreamDestroy(pt->train_stream);
     delete pt;
 }
 

@@ -11,14 +11,49 @@
 #pragma once
 
 #include <chrono>
+#include <cmath>
 #include <deque>
 #include <memory>
+#include <optional>
 #include <vector>
 
 #include "vierkant/RayBuilder.hpp"
 
 namespace vierkant
 {
+
+//! a sun folded into the light list as its first light (include/vierkant/PBRPathTracer.hpp:23-33)
+struct sunlight_params_t
+{
+    float color[3] = {0.f, 0.f, 0.f};
+    float intensity = 0.f;
+    //! elevation and azimuth in radians
+    float spherical_coords[2] = {0.52359877559829887f, 0.34906585039886591f};
+    //! apex half-angle of the sun disc in radians; 0 is a delta light
+    float angular_size = 0.f;
+};
+
+//! light_t, the packed light record (include/vierkant/punctual_light.hpp:64-89)
+using light_t = vkrt_light;
+
+//! the sun's light record as the reference builds it (src/PBRPathTracer.cpp:611-624): propagation direction
+//! quat(vec3(elevation, azimuth, 0)) * (0, 0, 1) = (cos e sin a, -sin e, cos e cos a), normalized
+inline light_t sun_light(const sunlight_params_t &p)
+{
+    light_t l{};
+    l.type = VKRT_LIGHT_DIRECTIONAL;
+    for(int k = 0; k < 3; ++k) l.color[k] = p.color[k];
+    l.intensity = p.intensity;
+    const float e = p.spherical_coords[0], a = p.spherical_coords[1];
+    float d[3] = {std::cos(e) * std::sin(a), -std::sin(e), std::cos(e) * std::cos(a)};
+    const float len2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if(len2 > 0.f)
+        for(float &c: d) c /= std::sqrt(len2);
+    for(int k = 0; k < 3; ++k) l.direction[k] = d[k];
+    l.range = INFINITY;
+    l.angular_size = p.angular_size;
+    return l;
+}
 
 class PBRPathTracer
 {
@@ -43,6 +78,8 @@ public:
         float exposure = 2.0f;
         bool depth_of_field = false;
         bool suppress_reset = false;
+        //! optional sun, folded in as the first light
+        std::optional<sunlight_params_t> sunlight_params;
         uint32_t timing_history_size = 300;
     };
 
@@ -74,11 +111,21 @@ public:
     //! upload a scene and build its acceleration structure
     void set_scene(const scene_t &scene) { m_ctx->check(vkrt_pt_set_scene(m_pt, &scene)); }
 
-    //! one frame: settings are copied, a batch is traced unless suspended
+    //! the scene's light sources (the reference gathers them from lightsource components each frame)
+    void set_lights(std::vector<light_t> lights) { m_lights = std::move(lights); }
+
+    //! one frame: settings are copied and the light list (sun first) is gathered, a batch is traced unless
+    //! suspended
     void render_scene(const camera_t &camera)
     {
         vkrt_pt_settings s = to_c(settings);
         m_ctx->check(vkrt_pt_set_settings(m_pt, &s));
+        std::vector<light_t> frame_lights;
+        frame_lights.reserve(m_lights.size() + 1);
+        if(settings.sunlight_params && settings.sunlight_params->intensity > 0.f)
+            frame_lights.push_back(sun_light(*settings.sunlight_params));
+        frame_lights.insert(frame_lights.end(), m_lights.begin(), m_lights.end());
+        m_ctx->check(vkrt_pt_set_lights(m_pt, frame_lights.data(), static_cast<uint32_t>(frame_lights.size())));
         m_ctx->check(vkrt_pt_render(m_pt, &camera));
         vkrt_pt_stats st{};
         vkrt_pt_get_stats(m_pt, &st);
@@ -144,6 +191,8 @@ private:
         c.suppress_reset = s.suppress_reset ? 1u : 0u;
         c.denoising = s.denoising ? 1u : 0u;
         c.tonemap = s.tonemap ? 1u : 0u;
+        c.mis_mode = s.mis_mode;
+        c.suppress_refractive_caustics = s.suppress_refractive_caustics ? 1u : 0u;
         c.gamma = s.gamma;
         c.exposure = s.exposure;
         return c;
@@ -151,6 +200,7 @@ private:
 
     gpu_context_ptr m_ctx;
     vkrt_pt *m_pt = nullptr;
+    std::vector<light_t> m_lights;
     std::deque<statistics_t> m_statistics;
 };
 
