@@ -302,6 +302,8 @@ def main():
     ap.add_argument("--verify-rays", type=int, default=65536)
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
     ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
+    ap.add_argument("--builder", choices=("lbvh", "ploc"), default="ploc", help="BVH builder of the headline scene")
+    ap.add_argument("--ploc-radius", type=int, default=0, help="PLOC search radius (0: the library default, 16)")
     ap.add_argument("--pt-lit-spp", type=int, default=16,
                     help="samples per pixel of the lit path-tracer frame (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
@@ -331,7 +333,8 @@ def main():
 
     log("building BVH")
     t0 = time.perf_counter()
-    bvh = ctx.bvh_build(pos, idx, max_leaf_size=MAX_LEAF)
+    builder = {"lbvh": vk.BUILDER_LBVH, "ploc": vk.BUILDER_PLOC}[args.builder]
+    bvh = ctx.bvh_build(pos, idx, builder=builder, max_leaf_size=MAX_LEAF, ploc_radius=args.ploc_radius)
     build_wall_ms = (time.perf_counter() - t0) * 1e3
     info = bvh.info()
 
@@ -450,7 +453,7 @@ def main():
                         "icospheres; 4,194,304 pinhole (2048x2048) + 4,194,304 cosine bounce rays per step",
             "n_tri": int(len(tris)),
             "rays_per_step": int(rays_per_step),
-            "bvh": {"builder": "lbvh", "max_leaf": MAX_LEAF, "nodes": int(info.n_nodes), "depth": int(info.depth),
+            "bvh": {"builder": args.builder, "ploc_radius": args.ploc_radius or 16, "max_leaf": MAX_LEAF, "nodes": int(info.n_nodes), "depth": int(info.depth),
                     "wide_nodes": int(info.n_wide_nodes), "wide_stack_bound": int(info.wide_stack),
                     "sah_cost": round(float(info.sah_cost), 2), "build_ms": round(float(info.build_ms), 2),
                     "build_wall_ms": round(build_wall_ms, 2)},

@@ -188,7 +188,9 @@ int vkrt_intersect_brute_any(vkrt_ctx *ctx, const float *tris, uint32_t n_tri, c
 enum
 {
     VKRT_BUILDER_LBVH = 0, /* Morton codes + radix sort + Karras hierarchy + bottom-up refit */
-    VKRT_BUILDER_PLOC = 1  /* LBVH order + parallel locally-ordered agglomerative clustering */
+    VKRT_BUILDER_PLOC = 1  /* the Morton order, then parallel locally-ordered clustering (Meister & Bittner
+                            * 2018, radius 16): about 20 % lower SAH cost, 10-35 % faster traversal, builds
+                            * about 25 % slower; leaves are reordered depth-first */
 };
 
 typedef struct vkrt_bvh_params
@@ -197,7 +199,8 @@ typedef struct vkrt_bvh_params
     uint32_t max_leaf_size; /* triangles per leaf, 1..8 (0 = default: 2, fastest with the 4-wide layout) */
     uint32_t width;         /* traversal layout: 0 = 4-wide collapse when its stack bound fits (default),
                              * 2 = the binary tree only. Results do not depend on it. */
-    uint32_t reserved[5];
+    uint32_t ploc_radius;   /* PLOC neighbour search radius in sorted positions, 1..32 (0 = default: 16) */
+    uint32_t reserved[4];
 } vkrt_bvh_params;
 
 typedef struct vkrt_bvh_info

@@ -36,7 +36,7 @@ class VkrtError(RuntimeError):
 
 class BvhParams(ctypes.Structure):
     _fields_ = [("builder", ctypes.c_uint32), ("max_leaf_size", ctypes.c_uint32), ("width", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 5)]
+                ("ploc_radius", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
 
 
 class TraceConfig(ctypes.Structure):
@@ -507,9 +507,11 @@ class Context:
         return out
 
     # -- BVH ------------------------------------------------------------------------------------
-    def bvh_build(self, positions, indices, builder=BUILDER_LBVH, max_leaf_size=0, stride=12, width=0):
-        """width: 0 = traverse the 4-wide collapse when its stack bound fits, 2 = the binary tree only"""
-        params = BvhParams(builder, max_leaf_size, width)
+    def bvh_build(self, positions, indices, builder=BUILDER_LBVH, max_leaf_size=0, stride=12, width=0,
+                  ploc_radius=0):
+        """width: 0 = traverse the 4-wide collapse when its stack bound fits, 2 = the binary tree only;
+        ploc_radius: PLOC's neighbour search radius (0 = 16)"""
+        params = BvhParams(builder, max_leaf_size, width, ploc_radius)
         n_vert = _n(positions) * 4 // stride
         n_tri = _n(indices, 3)
         h = _vp()

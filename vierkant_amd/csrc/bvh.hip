@@ -8,7 +8,8 @@
 //   k_gather    indexed mesh -> per-triangle (v0, e1, e2), exact AABB, centroid-bounds atomics
 //   k_morton    30-bit Morton code of the AABB center in the centroid bounds
 //   radix sort  (code, triangle) pairs, project-written (radix_sort.hip)
-//   k_karras    binary radix tree over the sorted codes (Karras 2012), ties broken by position
+//   k_karras    binary radix tree over the sorted codes (Karras 2012), ties broken by position; or, with
+//               VKRT_BUILDER_PLOC, agglomerative clustering over the sorted leaves (k_ploc_*)
 //   k_refit     bottom-up AABB union, one thread per leaf, second arrival at a node continues
 //   k_layout    BVH2 nodes holding both children's boxes (64 B), subtrees of <= max_leaf_size
 //               triangles collapsed into leaves over the contiguous sorted range
@@ -297,6 +298,7 @@ static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
         auto [ref, depth] = todo.back();
         todo.pop_back();
         max_depth = std::max(max_depth, depth);
+        if(depth > STACK_SIZE) throw error(VKRT_E_INTERNAL, "BVH deeper than the traversal stack");
         const vkrt_bvh_node &nd = nodes[ref];
         const int refs[2] = {nd.d.x, nd.d.y};
         const double ar[2] = {area(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y),
@@ -322,6 +324,8 @@ static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
     info.n_leaves = leaves;
     if(max_depth > STACK_SIZE) throw error(VKRT_E_INTERNAL, "BVH deeper than the traversal stack");
 }
+
+static void build_topology_ploc(vkrt_ctx *ctx, vkrt_bvh *bvh);
 
 static void build_topology(vkrt_ctx *ctx, vkrt_bvh *bvh, const uint8_t *pos, uint32_t stride, const uint32_t *idx)
 {
@@ -361,12 +365,384 @@ static void build_topology(vkrt_ctx *ctx, vkrt_bvh *bvh, const uint8_t *pos, uin
         bvh->nbox.reserve((size_t)ni * 6 * sizeof(float));
         bvh->flags.reserve(ni * sizeof(uint32_t));
         int32_t *parent_int = bvh->parent.as<int32_t>(), *parent_leaf = parent_int + ni;
-        k_karras<<<div_up(ni, 256), 256, 0, s>>>(keys, (int)n, bvh->child.as<int2>(), bvh->range.as<uint2>(),
-                                                 parent_int, parent_leaf);
+        if(bvh->builder == VKRT_BUILDER_PLOC) build_topology_ploc(ctx, bvh);
+        else
+            k_karras<<<div_up(ni, 256), 256, 0, s>>>(keys, (int)n, bvh->child.as<int2>(), bvh->range.as<uint2>(),
+                                                     parent_int, parent_leaf);
     }
     VKRT_HIP(hipStreamSynchronize(s));
     VKRT_HIP(hipGetLastError());
     if(err_host) throw error(VKRT_E_INVALID_ARG, "triangle index out of range of the vertex count");
+}
+
+// ---------------------------------------------------------------------------------------------
+// PLOC (Meister & Bittner 2018): agglomerative clustering over the Morton-sorted leaves. Each round every
+// cluster finds, within PLOC_RADIUS positions of the sorted list, the cluster whose merged box has the
+// smallest surface area; mutual nearest neighbours merge into a new node at the lower position, and the list
+// is compacted. Pairs are ordered by (area, lower position, higher position), so the globally smallest pair
+// is always mutual and every round merges. Internal nodes are numbered in merge order, then renumbered so the
+// root is node 0 and parents precede children; leaves are finally put in depth-first order so that every
+// subtree covers a contiguous range, which the leaf collapse of k_layout and the refit rely on.
+// ---------------------------------------------------------------------------------------------
+constexpr int PLOC_RADIUS = 16;    // default search radius
+constexpr int PLOC_MAX_RADIUS = 32;
+constexpr int PLOC_BLOCK = 256;
+constexpr uint32_t PLOC_MAX_DEPTH = 4096;// bound of the upward walks; deeper trees are rejected after layout
+
+__device__ __forceinline__ float merged_half_area(const float4 &alo, const float4 &ahi, const float4 &blo,
+                                                  const float4 &bhi)
+{
+    const float dx = fmaxf(ahi.x, bhi.x) - fminf(alo.x, blo.x);
+    const float dy = fmaxf(ahi.y, bhi.y) - fminf(alo.y, blo.y);
+    const float dz = fmaxf(ahi.z, bhi.z) - fminf(alo.z, blo.z);
+    return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void k_ploc_init(const float *__restrict__ tbox, uint32_t n, int32_t *__restrict__ cl_id,
+                            float4 *__restrict__ lo, float4 *__restrict__ hi, int32_t *__restrict__ parent_int,
+                            uint32_t ni)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j < ni) parent_int[j] = -1;
+    if(j >= n) return;
+    const float *b = tbox + 6 * (size_t)j;
+    cl_id[j] = ~static_cast<int32_t>(j);
+    lo[j] = make_float4(b[0], b[1], b[2], 0.f);
+    hi[j] = make_float4(b[3], b[4], b[5], 0.f);
+}
+
+//! nearest neighbour of every cluster within the window; ties go to the smallest position, which is the
+//! smallest pair in (area, lower, higher) order
+__global__ void __launch_bounds__(PLOC_BLOCK) k_ploc_nn(const float4 *__restrict__ lo, const float4 *__restrict__ hi,
+                                                        uint32_t nc, int radius, int32_t *__restrict__ nn)
+{
+    __shared__ float4 slo[PLOC_BLOCK + 2 * PLOC_MAX_RADIUS], shi[PLOC_BLOCK + 2 * PLOC_MAX_RADIUS];
+    const int base = static_cast<int>(blockIdx.x * PLOC_BLOCK) - radius;
+    for(int k = threadIdx.x; k < PLOC_BLOCK + 2 * radius; k += PLOC_BLOCK)
+    {
+        const int g = base + k;
+        if(g >= 0 && g < static_cast<int>(nc))
+        {
+            slo[k] = lo[g];
+            shi[k] = hi[g];
+        }
+    }
+    __syncthreads();
+    const int i = static_cast<int>(blockIdx.x * PLOC_BLOCK + threadIdx.x);
+    if(i >= static_cast<int>(nc)) return;
+    const int li = threadIdx.x + radius;
+    float best = 0.f;
+    int bj = -1;
+    for(int o = -radius; o <= radius; ++o)
+    {
+        const int j = i + o;
+        if(o == 0 || j < 0 || j >= static_cast<int>(nc)) continue;
+        const float d = merged_half_area(slo[li], shi[li], slo[li + o], shi[li + o]);
+        if(bj < 0 || d < best)
+        {
+            best = d;
+            bj = j;
+        }
+    }
+    nn[i] = bj;
+}
+
+__global__ void k_ploc_flags(const int32_t *__restrict__ nn, uint32_t nc, uint32_t *__restrict__ merge,
+                             uint32_t *__restrict__ keep)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if(i >= static_cast<int>(nc)) return;
+    const int j = nn[i];
+    const bool mutual = nn[j] == i;
+    merge[i] = (mutual && i < j) ? 1u : 0u;
+    keep[i] = (mutual && i > j) ? 0u : 1u;
+}
+
+__device__ __forceinline__ uint32_t leaf_count(const uint32_t *__restrict__ cnt, int32_t ref)
+{
+    return ref < 0 ? 1u : cnt[ref];
+}
+
+__global__ void k_ploc_merge(const int32_t *__restrict__ nn, uint32_t nc, const uint32_t *__restrict__ merge,
+                             const uint32_t *__restrict__ moff, const uint32_t *__restrict__ keep,
+                             const uint32_t *__restrict__ koff, uint32_t base, const int32_t *__restrict__ cl_id,
+                             const float4 *__restrict__ lo, const float4 *__restrict__ hi,
+                             int32_t *__restrict__ out_id, float4 *__restrict__ out_lo, float4 *__restrict__ out_hi,
+                             int2 *__restrict__ child, uint32_t *__restrict__ cnt, int32_t *__restrict__ parent_int,
+                             int32_t *__restrict__ parent_leaf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= nc || !keep[i]) return;
+    const uint32_t pos = koff[i];
+    if(!merge[i])
+    {
+        out_id[pos] = cl_id[i];
+        out_lo[pos] = lo[i];
+        out_hi[pos] = hi[i];
+        return;
+    }
+    const uint32_t j = static_cast<uint32_t>(nn[i]);
+    const int32_t id = static_cast<int32_t>(base + moff[i]);
+    const int32_t a = cl_id[i], b = cl_id[j];
+    child[id] = make_int2(a, b);
+    cnt[id] = leaf_count(cnt, a) + leaf_count(cnt, b);
+    if(a < 0) parent_leaf[~a] = id;
+    else parent_int[a] = id;
+    if(b < 0) parent_leaf[~b] = id;
+    else parent_int[b] = id;
+    const float4 la = lo[i], lb = lo[j], ha = hi[i], hb = hi[j];
+    out_id[pos] = id;
+    out_lo[pos] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.f);
+    out_hi[pos] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.f);
+}
+
+//! internal node k becomes ni - 1 - k: the last merge (the root) is node 0 and parents precede children
+__global__ void k_ploc_renumber(uint32_t ni, uint32_t n, const int2 *__restrict__ child, const uint32_t *__restrict__ cnt,
+                                const int32_t *__restrict__ parent_int, const int32_t *__restrict__ parent_leaf,
+                                int2 *__restrict__ child2, uint32_t *__restrict__ cnt2, int32_t *__restrict__ parent_int2,
+                                int32_t *__restrict__ parent_leaf2)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    auto map = [ni](int32_t r) { return r < 0 ? r : static_cast<int32_t>(ni - 1u) - r; };
+    if(k < ni)
+    {
+        const uint32_t nk = ni - 1u - k;
+        const int2 c = child[k];
+        child2[nk] = make_int2(map(c.x), map(c.y));
+        cnt2[nk] = cnt[k];
+        parent_int2[nk] = parent_int[k] < 0 ? -1 : map(parent_int[k]);
+    }
+    if(k < n) parent_leaf2[k] = map(parent_leaf[k]);
+}
+
+//! depth-first position of every leaf and the first position of every internal node's range: the leaf
+//! counts of all left siblings on the way up
+__global__ void k_ploc_dfs(uint32_t ni, uint32_t n, const int2 *__restrict__ child, const uint32_t *__restrict__ cnt,
+                           const int32_t *__restrict__ parent_int, const int32_t *__restrict__ parent_leaf,
+                           uint32_t *__restrict__ leaf_pos, uint2 *__restrict__ range)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < n)
+    {
+        uint32_t off = 0;
+        int32_t cur = ~static_cast<int32_t>(k), node = parent_leaf[k];
+        for(uint32_t step = 0; node >= 0 && step < PLOC_MAX_DEPTH; ++step)
+        {
+            const int2 c = child[node];
+            if(c.y == cur) off += leaf_count(cnt, c.x);
+            cur = node;
+            node = parent_int[node];
+        }
+        leaf_pos[k] = off;
+    }
+    if(k < ni)
+    {
+        uint32_t off = 0;
+        int32_t cur = static_cast<int32_t>(k), node = parent_int[k];
+        for(uint32_t step = 0; node >= 0 && step < PLOC_MAX_DEPTH; ++step)
+        {
+            const int2 c = child[node];
+            if(c.y == cur) off += leaf_count(cnt, c.x);
+            cur = node;
+            node = parent_int[node];
+        }
+        range[k] = make_uint2(off, off + cnt[k] - 1u);
+    }
+}
+
+//! leaves to depth-first order: the triangle order, sorted boxes, leaf parents and leaf child refs
+__global__ void k_ploc_permute(uint32_t ni, uint32_t n, const uint32_t *__restrict__ leaf_pos,
+                               const uint32_t *__restrict__ order_in, const float *__restrict__ tbox_in,
+                               const int32_t *__restrict__ parent_leaf_in, uint32_t *__restrict__ order_out,
+                               float *__restrict__ tbox_out, int32_t *__restrict__ parent_leaf_out,
+                               const int2 *__restrict__ child_in, int2 *__restrict__ child_out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if(k < n)
+    {
+        const uint32_t d = leaf_pos[k];
+        if(d >= n) return;// only a malformed tree gets here; compute_stats rejects it
+        order_out[d] = order_in[k];
+        for(int a = 0; a < 6; ++a) tbox_out[6 * (size_t)d + a] = tbox_in[6 * (size_t)k + a];
+        parent_leaf_out[d] = parent_leaf_in[k];
+    }
+    if(k < ni)
+    {
+        const int2 c = child_in[k];
+        auto map = [leaf_pos](int32_t r) { return r < 0 ? ~static_cast<int32_t>(leaf_pos[~r]) : r; };
+        child_out[k] = make_int2(map(c.x), map(c.y));
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_local(const uint32_t *__restrict__ in, uint32_t n,
+                                                     uint32_t *__restrict__ out, uint32_t *__restrict__ block_sums)
+{
+    __shared__ uint32_t wsum[16];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    const uint32_t x = i < n ? in[i] : 0u;
+    uint32_t sc = x;
+    for(int off = 1; off < 64; off <<= 1)
+    {
+        const uint32_t y = __shfl_up(sc, off);
+        if(lane >= (uint32_t)off) sc += y;
+    }
+    if(lane == 63) wsum[wave] = sc;
+    __syncthreads();
+    if(threadIdx.x < 16)
+    {
+        const uint32_t w = wsum[threadIdx.x];
+        uint32_t acc = w;
+        for(int off = 1; off < 16; off <<= 1)
+        {
+            const uint32_t y = __shfl_up(acc, off, 16);
+            if(threadIdx.x >= (uint32_t)off) acc += y;
+        }
+        wsum[threadIdx.x] = acc - w;
+        if(threadIdx.x == 15) block_sums[blockIdx.x] = acc;
+    }
+    __syncthreads();
+    if(i < n) out[i] = wsum[wave] + sc - x;
+}
+
+//! exclusive scan of the per-block sums in one workgroup, 1024 at a time with a running carry
+__global__ void __launch_bounds__(1024) k_scan_blocks(uint32_t *__restrict__ data, uint32_t n)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry;
+    if(threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for(uint32_t base = 0; base < n; base += 1024)
+    {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t x = i < n ? data[i] : 0u;
+        uint32_t sc = x;
+        for(int off = 1; off < 64; off <<= 1)
+        {
+            const uint32_t y = __shfl_up(sc, off);
+            if(lane >= (uint32_t)off) sc += y;
+        }
+        if(lane == 63) wsum[wave] = sc;
+        __syncthreads();
+        if(threadIdx.x < 16)
+        {
+            const uint32_t w = wsum[threadIdx.x];
+            uint32_t acc = w;
+            for(int off = 1; off < 16; off <<= 1)
+            {
+                const uint32_t y = __shfl_up(acc, off, 16);
+                if(threadIdx.x >= (uint32_t)off) acc += y;
+            }
+            wsum[threadIdx.x] = acc - w;
+        }
+        __syncthreads();
+        const uint32_t c = carry;
+        if(i < n) data[i] = c + wsum[wave] + sc - x;
+        __syncthreads();
+        if(threadIdx.x == 1023) carry = c + wsum[wave] + sc;
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_add(const uint32_t *__restrict__ in, uint32_t n,
+                                                   uint32_t *__restrict__ out, const uint32_t *__restrict__ block_off,
+                                                   uint32_t *__restrict__ total)
+{
+    const uint32_t i = blockIdx.x * 1024 + threadIdx.x;
+    if(i >= n) return;
+    const uint32_t v = out[i] + block_off[blockIdx.x];
+    out[i] = v;
+    if(i == n - 1) *total = v + in[i];
+}
+
+//! exclusive prefix sum of n values; *total receives the sum. block_sums holds div_up(n, 1024) entries
+static void exclusive_scan(hipStream_t s, const uint32_t *in, uint32_t *out, uint32_t n, uint32_t *block_sums,
+                           uint32_t *total)
+{
+    const uint32_t nb = div_up(n, 1024);
+    k_scan_local<<<nb, 1024, 0, s>>>(in, n, out, block_sums);
+    k_scan_blocks<<<1, 1024, 0, s>>>(block_sums, nb);
+    k_scan_add<<<nb, 1024, 0, s>>>(in, n, out, block_sums, total);
+}
+
+static void build_topology_ploc(vkrt_ctx *ctx, vkrt_bvh *bvh)
+{
+    hipStream_t s = ctx->stream;
+    const uint32_t radius = bvh->ploc_radius ? bvh->ploc_radius : PLOC_RADIUS;
+    const uint32_t n = bvh->n_tri, ni = n - 1;
+    // working set: two cluster lists (ids, lo, hi), neighbours, flags and offsets, leaf counts, scan sums
+    device_buffer &w = ctx->scratch[3];
+    const size_t nb = div_up(n, 1024);
+    const size_t bytes = (size_t)n * (2 * 4 + 4 * 16 + 4 + 4 * 4 + 4) + (size_t)ni * 4 + nb * 4 + 512;
+    w.reserve(bytes);
+    uint8_t *q = w.as<uint8_t>();
+    auto take = [&q](size_t b) {
+        uint8_t *r = q;
+        q += (b + 15) & ~size_t(15);
+        return r;
+    };
+    int32_t *id[2] = {reinterpret_cast<int32_t *>(take((size_t)n * 4)), reinterpret_cast<int32_t *>(take((size_t)n * 4))};
+    float4 *lo[2] = {reinterpret_cast<float4 *>(take((size_t)n * 16)), reinterpret_cast<float4 *>(take((size_t)n * 16))};
+    float4 *hi[2] = {reinterpret_cast<float4 *>(take((size_t)n * 16)), reinterpret_cast<float4 *>(take((size_t)n * 16))};
+    int32_t *nn = reinterpret_cast<int32_t *>(take((size_t)n * 4));
+    uint32_t *merge = reinterpret_cast<uint32_t *>(take((size_t)n * 4));
+    uint32_t *moff = reinterpret_cast<uint32_t *>(take((size_t)n * 4));
+    uint32_t *keep = reinterpret_cast<uint32_t *>(take((size_t)n * 4));
+    uint32_t *koff = reinterpret_cast<uint32_t *>(take((size_t)n * 4));
+    uint32_t *leaf_pos = reinterpret_cast<uint32_t *>(take((size_t)n * 4));
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(take((size_t)ni * 4));
+    uint32_t *sums = reinterpret_cast<uint32_t *>(take(nb * 4));
+    uint32_t *totals = reinterpret_cast<uint32_t *>(take(16));
+    VKRT_REQUIRE(q <= w.as<uint8_t>() + bytes, "PLOC working set overrun");
+
+    int2 *child = bvh->child.as<int2>();
+    int32_t *parent_int = bvh->parent.as<int32_t>(), *parent_leaf = parent_int + ni;
+    k_ploc_init<<<div_up(n, 256), 256, 0, s>>>(bvh->tbox.as<float>(), n, id[0], lo[0], hi[0], parent_int, ni);
+    uint32_t nc = n, base = 0;
+    int cur = 0;
+    uint32_t rounds = 0;
+    while(nc > 1)
+    {
+        VKRT_REQUIRE(++rounds <= n, "PLOC made no progress");
+        k_ploc_nn<<<div_up(nc, PLOC_BLOCK), PLOC_BLOCK, 0, s>>>(lo[cur], hi[cur], nc, static_cast<int>(radius), nn);
+        k_ploc_flags<<<div_up(nc, 256), 256, 0, s>>>(nn, nc, merge, keep);
+        exclusive_scan(s, merge, moff, nc, sums, totals);
+        exclusive_scan(s, keep, koff, nc, sums, totals + 1);
+        k_ploc_merge<<<div_up(nc, 256), 256, 0, s>>>(nn, nc, merge, moff, keep, koff, base, id[cur], lo[cur], hi[cur],
+                                                     id[cur ^ 1], lo[cur ^ 1], hi[cur ^ 1], child, cnt, parent_int,
+                                                     parent_leaf);
+        uint32_t t[2];
+        VKRT_HIP(hipMemcpyAsync(t, totals, 8, hipMemcpyDeviceToHost, s));
+        VKRT_HIP(hipStreamSynchronize(s));
+        VKRT_REQUIRE(t[0] >= 1 && t[1] == nc - t[0], "PLOC round inconsistent");
+        base += t[0];
+        nc = t[1];
+        cur ^= 1;
+    }
+    VKRT_REQUIRE(base == ni, "PLOC did not build n - 1 nodes");
+    // the cluster lists and flag arrays are free now: renumbered tree (root first), then depth-first leaf
+    // positions and ranges, then the permuted copies, which are copied back into the BVH's own arrays.
+    // lo[1] and hi[0] are adjacent (32 n bytes), enough for the 24 n bytes of permuted boxes
+    int2 *child2 = reinterpret_cast<int2 *>(lo[0]);
+    uint32_t *cnt2 = merge;
+    int32_t *pint2 = reinterpret_cast<int32_t *>(moff);
+    int32_t *pleaf2 = reinterpret_cast<int32_t *>(keep);
+    k_ploc_renumber<<<div_up(n, 256), 256, 0, s>>>(ni, n, child, cnt, parent_int, parent_leaf, child2, cnt2, pint2,
+                                                   pleaf2);
+    k_ploc_dfs<<<div_up(n, 256), 256, 0, s>>>(ni, n, child2, cnt2, pint2, pleaf2, leaf_pos, bvh->range.as<uint2>());
+    uint32_t *order_out = koff;
+    float *tbox_out = reinterpret_cast<float *>(lo[1]);
+    int32_t *pleaf_out = id[0];
+    int2 *child_out = reinterpret_cast<int2 *>(hi[1]);
+    k_ploc_permute<<<div_up(n, 256), 256, 0, s>>>(ni, n, leaf_pos, bvh->order.as<uint32_t>(), bvh->tbox.as<float>(),
+                                                  pleaf2, order_out, tbox_out, pleaf_out, child2, child_out);
+    VKRT_HIP(hipMemcpyAsync(bvh->order.ptr, order_out, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    VKRT_HIP(hipMemcpyAsync(bvh->tbox.ptr, tbox_out, (size_t)n * 24, hipMemcpyDeviceToDevice, s));
+    VKRT_HIP(hipMemcpyAsync(parent_leaf, pleaf_out, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    VKRT_HIP(hipMemcpyAsync(parent_int, pint2, (size_t)ni * 4, hipMemcpyDeviceToDevice, s));
+    VKRT_HIP(hipMemcpyAsync(child, child_out, (size_t)ni * 8, hipMemcpyDeviceToDevice, s));
+    VKRT_HIP(hipGetLastError());
 }
 
 static void build_bounds(vkrt_ctx *ctx, vkrt_bvh *bvh)
@@ -578,7 +954,8 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
         VKRT_REQUIRE(n_tri < (1u << 28), "at most 2^28 - 1 triangles per BVH");
         uint32_t leaf = params && params->max_leaf_size ? params->max_leaf_size : 2;
         VKRT_REQUIRE(leaf >= 1 && leaf <= (uint32_t)MAX_LEAF, "max_leaf_size must be in 1..8");
-        VKRT_REQUIRE(!params || params->builder == VKRT_BUILDER_LBVH, "builder not supported");
+        VKRT_REQUIRE(!params || params->builder == VKRT_BUILDER_LBVH || params->builder == VKRT_BUILDER_PLOC,
+                     "builder must be VKRT_BUILDER_LBVH or VKRT_BUILDER_PLOC");
         VKRT_REQUIRE(!params || params->width == 0 || params->width == 2 || params->width == 4,
                      "width must be 0 (auto), 2 or 4");
         auto t_start = std::chrono::steady_clock::now();
@@ -589,6 +966,9 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
         bvh->max_leaf = leaf;
         bvh->empty = n_tri == 0;
         bvh->allow_wide = !params || params->width != 2;
+        bvh->builder = params ? params->builder : VKRT_BUILDER_LBVH;
+        VKRT_REQUIRE(!params || params->ploc_radius <= (uint32_t)PLOC_MAX_RADIUS, "ploc_radius must be in 0..32");
+        bvh->ploc_radius = params ? params->ploc_radius : 0u;
         if(n_tri)
         {
             const size_t pos_bytes = (size_t)(n_vert - 1) * stride_bytes + 12;

@@ -29,6 +29,8 @@ struct vkrt_bvh
 {
     int device = 0;
     uint32_t n_tri = 0, n_vert = 0, n_nodes = 0, max_leaf = 4;
+    uint32_t builder = 0;// VKRT_BUILDER_*
+    uint32_t ploc_radius = 0;// 0: default
     int32_t root_ref = 0;
     bool empty = true;
     vkrt::device_buffer nodes, tris;

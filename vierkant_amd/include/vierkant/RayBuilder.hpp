@@ -59,7 +59,7 @@ public:
 
     struct build_params_t
     {
-        uint32_t builder = VKRT_BUILDER_LBVH;
+        uint32_t builder = VKRT_BUILDER_PLOC;
         uint32_t max_leaf_size = 2;
         uint32_t width = 0;// 0: 4-wide traversal layout when it fits, 2: binary tree only
     };
