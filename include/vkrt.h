@@ -438,6 +438,39 @@ typedef struct vkrt_pt_scene
     float environment; /* scale of the procedural sky (environment_white) */
 } vkrt_pt_scene;
 
+/* Scene files into a flat scene for the tracer: Wavefront OBJ + MTL and glTF 2.0 (.gltf with external or
+ * base64 buffers, .glb), the formats of vierkant::model::wavefront_obj and vierkant::model::gltf
+ * (src/model/wavefront_obj.cpp:11-137, src/model/gltf.cpp:136-660, :933-1218). OBJ: every face corner is a
+ * vertex, polygons are fanned, missing normals are computed as Geometry::compute_vertex_normals, one object
+ * per shape with the material of its first face. glTF: one object per triangle primitive of every mesh node
+ * reachable from the scene, node transforms baked into world-space vertices (double precision, rounded
+ * once), material factors of the core spec and of KHR_materials_emissive_strength / transmission / ior /
+ * clearcoat / sheen / diffuse_transmission. Textures are counted but not used. Host code only: no context
+ * or GPU is needed. Errors: VKRT_E_INVALID_ARG with the message in vkrt_model_error (per thread). */
+typedef struct vkrt_model vkrt_model;
+typedef struct vkrt_model_view
+{
+    const float *positions; /* 3 per vertex, world space */
+    const float *normals;   /* 3 per vertex */
+    const float *texcoords; /* 2 per vertex (0 where the file has none) */
+    uint32_t n_vert;
+    const uint32_t *indices; /* 3 per triangle */
+    uint32_t n_tri;
+    const uint32_t *tri_object;
+    const uint32_t *object_material;
+    uint32_t n_objects;
+    const vkrt_material *materials;
+    uint32_t n_materials;
+    uint32_t n_textures;     /* images the file references */
+} vkrt_model_view;
+int vkrt_model_load(const char *path, vkrt_model **out);
+const char *vkrt_model_error(void);
+/* Views stay valid until vkrt_model_destroy. */
+int vkrt_model_get(const vkrt_model *model, vkrt_model_view *out);
+/* The view as the tracer's scene (vkrt_pt_set_scene), with the given sky scale. */
+int vkrt_model_pt_scene(const vkrt_model *model, float environment, vkrt_pt_scene *out);
+void vkrt_model_destroy(vkrt_model *model);
+
 /* Pinhole / thin-lens camera (camera_params_t): eye, unit forward and up vectors, vertical field of view
  * in radians, aperture diameter (0 = pinhole) and focal distance. */
 typedef struct vkrt_camera

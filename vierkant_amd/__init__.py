@@ -55,6 +55,13 @@ class PtSettings(ctypes.Structure):
                 ("suppress_refractive_caustics", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 1)]
 
 
+class ModelView(ctypes.Structure):
+    _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("texcoords", ctypes.c_void_p),
+                ("n_vert", ctypes.c_uint32), ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32),
+                ("tri_object", ctypes.c_void_p), ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
+                ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("n_textures", ctypes.c_uint32)]
+
+
 class PtScene(ctypes.Structure):
     _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("n_vert", ctypes.c_uint32),
                 ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32), ("tri_object", ctypes.c_void_p),
@@ -197,6 +204,11 @@ _SIGNATURES = {
     "vkrt_pt_get_settings": (_i32, [_vp, ctypes.POINTER(PtSettings)]),
     "vkrt_pt_set_scene": (_i32, [_vp, ctypes.POINTER(PtScene)]),
     "vkrt_pt_set_lights": (_i32, [_vp, _vp, _u32]),
+    "vkrt_model_load": (_i32, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "vkrt_model_error": (ctypes.c_char_p, []),
+    "vkrt_model_get": (_i32, [_vp, ctypes.POINTER(ModelView)]),
+    "vkrt_model_pt_scene": (_i32, [_vp, ctypes.c_float, ctypes.POINTER(PtScene)]),
+    "vkrt_model_destroy": (None, [_vp]),
     "vkrt_pt_render": (_i32, [_vp, ctypes.POINTER(Camera)]),
     "vkrt_pt_reset_accumulator": (_i32, [_vp]),
     "vkrt_pt_current_batch": (_u32, [_vp]),
@@ -882,6 +894,40 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+# -- scene files (host) -------------------------------------------------------------------------
+def load_model(path, environment=1.0):
+    """an OBJ (+ MTL) or glTF 2.0 (.gltf / .glb) file as a scene dict for PathTracer.set_scene (copies of the
+    library's arrays: positions, normals, texcoords, indices, tri_object, object_material, materials,
+    environment, n_textures); raises VkrtError with the loader's message"""
+    h = _vp()
+    rc = lib().vkrt_model_load(str(path).encode(), ctypes.byref(h))
+    if rc != OK:
+        raise VkrtError(rc, lib().vkrt_model_error().decode())
+    try:
+        v = ModelView()
+        if lib().vkrt_model_get(h, ctypes.byref(v)) != OK:
+            raise VkrtError(E_INVALID_ARG, "vkrt_model_get failed")
+
+        def arr(ptr, n, dtype):
+            if n == 0 or not ptr:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                         (n,)).copy()
+
+        mats = np.zeros(v.n_materials, MATERIAL_DTYPE)
+        if v.n_materials:
+            ctypes.memmove(mats.ctypes.data, v.materials, v.n_materials * MATERIAL_DTYPE.itemsize)
+        return {"positions": arr(v.positions, 3 * v.n_vert, np.float32).reshape(-1, 3),
+                "normals": arr(v.normals, 3 * v.n_vert, np.float32).reshape(-1, 3),
+                "texcoords": arr(v.texcoords, 2 * v.n_vert, np.float32).reshape(-1, 2),
+                "indices": arr(v.indices, 3 * v.n_tri, np.uint32),
+                "tri_object": arr(v.tri_object, v.n_tri, np.uint32),
+                "object_material": arr(v.object_material, v.n_objects, np.uint32),
+                "materials": mats, "environment": float(environment), "n_textures": int(v.n_textures)}
+    finally:
+        lib().vkrt_model_destroy(h)
 
 
 # -- procedural geometry (host) ----------------------------------------------------------------
