@@ -418,8 +418,29 @@ typedef struct vkrt_material
     float sheen_color[4];
     float diffuse_transmission_color[4];
     uint32_t two_sided; /* its triangles are never culled as back faces (VK_GEOMETRY_INSTANCE_TRIANGLE_FACING_CULL_DISABLE) */
-    uint32_t reserved[3];
+    uint32_t color_texture; /* 1 + index into the scene's textures (0: none): color and alpha are multiplied by
+                             * the texture, sampled trilinearly with repeat addressing at the ray-cone level of
+                             * detail (raypipeline.slang:300-321, :526-531) */
+    uint32_t alpha_mode;    /* VKRT_ALPHA_*: with MASK every ray skips the surface where alpha < alpha_cutoff */
+    float alpha_cutoff;
 } vkrt_material;
+
+/* BlendMode (include/vierkant/Material.hpp:129): MASK is the any-hit cutout of raypipeline.slang:686-720 and
+ * visibility_test.slang:145-150; BLEND is traced as OPAQUE. */
+enum
+{
+    VKRT_ALPHA_OPAQUE = 0,
+    VKRT_ALPHA_MASK = 1,
+    VKRT_ALPHA_BLEND = 2
+};
+
+/* An RGBA8 UNORM image, rows from the top; the tracer builds its mip chain (2x2 box filter, as the
+ * reference's linear blits of a level of even size). */
+typedef struct vkrt_texture
+{
+    uint32_t width, height;
+    const uint8_t *rgba;
+} vkrt_texture;
 
 /* A flat scene: an indexed triangle mesh with shading normals, split into objects (entities) that each
  * carry one material. All pointers are host pointers; the tracer keeps device copies. */
@@ -436,6 +457,9 @@ typedef struct vkrt_pt_scene
     const vkrt_material *materials;
     uint32_t n_materials;
     float environment; /* scale of the procedural sky (environment_white) */
+    const float *texcoords;        /* 2 per vertex, or null (all 0) */
+    const vkrt_texture *textures;  /* referenced by vkrt_material::color_texture */
+    uint32_t n_textures;
 } vkrt_pt_scene;
 
 /* Scene files into a flat scene for the tracer: Wavefront OBJ + MTL and glTF 2.0 (.gltf with external or
@@ -445,7 +469,8 @@ typedef struct vkrt_pt_scene
  * per shape with the material of its first face. glTF: one object per triangle primitive of every mesh node
  * reachable from the scene, node transforms baked into world-space vertices (double precision, rounded
  * once), material factors of the core spec and of KHR_materials_emissive_strength / transmission / ior /
- * clearcoat / sheen / diffuse_transmission. Textures are counted but not used. Host code only: no context
+ * clearcoat / sheen / diffuse_transmission, base color textures (PNG) with alphaMode / alphaCutoff; OBJ map_Kd
+ * textures are flipped vertically as the reference does (wavefront_obj.cpp:103-109). Host code only: no context
  * or GPU is needed. Errors: VKRT_E_INVALID_ARG with the message in vkrt_model_error (per thread). */
 typedef struct vkrt_model vkrt_model;
 typedef struct vkrt_model_view
@@ -461,7 +486,9 @@ typedef struct vkrt_model_view
     uint32_t n_objects;
     const vkrt_material *materials;
     uint32_t n_materials;
-    uint32_t n_textures;     /* images the file references */
+    const vkrt_texture *textures; /* decoded PNG images (8 or 16 bit, any color type, not interlaced) */
+    uint32_t n_textures;
+    uint32_t n_skipped_textures;  /* images that are not PNG or failed to decode; their materials are untextured */
 } vkrt_model_view;
 int vkrt_model_load(const char *path, vkrt_model **out);
 const char *vkrt_model_error(void);

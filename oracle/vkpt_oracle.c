@@ -163,6 +163,10 @@ typedef struct
     float metal, rough, trans, ior, cc, cc_rough, sheen_rough, dt;
     v3 sheen, dt_color;
     int two_sided;
+    float alpha;        /* color.a */
+    uint32_t tex;       /* base color texture + 1, 0 = none */
+    int mask;           /* alpha mode MASK: the any-hit cutout test */
+    float alpha_cutoff;
 } mat_t;
 
 typedef struct
@@ -358,6 +362,136 @@ static float round_half(float x)
     return out;
 }
 
+/* ---- textures ---------------------------------------------------------------------------------
+ * RGBA8 UNORM images with a full mip chain, sampled as the path tracer's Sampler2D (raypipeline.slang:
+ * 312-321 SampleLevel; sampler defaults of Material.hpp:175-179: linear filtering, linear mip blending,
+ * repeat addressing). Each level halves the previous one (at least one texel) with a 2 x 2 box filter whose
+ * coordinates clamp at an odd edge, rounded to nearest: (a + b + c + d + 2) / 4. */
+#define ORC_MAX_LEVELS 16
+typedef struct
+{
+    uint32_t w, h, levels;
+    uint8_t *lv[ORC_MAX_LEVELS];
+} otex_t;
+
+static void otex_build(otex_t *t, const orc_texture *src)
+{
+    memset(t, 0, sizeof(*t));
+    t->w = src->width;
+    t->h = src->height;
+    size_t bytes = (size_t)t->w * t->h * 4;
+    t->lv[0] = (uint8_t *)malloc(bytes ? bytes : 4);
+    memcpy(t->lv[0], src->rgba, bytes);
+    t->levels = 1;
+    uint32_t w = t->w, h = t->h;
+    while((w > 1 || h > 1) && t->levels < ORC_MAX_LEVELS)
+    {
+        const uint32_t nw = w > 1 ? w / 2 : 1, nh = h > 1 ? h / 2 : 1;
+        const uint8_t *p = t->lv[t->levels - 1];
+        uint8_t *q = (uint8_t *)malloc((size_t)nw * nh * 4);
+        for(uint32_t y = 0; y < nh; ++y)
+            for(uint32_t x = 0; x < nw; ++x)
+            {
+                const uint32_t xs[2] = {2 * x < w - 1 ? 2 * x : w - 1, 2 * x + 1 < w - 1 ? 2 * x + 1 : w - 1};
+                const uint32_t ys[2] = {2 * y < h - 1 ? 2 * y : h - 1, 2 * y + 1 < h - 1 ? 2 * y + 1 : h - 1};
+                for(int c = 0; c < 4; ++c)
+                {
+                    uint32_t sum = 2;
+                    for(int a = 0; a < 2; ++a)
+                        for(int b = 0; b < 2; ++b) sum += p[4 * ((size_t)ys[a] * w + xs[b]) + c];
+                    q[4 * ((size_t)y * nw + x) + c] = (uint8_t)(sum / 4);
+                }
+            }
+        t->lv[t->levels++] = q;
+        w = nw;
+        h = nh;
+    }
+}
+
+static void otex_free(otex_t *t)
+{
+    for(uint32_t l = 0; l < t->levels; ++l) free(t->lv[l]);
+}
+
+/* bilinear filtering of one level with repeat addressing; texel centres at (i + 0.5) / size */
+static void otex_level(const otex_t *t, uint32_t l, float u, float v, float out[4])
+{
+    const int w = (int)(t->w >> l ? t->w >> l : 1), h = (int)(t->h >> l ? t->h >> l : 1);
+    const float fx = u * (float)w - 0.5f, fy = v * (float)h - 0.5f;
+    const float x0f = floorf(fx), y0f = floorf(fy);
+    const float ax = fx - x0f, ay = fy - y0f;
+    const long long xl = (long long)x0f, yl = (long long)y0f;
+    const int x0 = (int)(((xl % w) + w) % w), y0 = (int)(((yl % h) + h) % h);
+    const int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
+    const uint8_t *p = t->lv[l];
+    for(int c = 0; c < 4; ++c)
+    {
+        const float c00 = (float)p[4 * ((size_t)y0 * w + x0) + c] / 255.0f;
+        const float c10 = (float)p[4 * ((size_t)y0 * w + x1) + c] / 255.0f;
+        const float c01 = (float)p[4 * ((size_t)y1 * w + x0) + c] / 255.0f;
+        const float c11 = (float)p[4 * ((size_t)y1 * w + x1) + c] / 255.0f;
+        out[c] = lerpf(lerpf(c00, c10, ax), lerpf(c01, c11, ax), ay);
+    }
+}
+
+/* SampleLevel: lambda clamped to [0, levels - 1] (NaN and -inf give level 0), linear between the two
+ * levels around it; coordinates beyond 1e9 in magnitude carry no fraction and read as (0, 0) */
+static void otex_sample(const otex_t *t, float u, float v, float lambda, float out[4])
+{
+    const float top = (float)(t->levels - 1);
+    if(!(lambda > 0.0f)) lambda = 0.0f;
+    if(lambda > top) lambda = top;
+    if(!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) u = v = 0.0f;
+    const uint32_t l0 = (uint32_t)lambda;
+    const float f = lambda - (float)l0;
+    otex_level(t, l0, u, v, out);
+    if(f == 0.0f) return;
+    float b[4];
+    otex_level(t, l0 + 1 < t->levels ? l0 + 1 : t->levels - 1, u, v, b);
+    for(int c = 0; c < 4; ++c) out[c] = lerpf(out[c], b[c], f);
+}
+
+void orc_texture_sample(const orc_texture *tex, const float *uv, const float *lambda, size_t n, float *out4)
+{
+    otex_t t;
+    otex_build(&t, tex);
+    for(size_t i = 0; i < n; ++i) otex_sample(&t, uv[2 * i], uv[2 * i + 1], lambda[i], out4 + 4 * i);
+    otex_free(&t);
+}
+
+/* what the any-hit stages need of the scene: the alpha-cutout test of MASK materials (raypipeline.slang:
+ * 686-720 for closest-hit rays, visibility_test.slang:136-155 for shadow rays) */
+typedef struct
+{
+    const mat_t *mats;
+    const uint32_t *mat_of_prim;
+    const uint32_t *indices;
+    const float *uv; /* 2 per vertex */
+    const otex_t *tex;
+} alpha_ctx;
+
+/* 1 when the candidate is kept: color.a (times the base color texture's alpha at level 0, at the hit's
+ * interpolated coordinates) is not below the cutoff. disable_material stands the color in by (0.8, 1) for
+ * closest-hit rays; the shadow-ray test reads the material as it is. */
+static int alpha_keep(const alpha_ctx *ac, uint32_t prim, float bu, float bv, int disable_material)
+{
+    const mat_t *m = &ac->mats[ac->mat_of_prim[prim]];
+    if(!m->mask) return 1;
+    if(disable_material) return !(1.0f < m->alpha_cutoff);
+    float a = m->alpha;
+    if(m->tex)
+    {
+        const uint32_t *ix = ac->indices + 3 * (size_t)prim;
+        const float *t0 = ac->uv + 2 * (size_t)ix[0], *t1 = ac->uv + 2 * (size_t)ix[1], *t2 = ac->uv + 2 * (size_t)ix[2];
+        const float b0 = 1.0f - bu - bv;
+        const float u = (t0[0] * b0 + t1[0] * bu) + t2[0] * bv, v = (t0[1] * b0 + t1[1] * bu) + t2[1] * bv;
+        float c[4];
+        otex_sample(&ac->tex[m->tex - 1], u, v, 0.0f, c);
+        a = a * c[3];
+    }
+    return !(a < m->alpha_cutoff);
+}
+
 /* ---- a CPU BVH with exact closest-hit semantics ----------------------------------------------- */
 typedef struct
 {
@@ -374,6 +508,7 @@ typedef struct
     cnode *nodes;
     uint32_t n_nodes;
     uint32_t *order;
+    const alpha_ctx *ac; /* the any-hit test of masked materials */
 } cbvh;
 
 static void tri_box(const float *t, double lo[3], double hi[3])
@@ -465,8 +600,10 @@ static int box_hit(const cnode *nd, const double o[3], const double inv[3], doub
     return t0 <= t1 && t1 >= tmin && t0 <= tmax;
 }
 
-/* closest hit with tmin < t < tmax, lowest index on ties, back faces culled when `cull` */
-static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int cull, float *bt, float *bu, float *bv)
+/* closest hit with tmin < t < tmax, lowest index on ties, back faces culled when `cull`; masked candidates
+ * whose alpha falls below their cutoff are ignored (any-hit IgnoreHit) */
+static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int cull, int disable_material, float *bt,
+                        float *bu, float *bv)
 {
     double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
     uint32_t best = 0xFFFFFFFFu;
@@ -496,6 +633,7 @@ static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int c
                 if(!(t > tmin)) continue;
                 if(t < best_t || (t == best_t && best != 0xFFFFFFFFu && prim < best))
                 {
+                    if(!alpha_keep(b->ac, prim, u, v, disable_material)) continue;
                     best_t = t;
                     *bu = u;
                     *bv = v;
@@ -513,8 +651,8 @@ static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int c
     return best;
 }
 
-/* transmittance_test (visibility_test.slang:94-165) without culling: 0 when an opaque triangle lies in
- * (tmin, tmax), else the product of transmission * color over the transmissive hits in ascending
+/* transmittance_test (visibility_test.slang:94-165) without culling: 0 when an opaque triangle (not cut out
+ * by its alpha mask) lies in (tmin, tmax), else the product of transmission * color over the transmissive hits in ascending
  * (t, triangle) order; mat_of_prim maps a triangle to its material */
 #define ORC_MAX_TRANSMISSIVE 4096
 typedef struct
@@ -546,7 +684,12 @@ static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim
             float t, u, v;
             if(!orc_intersect_tri(b->tris + 9 * (size_t)prim, &o.x, &d.x, &t, &u, &v)) continue;
             if(!(t > tmin && t < tmax)) continue;
-            if(!(mats[mat_of_prim[prim]].trans > 0.0f)) return V3(0, 0, 0);
+            if(!(mats[mat_of_prim[prim]].trans > 0.0f))
+            {
+                /* alpha-cutout surfaces let the shadow ray through (visibility_test.slang:145-150) */
+                if(!alpha_keep(b->ac, prim, u, v, 0)) continue;
+                return V3(0, 0, 0);
+            }
             if(nh == ORC_MAX_TRANSMISSIVE) abort();
             hits[nh].t = t;
             hits[nh].prim = prim;
@@ -894,6 +1037,8 @@ typedef struct
     mat_t *mats;
     const uint32_t *mat_of_prim;
     const light_t *lights;
+    const otex_t *tex;
+    const float *uv; /* 2 per vertex */
     float *obj_tmax;
     float *out; /* 8 floats per pixel: r g b alpha depth normal(bits) object(bits) albedo(bits) */
     uint32_t y0, y1;
@@ -905,6 +1050,8 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
     const orc_pt_params *pp = j->pp;
     const uint32_t W = pp->width, H = pp->height;
     uint32_t rng = orc_xxhash32(pp->seed, W * y + x);
+    /* the ray cone's spread angle (raypipeline.slang:102), a half like the payload's packed cone */
+    const float cone_spread = orc_half_value(atanf(2.0f * tanf(pp->fov_y / 2.0f) / (float)H));
     v3 result = V3(0, 0, 0);
     float coverage = 0.f;
     float offset = 2.0f * rnd(&rng) - 1.0f;
@@ -939,6 +1086,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
         int regularize = 0, caustic = 0, first_miss = 1;
         v3 beta = V3(1, 1, 1), rad = V3(0, 0, 0);
         float prev_pdf = 0.0f; /* payload.prev_bsdf_pdf */
+        float cone_width = 0.0f;
         const uint32_t nl = pp->n_lights;
         for(uint32_t i = 0; i < pp->max_depth; ++i)
         {
@@ -948,7 +1096,8 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
             const float seg_tmax = tmax, seg_prev = prev_pdf;
             const int seg_caustic = caustic;
             float t = 0.f, bu = 0.f, bv = 0.f;
-            uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, &t, &bu, &bv);
+            uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, (int)pp->disable_material, &t,
+                                    &bu, &bv);
             int stop = 0, media_op = 0;
             float next_ior = 1.f;
             uint32_t obj = 0xFFFFu;
@@ -991,8 +1140,31 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                 if(backface) nrm = neg(nrm);
                 obj = sc->tri_object[prim];
                 mat_t m = j->mats[sc->object_material[obj]];
-                if(pp->disable_material) m.color = V3(0.8f, 0.8f, 0.8f);
                 v3 Vw = neg(dir);
+                if(m.tex)
+                {
+                    /* the base color texture at the ray cone's level of detail (raypipeline.slang:300-321,
+                     * 526-531): 0.5 log2(uv area / world area) + log2|cone width| + 0.5 log2(w h) - log2(N.V),
+                     * with the width the cone had before this segment */
+                    const otex_t *tx = &j->tex[m.tex - 1];
+                    const float *T0 = j->uv + 2 * (size_t)ix[0], *T1 = j->uv + 2 * (size_t)ix[1],
+                                *T2 = j->uv + 2 * (size_t)ix[2];
+                    float tu = (T0[0] * bx + T1[0] * by) + T2[0] * bz, tv = (T0[1] * bx + T1[1] * by) + T2[1] * bz;
+                    v3 pc = cross(sub(v1, v0), sub(v2, v0));
+                    float p_a = sqrtf(dot(pc, pc));
+                    float t_a = fabsf((T1[0] - T0[0]) * (T2[1] - T0[1]) - (T2[0] - T0[0]) * (T1[1] - T0[1]));
+                    float lambda = 0.5f * log2f(t_a / p_a);
+                    lambda += log2f(fabsf(cone_width));
+                    lambda += 0.5f * log2f((float)tx->w * (float)tx->h);
+                    lambda -= log2f(fabsf(dot(Vw, nrm)));
+                    float c[4];
+                    otex_sample(tx, tu, tv, lambda, c);
+                    m.color = V3(m.color.x * c[0], m.color.y * c[1], m.color.z * c[2]);
+                    m.alpha = m.alpha * c[3];
+                }
+                /* propagate(cone, 0, hit t) (raypipeline.slang:280-286, :608); the payload keeps halves */
+                cone_width = orc_half_value(cone_width + cone_spread * t);
+                if(pp->disable_material) m.color = V3(0.8f, 0.8f, 0.8f);
                 org = pos;
                 float eta = backface ? m.ior / iors[media_index > 0 ? media_index - 1 : 0] : iors[media_index] / m.ior;
                 eta += EPS;
@@ -1150,9 +1322,15 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         mats[m].dt = x[15];
         mats[m].sheen = V3(x[16], x[17], x[18]);
         mats[m].dt_color = V3(x[20], x[21], x[22]);
-        uint32_t ts;
+        uint32_t ts, tex, mode;
         memcpy(&ts, &x[24], 4);
         mats[m].two_sided = ts != 0;
+        mats[m].alpha = x[3];
+        memcpy(&tex, &x[25], 4);
+        memcpy(&mode, &x[26], 4);
+        mats[m].tex = tex <= sc->n_textures ? tex : 0;
+        mats[m].mask = mode == 1; /* BLEND traces as OPAQUE */
+        mats[m].alpha_cutoff = x[27];
     }
     for(uint32_t t = 0; t < n; ++t)
     {
@@ -1197,6 +1375,12 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     }
     uint32_t *mat_of_prim = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
     for(uint32_t t = 0; t < n; ++t) mat_of_prim[t] = sc->object_material[sc->tri_object[t]];
+    float *uv = (float *)calloc(2 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
+    if(sc->texcoords) memcpy(uv, sc->texcoords, sizeof(float) * 2 * (size_t)sc->n_vert);
+    otex_t *tex = (otex_t *)calloc(sc->n_textures ? sc->n_textures : 1, sizeof(otex_t));
+    for(uint32_t k = 0; k < sc->n_textures; ++k) otex_build(&tex[k], &sc->textures[k]);
+    alpha_ctx ac = {mats, mat_of_prim, sc->indices, uv, tex};
+    b.ac = &ac;
     light_t *lights = (light_t *)calloc(pp->n_lights ? pp->n_lights : 1, sizeof(light_t));
     for(uint32_t li = 0; li < pp->n_lights; ++li) lights[li] = read_light(pp->lights + 20 * (size_t)li);
     if(nthreads < 1) nthreads = 1;
@@ -1210,6 +1394,8 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         jobs[k].mats = mats;
         jobs[k].mat_of_prim = mat_of_prim;
         jobs[k].lights = lights;
+        jobs[k].tex = tex;
+        jobs[k].uv = uv;
         jobs[k].obj_tmax = obj_tmax;
         jobs[k].out = out8;
         jobs[k].y0 = (uint32_t)((uint64_t)pp->height * k / nthreads);
@@ -1220,6 +1406,9 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     free(th);
     free(jobs);
     free(lights);
+    for(uint32_t k = 0; k < sc->n_textures; ++k) otex_free(&tex[k]);
+    free(tex);
+    free(uv);
     free(mat_of_prim);
     free(lo);
     free(hi);

@@ -11,6 +11,13 @@
 extern "C" {
 #endif
 
+/* an RGBA8 UNORM image, rows from the top, 4 bytes per texel */
+typedef struct orc_texture
+{
+    uint32_t width, height;
+    const uint8_t *rgba;
+} orc_texture;
+
 typedef struct orc_pt_scene
 {
     const float *positions; /* 3 per vertex */
@@ -24,6 +31,9 @@ typedef struct orc_pt_scene
     const float *materials; /* vkrt_material records, 28 words each */
     uint32_t n_materials;
     float environment;
+    const float *texcoords; /* 2 per vertex, or NULL (all zero) */
+    const orc_texture *textures;
+    uint32_t n_textures;
 } orc_pt_scene;
 
 typedef struct orc_pt_params
@@ -34,6 +44,7 @@ typedef struct orc_pt_params
     float tan_half_x, tan_half_y, aperture, focal_distance;
     const float *lights; /* light_t records, 20 words each, already gathered (positive intensity, no Area) */
     uint32_t n_lights, mis_mode, suppress_refractive_caustics;
+    float fov_y; /* vertical field of view: the ray cone's spread angle */
 } orc_pt_params;
 
 /* One batch: out8 = 8 floats per pixel (radiance rgb / num_samples, coverage, linear depth, -, object id
@@ -43,6 +54,8 @@ void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float
 /* binary16 rounding (nearest, ties to even) and the exposure/gamma composite of the output image */
 float orc_half_value(float x);
 void orc_tonemap(const float *in4, size_t n, float exposure, float gamma, float *out4);
+/* SampleLevel(u, v, lambda) of one texture through its box-filtered mip chain: rgba in [0, 1] */
+void orc_texture_sample(const orc_texture *tex, const float *uv, const float *lambda, size_t n, float *out4);
 
 #ifdef __cplusplus
 }

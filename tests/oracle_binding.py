@@ -270,7 +270,12 @@ class _PtScene(ctypes.Structure):
     _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("n_vert", ctypes.c_uint32),
                 ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32), ("tri_object", ctypes.c_void_p),
                 ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
-                ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float)]
+                ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float),
+                ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("n_textures", ctypes.c_uint32)]
+
+
+class _Texture(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba", ctypes.c_void_p)]
 
 
 class _PtParams(ctypes.Structure):
@@ -280,7 +285,22 @@ class _PtParams(ctypes.Structure):
                 ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("tan_half_x", ctypes.c_float),
                 ("tan_half_y", ctypes.c_float), ("aperture", ctypes.c_float), ("focal_distance", ctypes.c_float),
                 ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("mis_mode", ctypes.c_uint32),
-                ("suppress_refractive_caustics", ctypes.c_uint32)]
+                ("suppress_refractive_caustics", ctypes.c_uint32), ("fov_y", ctypes.c_float)]
+
+
+def texture_sample(rgba, uv, lam):
+    """SampleLevel of an (H, W, 4) uint8 image through its mip chain at uv (n, 2) and levels lam (n): (n, 4)"""
+    h = lib()
+    h.orc_texture_sample.restype = None
+    h.orc_texture_sample.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_void_p]
+    img = np.ascontiguousarray(rgba, np.uint8)
+    t = _Texture(img.shape[1], img.shape[0], img.ctypes.data)
+    uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+    lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float32), (len(uv),)))
+    out = np.zeros((len(uv), 4), np.float32)
+    h.orc_texture_sample(ctypes.byref(t), uv.ctypes.data, lam.ctypes.data, len(uv), out.ctypes.data)
+    return out
 
 
 def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_path_beta=0.0, disable_material=0,
@@ -293,10 +313,15 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
     h.orc_pt_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
     arr = {k: np.ascontiguousarray(scene[k]) for k in
            ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
+    uv = scene.get("texcoords")
+    uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
+    imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
+    tex = (_Texture * max(1, len(imgs)))(*[_Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in imgs])
     sc = _PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
                   arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
                   arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,
-                  len(arr["materials"]), float(scene.get("environment", 1.0)))
+                  len(arr["materials"]), float(scene.get("environment", 1.0)),
+                  uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs))
     eye, fw, rt, up, thx, thy = W.camera_basis(camera, width, height)
     # the tracer's gather: positive intensity, no Area lights, order kept
     la = np.zeros(0, vk.LIGHT_DTYPE) if lights is None else np.asarray(lights, vk.LIGHT_DTYPE).reshape(-1)
@@ -305,7 +330,8 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
                    (ctypes.c_float * 3)(*eye), (ctypes.c_float * 3)(*fw), (ctypes.c_float * 3)(*rt),
                    (ctypes.c_float * 3)(*up), thx, thy, float(camera.aperture),
                    float(camera.focal_distance) if camera.focal_distance > 0 else 1.0,
-                   la.ctypes.data if len(la) else None, len(la), mis_mode, suppress_refractive_caustics)
+                   la.ctypes.data if len(la) else None, len(la), mis_mode, suppress_refractive_caustics,
+                   float(camera.fov_y))
     out = np.zeros((height, width, 8), np.float32)
     h.orc_pt_render(ctypes.byref(sc), ctypes.byref(pp), out.ctypes.data, nthreads or threads())
     return out

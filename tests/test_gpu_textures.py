@@ -1,0 +1,130 @@
+"""GPU textured materials: base color textures at the ray cone's level of detail (raypipeline.slang:280-321,
+526-531, 608), the alpha-mask any-hit stages of camera and shadow rays (raypipeline.slang:686-720,
+visibility_test.slang:136-155), against the CPU integrator and the geometry-derived pins of
+test_texture_oracle, and a textured model loaded from OBJ + MTL + PNG."""
+import numpy as np
+import pytest
+
+import oracle_binding as orc
+import vierkant_amd as vk
+from vierkant_amd import workloads as W
+import test_texture_oracle as T
+
+pytestmark = pytest.mark.gpu
+
+
+def settings(w, h, spp, depth, disable=0):
+    s = vk.default_settings()
+    s.resolution[0], s.resolution[1] = w, h
+    s.num_samples, s.max_trace_depth = spp, depth
+    s.disable_material = disable
+    return s
+
+
+def gpu_render(ctx, scene, lights, cam, w, h, spp, depth, seed, disable=0):
+    pt = vk.PathTracer(ctx, settings(w, h, spp, depth, disable), seed=seed)
+    pt.set_scene(scene)
+    pt.set_lights(lights)
+    pt.render(cam)
+    px = pt.pixels()
+    pt.close()
+    return px["radiance"]
+
+
+def oracle_seed(seed):
+    return W.minstd_rand0(seed, 1)[0]
+
+
+@pytest.mark.parametrize("disable", [0, 1])
+def test_gpu_alpha_mask_camera_rays(ctx, disable):
+    """camera rays skip the cut-out half of the masked quad (all of it kept under disable_material), bit for
+    bit as the renders without the quad and with it opaque, and equal to the CPU integrator"""
+    render = lambda sc, lights, dis: gpu_render(ctx, sc, lights, T.overhead_camera(), T.SIZE, T.SIZE, 4, 2, 7, dis)
+    r = T.alpha_renders(render, (1.0, -1.0, 1.0), T.DOWN, disable)
+    T.check_camera_ray_cutout(r, disable)
+    want = orc.pt_render(T.masked_quad_scene(1.0, -1.0, 1.0), T.overhead_camera(), T.SIZE, T.SIZE, 4, 2,
+                         oracle_seed(7), lights=[T.DOWN], disable_material=disable)[..., :3]
+    assert np.allclose(r["masked"], want, rtol=1e-4, atol=1e-6)
+
+
+@pytest.mark.parametrize("disable", [0, 1])
+def test_gpu_alpha_mask_shadow_rays(ctx, disable):
+    """shadow rays of next-event estimation pass the cut-out half and stop at the rest, with the material as
+    it is under disable_material too"""
+    render = lambda sc, lights, dis: gpu_render(ctx, sc, lights, T.overhead_camera(), T.SIZE, T.SIZE, 4, 2, 7, dis)
+    r = T.alpha_renders(render, T.SHADOW_QUAD, T.SLANT, disable)
+    T.check_shadow_ray_cutout(r)
+    want = orc.pt_render(T.masked_quad_scene(*T.SHADOW_QUAD), T.overhead_camera(), T.SIZE, T.SIZE, 4, 2,
+                         oracle_seed(7), lights=[T.SLANT], disable_material=disable)[..., :3]
+    assert np.allclose(r["masked"], want, rtol=1e-4, atol=1e-6)
+
+
+def test_gpu_ray_cone_level_of_detail(ctx):
+    """the mirror's texture through the slab at the cone's level of detail, directly at level 0: the geometry
+    pin, and the CPU integrator pixel for pixel"""
+    seed = 3
+    render = lambda sc: gpu_render(ctx, sc, [], T.lod_camera(), T.LOD_W, T.LOD_H, 1, 7, seed)
+    T.check_lod(render, seed=oracle_seed(seed))
+    got = render(T.lod_scene(True))
+    want = orc.pt_render(T.lod_scene(True), T.lod_camera(), T.LOD_W, T.LOD_H, 1, 7, oracle_seed(seed))[..., :3]
+    assert np.abs(got - want).max() < 1e-4
+
+
+def textured_room(seed=0):
+    """the room scene with planar texture coordinates, two textures (one with a hole pattern in its alpha)
+    on half of its materials, and a few of those masked"""
+    sc = W.room_scene()
+    p = sc["positions"]
+    sc["texcoords"] = np.stack([0.37 * p[:, 0] + 0.11 * p[:, 1], 0.41 * p[:, 2] + 0.23 * p[:, 1]], -1).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    a = rng.integers(40, 256, (64, 64, 4), dtype=np.uint8)
+    b = rng.integers(0, 256, (21, 37, 4), dtype=np.uint8)
+    yy, xx = np.mgrid[0:21, 0:37]
+    b[..., 3] = np.where((xx // 4 + yy // 4) % 2 == 0, 255, 0)
+    sc["textures"] = [a, b]
+    mats = sc["materials"].copy()
+    for k in range(len(mats)):
+        if k % 2 == 0:
+            mats[k]["color_texture"] = 1 + (k // 2) % 2
+        if k % 6 == 2:
+            mats[k]["alpha_mode"] = vk.ALPHA_MASK
+            mats[k]["alpha_cutoff"] = 0.5
+    sc["materials"] = mats
+    return sc
+
+
+def test_gpu_textured_room_matches_cpu_integrator(ctx):
+    """the textured, partly masked room with its lights, 160x90, 4 spp, depth 8: per-pixel radiance equals
+    the CPU integrator up to transcendental ulps (log2 in the level of detail among them)"""
+    sc, cam, lights = textured_room(), W.room_camera(), W.room_lights()
+    w, h, spp = 160, 90, 4
+    got = gpu_render(ctx, sc, lights, cam, w, h, spp, 8, 0)
+    want = orc.pt_render(sc, cam, w, h, spp, 8, oracle_seed(0), lights=lights)[..., :3]
+    assert np.isfinite(got).all()
+    diff = np.abs(got - want).max(axis=2)
+    close = np.mean(diff <= 1e-4 * np.maximum(1.0, want.max(axis=2)))
+    rmse = float(np.sqrt(np.mean((got - want) ** 2)))
+    assert close > 0.97, f"only {close:.4f} of pixels within 1e-4"
+    assert rmse < 0.02 * float(want.mean()) + 1e-6
+    # the textures change the picture
+    plain = gpu_render(ctx, W.room_scene(), lights, cam, w, h, spp, 8, 0)
+    assert np.abs(plain - got).mean() > 0.01 * float(want.mean())
+
+
+def test_gpu_loaded_textured_obj(ctx, tmp_path):
+    """an OBJ + MTL + PNG quad loaded by the library renders as the CPU integrator renders the loaded arrays
+    (the v flip of the diffuse texture's transform included)"""
+    from PIL import Image
+    rng = np.random.default_rng(4)
+    Image.fromarray(rng.integers(0, 256, (16, 24, 3), dtype=np.uint8)).save(tmp_path / "t.png")
+    (tmp_path / "q.mtl").write_text("newmtl m\nKd 1 1 1\nmap_Kd t.png\n")
+    (tmp_path / "q.obj").write_text("mtllib q.mtl\nv -1 0 -1\nv 1 0 -1\nv 1 0 1\nv -1 0 1\n"
+                                    "vt 0 0\nvt 2 0\nvt 2 3\nvt 0 3\nvn 0 1 0\nusemtl m\nf 1/1/1 4/4/1 3/3/1 2/2/1\n")
+    sc = vk.load_model(str(tmp_path / "q.obj"), environment=1.0)
+    assert len(sc["textures"]) == 1 and sc["materials"][0]["color_texture"] == 1
+    cam = vk.make_camera(eye=(0.0, 3.0, 0.0), forward=(0.0, -1.0, 0.0), up=(0.0, 0.0, -1.0),
+                         fov_y=float(np.radians(50.0)))
+    got = gpu_render(ctx, sc, [T.DOWN], cam, 48, 48, 4, 3, 1)
+    want = orc.pt_render(sc, cam, 48, 48, 4, 3, oracle_seed(1), lights=[T.DOWN])[..., :3]
+    close = np.mean(np.abs(got - want).max(axis=2) <= 1e-4 * np.maximum(1.0, want.max(axis=2)))
+    assert close > 0.97, close

@@ -1,0 +1,456 @@
+"""Pins the textured-material parts of the CPU integrator (oracle/vkpt_oracle.c) and the library's image
+decoding: the mip chain and SampleLevel against scipy's periodic linear interpolation, PNG decoding against
+Pillow, the alpha-mask any-hit stages against renders that differ only by the masked surface, and the ray
+cone's level of detail against geometry worked out here (raypipeline.slang:280-321, 526-531, 608, 686-720;
+visibility_test.slang:136-155)."""
+import struct
+import zlib
+
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle_binding as orc
+import vierkant_amd as vk
+from vierkant_amd import workloads as W
+
+
+# ---- mip chain and sampling ---------------------------------------------------------------------------------
+def mip_chain(img):
+    """levels of an (h, w, 4) uint8 image: 2 x 2 box filter with edge clamping, (a + b + c + d + 2) // 4"""
+    levels = [img.astype(np.int64)]
+    while levels[-1].shape[0] > 1 or levels[-1].shape[1] > 1:
+        p = levels[-1]
+        h, w = p.shape[:2]
+        nh, nw = max(1, h // 2), max(1, w // 2)
+        ys = np.minimum(np.stack([2 * np.arange(nh), 2 * np.arange(nh) + 1]), h - 1)
+        xs = np.minimum(np.stack([2 * np.arange(nw), 2 * np.arange(nw) + 1]), w - 1)
+        acc = sum(p[ys[a]][:, xs[b]] for a in range(2) for b in range(2))
+        levels.append((acc + 2) // 4)
+    return [lv.astype(np.float64) / 255.0 for lv in levels]
+
+
+def sample_level(level, u, v):
+    """bilinear filtering with repeat addressing, texel centres at (i + 0.5) / size (scipy grid-wrap)"""
+    h, w = level.shape[:2]
+    coords = [v * h - 0.5, u * w - 0.5]
+    return np.stack([ndimage.map_coordinates(level[..., c], coords, order=1, mode="grid-wrap") for c in range(4)], -1)
+
+
+def sample(levels, u, v, lam):
+    lam = np.nan_to_num(np.asarray(lam, np.float64), nan=0.0, neginf=0.0, posinf=len(levels) - 1)
+    lam = np.clip(lam, 0, len(levels) - 1)
+    l0 = np.floor(lam).astype(int)
+    f = (lam - l0)[:, None]
+    out = np.zeros((len(u), 4))
+    for k in range(len(levels)):
+        sel = l0 == k
+        if sel.any():
+            a = sample_level(levels[k], u[sel], v[sel])
+            b = sample_level(levels[min(k + 1, len(levels) - 1)], u[sel], v[sel])
+            out[sel] = a + f[sel] * (b - a)
+    return out
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (7, 12), (1, 9), (33, 5)])
+def test_sampler_matches_periodic_linear_interpolation(shape):
+    """SampleLevel over the box-filtered chain, odd and one-texel sizes included: scipy's grid-wrap linear
+    interpolation on each level, linear between levels, within float32 rounding"""
+    rng = np.random.default_rng(sum(shape))
+    img = rng.integers(0, 256, (*shape, 4), dtype=np.uint8)
+    n = 4000
+    uv = rng.uniform(-3.0, 3.0, (n, 2)).astype(np.float32)
+    lam = rng.uniform(-1.0, 7.0, n).astype(np.float32)
+    lam[:5] = [np.nan, -np.inf, np.inf, 0.0, 100.0]
+    got = orc.texture_sample(img, uv, lam)
+    want = sample(mip_chain(img), uv[:, 0].astype(np.float64), uv[:, 1].astype(np.float64), lam)
+    assert np.abs(got - want).max() < 2e-5
+
+
+def test_mip_levels_of_a_constant_and_a_checkerboard():
+    """a constant image keeps its value on every level; a one-texel checkerboard of 0 and 255 averages to
+    (0 + 255 + 0 + 255 + 2) // 4 = 128 from level 1 on, at any coordinate"""
+    c = np.full((8, 8, 4), 77, np.uint8)
+    uv = np.random.default_rng(0).uniform(0, 1, (64, 2)).astype(np.float32)
+    for lam in (0.0, 1.5, 3.0):
+        assert np.allclose(orc.texture_sample(c, uv, lam), 77 / 255, atol=1e-6)
+    chk = np.zeros((8, 8, 4), np.uint8)
+    chk[(np.arange(8)[:, None] + np.arange(8)) % 2 == 0] = 255
+    for lam in (1.0, 2.0, 3.0):
+        assert np.allclose(orc.texture_sample(chk, uv, lam), 128 / 255, atol=1e-6)
+    # texel centres of level 0 read the texels themselves
+    ij = np.stack(np.meshgrid(np.arange(8), np.arange(8), indexing="ij"), -1).reshape(-1, 2)
+    centres = ((ij[:, ::-1] + 0.5) / 8).astype(np.float32)
+    got = orc.texture_sample(chk, centres, 0.0)
+    assert np.allclose(got, chk.reshape(-1, 4) / 255, atol=1e-6)
+
+
+# ---- PNG decoding -------------------------------------------------------------------------------------------
+def _chunk(kind, body):
+    return struct.pack(">I", len(body)) + kind + body + struct.pack(">I", zlib.crc32(kind + body) & 0xFFFFFFFF)
+
+
+def png_bytes(rows_by_pass, width, height, depth, ctype, interlace=0, filters=(0,), plte=None, trns=None):
+    """a PNG from raw scanlines (bytes per row, per pass), each row given the next filter type in turn and
+    encoded by it (the decoder must undo each)"""
+    out = b""
+    k = 0
+    for rows in rows_by_pass:
+        prev = None
+        for r in rows:
+            f = filters[k % len(filters)]
+            k += 1
+            bpp = max(1, {0: 1, 2: 3, 3: 1, 4: 2, 6: 4}[ctype] * depth // 8)
+            cur = np.frombuffer(r, np.uint8).astype(np.int64)
+            up = np.zeros_like(cur) if prev is None else prev
+            enc = np.zeros_like(cur)
+            for x in range(len(cur)):
+                a = cur[x - bpp] if x >= bpp else 0
+                b = up[x]
+                c = up[x - bpp] if x >= bpp else 0
+                if f == 0:
+                    p = 0
+                elif f == 1:
+                    p = a
+                elif f == 2:
+                    p = b
+                elif f == 3:
+                    p = (a + b) // 2
+                else:
+                    pp = a + b - c
+                    pa, pb, pc = abs(pp - a), abs(pp - b), abs(pp - c)
+                    p = a if pa <= pb and pa <= pc else (b if pb <= pc else c)
+                enc[x] = (cur[x] - p) % 256
+            out += bytes([f]) + enc.astype(np.uint8).tobytes()
+            prev = cur
+    data = b"\x89PNG\r\n\x1a\n" + _chunk(b"IHDR", struct.pack(">IIBBBBB", width, height, depth, ctype, 0, 0,
+                                                                 interlace))
+    if plte is not None:
+        data += _chunk(b"PLTE", plte)
+    if trns is not None:
+        data += _chunk(b"tRNS", trns)
+    return data + _chunk(b"IDAT", zlib.compress(out, 9)) + _chunk(b"IEND", b"")
+
+
+def pack_rows(samples, depth):
+    """(h, w*channels) integer samples -> packed scanlines at the bit depth"""
+    rows = []
+    for r in samples:
+        if depth == 16:
+            rows.append(np.asarray(r, ">u2").tobytes())
+        elif depth == 8:
+            rows.append(np.asarray(r, np.uint8).tobytes())
+        else:
+            bits = np.unpackbits(np.asarray(r, np.uint8)[:, None], axis=1)[:, 8 - depth:].reshape(-1)
+            rows.append(np.packbits(bits).tobytes())
+    return rows
+
+
+ADAM7 = [(0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2), (0, 1, 1, 2)]
+
+
+def encode(samples, depth, ctype, channels, interlace, **kw):
+    h, w = samples.shape[:2]
+    if not interlace:
+        return png_bytes([pack_rows(samples.reshape(h, w * channels), depth)], w, h, depth, ctype, **kw)
+    passes = []
+    for x0, y0, dx, dy in ADAM7:
+        sub = samples[y0::dy, x0::dx]
+        if sub.size:
+            passes.append(pack_rows(sub.reshape(sub.shape[0], -1), depth))
+    return png_bytes(passes, w, h, depth, ctype, interlace=1, **kw)
+
+
+def load_png(tmp_path, data, name="t.png"):
+    (tmp_path / name).write_bytes(data)
+    (tmp_path / "m.mtl").write_text(f"newmtl m\nmap_Kd {name}\n")
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl m\nf 1 2 3\n")
+    m = vk.load_model(str(tmp_path / "m.obj"))
+    assert m["n_skipped_textures"] == 0 and len(m["textures"]) == 1
+    return m["textures"][0]
+
+
+CASES = [  # depth, ctype, channels, interlace, filters
+    (8, 6, 4, 0, (0, 1, 2, 3, 4)), (8, 2, 3, 0, (4, 3, 2, 1, 0)), (8, 0, 1, 0, (1, 4)), (8, 4, 2, 0, (3,)),
+    (16, 6, 4, 0, (4, 2)), (16, 0, 1, 0, (1,)), (4, 0, 1, 0, (2, 4)), (2, 0, 1, 0, (0,)), (1, 0, 1, 0, (1, 3)),
+    (8, 6, 4, 1, (0, 1, 2, 3, 4)), (16, 2, 3, 1, (4,)), (1, 0, 1, 1, (2,)),
+]
+
+
+@pytest.mark.parametrize("depth,ctype,channels,interlace,filters", CASES)
+def test_png_decoding_matches_pillow(tmp_path, depth, ctype, channels, interlace, filters):
+    """every colour type, bit depth, row filter and Adam7 interlacing, odd sizes: the RGBA8 texels equal
+    Pillow's reading of the same file (16-bit samples keep their high byte; grey below 8 bits scales to
+    0..255)"""
+    from PIL import Image
+    rng = np.random.default_rng(depth * 10 + ctype + 100 * interlace)
+    h, w = 13, 11
+    samples = rng.integers(0, 1 << depth, (h, w, channels))
+    data = encode(samples, depth, ctype, channels, interlace, filters=filters)
+    got = load_png(tmp_path, data)
+    (tmp_path / "p.png").write_bytes(data)
+    im = Image.open(tmp_path / "p.png")
+    if depth == 16 and ctype == 0:
+        # Pillow keeps 16-bit grey at full depth (mode I;16); its high byte is the 8-bit value
+        g = np.asarray(im).astype(np.int64)[..., None] >> 8
+        want = np.concatenate([g, g, g, np.full((h, w, 1), 255)], -1)
+    else:
+        want = np.asarray(im.convert("RGBA")).astype(np.int64)
+    assert np.array_equal(got.astype(np.int64), want)
+
+
+def test_png_palette_and_transparency_keys(tmp_path):
+    """a 4-bit palette with tRNS alphas, and grey / RGB colour keys that make one value transparent"""
+    from PIL import Image
+    rng = np.random.default_rng(5)
+    pal = rng.integers(0, 256, (16, 3), dtype=np.uint8)
+    alphas = rng.integers(0, 256, 10, dtype=np.uint8)
+    idx = rng.integers(0, 16, (9, 14, 1))
+    data = encode(idx, 4, 3, 1, 0, filters=(1, 2), plte=pal.tobytes(), trns=alphas.tobytes())
+    got = load_png(tmp_path, data)
+    (tmp_path / "p.png").write_bytes(data)
+    want = np.asarray(Image.open(tmp_path / "p.png").convert("RGBA"))
+    assert np.array_equal(got, want)
+    grey = rng.integers(0, 4, (6, 5, 1))
+    data = encode(grey, 8, 0, 1, 0, trns=struct.pack(">H", 2))
+    got = load_png(tmp_path, data, "g.png")
+    assert np.array_equal(got[..., 3], np.where(grey[..., 0] == 2, 0, 255))
+    assert np.array_equal(got[..., 0], grey[..., 0])
+    rgb = rng.integers(0, 2, (6, 5, 3))
+    data = encode(rgb, 8, 2, 3, 1, trns=struct.pack(">HHH", 1, 0, 1))
+    got = load_png(tmp_path, data, "c.png")
+    key = (rgb == [1, 0, 1]).all(-1)
+    assert np.array_equal(got[..., 3], np.where(key, 0, 255))
+
+
+def test_obj_coordinates_flip_only_under_a_diffuse_texture(tmp_path):
+    """the diffuse texture's transform scales v by -1 (wavefront_obj.cpp:100-111); faces of other materials
+    keep their coordinates"""
+    Image = pytest.importorskip("PIL.Image")
+    Image.fromarray(np.full((2, 2, 4), 200, np.uint8)).save(tmp_path / "t.png")
+    (tmp_path / "m.mtl").write_text("newmtl tex\nmap_Kd t.png\nnewmtl plain\nKd 0.5 0.5 0.5\n")
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0.25 0.75\nvt 1 0.5\nvt 0 1\n"
+                                    "o a\nusemtl tex\nf 1/1 2/2 3/3\no b\nusemtl plain\nf 1/1 2/2 3/3\n")
+    m = vk.load_model(str(tmp_path / "m.obj"))
+    want = np.float32([[0.25, 0.75], [1, 0.5], [0, 1]])
+    assert np.array_equal(m["texcoords"][:3], want * np.float32([1, -1]))
+    assert np.array_equal(m["texcoords"][3:], want)
+
+
+# ---- alpha mask ---------------------------------------------------------------------------------------------
+def half_alpha_texture(n=64):
+    """alpha 0 for u < 0.5, 255 from there; colour bright"""
+    t = np.full((n, n, 4), 230, np.uint8)
+    t[:, : n // 2, 3] = 0
+    t[:, n // 2:, 3] = 255
+    return t
+
+
+def masked_quad_scene(y, z0, z1, mode=vk.ALPHA_MASK, with_quad=True):
+    """the rough plane of the closed-form NEE tests plus a one-sided quad at height y facing up, x in [-1, 1],
+    z in [z0, z1], u = (x + 1) / 2, textured with half_alpha_texture"""
+    from test_nee_oracle import plane_scene
+    sc = plane_scene()
+    if not with_quad:
+        sc["texcoords"] = np.zeros((4, 2), np.float32)
+        return sc
+    q = np.array([[-1, y, z0], [1, y, z0], [1, y, z1], [-1, y, z1]], np.float32)
+    sc["positions"] = np.concatenate([sc["positions"], q])
+    sc["normals"] = np.tile(np.float32([0, 1, 0]), (8, 1))
+    sc["indices"] = np.concatenate([sc["indices"], np.array([4, 6, 5, 4, 7, 6], np.uint32)])
+    sc["tri_object"] = np.array([0, 0, 1, 1], np.uint32)
+    sc["object_material"] = np.array([0, 1], np.uint32)
+    quad = vk.material(color=(0.3, 0.6, 0.9), color_texture=1, alpha_mode=mode, alpha_cutoff=0.5)
+    sc["materials"] = np.concatenate([sc["materials"], np.array([quad], vk.MATERIAL_DTYPE)])
+    uvq = np.stack([(q[:, 0] + 1) / 2, (q[:, 2] - z0) / (z1 - z0)], -1)
+    sc["texcoords"] = np.concatenate([np.zeros((4, 2), np.float32), uvq]).astype(np.float32)
+    sc["textures"] = [half_alpha_texture()]
+    return sc
+
+
+def overhead_camera():
+    """straight down from y = 5 with a 16 degree field: the plane within 0.7 units of the origin"""
+    return vk.make_camera(eye=(0.0, 5.0, 0.0), forward=(0.0, -1.0, 0.0), up=(0.0, 0.0, -1.0),
+                          fov_y=float(np.radians(16.0)))
+
+
+SIZE = 24
+SIDE = (slice(None), slice(0, SIZE // 2 - 2)), (slice(None), slice(SIZE // 2 + 2, SIZE))
+
+
+def alpha_renders(render, scene_args, light, disable):
+    out = {}
+    for key, kw in (("masked", {}), ("opaque", {"mode": vk.ALPHA_OPAQUE}), ("none", {"with_quad": False})):
+        out[key] = render(masked_quad_scene(*scene_args, **kw), [light], disable)
+    return out
+
+
+def oracle_render(sc, lights, disable):
+    return orc.pt_render(sc, overhead_camera(), SIZE, SIZE, 4, 2, 7, lights=lights, disable_material=disable)[..., :3]
+
+
+def check_camera_ray_cutout(r, disable):
+    """the quad over the whole view, light straight down: where the quad's alpha is cut out the image is the
+    one without the quad, bit for bit; elsewhere the opaque quad's. With disable_material the camera rays keep
+    the whole quad (stand-in alpha 1)"""
+    left, right = SIDE
+    assert np.array_equal(r["masked"][right], r["opaque"][right])
+    if disable:
+        assert np.array_equal(r["masked"][left], r["opaque"][left])
+    else:
+        assert np.array_equal(r["masked"][left], r["none"][left])
+        assert not np.array_equal(r["none"][left], r["opaque"][left])
+
+
+def check_shadow_ray_cutout(r):
+    """the quad outside the view, light from 45 degrees behind it: plane points whose shadow ray crosses the
+    cut-out half are lit as without the quad, the others shadowed as by the opaque quad, whatever
+    disable_material says"""
+    left, right = SIDE
+    assert np.array_equal(r["masked"][left], r["none"][left])
+    assert np.array_equal(r["masked"][right], r["opaque"][right])
+    assert r["opaque"][right].max() < 0.1 * r["none"][right].min()
+
+
+# at y = 3 the camera sees |z| < 0.28; shadow rays toward SLANT cross y = 3 at z in [2.3, 3.7]
+SHADOW_QUAD = (3.0, 2.0, 4.0)
+DOWN = vk.light(vk.LIGHT_DIRECTIONAL, direction=(0, -1, 0), intensity=3.0)
+SLANT = vk.light(vk.LIGHT_DIRECTIONAL, direction=(0, -np.sqrt(0.5), -np.sqrt(0.5)), intensity=3.0)
+
+
+@pytest.mark.parametrize("disable", [0, 1])
+def test_alpha_mask_camera_rays(disable):
+    check_camera_ray_cutout(alpha_renders(oracle_render, (1.0, -1.0, 1.0), DOWN, disable), disable)
+
+
+@pytest.mark.parametrize("disable", [0, 1])
+def test_alpha_mask_shadow_rays(disable):
+    check_shadow_ray_cutout(alpha_renders(oracle_render, SHADOW_QUAD, SLANT, disable))
+
+
+# ---- ray cone level of detail -------------------------------------------------------------------------------
+LOD_W, LOD_H, LOD_FOV, LOD_SEED = 40, 32, 20.0, 3
+TEX_N = 256
+
+
+def lod_texture():
+    rng = np.random.default_rng(11)
+    t = rng.integers(96, 256, (TEX_N, TEX_N, 4), dtype=np.uint8)
+    t[..., 3] = 255
+    return t
+
+
+def lod_scene(textured):
+    """a thin index-matched slab (z = -2 and -2.1, x < 0) in front of a textured mirror at z = -5 whose
+    coordinates span 4 units per unit; camera at the origin looking down -z. Rays through the slab reach the
+    mirror with a cone widened over two segments (and bent by the integrator's eta bias of 1e-4 at each
+    face); the others reach it first"""
+    pos, idx, tri_obj, uv = [], [], [], []
+
+    def quad(corners, obj, tex=None):
+        base = len(pos)
+        pos.extend(corners)
+        uv.extend(tex if tex is not None else [[0, 0]] * 4)
+        idx.extend([base, base + 1, base + 2, base, base + 2, base + 3])
+        tri_obj.extend([obj, obj])
+
+    quad([[-10, -10, -2], [0, -10, -2], [0, 10, -2], [-10, 10, -2]], 0)          # slab front, facing +z
+    quad([[-10, -10, -2.1], [-10, 10, -2.1], [0, 10, -2.1], [0, -10, -2.1]], 0)  # slab back, facing -z
+    corners = [[-2, -2, -5], [2, -2, -5], [2, 2, -5], [-2, 2, -5]]
+    quad(corners, 1, [[(c[0] + 2) / 4, (c[1] + 2) / 4] for c in corners])
+    pos = np.array(pos, np.float32)
+    tris = np.array(idx).reshape(-1, 3)
+    p = pos[tris]
+    fn = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    nrm = np.zeros_like(pos)
+    for t, tri in enumerate(tris):
+        nrm[tri] = fn[t] / np.linalg.norm(fn[t])
+    slab = vk.material(color=(1, 1, 1), roughness=0.0, transmission=1.0, ior=1.0)
+    mirror = vk.material(color=(1, 1, 1), roughness=0.0, metalness=1.0, color_texture=1 if textured else 0)
+    return {"positions": pos, "normals": nrm.astype(np.float32), "indices": np.array(idx, np.uint32),
+            "tri_object": np.array(tri_obj, np.uint32), "object_material": np.array([0, 1], np.uint32),
+            "materials": np.array([slab, mirror], vk.MATERIAL_DTYPE), "environment": 1.0,
+            "texcoords": np.array(uv, np.float32), "textures": [lod_texture()]}
+
+
+def lod_camera():
+    return vk.make_camera(eye=(0.0, 0.0, 0.0), forward=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0),
+                          fov_y=float(np.radians(LOD_FOV)))
+
+
+def lod_lookups(seed=LOD_SEED):
+    """per pixel: where the mirror is hit (u, v), the level of detail and whether the ray came through the
+    slab. The camera rays are rebuilt from the pixel seeds (one sample: jitter = frac(2 rnd - 1) on both
+    axes); the cone's spread and widths follow from the geometry, rounded to halves after each segment as
+    the payload keeps them; lambda = 0.5 log2(uv area / world area) + log2(width) + 0.5 log2(n n) - log2(N.V)
+    with the width from before the mirror's segment (0 for a first hit)"""
+    half = lambda x: np.float64(np.float16(x))
+    _, _, _, _, thx, thy = W.camera_basis(lod_camera(), LOD_W, LOD_H)
+    spread = half(np.arctan(2.0 * np.tan(np.radians(LOD_FOV) / 2.0) / LOD_H))
+    ys, xs = np.mgrid[0:LOD_H, 0:LOD_W]
+    jit = np.zeros((LOD_H, LOD_W))
+    for y in range(LOD_H):
+        for x in range(LOD_W):
+            off = 2.0 * float(orc.rnd_sequence(orc.xxhash32(seed, LOD_W * y + x), 1)[0][0]) - 1.0
+            jit[y, x] = off - np.floor(off)
+    dx = 2.0 * (xs + jit) / LOD_W - 1.0
+    dy = 2.0 * (ys + jit) / LOD_H - 1.0
+    d = np.stack([dx * float(thx), -dy * float(thy), -np.ones_like(dx)], -1)
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    through = d[..., 0] < 0  # crosses z = -2 at x < 0
+
+    def refract(i, eta=1.0 + 1e-4):
+        # both faces see eta = 1 / 1 + 1e-4 (the integrator's eta bias), normal +z facing the ray
+        n = np.array([0.0, 0.0, 1.0])
+        ndi = i @ n
+        k = 1.0 - eta * eta * (1.0 - ndi * ndi)
+        r = eta * i - (eta * ndi + np.sqrt(k))[..., None] * n
+        return r / np.linalg.norm(r, axis=-1, keepdims=True)
+
+    d1 = refract(d)
+    d2 = refract(d1)
+    t1, t2 = 2.0 / -d[..., 2], 0.1 / -d1[..., 2]
+    t3 = 2.9 / -d2[..., 2]
+    hit_through = d * t1[..., None] + d1 * t2[..., None] + d2 * t3[..., None]
+    hit_direct = d * (5.0 / -d[..., 2])[..., None]
+    hit = np.where(through[..., None], hit_through, hit_direct)
+    u, v = (hit[..., 0] + 2) / 4, (hit[..., 1] + 2) / 4
+    w1 = np.vectorize(half)(spread * t1)
+    w2 = np.vectorize(half)(w1 + spread * t2)
+    nov = np.where(through, -d2[..., 2], -d[..., 2])
+    with np.errstate(divide="ignore"):
+        lam = np.where(through, 0.5 * np.log2(1 / 16) + np.log2(w2) + np.log2(TEX_N) - np.log2(nov), -np.inf)
+    return u, v, lam, through
+
+
+def lod_expected(level0=False, seed=LOD_SEED):
+    u, v, lam, through = lod_lookups(seed)
+    lam = np.zeros_like(lam) if level0 else lam
+    val = sample(mip_chain(lod_texture()), u.ravel(), v.ravel(), lam.ravel())
+    return val[:, :3].reshape(LOD_H, LOD_W, 3), through, lam
+
+
+def oracle_lod(sc):
+    return orc.pt_render(sc, lod_camera(), LOD_W, LOD_H, 1, 7, LOD_SEED)[..., :3]
+
+
+def check_lod(render, seed=LOD_SEED):
+    """the mirror reflects the sky with its base colour (metal, normal incidence): the image with the texture
+    over the one without is the texture value at the pixel's lookup"""
+    ratio = render(lod_scene(True)) / render(lod_scene(False))
+    want, through, lam = lod_expected(seed=seed)
+    # the case covers lookups between levels 0 and 2 through the slab and level 0 for first hits
+    assert lam[through].min() > 0.2 and lam[through].max() < 2.5 and (~through).sum() > 100
+    err = np.abs(ratio - want).max(axis=2)
+    # (a width on a half's rounding boundary can land one half-ulp off: 7e-4 in lambda)
+    assert err[~through].max() < 1e-4 and err[through].max() < 5e-4, (err[through].max(), err[~through].max())
+    assert np.percentile(err[through], 95) < 1e-4
+    # a lost cone width (level 0 through the slab) would miss by far more
+    flat, _, _ = lod_expected(level0=True, seed=seed)
+    assert np.abs(ratio - flat).max(axis=2)[through].mean() > 0.03
+
+
+def test_ray_cone_level_of_detail():
+    """the mirror's texture seen through the slab is sampled at the cone's level of detail; seen directly
+    (cone width 0) at level 0"""
+    check_lod(oracle_lod)

@@ -59,7 +59,7 @@ def build_lib(verbose=False):
             print(warn, file=sys.stderr)
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", *[str(o) for o, _ in results],
-           "-o", str(tmp), "-lpthread"]
+           "-o", str(tmp), "-lpthread", "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError("link of libvkrt.so failed:\n" + res.stderr)

@@ -52,6 +52,7 @@ constexpr int STACK_SIZE = 64;      // traversal stack entries per ray; the buil
 constexpr float BOX_PAD = 0x1p-16f; // relative growth of every stored box coordinate
 constexpr float T_PAD = 0x1p-16f;   // relative growth of every slab interval
 constexpr uint32_t TRI_TWO_SIDED = 1u;// flag bit in a leaf triangle record: never culled as a back face
+constexpr uint32_t TRI_ALPHA_MASK = 2u;// flag bit: the record's material is alpha-masked (any-hit test)
 constexpr int32_t EMPTY_CHILD = 0x7fffffff;// unused BVH4 child slot (no node index reaches it)
 
 }// namespace vkrt

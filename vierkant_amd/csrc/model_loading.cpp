@@ -17,7 +17,7 @@
 //         KHR_materials_emissive_strength / transmission / ior / clearcoat / sheen / diffuse_transmission
 //         factors. Node transforms (matrix, or translation * rotation * scale, :136-160) are composed from the
 //         scene's roots and baked into world-space vertices in double precision, rounded to float once.
-// Textures are listed by the loaders but not used by the tracer.
+// Base color textures are decoded (PNG, via zlib's inflate) into RGBA8; other images are counted as skipped.
 #include <algorithm>
 #include <array>
 #include <cctype>
@@ -27,12 +27,15 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include <zlib.h>
 
 #include "vkrt.h"
 
@@ -353,6 +356,189 @@ vkrt_material default_material()
     return m;
 }
 
+//! an RGBA8 image, rows from the top
+struct image_rgba8
+{
+    uint32_t width = 0, height = 0;
+    std::vector<uint8_t> rgba;
+};
+
+uint32_t be32(const uint8_t *p) { return (uint32_t(p[0]) << 24) | (uint32_t(p[1]) << 16) | (uint32_t(p[2]) << 8) | p[3]; }
+
+//! PNG (ISO/IEC 15948): 1-16 bit depths of every color type, not interlaced; 16-bit samples keep their high
+//! byte, gray expands to rgb, palettes and tRNS to rgba
+bool decode_png(const uint8_t *data, size_t size, image_rgba8 &out, std::string &why)
+{
+    static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
+    if(size < 8 || std::memcmp(data, sig, 8) != 0)
+    {
+        why = "not a PNG";
+        return false;
+    }
+    uint32_t w = 0, h = 0;
+    int depth = 0, ctype = 0, interlace = 0;
+    std::vector<uint8_t> idat, plte, trns;
+    size_t off = 8;
+    while(off + 12 <= size)
+    {
+        const uint32_t len = be32(data + off);
+        const uint8_t *type = data + off + 4, *body = data + off + 8;
+        if(off + 12 + (size_t)len > size)
+        {
+            why = "truncated chunk";
+            return false;
+        }
+        if(!std::memcmp(type, "IHDR", 4) && len >= 13)
+        {
+            w = be32(body);
+            h = be32(body + 4);
+            depth = body[8];
+            ctype = body[9];
+            interlace = body[12];
+        }
+        else if(!std::memcmp(type, "PLTE", 4)) plte.assign(body, body + len);
+        else if(!std::memcmp(type, "tRNS", 4)) trns.assign(body, body + len);
+        else if(!std::memcmp(type, "IDAT", 4)) idat.insert(idat.end(), body, body + len);
+        else if(!std::memcmp(type, "IEND", 4)) break;
+        off += 12 + (size_t)len;
+    }
+    const int channels = ctype == 0 ? 1 : ctype == 2 ? 3 : ctype == 3 ? 1 : ctype == 4 ? 2 : ctype == 6 ? 4 : 0;
+    if(!w || !h || !channels || interlace > 1 || !(depth == 1 || depth == 2 || depth == 4 || depth == 8 || depth == 16) ||
+       (ctype != 0 && ctype != 3 && depth < 8) || (ctype == 3 && depth > 8) || (uint64_t)w * h > (1u << 28))
+    {
+        why = "unsupported PNG header";
+        return false;
+    }
+    const size_t bits = (size_t)channels * depth, bpp = std::max<size_t>(1, bits / 8);
+    // the sub-images of the stream: the whole image, or Adam7's seven passes (start and step per axis)
+    struct pass_t
+    {
+        uint32_t x0, y0, dx, dy;
+    };
+    static const pass_t adam7[7] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                    {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+    const pass_t whole{0, 0, 1, 1};
+    const int n_pass = interlace ? 7 : 1;
+    auto pass_size = [&](const pass_t &q, uint32_t &pw, uint32_t &ph) {
+        pw = w > q.x0 ? (w - q.x0 + q.dx - 1) / q.dx : 0;
+        ph = h > q.y0 ? (h - q.y0 + q.dy - 1) / q.dy : 0;
+    };
+    size_t total = 0;
+    for(int k = 0; k < n_pass; ++k)
+    {
+        uint32_t pw, ph;
+        pass_size(interlace ? adam7[k] : whole, pw, ph);
+        if(pw && ph) total += (((size_t)pw * bits + 7) / 8 + 1) * ph;
+    }
+    std::vector<uint8_t> raw(total);
+    z_stream zs{};
+    if(inflateInit(&zs) != Z_OK)
+    {
+        why = "zlib";
+        return false;
+    }
+    zs.next_in = idat.data();
+    zs.avail_in = static_cast<uInt>(idat.size());
+    zs.next_out = raw.data();
+    zs.avail_out = static_cast<uInt>(raw.size());
+    const int zr = inflate(&zs, Z_FINISH);
+    inflateEnd(&zs);
+    if(zr != Z_STREAM_END || zs.avail_out != 0)
+    {
+        why = "bad image data";
+        return false;
+    }
+    // samples at their stored depth, channels interleaved, for the whole image
+    std::vector<uint16_t> px((size_t)w * h * channels);
+    const uint8_t *in = raw.data();
+    for(int k = 0; k < n_pass; ++k)
+    {
+        const pass_t q = interlace ? adam7[k] : whole;
+        uint32_t pw, ph;
+        pass_size(q, pw, ph);
+        if(!pw || !ph) continue;
+        const size_t row = ((size_t)pw * bits + 7) / 8;
+        // undo the per-row filters (none, sub, up, average, Paeth)
+        std::vector<uint8_t> cur(row), prev(row, 0);
+        for(uint32_t y = 0; y < ph; ++y, in += row + 1)
+        {
+            const uint8_t f = in[0];
+            const uint8_t *src = in + 1;
+            for(size_t x = 0; x < row; ++x)
+            {
+                const int a = x >= bpp ? cur[x - bpp] : 0, b = prev[x], c = x >= bpp ? prev[x - bpp] : 0;
+                int v = src[x];
+                switch(f)
+                {
+                    case 0: break;
+                    case 1: v += a; break;
+                    case 2: v += b; break;
+                    case 3: v += (a + b) / 2; break;
+                    case 4:
+                    {
+                        const int pp = a + b - c, pa = std::abs(pp - a), pb = std::abs(pp - b), pc = std::abs(pp - c);
+                        v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+                        break;
+                    }
+                    default: why = "bad filter"; return false;
+                }
+                cur[x] = static_cast<uint8_t>(v);
+            }
+            for(uint32_t x = 0; x < pw; ++x)
+                for(int ch = 0; ch < channels; ++ch)
+                {
+                    const size_t i = (size_t)x * channels + ch;
+                    uint32_t v;
+                    if(depth == 16) v = (uint32_t)cur[2 * i] << 8 | cur[2 * i + 1];
+                    else if(depth == 8) v = cur[i];
+                    else v = (cur[i * depth / 8] >> (8 - depth - (i * depth) % 8)) & ((1u << depth) - 1u);
+                    px[(((size_t)(q.y0 + y * q.dy)) * w + q.x0 + x * q.dx) * channels + ch] = static_cast<uint16_t>(v);
+                }
+            std::swap(cur, prev);
+        }
+    }
+    // to 8 bits: 16-bit samples keep their high byte, grey levels below 8 bits scale to 0..255
+    auto to8 = [&](uint32_t v) -> uint8_t {
+        return static_cast<uint8_t>(depth == 16 ? v >> 8 : depth == 8 ? v : v * 255u / ((1u << depth) - 1u));
+    };
+    // tRNS: palette alphas, or the one grey / RGB value (at the stored depth) that is fully transparent
+    const bool key = (ctype == 0 && trns.size() >= 2) || (ctype == 2 && trns.size() >= 6);
+    uint32_t kv[3] = {0, 0, 0};
+    if(key)
+        for(int k = 0; k < (ctype == 0 ? 1 : 3); ++k) kv[k] = ((uint32_t)trns[2 * k] << 8 | trns[2 * k + 1]) & ((1u << depth) - 1u);
+    out.width = w;
+    out.height = h;
+    out.rgba.resize((size_t)w * h * 4);
+    for(size_t i = 0; i < (size_t)w * h; ++i)
+    {
+        const uint16_t *s = px.data() + i * channels;
+        uint8_t *o = out.rgba.data() + i * 4;
+        if(ctype == 3)
+        {
+            if(3 * (size_t)s[0] + 2 >= plte.size())
+            {
+                why = "palette index out of range";
+                return false;
+            }
+            o[0] = plte[3 * s[0]];
+            o[1] = plte[3 * s[0] + 1];
+            o[2] = plte[3 * s[0] + 2];
+            o[3] = s[0] < trns.size() ? trns[s[0]] : 255;
+        }
+        else if(ctype == 0 || ctype == 4)
+        {
+            o[0] = o[1] = o[2] = to8(s[0]);
+            o[3] = ctype == 4 ? to8(s[1]) : (key && s[0] == kv[0] ? 0 : 255);
+        }
+        else
+        {
+            for(int k = 0; k < 3; ++k) o[k] = to8(s[k]);
+            o[3] = ctype == 6 ? to8(s[3]) : (key && s[0] == kv[0] && s[1] == kv[1] && s[2] == kv[2] ? 0 : 255);
+        }
+    }
+    return true;
+}
+
 }// namespace
 
 //! the loaded scene: world-space vertices, triangles grouped into objects (one per OBJ shape / glTF primitive
@@ -362,8 +548,29 @@ struct vkrt_model
     std::vector<float> positions, normals, texcoords;
     std::vector<uint32_t> indices, tri_object, object_material;
     std::vector<vkrt_material> materials;
-    std::vector<std::string> object_names, material_names, textures;
-    uint32_t n_objects = 0;
+    std::vector<std::string> object_names, material_names;
+    std::vector<image_rgba8> images;
+    std::vector<vkrt_texture> textures;// views of `images`, filled when the model is complete
+    uint32_t n_objects = 0, n_skipped_textures = 0;
+
+    //! decode one image; returns 1 + its texture index, or 0 (counted as skipped) when it cannot be decoded
+    uint32_t add_image(const uint8_t *data, size_t size)
+    {
+        image_rgba8 img;
+        std::string why;
+        if(!decode_png(data, size, img, why))
+        {
+            ++n_skipped_textures;
+            return 0;
+        }
+        images.push_back(std::move(img));
+        return static_cast<uint32_t>(images.size());
+    }
+    void finish()
+    {
+        textures.clear();
+        for(const image_rgba8 &im: images) textures.push_back(vkrt_texture{im.width, im.height, im.rgba.data()});
+    }
 
     //! append one object's geometry (indices relative to its own vertices)
     void add_object(const std::string &name, const std::vector<float> &pos, const std::vector<float> &nrm,
@@ -550,9 +757,31 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
         }
     }
     auto model = std::make_unique<vkrt_model>();
+    std::map<std::string, uint32_t> tex_cache;
+    // a material with a diffuse texture gets a texture transform that flips v (wavefront_obj.cpp:100-111); it
+    // is applied here, to the coordinates of the faces that use it
+    std::vector<bool> flip_v;
     for(const obj_material &m: mtl)
     {
         vkrt_material r = default_material();
+        flip_v.push_back(!m.diffuse_tex.empty());
+        if(!m.diffuse_tex.empty())
+        {
+            auto it = tex_cache.find(m.diffuse_tex);
+            if(it == tex_cache.end())
+            {
+                uint32_t id = 0;
+                std::ifstream f(base + m.diffuse_tex, std::ios::binary);
+                if(f)
+                {
+                    const std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+                    id = model->add_image(reinterpret_cast<const uint8_t *>(bytes.data()), bytes.size());
+                }
+                else ++model->n_skipped_textures;
+                it = tex_cache.emplace(m.diffuse_tex, id).first;
+            }
+            r.color_texture = it->second;
+        }
         r.color[0] = m.diffuse[0];
         r.color[1] = m.diffuse[1];
         r.color[2] = m.diffuse[2];
@@ -566,8 +795,6 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
         r.ior = m.ior;
         model->materials.push_back(r);
         model->material_names.push_back(m.name);
-        if(!m.diffuse_tex.empty()) model->textures.push_back(m.diffuse_tex);
-        if(!m.normal_tex.empty()) model->textures.push_back(m.normal_tex);
     }
     if(model->materials.empty())
     {
@@ -579,6 +806,8 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
     {
         if(sh.corners.empty()) continue;
         const size_t n = sh.corners.size();
+        const uint32_t material = static_cast<uint32_t>(std::max(sh.first_material, 0));
+        const bool flip = material < flip_v.size() && flip_v[material];
         std::vector<float> pos(3 * n, 0.f), nrm, uv(2 * n, 0.f);
         std::vector<uint32_t> idx(n);
         if(has_normals) nrm.assign(3 * n, 0.f);
@@ -589,11 +818,13 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
             if(has_normals && c.vn >= 0)
                 for(int a = 0; a < 3; ++a) nrm[3 * i + a] = vn[3 * (size_t)c.vn + a];
             if(c.vt >= 0)
-                for(int a = 0; a < 2; ++a) uv[2 * i + a] = vt[2 * (size_t)c.vt + a];
+            {
+                uv[2 * i] = vt[2 * (size_t)c.vt];
+                uv[2 * i + 1] = flip ? -vt[2 * (size_t)c.vt + 1] : vt[2 * (size_t)c.vt + 1];
+            }
             idx[i] = static_cast<uint32_t>(i);
         }
         if(!has_normals) compute_vertex_normals(pos, idx, nrm);
-        const uint32_t material = static_cast<uint32_t>(std::max(sh.first_material, 0));
         model->add_object(sh.name, pos, nrm, uv, idx, material);
     }
     return model;
@@ -793,9 +1024,15 @@ mat4d node_matrix(const json &node)
     return m;
 }
 
-vkrt_material convert_material(const json &jm)
+template<class F>
+vkrt_material convert_material(const json &jm, F &&texture_of)
 {
     vkrt_material m = default_material();
+    const std::string &mode = jm["alphaMode"].str;
+    m.alpha_mode = mode == "MASK" ? VKRT_ALPHA_MASK : mode == "BLEND" ? VKRT_ALPHA_BLEND : VKRT_ALPHA_OPAQUE;
+    m.alpha_cutoff = static_cast<float>(jm["alphaCutoff"].number(0.5));
+    if(jm["pbrMetallicRoughness"]["baseColorTexture"].has("index"))
+        m.color_texture = texture_of(jm["pbrMetallicRoughness"]["baseColorTexture"]["index"].integer(-1));
     const json &pbr = jm["pbrMetallicRoughness"];
     for(int k = 0; k < 4; ++k) m.color[k] = static_cast<float>(pbr["baseColorFactor"][k].number(1.0));
     for(int k = 0; k < 3; ++k) m.emission[k] = static_cast<float>(jm["emissiveFactor"][k].number(0.0));
@@ -837,15 +1074,52 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
     gltf_file g = read_gltf(path);
     const json &doc = g.doc;
     auto model = std::make_unique<vkrt_model>();
+    // images decode once, on first use by a material
+    std::map<int, uint32_t> image_texture;
+    auto texture_of = [&](int tex) -> uint32_t {
+        const int src = doc["textures"][static_cast<size_t>(tex < 0 ? 0 : tex)]["source"].integer(-1);
+        if(tex < 0 || src < 0) return 0u;
+        auto it = image_texture.find(src);
+        if(it != image_texture.end()) return it->second;
+        const json &im = doc["images"][static_cast<size_t>(src)];
+        uint32_t id = 0;
+        if(im.has("bufferView"))
+        {
+            const json &bv = doc["bufferViews"][static_cast<size_t>(im["bufferView"].integer(0))];
+            const int bi = bv["buffer"].integer(-1);
+            const size_t o = static_cast<size_t>(bv["byteOffset"].number(0)), n = static_cast<size_t>(bv["byteLength"].number(0));
+            if(bi >= 0 && bi < static_cast<int>(g.buffers.size()) && o + n <= g.buffers[bi].size())
+                id = model->add_image(g.buffers[bi].data() + o, n);
+            else ++model->n_skipped_textures;
+        }
+        else if(im.has("uri"))
+        {
+            const std::string &uri = im["uri"].str;
+            std::vector<uint8_t> bytes;
+            if(uri.rfind("data:", 0) == 0)
+            {
+                const size_t comma = uri.find(',');
+                if(comma != std::string::npos) bytes = base64_decode(uri.substr(comma + 1));
+            }
+            else
+            {
+                std::ifstream f(directory_of(path) + uri, std::ios::binary);
+                if(f) bytes.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+            }
+            if(!bytes.empty()) id = model->add_image(bytes.data(), bytes.size());
+            else ++model->n_skipped_textures;
+        }
+        image_texture[src] = id;
+        return id;
+    };
     const json &mats = doc["materials"];
     for(size_t i = 0; i < mats.size(); ++i)
     {
-        model->materials.push_back(convert_material(mats[i]));
+        model->materials.push_back(convert_material(mats[i], texture_of));
         model->material_names.push_back(mats[i]["name"].str);
     }
     const uint32_t default_index = static_cast<uint32_t>(model->materials.size());
     bool default_used = false;
-    for(size_t i = 0; i < doc["images"].size(); ++i) model->textures.push_back(doc["images"][i]["uri"].str);
 
     // the scene's root nodes (the default scene, else the first, else every node that is no child)
     std::vector<int> roots;
@@ -990,6 +1264,7 @@ int vkrt_model_load(const char *path, vkrt_model **out)
             m->object_names.emplace_back();
             m->n_objects = 1;
         }
+        m->finish();
         g_model_error.clear();
         *out = m.release();
         return VKRT_OK;
@@ -1025,7 +1300,9 @@ int vkrt_model_get(const vkrt_model *m, vkrt_model_view *out)
     out->n_objects = m->n_objects;
     out->materials = m->materials.data();
     out->n_materials = static_cast<uint32_t>(m->materials.size());
+    out->textures = m->textures.data();
     out->n_textures = static_cast<uint32_t>(m->textures.size());
+    out->n_skipped_textures = m->n_skipped_textures;
     return VKRT_OK;
 }
 
@@ -1035,7 +1312,8 @@ int vkrt_model_pt_scene(const vkrt_model *m, float environment, vkrt_pt_scene *o
     const int rc = vkrt_model_get(m, &v);
     if(rc != VKRT_OK) return rc;
     *out = vkrt_pt_scene{v.positions, v.normals, v.n_vert, v.indices, v.n_tri, v.tri_object, v.object_material,
-                         v.n_objects, v.materials, v.n_materials, environment};
+                         v.n_objects, v.materials, v.n_materials, environment, v.texcoords, v.textures,
+                         v.n_textures};
     return VKRT_OK;
 }
 

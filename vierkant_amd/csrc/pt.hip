@@ -18,6 +18,7 @@
 // coalesced; queues hold pixel indices.
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <memory>
 #include <random>
 #include <vector>
@@ -68,6 +69,9 @@ __host__ __device__ inline float round_half(float x)
     return out;
 }
 
+//! f32tof16 / f16tof32: the value as binary16 stores it (round to nearest even, overflow to infinity)
+__device__ inline float half_value(float x) { return static_cast<float>(static_cast<_Float16>(x)); }
+
 struct gpu_material
 {
     float4 color;    // rgb, alpha
@@ -76,6 +80,16 @@ struct gpu_material
     float4 cc;       // clearcoat, clearcoat_roughness, sheen_roughness, diffuse_transmission
     float4 sheen;    // sheen color rgb, two_sided flag
     float4 dtc;      // diffuse transmission color rgb, -
+    uint4 tex;       // 1 + color texture (0: none), alpha mode, alpha cutoff bits, -
+};
+
+constexpr uint32_t MAX_TEX_LEVELS = 16;
+
+//! one texture's mip chain inside the shared texel array (RGBA8 packed in a uint32)
+struct tex_desc
+{
+    uint32_t width, height, levels, pad;
+    uint32_t offset[MAX_TEX_LEVELS];
 };
 
 struct pt_frame
@@ -89,6 +103,7 @@ struct pt_frame
     uint32_t cache_on, cache_depth, train_stride, train_offset, n_train_paths, train_max;
     // analytic lights: next-event estimation and the light bodies BSDF rays meet
     uint32_t n_lights, mis_mode, suppress_caustics;
+    float cone_spread;// atan(2 tan(fov / 2) / height) as a half (raypipeline.slang:102)
     vec3 eye, right, up, forward;
     float tan_half_x, tan_half_y, aperture, focal_distance;
 };
@@ -112,6 +127,7 @@ struct pt_buffers
     float4 *tvtx;           // training paths' vertices [path][depth] (VERTEX_F4 float4 each)
     float4 *trec, *ttgt;    // training records (4 float4) and targets
     float4 *shadow;         // next-event shadow rays of one depth (SHADOW_F4 float4 each), count in counter[6]
+    float2 *cone;           // ray cone per path: spread angle, width
 };
 
 struct pt_scene_dev
@@ -121,9 +137,93 @@ struct pt_scene_dev
     const float *object_tmax;
     const gpu_material *mats;
     const vkrt_light *lights;
+    const float2 *uv;         // texture coordinates per vertex
+    const tex_desc *tex;      // texture descriptors
+    const uint32_t *texels;   // RGBA8 mip levels of all textures
 };
 
 __device__ __forceinline__ vec3 ld3f(const float *p, uint32_t i) { return vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+
+// ---------------------------------------------------------------------------------------------
+// textures: RGBA8 UNORM, repeat addressing, bilinear within a level and linear between levels, the
+// sampler of model_loading.cpp:70 and Material.hpp:175-179 (texture_sampler_t defaults)
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float4 unpack_unorm8(uint32_t c)
+{
+    return make_float4(static_cast<float>(c & 0xFFu) / 255.0f, static_cast<float>((c >> 8) & 0xFFu) / 255.0f,
+                       static_cast<float>((c >> 16) & 0xFFu) / 255.0f, static_cast<float>(c >> 24) / 255.0f);
+}
+
+__device__ __forceinline__ int wrap_repeat(int x, int n)
+{
+    const int r = x % n;
+    return r < 0 ? r + n : r;
+}
+
+__device__ __forceinline__ float4 lerp4(const float4 &a, const float4 &b, float t)
+{
+    return make_float4(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z), a.w + t * (b.w - a.w));
+}
+
+__device__ float4 sample_level(const tex_desc &d, const uint32_t *__restrict__ texels, uint32_t level, float u, float v)
+{
+    const int w = static_cast<int>(max(1u, d.width >> level)), h = static_cast<int>(max(1u, d.height >> level));
+    const float fx = u * static_cast<float>(w) - 0.5f, fy = v * static_cast<float>(h) - 0.5f;
+    const float x0f = floorf(fx), y0f = floorf(fy);
+    const float ax = fx - x0f, ay = fy - y0f;
+    // texel coordinates taken modulo the level size in 64-bit so that far-away tiles wrap exactly
+    const long long xl = static_cast<long long>(x0f), yl = static_cast<long long>(y0f);
+    const int x0 = static_cast<int>(((xl % w) + w) % w), y0 = static_cast<int>(((yl % h) + h) % h);
+    const int x1 = wrap_repeat(x0 + 1, w), y1 = wrap_repeat(y0 + 1, h);
+    const uint32_t *t = texels + d.offset[level];
+    const float4 c00 = unpack_unorm8(t[y0 * w + x0]), c10 = unpack_unorm8(t[y0 * w + x1]);
+    const float4 c01 = unpack_unorm8(t[y1 * w + x0]), c11 = unpack_unorm8(t[y1 * w + x1]);
+    return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
+}
+
+//! SampleLevel(uv, lambda): lambda clamped to the chain (a NaN or negative level is level 0)
+__device__ float4 sample_texture(const tex_desc &d, const uint32_t *__restrict__ texels, float u, float v, float lambda)
+{
+    const float top = static_cast<float>(d.levels - 1u);
+    if(!(lambda > 0.0f)) lambda = 0.0f;
+    if(lambda > top) lambda = top;
+    if(!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) u = v = 0.0f;// no fraction is left there; keeps texel indices in range
+    const uint32_t l0 = static_cast<uint32_t>(lambda);
+    const float f = lambda - static_cast<float>(l0);
+    const float4 a = sample_level(d, texels, l0, u, v);
+    if(f == 0.0f) return a;
+    return lerp4(a, sample_level(d, texels, min(l0 + 1u, d.levels - 1u), u, v), f);
+}
+
+//! the any-hit alpha test of masked materials inside traversal: the candidate's color alpha times its
+//! texture's alpha at level 0, against the cutoff. Closest-hit rays (raypipeline.slang:686-720) see
+//! disable_material's stand-in color (0.8, 1); shadow rays (visibility_test.slang:136-155) the material as it is.
+struct alpha_filter
+{
+    static constexpr bool active = true;
+    const uint32_t *idx;
+    const float2 *uv;
+    const gpu_material *mats;
+    const tex_desc *tex;
+    const uint32_t *texels;
+    bool disable_material;
+
+    __device__ bool operator()(const float4 &rec, float bu, float bv) const
+    {
+        const gpu_material &m = mats[__float_as_uint(rec.z)];
+        if(disable_material) return !(1.0f < __uint_as_float(m.tex.z));
+        float a = m.color.w;
+        if(m.tex.x)
+        {
+            const uint32_t prim = __float_as_uint(rec.y);
+            const float2 t0 = uv[idx[3 * prim]], t1 = uv[idx[3 * prim + 1]], t2 = uv[idx[3 * prim + 2]];
+            const float b0 = 1.0f - bu - bv;
+            const float u = (t0.x * b0 + t1.x * bu) + t2.x * bv, v = (t0.y * b0 + t1.y * bu) + t2.y * bv;
+            a = a * sample_texture(tex[m.tex.x - 1u], texels, u, v, 0.0f).w;
+        }
+        return !(a < __uint_as_float(m.tex.z));
+    }
+};
 __device__ __forceinline__ vec3 xyz(const float4 &f) { return vec3{f.x, f.y, f.z}; }
 __device__ __forceinline__ float frac(float x) { return x - floorf(x); }
 
@@ -192,6 +292,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
     b.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     b.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     b.iors[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+    b.cone[p] = make_float2(f.cone_spread, 0.f);
     if(n_train == 0u) b.queue[0][p] = p;
     else if(st.y & F_TRAIN) b.train_queue[(p - f.train_offset) / f.train_stride] = p;
     else
@@ -213,7 +314,8 @@ __global__ void k_pt_reset_depth(uint32_t *c, int next)
 //! the closest hit of path p's current ray; back faces are culled while the path is in air
 __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                            int32_t root, bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
-                                           int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem)
+                                           int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem,
+                                           const alpha_filter &flt)
 {
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
@@ -223,9 +325,9 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
         const bool in_air = (b.state[p].y & F_MEDIA_MASK) == 0u;
         // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
         if(wide)
-            traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air);
+            traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air, flt);
         else
-            traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem, t, u, v, prim, in_air);
+            traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem, t, u, v, prim, in_air, flt);
     }
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
 }
@@ -233,12 +335,12 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                        const float4 *__restrict__ tris, int32_t root, bool empty,
                                                        const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
-                                                       bool wide, pt_buffers b, int in)
+                                                       bool wide, pt_buffers b, int in, alpha_filter flt)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
     if(q >= b.counter[in]) return;
-    trace_path(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x);
+    trace_path(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x, flt);
 }
 
 //! eval_distant_lights (raypipeline.slang:728-744): sun discs seen by a ray that left the scene
@@ -305,6 +407,7 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
     const float seg_tmin = ro.w, seg_tmax = rd.w, seg_prev_pdf = prev_pdf;
     const bool seg_caustic = (flags & F_CAUSTIC) != 0u;
     vec3 radiance = xyz(b.rad[p]);
+    float2 cone = b.cone[p];// ray cone: spread angle, width (half precision values)
     float4 iors = b.iors[p];
     float ior_stack[MEDIA_STACK] = {iors.x, iors.y, iors.z, iors.w};
     uint32_t media_index = flags & F_MEDIA_MASK;
@@ -356,9 +459,30 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         const vec3 V = -dir;
         ro = make_float4(position.x, position.y, position.z, ro.w);
 
+        vec3 base = xyz(gm.color);
+        float alpha = gm.color.w;
+        if(gm.tex.x)
+        {
+            // the base color texture at the ray cone's level of detail (raypipeline.slang:300-321, 526-531):
+            // lambda = 0.5 log2(uv area / world area) + log2|cone width| + 0.5 log2(w h) - log2(N.V)
+            const float2 t0 = s.uv[i0], t1 = s.uv[i1], t2 = s.uv[i2];
+            const float tu = (t0.x * bx + t1.x * by) + t2.x * bz, tv = (t0.y * bx + t1.y * by) + t2.y * bz;
+            const float p_a = length3(vierkant::rt::cross(v1 - v0, v2 - v0));
+            const float t_a = fabsf((t1.x - t0.x) * (t2.y - t0.y) - (t2.x - t0.x) * (t1.y - t0.y));
+            const tex_desc &td = s.tex[gm.tex.x - 1u];
+            float lambda = 0.5f * log2f(t_a / p_a);
+            lambda += log2f(fabsf(cone.y));
+            lambda += 0.5f * log2f(static_cast<float>(td.width) * static_cast<float>(td.height));
+            lambda -= log2f(fabsf(dot(V, normal)));
+            const float4 c = sample_texture(td, s.texels, tu, tv, lambda);
+            base = vec3{base.x * c.x, base.y * c.y, base.z * c.z};
+            alpha = alpha * c.w;
+        }
+        // propagate(cone, 0, RayTCurrent()) (raypipeline.slang:280-286, :608), stored as halves
+        cone.y = half_value(cone.y + cone.x * __uint_as_float(h.x));
         material m;
-        m.color = f.disable_material ? splat(0.8f) : xyz(gm.color);
-        m.alpha = f.disable_material ? 1.0f : gm.color.w;
+        m.color = f.disable_material ? splat(0.8f) : base;
+        m.alpha = f.disable_material ? 1.0f : alpha;
         m.emission = xyz(gm.emission);
         m.emission_strength = gm.emission.w;
         m.metalness = gm.mrti.x;
@@ -528,6 +652,7 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         b.ray_d[p] = make_float4(dir.x, dir.y, dir.z, rd.w);
         b.beta[p] = make_float4(beta.x, beta.y, beta.z, prev_pdf);
         b.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.f);
+        b.cone[p] = cone;
         b.iors[p] = make_float4(ior_stack[0], ior_stack[1], ior_stack[2], ior_stack[3]);
     }
     else
@@ -682,7 +807,7 @@ constexpr int TR_STACK = 64;
 __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                      const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide, bool empty,
                                      const float4 *__restrict__ tris, const gpu_material *__restrict__ mats,
-                                     const ray_setup &r, bool cull, int32_t *lds)
+                                     const ray_setup &r, bool cull, int32_t *lds, const alpha_filter &flt)
 {
     int32_t spill[TR_STACK - PT_LDS];
     lane_stack<PT_BLOCK, PT_LDS> st{lds, spill};
@@ -749,6 +874,8 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
                     const gpu_material m = mats[mid];
                     if(!(m.mrti.z > 0.0f))
                     {
+                        // alpha-cutout surfaces let the shadow ray through (visibility_test.slang:145-150)
+                        if(filtered_out(flt, p2, u, v)) continue;
                         blocked = true;
                         break;
                     }
@@ -810,13 +937,13 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_no
                                                                const float4 *__restrict__ tris,
                                                                const gpu_material *__restrict__ mats,
                                                                const float4 *__restrict__ rays, uint32_t n, bool cull,
-                                                               float4 *__restrict__ out)
+                                                               float4 *__restrict__ out, alpha_filter flt)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
     if(i >= n) return;
     out[i] = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, setup_ray(rays, i), cull,
-                                smem + threadIdx.x);
+                                smem + threadIdx.x, flt);
 }
 
 //! trace one shadow record and add its next-event estimate to the pixel (and to its training vertex)
@@ -824,11 +951,12 @@ __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ no
                                             const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide,
                                             bool empty, const float4 *__restrict__ tris,
                                             const gpu_material *__restrict__ mats, const pt_frame &f,
-                                            const pt_buffers &b, const float4 (&rec)[SHADOW_F4], int32_t *smem)
+                                            const pt_buffers &b, const float4 (&rec)[SHADOW_F4], int32_t *smem,
+                                            const alpha_filter &flt)
 {
     const float4 o = rec[0], d = rec[1];
     const float4 T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d),
-                                        false, smem);
+                                        false, smem, flt);
     if(!(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)) return;
     const float4 lr = rec[2], fc = rec[3], bp = rec[4], id = rec[5];
     vec3 nee = vec3{T.x, T.y, T.z} * lr.w;
@@ -856,7 +984,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__r
                                                         const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
                                                         bool wide, bool empty, const float4 *__restrict__ tris,
                                                         const gpu_material *__restrict__ mats, pt_frame f,
-                                                        pt_buffers b)
+                                                        pt_buffers b, alpha_filter flt)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
@@ -864,7 +992,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__r
     const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
     float4 r[SHADOW_F4];
     for(uint32_t k = 0; k < SHADOW_F4; ++k) r[k] = rec[k];
-    shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x);
+    shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x, flt);
 }
 
 //! the cache mode's training paths, one lane per path through every depth: trace, shade and the shadow ray
@@ -882,17 +1010,20 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node
     const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
     if(q >= b.counter[0]) return;
     const uint32_t p = b.queue[0][q];
+    const alpha_filter flt{s.idx, s.uv, s.mats, s.tex, s.texels, f.disable_material != 0u};
+    const alpha_filter shadow_flt{s.idx, s.uv, s.mats, s.tex, s.texels, false};
     uint32_t segments = 0;
     for(uint32_t depth = 0; depth < f.max_depth; ++depth)
     {
-        trace_path(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x);
+        trace_path(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x, flt);
         bool alive = false, to_cache = false, need_shadow = false;
         float4 shadow_rec[SHADOW_F4];
         float4 qfeat[QUERY_F4];
         shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
         ++segments;
         if(LIGHTS && need_shadow)
-            shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec, smem + threadIdx.x);
+            shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec, smem + threadIdx.x,
+                        shadow_flt);
         if(!alive) break;
     }
     atomicAdd(reinterpret_cast<unsigned long long *>(&b.counter[2]), (unsigned long long)segments);
@@ -912,15 +1043,13 @@ struct tonemap_params
     float exposure, inv_gamma;
 };
 
-//! the stored value of one R16G16B16A16_SFLOAT channel: round to nearest half, ties to even
-__device__ inline float to_half_value(float v) { return static_cast<float>(static_cast<_Float16>(v)); }
 
 __device__ inline float4 tonemap_texel(float4 c, const tonemap_params &tm)
 {
     if(!tm.on) return c;
-    return make_float4(to_half_value(tonemap_channel(c.x, tm.exposure, tm.inv_gamma)),
-                       to_half_value(tonemap_channel(c.y, tm.exposure, tm.inv_gamma)),
-                       to_half_value(tonemap_channel(c.z, tm.exposure, tm.inv_gamma)), to_half_value(c.w));
+    return make_float4(half_value(tonemap_channel(c.x, tm.exposure, tm.inv_gamma)),
+                       half_value(tonemap_channel(c.y, tm.exposure, tm.inv_gamma)),
+                       half_value(tonemap_channel(c.z, tm.exposure, tm.inv_gamma)), half_value(c.w));
 }
 
 //! one pass of denoise.slang (16 x 16 pixels per workgroup, as its numthreads); the last pass of a chain
@@ -984,7 +1113,8 @@ __global__ void k_pt_tag(float4 *__restrict__ tris, uint32_t n, const uint32_t *
     const uint32_t prim = __float_as_uint(r.y);
     const uint32_t mat = object_material[tri_object[prim]];
     r.z = __uint_as_float(mat);
-    r.w = __uint_as_float(mats[mat].sheen.w != 0.f ? TRI_TWO_SIDED : 0u);
+    r.w = __uint_as_float((mats[mat].sheen.w != 0.f ? TRI_TWO_SIDED : 0u) |
+                          (mats[mat].tex.y == VKRT_ALPHA_MASK ? TRI_ALPHA_MASK : 0u));
     tris[3 * (size_t)j + 2] = r;
 }
 
@@ -1003,8 +1133,10 @@ struct vkrt_pt
     uint32_t n_tri = 0, n_vert = 0, n_objects = 0, n_materials = 0;
     float environment = 1.f;
     vkrt::device_buffer pos, nrm, idx, tri_object, object_material, object_tmax, mats;
+    // texture coordinates, texture descriptors and their texels (all mip levels)
+    vkrt::device_buffer uv, tex_desc_buf, texels;
     uint32_t width = 0, height = 0;
-    vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels;
+    vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels, cone;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the cache mode's training paths: their own queues, traced on a second stream beside the main paths
     hipStream_t train_stream = nullptr;
@@ -1065,6 +1197,7 @@ static void resize_storage(vkrt_pt *pt, uint32_t w, uint32_t h)
     pt->rad.reserve(n * 16);
     pt->state.reserve(n * 16);
     pt->iors.reserve(n * 16);
+    pt->cone.reserve(n * 8);
     pt->acc.reserve(n * 16);
     pt->info.reserve(n * 16);
     pt->queue0.reserve(n * 4);
@@ -1087,6 +1220,7 @@ static pt_buffers buffers(vkrt_pt *pt)
     b.rad = pt->rad.as<float4>();
     b.state = pt->state.as<uint4>();
     b.iors = pt->iors.as<float4>();
+    b.cone = pt->cone.as<float2>();
     b.acc = pt->acc.as<float4>();
     b.info = pt->info.as<uint4>();
     b.queue[0] = pt->queue0.as<uint32_t>();
@@ -1096,6 +1230,28 @@ static pt_buffers buffers(vkrt_pt *pt)
     b.qrec = b.tvtx = b.trec = b.ttgt = b.shadow = nullptr;
     b.train_queue = nullptr;
     return b;
+}
+
+//! the scene's device arrays as the kernels take them
+static pt_scene_dev scene_dev(vkrt_pt *pt)
+{
+    return pt_scene_dev{pt->pos.as<float>(), pt->nrm.as<float>(), pt->idx.as<uint32_t>(),
+                        pt->tri_object.as<uint32_t>(), pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(),
+                        pt->mats.as<gpu_material>(), pt->lights.as<vkrt_light>(), pt->uv.as<float2>(),
+                        pt->tex_desc_buf.as<tex_desc>(), pt->texels.as<uint32_t>()};
+}
+
+//! f32tof16 on the host: round to nearest even at the binade's spacing of halves (2^-24 below 2^-14),
+//! overflow to infinity
+static float half_round_host(float x)
+{
+    if(!std::isfinite(x) || x == 0.f) return x;
+    int k;
+    std::frexp(x, &k);
+    const int q = (k - 1 >= -14) ? (k - 1) - 10 : -24;
+    const double r = std::ldexp(std::nearbyint(std::ldexp((double)x, -q)), q);
+    if(std::fabs(r) > 65504.0) return std::copysign(INFINITY, x);
+    return static_cast<float>(r);
 }
 
 static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
@@ -1137,6 +1293,8 @@ static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     f.tan_half_x = (float)(th * (double)pt->width / (double)pt->height);
     f.aperture = cam.aperture;
     f.focal_distance = cam.focal_distance > 0.f ? cam.focal_distance : 1.f;
+    // the ray cone's spread angle (raypipeline.slang:102), kept as a half like the payload's packed cone
+    f.cone_spread = half_round_host(std::atan(2.0f * std::tan(cam.fov_y / 2.0f) / static_cast<float>(pt->height)));
     f.cache_on = pt->rc ? 1u : 0u;
     f.cache_depth = pt->cache_cfg.cache_depth;
     f.train_stride = std::max<uint32_t>(1u, pt->cache_cfg.train_stride);
@@ -1172,9 +1330,9 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     hipStream_t s = pt->ctx->stream;
     pt_frame f = make_frame(pt, cam, seed);
     pt_buffers b = buffers(pt);
-    pt_scene_dev sc{pt->pos.as<float>(), pt->nrm.as<float>(), pt->idx.as<uint32_t>(), pt->tri_object.as<uint32_t>(),
-                    pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(), pt->mats.as<gpu_material>(),
-                    pt->lights.as<vkrt_light>()};
+    const pt_scene_dev sc = scene_dev(pt);
+    const alpha_filter flt{sc.idx, sc.uv, sc.mats, sc.tex, sc.texels, f.disable_material != 0u};
+    const alpha_filter shadow_flt{sc.idx, sc.uv, sc.mats, sc.tex, sc.texels, false};
     const uint32_t blocks = div_up(f.n_pix, PT_BLOCK);
     const vkrt_bvh *bvh = pt->bvh;
     if(f.cache_on)
@@ -1218,12 +1376,12 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t grid, uint32_t depth, int in) {
         k_pt_reset_depth<<<1, 1, 0, st>>>(bb.counter, in ^ 1);
         k_pt_trace<<<grid, PT_BLOCK, 0, st>>>(nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide,
-                                              bb, in);
+                                              bb, in, flt);
         if(f.n_lights)
         {
             k_pt_shade<true><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
             k_pt_shadow<<<grid, PT_BLOCK, 0, st>>>(nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty,
-                                                   tris, sc.mats, f, bb);
+                                                   tris, sc.mats, f, bb, shadow_flt);
         }
         else
             k_pt_shade<false><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
@@ -1441,6 +1599,65 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
             mats[m].sheen = make_float4(x.sheen_color[0], x.sheen_color[1], x.sheen_color[2], x.two_sided ? 1.f : 0.f);
             mats[m].dtc = make_float4(x.diffuse_transmission_color[0], x.diffuse_transmission_color[1],
                                       x.diffuse_transmission_color[2], 0.f);
+            VKRT_REQUIRE(x.color_texture <= sc->n_textures, "material texture index out of range");
+            VKRT_REQUIRE(x.alpha_mode <= VKRT_ALPHA_BLEND, "unknown alpha mode");
+            uint32_t cutoff_bits;
+            std::memcpy(&cutoff_bits, &x.alpha_cutoff, 4);
+            // BLEND is traced as OPAQUE; MASK drives the any-hit test
+            mats[m].tex = make_uint4(x.color_texture, x.alpha_mode == VKRT_ALPHA_MASK ? VKRT_ALPHA_MASK : 0u,
+                                     cutoff_bits, 0u);
+        }
+        // texture coordinates (zeros when the scene has none) and the textures' mip chains: each level halves
+        // the previous one (at least 1 texel) with a 2 x 2 box filter, coordinates clamped at odd edges,
+        // rounded to nearest ((sum + 2) / 4)
+        std::vector<float> uv(2 * (size_t)sc->n_vert, 0.f);
+        if(sc->texcoords) std::memcpy(uv.data(), sc->texcoords, uv.size() * 4);
+        VKRT_REQUIRE(sc->n_textures == 0 || sc->textures, "null texture array");
+        std::vector<tex_desc> descs(sc->n_textures);
+        std::vector<uint32_t> texels;
+        for(uint32_t t = 0; t < sc->n_textures; ++t)
+        {
+            const vkrt_texture &tx = sc->textures[t];
+            VKRT_REQUIRE(tx.rgba && tx.width > 0 && tx.height > 0 && tx.width <= 16384 && tx.height <= 16384,
+                         "textures need pixels and sizes in 1..16384");
+            tex_desc &d = descs[t];
+            d = tex_desc{};
+            d.width = tx.width;
+            d.height = tx.height;
+            uint32_t w = tx.width, h = tx.height;
+            d.offset[0] = static_cast<uint32_t>(texels.size());
+            for(size_t i = 0; i < (size_t)w * h; ++i)
+            {
+                uint32_t c;
+                std::memcpy(&c, tx.rgba + 4 * i, 4);
+                texels.push_back(c);
+            }
+            d.levels = 1;
+            while((w > 1 || h > 1) && d.levels < MAX_TEX_LEVELS)
+            {
+                const uint32_t pw = w, ph = h, src = d.offset[d.levels - 1];
+                w = std::max(1u, w / 2);
+                h = std::max(1u, h / 2);
+                d.offset[d.levels] = static_cast<uint32_t>(texels.size());
+                for(uint32_t y = 0; y < h; ++y)
+                    for(uint32_t x = 0; x < w; ++x)
+                    {
+                        const uint32_t xa = std::min(2 * x, pw - 1), xb = std::min(2 * x + 1, pw - 1);
+                        const uint32_t ya = std::min(2 * y, ph - 1), yb = std::min(2 * y + 1, ph - 1);
+                        const uint32_t q[4] = {texels[src + ya * pw + xa], texels[src + ya * pw + xb],
+                                               texels[src + yb * pw + xa], texels[src + yb * pw + xb]};
+                        uint32_t out = 0;
+                        for(int ch = 0; ch < 4; ++ch)
+                        {
+                            uint32_t sum = 2;
+                            for(uint32_t qq: q) sum += (qq >> (8 * ch)) & 0xFFu;
+                            out |= (sum / 4) << (8 * ch);
+                        }
+                        texels.push_back(out);
+                    }
+                ++d.levels;
+            }
+            VKRT_REQUIRE(texels.size() < (1ull << 31), "textures too large");
         }
         auto upload = [&](device_buffer &buf, const void *src, size_t bytes) {
             buf.reserve(bytes);
@@ -1453,6 +1670,9 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         upload(pt->object_material, sc->object_material, (size_t)sc->n_objects * 4);
         upload(pt->object_tmax, tmax.data(), tmax.size() * 4);
         upload(pt->mats, mats.data(), mats.size() * sizeof(gpu_material));
+        upload(pt->uv, uv.data(), uv.size() * 4);
+        upload(pt->tex_desc_buf, descs.data(), descs.size() * sizeof(tex_desc));
+        upload(pt->texels, texels.data(), texels.size() * 4);
         VKRT_HIP(hipStreamSynchronize(s));
         if(pt->bvh)
         {
@@ -1493,10 +1713,12 @@ int vkrt_pt_trace_transmittance(vkrt_pt *pt, const vkrt_ray *rays, uint32_t n, f
         const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
         float4 *dout = reinterpret_cast<float4 *>(output(ctx, 2, out4, (size_t)n * 4, flags));
         const vkrt_bvh *b = pt->bvh;
+        const pt_scene_dev sc = scene_dev(pt);
+        const alpha_filter flt{sc.idx, sc.uv, sc.mats, sc.tex, sc.texels, false};
         k_pt_transmittance<<<div_up(n, PT_BLOCK), PT_BLOCK, 0, ctx->stream>>>(
                 b->nodes.as<vkrt_bvh_node>(), b->root_ref, b->nodes4.as<vkrt_bvh4_node>(), b->root4, b->wide, b->empty,
                 b->tris.as<float4>(), pt->mats.as<gpu_material>(), reinterpret_cast<const float4 *>(dr), n,
-                (flags & VKRT_CULL_BACK) != 0, dout);
+                (flags & VKRT_CULL_BACK) != 0, dout, flt);
         VKRT_HIP(hipGetLastError());
         finish_output(ctx, 2, out4, (size_t)n * 4, flags);
         finish_call(ctx, flags);

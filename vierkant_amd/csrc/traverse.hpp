@@ -67,14 +67,33 @@ __device__ __forceinline__ void leaf_decode(int ref, uint32_t &first, uint32_t &
     count = (v & 7u) + 1;
 }
 
+//! the candidate filter of a traversal: a candidate that would become the closest hit and whose record is
+//! flagged TRI_ALPHA_MASK is kept only if filter(record's last float4, u, v) says so (the any-hit stage);
+//! keep_all, the default, compiles the test out
+struct keep_all
+{
+    static constexpr bool active = false;
+    __device__ bool operator()(const float4 &, float, float) const { return true; }
+};
+
+template<class F>
+__device__ __forceinline__ bool filtered_out(const F &filter, const float4 &rec, float u, float v)
+{
+    if constexpr(F::active)
+        return (__float_as_uint(rec.w) & TRI_ALPHA_MASK) && !filter(rec, u, v);
+    else
+        return false;
+}
+
 //! closest (ANY = false) or any (ANY = true) hit with tmin < t < tmax; ties on t go to the lowest
 //! original triangle index. `lds` points at this lane's column of a [LDS_DEPTH][BLOCK] int array.
 //! With `cull_back`, triangles hit from behind are skipped unless their record is flagged two-sided
 //! (bit 0 of the fourth word of the triangle's last float4).
-template<bool ANY, int BLOCK, int LDS_DEPTH>
+template<bool ANY, int BLOCK, int LDS_DEPTH, class F = keep_all>
 __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                          int32_t root, const ray_setup &r, int32_t *lds, float &best_t,
-                                         float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false)
+                                         float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false,
+                                         const F &filter = F{})
 {
     constexpr int SPILL = STACK_SIZE - LDS_DEPTH > 0 ? STACK_SIZE - LDS_DEPTH : 1;
     int32_t spill[SPILL];
@@ -125,7 +144,8 @@ __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes
                    t > r.tmin)
                 {
                     const uint32_t prim = __float_as_uint(p2.y);
-                    if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim))
+                    if((t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim)) &&
+                       !filtered_out(filter, p2, u, v))
                     {
                         best_t = t;
                         best_u = u;
@@ -144,10 +164,10 @@ __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes
 }
 
 //! one leaf's triangles (shared by both traversals); returns true when ANY and a hit was found
-template<bool ANY>
+template<bool ANY, class F = keep_all>
 __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32_t ref, const ray_setup &r,
                                           float &best_t, float &best_u, float &best_v, uint32_t &best_prim,
-                                          bool cull_back)
+                                          bool cull_back, const F &filter = F{})
 {
     uint32_t first, count;
     leaf_decode(ref, first, count);
@@ -162,7 +182,8 @@ __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32
            t > r.tmin)
         {
             const uint32_t prim = __float_as_uint(p2.y);
-            if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim))
+            if((t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM && prim < best_prim)) &&
+               !filtered_out(filter, p2, u, v))
             {
                 best_t = t;
                 best_u = u;
@@ -201,9 +222,10 @@ __device__ __forceinline__ void trav_init(trav_state &s, const ray_setup &r, int
 //! finished (stack empty, or the first hit when ANY). `lds` is this lane's column of a [LDS_DEPTH][BLOCK]
 //! array; `spill` holds entries past LDS_DEPTH. A child is entered under the same condition as in
 //! traverse(), and the closest-hit rule does not depend on the visiting order, so the result is the same.
-template<bool ANY, int BLOCK, int LDS_DEPTH>
+template<bool ANY, int BLOCK, int LDS_DEPTH, class F = keep_all>
 __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, const float4 *__restrict__ tris,
-                                      trav_state &s, int32_t *lds, int32_t *spill, bool cull_back)
+                                      trav_state &s, int32_t *lds, int32_t *spill, bool cull_back,
+                                      const F &filter = F{})
 {
     const ray_setup &r = s.r;
     if(s.ref >= 0)
@@ -256,7 +278,7 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
             return false;
         }
     }
-    else if(test_leaf<ANY>(tris, s.ref, r, s.best_t, s.best_u, s.best_v, s.best_prim, cull_back))
+    else if(test_leaf<ANY, F>(tris, s.ref, r, s.best_t, s.best_u, s.best_v, s.best_prim, cull_back, filter))
         return true;
     if(s.sp == 0) return true;
     --s.sp;
@@ -325,15 +347,16 @@ struct lane_stack
 constexpr int spill_entries(int lds_depth) { return STACK_SIZE - lds_depth > 0 ? STACK_SIZE - lds_depth : 1; }
 
 //! the whole 4-wide walk for one ray (see step4)
-template<bool ANY, int BLOCK, int LDS_DEPTH>
+template<bool ANY, int BLOCK, int LDS_DEPTH, class F = keep_all>
 __device__ __forceinline__ bool traverse4(const vkrt_bvh4_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                           int32_t root, const ray_setup &r, int32_t *lds, float &best_t,
-                                          float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false)
+                                          float &best_u, float &best_v, uint32_t &best_prim, bool cull_back = false,
+                                          const F &filter = F{})
 {
     int32_t spill[spill_entries(LDS_DEPTH)];
     trav_state s;
     trav_init(s, r, root);
-    while(!step4<ANY, BLOCK, LDS_DEPTH>(nodes, tris, s, lds, spill, cull_back)) {}
+    while(!step4<ANY, BLOCK, LDS_DEPTH, F>(nodes, tris, s, lds, spill, cull_back, filter)) {}
     best_t = s.best_t;
     best_u = s.best_u;
     best_v = s.best_v;
