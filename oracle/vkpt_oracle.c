@@ -165,7 +165,7 @@ typedef struct
     int two_sided;
     float alpha;        /* color.a */
     uint32_t tex;       /* base color texture + 1, 0 = none */
-    int mask;           /* alpha mode MASK: the any-hit cutout test */
+    int alpha_mode;     /* 0 opaque, 1 MASK (cutoff), 2 BLEND (uniform draw): the any-hit test */
     float alpha_cutoff;
 } mat_t;
 
@@ -471,15 +471,18 @@ typedef struct
 } alpha_ctx;
 
 /* 1 when the candidate is kept: color.a (times the base color texture's alpha at level 0, at the hit's
- * interpolated coordinates) is not below the cutoff. disable_material stands the color in by (0.8, 1) for
- * closest-hit rays; the shadow-ray test reads the material as it is. */
-static int alpha_keep(const alpha_ctx *ac, uint32_t prim, float bu, float bv, int disable_material)
+ * interpolated coordinates) is not below the cutoff (MASK) or a uniform draw (BLEND). disable_material stands
+ * the color in by (0.8, 1) for closest-hit rays, which draw once per trace from a copy of the path's state
+ * (blend_xi); the shadow-ray test reads the material as it is and draws per candidate, here rnd() of a copy
+ * of xxhash32(blend_seed, triangle), so that no traversal order enters (per_candidate). */
+static int alpha_keep(const alpha_ctx *ac, uint32_t prim, float bu, float bv, int disable_material, float blend_xi,
+                      uint32_t blend_seed, int per_candidate)
 {
     const mat_t *m = &ac->mats[ac->mat_of_prim[prim]];
-    if(!m->mask) return 1;
-    if(disable_material) return !(1.0f < m->alpha_cutoff);
+    if(!m->alpha_mode) return 1;
     float a = m->alpha;
-    if(m->tex)
+    if(disable_material) a = 1.0f;
+    else if(m->tex)
     {
         const uint32_t *ix = ac->indices + 3 * (size_t)prim;
         const float *t0 = ac->uv + 2 * (size_t)ix[0], *t1 = ac->uv + 2 * (size_t)ix[1], *t2 = ac->uv + 2 * (size_t)ix[2];
@@ -489,7 +492,13 @@ static int alpha_keep(const alpha_ctx *ac, uint32_t prim, float bu, float bv, in
         otex_sample(&ac->tex[m->tex - 1], u, v, 0.0f, c);
         a = a * c[3];
     }
-    return !(a < m->alpha_cutoff);
+    if(m->alpha_mode == 1) return !(a < m->alpha_cutoff);
+    if(per_candidate)
+    {
+        uint32_t st = orc_xxhash32(blend_seed, prim);
+        blend_xi = rnd(&st);
+    }
+    return !(a < blend_xi);
 }
 
 /* ---- a CPU BVH with exact closest-hit semantics ----------------------------------------------- */
@@ -602,8 +611,8 @@ static int box_hit(const cnode *nd, const double o[3], const double inv[3], doub
 
 /* closest hit with tmin < t < tmax, lowest index on ties, back faces culled when `cull`; masked candidates
  * whose alpha falls below their cutoff are ignored (any-hit IgnoreHit) */
-static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int cull, int disable_material, float *bt,
-                        float *bu, float *bv)
+static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int cull, int disable_material,
+                        float blend_xi, float *bt, float *bu, float *bv)
 {
     double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
     uint32_t best = 0xFFFFFFFFu;
@@ -633,7 +642,7 @@ static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int c
                 if(!(t > tmin)) continue;
                 if(t < best_t || (t == best_t && best != 0xFFFFFFFFu && prim < best))
                 {
-                    if(!alpha_keep(b->ac, prim, u, v, disable_material)) continue;
+                    if(!alpha_keep(b->ac, prim, u, v, disable_material, blend_xi, 0u, 0)) continue;
                     best_t = t;
                     *bu = u;
                     *bv = v;
@@ -661,7 +670,8 @@ typedef struct
     uint32_t prim;
 } thit_t;
 
-static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim, v3 o, v3 d, float tmin, float tmax)
+static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim, v3 o, v3 d, float tmin, float tmax,
+                   uint32_t blend_seed)
 {
     double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
     static __thread thit_t hits[ORC_MAX_TRANSMISSIVE];
@@ -687,7 +697,7 @@ static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim
             if(!(mats[mat_of_prim[prim]].trans > 0.0f))
             {
                 /* alpha-cutout surfaces let the shadow ray through (visibility_test.slang:145-150) */
-                if(!alpha_keep(b->ac, prim, u, v, 0)) continue;
+                if(!alpha_keep(b->ac, prim, u, v, 0, 0.0f, blend_seed, 1)) continue;
                 return V3(0, 0, 0);
             }
             if(nh == ORC_MAX_TRANSMISSIVE) abort();
@@ -1096,8 +1106,11 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
             const float seg_tmax = tmax, seg_prev = prev_pdf;
             const int seg_caustic = caustic;
             float t = 0.f, bu = 0.f, bv = 0.f;
-            uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, (int)pp->disable_material, &t,
-                                    &bu, &bv);
+            /* BLEND candidates compare against rnd() of a copy of payload.rng_state (raypipeline.slang:693) */
+            uint32_t peek = rng;
+            const float blend_xi = rnd(&peek);
+            uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, (int)pp->disable_material,
+                                    blend_xi, &t, &bu, &bv);
             int stop = 0, media_op = 0;
             float next_ior = 1.f;
             uint32_t obj = 0xFFFFu;
@@ -1191,7 +1204,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                     if(ls.radiance.x > 0.0f || ls.radiance.y > 0.0f || ls.radiance.z > 0.0f)
                     {
                         v3 so = add(pos, scl(nrm, EPS));
-                        v3 T = transmit(&j->bvh, j->mats, j->mat_of_prim, so, ls.dir, 0.0f, ls.dist);
+                        v3 T = transmit(&j->bvh, j->mats, j->mat_of_prim, so, ls.dir, 0.0f, ls.dist, lr);
                         if(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)
                         {
                             float bpdf = 0.0f;
@@ -1329,7 +1342,7 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         memcpy(&tex, &x[25], 4);
         memcpy(&mode, &x[26], 4);
         mats[m].tex = tex <= sc->n_textures ? tex : 0;
-        mats[m].mask = mode == 1; /* BLEND traces as OPAQUE */
+        mats[m].alpha_mode = mode <= 2 ? (int)mode : 0;
         mats[m].alpha_cutoff = x[27];
     }
     for(uint32_t t = 0; t < n; ++t)

@@ -1,6 +1,6 @@
 """GPU textured materials: base color textures at the ray cone's level of detail (raypipeline.slang:280-321,
-526-531, 608), the alpha-mask any-hit stages of camera and shadow rays (raypipeline.slang:686-720,
-visibility_test.slang:136-155), against the CPU integrator and the geometry-derived pins of
+526-531, 608), the alpha-mask and alpha-blend any-hit stages of camera and shadow rays (raypipeline.slang:
+686-720, visibility_test.slang:136-155), against the CPU integrator and the geometry-derived pins of
 test_texture_oracle, and a textured model loaded from OBJ + MTL + PNG."""
 import numpy as np
 import pytest
@@ -70,9 +70,26 @@ def test_gpu_ray_cone_level_of_detail(ctx):
     assert np.abs(got - want).max() < 1e-4
 
 
+def test_gpu_alpha_blend_camera_rays(ctx):
+    """a pixel's camera ray keeps the BLEND quad exactly when the draw from a copy of its path state allows,
+    as in the CPU integrator"""
+    seed = 5
+
+    def render(sc, with_quad):
+        if not with_quad:
+            sc["materials"][1]["color"][3] = 0.0
+        return gpu_render(ctx, sc, [], T.overhead_camera(), T.SIZE, T.SIZE, 1, 2, seed)
+    T.check_blend_camera_rays(render, oracle_seed(seed))
+
+
+def test_gpu_alpha_blend_shadow_rays(ctx):
+    """each BLEND layer passes a shadow ray with probability 1 - alpha, independently"""
+    T.check_blend_shadow_rays(lambda sc: gpu_render(ctx, sc, [T.DOWN], T.BLEND_CAM, 16, 16, 256, 2, 3))
+
+
 def textured_room(seed=0):
     """the room scene with planar texture coordinates, two textures (one with a hole pattern in its alpha)
-    on half of its materials, and a few of those masked"""
+    on half of its materials, a few of those masked and a few blended"""
     sc = W.room_scene()
     p = sc["positions"]
     sc["texcoords"] = np.stack([0.37 * p[:, 0] + 0.11 * p[:, 1], 0.41 * p[:, 2] + 0.23 * p[:, 1]], -1).astype(np.float32)
@@ -89,13 +106,16 @@ def textured_room(seed=0):
         if k % 6 == 2:
             mats[k]["alpha_mode"] = vk.ALPHA_MASK
             mats[k]["alpha_cutoff"] = 0.5
+        elif k % 6 == 4:
+            mats[k]["alpha_mode"] = vk.ALPHA_BLEND
+            mats[k]["color"][3] = 0.7
     sc["materials"] = mats
     return sc
 
 
 def test_gpu_textured_room_matches_cpu_integrator(ctx):
-    """the textured, partly masked room with its lights, 160x90, 4 spp, depth 8: per-pixel radiance equals
-    the CPU integrator up to transcendental ulps (log2 in the level of detail among them)"""
+    """the textured, partly masked and blended room with its lights, 160x90, 4 spp, depth 8: per-pixel
+    radiance equals the CPU integrator up to transcendental ulps (log2 in the level of detail among them)"""
     sc, cam, lights = textured_room(), W.room_camera(), W.room_lights()
     w, h, spp = 160, 90, 4
     got = gpu_render(ctx, sc, lights, cam, w, h, spp, 8, 0)

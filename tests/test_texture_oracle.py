@@ -454,3 +454,132 @@ def test_ray_cone_level_of_detail():
     """the mirror's texture seen through the slab is sampled at the cone's level of detail; seen directly
     (cone width 0) at level 0"""
     check_lod(oracle_lod)
+
+
+def test_gltf_base_color_texture_alpha_mode_and_texture_transform(tmp_path):
+    """a glTF material with an embedded PNG base color texture, MASK mode with its cutoff, and
+    KHR_texture_transform (offset, rotation about -z, scale: gltf.cpp:162-194) folded into the coordinates:
+    u' = cos sx u + sin sy v + ox, v' = -sin sx u + cos sy v + oy"""
+    import base64
+    import json
+    from PIL import Image
+    import io
+    rng = np.random.default_rng(9)
+    img = rng.integers(0, 256, (5, 7, 4), dtype=np.uint8)
+    buf = io.BytesIO()
+    Image.fromarray(img, "RGBA").save(buf, format="PNG")
+    png = buf.getvalue()
+    pos = np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    uv = np.float32([[0.1, 0.2], [0.9, 0.3], [0.4, 0.8]])
+    data = pos.tobytes() + uv.tobytes() + np.uint16([0, 1, 2, 0]).tobytes() + png
+    data += b"\0" * (-len(data) % 4)
+    off, rot, scale = [0.25, -0.5], 0.7, [2.0, 3.0]
+    doc = {"asset": {"version": "2.0"}, "scenes": [{"nodes": [0, 1]}],
+           "nodes": [{"mesh": 0}, {"mesh": 1}],
+           "meshes": [{"primitives": [{"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "indices": 2, "material": 0}]},
+                      {"primitives": [{"attributes": {"POSITION": 0, "TEXCOORD_0": 1}, "indices": 2, "material": 1}]}],
+           "materials": [{"alphaMode": "MASK", "alphaCutoff": 0.3,
+                          "pbrMetallicRoughness": {"baseColorTexture": {
+                              "index": 0, "extensions": {"KHR_texture_transform": {
+                                  "offset": off, "rotation": rot, "scale": scale}}}}},
+                         {"alphaMode": "BLEND", "pbrMetallicRoughness": {"baseColorTexture": {"index": 0}}}],
+           "textures": [{"source": 0}], "images": [{"bufferView": 3, "mimeType": "image/png"}],
+           "buffers": [{"byteLength": len(data),
+                        "uri": "data:application/octet-stream;base64," + base64.b64encode(data).decode()}],
+           "bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": 36},
+                           {"buffer": 0, "byteOffset": 36, "byteLength": 24},
+                           {"buffer": 0, "byteOffset": 60, "byteLength": 6},
+                           {"buffer": 0, "byteOffset": 68, "byteLength": len(png)}],
+           "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"},
+                         {"bufferView": 1, "componentType": 5126, "count": 3, "type": "VEC2"},
+                         {"bufferView": 2, "componentType": 5123, "count": 3, "type": "SCALAR"}]}
+    (tmp_path / "t.gltf").write_text(json.dumps(doc))
+    m = vk.load_model(str(tmp_path / "t.gltf"))
+    assert len(m["textures"]) == 1 and np.array_equal(m["textures"][0], img)
+    a, b = m["materials"][:2]
+    assert a["color_texture"] == 1 and a["alpha_mode"] == vk.ALPHA_MASK and a["alpha_cutoff"] == np.float32(0.3)
+    assert b["color_texture"] == 1 and b["alpha_mode"] == vk.ALPHA_BLEND
+    c, s = np.cos(rot), np.sin(rot)
+    u, v = uv[:, 0].astype(np.float64), uv[:, 1].astype(np.float64)
+    want = np.stack([c * scale[0] * u + s * scale[1] * v + off[0], -s * scale[0] * u + c * scale[1] * v + off[1]], -1)
+    assert np.allclose(m["texcoords"][:3], want, atol=1e-6)
+    assert np.array_equal(m["texcoords"][3:], uv)  # no transform on the second material
+
+
+# ---- alpha blend --------------------------------------------------------------------------------------------
+def blend_quads_scene(y_levels, z0, z1, alpha, mode=vk.ALPHA_BLEND):
+    """the rough plane plus one-sided quads facing up at the given heights (x in [-1, 1], z in [z0, z1]),
+    untextured, color alpha `alpha`"""
+    from test_nee_oracle import plane_scene
+    sc = plane_scene()
+    pos, idx, tobj = [sc["positions"]], [sc["indices"]], [0, 0]
+    for k, y in enumerate(y_levels):
+        base = 4 * (k + 1)
+        pos.append(np.array([[-1, y, z0], [1, y, z0], [1, y, z1], [-1, y, z1]], np.float32))
+        idx.append(np.array([base, base + 2, base + 1, base, base + 3, base + 2], np.uint32))
+        tobj += [1, 1]
+    sc["positions"] = np.concatenate(pos)
+    sc["normals"] = np.tile(np.float32([0, 1, 0]), (len(sc["positions"]), 1))
+    sc["indices"] = np.concatenate(idx)
+    sc["tri_object"] = np.array(tobj, np.uint32)
+    sc["object_material"] = np.array([0, 1], np.uint32)
+    quad = vk.material(color=(0.3, 0.6, 0.9), alpha=alpha, alpha_mode=mode)
+    sc["materials"] = np.concatenate([sc["materials"], np.array([quad], vk.MATERIAL_DTYPE)])
+    return sc
+
+
+def blend_threshold(seed, x, y, width):
+    """rnd() of a copy of the payload state at the first trace of a pixel's first sample: the pixel state
+    xxhash32(seed, W y + x) after the sample offset and the two lens draws, hashed with depth 0"""
+    _, s = orc.rnd_sequence(orc.xxhash32(seed, width * y + x), 3)
+    return orc.rnd_sequence(orc.xxhash32(s, 0), 1)[0][0]
+
+
+def check_blend_camera_rays(render, seed):
+    """one sample per pixel, the BLEND quad (alpha 0.4) over the whole view under the sky, no lights: a pixel
+    shows the quad exactly when its trace's draw is at most 0.4, and then equals the opaque render, else the
+    render without the quad (bounce rays meet only the quad's culled back)"""
+    a = np.float32(0.4)
+
+    def scene(mode):
+        sc = blend_quads_scene([1.0], -1.0, 1.0, a, mode)
+        sc["environment"] = 1.0
+        return sc
+    r = {k: render(scene(mode), m) for k, (mode, m) in
+         {"blend": (vk.ALPHA_BLEND, 1), "opaque": (vk.ALPHA_OPAQUE, 1), "none": (vk.ALPHA_BLEND, 0)}.items()}
+    keep = np.array([[not (a < blend_threshold(seed, x, y, SIZE)) for x in range(SIZE)] for y in range(SIZE)])
+    assert 0.2 < keep.mean() < 0.6
+    assert np.array_equal(r["blend"][keep], r["opaque"][keep])
+    assert np.array_equal(r["blend"][~keep], r["none"][~keep])
+
+
+def test_alpha_blend_camera_rays():
+    def render(sc, with_quad):
+        if not with_quad:
+            sc["materials"][1]["color"][3] = 0.0  # alpha 0 < any draw: the quad is never kept
+        return orc.pt_render(sc, overhead_camera(), SIZE, SIZE, 1, 2, 5)[..., :3]
+    check_blend_camera_rays(render, 5)
+
+
+def check_blend_shadow_rays(render):
+    """BLEND occluders between the plane and a light straight down, out of the camera's view: each layer
+    passes a shadow ray with probability 1 - alpha, independently (the per-candidate draws), so the lit image
+    is (1 - alpha)^layers of the unoccluded one"""
+    a = 0.3
+    free = render(blend_quads_scene([], -3.0, 3.0, a)).mean()
+    one = render(blend_quads_scene([3.0], -3.0, 3.0, a)).mean()
+    two = render(blend_quads_scene([3.0, 3.5], -3.0, 3.0, a)).mean()
+    assert abs(one / free - (1 - a)) < 0.01, (one / free, 1 - a)
+    assert abs(two / free - (1 - a) ** 2) < 0.01, (two / free, (1 - a) ** 2)
+
+
+BLEND_CAM = vk.make_camera(eye=(0.0, 0.5, 0.0), forward=(0.0, -1.0, 0.0), up=(0.0, 0.0, -1.0),
+                           fov_y=float(np.radians(16.0)))
+
+
+def test_alpha_blend_shadow_rays():
+    # the camera half a unit above the plane sees none of the quads at y = 3 and 3.5; every shadow ray from
+    # its view goes straight up through them
+    def render(sc):
+        return orc.pt_render(sc, BLEND_CAM, 16, 16, 256, 2, 3, lights=[DOWN])[..., :3]
+    check_blend_shadow_rays(render)

@@ -1024,6 +1024,36 @@ mat4d node_matrix(const json &node)
     return m;
 }
 
+//! KHR_texture_transform of the base color texture (texture_transform, gltf.cpp:162-194): uv' = offset +
+//! R * (scale * uv) with R the rotation about -z, as (a, b, c, d, offset x, offset y) for
+//! u' = a u + b v + ox, v' = c u + d v + oy. The path tracer applies it to every coordinate of the primitive
+//! (the entry's texture_matrix, RayBuilder.cpp:596, raypipeline.slang:350); it is folded into the
+//! coordinates at load time.
+struct uv_transform
+{
+    bool active = false;
+    float m[6] = {1.f, 0.f, 0.f, 1.f, 0.f, 0.f};
+};
+
+uv_transform base_color_transform(const json &jm)
+{
+    uv_transform r;
+    const json &t = jm["pbrMetallicRoughness"]["baseColorTexture"]["extensions"]["KHR_texture_transform"];
+    if(t.kind != json::object_t) return r;
+    r.active = true;
+    const float ox = static_cast<float>(t["offset"][0].number(0.0)), oy = static_cast<float>(t["offset"][1].number(0.0));
+    const float rot = static_cast<float>(t["rotation"].number(0.0));
+    const float sx = static_cast<float>(t["scale"][0].number(1.0)), sy = static_cast<float>(t["scale"][1].number(1.0));
+    const float c = std::cos(rot), sn = std::sin(rot);
+    r.m[0] = c * sx;
+    r.m[1] = sn * sy;
+    r.m[2] = -sn * sx;
+    r.m[3] = c * sy;
+    r.m[4] = ox;
+    r.m[5] = oy;
+    return r;
+}
+
 template<class F>
 vkrt_material convert_material(const json &jm, F &&texture_of)
 {
@@ -1113,10 +1143,12 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
         return id;
     };
     const json &mats = doc["materials"];
+    std::vector<uv_transform> uv_xform;
     for(size_t i = 0; i < mats.size(); ++i)
     {
         model->materials.push_back(convert_material(mats[i], texture_of));
         model->material_names.push_back(mats[i]["name"].str);
+        uv_xform.push_back(model->materials.back().color_texture ? base_color_transform(mats[i]) : uv_transform{});
     }
     const uint32_t default_index = static_cast<uint32_t>(model->materials.size());
     bool default_used = false;
@@ -1226,6 +1258,16 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
                 {
                     mi = static_cast<int>(default_index);
                     default_used = true;
+                }
+                else if(uv_xform[mi].active)
+                {
+                    const float *m = uv_xform[mi].m;
+                    for(size_t k = 0; k + 1 < uv.size(); k += 2)
+                    {
+                        const float u = uv[k], v = uv[k + 1];
+                        uv[k] = (m[0] * u + m[1] * v) + m[4];
+                        uv[k + 1] = (m[2] * u + m[3] * v) + m[5];
+                    }
                 }
                 model->add_object(mesh["name"].str, pos, nrm, uv, idx, static_cast<uint32_t>(mi));
             }

@@ -195,9 +195,15 @@ __device__ float4 sample_texture(const tex_desc &d, const uint32_t *__restrict__
     return lerp4(a, sample_level(d, texels, min(l0 + 1u, d.levels - 1u), u, v), f);
 }
 
-//! the any-hit alpha test of masked materials inside traversal: the candidate's color alpha times its
-//! texture's alpha at level 0, against the cutoff. Closest-hit rays (raypipeline.slang:686-720) see
-//! disable_material's stand-in color (0.8, 1); shadow rays (visibility_test.slang:136-155) the material as it is.
+//! rnd() of a copy of an RNG state: the draw a stage makes without advancing its caller's stream
+__device__ __forceinline__ float rnd_peek(uint32_t state) { return rnd(state); }
+
+//! the any-hit test of MASK and BLEND materials inside traversal: the candidate's color alpha times its
+//! texture's alpha at level 0, kept when not below the cutoff (MASK) or a uniform draw (BLEND).
+//! Closest-hit rays (raypipeline.slang:686-720) see disable_material's stand-in color (0.8, 1) and draw once
+//! per trace from a copy of the path's RNG state (the same `xi` for every candidate); shadow rays
+//! (visibility_test.slang:136-155) see the material as it is and draw per candidate, here from
+//! xxhash32(seed, triangle) so that the outcome does not depend on the traversal order.
 struct alpha_filter
 {
     static constexpr bool active = true;
@@ -207,21 +213,28 @@ struct alpha_filter
     const tex_desc *tex;
     const uint32_t *texels;
     bool disable_material;
+    bool per_candidate = false;
+    float xi = 0.0f;      // closest-hit rays: the trace's BLEND threshold
+    uint32_t seed = 0u;   // shadow rays: per-candidate BLEND thresholds
 
     __device__ bool operator()(const float4 &rec, float bu, float bv) const
     {
         const gpu_material &m = mats[__float_as_uint(rec.z)];
-        if(disable_material) return !(1.0f < __uint_as_float(m.tex.z));
-        float a = m.color.w;
-        if(m.tex.x)
+        const uint32_t prim = __float_as_uint(rec.y);
+        float a = 1.0f;
+        if(!disable_material)
         {
-            const uint32_t prim = __float_as_uint(rec.y);
-            const float2 t0 = uv[idx[3 * prim]], t1 = uv[idx[3 * prim + 1]], t2 = uv[idx[3 * prim + 2]];
-            const float b0 = 1.0f - bu - bv;
-            const float u = (t0.x * b0 + t1.x * bu) + t2.x * bv, v = (t0.y * b0 + t1.y * bu) + t2.y * bv;
-            a = a * sample_texture(tex[m.tex.x - 1u], texels, u, v, 0.0f).w;
+            a = m.color.w;
+            if(m.tex.x)
+            {
+                const float2 t0 = uv[idx[3 * prim]], t1 = uv[idx[3 * prim + 1]], t2 = uv[idx[3 * prim + 2]];
+                const float b0 = 1.0f - bu - bv;
+                const float u = (t0.x * b0 + t1.x * bu) + t2.x * bv, v = (t0.y * b0 + t1.y * bu) + t2.y * bv;
+                a = a * sample_texture(tex[m.tex.x - 1u], texels, u, v, 0.0f).w;
+            }
         }
-        return !(a < __uint_as_float(m.tex.z));
+        if(m.tex.y == VKRT_ALPHA_MASK) return !(a < __uint_as_float(m.tex.z));
+        return !(a < (per_candidate ? rnd_peek(xxhash32(seed, prim)) : xi));
     }
 };
 __device__ __forceinline__ vec3 xyz(const float4 &f) { return vec3{f.x, f.y, f.z}; }
@@ -311,18 +324,22 @@ __global__ void k_pt_reset_depth(uint32_t *c, int next)
     c[6] = 0u;
 }
 
-//! the closest hit of path p's current ray; back faces are culled while the path is in air
+//! the closest hit of path p's current ray at this depth; back faces are culled while the path is in air.
+//! BLEND candidates compare against rnd() of a copy of payload.rng_state = xxhash32(rng_state, depth)
+//! (raypipeline.slang:150-152, :693, :716)
 __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                            int32_t root, bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
                                            int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem,
-                                           const alpha_filter &flt)
+                                           alpha_filter flt, uint32_t depth)
 {
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
     if(!empty)
     {
         ray_setup r = make_ray_setup(b.ray_o[p], b.ray_d[p]);
-        const bool in_air = (b.state[p].y & F_MEDIA_MASK) == 0u;
+        const uint4 st = b.state[p];
+        const bool in_air = (st.y & F_MEDIA_MASK) == 0u;
+        flt.xi = rnd_peek(xxhash32(st.x, depth));
         // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
         if(wide)
             traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air, flt);
@@ -335,12 +352,13 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                        const float4 *__restrict__ tris, int32_t root, bool empty,
                                                        const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
-                                                       bool wide, pt_buffers b, int in, alpha_filter flt)
+                                                       bool wide, pt_buffers b, int in, alpha_filter flt,
+                                                       uint32_t depth)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
     if(q >= b.counter[in]) return;
-    trace_path(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x, flt);
+    trace_path(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x, flt, depth);
 }
 
 //! eval_distant_lights (raypipeline.slang:728-744): sun discs seen by a ray that left the scene
@@ -551,7 +569,8 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
                 shadow_rec[3] = make_float4(F.x, F.y, F.z, cos_theta);
                 shadow_rec[4] = make_float4(beta.x, beta.y, beta.z, ls.pdf);
                 const uint32_t vtx = train ? (p / f.train_stride) * f.max_depth + depth : NO_VERTEX;
-                shadow_rec[5] = make_float4(__uint_as_float(p), __uint_as_float(vtx), 0.f, 0.f);
+                // the state the transmittance test would continue drawing from: BLEND candidates' seed
+                shadow_rec[5] = make_float4(__uint_as_float(p), __uint_as_float(vtx), __uint_as_float(rng), 0.f);
                 need_shadow = true;
             }
         }
@@ -952,9 +971,11 @@ __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ no
                                             bool empty, const float4 *__restrict__ tris,
                                             const gpu_material *__restrict__ mats, const pt_frame &f,
                                             const pt_buffers &b, const float4 (&rec)[SHADOW_F4], int32_t *smem,
-                                            const alpha_filter &flt)
+                                            alpha_filter flt)
 {
     const float4 o = rec[0], d = rec[1];
+    flt.per_candidate = true;
+    flt.seed = __float_as_uint(rec[5].z);
     const float4 T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d),
                                         false, smem, flt);
     if(!(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)) return;
@@ -1015,7 +1036,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node
     uint32_t segments = 0;
     for(uint32_t depth = 0; depth < f.max_depth; ++depth)
     {
-        trace_path(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x, flt);
+        trace_path(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x, flt, depth);
         bool alive = false, to_cache = false, need_shadow = false;
         float4 shadow_rec[SHADOW_F4];
         float4 qfeat[QUERY_F4];
@@ -1114,7 +1135,7 @@ __global__ void k_pt_tag(float4 *__restrict__ tris, uint32_t n, const uint32_t *
     const uint32_t mat = object_material[tri_object[prim]];
     r.z = __uint_as_float(mat);
     r.w = __uint_as_float((mats[mat].sheen.w != 0.f ? TRI_TWO_SIDED : 0u) |
-                          (mats[mat].tex.y == VKRT_ALPHA_MASK ? TRI_ALPHA_MASK : 0u));
+                          (mats[mat].tex.y != VKRT_ALPHA_OPAQUE ? TRI_ALPHA_MASK : 0u));
     tris[3 * (size_t)j + 2] = r;
 }
 
@@ -1376,7 +1397,7 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t grid, uint32_t depth, int in) {
         k_pt_reset_depth<<<1, 1, 0, st>>>(bb.counter, in ^ 1);
         k_pt_trace<<<grid, PT_BLOCK, 0, st>>>(nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide,
-                                              bb, in, flt);
+                                              bb, in, flt, depth);
         if(f.n_lights)
         {
             k_pt_shade<true><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
@@ -1603,9 +1624,8 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
             VKRT_REQUIRE(x.alpha_mode <= VKRT_ALPHA_BLEND, "unknown alpha mode");
             uint32_t cutoff_bits;
             std::memcpy(&cutoff_bits, &x.alpha_cutoff, 4);
-            // BLEND is traced as OPAQUE; MASK drives the any-hit test
-            mats[m].tex = make_uint4(x.color_texture, x.alpha_mode == VKRT_ALPHA_MASK ? VKRT_ALPHA_MASK : 0u,
-                                     cutoff_bits, 0u);
+            // MASK and BLEND drive the any-hit test
+            mats[m].tex = make_uint4(x.color_texture, x.alpha_mode, cutoff_bits, 0u);
         }
         // texture coordinates (zeros when the scene has none) and the textures' mip chains: each level halves
         // the previous one (at least 1 texel) with a 2 x 2 box filter, coordinates clamped at odd edges,
