@@ -421,12 +421,22 @@ typedef struct vkrt_material
     uint32_t color_texture; /* 1 + index into the scene's textures (0: none): color and alpha are multiplied by
                              * the texture, sampled trilinearly with repeat addressing at the ray-cone level of
                              * detail (raypipeline.slang:300-321, :526-531) */
-    uint32_t alpha_mode;    /* VKRT_ALPHA_*: with MASK every ray skips the surface where alpha < alpha_cutoff */
+    uint32_t alpha_mode;    /* VKRT_ALPHA_*: the any-hit test of every ray (MASK: alpha < alpha_cutoff skips the
+                             * surface; BLEND: alpha < a uniform draw skips it) */
     float alpha_cutoff;
+    /* further texture slots (1 + texture index, 0: none), sampled at the same level of detail as the color
+     * (raypipeline.slang:536-606, material_t texture slots of ray/types.slang:85-103) */
+    uint32_t normal_texture;        /* tangent-space normal in rg (z = sqrt(1 - x^2 - y^2)); needs tangents */
+    uint32_t emission_texture;      /* rgb multiplies the emission, or replaces it where the emission is black */
+    uint32_t ao_rough_metal_texture; /* g scales roughness, b metalness (glTF metallicRoughnessTexture) */
+    uint32_t transmission_texture;  /* r scales transmission */
+    uint32_t diffuse_transmission_texture;       /* a scales diffuse_transmission */
+    uint32_t diffuse_transmission_color_texture; /* rgb tints diffuse_transmission_color */
+    uint32_t reserved[2];
 } vkrt_material;
 
-/* BlendMode (include/vierkant/Material.hpp:129): MASK is the any-hit cutout of raypipeline.slang:686-720 and
- * visibility_test.slang:145-150; BLEND is traced as OPAQUE. */
+/* BlendMode (include/vierkant/Material.hpp:129): the any-hit stages of raypipeline.slang:686-720 and
+ * visibility_test.slang:136-155. */
 enum
 {
     VKRT_ALPHA_OPAQUE = 0,
@@ -458,8 +468,9 @@ typedef struct vkrt_pt_scene
     uint32_t n_materials;
     float environment; /* scale of the procedural sky (environment_white) */
     const float *texcoords;        /* 2 per vertex, or null (all 0) */
-    const vkrt_texture *textures;  /* referenced by vkrt_material::color_texture */
+    const vkrt_texture *textures;  /* referenced by the material texture slots */
     uint32_t n_textures;
+    const float *tangents;         /* 3 per vertex (normal maps), or null (all 0) */
 } vkrt_pt_scene;
 
 /* Scene files into a flat scene for the tracer: Wavefront OBJ + MTL and glTF 2.0 (.gltf with external or
@@ -478,6 +489,7 @@ typedef struct vkrt_model_view
     const float *positions; /* 3 per vertex, world space */
     const float *normals;   /* 3 per vertex */
     const float *texcoords; /* 2 per vertex (0 where the file has none) */
+    const float *tangents;  /* 3 per vertex: the file's (glTF TANGENT xyz) or Geometry::compute_tangents */
     uint32_t n_vert;
     const uint32_t *indices; /* 3 per triangle */
     uint32_t n_tri;
@@ -486,7 +498,7 @@ typedef struct vkrt_model_view
     uint32_t n_objects;
     const vkrt_material *materials;
     uint32_t n_materials;
-    const vkrt_texture *textures; /* decoded PNG images (8 or 16 bit, any color type, not interlaced) */
+    const vkrt_texture *textures; /* decoded PNG images (any bit depth and color type, Adam7 included) */
     uint32_t n_textures;
     uint32_t n_skipped_textures;  /* images that are not PNG or failed to decode; their materials are untextured */
 } vkrt_model_view;

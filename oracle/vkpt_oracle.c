@@ -165,6 +165,7 @@ typedef struct
     int two_sided;
     float alpha;        /* color.a */
     uint32_t tex;       /* base color texture + 1, 0 = none */
+    uint32_t tex_normal, tex_emission, tex_arm, tex_transmission, tex_dt, tex_dtc; /* further slots, same code */
     int alpha_mode;     /* 0 opaque, 1 MASK (cutoff), 2 BLEND (uniform draw): the any-hit test */
     float alpha_cutoff;
 } mat_t;
@@ -1048,7 +1049,8 @@ typedef struct
     const uint32_t *mat_of_prim;
     const light_t *lights;
     const otex_t *tex;
-    const float *uv; /* 2 per vertex */
+    const float *uv;  /* 2 per vertex */
+    const float *tan; /* 3 per vertex */
     float *obj_tmax;
     float *out; /* 8 floats per pixel: r g b alpha depth normal(bits) object(bits) albedo(bits) */
     uint32_t y0, y1;
@@ -1154,26 +1156,83 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                 obj = sc->tri_object[prim];
                 mat_t m = j->mats[sc->object_material[obj]];
                 v3 Vw = neg(dir);
-                if(m.tex)
+                const int media_boundary = m.trans > 0.0f; /* before the transmission texture (raypipeline.slang:517-522) */
+                if(m.tex | m.tex_normal | m.tex_emission | m.tex_arm | m.tex_transmission | m.tex_dt | m.tex_dtc)
                 {
-                    /* the base color texture at the ray cone's level of detail (raypipeline.slang:300-321,
-                     * 526-531): 0.5 log2(uv area / world area) + log2|cone width| + 0.5 log2(w h) - log2(N.V),
+                    /* every texture at the ray cone's level of detail (raypipeline.slang:300-321, 368-375,
+                     * 526-606): 0.5 log2(uv area / world area) + log2|cone width| + 0.5 log2(w h) - log2(N.V),
                      * with the width the cone had before this segment */
-                    const otex_t *tx = &j->tex[m.tex - 1];
                     const float *T0 = j->uv + 2 * (size_t)ix[0], *T1 = j->uv + 2 * (size_t)ix[1],
                                 *T2 = j->uv + 2 * (size_t)ix[2];
                     float tu = (T0[0] * bx + T1[0] * by) + T2[0] * bz, tv = (T0[1] * bx + T1[1] * by) + T2[1] * bz;
                     v3 pc = cross(sub(v1, v0), sub(v2, v0));
                     float p_a = sqrtf(dot(pc, pc));
                     float t_a = fabsf((T1[0] - T0[0]) * (T2[1] - T0[1]) - (T2[0] - T0[0]) * (T1[1] - T0[1]));
-                    float lambda = 0.5f * log2f(t_a / p_a);
-                    lambda += log2f(fabsf(cone_width));
-                    lambda += 0.5f * log2f((float)tx->w * (float)tx->h);
-                    lambda -= log2f(fabsf(dot(Vw, nrm)));
+                    const float lod_cone = 0.5f * log2f(t_a / p_a) + log2f(fabsf(cone_width));
+                    const float lod_nov = log2f(fabsf(dot(Vw, nrm)));
                     float c[4];
-                    otex_sample(tx, tu, tv, lambda, c);
-                    m.color = V3(m.color.x * c[0], m.color.y * c[1], m.color.z * c[2]);
-                    m.alpha = m.alpha * c[3];
+#define ORC_LOOKUP(slot)                                                                                   \
+    do                                                                                                     \
+    {                                                                                                      \
+        const otex_t *tx = &j->tex[(slot) - 1];                                                           \
+        float lambda = lod_cone;                                                                           \
+        lambda += 0.5f * log2f((float)tx->w * (float)tx->h);                                              \
+        lambda -= lod_nov;                                                                                 \
+        otex_sample(tx, tu, tv, lambda, c);                                                                \
+    } while(0)
+                    if(m.tex)
+                    {
+                        ORC_LOOKUP(m.tex);
+                        m.color = V3(m.color.x * c[0], m.color.y * c[1], m.color.z * c[2]);
+                        m.alpha = m.alpha * c[3];
+                    }
+                    if(m.tex_normal)
+                    {
+                        /* tangent-space normal, bitangent flipped on back faces, hemisphere clamp (:536-552) */
+                        ORC_LOOKUP(m.tex_normal);
+                        v3 nt;
+                        nt.x = 2.0f * (c[0] - 0.5f);
+                        nt.y = 2.0f * (c[1] - 0.5f);
+                        nt.z = sqrtf(fmaxf(0.0f, 1.0f - nt.x * nt.x - nt.y * nt.y));
+                        const float *G = j->tan;
+                        v3 tg = nrmz(add(add(scl(V3(G[3 * ix[0]], G[3 * ix[0] + 1], G[3 * ix[0] + 2]), bx),
+                                             scl(V3(G[3 * ix[1]], G[3 * ix[1] + 1], G[3 * ix[1] + 2]), by)),
+                                         scl(V3(G[3 * ix[2]], G[3 * ix[2] + 1], G[3 * ix[2] + 2]), bz)));
+                        v3 bt = nrmz(cross(nrm, tg));
+                        if(backface) bt = neg(bt);
+                        v3 nm = nrmz(add(add(scl(tg, nt.x), scl(bt, nt.y)), scl(nrm, nt.z)));
+                        nrm = nrmz(add(nm, scl(nrm, fmaxf(0.0f, -dot(nm, nrm) + 1e-3f))));
+                    }
+                    if(m.tex_emission)
+                    {
+                        ORC_LOOKUP(m.tex_emission);
+                        if(m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f)
+                            m.emission = V3(m.emission.x * c[0], m.emission.y * c[1], m.emission.z * c[2]);
+                        else
+                            m.emission = V3(c[0], c[1], c[2]);
+                    }
+                    if(m.tex_arm)
+                    {
+                        ORC_LOOKUP(m.tex_arm);
+                        m.rough = m.rough * c[1];
+                        m.metal = m.metal * c[2];
+                    }
+                    if(m.tex_transmission)
+                    {
+                        ORC_LOOKUP(m.tex_transmission);
+                        m.trans = m.trans * c[0];
+                    }
+                    if(m.tex_dt)
+                    {
+                        ORC_LOOKUP(m.tex_dt);
+                        m.dt = m.dt * c[3];
+                    }
+                    if(m.tex_dtc)
+                    {
+                        ORC_LOOKUP(m.tex_dtc);
+                        m.dt_color = V3(m.dt_color.x * c[0], m.dt_color.y * c[1], m.dt_color.z * c[2]);
+                    }
+#undef ORC_LOOKUP
                 }
                 /* propagate(cone, 0, hit t) (raypipeline.slang:280-286, :608); the payload keeps halves */
                 cone_width = orc_half_value(cone_width + cone_spread * t);
@@ -1182,7 +1241,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                 float eta = backface ? m.ior / iors[media_index > 0 ? media_index - 1 : 0] : iors[media_index] / m.ior;
                 eta += EPS;
                 next_ior = m.ior;
-                media_op = !(m.trans > 0.0f) ? 0 : (backface ? 2 : 1);
+                media_op = !media_boundary ? 0 : (backface ? 2 : 1);
                 if(i == 0) alb = m.color;
                 v3 ff = dot(dir, nrm) < 0.f ? nrm : neg(nrm);
                 v3 em = scl(m.emission, dot(nrm, ff) > 0.f ? m.emission_strength : 0.0f);
@@ -1321,7 +1380,7 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     mat_t *mats = (mat_t *)calloc(sc->n_materials ? sc->n_materials : 1, sizeof(mat_t));
     for(uint32_t m = 0; m < sc->n_materials; ++m)
     {
-        const float *x = sc->materials + 28 * (size_t)m; /* vkrt_material layout, 28 words */
+        const float *x = sc->materials + 36 * (size_t)m; /* vkrt_material layout, 36 words */
         mats[m].color = V3(x[0], x[1], x[2]);
         mats[m].emission = V3(x[4], x[5], x[6]);
         mats[m].emission_strength = x[7];
@@ -1344,6 +1403,15 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         mats[m].tex = tex <= sc->n_textures ? tex : 0;
         mats[m].alpha_mode = mode <= 2 ? (int)mode : 0;
         mats[m].alpha_cutoff = x[27];
+        uint32_t slot[6];
+        memcpy(slot, &x[28], sizeof(slot));
+        for(int k = 0; k < 6; ++k) slot[k] = slot[k] <= sc->n_textures ? slot[k] : 0;
+        mats[m].tex_normal = slot[0];
+        mats[m].tex_emission = slot[1];
+        mats[m].tex_arm = slot[2];
+        mats[m].tex_transmission = slot[3];
+        mats[m].tex_dt = slot[4];
+        mats[m].tex_dtc = slot[5];
     }
     for(uint32_t t = 0; t < n; ++t)
     {
@@ -1390,6 +1458,8 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     for(uint32_t t = 0; t < n; ++t) mat_of_prim[t] = sc->object_material[sc->tri_object[t]];
     float *uv = (float *)calloc(2 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
     if(sc->texcoords) memcpy(uv, sc->texcoords, sizeof(float) * 2 * (size_t)sc->n_vert);
+    float *tan = (float *)calloc(3 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
+    if(sc->tangents) memcpy(tan, sc->tangents, sizeof(float) * 3 * (size_t)sc->n_vert);
     otex_t *tex = (otex_t *)calloc(sc->n_textures ? sc->n_textures : 1, sizeof(otex_t));
     for(uint32_t k = 0; k < sc->n_textures; ++k) otex_build(&tex[k], &sc->textures[k]);
     alpha_ctx ac = {mats, mat_of_prim, sc->indices, uv, tex};
@@ -1409,6 +1479,7 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         jobs[k].lights = lights;
         jobs[k].tex = tex;
         jobs[k].uv = uv;
+        jobs[k].tan = tan;
         jobs[k].obj_tmax = obj_tmax;
         jobs[k].out = out8;
         jobs[k].y0 = (uint32_t)((uint64_t)pp->height * k / nthreads);
@@ -1422,6 +1493,7 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     for(uint32_t k = 0; k < sc->n_textures; ++k) otex_free(&tex[k]);
     free(tex);
     free(uv);
+    free(tan);
     free(mat_of_prim);
     free(lo);
     free(hi);

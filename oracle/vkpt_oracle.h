@@ -28,12 +28,13 @@ typedef struct orc_pt_scene
     const uint32_t *tri_object;
     const uint32_t *object_material;
     uint32_t n_objects;
-    const float *materials; /* vkrt_material records, 28 words each */
+    const float *materials; /* vkrt_material records, 36 words each */
     uint32_t n_materials;
     float environment;
     const float *texcoords; /* 2 per vertex, or NULL (all zero) */
     const orc_texture *textures;
     uint32_t n_textures;
+    const float *tangents; /* 3 per vertex, or NULL (all zero) */
 } orc_pt_scene;
 
 typedef struct orc_pt_params

@@ -271,7 +271,8 @@ class _PtScene(ctypes.Structure):
                 ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32), ("tri_object", ctypes.c_void_p),
                 ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
                 ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float),
-                ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("n_textures", ctypes.c_uint32)]
+                ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("n_textures", ctypes.c_uint32),
+                ("tangents", ctypes.c_void_p)]
 
 
 class _Texture(ctypes.Structure):
@@ -315,13 +316,16 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
            ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
     uv = scene.get("texcoords")
     uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
+    tg = scene.get("tangents")
+    tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
     imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
     tex = (_Texture * max(1, len(imgs)))(*[_Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in imgs])
     sc = _PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
                   arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
                   arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,
                   len(arr["materials"]), float(scene.get("environment", 1.0)),
-                  uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs))
+                  uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
+                  tg.ctypes.data if tg is not None else None)
     eye, fw, rt, up, thx, thy = W.camera_basis(camera, width, height)
     # the tracer's gather: positive intensity, no Area lights, order kept
     la = np.zeros(0, vk.LIGHT_DTYPE) if lights is None else np.asarray(lights, vk.LIGHT_DTYPE).reshape(-1)

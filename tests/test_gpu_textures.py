@@ -148,3 +148,77 @@ def test_gpu_loaded_textured_obj(ctx, tmp_path):
     want = orc.pt_render(sc, cam, 48, 48, 4, 3, oracle_seed(1), lights=[T.DOWN])[..., :3]
     close = np.mean(np.abs(got - want).max(axis=2) <= 1e-4 * np.maximum(1.0, want.max(axis=2)))
     assert close > 0.97, close
+
+
+@pytest.mark.parametrize("slot", list(T.SLOT_CASES))
+def test_gpu_texture_slots_scale_their_factors(ctx, slot):
+    """a constant texture in a slot renders as the factor multiplied by its texel, bit for bit, and as the CPU
+    integrator"""
+    a, b = T.slot_scenes(slot)
+    ra = gpu_render(ctx, a, [], W.room_camera(), 64, 36, 2, 6, 3)
+    rb = gpu_render(ctx, b, [], W.room_camera(), 64, 36, 2, 6, 3)
+    assert np.array_equal(ra, rb)
+    want = orc.pt_render(a, W.room_camera(), 64, 36, 2, 6, oracle_seed(3))[..., :3]
+    close = np.mean(np.abs(ra - want).max(axis=2) <= 1e-4 * np.maximum(1.0, want.max(axis=2)))
+    assert close > 0.97, close
+
+
+def test_gpu_emission_texture(ctx):
+    from test_nee_oracle import down_camera
+    t = (51, 102, 153, 255)
+    tex = np.float32(t[:3]) / np.float32(255)
+    img = gpu_render(ctx, T.emissive_quad((0.5, 1.0, 2.0), 3.0, t), [], down_camera(), 4, 4, 2, 1, 1)
+    assert np.array_equal(img, np.broadcast_to((np.float32([0.5, 1.0, 2.0]) * tex) * np.float32(3.0), img.shape))
+    img = gpu_render(ctx, T.emissive_quad((0.0, 0.0, 0.0), 3.0, t), [], down_camera(), 4, 4, 2, 1, 1)
+    assert np.array_equal(img, np.broadcast_to(tex * np.float32(3.0), img.shape))
+
+
+@pytest.mark.parametrize("texel", [(160, 128, 255, 255), (90, 200, 255, 255)])
+def test_gpu_normal_map_tilts_like_geometry(ctx, texel):
+    from test_nee_oracle import down_camera
+    flat, tilted, n = T.normal_map_scenes(texel)
+    light = vk.light(vk.LIGHT_DIRECTIONAL, direction=(0, -1, 0), intensity=2.0)
+    a = gpu_render(ctx, flat, [light], down_camera(), 8, 8, 64, 2, 4)
+    b = gpu_render(ctx, tilted, [light], down_camera(), 8, 8, 64, 2, 4)
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-7), np.abs(a - b).max()
+    want = orc.pt_render(flat, down_camera(), 8, 8, 64, 2, oracle_seed(4), lights=[light])[..., :3]
+    assert np.allclose(a, want, rtol=1e-4, atol=1e-6)
+
+
+def fully_textured_room(seed=0):
+    """the textured room with every texture slot in use on some materials (random images), tangents
+    orthogonal to the normals"""
+    sc = textured_room(seed)
+    rng = np.random.default_rng(seed + 1)
+    sc["textures"] = sc["textures"] + [rng.integers(0, 256, (32, 16, 4), dtype=np.uint8),
+                                       rng.integers(100, 256, (8, 8, 4), dtype=np.uint8)]
+    nrm = sc["normals"]
+    t = np.cross(nrm, np.float32([0.3, 0.9, 0.2]))
+    sc["tangents"] = (t / np.linalg.norm(t, axis=1, keepdims=True)).astype(np.float32)
+    mats = sc["materials"]
+    for k in range(len(mats)):
+        if k % 3 == 0:
+            mats[k]["normal_texture"] = 3
+        if k % 4 == 1:
+            mats[k]["ao_rough_metal_texture"] = 4
+        if k % 5 == 2:
+            mats[k]["emission_texture"] = 3
+            mats[k]["emission"][3] = 0.5
+        if k % 7 == 3:
+            mats[k]["transmission_texture"] = 4
+            mats[k]["diffuse_transmission"] = 0.4
+            mats[k]["diffuse_transmission_texture"] = 3
+            mats[k]["diffuse_transmission_color_texture"] = 4
+    return sc
+
+
+def test_gpu_fully_textured_room_matches_cpu_integrator(ctx):
+    sc, cam, lights = fully_textured_room(), W.room_camera(), W.room_lights()
+    w, h, spp = 160, 90, 4
+    got = gpu_render(ctx, sc, lights, cam, w, h, spp, 8, 0)
+    want = orc.pt_render(sc, cam, w, h, spp, 8, oracle_seed(0), lights=lights)[..., :3]
+    assert np.isfinite(got).all()
+    close = np.mean(np.abs(got - want).max(axis=2) <= 1e-4 * np.maximum(1.0, want.max(axis=2)))
+    rmse = float(np.sqrt(np.mean((got - want) ** 2)))
+    assert close > 0.97, f"only {close:.4f} of pixels within 1e-4"
+    assert rmse < 0.02 * float(want.mean()) + 1e-6

@@ -583,3 +583,196 @@ def test_alpha_blend_shadow_rays():
     def render(sc):
         return orc.pt_render(sc, BLEND_CAM, 16, 16, 256, 2, 3, lights=[DOWN])[..., :3]
     check_blend_shadow_rays(render)
+
+
+# ---- the other texture slots --------------------------------------------------------------------------------
+def const_texture(rgba, size=(4, 6)):
+    return np.tile(np.array(rgba, np.uint8), (*size, 1))
+
+
+def with_planar_uv(sc):
+    p = sc["positions"]
+    sc["texcoords"] = np.stack([0.37 * p[:, 0] + 0.11 * p[:, 1], 0.41 * p[:, 2] + 0.23 * p[:, 1]], -1).astype(np.float32)
+    return sc
+
+
+SLOT_CASES = {
+    # slot: (texel, material edit equivalent to the texture at that texel)
+    "ao_rough_metal_texture": ((0, 128, 64, 255),
+                               lambda m, t: (m.__setitem__("roughness", m["roughness"] * np.float32(t[1] / np.float32(255))),
+                                             m.__setitem__("metalness", m["metalness"] * np.float32(t[2] / np.float32(255))))),
+    "transmission_texture": ((200, 0, 0, 255),
+                             lambda m, t: m.__setitem__("transmission", m["transmission"] * np.float32(t[0] / np.float32(255)))),
+    "diffuse_transmission_texture": ((0, 0, 0, 170),
+                                     lambda m, t: m.__setitem__("diffuse_transmission", m["diffuse_transmission"] * np.float32(t[3] / np.float32(255)))),
+    "diffuse_transmission_color_texture": ((51, 102, 204, 0),
+                                           lambda m, t: m.__setitem__("diffuse_transmission_color", np.concatenate(
+                                               [m["diffuse_transmission_color"][:3] * (np.float32(t[:3]) / np.float32(255)), [0]]))),
+}
+
+
+def slot_scenes(slot):
+    """the room under its sky with a constant texture in one slot of every material, and the same room with
+    each material's factor multiplied by the texel instead: the two render bit for bit alike"""
+    texel, edit = SLOT_CASES[slot]
+    a = with_planar_uv(W.room_scene())
+    a["materials"] = a["materials"].copy()
+    if slot.startswith("diffuse_transmission"):
+        a["materials"]["diffuse_transmission"] = 0.5
+        a["materials"]["diffuse_transmission_color"] = (0.9, 0.8, 0.7, 0.0)
+    if slot == "transmission_texture":
+        a["materials"]["transmission"] = np.where(a["materials"]["transmission"] > 0, a["materials"]["transmission"], 0.5)
+    b = {k: (v.copy() if hasattr(v, "copy") else v) for k, v in a.items()}
+    a["textures"] = [const_texture(texel)]
+    a["materials"][slot] = 1
+    for m in b["materials"]:
+        edit(m, texel)
+    return a, b
+
+
+@pytest.mark.parametrize("slot", list(SLOT_CASES))
+def test_texture_slots_scale_their_factors(slot):
+    a, b = slot_scenes(slot)
+    ra = orc.pt_render(a, W.room_camera(), 64, 36, 2, 6, 3)
+    rb = orc.pt_render(b, W.room_camera(), 64, 36, 2, 6, 3)
+    assert np.array_equal(ra, rb)
+
+
+def emissive_quad(emission, strength, texel):
+    """a quad facing a camera straight above it, emitting; the emission texture constant"""
+    from test_nee_oracle import plane_scene
+    sc = plane_scene(half=2.0)
+    sc["materials"] = np.array([vk.material(emission=emission, emission_strength=strength, emission_texture=0)],
+                               vk.MATERIAL_DTYPE)
+    sc["materials"][0]["emission_texture"] = 1
+    sc["texcoords"] = np.float32([[0, 0], [1, 0], [1, 1], [0, 1]])
+    sc["textures"] = [const_texture(texel)]
+    return sc
+
+
+def test_emission_texture_multiplies_or_replaces():
+    """seen directly (one segment), the radiance is the emission: the texel times a coloured emission, or the
+    texel alone where the material's emission is black (raypipeline.slang:556-566), times the strength"""
+    from test_nee_oracle import down_camera
+    t = (51, 102, 153, 255)
+    tex = np.float32(t[:3]) / np.float32(255)
+    img = orc.pt_render(emissive_quad((0.5, 1.0, 2.0), 3.0, t), down_camera(), 4, 4, 2, 1, 1)
+    want = (np.float32([0.5, 1.0, 2.0]) * tex) * np.float32(3.0)
+    assert np.array_equal(img[..., :3], np.broadcast_to(want, img[..., :3].shape))
+    img = orc.pt_render(emissive_quad((0.0, 0.0, 0.0), 3.0, t), down_camera(), 4, 4, 2, 1, 1)
+    assert np.array_equal(img[..., :3], np.broadcast_to(tex * np.float32(3.0), img[..., :3].shape))
+
+
+def normal_map_scenes(texel):
+    """a flat plane with a constant normal map, and the plane tilted so that its geometric normal is the
+    mapped one. The tangent follows Geometry::compute_tangents' sign (for u along +x, v along +z and the
+    normal +y: dot(cross(n, t), b) = dot(-z, +z) < 0 keeps t = +x) and the bitangent is cross(n, t) = -z.
+    Both quads are shifted so that their diagonals (where the triangle test may leave a crack) lie far
+    outside the view"""
+    from test_nee_oracle import plane_scene
+    flat = plane_scene()
+    flat["positions"] = flat["positions"] + np.float32([20, 0, -20])
+    flat["texcoords"] = (flat["positions"][:, [0, 2]] / 10.0).astype(np.float32)
+    flat["tangents"] = np.tile(np.float32([1, 0, 0]), (4, 1))
+    flat["textures"] = [const_texture(texel)]
+    flat["materials"]["normal_texture"] = 1
+    c = np.float32(texel[:2]) / np.float32(255)
+    nt = np.array([2 * (c[0] - 0.5), 2 * (c[1] - 0.5), 0.0])
+    nt[2] = np.sqrt(max(0.0, 1 - nt[0] ** 2 - nt[1] ** 2))
+    T, B, N = np.array([1.0, 0, 0]), np.array([0, 0, -1.0]), np.array([0, 1.0, 0])
+    n = nt[0] * T + nt[1] * B + nt[2] * N
+    n /= np.linalg.norm(n)
+    # a plane through the origin with normal n, spanned by two unit vectors orthogonal to it
+    e1 = np.cross(n, [0, 0, 1.0]); e1 /= np.linalg.norm(e1)
+    e2 = np.cross(e1, n)
+    c = 20 * (e1 - e2)
+    corners = np.array([c - 50 * e1 - 50 * e2, c + 50 * e1 - 50 * e2, c + 50 * e1 + 50 * e2, c - 50 * e1 + 50 * e2],
+                       np.float32)
+    tilted = plane_scene()
+    tilted["positions"] = corners
+    tilted["normals"] = np.tile(n.astype(np.float32), (4, 1))
+    tilted["indices"] = np.array([0, 1, 2, 0, 2, 3], np.uint32) if np.dot(np.cross(corners[1] - corners[0], corners[2] - corners[0]), n) > 0 \
+        else np.array([0, 2, 1, 0, 3, 2], np.uint32)
+    return flat, tilted, n
+
+
+@pytest.mark.parametrize("texel", [(160, 128, 255, 255), (90, 200, 255, 255), (128, 128, 255, 255)])
+def test_normal_map_tilts_like_geometry(texel):
+    """the normal-mapped plane shades as the plane tilted to the mapped normal: NEE from a light straight down
+    is the same function of (shading normal, view, light) for both, to rounding"""
+    from test_nee_oracle import down_camera
+    flat, tilted, n = normal_map_scenes(texel)
+    light = vk.light(vk.LIGHT_DIRECTIONAL, direction=(0, -1, 0), intensity=2.0)
+    a = orc.pt_render(flat, down_camera(), 8, 8, 64, 2, 4, lights=[light])[..., :3]
+    b = orc.pt_render(tilted, down_camera(), 8, 8, 64, 2, 4, lights=[light])[..., :3]
+    assert np.allclose(a, b, rtol=1e-5, atol=1e-7), (np.abs(a - b).max(), n)
+    flat["materials"]["normal_texture"] = 0
+    c = orc.pt_render(flat, down_camera(), 8, 8, 64, 2, 4, lights=[light])[..., :3]
+    assert np.abs(a - c).max() > 0.01 * a.mean()
+
+
+def test_obj_tangents_follow_compute_tangents(tmp_path):
+    """a quad in the xz plane, u along +x and v along +z, normal +y: the s direction is +x and the handedness
+    test dot(cross(n, t), b) = dot(-z, +z) < 0 keeps its sign (Geometry.cpp:242-253); a normal map in MTL's
+    `norm` fills the material's normal slot"""
+    Image = pytest.importorskip("PIL.Image")
+    Image.fromarray(np.full((2, 2, 4), 128, np.uint8)).save(tmp_path / "n.png")
+    (tmp_path / "q.mtl").write_text("newmtl m\nnorm n.png\n")
+    (tmp_path / "q.obj").write_text("mtllib q.mtl\nv 0 0 0\nv 1 0 0\nv 1 0 1\nv 0 0 1\nvt 0 0\nvt 1 0\nvt 1 1\n"
+                                    "vt 0 1\nvn 0 1 0\nusemtl m\nf 1/1/1 4/4/1 3/3/1 2/2/1\n")
+    m = vk.load_model(str(tmp_path / "q.obj"))
+    assert np.array_equal(m["tangents"], np.tile(np.float32([1, 0, 0]), (6, 1)))
+    assert m["materials"][0]["normal_texture"] == 1 and m["materials"][0]["color_texture"] == 0
+    # no v flip without a diffuse texture
+    assert np.array_equal(m["texcoords"][:3], np.float32([[0, 0], [0, 1], [1, 1]]))
+
+
+def test_gltf_texture_slots_and_tangents(tmp_path):
+    """glTF texture slots (metallicRoughness, normal, emissive with the factor zeroed, KHR transmission and
+    diffuse transmission textures) and a TANGENT attribute (xyz kept, w dropped) turned by the node's
+    rotation; a primitive without TANGENT gets compute_tangents"""
+    import base64
+    import io
+    import json
+    from PIL import Image
+    buf = io.BytesIO()
+    Image.fromarray(np.full((2, 2, 4), 200, np.uint8), "RGBA").save(buf, format="PNG")
+    png = buf.getvalue()
+    pos = np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    uv = np.float32([[0, 0], [1, 0], [0, 1]])
+    nrm = np.float32([[0, 0, 1]] * 3)
+    tg = np.float32([[1, 0, 0, -1]] * 3)
+    data = pos.tobytes() + uv.tobytes() + nrm.tobytes() + tg.tobytes() + png
+    data += b"\0" * (-len(data) % 4)
+    views = [(0, 36), (36, 24), (60, 36), (96, 48), (144, len(png))]
+    tex = lambda i: {"index": i}
+    doc = {"asset": {"version": "2.0"}, "scenes": [{"nodes": [0, 1]}],
+           "nodes": [{"mesh": 0, "rotation": [0, 0.7071067811865476, 0, 0.7071067811865476]}, {"mesh": 1}],
+           "meshes": [{"primitives": [{"attributes": {"POSITION": 0, "TEXCOORD_0": 1, "NORMAL": 2, "TANGENT": 3},
+                                       "material": 0}]},
+                      {"primitives": [{"attributes": {"POSITION": 0, "TEXCOORD_0": 1, "NORMAL": 2}, "material": 0}]}],
+           "materials": [{"pbrMetallicRoughness": {"metallicRoughnessTexture": tex(0)}, "normalTexture": tex(0),
+                          "emissiveFactor": [1, 1, 1], "emissiveTexture": tex(0),
+                          "extensions": {"KHR_materials_transmission": {"transmissionTexture": tex(0)},
+                                         "KHR_materials_diffuse_transmission": {
+                                             "diffuseTransmissionTexture": tex(0),
+                                             "diffuseTransmissionColorTexture": tex(0)}}}],
+           "textures": [{"source": 0}], "images": [{"bufferView": 4, "mimeType": "image/png"}],
+           "buffers": [{"byteLength": len(data),
+                        "uri": "data:application/octet-stream;base64," + base64.b64encode(data).decode()}],
+           "bufferViews": [{"buffer": 0, "byteOffset": o, "byteLength": n} for o, n in views],
+           "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"},
+                         {"bufferView": 1, "componentType": 5126, "count": 3, "type": "VEC2"},
+                         {"bufferView": 2, "componentType": 5126, "count": 3, "type": "VEC3"},
+                         {"bufferView": 3, "componentType": 5126, "count": 3, "type": "VEC4"}]}
+    (tmp_path / "s.gltf").write_text(json.dumps(doc))
+    m = vk.load_model(str(tmp_path / "s.gltf"))
+    mat = m["materials"][0]
+    for slot in ("ao_rough_metal_texture", "normal_texture", "emission_texture", "transmission_texture",
+                 "diffuse_transmission_texture", "diffuse_transmission_color_texture"):
+        assert mat[slot] == 1, slot
+    assert np.array_equal(mat["emission"][:3], [0, 0, 0])
+    # rotation by 90 degrees about +y turns +x into -z
+    assert np.allclose(m["tangents"][:3], np.tile([0, 0, -1], (3, 1)), atol=1e-7)
+    # computed for the second primitive: s direction +x, normal +z, b = +y, cross(n, t) = +y -> sign -1
+    assert np.array_equal(m["tangents"][3:], np.tile(np.float32([-1, 0, 0]), (3, 1)))

@@ -61,7 +61,8 @@ class Texture(ctypes.Structure):
 
 class ModelView(ctypes.Structure):
     _fields_ = [("positions", ctypes.c_void_p), ("normals", ctypes.c_void_p), ("texcoords", ctypes.c_void_p),
-                ("n_vert", ctypes.c_uint32), ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32),
+                ("tangents", ctypes.c_void_p), ("n_vert", ctypes.c_uint32), ("indices", ctypes.c_void_p),
+                ("n_tri", ctypes.c_uint32),
                 ("tri_object", ctypes.c_void_p), ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
                 ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("textures", ctypes.c_void_p),
                 ("n_textures", ctypes.c_uint32), ("n_skipped_textures", ctypes.c_uint32)]
@@ -72,7 +73,8 @@ class PtScene(ctypes.Structure):
                 ("indices", ctypes.c_void_p), ("n_tri", ctypes.c_uint32), ("tri_object", ctypes.c_void_p),
                 ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
                 ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float),
-                ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("n_textures", ctypes.c_uint32)]
+                ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("n_textures", ctypes.c_uint32),
+                ("tangents", ctypes.c_void_p)]
 
 
 class Camera(ctypes.Structure):
@@ -90,7 +92,11 @@ MATERIAL_DTYPE = np.dtype([("color", "<f4", 4), ("emission", "<f4", 4), ("metaln
                            ("clearcoat_roughness", "<f4"), ("sheen_roughness", "<f4"),
                            ("diffuse_transmission", "<f4"), ("sheen_color", "<f4", 4),
                            ("diffuse_transmission_color", "<f4", 4), ("two_sided", "<u4"), ("color_texture", "<u4"),
-                           ("alpha_mode", "<u4"), ("alpha_cutoff", "<f4")])
+                           ("alpha_mode", "<u4"), ("alpha_cutoff", "<f4"), ("normal_texture", "<u4"),
+                           ("emission_texture", "<u4"), ("ao_rough_metal_texture", "<u4"),
+                           ("transmission_texture", "<u4"), ("diffuse_transmission_texture", "<u4"),
+                           ("diffuse_transmission_color_texture", "<u4"), ("reserved", "<u4", 2)])
+assert MATERIAL_DTYPE.itemsize == 144
 ALPHA_OPAQUE, ALPHA_MASK, ALPHA_BLEND = 0, 1, 2
 PIXEL_DTYPE = np.dtype([("radiance", "<f4", 3), ("alpha", "<f4"), ("depth", "<f4"), ("normal", "<u4"),
                         ("object_id", "<u4"), ("albedo", "<u4")])
@@ -98,12 +104,14 @@ PIXEL_DTYPE = np.dtype([("radiance", "<f4", 3), ("alpha", "<f4"), ("depth", "<f4
 
 def material(color=(0.8, 0.8, 0.8), emission=(0.0, 0.0, 0.0), emission_strength=0.0, metalness=0.0, roughness=1.0,
              transmission=0.0, ior=1.5, two_sided=False, alpha=1.0, color_texture=0, alpha_mode=0,
-             alpha_cutoff=0.5):
+             alpha_cutoff=0.5, **texture_slots):
     """one vkrt_material record (material_t defaults: alpha 1, no clearcoat/sheen/diffuse transmission);
     color_texture is 1 + the index of the scene's base color texture (0: none)"""
     m = np.zeros((), dtype=MATERIAL_DTYPE)
     m["color"] = (*color, alpha)
     m["color_texture"], m["alpha_mode"], m["alpha_cutoff"] = color_texture, alpha_mode, alpha_cutoff
+    for k, v in texture_slots.items():  # normal_texture, emission_texture, ao_rough_metal_texture, ...
+        m[k] = v
     m["emission"] = (*emission, emission_strength)
     m["metalness"], m["roughness"], m["transmission"], m["ior"] = metalness, roughness, transmission, ior
     m["two_sided"] = 1 if two_sided else 0
@@ -689,19 +697,23 @@ class PathTracer:
 
     def set_scene(self, scene):
         """scene: dict with positions [n,3] f32, normals [n,3] f32, indices u32, tri_object u32,
-        object_material u32, materials (MATERIAL_DTYPE array), environment float; optional texcoords [n,2] f32
-        and textures (a list of [h,w,4] uint8 RGBA images, referenced by materials' color_texture = index + 1)"""
+        object_material u32, materials (MATERIAL_DTYPE array), environment float; optional texcoords [n,2] f32,
+        tangents [n,3] f32 (for normal maps) and textures (a list of [h,w,4] uint8 RGBA images, referenced by the
+        materials' texture slots as index + 1)"""
         arr = {k: np.ascontiguousarray(scene[k]) for k in
                ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
         uv = scene.get("texcoords")
         uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
+        tg = scene.get("tangents")
+        tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
         imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
         tex = (Texture * max(1, len(imgs)))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in imgs])
         sc = PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
                      arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
                      arr["object_material"].ctypes.data, len(arr["object_material"]),
                      arr["materials"].ctypes.data, len(arr["materials"]), float(scene.get("environment", 1.0)),
-                     uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs))
+                     uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
+                     tg.ctypes.data if tg is not None else None)
         self.ctx.check(lib().vkrt_pt_set_scene(self._h, ctypes.byref(sc)))
 
     def set_lights(self, lights):
@@ -945,6 +957,7 @@ def load_model(path, environment=1.0):
         return {"positions": arr(v.positions, 3 * v.n_vert, np.float32).reshape(-1, 3),
                 "normals": arr(v.normals, 3 * v.n_vert, np.float32).reshape(-1, 3),
                 "texcoords": arr(v.texcoords, 2 * v.n_vert, np.float32).reshape(-1, 2),
+                "tangents": arr(v.tangents, 3 * v.n_vert, np.float32).reshape(-1, 3),
                 "indices": arr(v.indices, 3 * v.n_tri, np.uint32),
                 "tri_object": arr(v.tri_object, v.n_tri, np.uint32),
                 "object_material": arr(v.object_material, v.n_objects, np.uint32),

@@ -343,6 +343,49 @@ void compute_vertex_normals(const std::vector<float> &pos, const std::vector<uin
     }
 }
 
+//! Geometry::compute_tangents (src/Geometry.cpp:196-254): per triangle the texture-space s and t directions,
+//! summed per vertex in index order; then Gram-Schmidt against the normal, the sign from the handedness test
+//! as the reference writes it (flipped when dot(cross(n, t), b) is not negative). Degenerate coordinates give
+//! non-finite tangents, as there.
+void compute_tangents(const std::vector<float> &pos, const std::vector<float> &nrm, const std::vector<float> &uv,
+                      const std::vector<uint32_t> &idx, std::vector<float> &tan)
+{
+    const size_t nv = pos.size() / 3;
+    tan.assign(3 * nv, 0.f);
+    if(idx.size() % 3 || uv.size() != 2 * nv || nrm.size() != 3 * nv) return;
+    std::vector<float> ts(3 * nv, 0.f), bs(3 * nv, 0.f);
+    for(size_t i = 0; i < idx.size(); i += 3)
+    {
+        const uint32_t a = idx[i], b = idx[i + 1], c = idx[i + 2];
+        const float x1 = pos[3 * b] - pos[3 * a], x2 = pos[3 * c] - pos[3 * a];
+        const float y1 = pos[3 * b + 1] - pos[3 * a + 1], y2 = pos[3 * c + 1] - pos[3 * a + 1];
+        const float z1 = pos[3 * b + 2] - pos[3 * a + 2], z2 = pos[3 * c + 2] - pos[3 * a + 2];
+        const float s1 = uv[2 * b] - uv[2 * a], s2 = uv[2 * c] - uv[2 * a];
+        const float t1 = uv[2 * b + 1] - uv[2 * a + 1], t2 = uv[2 * c + 1] - uv[2 * a + 1];
+        const float r = 1.f / (s1 * t2 - s2 * t1);
+        const float sd[3] = {(t2 * x1 - t1 * x2) * r, (t2 * y1 - t1 * y2) * r, (t2 * z1 - t1 * z2) * r};
+        const float td[3] = {(s1 * x2 - s2 * x1) * r, (s1 * y2 - s2 * y1) * r, (s1 * z2 - s2 * z1) * r};
+        for(uint32_t v: {a, b, c})
+            for(int k = 0; k < 3; ++k)
+            {
+                ts[3 * v + k] += sd[k];
+                bs[3 * v + k] += td[k];
+            }
+    }
+    for(size_t v = 0; v < nv; ++v)
+    {
+        const v3f n{nrm[3 * v], nrm[3 * v + 1], nrm[3 * v + 2]}, t{ts[3 * v], ts[3 * v + 1], ts[3 * v + 2]};
+        const v3f bt{bs[3 * v], bs[3 * v + 1], bs[3 * v + 2]};
+        const float nt = (n.x * t.x + n.y * t.y) + n.z * t.z;
+        const v3f o = normalize(v3f{t.x - n.x * nt, t.y - n.y * nt, t.z - n.z * nt});
+        const v3f c{n.y * t.z - t.y * n.z, n.z * t.x - t.z * n.x, n.x * t.y - t.x * n.y};
+        const float sign = ((c.x * bt.x + c.y * bt.y) + c.z * bt.z) < 0.f ? 1.f : -1.f;
+        tan[3 * v] = o.x * sign;
+        tan[3 * v + 1] = o.y * sign;
+        tan[3 * v + 2] = o.z * sign;
+    }
+}
+
 //! vierkant::material_t defaults (include/vierkant/Material.hpp:85-142) in the tracer's record
 vkrt_material default_material()
 {
@@ -545,7 +588,7 @@ bool decode_png(const uint8_t *data, size_t size, image_rgba8 &out, std::string 
 //! instance) with one material each
 struct vkrt_model
 {
-    std::vector<float> positions, normals, texcoords;
+    std::vector<float> positions, normals, texcoords, tangents;
     std::vector<uint32_t> indices, tri_object, object_material;
     std::vector<vkrt_material> materials;
     std::vector<std::string> object_names, material_names;
@@ -574,13 +617,16 @@ struct vkrt_model
 
     //! append one object's geometry (indices relative to its own vertices)
     void add_object(const std::string &name, const std::vector<float> &pos, const std::vector<float> &nrm,
-                    const std::vector<float> &uv, const std::vector<uint32_t> &idx, uint32_t material)
+                    const std::vector<float> &uv, const std::vector<float> &tan, const std::vector<uint32_t> &idx,
+                    uint32_t material)
     {
         const uint32_t base = static_cast<uint32_t>(positions.size() / 3);
         positions.insert(positions.end(), pos.begin(), pos.end());
         normals.insert(normals.end(), nrm.begin(), nrm.end());
         if(uv.size() == pos.size() / 3 * 2) texcoords.insert(texcoords.end(), uv.begin(), uv.end());
         else texcoords.resize(positions.size() / 3 * 2, 0.f);
+        if(tan.size() == pos.size()) tangents.insert(tangents.end(), tan.begin(), tan.end());
+        else tangents.resize(positions.size(), 0.f);
         for(uint32_t i: idx) indices.push_back(base + i);
         tri_object.resize(indices.size() / 3, n_objects);
         object_material.push_back(material);
@@ -640,7 +686,7 @@ std::vector<obj_material> parse_mtl(const std::string &text)
         else if(key == "Pm") ls >> m.metallic;
         else if(key == "Pcr") ls >> m.clearcoat_roughness;
         else if(key == "map_Kd") std::getline(ls >> std::ws, m.diffuse_tex);
-        else if(key == "norm" || key == "map_Bump" || key == "bump") std::getline(ls >> std::ws, m.normal_tex);
+        else if(key == "norm") std::getline(ls >> std::ws, m.normal_tex);// tinyobj's normal_texname
     }
     return out;
 }
@@ -761,27 +807,28 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
     // a material with a diffuse texture gets a texture transform that flips v (wavefront_obj.cpp:100-111); it
     // is applied here, to the coordinates of the faces that use it
     std::vector<bool> flip_v;
+    auto load_texture = [&](const std::string &name) -> uint32_t {
+        auto it = tex_cache.find(name);
+        if(it == tex_cache.end())
+        {
+            uint32_t id = 0;
+            std::ifstream f(base + name, std::ios::binary);
+            if(f)
+            {
+                const std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+                id = model->add_image(reinterpret_cast<const uint8_t *>(bytes.data()), bytes.size());
+            }
+            else ++model->n_skipped_textures;
+            it = tex_cache.emplace(name, id).first;
+        }
+        return it->second;
+    };
     for(const obj_material &m: mtl)
     {
         vkrt_material r = default_material();
         flip_v.push_back(!m.diffuse_tex.empty());
-        if(!m.diffuse_tex.empty())
-        {
-            auto it = tex_cache.find(m.diffuse_tex);
-            if(it == tex_cache.end())
-            {
-                uint32_t id = 0;
-                std::ifstream f(base + m.diffuse_tex, std::ios::binary);
-                if(f)
-                {
-                    const std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
-                    id = model->add_image(reinterpret_cast<const uint8_t *>(bytes.data()), bytes.size());
-                }
-                else ++model->n_skipped_textures;
-                it = tex_cache.emplace(m.diffuse_tex, id).first;
-            }
-            r.color_texture = it->second;
-        }
+        if(!m.diffuse_tex.empty()) r.color_texture = load_texture(m.diffuse_tex);
+        if(!m.normal_tex.empty()) r.normal_texture = load_texture(m.normal_tex);
         r.color[0] = m.diffuse[0];
         r.color[1] = m.diffuse[1];
         r.color[2] = m.diffuse[2];
@@ -825,7 +872,11 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
             idx[i] = static_cast<uint32_t>(i);
         }
         if(!has_normals) compute_vertex_normals(pos, idx, nrm);
-        model->add_object(sh.name, pos, nrm, uv, idx, material);
+        // texture coordinates are zero-filled per corner, so the reference computes tangents for every shape
+        // (wavefront_obj.cpp:21, :44)
+        std::vector<float> tan;
+        compute_tangents(pos, nrm, uv, idx, tan);
+        model->add_object(sh.name, pos, nrm, uv, tan, idx, material);
     }
     return model;
 }
@@ -1096,6 +1147,16 @@ vkrt_material convert_material(const json &jm, F &&texture_of)
         for(int k = 0; k < 3; ++k)
             m.diffuse_transmission_color[k] = static_cast<float>(dt["diffuseTransmissionColorFactor"][k].number(1.0));
     }
+    // the other texture slots (gltf.cpp:366-378, :470-476, :622-632); occlusion only fills the red channel
+    // of the rough/metal image, which the tracer does not read
+    auto slot = [&](const json &info) -> uint32_t { return info.has("index") ? texture_of(info["index"].integer(-1)) : 0u; };
+    m.ao_rough_metal_texture = slot(pbr["metallicRoughnessTexture"]);
+    m.normal_texture = slot(jm["normalTexture"]);
+    m.emission_texture = slot(jm["emissiveTexture"]);
+    m.transmission_texture = slot(ext["KHR_materials_transmission"]["transmissionTexture"]);
+    m.diffuse_transmission_texture = slot(ext["KHR_materials_diffuse_transmission"]["diffuseTransmissionTexture"]);
+    m.diffuse_transmission_color_texture =
+            slot(ext["KHR_materials_diffuse_transmission"]["diffuseTransmissionColorTexture"]);
     return m;
 }
 
@@ -1253,6 +1314,27 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
                     if(comps == 2 && t.size() == 2 * nv)
                         for(double x: t) uv.push_back(static_cast<float>(x));
                 }
+                // tangents: the file's TANGENT (its xyz; create_geometry's strided copy drops w) turned by the
+                // node, else Geometry::compute_tangents from the file's coordinates (gltf.cpp:277, :302)
+                std::vector<float> tan;
+                if(attr.has("TANGENT"))
+                {
+                    const std::vector<double> tv = read_accessor(g, attr["TANGENT"].integer(-1), comps);
+                    if((comps == 3 || comps == 4) && tv.size() == nv * comps)
+                    {
+                        tan.resize(3 * nv);
+                        for(size_t k = 0; k < nv; ++k)
+                        {
+                            const double *t = tv.data() + k * comps;
+                            double w[3];
+                            for(int r = 0; r < 3; ++r) w[r] = world[r] * t[0] + world[4 + r] * t[1] + world[8 + r] * t[2];
+                            const double l = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+                            const double sc = (l > 0 && std::fabs(l - 1.0) > 1e-6) ? 1.0 / l : 1.0;
+                            for(int r = 0; r < 3; ++r) tan[3 * k + r] = static_cast<float>(w[r] * sc);
+                        }
+                    }
+                }
+                if(tan.empty() && !uv.empty()) compute_tangents(pos, nrm, uv, idx, tan);
                 int mi = prim["material"].integer(-1);
                 if(mi < 0 || mi >= static_cast<int>(default_index))
                 {
@@ -1269,7 +1351,7 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
                         uv[k + 1] = (m[2] * u + m[3] * v) + m[5];
                     }
                 }
-                model->add_object(mesh["name"].str, pos, nrm, uv, idx, static_cast<uint32_t>(mi));
+                model->add_object(mesh["name"].str, pos, nrm, uv, tan, idx, static_cast<uint32_t>(mi));
             }
         }
         const json &ch = node["children"];
@@ -1334,6 +1416,7 @@ int vkrt_model_get(const vkrt_model *m, vkrt_model_view *out)
     out->positions = m->positions.data();
     out->normals = m->normals.data();
     out->texcoords = m->texcoords.data();
+    out->tangents = m->tangents.data();
     out->n_vert = static_cast<uint32_t>(m->positions.size() / 3);
     out->indices = m->indices.data();
     out->n_tri = static_cast<uint32_t>(m->indices.size() / 3);
@@ -1355,7 +1438,7 @@ int vkrt_model_pt_scene(const vkrt_model *m, float environment, vkrt_pt_scene *o
     if(rc != VKRT_OK) return rc;
     *out = vkrt_pt_scene{v.positions, v.normals, v.n_vert, v.indices, v.n_tri, v.tri_object, v.object_material,
                          v.n_objects, v.materials, v.n_materials, environment, v.texcoords, v.textures,
-                         v.n_textures};
+                         v.n_textures, v.tangents};
     return VKRT_OK;
 }
 

@@ -81,6 +81,8 @@ struct gpu_material
     float4 sheen;    // sheen color rgb, two_sided flag
     float4 dtc;      // diffuse transmission color rgb, -
     uint4 tex;       // 1 + color texture (0: none), alpha mode, alpha cutoff bits, -
+    uint4 tex2;      // 1 + normal, emission, ao/rough/metal, transmission texture
+    uint4 tex3;      // 1 + diffuse transmission, diffuse transmission color texture, -, -
 };
 
 constexpr uint32_t MAX_TEX_LEVELS = 16;
@@ -140,6 +142,7 @@ struct pt_scene_dev
     const float2 *uv;         // texture coordinates per vertex
     const tex_desc *tex;      // texture descriptors
     const uint32_t *texels;   // RGBA8 mip levels of all textures
+    const float *tan;         // tangents per vertex (normal maps)
 };
 
 __device__ __forceinline__ vec3 ld3f(const float *p, uint32_t i) { return vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
@@ -477,30 +480,9 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         const vec3 V = -dir;
         ro = make_float4(position.x, position.y, position.z, ro.w);
 
-        vec3 base = xyz(gm.color);
-        float alpha = gm.color.w;
-        if(gm.tex.x)
-        {
-            // the base color texture at the ray cone's level of detail (raypipeline.slang:300-321, 526-531):
-            // lambda = 0.5 log2(uv area / world area) + log2|cone width| + 0.5 log2(w h) - log2(N.V)
-            const float2 t0 = s.uv[i0], t1 = s.uv[i1], t2 = s.uv[i2];
-            const float tu = (t0.x * bx + t1.x * by) + t2.x * bz, tv = (t0.y * bx + t1.y * by) + t2.y * bz;
-            const float p_a = length3(vierkant::rt::cross(v1 - v0, v2 - v0));
-            const float t_a = fabsf((t1.x - t0.x) * (t2.y - t0.y) - (t2.x - t0.x) * (t1.y - t0.y));
-            const tex_desc &td = s.tex[gm.tex.x - 1u];
-            float lambda = 0.5f * log2f(t_a / p_a);
-            lambda += log2f(fabsf(cone.y));
-            lambda += 0.5f * log2f(static_cast<float>(td.width) * static_cast<float>(td.height));
-            lambda -= log2f(fabsf(dot(V, normal)));
-            const float4 c = sample_texture(td, s.texels, tu, tv, lambda);
-            base = vec3{base.x * c.x, base.y * c.y, base.z * c.z};
-            alpha = alpha * c.w;
-        }
-        // propagate(cone, 0, RayTCurrent()) (raypipeline.slang:280-286, :608), stored as halves
-        cone.y = half_value(cone.y + cone.x * __uint_as_float(h.x));
         material m;
-        m.color = f.disable_material ? splat(0.8f) : base;
-        m.alpha = f.disable_material ? 1.0f : alpha;
+        m.color = xyz(gm.color);
+        m.alpha = gm.color.w;
         m.emission = xyz(gm.emission);
         m.emission_strength = gm.emission.w;
         m.metalness = gm.mrti.x;
@@ -513,6 +495,75 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         m.diffuse_transmission = gm.cc.w;
         m.sheen_color = xyz(gm.sheen);
         m.diffuse_transmission_color = xyz(gm.dtc);
+        if(gm.tex.x | gm.tex2.x | gm.tex2.y | gm.tex2.z | gm.tex2.w | gm.tex3.x | gm.tex3.y)
+        {
+            // every texture at the ray cone's level of detail (raypipeline.slang:300-321, 368-375, 526-606):
+            // lambda = 0.5 log2(uv area / world area) + log2|cone width| + 0.5 log2(w h) - log2(N.V), with the
+            // width from before this segment's propagation
+            const float2 t0 = s.uv[i0], t1 = s.uv[i1], t2 = s.uv[i2];
+            const float tu = (t0.x * bx + t1.x * by) + t2.x * bz, tv = (t0.y * bx + t1.y * by) + t2.y * bz;
+            const float p_a = length3(vierkant::rt::cross(v1 - v0, v2 - v0));
+            const float t_a = fabsf((t1.x - t0.x) * (t2.y - t0.y) - (t2.x - t0.x) * (t1.y - t0.y));
+            const float lod_cone = 0.5f * log2f(t_a / p_a) + log2f(fabsf(cone.y));
+            const float lod_nov = log2f(fabsf(dot(V, normal)));
+            auto lookup = [&](uint32_t slot) {
+                const tex_desc &td = s.tex[slot - 1u];
+                float lambda = lod_cone;
+                lambda += 0.5f * log2f(static_cast<float>(td.width) * static_cast<float>(td.height));
+                lambda -= lod_nov;
+                return sample_texture(td, s.texels, tu, tv, lambda);
+            };
+            if(gm.tex.x)
+            {
+                const float4 c = lookup(gm.tex.x);
+                m.color = vec3{m.color.x * c.x, m.color.y * c.y, m.color.z * c.z};
+                m.alpha = m.alpha * c.w;
+            }
+            if(gm.tex2.x)
+            {
+                // tangent-space normal map, bitangent handedness flipped on back faces, hemisphere clamp
+                const float4 c = lookup(gm.tex2.x);
+                vec3 nt;
+                nt.x = 2.0f * (c.x - 0.5f);
+                nt.y = 2.0f * (c.y - 0.5f);
+                nt.z = sqrtf(fmaxf(0.0f, 1.0f - nt.x * nt.x - nt.y * nt.y));
+                const vec3 tg = normalize3((ld3f(s.tan, i0) * bx + ld3f(s.tan, i1) * by) + ld3f(s.tan, i2) * bz);
+                vec3 bt = normalize3(vierkant::rt::cross(normal, tg));
+                if(backface) bt = -bt;
+                const vec3 nm = normalize3((tg * nt.x + bt * nt.y) + normal * nt.z);
+                normal = normalize3(nm + normal * fmaxf(0.0f, -dot(nm, normal) + 1e-3f));
+            }
+            if(gm.tex2.y)
+            {
+                const float4 c = lookup(gm.tex2.y);
+                if(m.emission.x > 0.0f || m.emission.y > 0.0f || m.emission.z > 0.0f)
+                    m.emission = vec3{m.emission.x * c.x, m.emission.y * c.y, m.emission.z * c.z};
+                else
+                    m.emission = vec3{c.x, c.y, c.z};
+            }
+            if(gm.tex2.z)
+            {
+                const float4 c = lookup(gm.tex2.z);
+                m.roughness = m.roughness * c.y;
+                m.metalness = m.metalness * c.z;
+            }
+            if(gm.tex2.w) m.transmission = m.transmission * lookup(gm.tex2.w).x;
+            if(gm.tex3.x) m.diffuse_transmission = m.diffuse_transmission * lookup(gm.tex3.x).w;
+            if(gm.tex3.y)
+            {
+                const float4 c = lookup(gm.tex3.y);
+                m.diffuse_transmission_color = vec3{m.diffuse_transmission_color.x * c.x,
+                                                    m.diffuse_transmission_color.y * c.y,
+                                                    m.diffuse_transmission_color.z * c.z};
+            }
+        }
+        if(f.disable_material)
+        {
+            m.color = splat(0.8f);
+            m.alpha = 1.0f;
+        }
+        // propagate(cone, 0, RayTCurrent()) (raypipeline.slang:280-286, :608), stored as halves
+        cone.y = half_value(cone.y + cone.x * __uint_as_float(h.x));
 
         feat[0] = make_float4(position.x, position.y, position.z, m.roughness);
         feat[1] = make_float4(V.x, V.y, V.z, m.metalness);
@@ -531,7 +582,8 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         float eta = backface ? m.ior / last_ior : media_ior / m.ior;
         eta += EPS;
         next_ior = m.ior;
-        const bool media_boundary = m.transmission > 0.0f;
+        // decided on the material's own transmission, before its texture (raypipeline.slang:517-522)
+        const bool media_boundary = gm.mrti.z > 0.0f;
         media_op = !media_boundary ? 0u : (backface ? 2u : 1u);
         if(depth == 0) albedo = m.color;
         const vec3 sn = normal;
@@ -1155,7 +1207,7 @@ struct vkrt_pt
     float environment = 1.f;
     vkrt::device_buffer pos, nrm, idx, tri_object, object_material, object_tmax, mats;
     // texture coordinates, texture descriptors and their texels (all mip levels)
-    vkrt::device_buffer uv, tex_desc_buf, texels;
+    vkrt::device_buffer uv, tan, tex_desc_buf, texels;
     uint32_t width = 0, height = 0;
     vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels, cone;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1259,7 +1311,7 @@ static pt_scene_dev scene_dev(vkrt_pt *pt)
     return pt_scene_dev{pt->pos.as<float>(), pt->nrm.as<float>(), pt->idx.as<uint32_t>(),
                         pt->tri_object.as<uint32_t>(), pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(),
                         pt->mats.as<gpu_material>(), pt->lights.as<vkrt_light>(), pt->uv.as<float2>(),
-                        pt->tex_desc_buf.as<tex_desc>(), pt->texels.as<uint32_t>()};
+                        pt->tex_desc_buf.as<tex_desc>(), pt->texels.as<uint32_t>(), pt->tan.as<float>()};
 }
 
 //! f32tof16 on the host: round to nearest even at the binade's spacing of halves (2^-24 below 2^-14),
@@ -1626,12 +1678,19 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
             std::memcpy(&cutoff_bits, &x.alpha_cutoff, 4);
             // MASK and BLEND drive the any-hit test
             mats[m].tex = make_uint4(x.color_texture, x.alpha_mode, cutoff_bits, 0u);
+            const uint32_t slots[6] = {x.normal_texture, x.emission_texture, x.ao_rough_metal_texture,
+                                       x.transmission_texture, x.diffuse_transmission_texture,
+                                       x.diffuse_transmission_color_texture};
+            for(uint32_t k: slots) VKRT_REQUIRE(k <= sc->n_textures, "material texture index out of range");
+            mats[m].tex2 = make_uint4(slots[0], slots[1], slots[2], slots[3]);
+            mats[m].tex3 = make_uint4(slots[4], slots[5], 0u, 0u);
         }
         // texture coordinates (zeros when the scene has none) and the textures' mip chains: each level halves
         // the previous one (at least 1 texel) with a 2 x 2 box filter, coordinates clamped at odd edges,
         // rounded to nearest ((sum + 2) / 4)
-        std::vector<float> uv(2 * (size_t)sc->n_vert, 0.f);
+        std::vector<float> uv(2 * (size_t)sc->n_vert, 0.f), tan(3 * (size_t)sc->n_vert, 0.f);
         if(sc->texcoords) std::memcpy(uv.data(), sc->texcoords, uv.size() * 4);
+        if(sc->tangents) std::memcpy(tan.data(), sc->tangents, tan.size() * 4);
         VKRT_REQUIRE(sc->n_textures == 0 || sc->textures, "null texture array");
         std::vector<tex_desc> descs(sc->n_textures);
         std::vector<uint32_t> texels;
@@ -1691,6 +1750,7 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         upload(pt->object_tmax, tmax.data(), tmax.size() * 4);
         upload(pt->mats, mats.data(), mats.size() * sizeof(gpu_material));
         upload(pt->uv, uv.data(), uv.size() * 4);
+        upload(pt->tan, tan.data(), tan.size() * 4);
         upload(pt->tex_desc_buf, descs.data(), descs.size() * sizeof(tex_desc));
         upload(pt->texels, texels.data(), texels.size() * 4);
         VKRT_HIP(hipStreamSynchronize(s));
