@@ -480,9 +480,12 @@ typedef struct vkrt_pt_scene
  * per shape with the material of its first face. glTF: one object per triangle primitive of every mesh node
  * reachable from the scene, node transforms baked into world-space vertices (double precision, rounded
  * once), material factors of the core spec and of KHR_materials_emissive_strength / transmission / ior /
- * clearcoat / sheen / diffuse_transmission, base color textures (PNG) with alphaMode / alphaCutoff; OBJ map_Kd
- * textures are flipped vertically as the reference does (wavefront_obj.cpp:103-109). Host code only: no context
- * or GPU is needed. Errors: VKRT_E_INVALID_ARG with the message in vkrt_model_error (per thread). */
+ * clearcoat / sheen / diffuse_transmission, PNG textures in every slot the tracer samples (base color with
+ * alphaMode / alphaCutoff, metallicRoughness, normal, emissive, transmission, diffuse transmission and its
+ * color) with the base color texture's KHR_texture_transform folded into the coordinates, tangents from
+ * TANGENT or Geometry::compute_tangents. OBJ: map_Kd and norm textures; a map_Kd material's coordinates are
+ * flipped vertically as the reference's texture transform does (wavefront_obj.cpp:100-111). Host code only:
+ * no context or GPU is needed. Errors: VKRT_E_INVALID_ARG with the message in vkrt_model_error (per thread). */
 typedef struct vkrt_model vkrt_model;
 typedef struct vkrt_model_view
 {
