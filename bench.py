@@ -301,6 +301,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify-rays", type=int, default=65536)
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
+    ap.add_argument("--traversal", choices=("default", "step", "postponed"), default="default",
+                    help="loop of the 4-wide walk (default: the library's)")
+    ap.add_argument("--xcd-chunk", type=int, default=-1,
+                    help="consecutive ray blocks per XCD turn of the trace launch (-1: the library's, 0: linear)")
     ap.add_argument("--pt-spp", type=int, default=64, help="samples per pixel of the path-tracer frame (0: skip)")
     ap.add_argument("--builder", choices=("lbvh", "ploc"), default="ploc", help="BVH builder of the headline scene")
     ap.add_argument("--ploc-radius", type=int, default=0, help="PLOC search radius (0: the library default, 16)")
@@ -374,9 +378,18 @@ def main():
     flags = vk.ASYNC
     ev = vk.HipEvents(5)
 
+    if args.traversal != "default":
+        block, lds, _, _ = ctx.get_trace_config()
+        ctx.set_trace_config(block, lds, vk.TRAVERSAL_POSTPONED if args.traversal == "postponed" else vk.TRAVERSAL_STEP)
+    if args.xcd_chunk >= 0:
+        block, lds, _, _ = ctx.get_trace_config()
+        ctx.set_trace_config(block, lds, xcd_chunk=args.xcd_chunk)
+
     if args.sweep:
-        for block, lds in [(b, l) for b in (64, 128, 256) for l in (0, 8, 16, 32)]:
-            ctx.set_trace_config(block, lds)
+        for block, lds, trav, order in [(b, l, t, o) for o in (0, 1, 2, 4, 8, 16, 32, 64, 256)
+                                        for t in (vk.TRAVERSAL_STEP, vk.TRAVERSAL_POSTPONED)
+                                        for b in (64, 256) for l in (16,)]:
+            ctx.set_trace_config(block, lds, trav, order)
             ctx.trace_closest(bvh, d_primary, out=d_hits_p)
             times = []
             for _ in range(5):
@@ -388,7 +401,8 @@ def main():
                 ev.sync(2)
                 times.append((ev.ms(0, 1), ev.ms(1, 2)))
             p_ms, b_ms = np.median([t[0] for t in times]), np.median([t[1] for t in times])
-            print(json.dumps({"block": block, "lds": lds, "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4),
+            print(json.dumps({"block": block, "lds": lds, "traversal": trav, "xcd_chunk": order, "primary_ms": round(p_ms, 4),
+                              "bounce_ms": round(b_ms, 4),
                               "grays_per_s": round(rays_per_step_of(primary, bounce) / ((p_ms + b_ms) * 1e-3) / 1e9, 4)}),
                   flush=True)
         return

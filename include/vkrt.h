@@ -273,15 +273,26 @@ int vkrt_scene_trace_any(vkrt_ctx *ctx, vkrt_scene *scene, const vkrt_ray *rays,
 /* instance count, top-level nodes, worst-case traversal stack (top level + deepest mesh, at most 96) */
 int vkrt_scene_get_info(const vkrt_scene *scene, uint32_t *n_instances, uint32_t *n_nodes, uint32_t *stack_bound);
 
-/* Launch shape of the traversal kernels: threads per block (64, 128, 256) and traversal-stack entries
- * kept in LDS per ray (0, 8, 16, 32; deeper entries live in scratch memory). Results do not depend on
- * it. No reference counterpart: the Vulkan driver owns the traversal there. */
+/* Launch shape of the traversal kernels: threads per block (64, 128, 256), traversal-stack entries
+ * kept in LDS per ray (0, 8, 16, 32; deeper entries live in scratch memory), the loop of the 4-wide
+ * walk (VKRT_TRAVERSAL_*) and the order in which workgroups take rays: xcd_chunk 0 gives workgroup b the
+ * rays [b B, (b + 1) B); xcd_chunk C (a power of two up to 256) hands C consecutive ray blocks to the
+ * workgroups that the round-robin dispatch sends to one XCD, so neighbouring rays share that XCD's L2.
+ * Results do not depend on it. No reference counterpart: the Vulkan driver owns the traversal there. */
 typedef struct vkrt_trace_config
 {
     uint32_t block_size;
     uint32_t lds_stack_depth;
-    uint32_t reserved[2];
+    uint32_t traversal;
+    uint32_t xcd_chunk;
 } vkrt_trace_config;
+
+enum
+{
+    VKRT_TRAVERSAL_STEP = 0,     /* one node or one leaf per loop step */
+    VKRT_TRAVERSAL_POSTPONED = 1 /* leaves parked until every lane of the wave holds one, then tested together */
+};
+
 
 int vkrt_set_trace_config(vkrt_ctx *ctx, const vkrt_trace_config *cfg);
 int vkrt_get_trace_config(const vkrt_ctx *ctx, vkrt_trace_config *cfg);

@@ -144,22 +144,28 @@ def test_bvh_heightfield_primary_and_bounce(ctx, heightfield_case):
 
 
 def test_trace_configs_give_identical_results(ctx, icosphere_case):
-    """every launch shape (block size x LDS stack depth) and both traversal layouts return the same records"""
+    """every launch shape (block size x LDS stack depth x loop of the 4-wide walk) and both traversal layouts
+    return the same records"""
     pos, idx, tris, rays, want = icosphere_case
     saved = ctx.get_trace_config()
     try:
         for width in (0, 2):
             bvh = ctx.bvh_build(pos, idx, width=width)
-            for block in (64, 128, 256):
-                for lds in (0, 8, 16, 32):
-                    ctx.set_trace_config(block, lds)
-                    tag = f"width={width} block={block} lds={lds}"
+            for block, lds, trav, order in [(b, l, t, o) for b in (64, 128, 256) for l in (0, 8, 16, 32)
+                                            for t in (vk.TRAVERSAL_STEP, vk.TRAVERSAL_POSTPONED)
+                                            for o in (0, 1, 16)]:
+                    ctx.set_trace_config(block, lds, trav, order)
+                    tag = f"width={width} block={block} lds={lds} traversal={trav} xcd_chunk={order}"
                     assert_hits_equal(ctx.trace_closest(bvh, rays), want, tag)
                     occ = ctx.trace_any(bvh, rays)
                     assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8)), tag
             bvh.close()
         with pytest.raises(vk.VkrtError):
             ctx.set_trace_config(96, 16)
+        with pytest.raises(vk.VkrtError):
+            ctx.set_trace_config(256, 16, 2)
+        with pytest.raises(vk.VkrtError):
+            ctx.set_trace_config(256, 16, 0, 3)
     finally:
         ctx.set_trace_config(*saved)
 
@@ -183,5 +189,17 @@ def test_wide_collapse_equals_binary_tree(ctx, heightfield_case, leaf):
     assert np.array_equal(ctx.trace_any(wide, shadow), ctx.trace_any(binary, shadow))
     sel = np.arange(0, len(bounce), 97)
     assert_hits_equal(ctx.trace_closest(wide, bounce[sel]), ctx.brute_closest(tris, bounce[sel]), "wide vs brute")
+    # both loops of the 4-wide walk and the workgroup orders give the same records
+    saved = ctx.get_trace_config()
+    try:
+        for trav in (vk.TRAVERSAL_STEP, vk.TRAVERSAL_POSTPONED):
+            for order in (0, 4, 256):
+                ctx.set_trace_config(saved[0], saved[1], trav, order)
+                tag = f"traversal={trav} xcd_chunk={order}"
+                assert_hits_equal(ctx.trace_closest(wide, primary), hb, tag + " primary")
+                assert_hits_equal(ctx.trace_closest(wide, bounce), ctx.trace_closest(binary, bounce), tag + " bounce")
+                assert np.array_equal(ctx.trace_any(wide, shadow), ctx.trace_any(binary, shadow)), tag
+    finally:
+        ctx.set_trace_config(*saved)
     wide.close()
     binary.close()

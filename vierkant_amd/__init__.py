@@ -41,7 +41,10 @@ class BvhParams(ctypes.Structure):
 
 class TraceConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("lds_stack_depth", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("traversal", ctypes.c_uint32), ("xcd_chunk", ctypes.c_uint32)]
+
+
+TRAVERSAL_STEP, TRAVERSAL_POSTPONED = 0, 1
 
 
 class PtSettings(ctypes.Structure):
@@ -550,13 +553,19 @@ class Context:
                                         ctypes.byref(params), _flags([positions, indices]), ctypes.byref(h)))
         return Bvh(self, h)
 
-    def set_trace_config(self, block_size, lds_stack_depth):
-        self.check(lib().vkrt_set_trace_config(self._h, ctypes.byref(TraceConfig(block_size, lds_stack_depth))))
+    def set_trace_config(self, block_size, lds_stack_depth, traversal=None, xcd_chunk=None):
+        """launch shape of the trace kernels; traversal TRAVERSAL_*, xcd_chunk 0 or a power of two up to 256
+        (None keeps the current value)"""
+        cur = self.get_trace_config()
+        traversal = cur[2] if traversal is None else traversal
+        xcd_chunk = cur[3] if xcd_chunk is None else xcd_chunk
+        self.check(lib().vkrt_set_trace_config(self._h, ctypes.byref(TraceConfig(block_size, lds_stack_depth,
+                                                                                 traversal, xcd_chunk))))
 
     def get_trace_config(self):
         c = TraceConfig()
         self.check(lib().vkrt_get_trace_config(self._h, ctypes.byref(c)))
-        return c.block_size, c.lds_stack_depth
+        return c.block_size, c.lds_stack_depth, c.traversal, c.xcd_chunk
 
     def trace_closest(self, bvh, rays, out=None, flags=0):
         n = _n(rays, 8)

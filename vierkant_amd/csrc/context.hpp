@@ -88,7 +88,8 @@ struct vkrt_ctx
     bool own_stream = false;
     std::string last_error;
     int cu_count = 256;
-    vkrt_trace_config trace_cfg{256, 16, {0, 0}};// fastest shape of the block x LDS-depth sweep on MI355X
+    // fastest shape of the block x LDS depth x loop x XCD chunk sweep on MI355X (1.8M-triangle scene)
+    vkrt_trace_config trace_cfg{64, 16, VKRT_TRAVERSAL_STEP, 32};
     //! staging slots for host-pointer calls (inputs/outputs copied through these)
     vkrt::device_buffer staging[8];
     //! scratch for multi-pass kernels (keys, counters)

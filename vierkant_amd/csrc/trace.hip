@@ -3,7 +3,8 @@
 // Replaces the RayTracer::trace_rays launch (src/RayTracer.cpp:66-135) and the driver's TraceRay /
 // ray-query traversal (shaders/slang/raypipeline.slang:182, ray/visibility_test.slang:45-87). The kernels
 // are instantiated for block sizes {64, 128, 256} and LDS stack depths {0, 8, 16, 32}; the context's
-// trace configuration (set directly or by vkrt_autotune_trace) selects one.
+// trace configuration (vkrt_set_trace_config) selects one, with the loop of the 4-wide walk and the
+// order in which workgroups take rays.
 #include <chrono>
 #include <vector>
 
@@ -15,14 +16,22 @@
 namespace vkrt
 {
 
-template<bool ANY, int BLOCK, int LDS_DEPTH>
+template<bool ANY, int BLOCK, int LDS_DEPTH, bool POSTPONE>
 __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                  const float4 *__restrict__ tris, int32_t root, bool empty,
                                                  const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide,
-                                                 const float4 *__restrict__ rays, uint32_t n, void *__restrict__ out)
+                                                 const float4 *__restrict__ rays, uint32_t n, void *__restrict__ out,
+                                                 uint32_t chunk)
 {
     __shared__ int32_t smem[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
-    uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    // chunk C > 0: workgroup b = (g C + j) 8 + x runs on XCD x and takes ray block g 8 C + x C + j
+    uint32_t blk = blockIdx.x;
+    if(chunk)
+    {
+        const uint32_t x = blk % 8u, j = (blk / 8u) % chunk, g = blk / (8u * chunk);
+        blk = g * 8u * chunk + x * chunk + j;
+    }
+    uint32_t i = blk * BLOCK + threadIdx.x;
     if(i >= n) return;
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
@@ -30,8 +39,11 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict
     if(!empty)
     {
         ray_setup r = setup_ray(rays, i);
-        hit = wide ? traverse4<ANY, BLOCK, LDS_DEPTH>(nodes4, tris, root4, r, smem + threadIdx.x, t, u, v, prim)
-                   : traverse<ANY, BLOCK, LDS_DEPTH>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim);
+        if(!wide) hit = traverse<ANY, BLOCK, LDS_DEPTH>(nodes, tris, root, r, smem + threadIdx.x, t, u, v, prim);
+        else if(POSTPONE)
+            hit = traverse4_postponed<ANY, BLOCK, LDS_DEPTH>(nodes4, tris, root4, r, smem + threadIdx.x, t, u, v, prim);
+        else
+            hit = traverse4<ANY, BLOCK, LDS_DEPTH>(nodes4, tris, root4, r, smem + threadIdx.x, t, u, v, prim);
     }
     if(ANY) static_cast<uint8_t *>(out)[i] = hit ? 1 : 0;
     else
@@ -39,25 +51,33 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict
                                               : vkrt_hit{__int_as_float(0x7f800000), 0.f, 0.f, VKRT_INVALID_PRIM};
 }
 
-template<bool ANY, int BLOCK, int LDS>
-static void launch_one(hipStream_t s, const vkrt_bvh *bvh, const float4 *rays, uint32_t n, void *out)
+template<bool ANY, int BLOCK, int LDS, bool POSTPONE>
+static void launch_one(hipStream_t s, const vkrt_bvh *bvh, const float4 *rays, uint32_t n, void *out, uint32_t chunk)
 {
-    k_trace<ANY, BLOCK, LDS><<<div_up(n, BLOCK), BLOCK, 0, s>>>(bvh->nodes.as<vkrt_bvh_node>(),
-                                                                 bvh->tris.as<float4>(), bvh->root_ref,
-                                                                 bvh->empty, bvh->nodes4.as<vkrt_bvh4_node>(),
-                                                                 bvh->root4, bvh->wide, rays, n, out);
+    // with chunks, whole rounds of 8 C workgroups are launched; the ones past the last ray block return at once
+    const uint32_t blocks = chunk ? div_up(div_up(n, BLOCK), 8ull * chunk) * 8u * chunk : div_up(n, BLOCK);
+    k_trace<ANY, BLOCK, LDS, POSTPONE><<<blocks, BLOCK, 0, s>>>(
+            bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref, bvh->empty,
+            bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide, rays, n, out, chunk);
 }
 
 template<bool ANY, int BLOCK>
 static void launch_block(vkrt_ctx *ctx, const vkrt_trace_config &cfg, const vkrt_bvh *bvh, const float4 *rays,
                          uint32_t n, void *out)
 {
+    const bool p = cfg.traversal == VKRT_TRAVERSAL_POSTPONED;
+    const uint32_t x = cfg.xcd_chunk;
+    hipStream_t s = ctx->stream;
     switch(cfg.lds_stack_depth)
     {
-        case 0: launch_one<ANY, BLOCK, 0>(ctx->stream, bvh, rays, n, out); break;
-        case 8: launch_one<ANY, BLOCK, 8>(ctx->stream, bvh, rays, n, out); break;
-        case 16: launch_one<ANY, BLOCK, 16>(ctx->stream, bvh, rays, n, out); break;
-        case 32: launch_one<ANY, BLOCK, 32>(ctx->stream, bvh, rays, n, out); break;
+        case 0: p ? launch_one<ANY, BLOCK, 0, true>(s, bvh, rays, n, out, x)
+                  : launch_one<ANY, BLOCK, 0, false>(s, bvh, rays, n, out, x); break;
+        case 8: p ? launch_one<ANY, BLOCK, 8, true>(s, bvh, rays, n, out, x)
+                  : launch_one<ANY, BLOCK, 8, false>(s, bvh, rays, n, out, x); break;
+        case 16: p ? launch_one<ANY, BLOCK, 16, true>(s, bvh, rays, n, out, x)
+                   : launch_one<ANY, BLOCK, 16, false>(s, bvh, rays, n, out, x); break;
+        case 32: p ? launch_one<ANY, BLOCK, 32, true>(s, bvh, rays, n, out, x)
+                   : launch_one<ANY, BLOCK, 32, false>(s, bvh, rays, n, out, x); break;
         default: throw error(VKRT_E_INVALID_ARG, "lds_stack_depth must be 0, 8, 16 or 32");
     }
 }
@@ -66,7 +86,8 @@ bool valid_trace_config(const vkrt_trace_config &c)
 {
     return (c.block_size == 64 || c.block_size == 128 || c.block_size == 256) &&
            (c.lds_stack_depth == 0 || c.lds_stack_depth == 8 || c.lds_stack_depth == 16 ||
-            c.lds_stack_depth == 32);
+            c.lds_stack_depth == 32) &&
+           c.traversal <= VKRT_TRAVERSAL_POSTPONED && c.xcd_chunk <= 256 && (c.xcd_chunk & (c.xcd_chunk - 1)) == 0;
 }
 
 void launch_trace(vkrt_ctx *ctx, const vkrt_trace_config &cfg, bool any, const vkrt_bvh *bvh, const float4 *rays,

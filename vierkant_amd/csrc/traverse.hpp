@@ -364,4 +364,100 @@ __device__ __forceinline__ bool traverse4(const vkrt_bvh4_node *__restrict__ nod
     return best_prim != VKRT_INVALID_PRIM;
 }
 
+//! the 4-wide walk with leaves postponed ("while-while" with speculative traversal, Aila and Laine 2009): a
+//! lane that meets a leaf parks it and keeps descending until every lane of the wave still walking holds a
+//! parked leaf; then the wave tests its leaves together, so node visits and triangle tests are not
+//! interleaved across lanes. The same children are entered under the same condition (with a best_t that can
+//! lag behind: more nodes, never fewer) and the closest-hit rule does not depend on the visiting order, so
+//! the result is traverse4's.
+template<bool ANY, int BLOCK, int LDS_DEPTH, class F = keep_all>
+__device__ __forceinline__ bool traverse4_postponed(const vkrt_bvh4_node *__restrict__ nodes,
+                                                    const float4 *__restrict__ tris, int32_t root, const ray_setup &r,
+                                                    int32_t *lds, float &best_t, float &best_u, float &best_v,
+                                                    uint32_t &best_prim, bool cull_back = false,
+                                                    const F &filter = F{})
+{
+    constexpr int32_t NONE = EMPTY_CHILD, DONE = EMPTY_CHILD - 1;// neither is a node index or a leaf ref
+    int32_t spill[spill_entries(LDS_DEPTH)];
+    lane_stack<BLOCK, LDS_DEPTH> st{lds, spill};
+    int sp = 0;
+    best_t = r.tmax;
+    best_u = best_v = 0.f;
+    best_prim = VKRT_INVALID_PRIM;
+    auto pop = [&]() -> int32_t { return sp == 0 ? DONE : st.get(--sp); };
+    int32_t node = root, leaf = NONE;
+    if(node < 0)
+    {
+        leaf = node;
+        node = DONE;
+    }
+    while(true)
+    {
+        while(node >= 0 && node != DONE)
+        {
+            const vkrt_bvh4_node &nd = nodes[node];
+            const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
+            const int4 cr = nd.child;
+            float k0, k1, k2, k3, f;
+            slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
+            bool v0 = cr.x != EMPTY_CHILD && k0 <= f && f >= r.tmin && k0 <= best_t;
+            slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
+            bool v1 = cr.y != EMPTY_CHILD && k1 <= f && f >= r.tmin && k1 <= best_t;
+            slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
+            bool v2 = cr.z != EMPTY_CHILD && k2 <= f && f >= r.tmin && k2 <= best_t;
+            slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
+            bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= best_t;
+            int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
+            auto cx = [](bool &va, float &ka, int32_t &ca, bool &vb, float &kb, int32_t &cb) {
+                const bool sw = vb && (!va || kb < ka);
+                if(sw)
+                {
+                    const bool tv = va;
+                    va = vb;
+                    vb = tv;
+                    const float tk = ka;
+                    ka = kb;
+                    kb = tk;
+                    const int32_t tc = ca;
+                    ca = cb;
+                    cb = tc;
+                }
+            };
+            cx(v0, k0, c0, v1, k1, c1);
+            cx(v2, k2, c2, v3, k3, c3);
+            cx(v0, k0, c0, v2, k2, c2);
+            cx(v1, k1, c1, v3, k3, c3);
+            cx(v1, k1, c1, v2, k2, c2);
+            if(v0)
+            {
+                if(v3) st.put(sp++, c3);
+                if(v2) st.put(sp++, c2);
+                if(v1) st.put(sp++, c1);
+                node = c0;
+            }
+            else
+                node = pop();
+            if(node < 0 && leaf == NONE)
+            {
+                leaf = node;
+                node = pop();
+            }
+            if(__all(leaf != NONE)) break;
+        }
+        while(leaf != NONE)
+        {
+            if(test_leaf<ANY, F>(tris, leaf, r, best_t, best_u, best_v, best_prim, cull_back, filter)) return true;
+            if(node < 0)
+            {
+                leaf = node;
+                node = pop();
+            }
+            else
+                leaf = NONE;
+        }
+        if(node == DONE) break;
+    }
+    return best_prim != VKRT_INVALID_PRIM;
+}
+
 }// namespace vkrt
