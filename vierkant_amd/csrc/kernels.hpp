@@ -27,6 +27,22 @@ __host__ __device__ __forceinline__ uint64_t hit_key(float t, uint32_t prim)
     return (static_cast<uint64_t>(ordered_bits(t + 0.0f)) << 32) | prim;
 }
 
+//! XCD-chunked workgroup order: the dispatcher hands workgroups to the 8 XCDs round robin, so workgroup
+//! b = (g C + j) 8 + x runs on XCD x; it takes work block g 8 C + x C + j, i.e. each XCD takes C consecutive
+//! blocks per turn and neighbouring rays share its L2. chunk 0 keeps b. A bijection on [0, xcd_grid(n, C)).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t chunk)
+{
+    if(!chunk) return b;
+    const uint32_t x = b % 8u, j = (b / 8u) % chunk, g = b / (8u * chunk);
+    return g * 8u * chunk + x * chunk + j;
+}
+
+//! workgroups to launch for `blocks` work blocks: whole rounds of 8 C with a chunk (the surplus ones return)
+inline uint32_t xcd_grid(uint32_t blocks, uint32_t chunk)
+{
+    return chunk ? static_cast<uint32_t>((blocks + 8ull * chunk - 1) / (8ull * chunk) * 8ull * chunk) : blocks;
+}
+
 void launch_tri_prep(hipStream_t s, const float *tris, uint32_t n, float *pre);
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint32_t n, uint64_t value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t value);

@@ -48,11 +48,11 @@ __global__ void __launch_bounds__(BLOCK) k_scene_trace(const vkrt_bvh4_node *__r
                                                        const inst_dev *__restrict__ insts,
                                                        const blas_dev *__restrict__ blases,
                                                        const float4 *__restrict__ rays, uint32_t n,
-                                                       void *__restrict__ out)
+                                                       void *__restrict__ out, uint32_t chunk)
 {
     __shared__ int32_t smem[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
     int32_t spill[SCENE_STACK - LDS_DEPTH];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, chunk) * BLOCK + threadIdx.x;
     if(i >= n) return;
     lane_stack<BLOCK, LDS_DEPTH> st{smem + threadIdx.x, spill};
     const float4 ra = rays[2 * (size_t)i], rb = rays[2 * (size_t)i + 1];
@@ -394,9 +394,12 @@ static void check_instances(const vkrt_scene *sc, const vkrt_instance *inst, uin
 template<bool ANY>
 static void launch_scene(vkrt_scene *sc, const float4 *rays, uint32_t n, void *out)
 {
-    k_scene_trace<ANY, SCENE_BLOCK, SCENE_LDS><<<div_up(n, SCENE_BLOCK), SCENE_BLOCK, 0, sc->ctx->stream>>>(
+    // the workgroup order of the context's trace configuration (XCD chunks)
+    const uint32_t chunk = sc->ctx->trace_cfg.xcd_chunk;
+    k_scene_trace<ANY, SCENE_BLOCK, SCENE_LDS><<<xcd_grid(div_up(n, SCENE_BLOCK), chunk), SCENE_BLOCK, 0,
+                                                 sc->ctx->stream>>>(
             sc->nodes.as<vkrt_bvh4_node>(), sc->root, sc->empty ? 1u : 0u, sc->order.as<uint32_t>(),
-            sc->insts.as<inst_dev>(), sc->blases.as<blas_dev>(), rays, n, out);
+            sc->insts.as<inst_dev>(), sc->blases.as<blas_dev>(), rays, n, out, chunk);
     VKRT_HIP(hipGetLastError());
 }
 

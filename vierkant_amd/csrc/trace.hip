@@ -24,14 +24,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict
                                                  uint32_t chunk)
 {
     __shared__ int32_t smem[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
-    // chunk C > 0: workgroup b = (g C + j) 8 + x runs on XCD x and takes ray block g 8 C + x C + j
-    uint32_t blk = blockIdx.x;
-    if(chunk)
-    {
-        const uint32_t x = blk % 8u, j = (blk / 8u) % chunk, g = blk / (8u * chunk);
-        blk = g * 8u * chunk + x * chunk + j;
-    }
-    uint32_t i = blk * BLOCK + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, chunk) * BLOCK + threadIdx.x;
     if(i >= n) return;
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
@@ -54,9 +47,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const vkrt_bvh_node *__restrict
 template<bool ANY, int BLOCK, int LDS, bool POSTPONE>
 static void launch_one(hipStream_t s, const vkrt_bvh *bvh, const float4 *rays, uint32_t n, void *out, uint32_t chunk)
 {
-    // with chunks, whole rounds of 8 C workgroups are launched; the ones past the last ray block return at once
-    const uint32_t blocks = chunk ? div_up(div_up(n, BLOCK), 8ull * chunk) * 8u * chunk : div_up(n, BLOCK);
-    k_trace<ANY, BLOCK, LDS, POSTPONE><<<blocks, BLOCK, 0, s>>>(
+    k_trace<ANY, BLOCK, LDS, POSTPONE><<<xcd_grid(div_up(n, BLOCK), chunk), BLOCK, 0, s>>>(
             bvh->nodes.as<vkrt_bvh_node>(), bvh->tris.as<float4>(), bvh->root_ref, bvh->empty,
             bvh->nodes4.as<vkrt_bvh4_node>(), bvh->root4, bvh->wide, rays, n, out, chunk);
 }
