@@ -455,13 +455,30 @@ enum
     VKRT_ALPHA_BLEND = 2
 };
 
-/* An RGBA8 UNORM image, rows from the top; the tracer builds its mip chain (2x2 box filter, as the
- * reference's linear blits of a level of even size). */
+/* An RGBA8 UNORM image, rows from the top, and its sampler (texture_sampler_t, include/vierkant/Material.hpp:
+ * 157-181; glTF samplers map to it as src/model/gltf.cpp:662-690 does). The tracer builds the mip chain (2x2
+ * box filter, as the reference's linear blits of a level of even size) and blends levels linearly. Zero
+ * fields give the defaults: repeat addressing, linear filtering. */
 typedef struct vkrt_texture
 {
     uint32_t width, height;
     const uint8_t *rgba;
+    uint32_t address_mode_u, address_mode_v; /* VKRT_ADDRESS_* */
+    uint32_t min_filter, mag_filter;         /* VKRT_FILTER_*: min applies where lambda > 0, mag elsewhere */
 } vkrt_texture;
+
+enum
+{
+    VKRT_ADDRESS_REPEAT = 0,
+    VKRT_ADDRESS_MIRRORED_REPEAT = 1,
+    VKRT_ADDRESS_CLAMP_TO_EDGE = 2
+};
+
+enum
+{
+    VKRT_FILTER_LINEAR = 0,
+    VKRT_FILTER_NEAREST = 1
+};
 
 /* A flat scene: an indexed triangle mesh with shading normals, split into objects (entities) that each
  * carry one material. All pointers are host pointers; the tracer keeps device copies. */

@@ -372,6 +372,7 @@ static float round_half(float x)
 typedef struct
 {
     uint32_t w, h, levels;
+    uint32_t mode_u, mode_v, min_nearest, mag_nearest;
     uint8_t *lv[ORC_MAX_LEVELS];
 } otex_t;
 
@@ -380,6 +381,10 @@ static void otex_build(otex_t *t, const orc_texture *src)
     memset(t, 0, sizeof(*t));
     t->w = src->width;
     t->h = src->height;
+    t->mode_u = src->address_mode_u;
+    t->mode_v = src->address_mode_v;
+    t->min_nearest = src->min_filter == 1;
+    t->mag_nearest = src->mag_filter == 1;
     size_t bytes = (size_t)t->w * t->h * 4;
     t->lv[0] = (uint8_t *)malloc(bytes ? bytes : 4);
     memcpy(t->lv[0], src->rgba, bytes);
@@ -414,17 +419,36 @@ static void otex_free(otex_t *t)
     for(uint32_t l = 0; l < t->levels; ++l) free(t->lv[l]);
 }
 
-/* bilinear filtering of one level with repeat addressing; texel centres at (i + 0.5) / size */
-static void otex_level(const otex_t *t, uint32_t l, float u, float v, float out[4])
+/* Vulkan's address modes for texel index i of a level n wide: repeat, mirrored repeat, clamp to edge */
+static int otex_address(long long i, int n, uint32_t mode)
+{
+    if(mode == 2) return (int)(i < 0 ? 0 : (i >= n ? n - 1 : i));
+    if(mode == 1)
+    {
+        long long m = 2ll * n, r = ((i % m) + m) % m;
+        return (int)(r < n ? r : m - 1 - r);
+    }
+    return (int)(((i % n) + n) % n);
+}
+
+/* one level: bilinear (texel centres at (i + 0.5) / size) or nearest (texel floor(u size)) */
+static void otex_level(const otex_t *t, uint32_t l, float u, float v, int nearest, float out[4])
 {
     const int w = (int)(t->w >> l ? t->w >> l : 1), h = (int)(t->h >> l ? t->h >> l : 1);
+    const uint8_t *p = t->lv[l];
+    if(nearest)
+    {
+        const int x = otex_address((long long)floorf(u * (float)w), w, t->mode_u);
+        const int y = otex_address((long long)floorf(v * (float)h), h, t->mode_v);
+        for(int c = 0; c < 4; ++c) out[c] = (float)p[4 * ((size_t)y * w + x) + c] / 255.0f;
+        return;
+    }
     const float fx = u * (float)w - 0.5f, fy = v * (float)h - 0.5f;
     const float x0f = floorf(fx), y0f = floorf(fy);
     const float ax = fx - x0f, ay = fy - y0f;
     const long long xl = (long long)x0f, yl = (long long)y0f;
-    const int x0 = (int)(((xl % w) + w) % w), y0 = (int)(((yl % h) + h) % h);
-    const int x1 = (x0 + 1) % w, y1 = (y0 + 1) % h;
-    const uint8_t *p = t->lv[l];
+    const int x0 = otex_address(xl, w, t->mode_u), x1 = otex_address(xl + 1, w, t->mode_u);
+    const int y0 = otex_address(yl, h, t->mode_v), y1 = otex_address(yl + 1, h, t->mode_v);
     for(int c = 0; c < 4; ++c)
     {
         const float c00 = (float)p[4 * ((size_t)y0 * w + x0) + c] / 255.0f;
@@ -436,19 +460,21 @@ static void otex_level(const otex_t *t, uint32_t l, float u, float v, float out[
 }
 
 /* SampleLevel: lambda clamped to [0, levels - 1] (NaN and -inf give level 0), linear between the two
- * levels around it; coordinates beyond 1e9 in magnitude carry no fraction and read as (0, 0) */
+ * levels around it, the min filter where lambda > 0 and the mag filter elsewhere; coordinates beyond 1e9 in
+ * magnitude carry no fraction and read as (0, 0) */
 static void otex_sample(const otex_t *t, float u, float v, float lambda, float out[4])
 {
     const float top = (float)(t->levels - 1);
+    const int nearest = lambda > 0.0f ? (int)t->min_nearest : (int)t->mag_nearest;
     if(!(lambda > 0.0f)) lambda = 0.0f;
     if(lambda > top) lambda = top;
     if(!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) u = v = 0.0f;
     const uint32_t l0 = (uint32_t)lambda;
     const float f = lambda - (float)l0;
-    otex_level(t, l0, u, v, out);
+    otex_level(t, l0, u, v, nearest, out);
     if(f == 0.0f) return;
     float b[4];
-    otex_level(t, l0 + 1 < t->levels ? l0 + 1 : t->levels - 1, u, v, b);
+    otex_level(t, l0 + 1 < t->levels ? l0 + 1 : t->levels - 1, u, v, nearest, b);
     for(int c = 0; c < 4; ++c) out[c] = lerpf(out[c], b[c], f);
 }
 

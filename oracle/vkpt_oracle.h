@@ -11,11 +11,13 @@
 extern "C" {
 #endif
 
-/* an RGBA8 UNORM image, rows from the top, 4 bytes per texel */
+/* an RGBA8 UNORM image, rows from the top, 4 bytes per texel, and its sampler: address modes 0 repeat,
+ * 1 mirrored repeat, 2 clamp to edge; filters 0 linear, 1 nearest */
 typedef struct orc_texture
 {
     uint32_t width, height;
     const uint8_t *rgba;
+    uint32_t address_mode_u, address_mode_v, min_filter, mag_filter;
 } orc_texture;
 
 typedef struct orc_pt_scene

@@ -276,7 +276,9 @@ class _PtScene(ctypes.Structure):
 
 
 class _Texture(ctypes.Structure):
-    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba", ctypes.c_void_p)]
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba", ctypes.c_void_p),
+                ("address_mode_u", ctypes.c_uint32), ("address_mode_v", ctypes.c_uint32),
+                ("min_filter", ctypes.c_uint32), ("mag_filter", ctypes.c_uint32)]
 
 
 class _PtParams(ctypes.Structure):
@@ -289,14 +291,15 @@ class _PtParams(ctypes.Structure):
                 ("suppress_refractive_caustics", ctypes.c_uint32), ("fov_y", ctypes.c_float)]
 
 
-def texture_sample(rgba, uv, lam):
-    """SampleLevel of an (H, W, 4) uint8 image through its mip chain at uv (n, 2) and levels lam (n): (n, 4)"""
+def texture_sample(rgba, uv, lam, sampler=(0, 0, 0, 0)):
+    """SampleLevel of an (H, W, 4) uint8 image through its mip chain at uv (n, 2) and levels lam (n): (n, 4);
+    sampler = (address mode u, address mode v, min filter, mag filter)"""
     h = lib()
     h.orc_texture_sample.restype = None
     h.orc_texture_sample.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                      ctypes.c_void_p]
     img = np.ascontiguousarray(rgba, np.uint8)
-    t = _Texture(img.shape[1], img.shape[0], img.ctypes.data)
+    t = _Texture(img.shape[1], img.shape[0], img.ctypes.data, *sampler)
     uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
     lam = np.ascontiguousarray(np.broadcast_to(np.asarray(lam, np.float32), (len(uv),)))
     out = np.zeros((len(uv), 4), np.float32)
@@ -319,7 +322,9 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
     tg = scene.get("tangents")
     tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
     imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
-    tex = (_Texture * max(1, len(imgs)))(*[_Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in imgs])
+    smp = list(scene.get("texture_samplers") or [(0, 0, 0, 0)] * len(imgs))
+    tex = (_Texture * max(1, len(imgs)))(*[_Texture(t.shape[1], t.shape[0], t.ctypes.data, *s)
+                                           for t, s in zip(imgs, smp)])
     sc = _PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
                   arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
                   arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,

@@ -222,3 +222,29 @@ def test_gpu_fully_textured_room_matches_cpu_integrator(ctx):
     rmse = float(np.sqrt(np.mean((got - want) ** 2)))
     assert close > 0.97, f"only {close:.4f} of pixels within 1e-4"
     assert rmse < 0.02 * float(want.mean()) + 1e-6
+
+
+def test_gpu_texture_samplers_match_cpu_integrator(ctx):
+    """the fully textured room with mirrored, clamped and nearest-filtered samplers on its textures: per-pixel
+    radiance within 1e-4 of the CPU integrator"""
+    sc, cam, lights = fully_textured_room(2), W.room_camera(), W.room_lights()
+    sc["texture_samplers"] = [(vk.ADDRESS_MIRRORED_REPEAT, vk.ADDRESS_CLAMP_TO_EDGE, vk.FILTER_NEAREST, vk.FILTER_LINEAR),
+                              (vk.ADDRESS_CLAMP_TO_EDGE, vk.ADDRESS_MIRRORED_REPEAT, vk.FILTER_LINEAR, vk.FILTER_NEAREST),
+                              (vk.ADDRESS_MIRRORED_REPEAT, vk.ADDRESS_MIRRORED_REPEAT, vk.FILTER_NEAREST, vk.FILTER_NEAREST),
+                              (vk.ADDRESS_CLAMP_TO_EDGE, vk.ADDRESS_CLAMP_TO_EDGE, vk.FILTER_LINEAR, vk.FILTER_LINEAR)]
+    w, h, spp = 160, 90, 4
+    got = gpu_render(ctx, sc, lights, cam, w, h, spp, 8, 0)
+    want = orc.pt_render(sc, cam, w, h, spp, 8, oracle_seed(0), lights=lights)[..., :3]
+    close = np.mean(np.abs(got - want).max(axis=2) <= 1e-4 * np.maximum(1.0, want.max(axis=2)))
+    assert close > 0.97, close
+    plain = gpu_render(ctx, fully_textured_room(2), lights, cam, w, h, spp, 8, 0)
+    assert np.abs(plain - got).mean() > 0.001 * float(want.mean())
+
+
+def test_gpu_texture_sampler_validation(ctx):
+    sc = textured_room()
+    sc["texture_samplers"] = [(3, 0, 0, 0), (0, 0, 0, 0)]
+    pt = vk.PathTracer(ctx, settings(8, 8, 1, 2), seed=0)
+    with pytest.raises(vk.VkrtError):
+        pt.set_scene(sc)
+    pt.close()

@@ -776,3 +776,119 @@ def test_gltf_texture_slots_and_tangents(tmp_path):
     assert np.allclose(m["tangents"][:3], np.tile([0, 0, -1], (3, 1)), atol=1e-7)
     # computed for the second primitive: s direction +x, normal +z, b = +y, cross(n, t) = +y -> sign -1
     assert np.array_equal(m["tangents"][3:], np.tile(np.float32([-1, 0, 0]), (3, 1)))
+
+
+# ---- samplers -------------------------------------------------------------------------------------------------
+SCIPY_MODE = {vk.ADDRESS_REPEAT: "grid-wrap", vk.ADDRESS_MIRRORED_REPEAT: "reflect", vk.ADDRESS_CLAMP_TO_EDGE: "nearest"}
+
+
+def sample_level_modes(level, u, v, mode_u, mode_v, nearest):
+    """one level under Vulkan's address modes via scipy (grid-wrap = repeat, reflect = mirrored repeat, nearest =
+    clamp to edge; order 0 = nearest filtering at texel centres). Mixed modes per axis: sample a copy padded
+    with the v-axis mode, then interpolate along u with its own."""
+    h, w = level.shape[:2]
+    x, y = u * w - 0.5, v * h - 0.5
+    out = np.zeros((len(u), 4))
+    for c in range(4):
+        ch = level[..., c]
+        if mode_u == mode_v:
+            out[:, c] = ndimage.map_coordinates(ch, [y, x], order=0 if nearest else 1, mode=SCIPY_MODE[mode_u])
+        else:
+            # separable: linear (or nearest) along v with mode_v at the two (or one) columns around x
+            if nearest:
+                xi = np.floor(u * w).astype(np.int64)
+                cols = np.array([_address(k, w, mode_u) for k in xi])
+                colv = np.stack([ndimage.map_coordinates(ch[:, k], [yy], order=0, mode=SCIPY_MODE[mode_v])
+                                 for k, yy in zip(cols, y[:, None])])[:, 0]
+                out[:, c] = colv
+            else:
+                x0 = np.floor(x).astype(np.int64)
+                ax = x - x0
+                a = np.array([ndimage.map_coordinates(ch[:, _address(k, w, mode_u)], [[yy]], order=1,
+                                                      mode=SCIPY_MODE[mode_v])[0] for k, yy in zip(x0, y)])
+                b = np.array([ndimage.map_coordinates(ch[:, _address(k + 1, w, mode_u)], [[yy]], order=1,
+                                                      mode=SCIPY_MODE[mode_v])[0] for k, yy in zip(x0, y)])
+                out[:, c] = a + ax * (b - a)
+    return out
+
+
+def _address(i, n, mode):
+    if mode == vk.ADDRESS_CLAMP_TO_EDGE:
+        return min(max(i, 0), n - 1)
+    if mode == vk.ADDRESS_MIRRORED_REPEAT:
+        t = i % (2 * n)
+        return t if t < n else 2 * n - 1 - t
+    return i % n
+
+
+@pytest.mark.parametrize("sampler", [(1, 1, 0, 0), (2, 2, 0, 0), (1, 2, 0, 0), (0, 0, 1, 1), (2, 1, 1, 1),
+                                     (0, 1, 1, 0), (2, 0, 0, 1)])
+def test_sampler_modes_match_scipy(sampler):
+    """mirrored repeat (scipy's reflect: the edge texel repeats), clamp to edge (scipy's nearest extension) and
+    repeat per axis; nearest filtering where the filter for the level of detail (min above 0, mag at or
+    below) says so; the two levels around lambda blended linearly"""
+    rng = np.random.default_rng(sum(sampler) + 7 * sampler[0])
+    img = rng.integers(0, 256, (7, 12, 4), dtype=np.uint8)
+    n = 600
+    uv = rng.uniform(-2.5, 3.5, (n, 2)).astype(np.float32)
+    lam = rng.uniform(-1.0, 3.0, n).astype(np.float32)
+    got = orc.texture_sample(img, uv, lam, sampler)
+    levels = mip_chain(img)
+    u, v = uv[:, 0].astype(np.float64), uv[:, 1].astype(np.float64)
+    lam_c = np.clip(lam.astype(np.float64), 0, len(levels) - 1)
+    l0 = np.floor(lam_c).astype(int)
+    f = lam_c - l0
+    want = np.zeros((n, 4))
+    for k in range(len(levels)):
+        sel = l0 == k
+        if not sel.any():
+            continue
+        near = np.where(lam[sel] > 0, sampler[2] == 1, sampler[3] == 1)
+        for nearest in (False, True):
+            s2 = sel.copy()
+            s2[sel] = near == nearest
+            if not s2.any():
+                continue
+            a = sample_level_modes(levels[k], u[s2], v[s2], sampler[0], sampler[1], nearest)
+            b = sample_level_modes(levels[min(k + 1, len(levels) - 1)], u[s2], v[s2], sampler[0], sampler[1], nearest)
+            want[s2] = a + f[s2][:, None] * (b - a)
+    assert np.abs(got - want).max() < 2e-5
+
+
+def test_gltf_samplers(tmp_path):
+    """glTF sampler fields map as convert_sampler does (CLAMP_TO_EDGE 33071, MIRRORED_REPEAT 33648, NEAREST
+    9728; anything else repeat / linear); one texture per (image, sampler) pair"""
+    import base64
+    import io
+    import json
+    from PIL import Image
+    buf = io.BytesIO()
+    Image.fromarray(np.full((2, 2, 4), 90, np.uint8), "RGBA").save(buf, format="PNG")
+    png = buf.getvalue()
+    pos = np.float32([[0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    data = pos.tobytes() + png
+    data += b"\0" * (-len(data) % 4)
+    tex = lambda i: {"index": i}
+    doc = {"asset": {"version": "2.0"}, "scenes": [{"nodes": [0]}], "nodes": [{"mesh": 0}],
+           "meshes": [{"primitives": [{"attributes": {"POSITION": 0}, "material": 0}]}],
+           "materials": [{"pbrMetallicRoughness": {"baseColorTexture": tex(0), "metallicRoughnessTexture": tex(1)},
+                          "normalTexture": tex(2), "emissiveTexture": tex(0)}],
+           "samplers": [{"wrapS": 33071, "wrapT": 33648, "minFilter": 9728, "magFilter": 9729},
+                        {"minFilter": 9987, "magFilter": 9728}],
+           "textures": [{"source": 0, "sampler": 0}, {"source": 0, "sampler": 1}, {"source": 0}],
+           "images": [{"bufferView": 1, "mimeType": "image/png"}],
+           "buffers": [{"byteLength": len(data),
+                        "uri": "data:application/octet-stream;base64," + base64.b64encode(data).decode()}],
+           "bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": 36},
+                           {"buffer": 0, "byteOffset": 36, "byteLength": len(png)}],
+           "accessors": [{"bufferView": 0, "componentType": 5126, "count": 3, "type": "VEC3"}]}
+    (tmp_path / "s.gltf").write_text(json.dumps(doc))
+    m = vk.load_model(str(tmp_path / "s.gltf"))
+    mat = m["materials"][0]
+    assert len(m["textures"]) == 3
+    smp = m["texture_samplers"]
+    assert smp[mat["color_texture"] - 1] == (vk.ADDRESS_CLAMP_TO_EDGE, vk.ADDRESS_MIRRORED_REPEAT, vk.FILTER_NEAREST,
+                                            vk.FILTER_LINEAR)
+    assert smp[mat["ao_rough_metal_texture"] - 1] == (0, 0, vk.FILTER_LINEAR, vk.FILTER_NEAREST)
+    assert smp[mat["normal_texture"] - 1] == (0, 0, 0, 0)
+    assert mat["emission_texture"] == mat["color_texture"]

@@ -59,7 +59,13 @@ class PtSettings(ctypes.Structure):
 
 
 class Texture(ctypes.Structure):
-    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba", ctypes.c_void_p)]
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("rgba", ctypes.c_void_p),
+                ("address_mode_u", ctypes.c_uint32), ("address_mode_v", ctypes.c_uint32),
+                ("min_filter", ctypes.c_uint32), ("mag_filter", ctypes.c_uint32)]
+
+
+ADDRESS_REPEAT, ADDRESS_MIRRORED_REPEAT, ADDRESS_CLAMP_TO_EDGE = 0, 1, 2
+FILTER_LINEAR, FILTER_NEAREST = 0, 1
 
 
 class ModelView(ctypes.Structure):
@@ -708,7 +714,8 @@ class PathTracer:
         """scene: dict with positions [n,3] f32, normals [n,3] f32, indices u32, tri_object u32,
         object_material u32, materials (MATERIAL_DTYPE array), environment float; optional texcoords [n,2] f32,
         tangents [n,3] f32 (for normal maps) and textures (a list of [h,w,4] uint8 RGBA images, referenced by the
-        materials' texture slots as index + 1)"""
+        materials' texture slots as index + 1) with texture_samplers (one (address_mode_u, address_mode_v,
+        min_filter, mag_filter) per texture; default repeat and linear)"""
         arr = {k: np.ascontiguousarray(scene[k]) for k in
                ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
         uv = scene.get("texcoords")
@@ -716,7 +723,9 @@ class PathTracer:
         tg = scene.get("tangents")
         tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
         imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
-        tex = (Texture * max(1, len(imgs)))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data) for t in imgs])
+        smp = list(scene.get("texture_samplers") or [(0, 0, 0, 0)] * len(imgs))
+        tex = (Texture * max(1, len(imgs)))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data, *s)
+                                              for t, s in zip(imgs, smp)])
         sc = PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
                      arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
                      arr["object_material"].ctypes.data, len(arr["object_material"]),
@@ -957,12 +966,13 @@ def load_model(path, environment=1.0):
         mats = np.zeros(v.n_materials, MATERIAL_DTYPE)
         if v.n_materials:
             ctypes.memmove(mats.ctypes.data, v.materials, v.n_materials * MATERIAL_DTYPE.itemsize)
-        textures = []
+        textures, samplers = [], []
         if v.n_textures:
             tv = ctypes.cast(v.textures, ctypes.POINTER(Texture))
             for k in range(v.n_textures):
                 t = tv[k]
                 textures.append(arr(t.rgba, 4 * t.width * t.height, np.uint8).reshape(t.height, t.width, 4))
+                samplers.append((t.address_mode_u, t.address_mode_v, t.min_filter, t.mag_filter))
         return {"positions": arr(v.positions, 3 * v.n_vert, np.float32).reshape(-1, 3),
                 "normals": arr(v.normals, 3 * v.n_vert, np.float32).reshape(-1, 3),
                 "texcoords": arr(v.texcoords, 2 * v.n_vert, np.float32).reshape(-1, 2),
@@ -971,6 +981,7 @@ def load_model(path, environment=1.0):
                 "tri_object": arr(v.tri_object, v.n_tri, np.uint32),
                 "object_material": arr(v.object_material, v.n_objects, np.uint32),
                 "materials": mats, "environment": float(environment), "textures": textures,
+                "texture_samplers": samplers,
                 "n_skipped_textures": int(v.n_skipped_textures)}
     finally:
         lib().vkrt_model_destroy(h)

@@ -593,10 +593,13 @@ struct vkrt_model
     std::vector<vkrt_material> materials;
     std::vector<std::string> object_names, material_names;
     std::vector<image_rgba8> images;
+    //! (image, sampler) pairs: one texture per distinct pair, as the reference keys textures by
+    //! (texture id, sampler id)
+    std::vector<std::array<uint32_t, 5>> texture_refs;
     std::vector<vkrt_texture> textures;// views of `images`, filled when the model is complete
     uint32_t n_objects = 0, n_skipped_textures = 0;
 
-    //! decode one image; returns 1 + its texture index, or 0 (counted as skipped) when it cannot be decoded
+    //! decode one image; returns 1 + its image index, or 0 (counted as skipped) when it cannot be decoded
     uint32_t add_image(const uint8_t *data, size_t size)
     {
         image_rgba8 img;
@@ -609,10 +612,25 @@ struct vkrt_model
         images.push_back(std::move(img));
         return static_cast<uint32_t>(images.size());
     }
+    //! 1 + the texture index of image `image_id` (1-based, 0: none) under a sampler (address u, address v,
+    //! min filter, mag filter); 0 for no image
+    uint32_t add_texture(uint32_t image_id, const std::array<uint32_t, 4> &sampler)
+    {
+        if(!image_id) return 0;
+        const std::array<uint32_t, 5> key{image_id - 1, sampler[0], sampler[1], sampler[2], sampler[3]};
+        for(size_t k = 0; k < texture_refs.size(); ++k)
+            if(texture_refs[k] == key) return static_cast<uint32_t>(k + 1);
+        texture_refs.push_back(key);
+        return static_cast<uint32_t>(texture_refs.size());
+    }
     void finish()
     {
         textures.clear();
-        for(const image_rgba8 &im: images) textures.push_back(vkrt_texture{im.width, im.height, im.rgba.data()});
+        for(const auto &r: texture_refs)
+        {
+            const image_rgba8 &im = images[r[0]];
+            textures.push_back(vkrt_texture{im.width, im.height, im.rgba.data(), r[1], r[2], r[3], r[4]});
+        }
     }
 
     //! append one object's geometry (indices relative to its own vertices)
@@ -819,7 +837,7 @@ std::unique_ptr<vkrt_model> load_obj(const std::string &path)
                 id = model->add_image(reinterpret_cast<const uint8_t *>(bytes.data()), bytes.size());
             }
             else ++model->n_skipped_textures;
-            it = tex_cache.emplace(name, id).first;
+            it = tex_cache.emplace(name, model->add_texture(id, {0, 0, 0, 0})).first;
         }
         return it->second;
     };
@@ -1165,13 +1183,26 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
     gltf_file g = read_gltf(path);
     const json &doc = g.doc;
     auto model = std::make_unique<vkrt_model>();
-    // images decode once, on first use by a material
+    // images decode once, on first use by a material; a texture is an image with the glTF texture's sampler
+    // (convert_sampler, gltf.cpp:662-690: NEAREST only for the exact NEAREST filter, the wrap modes as named)
     std::map<int, uint32_t> image_texture;
+    auto sampler_of = [&](int tex) -> std::array<uint32_t, 4> {
+        const int si = doc["textures"][static_cast<size_t>(tex)]["sampler"].integer(-1);
+        const json &sm = doc["samplers"][static_cast<size_t>(si < 0 ? 0 : si)];
+        if(si < 0 || sm.kind != json::object_t) return {0, 0, 0, 0};
+        auto wrap = [](int w) -> uint32_t {
+            return w == 33071 ? VKRT_ADDRESS_CLAMP_TO_EDGE : w == 33648 ? VKRT_ADDRESS_MIRRORED_REPEAT
+                                                                        : VKRT_ADDRESS_REPEAT;
+        };
+        auto filter = [](int f) -> uint32_t { return f == 9728 ? VKRT_FILTER_NEAREST : VKRT_FILTER_LINEAR; };
+        return {wrap(sm["wrapS"].integer(10497)), wrap(sm["wrapT"].integer(10497)), filter(sm["minFilter"].integer(-1)),
+                filter(sm["magFilter"].integer(-1))};
+    };
     auto texture_of = [&](int tex) -> uint32_t {
         const int src = doc["textures"][static_cast<size_t>(tex < 0 ? 0 : tex)]["source"].integer(-1);
         if(tex < 0 || src < 0) return 0u;
         auto it = image_texture.find(src);
-        if(it != image_texture.end()) return it->second;
+        if(it != image_texture.end()) return model->add_texture(it->second, sampler_of(tex));
         const json &im = doc["images"][static_cast<size_t>(src)];
         uint32_t id = 0;
         if(im.has("bufferView"))
@@ -1201,7 +1232,7 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
             else ++model->n_skipped_textures;
         }
         image_texture[src] = id;
-        return id;
+        return model->add_texture(id, sampler_of(tex));
     };
     const json &mats = doc["materials"];
     std::vector<uv_transform> uv_xform;

@@ -90,7 +90,8 @@ constexpr uint32_t MAX_TEX_LEVELS = 16;
 //! one texture's mip chain inside the shared texel array (RGBA8 packed in a uint32)
 struct tex_desc
 {
-    uint32_t width, height, levels, pad;
+    uint32_t width, height, levels;
+    uint32_t sampler;// bits 0-1 address mode u, 2-3 address mode v, 4 min filter nearest, 5 mag filter nearest
     uint32_t offset[MAX_TEX_LEVELS];
 };
 
@@ -157,10 +158,17 @@ __device__ __forceinline__ float4 unpack_unorm8(uint32_t c)
                        static_cast<float>((c >> 16) & 0xFFu) / 255.0f, static_cast<float>(c >> 24) / 255.0f);
 }
 
-__device__ __forceinline__ int wrap_repeat(int x, int n)
+//! texel index i of a level n texels wide under an address mode (Vulkan's repeat, mirrored repeat and
+//! clamp to edge); i is taken modulo 2n first in 64-bit so that far-away tiles wrap exactly
+__device__ __forceinline__ int address_texel(long long i, int n, uint32_t mode)
 {
-    const int r = x % n;
-    return r < 0 ? r + n : r;
+    if(mode == VKRT_ADDRESS_CLAMP_TO_EDGE) return static_cast<int>(i < 0 ? 0 : (i >= n ? n - 1 : i));
+    if(mode == VKRT_ADDRESS_MIRRORED_REPEAT)
+    {
+        const long long m = 2ll * n, t = ((i % m) + m) % m;
+        return static_cast<int>(t < n ? t : m - 1 - t);
+    }
+    return static_cast<int>(((i % n) + n) % n);
 }
 
 __device__ __forceinline__ float4 lerp4(const float4 &a, const float4 &b, float t)
@@ -168,34 +176,44 @@ __device__ __forceinline__ float4 lerp4(const float4 &a, const float4 &b, float 
     return make_float4(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z), a.w + t * (b.w - a.w));
 }
 
-__device__ float4 sample_level(const tex_desc &d, const uint32_t *__restrict__ texels, uint32_t level, float u, float v)
+//! one level: bilinear (texel centres at (i + 0.5) / n) or nearest (texel floor(u n))
+__device__ float4 sample_level(const tex_desc &d, const uint32_t *__restrict__ texels, uint32_t level, float u, float v,
+                               bool nearest)
 {
     const int w = static_cast<int>(max(1u, d.width >> level)), h = static_cast<int>(max(1u, d.height >> level));
+    const uint32_t mu = d.sampler & 3u, mv = (d.sampler >> 2) & 3u;
+    const uint32_t *t = texels + d.offset[level];
+    if(nearest)
+    {
+        const int x = address_texel(static_cast<long long>(floorf(u * static_cast<float>(w))), w, mu);
+        const int y = address_texel(static_cast<long long>(floorf(v * static_cast<float>(h))), h, mv);
+        return unpack_unorm8(t[y * w + x]);
+    }
     const float fx = u * static_cast<float>(w) - 0.5f, fy = v * static_cast<float>(h) - 0.5f;
     const float x0f = floorf(fx), y0f = floorf(fy);
     const float ax = fx - x0f, ay = fy - y0f;
-    // texel coordinates taken modulo the level size in 64-bit so that far-away tiles wrap exactly
     const long long xl = static_cast<long long>(x0f), yl = static_cast<long long>(y0f);
-    const int x0 = static_cast<int>(((xl % w) + w) % w), y0 = static_cast<int>(((yl % h) + h) % h);
-    const int x1 = wrap_repeat(x0 + 1, w), y1 = wrap_repeat(y0 + 1, h);
-    const uint32_t *t = texels + d.offset[level];
+    const int x0 = address_texel(xl, w, mu), x1 = address_texel(xl + 1, w, mu);
+    const int y0 = address_texel(yl, h, mv), y1 = address_texel(yl + 1, h, mv);
     const float4 c00 = unpack_unorm8(t[y0 * w + x0]), c10 = unpack_unorm8(t[y0 * w + x1]);
     const float4 c01 = unpack_unorm8(t[y1 * w + x0]), c11 = unpack_unorm8(t[y1 * w + x1]);
     return lerp4(lerp4(c00, c10, ax), lerp4(c01, c11, ax), ay);
 }
 
-//! SampleLevel(uv, lambda): lambda clamped to the chain (a NaN or negative level is level 0)
+//! SampleLevel(uv, lambda): lambda clamped to the chain (a NaN or negative level is level 0), the min filter
+//! where lambda > 0 and the mag filter elsewhere, levels blended linearly
 __device__ float4 sample_texture(const tex_desc &d, const uint32_t *__restrict__ texels, float u, float v, float lambda)
 {
     const float top = static_cast<float>(d.levels - 1u);
+    const bool nearest = lambda > 0.0f ? (d.sampler & 16u) != 0u : (d.sampler & 32u) != 0u;
     if(!(lambda > 0.0f)) lambda = 0.0f;
     if(lambda > top) lambda = top;
     if(!(fabsf(u) < 1e9f) || !(fabsf(v) < 1e9f)) u = v = 0.0f;// no fraction is left there; keeps texel indices in range
     const uint32_t l0 = static_cast<uint32_t>(lambda);
     const float f = lambda - static_cast<float>(l0);
-    const float4 a = sample_level(d, texels, l0, u, v);
+    const float4 a = sample_level(d, texels, l0, u, v, nearest);
     if(f == 0.0f) return a;
-    return lerp4(a, sample_level(d, texels, min(l0 + 1u, d.levels - 1u), u, v), f);
+    return lerp4(a, sample_level(d, texels, min(l0 + 1u, d.levels - 1u), u, v, nearest), f);
 }
 
 //! rnd() of a copy of an RNG state: the draw a stage makes without advancing its caller's stream
@@ -1703,6 +1721,10 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
             d = tex_desc{};
             d.width = tx.width;
             d.height = tx.height;
+            VKRT_REQUIRE(tx.address_mode_u <= VKRT_ADDRESS_CLAMP_TO_EDGE && tx.address_mode_v <= VKRT_ADDRESS_CLAMP_TO_EDGE &&
+                                 tx.min_filter <= VKRT_FILTER_NEAREST && tx.mag_filter <= VKRT_FILTER_NEAREST,
+                         "unknown texture address mode or filter");
+            d.sampler = tx.address_mode_u | (tx.address_mode_v << 2) | (tx.min_filter << 4) | (tx.mag_filter << 5);
             uint32_t w = tx.width, h = tx.height;
             d.offset[0] = static_cast<uint32_t>(texels.size());
             for(size_t i = 0; i < (size_t)w * h; ++i)
