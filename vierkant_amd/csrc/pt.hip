@@ -48,6 +48,10 @@ constexpr uint32_t NO_VERTEX = 0xffffffffu;
 constexpr uint32_t TRAIN_COUNTERS = 8;// offset of the training stream's counters
 constexpr int PT_BLOCK = 256;
 constexpr int PT_LDS = 16;
+// the closest-hit trace of the queued paths: the trace kernels' launch shape (64 lanes per workgroup; the
+// workgroups one XCD receives take PT_TRACE_CHUNK consecutive queue blocks in turn)
+constexpr int PT_TRACE_BLOCK = 64;
+constexpr uint32_t PT_TRACE_CHUNK = 8;
 
 //! f32tof16 / f16tof32 round trip (round to nearest even) of the media stack's packed ior
 __host__ __device__ inline float round_half(float x)
@@ -348,6 +352,7 @@ __global__ void k_pt_reset_depth(uint32_t *c, int next)
 //! the closest hit of path p's current ray at this depth; back faces are culled while the path is in air.
 //! BLEND candidates compare against rnd() of a copy of payload.rng_state = xxhash32(rng_state, depth)
 //! (raypipeline.slang:150-152, :693, :716)
+template<int BLOCK = PT_BLOCK>
 __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                            int32_t root, bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
                                            int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem,
@@ -363,23 +368,24 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
         flt.xi = rnd_peek(xxhash32(st.x, depth));
         // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
         if(wide)
-            traverse4<false, PT_BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air, flt);
+            traverse4<false, BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air, flt);
         else
-            traverse<false, PT_BLOCK, PT_LDS>(nodes, tris, root, r, smem, t, u, v, prim, in_air, flt);
+            traverse<false, BLOCK, PT_LDS>(nodes, tris, root, r, smem, t, u, v, prim, in_air, flt);
     }
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
 }
 
-__global__ void __launch_bounds__(PT_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
-                                                       const float4 *__restrict__ tris, int32_t root, bool empty,
-                                                       const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
-                                                       bool wide, pt_buffers b, int in, alpha_filter flt,
-                                                       uint32_t depth)
+__global__ void __launch_bounds__(PT_TRACE_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
+                                                             const float4 *__restrict__ tris, int32_t root, bool empty,
+                                                             const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
+                                                             bool wide, pt_buffers b, int in, alpha_filter flt,
+                                                             uint32_t depth)
 {
-    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
-    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
+    __shared__ int32_t smem[PT_LDS * PT_TRACE_BLOCK];
+    const uint32_t q = xcd_block(blockIdx.x, PT_TRACE_CHUNK) * PT_TRACE_BLOCK + threadIdx.x;
     if(q >= b.counter[in]) return;
-    trace_path(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x, flt, depth);
+    trace_path<PT_TRACE_BLOCK>(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x,
+                               flt, depth);
 }
 
 //! eval_distant_lights (raypipeline.slang:728-744): sun discs seen by a ray that left the scene
@@ -1466,8 +1472,8 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     // one depth of a queue of paths: trace, shade, and the shadow rays of next-event estimation
     auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t grid, uint32_t depth, int in) {
         k_pt_reset_depth<<<1, 1, 0, st>>>(bb.counter, in ^ 1);
-        k_pt_trace<<<grid, PT_BLOCK, 0, st>>>(nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide,
-                                              bb, in, flt, depth);
+        k_pt_trace<<<xcd_grid(div_up((uint64_t)grid * PT_BLOCK, PT_TRACE_BLOCK), PT_TRACE_CHUNK), PT_TRACE_BLOCK, 0,
+                     st>>>(nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, bb, in, flt, depth);
         if(f.n_lights)
         {
             k_pt_shade<true><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
