@@ -56,8 +56,9 @@ __device__ __forceinline__ void slab_interval(const ray_setup &r, float lox, flo
     float z0 = (loz - r.o.z) * r.inv.z, z1 = (hiz - r.o.z) * r.inv.z;
     float n_ = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
     float f_ = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
-    tn = n_ - fabsf(n_) * T_PAD;
-    tf = f_ + fabsf(f_) * T_PAD;
+    // one rounding per end (a fused multiply-add): at most half an ulp against the 2^-16 |t| of the pad
+    tn = fmaf(-fabsf(n_), T_PAD, n_);
+    tf = fmaf(fabsf(f_), T_PAD, f_);
 }
 
 __device__ __forceinline__ void leaf_decode(int ref, uint32_t &first, uint32_t &count)
@@ -196,6 +197,23 @@ __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32
     return false;
 }
 
+//! sort key of a 4-wide node's child: +inf when the ray misses it, else its entry distance clamped below
+//! +inf (so a hit child never sorts with the missed ones)
+constexpr float KEY_MISSED = __builtin_huge_valf();
+__device__ __forceinline__ float sort_key(bool hit, float k) { return hit ? fminf(k, 3.40282347e38f) : KEY_MISSED; }
+
+//! branchless compare-exchange of (key, ref) pairs: the smaller key first
+__device__ __forceinline__ void compare_exchange(float &ka, int32_t &ca, float &kb, int32_t &cb)
+{
+    const bool sw = kb < ka;
+    const float lo = sw ? kb : ka, hi = sw ? ka : kb;
+    const int32_t cl = sw ? cb : ca, ch = sw ? ca : cb;
+    ka = lo;
+    kb = hi;
+    ca = cl;
+    cb = ch;
+}
+
 //! per-ray traversal state of the 4-wide walk (the stack lives in LDS + a private spill array)
 struct trav_state
 {
@@ -243,37 +261,24 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
         slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
         bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= s.best_t;
         int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
-        // sort (valid first, then nearest first): 5 compare-exchanges
-        auto cx = [](bool &va, float &ka, int32_t &ca, bool &vb, float &kb, int32_t &cb) {
-            const bool sw = vb && (!va || kb < ka);
-            if(sw)
-            {
-                const bool tv = va;
-                va = vb;
-                vb = tv;
-                const float tk = ka;
-                ka = kb;
-                kb = tk;
-                const int32_t tc = ca;
-                ca = cb;
-                cb = tc;
-            }
-        };
-        cx(v0, k0, c0, v1, k1, c1);
-        cx(v2, k2, c2, v3, k3, c3);
-        cx(v0, k0, c0, v2, k2, c2);
-        cx(v1, k1, c1, v3, k3, c3);
-        cx(v1, k1, c1, v2, k2, c2);
-        if(v0)
+        // sort nearest first with the missed children last (sort keys: +inf for them, the entry distance
+        // clamped below +inf for the others): five branchless compare-exchanges of (key, ref)
+        float q0 = sort_key(v0, k0), q1 = sort_key(v1, k1), q2 = sort_key(v2, k2), q3 = sort_key(v3, k3);
+        compare_exchange(q0, c0, q1, c1);
+        compare_exchange(q2, c2, q3, c3);
+        compare_exchange(q0, c0, q2, c2);
+        compare_exchange(q1, c1, q3, c3);
+        compare_exchange(q1, c1, q2, c2);
+        if(q0 < KEY_MISSED)
         {
             auto push = [&](int32_t c) {
                 if(LDS_DEPTH > 0 && s.sp < LDS_DEPTH) lds[s.sp * BLOCK] = c;
                 else spill[s.sp - LDS_DEPTH] = c;
                 ++s.sp;
             };
-            if(v3) push(c3);
-            if(v2) push(c2);
-            if(v1) push(c1);
+            if(q3 < KEY_MISSED) push(c3);
+            if(q2 < KEY_MISSED) push(c2);
+            if(q1 < KEY_MISSED) push(c1);
             s.ref = c0;
             return false;
         }
@@ -305,27 +310,14 @@ __device__ __forceinline__ int visit4(const vkrt_bvh4_node &nd, const ray_setup 
     c[1] = cr.y;
     c[2] = cr.z;
     c[3] = cr.w;
-    auto cx = [&](int a, int b) {
-        const bool sw = v[b] && (!v[a] || k[b] < k[a]);
-        if(sw)
-        {
-            const bool tv = v[a];
-            v[a] = v[b];
-            v[b] = tv;
-            const float tk = k[a];
-            k[a] = k[b];
-            k[b] = tk;
-            const int32_t tc = c[a];
-            c[a] = c[b];
-            c[b] = tc;
-        }
-    };
-    cx(0, 1);
-    cx(2, 3);
-    cx(0, 2);
-    cx(1, 3);
-    cx(1, 2);
-    return (v[0] ? 1 : 0) + (v[1] ? 1 : 0) + (v[2] ? 1 : 0) + (v[3] ? 1 : 0);
+    float q[4] = {sort_key(v[0], k[0]), sort_key(v[1], k[1]), sort_key(v[2], k[2]), sort_key(v[3], k[3])};
+    compare_exchange(q[0], c[0], q[1], c[1]);
+    compare_exchange(q[2], c[2], q[3], c[3]);
+    compare_exchange(q[0], c[0], q[2], c[2]);
+    compare_exchange(q[1], c[1], q[3], c[3]);
+    compare_exchange(q[1], c[1], q[2], c[2]);
+    return (q[0] < KEY_MISSED ? 1 : 0) + (q[1] < KEY_MISSED ? 1 : 0) + (q[2] < KEY_MISSED ? 1 : 0) +
+           (q[3] < KEY_MISSED ? 1 : 0);
 }
 
 template<int BLOCK, int LDS_DEPTH>
@@ -408,31 +400,17 @@ __device__ __forceinline__ bool traverse4_postponed(const vkrt_bvh4_node *__rest
             slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
             bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= best_t;
             int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
-            auto cx = [](bool &va, float &ka, int32_t &ca, bool &vb, float &kb, int32_t &cb) {
-                const bool sw = vb && (!va || kb < ka);
-                if(sw)
-                {
-                    const bool tv = va;
-                    va = vb;
-                    vb = tv;
-                    const float tk = ka;
-                    ka = kb;
-                    kb = tk;
-                    const int32_t tc = ca;
-                    ca = cb;
-                    cb = tc;
-                }
-            };
-            cx(v0, k0, c0, v1, k1, c1);
-            cx(v2, k2, c2, v3, k3, c3);
-            cx(v0, k0, c0, v2, k2, c2);
-            cx(v1, k1, c1, v3, k3, c3);
-            cx(v1, k1, c1, v2, k2, c2);
-            if(v0)
+            float q0 = sort_key(v0, k0), q1 = sort_key(v1, k1), q2 = sort_key(v2, k2), q3 = sort_key(v3, k3);
+            compare_exchange(q0, c0, q1, c1);
+            compare_exchange(q2, c2, q3, c3);
+            compare_exchange(q0, c0, q2, c2);
+            compare_exchange(q1, c1, q3, c3);
+            compare_exchange(q1, c1, q2, c2);
+            if(q0 < KEY_MISSED)
             {
-                if(v3) st.put(sp++, c3);
-                if(v2) st.put(sp++, c2);
-                if(v1) st.put(sp++, c1);
+                if(q3 < KEY_MISSED) st.put(sp++, c3);
+                if(q2 < KEY_MISSED) st.put(sp++, c2);
+                if(q1 < KEY_MISSED) st.put(sp++, c1);
                 node = c0;
             }
             else
