@@ -23,6 +23,7 @@ struct ray_setup
 {
     vec3 o, d, inv;
     float tmin, tmax;
+    vec3 clo, chi;// slab offsets of the lo and hi planes (see make_ray_setup)
 };
 
 __device__ __forceinline__ ray_setup make_ray_setup(float4 a, float4 b)
@@ -36,6 +37,20 @@ __device__ __forceinline__ ray_setup make_ray_setup(float4 a, float4 b)
     // origin and |t| >= 1e20 |b - o| elsewhere, i.e. "inside the slab for all t" or "never"
     auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
     r.inv = vec3{1.0f / safe(r.d.x), 1.0f / safe(r.d.y), 1.0f / safe(r.d.z)};
+    // a plane b gives t = fmaf(b, inv, c): c = -o inv shifted by A = 2^-20 |o inv| toward smaller t for the
+    // plane the ray meets first on that axis (lo when inv > 0) and toward larger t for the other. fl(o inv)
+    // and the rounding of c each miss o inv by at most 2^-24 |o inv|, far inside A, so the shifted value
+    // bounds (b - o) inv from the conservative side on every axis; the relative error left (inv's and the
+    // fma's rounding, 2 x 2^-24 |t|) is the one slab_interval's T_PAD covers.
+    // (an infinite o inv takes no shift: both planes then sit at the same infinite t, which is exact)
+    auto offsets = [](float o, float inv, float &clo, float &chi) {
+        const float p = o * inv, a = fabsf(p) < __builtin_huge_valf() ? fabsf(p) * 0x1p-20f : 0.0f;
+        clo = inv >= 0.0f ? -p - a : -p + a;
+        chi = inv >= 0.0f ? -p + a : -p - a;
+    };
+    offsets(r.o.x, r.inv.x, r.clo.x, r.chi.x);
+    offsets(r.o.y, r.inv.y, r.clo.y, r.chi.y);
+    offsets(r.o.z, r.inv.z, r.clo.z, r.chi.z);
     return r;
 }
 
@@ -44,16 +59,17 @@ __device__ __forceinline__ ray_setup setup_ray(const float4 *__restrict__ rays, 
     return make_ray_setup(rays[2 * (size_t)i], rays[2 * (size_t)i + 1]);
 }
 
-//! Conservative ray/box interval. Each slab value t = RN(RN(b - o) * RN(1/d)) is within 3 eps
-//! (eps = 2^-24) relative of (b - o)/d, so growing the interval by T_PAD |t| (T_PAD = 2^-16) keeps the
-//! exact interval inside it; boxes were grown by BOX_PAD relative at build time to absorb the rounding
-//! of the reference test's hit point.
+//! Conservative ray/box interval. Each slab value t = fmaf(b, inv, c) is (b - o)/d within 2 eps (eps =
+//! 2^-24) relative, moved outward by the axis's offset shift (make_ray_setup) which covers the absolute
+//! error of o inv; growing the interval by T_PAD |t| (T_PAD = 2^-16) then keeps the exact interval inside it.
+//! Boxes were grown by BOX_PAD relative at build time to absorb the rounding of the reference test's hit
+//! point.
 __device__ __forceinline__ void slab_interval(const ray_setup &r, float lox, float hix, float loy, float hiy, float loz,
                                               float hiz, float &tn, float &tf)
 {
-    float x0 = (lox - r.o.x) * r.inv.x, x1 = (hix - r.o.x) * r.inv.x;
-    float y0 = (loy - r.o.y) * r.inv.y, y1 = (hiy - r.o.y) * r.inv.y;
-    float z0 = (loz - r.o.z) * r.inv.z, z1 = (hiz - r.o.z) * r.inv.z;
+    float x0 = fmaf(lox, r.inv.x, r.clo.x), x1 = fmaf(hix, r.inv.x, r.chi.x);
+    float y0 = fmaf(loy, r.inv.y, r.clo.y), y1 = fmaf(hiy, r.inv.y, r.chi.y);
+    float z0 = fmaf(loz, r.inv.z, r.clo.z), z1 = fmaf(hiz, r.inv.z, r.chi.z);
     float n_ = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
     float f_ = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
     // one rounding per end (a fused multiply-add): at most half an ulp against the 2^-16 |t| of the pad
