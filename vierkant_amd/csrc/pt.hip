@@ -375,7 +375,7 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
 }
 
-__global__ void __launch_bounds__(PT_TRACE_BLOCK) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
+__global__ void __launch_bounds__(PT_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
                                                              const float4 *__restrict__ tris, int32_t root, bool empty,
                                                              const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
                                                              bool wide, pt_buffers b, int in, alpha_filter flt,

@@ -43,7 +43,7 @@ struct blas_dev
 };
 
 template<bool ANY, int BLOCK, int LDS_DEPTH>
-__global__ void __launch_bounds__(BLOCK) k_scene_trace(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
+__global__ void __launch_bounds__(BLOCK, 6) k_scene_trace(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
                                                        uint32_t empty_scene, const uint32_t *__restrict__ order,
                                                        const inst_dev *__restrict__ insts,
                                                        const blas_dev *__restrict__ blases,
