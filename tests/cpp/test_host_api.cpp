@@ -4,11 +4,13 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <vector>
 
 #include "vierkant/PBRPathTracer.hpp"
 #include "vierkant/RayBuilder.hpp"
 #include "vierkant/intersection.hpp"
+#include "vierkant/model.hpp"
 
 using namespace vierkant;
 
@@ -204,6 +206,26 @@ int main()
     mean /= 64.0;
     const double want_lit = 0.52 / 3.14159265358979 * 2.0;
     if(std::fabs(mean / want_lit - 1.0) > 0.01) return fail("sun-lit plane");
+
+    // the same plane from an OBJ + MTL file through vierkant::model::load_model renders the same
+    const auto dir = std::filesystem::temp_directory_path();
+    std::ofstream(dir / "vkrt_plane.mtl") << "newmtl grey\nKd 0.5 0.5 0.5\nNs 0\nNi 1.5\n";
+    std::ofstream(dir / "vkrt_plane.obj") << "mtllib vkrt_plane.mtl\nv -50 0 -50\nv 50 0 -50\nv 50 0 50\n"
+                                             "v -50 0 50\nvn 0 1 0\nusemtl grey\nf 1//1 4//1 3//1 2//1\n";
+    std::string why;
+    auto file = model::load_model(dir / "vkrt_plane.obj", &why);
+    if(!file) return fail(("load_model: " + why).c_str());
+    const vkrt_model_view mv = file->view();
+    if(mv.n_tri != 2 || mv.n_objects != 1 || mv.n_materials != 1 || file->num_textures() != 0)
+        return fail("loaded plane layout");
+    if(model::load_model(dir / "vkrt_missing.obj", &why) || why.empty()) return fail("missing file error");
+    auto loaded = PBRPathTracer::create(ctx, lit_info);
+    loaded->set_scene(file->pt_scene(0.0f));
+    loaded->render_scene(down);
+    double lmean = 0.0;
+    for(const auto &p: loaded->pixels()) lmean += p.radiance[1];
+    lmean /= 64.0;
+    if(std::fabs(lmean / want_lit - 1.0) > 0.01) return fail("sun-lit plane from an OBJ file");
     std::printf("host api ok (%d hits of %zu rays)\n", n_hit, rays.size());
     return 0;
 }
