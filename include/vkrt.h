@@ -613,8 +613,9 @@ int vkrt_denoise(vkrt_ctx *ctx, const vkrt_pixel *pixels, uint32_t width, uint32
                  float *out_rgba, int flags);
 int vkrt_pt_get_stats(const vkrt_pt *pt, vkrt_pt_stats *out);
 /* transmittance_test (shaders/slang/ray/visibility_test.slang:94-165) through the tracer's scene: out4 =
- * (rgb transmittance, 0) per ray; 0 when an opaque triangle (transmission == 0) is hit in (tmin, tmax),
- * else the product of transmission * color over every transmissive hit, taken in ascending (t, triangle)
+ * (rgb transmittance, 0) per ray; 0 when an opaque triangle (transmission == 0) is hit in (tmin, tmax)
+ * and not cut out by its alpha mask (MASK: alpha below the cutoff lets the ray through; BLEND surfaces block,
+ * as this call draws no random numbers), else the product of transmission * color over every transmissive hit, taken in ascending (t, triangle)
  * order (the reference's order is the driver's). VKRT_CULL_BACK in flags culls back faces of one-sided
  * materials (the RAY_FLAG_CULL_BACK_FACING_TRIANGLES call, direct_lighting.slang:354). Materials have no
  * interior absorption here (attenuation_distance = infinity). */
