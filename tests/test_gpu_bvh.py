@@ -160,6 +160,13 @@ def test_trace_configs_give_identical_results(ctx, icosphere_case):
                     occ = ctx.trace_any(bvh, rays)
                     assert np.array_equal(occ, (want["prim"] != 0xFFFFFFFF).astype(np.uint8)), tag
             bvh.close()
+        # ragged ray counts under XCD chunks: partial last blocks and whole idle rounds of workgroups
+        bvh = ctx.bvh_build(pos, idx)
+        for n in (1, 63, 1000, 16385):
+            for chunk in (0, 1, 32, 256):
+                ctx.set_trace_config(64, 16, vk.TRAVERSAL_STEP, chunk)
+                assert_hits_equal(ctx.trace_closest(bvh, rays[:n]), want[:n], f"n={n} xcd_chunk={chunk}")
+        bvh.close()
         with pytest.raises(vk.VkrtError):
             ctx.set_trace_config(96, 16)
         with pytest.raises(vk.VkrtError):

@@ -248,3 +248,23 @@ def test_gpu_texture_sampler_validation(ctx):
     with pytest.raises(vk.VkrtError):
         pt.set_scene(sc)
     pt.close()
+
+
+def test_gpu_transmittance_api_alpha(ctx):
+    """vkrt_pt_trace_transmittance through a masked quad: rays through its cut-out half pass (1), through the
+    rest they stop (0); a BLEND quad blocks every ray (the call draws no random numbers)"""
+    rng = np.random.default_rng(3)
+    n = 2000
+    x = rng.uniform(-0.95, 0.95, n).astype(np.float32)
+    z = rng.uniform(-0.95, 0.95, n).astype(np.float32)
+    keep = np.abs(x) > 0.02  # away from the alpha edge at x = 0
+    x, z = x[keep], z[keep]
+    o = np.stack([x, np.full_like(x, 2.0), z], -1)
+    d = np.tile(np.float32([0, -1, 0]), (len(x), 1))
+    rays = W.make_rays(o, d, 0.0, 1.5)  # reaches the quad at y = 1, stops before the plane
+    for mode, want in ((vk.ALPHA_MASK, (x < 0).astype(np.float32)), (vk.ALPHA_BLEND, np.zeros(len(x), np.float32))):
+        pt = vk.PathTracer(ctx, settings(8, 8, 1, 2), seed=0)
+        pt.set_scene(T.masked_quad_scene(1.0, -1.0, 1.0, mode=mode))
+        got = pt.trace_transmittance(rays)
+        pt.close()
+        assert np.array_equal(got[:, 0], want), mode
