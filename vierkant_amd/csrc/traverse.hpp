@@ -102,6 +102,23 @@ __device__ __forceinline__ bool filtered_out(const F &filter, const float4 &rec,
         return false;
 }
 
+//! stack entry i of a lane: the LDS column for the top LDS_DEPTH entries, the private spill array past them.
+//! The LDS slot is read unconditionally (at a clamped index) and the spill entry only past the LDS depth, so
+//! the common pop is an LDS read.
+template<int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ int32_t stack_get(const int32_t *lds, const int32_t *spill, int i)
+{
+    if constexpr(LDS_DEPTH == 0)
+        return spill[i];
+    else
+    {
+        int32_t v = lds[(i < LDS_DEPTH ? i : LDS_DEPTH - 1) * BLOCK];
+        // (volatile keeps the two loads apart: merged, they become one load through a generic pointer)
+        if(i >= LDS_DEPTH) v = *static_cast<const volatile int32_t *>(spill + (i - LDS_DEPTH));
+        return v;
+    }
+}
+
 //! closest (ANY = false) or any (ANY = true) hit with tmin < t < tmax; ties on t go to the lowest
 //! original triangle index. `lds` points at this lane's column of a [LDS_DEPTH][BLOCK] int array.
 //! With `cull_back`, triangles hit from behind are skipped unless their record is flagged two-sided
@@ -175,7 +192,7 @@ __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes
         }
         if(sp == 0) break;
         --sp;
-        ref = (LDS_DEPTH > 0 && sp < LDS_DEPTH) ? lds[sp * BLOCK] : spill[sp - LDS_DEPTH];
+        ref = stack_get<BLOCK, LDS_DEPTH>(lds, spill, sp);
     }
     return best_prim != VKRT_INVALID_PRIM;
 }
@@ -303,7 +320,7 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
         return true;
     if(s.sp == 0) return true;
     --s.sp;
-    s.ref = (LDS_DEPTH > 0 && s.sp < LDS_DEPTH) ? lds[s.sp * BLOCK] : spill[s.sp - LDS_DEPTH];
+    s.ref = stack_get<BLOCK, LDS_DEPTH>(lds, spill, s.sp);
     return false;
 }
 
@@ -348,7 +365,7 @@ struct lane_stack
     }
     __device__ __forceinline__ int32_t get(int i) const
     {
-        return (LDS_DEPTH > 0 && i < LDS_DEPTH) ? lds[i * BLOCK] : spill[i - LDS_DEPTH];
+        return stack_get<BLOCK, LDS_DEPTH>(lds, spill, i);
     }
 };
 
