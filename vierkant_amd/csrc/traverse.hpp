@@ -235,6 +235,14 @@ __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32
 constexpr float KEY_MISSED = __builtin_huge_valf();
 __device__ __forceinline__ float sort_key(bool hit, float k) { return hit ? fminf(k, 3.40282347e38f) : KEY_MISSED; }
 
+//! does the ray's padded interval [tn, tf] of child c meet [tmin, best_t]? The terms are joined with '&',
+//! not '&&': a short-circuit puts the far-side reduction in a branch of its own per child, where the
+//! compiler no longer knows its operands are fma results and re-quiets each one before fminf/fmaxf
+__device__ __forceinline__ bool child_hit(int32_t c, float tn, float tf, float tmin, float best_t)
+{
+    return (c != EMPTY_CHILD) & (tn <= tf) & (tf >= tmin) & (tn <= best_t);
+}
+
 //! branchless compare-exchange of (key, ref) pairs: the smaller key first
 __device__ __forceinline__ void compare_exchange(float &ka, int32_t &ca, float &kb, int32_t &cb)
 {
@@ -286,13 +294,13 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
         const int4 cr = nd.child;
         float k0, k1, k2, k3, f;
         slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
-        bool v0 = cr.x != EMPTY_CHILD && k0 <= f && f >= r.tmin && k0 <= s.best_t;
+        bool v0 = child_hit(cr.x, k0, f, r.tmin, s.best_t);
         slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
-        bool v1 = cr.y != EMPTY_CHILD && k1 <= f && f >= r.tmin && k1 <= s.best_t;
+        bool v1 = child_hit(cr.y, k1, f, r.tmin, s.best_t);
         slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
-        bool v2 = cr.z != EMPTY_CHILD && k2 <= f && f >= r.tmin && k2 <= s.best_t;
+        bool v2 = child_hit(cr.z, k2, f, r.tmin, s.best_t);
         slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
-        bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= s.best_t;
+        bool v3 = child_hit(cr.w, k3, f, r.tmin, s.best_t);
         int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
         // sort nearest first with the missed children last (sort keys: +inf for them, the entry distance
         // clamped below +inf for the others): five branchless compare-exchanges of (key, ref)
@@ -332,13 +340,13 @@ __device__ __forceinline__ int visit4(const vkrt_bvh4_node &nd, const ray_setup 
     float k[4], f;
     bool v[4];
     slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k[0], f);
-    v[0] = cr.x != EMPTY_CHILD && k[0] <= f && f >= r.tmin && k[0] <= best_t;
+    v[0] = child_hit(cr.x, k[0], f, r.tmin, best_t);
     slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k[1], f);
-    v[1] = cr.y != EMPTY_CHILD && k[1] <= f && f >= r.tmin && k[1] <= best_t;
+    v[1] = child_hit(cr.y, k[1], f, r.tmin, best_t);
     slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k[2], f);
-    v[2] = cr.z != EMPTY_CHILD && k[2] <= f && f >= r.tmin && k[2] <= best_t;
+    v[2] = child_hit(cr.z, k[2], f, r.tmin, best_t);
     slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k[3], f);
-    v[3] = cr.w != EMPTY_CHILD && k[3] <= f && f >= r.tmin && k[3] <= best_t;
+    v[3] = child_hit(cr.w, k[3], f, r.tmin, best_t);
     c[0] = cr.x;
     c[1] = cr.y;
     c[2] = cr.z;
@@ -425,13 +433,13 @@ __device__ __forceinline__ bool traverse4_postponed(const vkrt_bvh4_node *__rest
             const int4 cr = nd.child;
             float k0, k1, k2, k3, f;
             slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
-            bool v0 = cr.x != EMPTY_CHILD && k0 <= f && f >= r.tmin && k0 <= best_t;
+            bool v0 = child_hit(cr.x, k0, f, r.tmin, best_t);
             slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
-            bool v1 = cr.y != EMPTY_CHILD && k1 <= f && f >= r.tmin && k1 <= best_t;
+            bool v1 = child_hit(cr.y, k1, f, r.tmin, best_t);
             slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
-            bool v2 = cr.z != EMPTY_CHILD && k2 <= f && f >= r.tmin && k2 <= best_t;
+            bool v2 = child_hit(cr.z, k2, f, r.tmin, best_t);
             slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
-            bool v3 = cr.w != EMPTY_CHILD && k3 <= f && f >= r.tmin && k3 <= best_t;
+            bool v3 = child_hit(cr.w, k3, f, r.tmin, best_t);
             int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
             float q0 = sort_key(v0, k0), q1 = sort_key(v1, k1), q2 = sort_key(v2, k2), q3 = sort_key(v3, k3);
             compare_exchange(q0, c0, q1, c1);
