@@ -86,23 +86,25 @@ VK_HD bool mt_reference(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3
 VK_HD bool mt_fast(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1, const vec3 &e2, float &t, float &u,
                    float &v, bool cull_back = false)
 {
+    // every division-free test is evaluated and joined with '|' into one branch: on a GPU the early-outs
+    // cost more in divergent branching than the few products they skip
     vec3 pvec = cross(d, e2);
     float det = dot(e1, pvec);
     constexpr float epsilon = 10.0e-10f;
-    if(det > -epsilon && det < epsilon) return false;
-    if(cull_back && det < 0.f) return false;
     vec3 tvec = o - v0;
     float a = dot(tvec, pvec);
+    vec3 qvec = cross(tvec, e1);
+    float b = dot(d, qvec);
     const bool neg = det < 0.f;
     const float D = neg ? -det : det;
     const bool bounded = D <= 0x1p100f;
     const float lo = D * 0x1p-80f;
     const float A = neg ? -a : a;
-    if(bounded && (A <= -lo || A >= D * (1.0f + 0x1p-19f))) return false;
-    vec3 qvec = cross(tvec, e1);
-    float b = dot(d, qvec);
     const float B = neg ? -b : b;
-    if(bounded && (B <= -lo || (A + B) >= D * (1.0f + 0x1p-16f))) return false;
+    const bool reject = ((det > -epsilon) & (det < epsilon)) | (cull_back & neg) |
+                        (bounded & ((A <= -lo) | (A >= D * (1.0f + 0x1p-19f)) | (B <= -lo) |
+                                    ((A + B) >= D * (1.0f + 0x1p-16f))));
+    if(reject) return false;
     // reference tail (src/intersection.cpp:121-128)
     float inv_det = 1.0f / det;
     float uu = inv_det * a;
