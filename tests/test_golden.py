@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 import oracle_binding as orc
+from mt_cases import adversarial_mt_cases
 
 GOLD = pathlib.Path(__file__).resolve().parent / "golden"
 HOST = pathlib.Path(__file__).resolve().parent / "cpp" / "librt_math_host.so"
@@ -88,46 +89,11 @@ def test_host_header_mt_matches_oracle_bitwise():
 
 
 def test_host_header_mt_fast_decides_like_oracle_on_adversarial_pairs():
-    """mt_fast's division-free bounds take the oracle's decision where they are tight: hits exactly on edges
-    and vertices (u = 0, v = 0, u + v = 1), determinants straddling +-1e-9, determinants beyond 2^100 (the
-    bounds switch off), tiny triangles, and infinite or NaN inputs"""
+    """mt_fast's division-free bounds take the oracle's decision where they are tight (tests/mt_cases.py)"""
     h = host()
-    rng = np.random.default_rng(11)
-    cases = []
-    for k in range(400):
-        v0, e1, e2 = rng.normal(size=(3, 3)).astype(np.float32)
-        s = np.float32(rng.uniform(0, 1))
-        which = k % 4
-        target = [v0 + s * e1, v0 + s * e2, v0 + s * e1 + (1 - s) * e2, v0 + e1 * (k % 2)][which]   # v=0, u=0, u+v=1, vertex
-        d = rng.normal(size=3).astype(np.float32)
-        d /= np.linalg.norm(d)
-        cases.append((np.concatenate([v0, v0 + e1, v0 + e2]), (target - 2 * d).astype(np.float32), d))
-    for k in range(200):                       # rays almost in the triangle's plane: det near +-1e-9
-        v0 = np.zeros(3, np.float32)
-        e1 = np.array([1, 0, 0], np.float32)
-        e2 = np.array([0, 1, 0], np.float32)
-        dz = np.float32([1e-9, -1e-9, 1.0001e-9, 0.9999e-9, 2e-9, 5e-10][k % 6]) * np.float32(rng.choice([1, -1]))
-        d = np.array([0.3, 0.2, dz], np.float32)
-        o = np.array([0.1, 0.1, -dz * 3], np.float32) - 3 * d
-        cases.append((np.concatenate([v0, e1, e2]), o.astype(np.float32), d))
-    for scale in (1e-20, 1e-10, 1e20, 1e25, 1e30, 3e38):   # tiny and huge triangles
-        for k in range(30):
-            with np.errstate(over="ignore"):           # 3e38 x normal overflows to +-inf in float32
-                tri = (rng.normal(size=9) * scale).astype(np.float32)
-                o = (rng.normal(size=3) * scale).astype(np.float32)
-            d = rng.normal(size=3).astype(np.float32)
-            cases.append((tri, o, d))
-    special = [np.inf, -np.inf, np.nan, 0.0]
-    for k in range(64):                       # non-finite inputs
-        tri = rng.normal(size=9).astype(np.float32)
-        o = rng.normal(size=3).astype(np.float32)
-        d = rng.normal(size=3).astype(np.float32)
-        (tri if k % 2 else d)[k % 3] = special[k % 4]
-        cases.append((tri, o, d))
     tuv = np.zeros(3, np.float32)
     hits = 0
-    for tri, o, d in cases:
-        tri, o, d = (np.ascontiguousarray(x, np.float32) for x in (tri, o, d))
+    for tri, o, d in adversarial_mt_cases():
         want = orc.intersect_tri(tri, o, d)
         r = h.host_mt(tri.ctypes.data, o.ctypes.data, d.ctypes.data, 1, tuv.ctypes.data)
         assert r == want[0], (tri, o, d)

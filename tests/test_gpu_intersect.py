@@ -217,3 +217,28 @@ def test_bounding_sphere_special_inputs(ctx):
     pts = np.array([[np.nan, 0, 0], [1, 0, 0], [5, 0, 0]], np.float32)
     got, want = ctx.compute_bounding_sphere(pts), orc.bounding_sphere(pts)
     assert np.isnan(got[0]) and got[1:].tolist() == want[1:].tolist()
+
+
+def test_brute_closest_adversarial_pairs_match_oracle(ctx):
+    """the device's branch-joined triangle test takes the oracle's decisions and bits on the adversarial pairs
+    (edge and vertex hits, det near 1e-9, huge and tiny triangles, non-finite inputs): one triangle per call,
+    the ray's interval wide open so that every finite INTERSECT counts"""
+    from mt_cases import adversarial_mt_cases
+    checked = hits = 0
+    for tri, o, d in adversarial_mt_cases():
+        want = orc.intersect_tri(tri, o, d)
+        ray = np.zeros(1, vk.RAY_DTYPE)
+        ray["ox"], ray["oy"], ray["oz"] = o
+        ray["dx"], ray["dy"], ray["dz"] = d
+        ray["tmin"], ray["tmax"] = -3.4e38, 3.4e38
+        got = ctx.brute_closest(tri[None, :], ray)[0]
+        t = np.float32(want[1]) if want[0] else np.float32(np.nan)
+        if want[0] and not (-3.4e38 < t < 3.4e38):
+            continue                               # outside any ray interval: not a closest-hit candidate
+        checked += 1
+        assert (got["prim"] == 0) == bool(want[0]), (tri, o, d)
+        if want[0]:
+            hits += 1
+            assert bits(np.array([got["t"], got["u"], got["v"]], np.float32)).tolist() == \
+                bits(np.array(want[1:], np.float32)).tolist()
+    assert checked > 600 and hits > 100
