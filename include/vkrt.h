@@ -228,6 +228,31 @@ int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_
 int vkrt_bvh_get_info(const vkrt_bvh *bvh, vkrt_bvh_info *out);
 void vkrt_bvh_destroy(vkrt_bvh *bvh);
 
+/* BVH invariants, checked on the host from the device arrays. A diagnostic with no counterpart in the
+ * reference (whose acceleration structure is the driver's, RayBuilder.cpp:274): the walk traversal uses
+ * (4-wide collapse or binary tree) reaches every triangle exactly once, every child box lies inside the
+ * box its parent stores for it, every leaf box holds its triangles' vertices (read from `positions`, the
+ * mesh the BVH was built or last refitted from), and the worst-case traversal stack recomputed from the
+ * nodes equals the build's record and fits the 64-entry stack. */
+typedef struct vkrt_bvh_check
+{
+    uint32_t ok;              /* 1 when every count below is clean */
+    uint32_t wide;            /* the walk checked: 1 the 4-wide collapse, 0 the binary tree */
+    uint32_t tris_reached;    /* leaf triangle records reached from the root */
+    uint32_t tris_not_once;   /* triangles reached zero times or more than once */
+    uint32_t box_violations;  /* child boxes outside their parent's + leaf boxes missing a vertex */
+    uint32_t bad_refs;        /* child refs or leaf ranges out of bounds */
+    uint32_t stack_bound;     /* worst-case stack depth recomputed from the nodes */
+    uint32_t stack_ok;        /* stack_bound <= 64 and equal to the build's record */
+    /* the first box violation found: kind 1 a child box outside its parent's, kind 2 a triangle vertex
+     * outside its leaf's box; node and slot of the box, the triangle (kind 2), the box and the outer box or
+     * vertex (kind 2: the vertex in outer[0..2]) */
+    uint32_t bad_kind, bad_node, bad_slot, bad_tri;
+    float bad_box[6], bad_outer[6];
+} vkrt_bvh_check;
+int vkrt_bvh_validate(vkrt_ctx *ctx, const vkrt_bvh *bvh, const float *positions, uint32_t stride_bytes, int flags,
+                      vkrt_bvh_check *out);
+
 /* ---------------------------------------------------------------------------------------------
  * two-level scenes: instances of mesh BVHs with per-instance transforms (the top-level structure of
  * RayBuilder::build_scene_acceleration, src/RayBuilder.cpp:386-781; entry_t::transform / inv_transform,
@@ -325,11 +350,12 @@ typedef struct vkrt_batch_item
  * (a multiple of 64) and persistent blocks per compute unit (1..64) */
 typedef struct vkrt_batch_config
 {
-    uint32_t block_size;
-    uint32_t lds_stack_depth;
-    uint32_t tile;
-    uint32_t blocks_per_cu;
-    uint32_t reserved[4];
+    uint32_t block_size;      /* 64, 128 or 256 */
+    uint32_t lds_stack_depth; /* top entries of the 64-entry traversal stack kept in LDS: 0, 8, 16, 24, 32, 48, 64 */
+    uint32_t tile;            /* rays per claimed work item, a multiple of 64 */
+    uint32_t blocks_per_cu;   /* persistent blocks launched per compute unit */
+    uint32_t xcd_chunk;       /* 0: one queue in tile order; C: chunks of C tiles dealt to the 8 XCDs' queues */
+    uint32_t reserved[3];
 } vkrt_batch_config;
 
 /* Capture the batch (items are copied; their buffers must stay valid) into a graph with the default shape. */
@@ -341,9 +367,13 @@ int vkrt_batch_set_config(vkrt_batch *batch, const vkrt_batch_config *cfg);
 int vkrt_batch_get_config(const vkrt_batch *batch, vkrt_batch_config *out);
 /* Replay the graph: closest hits of every item's rays into its hit buffer. */
 int vkrt_batch_launch(vkrt_batch *batch, int flags);
-/* Time every launch shape (block x LDS depth x tile x blocks per CU), `repeats` replays each, and keep the
- * fastest; returns it and its time per launch. */
+/* Coordinate descent over the launch shapes of vkrt_batch_candidates (block size, LDS stack depth 16-64, tile,
+ * threads per CU, XCD chunk) from the batch's current shape, `repeats` replays per shape; keeps the fastest
+ * shape found and returns it and its time per launch. */
 int vkrt_batch_autotune(vkrt_batch *batch, uint32_t repeats, vkrt_batch_config *best, float *best_ms);
+/* The autotuner's shape space: writes up to `capacity` shapes, returns how many there are. The autotuner
+ * (a coordinate descent from the batch's current shape) only ever picks one of these. */
+uint32_t vkrt_batch_candidates(vkrt_batch_config *out, uint32_t capacity);
 uint64_t vkrt_batch_total_rays(const vkrt_batch *batch);
 /* One-shot: create, launch once, destroy. */
 int vkrt_batch_closest(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_items, int flags);

@@ -317,6 +317,37 @@ typedef struct
     uint8_t *occ;
 } job_t;
 
+/* Culling criterion of the library's BVH traversal (not reference arithmetic: the library's conservative box
+ * test, restated so that a test can decide whether a ray's brute-force winner could ever be culled).
+ * Boxes: the triangle's vertex min/max grown by x -/+ |x| 2^-16 (bvh.hip pad_lo/pad_hi); every enclosing BVH
+ * box contains this one. Slab: inv = 1/d (|d| < 1e-20 -> +-1e-20), plane t = fma(b, inv, c) with
+ * c = -RN(o inv) -/+ 2^-20 |RN(o inv)| shifted toward smaller t on the entry plane, interval ends grown by
+ * 2^-16 |t| with one fma each (traverse.hpp make_ray_setup / slab_interval). A box is entered iff
+ * tn <= tf, tf >= tmin and tn <= best t. The slab values only move outward when the box grows, so when the
+ * winner's own box is entered with best t = its t, no box holding the winner is ever culled and every BVH the
+ * library builds returns the brute-force record. Returns 1 when the winner at t passes. */
+static float cull_safe_dir(float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; }
+
+int orc_cull_safe(const float *tri, const float *ray8, float t)
+{
+    float n_ = -INFINITY, f_ = INFINITY;
+    for(int a = 0; a < 3; ++a)
+    {
+        const float x0 = tri[a], x1 = tri[3 + a], x2 = tri[6 + a];
+        const float mn = fminf(fminf(x0, x1), x2), mx = fmaxf(fmaxf(x0, x1), x2);
+        const float lo = mn - fabsf(mn) * 0x1p-16f, hi = mx + fabsf(mx) * 0x1p-16f;
+        const float inv = 1.0f / cull_safe_dir(ray8[4 + a]);
+        const float p = ray8[a] * inv;
+        const float sh = fabsf(p) < INFINITY ? fabsf(p) * 0x1p-20f : 0.0f;
+        const float clo = inv >= 0.0f ? -p - sh : -p + sh, chi = inv >= 0.0f ? -p + sh : -p - sh;
+        const float s0 = fmaf(lo, inv, clo), s1 = fmaf(hi, inv, chi);
+        n_ = fmaxf(n_, fminf(s0, s1));
+        f_ = fminf(f_, fmaxf(s0, s1));
+    }
+    const float tn = fmaf(-fabsf(n_), 0x1p-16f, n_), tf = fmaf(fabsf(f_), 0x1p-16f, f_);
+    return tn <= tf && tf >= ray8[3] && tn <= t;
+}
+
 static void closest_one(const float *tris, uint32_t n_tri, const float *r, hit16 *h)
 {
     float best = INFINITY, bu = 0.f, bv = 0.f;
@@ -342,7 +373,15 @@ static void closest_one(const float *tris, uint32_t n_tri, const float *r, hit16
 static void *closest_job(void *arg)
 {
     job_t *j = (job_t *)arg;
-    for(uint32_t r = j->r0; r < j->r1; ++r) closest_one(j->tris, j->n_tri, j->rays + 8 * (size_t)r, j->hits + r);
+    for(uint32_t r = j->r0; r < j->r1; ++r)
+    {
+        const float *ray = j->rays + 8 * (size_t)r;
+        closest_one(j->tris, j->n_tri, ray, j->hits + r);
+        if(j->occ)// the culling criterion of the winner (1 for a miss)
+            j->occ[r] = j->hits[r].prim == 0xFFFFFFFFu
+                                ? 1
+                                : (uint8_t)orc_cull_safe(j->tris + 9 * (size_t)j->hits[r].prim, ray, j->hits[r].t);
+    }
     return NULL;
 }
 
@@ -388,6 +427,12 @@ void orc_brute_closest(const float *tris, uint32_t n_tri, const float *rays8, ui
                        int nthreads)
 {
     run_jobs(closest_job, tris, n_tri, rays8, n_ray, (hit16 *)hits16, NULL, nthreads);
+}
+
+void orc_brute_closest_cull_safe(const float *tris, uint32_t n_tri, const float *rays8, uint32_t n_ray,
+                                 void *hits16, uint8_t *safe, int nthreads)
+{
+    run_jobs(closest_job, tris, n_tri, rays8, n_ray, (hit16 *)hits16, safe, nthreads);
 }
 
 void orc_brute_any(const float *tris, uint32_t n_tri, const float *rays8, uint32_t n_ray, uint8_t *occluded,

@@ -49,6 +49,12 @@ void orc_brute_closest(const float *tris, uint32_t n_tri, const float *rays8, ui
                        int nthreads);
 void orc_brute_any(const float *tris, uint32_t n_tri, const float *rays8, uint32_t n_ray, uint8_t *occluded,
                    int nthreads);
+/* brute-force closest hit plus, per ray, the library traversal's culling criterion for the winner
+ * (orc_cull_safe; 1 for a miss) */
+void orc_brute_closest_cull_safe(const float *tris, uint32_t n_tri, const float *rays8, uint32_t n_ray,
+                                 void *hits16, uint8_t *safe, int nthreads);
+/* 1 when the library's padded box of triangle `tri` is entered by the ray with best t = t (see vkoracle.c) */
+int orc_cull_safe(const float *tri, const float *ray8, float t);
 /* closest hit of one ray, evaluated in FP64 (same acceptance rule); returns prim or 0xFFFFFFFF */
 uint32_t orc_closest_f64(const float *tris, uint32_t n_tri, const float *ray8, double *t, double *u, double *v);
 

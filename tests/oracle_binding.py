@@ -31,6 +31,8 @@ _SIG = {
     "orc_transmittance": (None, [_vp, _vp, _vp, _vp, _u32, _vp, _u32, ctypes.c_int, _vp]),
     "orc_brute_closest": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
     "orc_brute_any": (None, [_vp, _u32, _vp, _u32, _vp, _i32]),
+    "orc_brute_closest_cull_safe": (None, [_vp, _u32, _vp, _u32, _vp, _vp, _i32]),
+    "orc_cull_safe": (_i32, [_vp, _vp, _f32]),
     "orc_closest_f64": (_u32, [_vp, _u32, _vp, _dp, _dp, _dp]),
     "orc_icosphere_size": (None, [_u32, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
     "orc_icosphere": (None, [_f32, _u32, _vp, _vp]),
@@ -179,6 +181,19 @@ def denoise(pixels, denoise=True):
     return out
 
 
+def denoise_passes(pixels, passes):
+    """only the first `passes` a-trous passes (steps 1, 2, 4, 8, 16) over a (H, W) PIXEL_DTYPE buffer"""
+    h_ = lib()
+    h_.orc_denoise_passes.restype = None
+    h_.orc_denoise_passes.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int,
+                                      ctypes.c_void_p]
+    px = np.ascontiguousarray(pixels)
+    h, w = px.shape
+    out = np.zeros((h, w, 4), np.float32)
+    h_.orc_denoise_passes(px.ctypes.data, w, h, int(passes), out.ctypes.data)
+    return out
+
+
 def half_value(x):
     """float32 array rounded to binary16 values (nearest, ties to even) by the C restatement"""
     h = lib()
@@ -215,6 +230,25 @@ def brute_closest(tris_flat, rays, nthreads=None):
     lib().orc_brute_closest(pt, tris_flat.size // 9, rays.ctypes.data, len(rays), out.ctypes.data,
                             nthreads or threads())
     return out
+
+
+def brute_closest_cull_safe(tris_flat, rays, nthreads=None):
+    """(hits, safe): the brute-force closest hits and, per ray, 1 when the library's padded box of the winning
+    triangle is entered with best t = the winner's t (then no BVH of the library can cull the winner)"""
+    from vierkant_amd import HIT_DTYPE
+    tris_flat, pt = _f(tris_flat)
+    rays = np.ascontiguousarray(rays)
+    out = np.zeros(len(rays), dtype=HIT_DTYPE)
+    safe = np.zeros(len(rays), dtype=np.uint8)
+    lib().orc_brute_closest_cull_safe(pt, tris_flat.size // 9, rays.ctypes.data, len(rays), out.ctypes.data,
+                                      safe.ctypes.data, nthreads or threads())
+    return out, safe
+
+
+def cull_safe(tri9, ray, t):
+    tri9 = np.ascontiguousarray(tri9, dtype=np.float32)
+    ray = np.ascontiguousarray(ray)
+    return int(lib().orc_cull_safe(tri9.ctypes.data, ray.ctypes.data, float(t)))
 
 
 def brute_any(tris_flat, rays, nthreads=None):
@@ -288,7 +322,38 @@ class _PtParams(ctypes.Structure):
                 ("right", ctypes.c_float * 3), ("up", ctypes.c_float * 3), ("tan_half_x", ctypes.c_float),
                 ("tan_half_y", ctypes.c_float), ("aperture", ctypes.c_float), ("focal_distance", ctypes.c_float),
                 ("lights", ctypes.c_void_p), ("n_lights", ctypes.c_uint32), ("mis_mode", ctypes.c_uint32),
-                ("suppress_refractive_caustics", ctypes.c_uint32), ("fov_y", ctypes.c_float)]
+                ("suppress_refractive_caustics", ctypes.c_uint32), ("fov_y", ctypes.c_float),
+                ("test_flags", ctypes.c_uint32)]
+
+
+PT_NO_ROULETTE, PT_ULP_BIAS = 1, 2
+
+
+def sample_disney(material, n, v, eta, seed, count):
+    """count draws of the integrator's sample_disney: (count, 8) = direction xyz, pdf, F rgb, transmission"""
+    h = lib()
+    h.orc_sample_disney_n.restype = None
+    h.orc_sample_disney_n.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                      ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+    m = np.ascontiguousarray(material).view(np.float32)
+    n3, v3 = np.asarray(n, np.float32), np.asarray(v, np.float32)
+    out = np.zeros((count, 8), np.float32)
+    h.orc_sample_disney_n(m.ctypes.data, n3.ctypes.data, v3.ctypes.data, eta, seed, count, out.ctypes.data)
+    return out
+
+
+def eval_disney(material, n, v, eta, dirs):
+    """the integrator's eval_disney at each direction of dirs (m, 3): (m, 4) = F rgb, pdf"""
+    h = lib()
+    h.orc_eval_disney_n.restype = None
+    h.orc_eval_disney_n.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                    ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]
+    m = np.ascontiguousarray(material).view(np.float32)
+    n3, v3 = np.asarray(n, np.float32), np.asarray(v, np.float32)
+    d = np.ascontiguousarray(dirs, np.float32)
+    out = np.zeros((len(d), 4), np.float32)
+    h.orc_eval_disney_n(m.ctypes.data, n3.ctypes.data, v3.ctypes.data, eta, d.ctypes.data, len(d), out.ctypes.data)
+    return out
 
 
 def texture_sample(rgba, uv, lam, sampler=(0, 0, 0, 0)):
@@ -308,7 +373,7 @@ def texture_sample(rgba, uv, lam, sampler=(0, 0, 0, 0)):
 
 
 def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_path_beta=0.0, disable_material=0,
-              nthreads=None, lights=None, mis_mode=0, suppress_refractive_caustics=0):
+              nthreads=None, lights=None, mis_mode=0, suppress_refractive_caustics=0, test_flags=0):
     """one batch of the CPU reference integrator: (H, W, 8) float32 (rgb, coverage, depth, -, object bits, -)"""
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
@@ -340,7 +405,7 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
                    (ctypes.c_float * 3)(*up), thx, thy, float(camera.aperture),
                    float(camera.focal_distance) if camera.focal_distance > 0 else 1.0,
                    la.ctypes.data if len(la) else None, len(la), mis_mode, suppress_refractive_caustics,
-                   float(camera.fov_y))
+                   float(camera.fov_y), test_flags)
     out = np.zeros((height, width, 8), np.float32)
     h.orc_pt_render(ctypes.byref(sc), ctypes.byref(pp), out.ctypes.data, nthreads or threads())
     return out

@@ -46,7 +46,8 @@ def test_batch_graph_replay_and_autotune(ctx, small_batch):
     batch = vk.Batch(ctx, list(zip(bvhs, rays, hits)))
     want = per_scene(ctx, small_batch)
     poison = np.full(max(h.nbytes for h in hits), 0xAB, np.uint8)
-    for cfg in [(64, 0, 64, 1), (128, 8, 256, 4), (256, 32, 4096, 8), (256, 16, 1024, 2)]:
+    for cfg in [(64, 0, 64, 1, 0), (128, 8, 256, 4, 4), (256, 32, 4096, 8, 0), (256, 16, 1024, 2, 4),
+                (64, 64, 256, 16, 1), (256, 64, 1024, 2, 64), (128, 24, 512, 8, 2), (64, 48, 1024, 4, 8)]:
         batch.config = cfg
         assert batch.config == cfg
         for _ in range(2):                          # replays of the same captured graph
@@ -56,12 +57,14 @@ def test_batch_graph_replay_and_autotune(ctx, small_batch):
             for s, (h, w) in enumerate(zip(hits, want)):
                 assert_hits_equal(h.host(), w, f"cfg {cfg} scene {s}")
     best, ms = batch.autotune(repeats=1)
-    assert ms > 0 and batch.config == best
+    assert ms > 0 and batch.config == best and best in vk.Batch.candidates()
     batch.launch()
     for s, (h, w) in enumerate(zip(hits, want)):
         assert_hits_equal(h.host(), w, f"autotuned {best} scene {s}")
-    with pytest.raises(vk.VkrtError):
-        batch.config = (96, 16, 1024, 8)
+    for bad in [(96, 16, 1024, 8, 0), (256, 12, 1024, 8, 0), (256, 16, 1000, 8, 0), (256, 16, 1024, 0, 0),
+                (256, 16, 1024, 8, 65)]:
+        with pytest.raises(vk.VkrtError):
+            batch.config = bad
 
 
 def test_batch_closest_one_shot_and_empty_items(ctx, small_batch):

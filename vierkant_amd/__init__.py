@@ -161,7 +161,10 @@ class BatchItem(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("block_size", ctypes.c_uint32), ("lds_stack_depth", ctypes.c_uint32), ("tile", ctypes.c_uint32),
-                ("blocks_per_cu", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("blocks_per_cu", ctypes.c_uint32), ("xcd_chunk", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+    def shape(self):
+        return (self.block_size, self.lds_stack_depth, self.tile, self.blocks_per_cu, self.xcd_chunk)
 
 
 class BvhInfo(ctypes.Structure):
@@ -169,6 +172,21 @@ class BvhInfo(ctypes.Structure):
                 ("depth", ctypes.c_uint32), ("sah_cost", ctypes.c_float), ("build_ms", ctypes.c_float),
                 ("device_bytes", ctypes.c_uint64), ("bounds", ctypes.c_float * 6),
                 ("n_wide_nodes", ctypes.c_uint32), ("wide_stack", ctypes.c_uint32)]
+
+
+class BvhCheck(ctypes.Structure):
+    _fields_ = [("ok", ctypes.c_uint32), ("wide", ctypes.c_uint32), ("tris_reached", ctypes.c_uint32),
+                ("tris_not_once", ctypes.c_uint32), ("box_violations", ctypes.c_uint32),
+                ("bad_refs", ctypes.c_uint32), ("stack_bound", ctypes.c_uint32), ("stack_ok", ctypes.c_uint32),
+                ("bad_kind", ctypes.c_uint32), ("bad_node", ctypes.c_uint32), ("bad_slot", ctypes.c_uint32),
+                ("bad_tri", ctypes.c_uint32), ("bad_box", ctypes.c_float * 6), ("bad_outer", ctypes.c_float * 6)]
+
+    def as_dict(self):
+        d = {name: int(getattr(self, name)) for name, _ in self._fields_[:12]}
+        if self.box_violations:
+            d["bad_box"] = [float(x) for x in self.bad_box]
+            d["bad_outer"] = [float(x) for x in self.bad_outer]
+        return d
 
 
 class RcParams(ctypes.Structure):
@@ -213,6 +231,7 @@ _SIGNATURES = {
     "vkrt_bvh_refit": (_i32, [_vp, _vp, _vp, _u32, _i32]),
     "vkrt_bvh_get_info": (_i32, [_vp, ctypes.POINTER(BvhInfo)]),
     "vkrt_bvh_destroy": (None, [_vp]),
+    "vkrt_bvh_validate": (_i32, [_vp, _vp, _vp, _u32, _i32, ctypes.POINTER(BvhCheck)]),
     "vkrt_set_trace_config": (_i32, [_vp, ctypes.POINTER(TraceConfig)]),
     "vkrt_get_trace_config": (_i32, [_vp, ctypes.POINTER(TraceConfig)]),
     "vkrt_trace_closest": (_i32, [_vp, _vp, _vp, _u32, _vp, _i32]),
@@ -224,6 +243,7 @@ _SIGNATURES = {
     "vkrt_batch_launch": (_i32, [_vp, _i32]),
     "vkrt_batch_autotune": (_i32, [_vp, _u32, ctypes.POINTER(BatchConfig), ctypes.POINTER(ctypes.c_float)]),
     "vkrt_batch_total_rays": (_u64, [_vp]),
+    "vkrt_batch_candidates": (_u32, [_vp, _u32]),
     "vkrt_batch_closest": (_i32, [_vp, _vp, _u32, _i32]),
     "vkrt_pt_default_settings": (None, [ctypes.POINTER(PtSettings)]),
     "vkrt_pt_create": (_i32, [_vp, ctypes.POINTER(PtSettings), _u32, ctypes.POINTER(_vp)]),
@@ -621,6 +641,13 @@ class Bvh:
         self.ctx.check(lib().vkrt_bvh_refit(self.ctx.handle, self._h, _ptr(positions)[0], stride,
                                             _flags([positions])))
 
+    def validate(self, positions, stride=12):
+        """host-side invariant check (vkrt_bvh_validate) against the mesh positions; returns a dict"""
+        out = BvhCheck()
+        self.ctx.check(lib().vkrt_bvh_validate(self.ctx.handle, self._h, _ptr(positions)[0], stride,
+                                               _flags([positions]), ctypes.byref(out)))
+        return out.as_dict()
+
     def close(self):
         if self._h:
             lib().vkrt_bvh_destroy(self._h)
@@ -649,9 +676,10 @@ class Batch:
 
     @property
     def config(self):
+        """(block_size, lds_stack_depth, tile, blocks_per_cu, xcd_chunk)"""
         c = BatchConfig()
         self.ctx.check(lib().vkrt_batch_get_config(self._h, ctypes.byref(c)))
-        return c.block_size, c.lds_stack_depth, c.tile, c.blocks_per_cu
+        return c.shape()
 
     @config.setter
     def config(self, cfg):
@@ -660,7 +688,15 @@ class Batch:
     def autotune(self, repeats=2):
         best, ms = BatchConfig(), ctypes.c_float()
         self.ctx.check(lib().vkrt_batch_autotune(self._h, repeats, ctypes.byref(best), ctypes.byref(ms)))
-        return (best.block_size, best.lds_stack_depth, best.tile, best.blocks_per_cu), ms.value
+        return best.shape(), ms.value
+
+    @staticmethod
+    def candidates():
+        """every launch shape the autotuner can pick"""
+        n = lib().vkrt_batch_candidates(None, 0)
+        arr = (BatchConfig * n)()
+        lib().vkrt_batch_candidates(arr, n)
+        return [c.shape() for c in arr]
 
     def total_rays(self):
         return lib().vkrt_batch_total_rays(self._h)

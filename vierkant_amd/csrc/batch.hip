@@ -4,11 +4,14 @@
 // The reference traces one scene per vkCmdTraceRaysKHR (src/RayTracer.cpp:128-129); this batches the work
 // of many independent scenes. Every scene's rays are cut into tiles of `tile` rays; a device table holds
 // each item's BVH and buffers and the prefix of its tile counts. Blocks of k_batch_trace claim tiles with
-// one atomic each and leave once the counter passes the number of tiles, so every wave reaches the exit.
-// The graph is [k_batch_reset (zeroes the counter with a kernel), k_batch_trace]; sizes and pointers live
-// in device memory, so the captured graph replays without re-capture until the launch shape changes.
+// one atomic each (from per-XCD queues, see claim_tile) and leave once every queue is exhausted, so every
+// wave reaches the exit. The graph is [k_batch_reset (zeroes the counters with a kernel), k_batch_trace];
+// sizes and pointers live in device memory, so the captured graph replays without re-capture until the
+// launch shape changes.
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -34,21 +37,48 @@ struct batch_item_dev
     uint32_t pad[3];
 };
 
+constexpr uint32_t XCDS = 8;           // MI355X: workgroup b is dispatched to XCD b % 8
+constexpr uint32_t COUNTER_STRIDE = 16;// one 64-B line per queue counter
+constexpr uint32_t N_COUNTERS = XCDS * COUNTER_STRIDE;
+
 __global__ void k_batch_reset(uint32_t *counter) { counter[threadIdx.x] = 0u; }
+
+//! Tile claims. chunk == 0: one queue, tiles in order. chunk == C > 0: the tile sequence is cut into chunks of
+//! C consecutive tiles dealt round robin to the 8 XCDs; XCD x's i-th claim is tile ((i / C) 8 + x) C + i % C,
+//! which grows with i, so a queue whose claim passes the tile count is exhausted for good. A block first
+//! drains its own XCD's queue (neighbouring tiles of one scene then share that XCD's L2), then the others in
+//! turn, so no XCD idles while tiles remain; every block leaves after at most one failed claim per queue.
+__device__ __forceinline__ uint32_t claim_tile(uint32_t *__restrict__ counters, uint32_t chunk, uint32_t home,
+                                               uint32_t total, uint32_t &queue)
+{
+    const uint32_t n_queues = chunk ? XCDS : 1u;
+    while(queue < n_queues)
+    {
+        const uint32_t x = (home + queue) % n_queues;
+        const uint64_t i = atomicAdd(&counters[x * COUNTER_STRIDE], 1u);
+        const uint64_t t = chunk ? ((i / chunk) * XCDS + x) * chunk + i % chunk : i;
+        if(t < total) return static_cast<uint32_t>(t);
+        ++queue;
+    }
+    return total;
+}
 
 template<int BLOCK, int LDS>
 __global__ void __launch_bounds__(BLOCK) k_batch_trace(const batch_item_dev *__restrict__ items, uint32_t n_items,
                                                        const uint32_t *__restrict__ tile_prefix, uint32_t tile,
-                                                       uint32_t *__restrict__ counter)
+                                                       uint32_t *__restrict__ counters, uint32_t chunk)
 {
     __shared__ int32_t smem[LDS > 0 ? LDS * BLOCK : 1];
     __shared__ uint32_t s_tile;
     const uint32_t total = tile_prefix[n_items];
+    const uint32_t home = blockIdx.x % XCDS;
+    uint32_t queue = 0;// thread 0's position in the queue order (only thread 0 claims)
     while(true)
     {
-        if(threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
+        if(threadIdx.x == 0) s_tile = claim_tile(counters, chunk, home, total, queue);
         __syncthreads();
-        const uint32_t t = s_tile;
+        // every lane reads the same word: make it scalar, so the item lookup and its fields stay in SGPRs
+        const uint32_t t = __builtin_amdgcn_readfirstlane(s_tile);
         __syncthreads();
         if(t >= total) break;
         // item owning tile t: the last item whose prefix is <= t (binary search, block-uniform)
@@ -84,31 +114,70 @@ __global__ void __launch_bounds__(BLOCK) k_batch_trace(const batch_item_dev *__r
 
 template<int BLOCK, int LDS>
 static void launch_batch(hipStream_t s, uint32_t grid, const batch_item_dev *items, uint32_t n_items,
-                         const uint32_t *prefix, uint32_t tile, uint32_t *counter)
+                         const uint32_t *prefix, uint32_t tile, uint32_t *counter, uint32_t chunk)
 {
-    k_batch_trace<BLOCK, LDS><<<grid, BLOCK, 0, s>>>(items, n_items, prefix, tile, counter);
+    k_batch_trace<BLOCK, LDS><<<grid, BLOCK, 0, s>>>(items, n_items, prefix, tile, counter, chunk);
 }
 
 template<int BLOCK>
 static void launch_batch_lds(hipStream_t s, uint32_t lds, uint32_t grid, const batch_item_dev *items,
-                             uint32_t n_items, const uint32_t *prefix, uint32_t tile, uint32_t *counter)
+                             uint32_t n_items, const uint32_t *prefix, uint32_t tile, uint32_t *counter,
+                             uint32_t chunk)
 {
     switch(lds)
     {
-        case 0: launch_batch<BLOCK, 0>(s, grid, items, n_items, prefix, tile, counter); break;
-        case 8: launch_batch<BLOCK, 8>(s, grid, items, n_items, prefix, tile, counter); break;
-        case 16: launch_batch<BLOCK, 16>(s, grid, items, n_items, prefix, tile, counter); break;
-        default: launch_batch<BLOCK, 32>(s, grid, items, n_items, prefix, tile, counter); break;
+        case 0: launch_batch<BLOCK, 0>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
+        case 8: launch_batch<BLOCK, 8>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
+        case 16: launch_batch<BLOCK, 16>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
+        case 24: launch_batch<BLOCK, 24>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
+        case 32: launch_batch<BLOCK, 32>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
+        case 48: launch_batch<BLOCK, 48>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
+        default: launch_batch<BLOCK, 64>(s, grid, items, n_items, prefix, tile, counter, chunk); break;
     }
 }
+
+constexpr size_t LDS_PER_CU = 160 * 1024;
 
 static bool valid_batch_config(const vkrt_batch_config &c)
 {
     const bool block = c.block_size == 64 || c.block_size == 128 || c.block_size == 256;
     const bool lds = c.lds_stack_depth == 0 || c.lds_stack_depth == 8 || c.lds_stack_depth == 16 ||
-                     c.lds_stack_depth == 32;
+                     c.lds_stack_depth == 24 || c.lds_stack_depth == 32 || c.lds_stack_depth == 48 ||
+                     c.lds_stack_depth == 64;
     const bool tile = c.tile >= 64 && c.tile <= (1u << 20) && (c.tile % 64) == 0;
-    return block && lds && tile && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 64;
+    const bool chunk = c.xcd_chunk <= 64;
+    // one block's LDS stack must fit the CU's 160 KiB
+    const bool fits = (size_t)c.lds_stack_depth * c.block_size * 4 <= LDS_PER_CU;
+    return block && lds && tile && chunk && fits && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 64;
+}
+
+// the autotuner's shape space: block sizes, LDS-resident stack depths over the 16-64 range (the traversal
+// stack is 64 entries; the rest lives in scratch), tiles, resident threads per CU (blocks per CU scale with
+// the block size) and the XCD chunk of the claim order. Shapes whose blocks cannot all hold their LDS stack
+// at once on a CU are left out.
+constexpr uint32_t TUNE_BLOCKS[] = {64, 128, 256};
+constexpr uint32_t TUNE_LDS[] = {16, 32, 48, 64};
+constexpr uint32_t TUNE_TILES[] = {256, 1024, 4096};
+constexpr uint32_t TUNE_THREADS_PER_CU[] = {512, 1024, 2048};
+constexpr uint32_t TUNE_CHUNKS[] = {0, 4};
+
+constexpr vkrt_batch_config DEFAULT_CONFIG{256, 16, 1024, 8, 4, {0, 0, 0}};// one of the candidates
+
+static std::vector<vkrt_batch_config> candidate_shapes()
+{
+    std::vector<vkrt_batch_config> out;
+    for(uint32_t bs: TUNE_BLOCKS)
+        for(uint32_t ld: TUNE_LDS)
+            for(uint32_t tl: TUNE_TILES)
+                for(uint32_t th: TUNE_THREADS_PER_CU)
+                    for(uint32_t ch: TUNE_CHUNKS)
+                    {
+                        const uint32_t per_cu = th / bs;
+                        if((size_t)per_cu * ld * bs * 4 > LDS_PER_CU) continue;
+                        vkrt_batch_config c{bs, ld, tl, per_cu, ch, {0, 0, 0}};
+                        if(valid_batch_config(c)) out.push_back(c);
+                    }
+    return out;
 }
 
 }// namespace vkrt
@@ -119,7 +188,7 @@ struct vkrt_batch
     std::vector<vkrt::batch_item_dev> items;
     std::vector<uint32_t> n_rays;
     vkrt::device_buffer d_items, d_prefix, d_counter;
-    vkrt_batch_config cfg{256, 16, 1024, 8, {0, 0, 0, 0}};
+    vkrt_batch_config cfg = vkrt::DEFAULT_CONFIG;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t total_rays = 0;
@@ -154,15 +223,16 @@ static void prepare(vkrt_batch *b)
     release_graph(b);
     const uint32_t grid = std::max<uint32_t>(1, b->cfg.blocks_per_cu * b->ctx->cu_count);
     VKRT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    k_batch_reset<<<1, 4, 0, s>>>(b->d_counter.as<uint32_t>());
+    k_batch_reset<<<1, N_COUNTERS, 0, s>>>(b->d_counter.as<uint32_t>());
     const batch_item_dev *items = b->d_items.as<batch_item_dev>();
     const uint32_t *pre = b->d_prefix.as<uint32_t>();
     uint32_t *counter = b->d_counter.as<uint32_t>();
+    const uint32_t ld = b->cfg.lds_stack_depth, tl = b->cfg.tile, ch = b->cfg.xcd_chunk;
     switch(b->cfg.block_size)
     {
-        case 64: launch_batch_lds<64>(s, b->cfg.lds_stack_depth, grid, items, n, pre, b->cfg.tile, counter); break;
-        case 128: launch_batch_lds<128>(s, b->cfg.lds_stack_depth, grid, items, n, pre, b->cfg.tile, counter); break;
-        default: launch_batch_lds<256>(s, b->cfg.lds_stack_depth, grid, items, n, pre, b->cfg.tile, counter); break;
+        case 64: launch_batch_lds<64>(s, ld, grid, items, n, pre, tl, counter, ch); break;
+        case 128: launch_batch_lds<128>(s, ld, grid, items, n, pre, tl, counter, ch); break;
+        default: launch_batch_lds<256>(s, ld, grid, items, n, pre, tl, counter, ch); break;
     }
     hipError_t launch_err = hipGetLastError();
     hipError_t end_err = hipStreamEndCapture(s, &b->graph);
@@ -227,7 +297,7 @@ int vkrt_batch_create(vkrt_ctx *ctx, const vkrt_batch_item *items, uint32_t n_it
         b->d_items.reserve(b->items.size() * sizeof(batch_item_dev));
         VKRT_HIP(hipMemcpyAsync(b->d_items.ptr, b->items.data(), b->items.size() * sizeof(batch_item_dev),
                                 hipMemcpyHostToDevice, ctx->stream));
-        b->d_counter.reserve(64);
+        b->d_counter.reserve(N_COUNTERS * sizeof(uint32_t));
         prepare(b.get());
         *out = b.release();
     });
@@ -247,7 +317,8 @@ int vkrt_batch_set_config(vkrt_batch *b, const vkrt_batch_config *cfg)
     if(!b) return VKRT_E_INVALID_ARG;
     return guarded(b->ctx, [&] {
         VKRT_REQUIRE(cfg && valid_batch_config(*cfg),
-                     "block_size 64/128/256, lds_stack_depth 0/8/16/32, tile a multiple of 64, blocks_per_cu 1..64");
+                     "block_size 64/128/256, lds_stack_depth 0/8/16/24/32/48/64 (at most 160 KiB per block), tile a "
+                     "multiple of 64, blocks_per_cu 1..64, xcd_chunk 0..64");
         b->cfg = *cfg;
         prepare(b);
     });
@@ -273,35 +344,62 @@ int vkrt_batch_autotune(vkrt_batch *b, uint32_t repeats, vkrt_batch_config *best
 {
     if(!b) return VKRT_E_INVALID_ARG;
     return guarded(b->ctx, [&] {
-        const uint32_t blocks[] = {64, 128, 256};
-        const uint32_t lds[] = {0, 8, 16, 32};
-        const uint32_t tiles[] = {256, 1024, 4096};
-        const uint32_t occupancy[] = {4, 8};
-        vkrt_batch_config win = b->cfg;
-        float win_ms = 3.4e38f;
+        // coordinate descent over the candidate shapes: starting from the current shape, each round moves to
+        // the fastest candidate that differs from the winner in one field, until a round changes nothing
+        const std::vector<vkrt_batch_config> cands = candidate_shapes();
         const int reps = repeats ? static_cast<int>(repeats) : 2;
-        for(uint32_t bs: blocks)
-            for(uint32_t ld: lds)
-                for(uint32_t tl: tiles)
-                    for(uint32_t oc: occupancy)
-                    {
-                        // LDS of one block must leave room for at least two blocks per CU (160 KiB)
-                        if((size_t)ld * bs * 4 > 64 * 1024) continue;
-                        if(bs * oc > 4096) continue;
-                        b->cfg = vkrt_batch_config{bs, ld, tl, oc, {0, 0, 0, 0}};
-                        prepare(b);
-                        const float ms = timed_launch(b, reps);
-                        if(ms < win_ms)
-                        {
-                            win_ms = ms;
-                            win = b->cfg;
-                        }
-                    }
+        auto key = [](const vkrt_batch_config &c) {
+            return std::array<uint32_t, 5>{c.block_size, c.lds_stack_depth, c.tile, c.blocks_per_cu, c.xcd_chunk};
+        };
+        std::map<std::array<uint32_t, 5>, float> timed;
+        auto measure = [&](const vkrt_batch_config &c) {
+            auto it = timed.find(key(c));
+            if(it != timed.end()) return it->second;
+            b->cfg = c;
+            prepare(b);
+            const float ms = timed_launch(b, reps);
+            timed[key(c)] = ms;
+            return ms;
+        };
+        // the descent starts from the current shape when it is a candidate, else from the default one
+        vkrt_batch_config win = DEFAULT_CONFIG;
+        for(const vkrt_batch_config &c: cands)
+            if(key(c) == key(b->cfg)) win = b->cfg;
+        float win_ms = measure(win);
+        for(int round = 0; round < 4; ++round)
+        {
+            bool moved = false;
+            for(const vkrt_batch_config &c: cands)
+            {
+                const auto a = key(c), w = key(win);
+                int diff = 0;
+                for(int f = 0; f < 5; ++f) diff += a[f] != w[f];
+                // blocks per CU follows the block size: a change of block size keeps the threads per CU
+                const bool same_threads = c.block_size * c.blocks_per_cu == win.block_size * win.blocks_per_cu;
+                const bool one_step = diff == 1 || (diff == 2 && a[0] != w[0] && a[3] != w[3] && same_threads);
+                if(!one_step) continue;
+                const float ms = measure(c);
+                if(ms < win_ms)
+                {
+                    win_ms = ms;
+                    win = c;
+                    moved = true;
+                }
+            }
+            if(!moved) break;
+        }
         b->cfg = win;
         prepare(b);
         if(best) *best = win;
         if(best_ms) *best_ms = win_ms;
     });
+}
+
+uint32_t vkrt_batch_candidates(vkrt_batch_config *out, uint32_t capacity)
+{
+    const std::vector<vkrt_batch_config> c = candidate_shapes();
+    for(uint32_t i = 0; out && i < capacity && i < c.size(); ++i) out[i] = c[i];
+    return static_cast<uint32_t>(c.size());
 }
 
 uint64_t vkrt_batch_total_rays(const vkrt_batch *b) { return b ? b->total_rays : 0u; }

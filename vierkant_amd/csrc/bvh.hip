@@ -22,6 +22,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
+#include <cmath>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -186,6 +188,21 @@ __device__ __forceinline__ void load_box(const float *p, float b[6])
     for(int a = 0; a < 6; ++a) b[a] = p[a];
 }
 
+//! one arrival at node p of the bottom-up refit: the atomic counts the arrivals, and the second one (the
+//! return value 1) owns the node, its children's boxes being published by then. Release side: the box
+//! store of the previous step must be written back to memory before the flag increment is visible on
+//! another XCD (each XCD has its own L2). The acq_rel atomic emits that write-back (buffer_wbl2 sc1); the
+//! wait for the store and the write-back to complete is spelled out here, because hipcc leaves it out when
+//! the store and the atomic meet across a loop back edge, and without it another XCD could read a stale
+//! child box and union a too-small parent box.
+__device__ __forceinline__ uint32_t refit_arrive(uint32_t *flags, int32_t p)
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return old;
+}
+
 __global__ void k_refit(uint32_t n, const int2 *__restrict__ child, const int32_t *__restrict__ parent_int,
                         const int32_t *__restrict__ parent_leaf, const float *__restrict__ tbox,
                         float *__restrict__ nbox, uint32_t *__restrict__ flags)
@@ -193,10 +210,9 @@ __global__ void k_refit(uint32_t n, const int2 *__restrict__ child, const int32_
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if(j >= n) return;
     int32_t p = parent_leaf[j];
-    while(p >= 0)
+    if(p < 0 || refit_arrive(flags, p) == 0) return;// the sibling subtree is not finished; its thread continues
+    while(true)
     {
-        uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if(old == 0) return;// the sibling subtree is not finished; its thread continues
         int2 c = child[p];
         float a[6], b[6];
         load_box(c.x < 0 ? tbox + 6 * (size_t)(~c.x) : nbox + 6 * (size_t)c.x, a);
@@ -205,6 +221,7 @@ __global__ void k_refit(uint32_t n, const int2 *__restrict__ child, const int32_
         for(int k = 0; k < 3; ++k) o[k] = fminf(a[k], b[k]);
         for(int k = 3; k < 6; ++k) o[k] = fmaxf(a[k], b[k]);
         p = parent_int[p];
+        if(p < 0 || refit_arrive(flags, p) == 0) return;
     }
 }
 
@@ -786,14 +803,16 @@ static void build_bounds(vkrt_ctx *ctx, vkrt_bvh *bvh)
 struct collapse_child
 {
     int32_t ref;
+    int32_t src;// (BVH2 parent node << 1) | side: where the box is stored, re-read by a refit
     float lo[3], hi[3];
 };
 
-__device__ __forceinline__ collapse_child child_of(const vkrt_bvh_node &nd, int k)
+__device__ __forceinline__ collapse_child child_of(const vkrt_bvh_node &nd, int32_t node, int k)
 {
     collapse_child c;
     const float4 xy = k == 0 ? nd.a : nd.b;
     c.ref = k == 0 ? nd.d.x : nd.d.y;
+    c.src = (node << 1) | k;
     c.lo[0] = xy.x;
     c.hi[0] = xy.y;
     c.lo[1] = xy.z;
@@ -820,8 +839,8 @@ __global__ void __launch_bounds__(256) k_collapse(const vkrt_bvh_node *__restric
     const int4 item = in_q[t];
     collapse_child ch[4];
     const vkrt_bvh_node root = n2[item.x];
-    ch[0] = child_of(root, 0);
-    ch[1] = child_of(root, 1);
+    ch[0] = child_of(root, item.x, 0);
+    ch[1] = child_of(root, item.x, 1);
     int cnt = 2;
     while(cnt < 4)
     {
@@ -838,18 +857,20 @@ __global__ void __launch_bounds__(256) k_collapse(const vkrt_bvh_node *__restric
             }
         }
         if(best < 0) break;
-        const vkrt_bvh_node nd = n2[ch[best].ref];
-        ch[best] = child_of(nd, 0);
-        ch[cnt] = child_of(nd, 1);
+        const int32_t opened = ch[best].ref;
+        const vkrt_bvh_node nd = n2[opened];
+        ch[best] = child_of(nd, opened, 0);
+        ch[cnt] = child_of(nd, opened, 1);
         ++cnt;
     }
     const int32_t depth_here = item.z + (cnt - 1);
     atomicMax(&counters[2], static_cast<uint32_t>(depth_here));
     vkrt_bvh4_node out;
     float lo[3][4], hi[3][4];
-    int32_t refs[4];
+    int32_t refs[4], srcs[4];
     for(int j = 0; j < 4; ++j)
     {
+        srcs[j] = j < cnt ? ch[j].src : -1;
         for(int a = 0; a < 3; ++a)
         {
             lo[a][j] = j < cnt ? ch[j].lo[a] : 0.0f;
@@ -873,8 +894,42 @@ __global__ void __launch_bounds__(256) k_collapse(const vkrt_bvh_node *__restric
     out.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
     out.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
     out.child = make_int4(refs[0], refs[1], refs[2], refs[3]);
-    out.pad = make_int4(0, 0, 0, 0);
+    out.src = make_int4(srcs[0], srcs[1], srcs[2], srcs[3]);
     n4[item.y] = out;
+}
+
+//! refit of the 4-wide layout: every used child slot re-reads its box from the BVH2 node it was collapsed
+//! from (the BVH2 keeps its topology and node numbering on refit), so the collapse's shape, its worst-case
+//! stack depth and the choice of the 4-wide walk stay what the build recorded
+__global__ void k_wide_refit(const vkrt_bvh_node *__restrict__ n2, uint32_t n4_count, vkrt_bvh4_node *__restrict__ n4)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= n4_count) return;
+    vkrt_bvh4_node nd = n4[i];
+    float *lox = &nd.lox.x, *hix = &nd.hix.x, *loy = &nd.loy.x, *hiy = &nd.hiy.x, *loz = &nd.loz.x, *hiz = &nd.hiz.x;
+    const int32_t src[4] = {nd.src.x, nd.src.y, nd.src.z, nd.src.w};
+    for(int j = 0; j < 4; ++j)
+    {
+        if(src[j] < 0) continue;
+        const vkrt_bvh_node p = n2[src[j] >> 1];
+        const collapse_child c = child_of(p, src[j] >> 1, src[j] & 1);
+        lox[j] = c.lo[0];
+        hix[j] = c.hi[0];
+        loy[j] = c.lo[1];
+        hiy[j] = c.hi[1];
+        loz[j] = c.lo[2];
+        hiz[j] = c.hi[2];
+    }
+    n4[i] = nd;
+}
+
+//! refreshes the boxes of an existing 4-wide layout after the BVH2 was refitted (no change of shape)
+static void refit_wide(vkrt_ctx *ctx, vkrt_bvh *bvh)
+{
+    if(bvh->empty || !bvh->wide || bvh->root4 < 0 || bvh->n_nodes4 == 0) return;
+    k_wide_refit<<<div_up(bvh->n_nodes4, 256), 256, 0, ctx->stream>>>(bvh->nodes.as<vkrt_bvh_node>(), bvh->n_nodes4,
+                                                                       bvh->nodes4.as<vkrt_bvh4_node>());
+    VKRT_HIP(hipGetLastError());
 }
 
 //! collapse the BVH2 into the BVH4 used for traversal; keeps the BVH2 path when the worst-case stack
@@ -924,7 +979,9 @@ static void build_wide(vkrt_ctx *ctx, vkrt_bvh *bvh)
     bvh->stack4 = host_cnt[2];
     bvh->root4 = 0;
     VKRT_REQUIRE(bvh->n_nodes4 <= cap, "BVH4 collapse overflow");
-    bvh->wide = bvh->stack4 <= (uint32_t)STACK_SIZE;
+    // the 4-wide walk addresses nodes with 32-bit byte offsets (node4_intervals)
+    bvh->wide = bvh->stack4 <= (uint32_t)STACK_SIZE &&
+                (uint64_t)bvh->n_nodes4 * sizeof(vkrt_bvh4_node) <= 0xffffffffull;
 }
 
 static void refresh_info(vkrt_bvh *bvh, hipStream_t s)
@@ -1009,9 +1066,203 @@ int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_
                                                 bvh->pre.as<float>(), box, cb, bvh->err.as<uint32_t>());
         k_sorted_boxes<<<div_up(n, 256), 256, 0, s>>>(bvh->order.as<uint32_t>(), n, box, bvh->tbox.as<float>());
         build_bounds(ctx, bvh);
-        build_wide(ctx, bvh);
+        refit_wide(ctx, bvh);
         VKRT_HIP(hipStreamSynchronize(s));
         refresh_info(bvh, s);
+    });
+}
+
+int vkrt_bvh_validate(vkrt_ctx *ctx, const vkrt_bvh *bvh, const float *positions, uint32_t stride_bytes, int flags,
+                      vkrt_bvh_check *out)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(bvh && out, "null bvh or output");
+        VKRT_REQUIRE(stride_bytes >= 12 && stride_bytes % 4 == 0, "stride must be >= 12 and a multiple of 4");
+        VKRT_REQUIRE(bvh->empty || positions, "null positions");
+        *out = vkrt_bvh_check{};
+        out->wide = bvh->wide ? 1u : 0u;
+        if(bvh->empty)
+        {
+            out->ok = 1;
+            out->stack_ok = 1;
+            return;
+        }
+        hipStream_t s = ctx->stream;
+        const uint32_t n = bvh->n_tri;
+        auto fetch = [&](const void *src, size_t bytes, bool device, std::vector<uint8_t> &dst) {
+            dst.resize(bytes);
+            if(!bytes) return;
+            VKRT_HIP(hipMemcpyAsync(dst.data(), src, bytes, device ? hipMemcpyDeviceToHost : hipMemcpyHostToHost, s));
+        };
+        std::vector<uint8_t> h_nodes, h_nodes4, h_tris, h_idx, h_pos;
+        fetch(bvh->nodes.ptr, (size_t)bvh->n_nodes * sizeof(vkrt_bvh_node), true, h_nodes);
+        if(bvh->wide) fetch(bvh->nodes4.ptr, (size_t)bvh->n_nodes4 * sizeof(vkrt_bvh4_node), true, h_nodes4);
+        fetch(bvh->tris.ptr, (size_t)n * 3 * sizeof(float4), true, h_tris);
+        fetch(bvh->indices.ptr, (size_t)n * 3 * sizeof(uint32_t), true, h_idx);
+        fetch(positions, (size_t)(bvh->n_vert - 1) * stride_bytes + 12, (flags & VKRT_DEVICE_PTRS) != 0, h_pos);
+        VKRT_HIP(hipStreamSynchronize(s));
+        const auto *nodes = reinterpret_cast<const vkrt_bvh_node *>(h_nodes.data());
+        const auto *nodes4 = reinterpret_cast<const vkrt_bvh4_node *>(h_nodes4.data());
+        const auto *tris = reinterpret_cast<const float4 *>(h_tris.data());
+        const auto *idx = reinterpret_cast<const uint32_t *>(h_idx.data());
+        std::vector<uint32_t> seen(n, 0);
+        auto vertex = [&](uint32_t v, int a) {
+            float x;
+            std::memcpy(&x, h_pos.data() + (size_t)v * stride_bytes + 4 * a, 4);
+            return x;
+        };
+        auto violation = [&](uint32_t kind, uint32_t node, uint32_t slot, uint32_t tri, const float *box,
+                             const float *outer, int n_outer) {
+            if(out->box_violations++ != 0) return;
+            out->bad_kind = kind;
+            out->bad_node = node;
+            out->bad_slot = slot;
+            out->bad_tri = tri;
+            std::memcpy(out->bad_box, box, sizeof out->bad_box);
+            for(int a = 0; a < n_outer; ++a) out->bad_outer[a] = outer[a];
+        };
+        // a leaf's triangles: counted, and their vertices inside the box the parent stores for the leaf
+        auto leaf = [&](int32_t ref, const float *box, uint32_t node, uint32_t slot) {
+            const uint32_t v = static_cast<uint32_t>(~ref), first = v >> 3, count = (v & 7u) + 1;
+            if((uint64_t)first + count > n)
+            {
+                ++out->bad_refs;
+                return;
+            }
+            for(uint32_t k = first; k < first + count; ++k)
+            {
+                ++out->tris_reached;
+                uint32_t orig;
+                std::memcpy(&orig, &tris[3 * (size_t)k + 2].y, 4);
+                if(orig >= n)
+                {
+                    ++out->bad_refs;
+                    continue;
+                }
+                ++seen[orig];
+                if(!box) continue;
+                bool finite = true, inside = true;
+                float outside[3] = {0.f, 0.f, 0.f};
+                for(int c = 0; c < 3; ++c)
+                {
+                    const uint32_t vi = idx[3 * (size_t)orig + c];
+                    if(vi >= bvh->n_vert)
+                    {
+                        finite = false;
+                        break;
+                    }
+                    bool in = true;
+                    for(int a = 0; a < 3; ++a)
+                    {
+                        const float x = vertex(vi, a);
+                        finite = finite && std::isfinite(x);
+                        in = in && x >= box[a] && x <= box[3 + a];
+                    }
+                    if(!in && inside)
+                        for(int a = 0; a < 3; ++a) outside[a] = vertex(vi, a);
+                    inside = inside && in;
+                }
+                // non-finite triangles get empty boxes
+                if(finite && !inside) violation(2, node, slot, orig, box, outside, 3);
+            }
+        };
+        auto inside = [](const float *in, const float *outer) {
+            for(int a = 0; a < 3; ++a)
+                if(!(in[a] >= outer[a] && in[3 + a] <= outer[3 + a])) return false;
+            return true;
+        };
+        uint32_t recorded = 0;
+        if(bvh->root_ref < 0)
+        {
+            leaf(bvh->root_ref, nullptr, 0, 0);// a single leaf: no box is stored for it
+            out->stack_bound = 0;
+        }
+        else if(bvh->wide)
+        {
+            recorded = bvh->stack4;
+            struct item
+            {
+                int32_t node;
+                uint32_t depth;
+                float box[6];
+                bool has_box;
+            };
+            std::vector<item> todo{{0, 0u, {}, false}};
+            while(!todo.empty())
+            {
+                const item it = todo.back();
+                todo.pop_back();
+                if(it.node < 0 || (uint32_t)it.node >= bvh->n_nodes4)
+                {
+                    ++out->bad_refs;
+                    continue;
+                }
+                const vkrt_bvh4_node &nd = nodes4[it.node];
+                const int32_t ch[4] = {nd.child.x, nd.child.y, nd.child.z, nd.child.w};
+                const float *lx = &nd.lox.x, *hx = &nd.hix.x, *ly = &nd.loy.x, *hy = &nd.hiy.x, *lz = &nd.loz.x,
+                            *hz = &nd.hiz.x;
+                uint32_t used = 0;
+                for(int j = 0; j < 4; ++j) used += ch[j] != EMPTY_CHILD ? 1u : 0u;
+                const uint32_t depth_here = it.depth + (used ? used - 1 : 0);
+                out->stack_bound = std::max(out->stack_bound, depth_here);
+                for(int j = 0; j < 4; ++j)
+                {
+                    if(ch[j] == EMPTY_CHILD) continue;
+                    const float b[6] = {lx[j], ly[j], lz[j], hx[j], hy[j], hz[j]};
+                    if(it.has_box && !inside(b, it.box)) violation(1, it.node, j, 0, b, it.box, 6);
+                    if(ch[j] < 0) leaf(ch[j], b, it.node, j);
+                    else
+                    {
+                        item c{ch[j], depth_here, {}, true};
+                        std::memcpy(c.box, b, sizeof b);
+                        todo.push_back(c);
+                    }
+                }
+            }
+        }
+        else
+        {
+            // binary tree: the stack holds at most one entry per ancestor of the deepest node
+            recorded = bvh->info.depth;
+            struct item
+            {
+                int32_t node;
+                uint32_t depth;
+                float box[6];
+                bool has_box;
+            };
+            std::vector<item> todo{{0, 1u, {}, false}};
+            while(!todo.empty())
+            {
+                const item it = todo.back();
+                todo.pop_back();
+                if(it.node < 0 || (uint32_t)it.node >= bvh->n_nodes)
+                {
+                    ++out->bad_refs;
+                    continue;
+                }
+                const vkrt_bvh_node &nd = nodes[it.node];
+                out->stack_bound = std::max(out->stack_bound, it.depth + 1);
+                const float b[2][6] = {{nd.a.x, nd.a.z, nd.c.x, nd.a.y, nd.a.w, nd.c.y},
+                                       {nd.b.x, nd.b.z, nd.c.z, nd.b.y, nd.b.w, nd.c.w}};
+                const int32_t ch[2] = {nd.d.x, nd.d.y};
+                for(int j = 0; j < 2; ++j)
+                {
+                    if(it.has_box && !inside(b[j], it.box)) violation(1, it.node, j, 0, b[j], it.box, 6);
+                    if(ch[j] < 0) leaf(ch[j], b[j], it.node, j);
+                    else
+                    {
+                        item c{ch[j], it.depth + 1, {}, true};
+                        std::memcpy(c.box, b[j], sizeof b[j]);
+                        todo.push_back(c);
+                    }
+                }
+            }
+        }
+        for(uint32_t k = 0; k < n; ++k) out->tris_not_once += seen[k] != 1 ? 1u : 0u;
+        out->stack_ok = (bvh->root_ref < 0 || out->stack_bound == recorded) && out->stack_bound <= (uint32_t)STACK_SIZE;
+        out->ok = out->tris_not_once == 0 && out->box_violations == 0 && out->bad_refs == 0 && out->stack_ok &&
+                  out->tris_reached == n;
     });
 }
 

@@ -22,7 +22,7 @@ struct vkrt_bvh4_node
 {
     float4 lox, hix, loy, hiy, loz, hiz;
     int4 child;// >= 0 BVH4 node, < 0 leaf (the BVH2 leaf encoding), EMPTY_CHILD unused
-    int4 pad;
+    int4 src;  // per slot (BVH2 node << 1) | side holding the child's box, -1 unused (read by a refit only)
 };
 
 struct vkrt_bvh

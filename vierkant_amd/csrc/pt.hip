@@ -931,7 +931,7 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
             {
                 int32_t c[4];
                 int nc;
-                if(wide) nc = visit4(nodes4[ref], r, r.tmax, c);
+                if(wide) nc = visit4(nodes4, ref, r, r.tmax, c);
                 else
                 {
                     const vkrt_bvh_node nd = nodes2[ref];

@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(BLOCK, 6) k_scene_trace(const vkrt_bvh4_node *
         if(ref >= 0)
         {
             int32_t c[4];
-            const int cnt = visit4(tlas[ref], rw, best_t, c);
+            const int cnt = visit4(tlas, ref, rw, best_t, c);
             if(cnt)
             {
                 for(int j = cnt - 1; j >= 1; --j) st.put(sp++, c[j]);
@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(BLOCK, 6) k_scene_trace(const vkrt_bvh4_node *
                     if(bref >= 0)
                     {
                         int32_t c[4];
-                        const int cnt = visit4(B.nodes[bref], ro, best_t, c);
+                        const int cnt = visit4(B.nodes, bref, ro, best_t, c);
                         if(cnt)
                         {
                             for(int j = cnt - 1; j >= 1; --j) st.put(bsp++, c[j]);
@@ -298,7 +298,7 @@ struct tlas_builder
         nd.loz = make_float4(lo_[2][0], lo_[2][1], lo_[2][2], lo_[2][3]);
         nd.hiz = make_float4(hi_[2][0], hi_[2][1], hi_[2][2], hi_[2][3]);
         nd.child = make_int4(ch[0], ch[1], ch[2], ch[3]);
-        nd.pad = make_int4(0, 0, 0, 0);
+        nd.src = make_int4(-1, -1, -1, -1);
         return static_cast<int32_t>(slot);
     }
 };

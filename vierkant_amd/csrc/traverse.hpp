@@ -24,6 +24,10 @@ struct ray_setup
     vec3 o, d, inv;
     float tmin, tmax;
     vec3 clo, chi;// slab offsets of the lo and hi planes (see make_ray_setup)
+    vec3 cn, cf;  // the same offsets for each axis's near plane (met first) and far plane
+    // byte offsets in a 4-wide node of each axis's near-plane float4 (lo when the direction component is
+    // >= 0, else hi); the far plane's float4 is at the offset ^ 16
+    uint32_t nx, ny, nz;
 };
 
 __device__ __forceinline__ ray_setup make_ray_setup(float4 a, float4 b)
@@ -51,6 +55,15 @@ __device__ __forceinline__ ray_setup make_ray_setup(float4 a, float4 b)
     offsets(r.o.x, r.inv.x, r.clo.x, r.chi.x);
     offsets(r.o.y, r.inv.y, r.clo.y, r.chi.y);
     offsets(r.o.z, r.inv.z, r.clo.z, r.chi.z);
+    // the near plane is lo where inv >= 0 (its offset is clo), hi elsewhere (its offset is chi): either way
+    // the smaller of the two offsets, the one shifted toward smaller t
+    r.cn = vec3{r.inv.x >= 0.0f ? r.clo.x : r.chi.x, r.inv.y >= 0.0f ? r.clo.y : r.chi.y,
+                r.inv.z >= 0.0f ? r.clo.z : r.chi.z};
+    r.cf = vec3{r.inv.x >= 0.0f ? r.chi.x : r.clo.x, r.inv.y >= 0.0f ? r.chi.y : r.clo.y,
+                r.inv.z >= 0.0f ? r.chi.z : r.clo.z};
+    r.nx = r.inv.x >= 0.0f ? 0u : 16u;
+    r.ny = r.inv.y >= 0.0f ? 32u : 48u;
+    r.nz = r.inv.z >= 0.0f ? 64u : 80u;
     return r;
 }
 
@@ -75,6 +88,42 @@ __device__ __forceinline__ void slab_interval(const ray_setup &r, float lox, flo
     // one rounding per end (a fused multiply-add): at most half an ulp against the 2^-16 |t| of the pad
     tn = fmaf(-fabsf(n_), T_PAD, n_);
     tf = fmaf(fabsf(f_), T_PAD, f_);
+}
+
+static_assert(sizeof(vkrt_bvh4_node) == 128, "4-wide node layout: lo/hi float4 pairs at 0/16, 32/48, 64/80");
+
+//! the padded intervals of a 4-wide node's four children and its child refs. Each axis's near-plane float4
+//! is loaded from the offset the ray chose at setup and the far-plane float4 from that offset ^ 16, so a
+//! child costs two fmas per axis and one max/min per end: for a valid box (lo <= hi) the near plane's
+//! value is the smaller of the two on every axis, the same value slab_interval's per-axis min picks. For a
+//! box with lo > hi (the empty box of non-finite triangles) the interval comes out empty. Node offsets are
+//! 32-bit: the build keeps the 4-wide layout only while the nodes fit 4 GiB.
+__device__ __forceinline__ void node4_intervals(const vkrt_bvh4_node *__restrict__ nodes, int32_t ref,
+                                                const ray_setup &r, float tn[4], float tf[4], int4 &child)
+{
+    const char *base = reinterpret_cast<const char *>(nodes);
+    const uint32_t nb = static_cast<uint32_t>(ref) * static_cast<uint32_t>(sizeof(vkrt_bvh4_node));
+    const uint32_t ax = nb + r.nx, ay = nb + r.ny, az = nb + r.nz;
+    const float4 nx = *reinterpret_cast<const float4 *>(base + ax);
+    const float4 fx = *reinterpret_cast<const float4 *>(base + (ax ^ 16u));
+    const float4 ny = *reinterpret_cast<const float4 *>(base + ay);
+    const float4 fy = *reinterpret_cast<const float4 *>(base + (ay ^ 16u));
+    const float4 nz = *reinterpret_cast<const float4 *>(base + az);
+    const float4 fz = *reinterpret_cast<const float4 *>(base + (az ^ 16u));
+    child = *reinterpret_cast<const int4 *>(base + nb + 96u);
+    const float n0[4] = {nx.x, nx.y, nx.z, nx.w}, f0[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float n1[4] = {ny.x, ny.y, ny.z, ny.w}, f1[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float n2[4] = {nz.x, nz.y, nz.z, nz.w}, f2[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+    for(int j = 0; j < 4; ++j)
+    {
+        const float n_ = fmaxf(fmaxf(fmaf(n0[j], r.inv.x, r.cn.x), fmaf(n1[j], r.inv.y, r.cn.y)),
+                               fmaf(n2[j], r.inv.z, r.cn.z));
+        const float f_ = fminf(fminf(fmaf(f0[j], r.inv.x, r.cf.x), fmaf(f1[j], r.inv.y, r.cf.y)),
+                               fmaf(f2[j], r.inv.z, r.cf.z));
+        tn[j] = fmaf(-fabsf(n_), T_PAD, n_);
+        tf[j] = fmaf(fabsf(f_), T_PAD, f_);
+    }
 }
 
 __device__ __forceinline__ void leaf_decode(int ref, uint32_t &first, uint32_t &count)
@@ -289,18 +338,14 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
     const ray_setup &r = s.r;
     if(s.ref >= 0)
     {
-        const vkrt_bvh4_node &nd = nodes[s.ref];
-        const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
-        const int4 cr = nd.child;
-        float k0, k1, k2, k3, f;
-        slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
-        bool v0 = child_hit(cr.x, k0, f, r.tmin, s.best_t);
-        slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
-        bool v1 = child_hit(cr.y, k1, f, r.tmin, s.best_t);
-        slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
-        bool v2 = child_hit(cr.z, k2, f, r.tmin, s.best_t);
-        slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
-        bool v3 = child_hit(cr.w, k3, f, r.tmin, s.best_t);
+        float tn[4], tf[4];
+        int4 cr;
+        node4_intervals(nodes, s.ref, r, tn, tf, cr);
+        const float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
+        bool v0 = child_hit(cr.x, k0, tf[0], r.tmin, s.best_t);
+        bool v1 = child_hit(cr.y, k1, tf[1], r.tmin, s.best_t);
+        bool v2 = child_hit(cr.z, k2, tf[2], r.tmin, s.best_t);
+        bool v3 = child_hit(cr.w, k3, tf[3], r.tmin, s.best_t);
         int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
         // sort nearest first with the missed children last (sort keys: +inf for them, the entry distance
         // clamped below +inf for the others): five branchless compare-exchanges of (key, ref)
@@ -332,21 +377,18 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
     return false;
 }
 
-//! children of a 4-wide node whose padded interval meets [tmin, best_t], nearest first; returns the count
-__device__ __forceinline__ int visit4(const vkrt_bvh4_node &nd, const ray_setup &r, float best_t, int32_t c[4])
+//! children of 4-wide node `ref` whose padded interval meets [tmin, best_t], nearest first; returns the count
+__device__ __forceinline__ int visit4(const vkrt_bvh4_node *__restrict__ nodes, int32_t ref, const ray_setup &r,
+                                      float best_t, int32_t c[4])
 {
-    const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
-    const int4 cr = nd.child;
-    float k[4], f;
+    float k[4], tf[4];
+    int4 cr;
+    node4_intervals(nodes, ref, r, k, tf, cr);
     bool v[4];
-    slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k[0], f);
-    v[0] = child_hit(cr.x, k[0], f, r.tmin, best_t);
-    slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k[1], f);
-    v[1] = child_hit(cr.y, k[1], f, r.tmin, best_t);
-    slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k[2], f);
-    v[2] = child_hit(cr.z, k[2], f, r.tmin, best_t);
-    slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k[3], f);
-    v[3] = child_hit(cr.w, k[3], f, r.tmin, best_t);
+    v[0] = child_hit(cr.x, k[0], tf[0], r.tmin, best_t);
+    v[1] = child_hit(cr.y, k[1], tf[1], r.tmin, best_t);
+    v[2] = child_hit(cr.z, k[2], tf[2], r.tmin, best_t);
+    v[3] = child_hit(cr.w, k[3], tf[3], r.tmin, best_t);
     c[0] = cr.x;
     c[1] = cr.y;
     c[2] = cr.z;
@@ -428,18 +470,14 @@ __device__ __forceinline__ bool traverse4_postponed(const vkrt_bvh4_node *__rest
     {
         while(node >= 0 && node != DONE)
         {
-            const vkrt_bvh4_node &nd = nodes[node];
-            const float4 lx = nd.lox, hx = nd.hix, ly = nd.loy, hy = nd.hiy, lz = nd.loz, hz = nd.hiz;
-            const int4 cr = nd.child;
-            float k0, k1, k2, k3, f;
-            slab_interval(r, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, k0, f);
-            bool v0 = child_hit(cr.x, k0, f, r.tmin, best_t);
-            slab_interval(r, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, k1, f);
-            bool v1 = child_hit(cr.y, k1, f, r.tmin, best_t);
-            slab_interval(r, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, k2, f);
-            bool v2 = child_hit(cr.z, k2, f, r.tmin, best_t);
-            slab_interval(r, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, k3, f);
-            bool v3 = child_hit(cr.w, k3, f, r.tmin, best_t);
+            float tn[4], tf[4];
+            int4 cr;
+            node4_intervals(nodes, node, r, tn, tf, cr);
+            const float k0 = tn[0], k1 = tn[1], k2 = tn[2], k3 = tn[3];
+            bool v0 = child_hit(cr.x, k0, tf[0], r.tmin, best_t);
+            bool v1 = child_hit(cr.y, k1, tf[1], r.tmin, best_t);
+            bool v2 = child_hit(cr.z, k2, tf[2], r.tmin, best_t);
+            bool v3 = child_hit(cr.w, k3, tf[3], r.tmin, best_t);
             int32_t c0 = cr.x, c1 = cr.y, c2 = cr.z, c3 = cr.w;
             float q0 = sort_key(v0, k0), q1 = sort_key(v1, k1), q2 = sort_key(v2, k2), q3 = sort_key(v3, k3);
             compare_exchange(q0, c0, q1, c1);
