@@ -98,7 +98,22 @@ static frame local_frame(v3 d)
 }
 static inline v3 to_local(frame f, v3 v) { return V3(dot(f.tx, v), dot(f.ty, v), dot(f.n, v)); }
 static inline v3 to_world(frame f, v3 v) { return add(add(scl(f.tx, v.x), scl(f.ty, v.y)), scl(f.n, v.z)); }
-static inline v3 sph(float s, float c, float phi) { return V3(s * cosf(phi), s * sinf(phi), c); }
+/* Sensitivity hook (test infrastructure, not reference behaviour): with g_ulp_bias set, every
+ * transcendental of the integrator (cos, sin, tan, atan, asin, pow, log2, exp) returns the next float above
+ * the library's result, so a test can measure how far a one-ulp difference between two math libraries moves
+ * an image. Set only for the duration of one orc_pt_render call (not for concurrent renders). */
+static int g_ulp_bias = 0;
+static inline float ulp_up(float r) { return g_ulp_bias ? nextafterf(r, INFINITY) : r; }
+static inline float tcos(float x) { return ulp_up(cosf(x)); }
+static inline float tsin(float x) { return ulp_up(sinf(x)); }
+static inline float ttan(float x) { return ulp_up(tanf(x)); }
+static inline float tatan(float x) { return ulp_up(atanf(x)); }
+static inline float tasin(float x) { return ulp_up(asinf(x)); }
+static inline float tpow(float x, float y) { return ulp_up(powf(x, y)); }
+static inline float tlog2(float x) { return ulp_up(log2f(x)); }
+static inline float texp(float x) { return ulp_up(expf(x)); }
+
+static inline v3 sph(float s, float c, float phi) { return V3(s * tcos(phi), s * tsin(phi), c); }
 
 static v3 hemi_cos(float xi_x, float xi_y)
 {
@@ -114,7 +129,7 @@ static v3 ggx_vndf(float xi_x, float xi_y, v3 wi, float a)
     float phi = (2.0f * xi_x - 1.0f) * PI;
     float z = fmaf(1.0f - xi_y, 1.0f + w.z, -w.z);
     float st = sqrtf(clampf(1.0f - z * z, 0.0f, 1.0f));
-    v3 c = V3(st * cosf(phi), st * sinf(phi), z);
+    v3 c = V3(st * tcos(phi), st * tsin(phi), z);
     v3 m = add(c, w);
     return nrmz(V3(m.x * a, m.y * a, m.z));
 }
@@ -123,7 +138,7 @@ static v3 gtr1_sample(float xi_x, float xi_y, float r)
 {
     float a = fmaxf(0.001f, r), a2 = a * a;
     float phi = xi_y * 2.0f * PI;
-    float c = sqrtf((1.0f - powf(a2, 1.0f - xi_x)) / (1.0f - a2));
+    float c = sqrtf((1.0f - tpow(a2, 1.0f - xi_x)) / (1.0f - a2));
     float s = clampf(sqrtf(1.0f - c * c), 0.0f, 1.0f);
     return sph(s, c, phi);
 }
@@ -342,6 +357,45 @@ static bs_t sample_disney(mat_t m, v3 N, v3 V, float eta, uint32_t *rng)
     }
     if(ret.pdf != ret.pdf) ret.pdf = 0.0f;
     return ret;
+}
+
+/* a vkrt_material record (36 words) as the integrator's material */
+static mat_t unpack_material(const float *x, uint32_t n_textures)
+{
+    mat_t mt;
+    memset(&mt, 0, sizeof(mt));
+    mt.color = V3(x[0], x[1], x[2]);
+    mt.emission = V3(x[4], x[5], x[6]);
+    mt.emission_strength = x[7];
+    mt.metal = x[8];
+    mt.rough = x[9];
+    mt.trans = x[10];
+    mt.ior = x[11];
+    mt.cc = x[12];
+    mt.cc_rough = x[13];
+    mt.sheen_rough = x[14];
+    mt.dt = x[15];
+    mt.sheen = V3(x[16], x[17], x[18]);
+    mt.dt_color = V3(x[20], x[21], x[22]);
+    uint32_t ts, tex, mode;
+    memcpy(&ts, &x[24], 4);
+    mt.two_sided = ts != 0;
+    mt.alpha = x[3];
+    memcpy(&tex, &x[25], 4);
+    memcpy(&mode, &x[26], 4);
+    mt.tex = tex <= n_textures ? tex : 0;
+    mt.alpha_mode = mode <= 2 ? (int)mode : 0;
+    mt.alpha_cutoff = x[27];
+    uint32_t slot[6];
+    memcpy(slot, &x[28], sizeof(slot));
+    for(int k = 0; k < 6; ++k) slot[k] = slot[k] <= n_textures ? slot[k] : 0;
+    mt.tex_normal = slot[0];
+    mt.tex_emission = slot[1];
+    mt.tex_arm = slot[2];
+    mt.tex_transmission = slot[3];
+    mt.tex_dt = slot[4];
+    mt.tex_dtc = slot[5];
+    return mt;
 }
 
 static v3 env_white(v3 d)
@@ -857,7 +911,7 @@ typedef struct
 static v3 sphere_cap(float xi_x, float xi_y, float sigma)
 {
     float phi = 2.0f * PI * xi_y;
-    float cs = cosf(sigma);
+    float cs = tcos(sigma);
     float ct = xi_x * (1.0f - cs) + cs;
     float st = sqrtf(1.0f - ct * ct);
     return sph(st, ct, phi);
@@ -878,7 +932,7 @@ static lsample_t sample_light(const light_t *l, v3 P, float xi_x, float xi_y)
         if(l->angular_size > 0.0f)
         {
             r.dir = to_world(local_frame(r.dir), sphere_cap(xi_x, xi_y, l->angular_size));
-            r.pdf = 1.0f / (2.0f * PI * (1.0f - cosf(l->angular_size)));
+            r.pdf = 1.0f / (2.0f * PI * (1.0f - tcos(l->angular_size)));
         }
     }
     else if(l->type == L_SPHERE)
@@ -895,7 +949,7 @@ static lsample_t sample_light(const light_t *l, v3 P, float xi_x, float xi_y)
         {
             float s2 = r2 / d2;
             float cs = sqrtf(1.0f - s2);
-            r.dir = to_world(local_frame(scl(tc, 1.0f / sqrtf(d2))), sphere_cap(xi_x, xi_y, asinf(sqrtf(s2))));
+            r.dir = to_world(local_frame(scl(tc, 1.0f / sqrtf(d2))), sphere_cap(xi_x, xi_y, tasin(sqrtf(s2))));
             r.pdf = (1.0f + cs) / (2.0f * PI * s2);
             float b = dot(r.dir, tc);
             r.dist = b - sqrtf(fmaxf(b * b - d2 + r2, 0.0f));
@@ -915,7 +969,7 @@ static lsample_t sample_light(const light_t *l, v3 P, float xi_x, float xi_y)
         else if(l->type == L_DISK)
         {
             float rr = sqrtf(xi_x), th = 2.0f * PI * xi_y;
-            float dx = rr * cosf(th), dy = rr * sinf(th);
+            float dx = rr * tcos(th), dy = rr * tsin(th);
             Q = add(l->position, to_world(local_frame(l->direction), V3(l->size_x * dx, l->size_x * dy, 0.0f)));
             n = l->direction;
             r.pdf = 1.0f / (PI * l->size_x * l->size_x);
@@ -923,7 +977,7 @@ static lsample_t sample_light(const light_t *l, v3 P, float xi_x, float xi_y)
         else
         {
             float phi = 2.0f * PI * xi_x;
-            n = to_world(local_frame(l->direction), V3(cosf(phi), sinf(phi), 0.0f));
+            n = to_world(local_frame(l->direction), V3(tcos(phi), tsin(phi), 0.0f));
             Q = add(add(l->position, scl(l->direction, (2.0f * xi_y - 1.0f) * l->size_y)), scl(n, l->size_x));
             r.pdf = 1.0f / (4.0f * PI * l->size_x * l->size_y);
         }
@@ -1040,7 +1094,7 @@ static float light_pdf(const light_t *l, v3 P, v3 d)
     if(l->type == L_DIRECTIONAL)
     {
         if(l->angular_size <= 0.0f) return 0.0f;
-        float cs = cosf(l->angular_size);
+        float cs = tcos(l->angular_size);
         return dot(d, neg(l->direction)) < cs ? 0.0f : 1.0f / (2.0f * PI * (1.0f - cs));
     }
     if(l->type == L_SPHERE)
@@ -1089,7 +1143,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
     const uint32_t W = pp->width, H = pp->height;
     uint32_t rng = orc_xxhash32(pp->seed, W * y + x);
     /* the ray cone's spread angle (raypipeline.slang:102), a half like the payload's packed cone */
-    const float cone_spread = orc_half_value(atanf(2.0f * tanf(pp->fov_y / 2.0f) / (float)H));
+    const float cone_spread = orc_half_value(tatan(2.0f * ttan(pp->fov_y / 2.0f) / (float)H));
     v3 result = V3(0, 0, 0);
     float coverage = 0.f;
     float offset = 2.0f * rnd(&rng) - 1.0f;
@@ -1112,7 +1166,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
         float u = cx / (float)W, v = cy / (float)H;
         float dx = u * 2.0f - 1.0f, dy = v * 2.0f - 1.0f;
         float r = sqrtf(xi_x), th = 2.0f * PI * xi_y;
-        float lx = pp->aperture / 2.0f * (r * cosf(th)), ly = pp->aperture / 2.0f * (r * sinf(th));
+        float lx = pp->aperture / 2.0f * (r * tcos(th)), ly = pp->aperture / 2.0f * (r * tsin(th));
         v3 org = add(add(eye, scl(right, lx)), scl(up, ly));
         v3 tg = nrmz(V3(dx * pp->tan_half_x, -dy * pp->tan_half_y, -1.0f));
         v3 fp = scl(tg, pp->focal_distance / fabsf(tg.z));
@@ -1194,15 +1248,15 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                     v3 pc = cross(sub(v1, v0), sub(v2, v0));
                     float p_a = sqrtf(dot(pc, pc));
                     float t_a = fabsf((T1[0] - T0[0]) * (T2[1] - T0[1]) - (T2[0] - T0[0]) * (T1[1] - T0[1]));
-                    const float lod_cone = 0.5f * log2f(t_a / p_a) + log2f(fabsf(cone_width));
-                    const float lod_nov = log2f(fabsf(dot(Vw, nrm)));
+                    const float lod_cone = 0.5f * tlog2(t_a / p_a) + tlog2(fabsf(cone_width));
+                    const float lod_nov = tlog2(fabsf(dot(Vw, nrm)));
                     float c[4];
 #define ORC_LOOKUP(slot)                                                                                   \
     do                                                                                                     \
     {                                                                                                      \
         const otex_t *tx = &j->tex[(slot) - 1];                                                           \
         float lambda = lod_cone;                                                                           \
-        lambda += 0.5f * log2f((float)tx->w * (float)tx->h);                                              \
+        lambda += 0.5f * tlog2((float)tx->w * (float)tx->h);                                              \
         lambda -= lod_nov;                                                                                 \
         otex_sample(tx, tu, tv, lambda, c);                                                                \
     } while(0)
@@ -1324,7 +1378,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                     beta = mul(beta, thr);
                     if(b.transmission) tmax = media_op == 1 ? j->obj_tmax[obj] : DEFAULT_TMAX;
                     media_op = b.transmission ? media_op : 0;
-                    if(max3(beta) <= 0.05f * (float)i)
+                    if(!(pp->test_flags & ORC_PT_NO_ROULETTE) && max3(beta) <= 0.05f * (float)i)
                     {
                         float q = 1.0f - max3(beta);
                         if(rnd(&lr) < q) stop = 1;
@@ -1395,8 +1449,38 @@ static void *render_rows(void *arg)
     return NULL;
 }
 
+void orc_sample_disney_n(const float *mat36, const float *N, const float *V, float eta, uint32_t seed, uint32_t n,
+                         float *out8)
+{
+    const mat_t m = unpack_material(mat36, 0);
+    const v3 nn = V3(N[0], N[1], N[2]), vv = V3(V[0], V[1], V[2]);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        uint32_t rng = orc_xxhash32(seed, i);
+        const bs_t b = sample_disney(m, nn, vv, eta, &rng);
+        float *o = out8 + 8 * (size_t)i;
+        o[0] = b.dir.x, o[1] = b.dir.y, o[2] = b.dir.z, o[3] = b.pdf;
+        o[4] = b.F.x, o[5] = b.F.y, o[6] = b.F.z, o[7] = (float)b.transmission;
+    }
+}
+
+void orc_eval_disney_n(const float *mat36, const float *N, const float *V, float eta, const float *L, uint32_t n,
+                       float *out4)
+{
+    const mat_t m = unpack_material(mat36, 0);
+    const v3 nn = V3(N[0], N[1], N[2]), vv = V3(V[0], V[1], V[2]);
+    for(uint32_t i = 0; i < n; ++i)
+    {
+        float pdf = 0.f;
+        const v3 F = eval_disney(m, V3(L[3 * i], L[3 * i + 1], L[3 * i + 2]), nn, vv, eta, &pdf);
+        float *o = out4 + 4 * (size_t)i;
+        o[0] = F.x, o[1] = F.y, o[2] = F.z, o[3] = pdf;
+    }
+}
+
 void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads)
 {
+    g_ulp_bias = (pp->test_flags & ORC_PT_ULP_BIAS) != 0;
     const uint32_t n = sc->n_tri;
     cbvh b;
     memset(&b, 0, sizeof(b));
@@ -1404,41 +1488,7 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     uint8_t *two = (uint8_t *)calloc(n ? n : 1, 1);
     double *cent = (double *)malloc(sizeof(double) * 3 * (size_t)(n ? n : 1));
     mat_t *mats = (mat_t *)calloc(sc->n_materials ? sc->n_materials : 1, sizeof(mat_t));
-    for(uint32_t m = 0; m < sc->n_materials; ++m)
-    {
-        const float *x = sc->materials + 36 * (size_t)m; /* vkrt_material layout, 36 words */
-        mats[m].color = V3(x[0], x[1], x[2]);
-        mats[m].emission = V3(x[4], x[5], x[6]);
-        mats[m].emission_strength = x[7];
-        mats[m].metal = x[8];
-        mats[m].rough = x[9];
-        mats[m].trans = x[10];
-        mats[m].ior = x[11];
-        mats[m].cc = x[12];
-        mats[m].cc_rough = x[13];
-        mats[m].sheen_rough = x[14];
-        mats[m].dt = x[15];
-        mats[m].sheen = V3(x[16], x[17], x[18]);
-        mats[m].dt_color = V3(x[20], x[21], x[22]);
-        uint32_t ts, tex, mode;
-        memcpy(&ts, &x[24], 4);
-        mats[m].two_sided = ts != 0;
-        mats[m].alpha = x[3];
-        memcpy(&tex, &x[25], 4);
-        memcpy(&mode, &x[26], 4);
-        mats[m].tex = tex <= sc->n_textures ? tex : 0;
-        mats[m].alpha_mode = mode <= 2 ? (int)mode : 0;
-        mats[m].alpha_cutoff = x[27];
-        uint32_t slot[6];
-        memcpy(slot, &x[28], sizeof(slot));
-        for(int k = 0; k < 6; ++k) slot[k] = slot[k] <= sc->n_textures ? slot[k] : 0;
-        mats[m].tex_normal = slot[0];
-        mats[m].tex_emission = slot[1];
-        mats[m].tex_arm = slot[2];
-        mats[m].tex_transmission = slot[3];
-        mats[m].tex_dt = slot[4];
-        mats[m].tex_dtc = slot[5];
-    }
+    for(uint32_t m = 0; m < sc->n_materials; ++m) mats[m] = unpack_material(sc->materials + 36 * (size_t)m, sc->n_textures);
     for(uint32_t t = 0; t < n; ++t)
     {
         for(int k = 0; k < 3; ++k)
@@ -1530,6 +1580,7 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     free(cent);
     free(two);
     free(tris);
+    g_ulp_bias = 0;
 }
 
 /* ------------------------------------------------------------------------------------------------
@@ -1566,7 +1617,19 @@ static void dn_albedo(uint32_t c, float a[3])
     a[2] = (float)((c >> 16) & 0xffu) / 255.0f;
 }
 
+static void denoise_passes(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, int passes, float *out4);
+
 void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4)
+{
+    denoise_passes(px8, w, h, denoise, 5, out4);
+}
+
+void orc_denoise_passes(const uint32_t *px8, uint32_t w, uint32_t h, int passes, float *out4)
+{
+    denoise_passes(px8, w, h, 1, passes < 1 ? 1 : (passes > 5 ? 5 : passes), out4);
+}
+
+static void denoise_passes(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, int passes, float *out4)
 {
     const size_t n = (size_t)w * h;
     if(!denoise)
@@ -1585,7 +1648,7 @@ void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float
     float *img[2];
     img[0] = (float *)malloc(n * 16);
     img[1] = (float *)malloc(n * 16);
-    for(int pass = 0; pass < 5; ++pass)
+    for(int pass = 0; pass < passes; ++pass)
     {
         const float *prev = pass ? img[(pass - 1) % 2] : NULL;
         float *dst = img[pass % 2];
@@ -1643,7 +1706,7 @@ void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float
                 dst[4 * c + 3] = 0.f;
             }
     }
-    memcpy(out4, img[0], n * 16);/* five passes end in image 0 */
+    memcpy(out4, img[(passes - 1) % 2], n * 16);/* pass k writes image k % 2 */
     free(img[0]);
     free(img[1]);
 }

@@ -48,12 +48,30 @@ typedef struct orc_pt_params
     const float *lights; /* light_t records, 20 words each, already gathered (positive intensity, no Area) */
     uint32_t n_lights, mis_mode, suppress_refractive_caustics;
     float fov_y; /* vertical field of view: the ray cone's spread angle */
+    /* test hooks, not reference behaviour: ORC_PT_NO_ROULETTE turns Russian roulette off,
+     * ORC_PT_ULP_BIAS moves every transcendental result of the integrator one float up */
+    uint32_t test_flags;
 } orc_pt_params;
+
+enum
+{
+    ORC_PT_NO_ROULETTE = 1,
+    ORC_PT_ULP_BIAS = 2
+};
 
 /* One batch: out8 = 8 floats per pixel (radiance rgb / num_samples, coverage, linear depth, -, object id
  * bits, -), row-major from the top-left. */
 void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads);
 void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4);
+/* test hook: only the first `passes` (1..5) of the five a-trous passes (steps 1, 2, 4, 8, 16) */
+void orc_denoise_passes(const uint32_t *px8, uint32_t w, uint32_t h, int passes, float *out4);
+/* Test hooks: sample_disney (bsdf_disney.slang:162-296) for n draws on one material record (36 words) at
+ * normal N, view V, eta; draw i starts from the RNG state xxhash32(seed, i). out8 per draw: direction xyz,
+ * pdf, F rgb, transmission flag. eval_disney (:298-381) for n directions L: out4 = F rgb, pdf. */
+void orc_sample_disney_n(const float *mat36, const float *N, const float *V, float eta, uint32_t seed, uint32_t n,
+                         float *out8);
+void orc_eval_disney_n(const float *mat36, const float *N, const float *V, float eta, const float *L, uint32_t n,
+                       float *out4);
 /* binary16 rounding (nearest, ties to even) and the exposure/gamma composite of the output image */
 float orc_half_value(float x);
 void orc_tonemap(const float *in4, size_t n, float exposure, float gamma, float *out4);
