@@ -46,25 +46,101 @@ def log(msg):
 
 def cpu_baseline(tris, rays, budget_s):
     """the oracle's brute-force closest hit (restatement of src/intersection.cpp:114-129 in a loop), timed
-    on this host on a bounded random subset of the primary + bounce rays, extrapolated to rays/s"""
+    on this host on a bounded random subset of the primary + bounce rays, extrapolated to rays/s: on every
+    thread this process may use (std::thread::hardware_concurrency of the reference's harness) and on one
+    thread, one warm-up and the median of 5 runs each"""
     import numpy as np
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_binding as orc
-    nthreads = min(16, os.cpu_count() or 1)
+    orc = oracle()
+    nthreads, model = host_threads()
     rng = np.random.default_rng(0)
     probe = rays[rng.choice(len(rays), 256, replace=False)]
     t0 = time.perf_counter()
     orc.brute_closest(tris, probe, nthreads)
     dt = time.perf_counter() - t0
-    n = int(min(16384, max(256, 256 * budget_s / max(dt, 1e-6))))
+    # about budget_s / 6 seconds per multi-threaded run (1 warm-up + 5 timed)
+    n = int(min(32768, max(256, 256 * budget_s / 6.0 / max(dt, 1e-6))))
     sample = rays[np.sort(rng.choice(len(rays), n, replace=False))]
-    t0 = time.perf_counter()
-    orc.brute_closest(tris, sample, nthreads)
-    dt = time.perf_counter() - t0
+    times = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        orc.brute_closest(tris, sample, nthreads)
+        if k:
+            times.append(time.perf_counter() - t0)
+    dt = float(np.median(times))
+    one = sample[: max(16, n // (4 * nthreads))]
+    times1 = []
+    for k in range(3):
+        t0 = time.perf_counter()
+        orc.brute_closest(tris, one, 1)
+        if k:
+            times1.append(time.perf_counter() - t0)
+    dt1 = float(np.median(times1))
     return {"value": n / dt / 1e9, "unit": "Grays/s", "cores": nthreads, "kind": "port",
+            "cpu_model": model, "one_thread_grays_per_s": len(one) / dt1 / 1e9,
             "sample": f"{n} random closest-hit rays of the workload, brute force over all {len(tris)} triangles "
-                      f"({n * len(tris):.3e} ray-triangle tests in {dt:.1f} s, {nthreads} threads), "
-                      "extrapolated to rays/s"}
+                      f"({n * len(tris):.3e} ray-triangle tests, median {dt:.2f} s of 5 runs on {nthreads} threads; "
+                      f"{len(one)} rays on 1 thread, median {dt1:.2f} s of 2), extrapolated to rays/s"}
+
+
+def brute_force_leg(ctx, reps=10):
+    """the brute-force closest hit of 65,536 rays (origins U[-3,3]^3, directions uniform on S^2, seed 0)
+    against the 20,480-triangle jittered icosphere (subdivision 5, radius 1, vertices N(0, 0.01)): one
+    ray-triangle test per pair, 1,342,177,280 per run, 52 FP32 operations each on the full path of
+    src/intersection.cpp:114-129 (no FMA). Timed on the GPU (HBM-resident arrays, hipEvents around `reps`
+    runs) and, as the CPU baseline of this workload, the oracle's full run on every host thread (1 warm-up,
+    median of 5) and a 4,096-ray sample on one thread. The GPU records are checked bit for bit against the
+    oracle's full run."""
+    import numpy as np
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+    orc = oracle()
+    pos, idx, rays = W.icosphere_rays()
+    tris = W.flat_triangles(pos, idx)
+    n_tests = len(rays) * len(tris)
+    d_tris, d_rays = vk.DeviceArray.from_host(ctx, tris), vk.DeviceArray.from_host(ctx, rays)
+    d_hits = vk.DeviceArray(ctx, len(rays), vk.HIT_DTYPE)
+    ctx.brute_closest(d_tris, d_rays, out=d_hits)
+    ev = vk.HipEvents(2)
+    ev.record(0, ctx.stream)
+    for _ in range(reps):
+        ctx.brute_closest(d_tris, d_rays, out=d_hits, flags=vk.ASYNC)
+    ev.record(1, ctx.stream)
+    ev.sync(1)
+    ms = ev.ms(0, 1) / reps
+    got = d_hits.host()
+    for a in (d_tris, d_rays, d_hits):
+        a.free()
+    nthreads, model = host_threads()
+    cpu = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        want = orc.brute_closest(tris, rays, nthreads)
+        if k:
+            cpu.append(time.perf_counter() - t0)
+    cpu_s = float(np.median(cpu))
+    one = rays[:4096]
+    t0 = time.perf_counter()
+    orc.brute_closest(tris, one, 1)
+    one_s = time.perf_counter() - t0
+    mismatch = int(np.sum((got["prim"] != want["prim"]) | (got["t"].view(np.uint32) != want["t"].view(np.uint32)) |
+                          (got["u"].view(np.uint32) != want["u"].view(np.uint32)) |
+                          (got["v"].view(np.uint32) != want["v"].view(np.uint32))))
+    flops = 52.0 * n_tests
+    tflops = flops / (ms * 1e-3) / 1e12
+    return {"workload": "brute-force closest hit, 65,536 rays x 20,480-triangle jittered icosphere (subdiv 5, "
+                        "r 1, N(0, 0.01)), seed 0", "tests_per_run": n_tests,
+            "gpu_ms": round(ms, 4), "gpu_tests_per_s": round(n_tests / (ms * 1e-3), 1),
+            "gpu_rays_per_s": round(len(rays) / (ms * 1e-3), 1),
+            "roofline": {"bound": "fp32 valu", "kernel": "k_brute_closest (whole call: prep + fill + test + finalize)",
+                         "achieved": round(tflops, 3), "peak": 157.3, "unit": "TFLOP/s",
+                         "frac": round(tflops / 157.3, 5), "flop_per_test": 52, "flop_per_run": flops},
+            "verified_vs_oracle_full_run": {"rays": len(rays), "mismatches": mismatch,
+                                            "hits": int(np.sum(want["prim"] != 0xFFFFFFFF))},
+            "cpu_full_run": {"threads": nthreads, "cpu_model": model, "median_s": round(cpu_s, 3),
+                             "tests_per_s": round(n_tests / cpu_s, 1), "rays_per_s": round(len(rays) / cpu_s, 1),
+                             "one_thread_tests_per_s": round(len(one) * len(tris) / one_s, 1),
+                             "kind": "port"},
+            "gpu_over_cpu": round(cpu_s / (ms * 1e-3), 1)}
 
 
 def path_tracer_frame(ctx, spp, lit=False):
@@ -93,15 +169,45 @@ def path_tracer_frame(ctx, spp, lit=False):
     st = pt.stats()
     px = pt.pixels()
     seg_bytes = 332      # per path segment: SoA ray, hit and path state in/out, 3 indices + 3 vertices, material
+    gbps = seg_bytes * st.segments / (st.raytrace_ms * 1e-3) / 1e9
     out = {"workload": "1920x1080, max_trace_depth 8, %d spp, 276,504 triangles%s"
                        % (spp, ", 6 analytic lights (NEE + MIS)" if lit else ""),
            "frame_ms": round(st.raytrace_ms, 3), "samples": int(st.samples), "segments": int(st.segments),
            "samples_per_s": round(st.samples / (st.raytrace_ms * 1e-3), 1),
            "segments_per_s": round(st.segments / (st.raytrace_ms * 1e-3), 1),
-           "segment_gbps": round(seg_bytes * st.segments / (st.raytrace_ms * 1e-3) / 1e9, 2),
+           "segment_gbps": round(gbps, 2),
+           "roofline": {"bound": "hbm", "achieved": round(gbps, 2), "peak": MI355X_HBM_GBPS, "unit": "GB/s",
+                        "frac": round(gbps / MI355X_HBM_GBPS, 5), "bytes_per_segment": seg_bytes},
            "mean_radiance": float(px["radiance"].mean())}
     pt.close()
+    if not lit:
+        out["vs_cpu_integrator"] = pt_vs_integrator(ctx, scene, cam)
     return out
+
+
+def pt_vs_integrator(ctx, scene, cam, w=480, h=270, spp=16):
+    """the GPU tracer and the CPU reference integrator (oracle, same RNG streams and arithmetic) on the same
+    scene, camera and seed at equal samples per pixel, at a reduced size: RMSE relative to the mean and the
+    share of pixels within 1e-4 (the integrator's one-ulp sensitivity bounds what two math libraries give)"""
+    import numpy as np
+    import vierkant_amd as vk
+    from vierkant_amd import workloads as W
+    orc = oracle()
+    s = vk.default_settings()
+    s.resolution[0], s.resolution[1] = w, h
+    s.num_samples, s.max_trace_depth = spp, 8
+    pt = vk.PathTracer(ctx, s, seed=0)
+    pt.set_scene(scene)
+    pt.render(cam)
+    g = pt.pixels()["radiance"].astype(np.float64)
+    pt.close()
+    t0 = time.perf_counter()
+    c = orc.pt_render(scene, cam, w, h, spp, 8, W.minstd_rand0(0, 1)[0])[..., :3].astype(np.float64)
+    cpu_s = time.perf_counter() - t0
+    close = float(np.mean(np.abs(g - c).max(axis=2) <= 1e-4 * np.maximum(1.0, c.max(axis=2))))
+    return {"size": [w, h], "spp": spp, "rmse": float(np.sqrt(np.mean((g - c) ** 2))),
+            "rmse_rel_mean": float(np.sqrt(np.mean((g - c) ** 2)) / c.mean()), "share_within_1e-4": round(close, 5),
+            "cpu_integrator_s": round(cpu_s, 2), "cpu_threads": host_threads()[0]}
 
 
 def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64, max_train_records=16384):
@@ -124,12 +230,17 @@ def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64,
         st.num_samples, st.max_trace_depth, st.max_num_batches = spp, 8, batches
         return st
 
+    # the reference: ref_spp / 64 batches of 64 spp (max_num_batches 0: each render replaces the image with
+    # a fresh batch under the next per-frame seed), accumulated in float64 on the host
     t0 = time.perf_counter()
-    ref_pt = vk.PathTracer(ctx, settings(64, max(1, ref_spp // 64)), seed=1000)
+    ref_pt = vk.PathTracer(ctx, settings(64, 0), seed=1000)
     ref_pt.set_scene(scene)
-    for _ in range(max(1, ref_spp // 64)):
+    n_ref = max(1, ref_spp // 64)
+    ref = np.zeros((1080, 1920, 3), np.float64)
+    for _ in range(n_ref):
         ref_pt.render(cam)
-    ref = ref_pt.pixels()["radiance"].astype(np.float64)
+        ref += ref_pt.pixels()["radiance"]
+    ref /= n_ref
     ref_pt.close()
     ref_s = time.perf_counter() - t0
 
@@ -141,7 +252,9 @@ def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64,
     # max_num_batches 0 keeps the batch index at 0 (min(index + 1, 0), PBRPathTracer.cpp:211), so every
     # render replaces the image: the first warms the kernels, the second is the timed 64-spp image
     plain.render(cam)
+    t0 = time.perf_counter()
     plain.render(cam)
+    plain_wall_ms = (time.perf_counter() - t0) * 1e3
     assert plain.current_batch() == 0
     target = rmse(plain.pixels()["radiance"])
     plain_ms = float(plain.stats().raytrace_ms)
@@ -162,28 +275,92 @@ def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64,
     cache = vk.RadianceCache(ctx, seed=1)
     pt.set_radiance_cache(cache, cache_depth=cache_depth, train_stride=train_stride,
                           max_train_records=max_train_records)
-    t_ms, hit, curve, frame_ms = 0.0, None, [], []
+    t_ms, wall_ms, hit, curve, frame_ms = 0.0, 0.0, None, [], []
     for k in range(1, max_frames + 1):
-        pt.render(cam)
+        t0 = time.perf_counter()
+        pt.render(cam)  # synchronous: the host's wall time of the whole frame (launches, queries, training)
+        wall_ms += (time.perf_counter() - t0) * 1e3
         frame_ms.append(float(pt.stats().raytrace_ms))
         t_ms += frame_ms[-1]
         r = rmse(pt.pixels()["radiance"])
         if k in (1, 2, 4, 8, 16, 32, 64, 128) or (hit is None and r <= target):
-            curve.append([k, round(t_ms, 2), round(r, 5)])
+            curve.append([k, round(t_ms, 2), round(wall_ms, 2), round(r, 5)])
         if hit is None and r <= target:
-            hit = (k, t_ms)
+            hit = (k, t_ms, wall_ms)
             break
     pt.close()
+    kernels = cache_kernel_rooflines(ctx, cache)
     cache.close()
     return {"workload": "room scene (276,504 triangles), 1920x1080, max_trace_depth 8; target = RMSE of plain "
                         f"path tracing at 64 spp against a {ref_spp}-spp plain reference",
             "target_rmse": round(target, 5), "plain_64spp_ms": round(plain_ms, 3),
             "cache_depth": cache_depth, "train_stride": train_stride, "max_train_records": max_train_records,
+            "reference": f"{n_ref} batches of 64 spp accumulated in float64",
             "cache_frames_to_target": None if hit is None else hit[0],
             "cache_ms_to_target": None if hit is None else round(hit[1], 3),
             "speedup": None if hit is None else round(plain_ms / hit[1], 3),
+            "plain_64spp_wall_ms": round(plain_wall_ms, 3),
+            "cache_wall_ms_to_target": None if hit is None else round(hit[2], 3),
+            "speedup_wall": None if hit is None else round(plain_wall_ms / hit[2], 3),
             "cache_frame_ms_mean": round(float(np.mean(frame_ms)), 3),
-            "rmse_curve": curve, "reference_s": round(ref_s, 2)}
+            "rmse_curve_frames_gpu_ms_wall_ms_rmse": curve, "reference_s": round(ref_s, 2),
+            "kernels": kernels}
+
+
+def cache_kernel_rooflines(ctx, cache, n_query=1 << 20, n_train=1 << 16, reps=10):
+    """k_rc_query and one training step (forward, backward, gradient scatter and Adam) timed alone with
+    hipEvents on HBM-resident records, against the dense BF16 matrix-core peak (2.5 PFLOP/s): 43,008 FLOP
+    per query (64-wide input, five 64x64 layers, 64 -> 16 output) and 3 x that per training record (forward
+    and two backward products), 512 B of table reads per query"""
+    import numpy as np
+    import vierkant_amd as vk
+    rng = np.random.default_rng(9)
+    def records(n):
+        # pos.xyz, roughness | wo.xyz, metalness | normal.xyz, transmission | albedo.rgb, - (the record layout)
+        r = np.zeros((n, 16), np.float32)
+        r[:, 0:3] = rng.uniform(0.0, 1.0, (n, 3))
+        r[:, 3] = rng.uniform(0.05, 1.0, n)
+        for k in (4, 8):
+            d = rng.normal(size=(n, 3))
+            r[:, k:k + 3] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        r[:, 7] = rng.uniform(0.0, 1.0, n) < 0.3
+        r[:, 12:15] = rng.uniform(0.0, 1.0, (n, 3))
+        return r
+    q = vk.DeviceArray.from_host(ctx, records(n_query))
+    qo = vk.DeviceArray(ctx, (n_query, 4), np.float32)
+    t = vk.DeviceArray.from_host(ctx, records(n_train))
+    tg = vk.DeviceArray.from_host(ctx, rng.uniform(0.0, 2.0, (n_train, 4)).astype(np.float32))
+    ev = vk.HipEvents(2)
+    lib, loss = vk.lib(), None
+    import ctypes
+    loss = ctypes.c_float()
+    flags = vk.DEVICE_PTRS | vk.ASYNC
+    ctx.check(lib.vkrt_rc_query(cache.handle, q.ptr, n_query, qo.ptr, flags))
+    ev.record(0, ctx.stream)
+    for _ in range(reps):
+        ctx.check(lib.vkrt_rc_query(cache.handle, q.ptr, n_query, qo.ptr, flags))
+    ev.record(1, ctx.stream)
+    ev.sync(1)
+    q_ms = ev.ms(0, 1) / reps
+    ctx.check(lib.vkrt_rc_train(cache.handle, t.ptr, tg.ptr, n_train, flags, ctypes.byref(loss)))
+    ctx.sync()
+    ev.record(0, ctx.stream)
+    for _ in range(reps):
+        ctx.check(lib.vkrt_rc_train(cache.handle, t.ptr, tg.ptr, n_train, flags, ctypes.byref(loss)))
+    ev.record(1, ctx.stream)
+    ev.sync(1)
+    t_ms = ev.ms(0, 1) / reps
+    for a in (q, qo, t, tg):
+        a.free()
+    peak = 2500.0
+    q_tf = 43008.0 * n_query / (q_ms * 1e-3) / 1e12
+    t_tf = 3 * 43008.0 * n_train / (t_ms * 1e-3) / 1e12
+    return {"query": {"records": n_query, "ms": round(q_ms, 4), "tflops": round(q_tf, 2),
+                      "mfma_bf16_frac": round(q_tf / peak, 5),
+                      "table_read_gbps": round(512.0 * n_query / (q_ms * 1e-3) / 1e9, 1)},
+            "train_step": {"records": n_train, "ms": round(t_ms, 4), "tflops": round(t_tf, 2),
+                           "mfma_bf16_frac": round(t_tf / peak, 5)},
+            "peak_tflops": peak}
 
 
 def instanced_frame(ctx, primary, bounce, verify_rays, reps=5):
@@ -239,30 +416,81 @@ def instanced_frame(ctx, primary, bounce, verify_rays, reps=5):
             {"rays": 2 * verify_rays, "mismatches": mismatches}}
 
 
-def batched_frame(ctx, n_scenes, verify_rays):
+def oracle():
+    """the CPU oracle binding (test infrastructure: used here only to check results and for the CPU baseline)"""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as orc
+    return orc
+
+
+def host_threads():
+    """threads this process may run on (the CPU share of the box), and the machine's model name"""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, model
+
+
+def record(h):
+    return {"t": float(h["t"]), "u": float(h["u"]), "v": float(h["v"]), "prim": int(h["prim"]),
+            "t_bits": int(h["t"].view("u4"))}
+
+
+def compare_with_oracle(got, want, safe, rays, limit, **where):
+    """mismatching rays (prim or t bits differ) as diagnostic dicts, at most `limit`; plus the count"""
+    import numpy as np
+    bad = np.nonzero((got["prim"] != want["prim"]) | (got["t"].view(np.uint32) != want["t"].view(np.uint32)))[0]
+    diags = []
+    for i in bad[:limit]:
+        r = rays[i]
+        diags.append(dict(where, sample=int(i), ray=[float(x) for x in (r["ox"], r["oy"], r["oz"], r["tmin"],
+                                                                        r["dx"], r["dy"], r["dz"], r["tmax"])],
+                          got=record(got[i]), oracle=record(want[i]), winner_cull_safe=int(safe[i])))
+    return len(bad), diags
+
+
+def batched_frame(ctx, n_scenes, verify_per_scene):
     """256 independent scenes (log-uniform 1k-500k triangles, 16k-2M rays each) traced by one persistent
     launch replayed as a hipGraph, after the launch-shape autotuner picked block size, LDS stack depth,
-    tile and blocks per CU"""
+    tile, blocks per CU and XCD chunk. `verify_per_scene` random rays of every scene are checked against the
+    oracle's brute force (bit-exact t and prim); every mismatch is written out with the scene, the ray, both
+    records, the launch shape and the host-side invariant check of that scene's BVH."""
     import numpy as np
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
+    orc = oracle()
     bvhs, rays, hits, host = [], [], [], []
     n_tri = 0
     t_build = 0.0
+    bvh_bad = []
     for pos, idx, r in W.many_scenes(n_scenes=n_scenes):
         t0 = time.perf_counter()
         bvhs.append(ctx.bvh_build(pos, idx))
         t_build += time.perf_counter() - t0
+        check = bvhs[-1].validate(pos)  # host-side invariants of every BVH (outside the timed launches)
+        if not check["ok"]:
+            bvh_bad.append(dict(check, scene=len(host)))
         rays.append(vk.DeviceArray.from_host(ctx, r))
         hits.append(vk.DeviceArray(ctx, len(r), vk.HIT_DTYPE))
         n_tri += len(idx) // 3
-        sub = np.random.default_rng(len(host)).choice(len(r), min(verify_rays, len(r)), replace=False)
-        host.append((W.flat_triangles(pos, idx), r[np.sort(sub)], np.sort(sub)))
+        sub = np.sort(np.random.default_rng(len(host)).choice(len(r), min(verify_per_scene, len(r)), replace=False))
+        host.append((pos, W.flat_triangles(pos, idx), r[sub], sub))
     batch = vk.Batch(ctx, list(zip(bvhs, rays, hits)))
     total = batch.total_rays()
     default_cfg = batch.config
     batch.launch()
     ev = vk.HipEvents(2)
+
     def timed(k):
         ev.record(0, ctx.stream)
         for _ in range(k):
@@ -271,25 +499,47 @@ def batched_frame(ctx, n_scenes, verify_rays):
         ev.sync(1)
         return ev.ms(0, 1) / k
     default_ms = timed(3)
+    t_tune = time.perf_counter()
     best, tune_ms = batch.autotune(repeats=2)
+    tune_s = time.perf_counter() - t_tune
     tuned_ms = timed(5)
-    mismatches = 0
-    for (flat, sub_rays, sub), h in zip(host, hits):
+    # verification against the CPU oracle (outside the timed launches)
+    n_checked = n_bad = n_safe = 0
+    diags = []
+    t_orc = time.perf_counter()
+    for s, ((pos, flat, sub_rays, sub), h) in enumerate(zip(host, hits)):
         got = h.host()[sub]
-        ref = ctx.brute_closest(flat, sub_rays)
-        mismatches += int(np.sum((got["prim"] != ref["prim"]) | (got["t"].view(np.uint32) != ref["t"].view(np.uint32))))
+        want, safe = orc.brute_closest_cull_safe(flat, sub_rays)
+        n_checked += len(sub)
+        n_safe += int(safe.sum())
+        nb, d = compare_with_oracle(got, want, safe, sub_rays, 8 - len(diags), scene=s, shape=list(best))
+        if nb:
+            check = bvhs[s].validate(pos)
+            for x in d:
+                x["ray_index"] = int(sub[x["sample"]])
+                x["bvh_check"] = check
+            diags += d
+        n_bad += nb
+    orc_s = time.perf_counter() - t_orc
     for a in rays + hits:
         a.free()
     batch.close()
     for b in bvhs:
         b.close()
+    achieved = CLOSEST_BYTES_PER_RAY * total / (tuned_ms * 1e-3) / 1e9
     return {"workload": f"{n_scenes} scenes, {n_tri} triangles, {total} closest-hit rays in one graph launch",
             "grays_per_s": round(total / (tuned_ms * 1e-3) / 1e9, 4), "launch_ms": round(tuned_ms, 3),
             "default_config": list(default_cfg), "default_launch_ms": round(default_ms, 3),
             "autotuned_config": {"block_size": best[0], "lds_stack_depth": best[1], "tile": best[2],
-                                 "blocks_per_cu": best[3]},
+                                 "blocks_per_cu": best[3], "xcd_chunk": best[4]},
+            "autotune_s": round(tune_s, 2), "candidate_shapes": len(vk.Batch.candidates()),
             "bvh_build_s": round(t_build, 3),
-            "verified_vs_brute_force": {"rays": int(sum(len(x[2]) for x in host)), "mismatches": mismatches}}
+            "bvh_invariants": {"checked": n_scenes, "failed": len(bvh_bad), "first_failures": bvh_bad[:4]},
+            "roofline": {"bound": "hbm", "kernel": "k_batch_trace", "achieved": round(achieved, 2),
+                         "peak": MI355X_HBM_GBPS, "unit": "GB/s", "frac": round(achieved / MI355X_HBM_GBPS, 5),
+                         "algorithmic_bytes_per_launch": CLOSEST_BYTES_PER_RAY * int(total)},
+            "verified_vs_oracle": {"rays": n_checked, "mismatches": n_bad, "winner_cull_safe": n_safe,
+                                   "oracle_s": round(orc_s, 1), "first_mismatches": diags}}
 
 
 def main():
@@ -299,7 +549,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--verify-rays", type=int, default=65536)
+    ap.add_argument("--verify-rays", type=int, default=65536,
+                    help="primary and bounce rays each checked against GPU brute force")
+    ap.add_argument("--oracle-rays", type=int, default=256,
+                    help="of those, primary and bounce rays each checked against the CPU oracle")
     ap.add_argument("--sweep", action="store_true", help="time every trace launch shape and exit")
     ap.add_argument("--traversal", choices=("default", "step", "postponed"), default="default",
                     help="loop of the 4-wide walk (default: the library's)")
@@ -311,6 +564,10 @@ def main():
     ap.add_argument("--pt-lit-spp", type=int, default=16,
                     help="samples per pixel of the lit path-tracer frame (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
+    ap.add_argument("--brute-force", type=int, default=1,
+                    help="time the brute-force leg (65,536 rays x 20,480 triangles) and its full CPU run (0: skip)")
+    ap.add_argument("--batch-verify", type=int, default=256,
+                    help="random rays per scene of the batched launch checked against the CPU oracle")
     ap.add_argument("--instanced", type=int, default=1, help="trace the two-level version of the scene (0: skip)")
     ap.add_argument("--rc-ref-spp", type=int, default=4096,
                     help="samples per pixel of the radiance-cache leg's reference image (0: skip the leg)")
@@ -366,7 +623,20 @@ def main():
             got = hits_np[sel]
             mismatch += int(np.sum((ref["prim"] != got["prim"]) |
                                    (ref["t"].view(np.uint32) != got["t"].view(np.uint32))))
-        verified = {"rays": 2 * args.verify_rays, "mismatches": mismatch}
+        verified = {"vs_gpu_brute_force": {"rays": 2 * args.verify_rays, "mismatches": mismatch}}
+        # a smaller sample against the CPU oracle itself, with the culling criterion of every winner
+        orc = oracle()
+        osel = sel[:: max(1, len(sel) // args.oracle_rays)][: args.oracle_rays]
+        n_bad = n_safe = 0
+        diags = []
+        for name, rays_np, hits_np in (("primary", primary, hits_p), ("bounce", bounce, hits_b)):
+            want, safe = orc.brute_closest_cull_safe(tris, rays_np[osel])
+            nb, d = compare_with_oracle(hits_np[osel], want, safe, rays_np[osel], 8, ray_set=name)
+            n_bad += nb
+            n_safe += int(safe.sum())
+            diags += d
+        verified["vs_oracle"] = {"rays": 2 * len(osel), "mismatches": n_bad, "winner_cull_safe": n_safe,
+                                 "first_mismatches": diags}
 
     if args.copy_mb:
         src = vk.DeviceArray(ctx, args.copy_mb << 20, np.uint8)
@@ -507,9 +777,12 @@ def main():
     if args.pt_lit_spp:
         log("lit path tracer frame")
         result["config"]["path_tracer_lit"] = path_tracer_frame(ctx, args.pt_lit_spp, lit=True)
+    if args.brute_force:
+        log("brute force 65,536 x 20,480")
+        result["config"]["brute_force"] = brute_force_leg(ctx)
     if args.batch_scenes:
         log("batched scenes")
-        result["config"]["batched"] = batched_frame(ctx, args.batch_scenes, 4096)
+        result["config"]["batched"] = batched_frame(ctx, args.batch_scenes, args.batch_verify)
     if args.rc_ref_spp:
         log("radiance cache: time to equal RMSE")
         result["config"]["radiance_cache"] = radiance_cache_run(ctx, args.rc_ref_spp, args.rc_frames)
