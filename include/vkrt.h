@@ -177,6 +177,18 @@ int vkrt_compute_bounding_sphere(vkrt_ctx *ctx, const float *points, uint32_t n,
  * lowest triangle index. */
 int vkrt_intersect_brute_closest(vkrt_ctx *ctx, const float *tris, uint32_t n_tri, const vkrt_ray *rays,
                                  uint32_t n_ray, vkrt_hit *out, int flags);
+/* Allocation-free closest hit. No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Results are identical to vkrt_intersect_brute_closest. Arrays must be device pointers
+ * (VKRT_DEVICE_PTRS is required). The call makes no hipMalloc/hipFree: its scratch (the triangles'
+ * (v0, e1, e2) and one 64-bit key per ray, 15 MiB) is a file-scope device array that each GPU the
+ * library runs on holds until the process ends. Sizes above VKRT_NOALLOC_MAX_TRI triangles or
+ * VKRT_NOALLOC_MAX_RAY rays are rejected with VKRT_E_INVALID_ARG. Every call on one GPU shares that
+ * GPU's array, so calls that overlap on one GPU (any contexts or streams) race. */
+#define VKRT_NOALLOC_MAX_TRI 262144u
+#define VKRT_NOALLOC_MAX_RAY 786432u
+int vkrt_intersect_brute_closest_noalloc(vkrt_ctx *ctx, const float *tris, uint32_t n_tri, const vkrt_ray *rays,
+                                         uint32_t n_ray, vkrt_hit *out, int flags);
 /* Any hit: occluded[i] = 1 iff some triangle gives INTERSECT with tmin < t < tmax. */
 int vkrt_intersect_brute_any(vkrt_ctx *ctx, const float *tris, uint32_t n_tri, const vkrt_ray *rays,
                              uint32_t n_ray, uint8_t *occluded, int flags);

@@ -226,6 +226,7 @@ _SIGNATURES = {
     "vkrt_compute_aabb": (_i32, [_vp, _vp, _u32, _vp, _i32]),
     "vkrt_compute_bounding_sphere": (_i32, [_vp, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_brute_closest": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
+    "vkrt_intersect_brute_closest_noalloc": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
     "vkrt_intersect_brute_any": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _i32]),
     "vkrt_bvh_build": (_i32, [_vp, _vp, _u32, _u32, _vp, _u32, _vp, _i32, ctypes.POINTER(_vp)]),
     "vkrt_bvh_refit": (_i32, [_vp, _vp, _vp, _u32, _i32]),

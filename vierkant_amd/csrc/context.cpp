@@ -1,5 +1,6 @@
 // context.cpp - C-ABI context lifetime, errors and streams.
 #include "context.hpp"
+#include "kernels.hpp"
 
 extern "C" {
 
@@ -35,6 +36,7 @@ int vkrt_create(int device, void *hip_stream, vkrt_ctx **out)
         std::string arch = prop.gcnArchName;
         if(arch.rfind("gfx950", 0) != 0)
             throw vkrt::error(VKRT_E_UNSUPPORTED, "libvkrt is built for gfx950 only; device is " + arch);
+        vkrt::noalloc_scratch_init(ctx);
     });
     if(rc != VKRT_OK)
     {

@@ -94,6 +94,8 @@ struct vkrt_ctx
     vkrt::device_buffer staging[8];
     //! scratch for multi-pass kernels (keys, counters)
     vkrt::device_buffer scratch[4];
+    //! this device's address of the allocation-free entry point's file-scope scratch array
+    void *noalloc_scratch = nullptr;
 };
 
 namespace vkrt

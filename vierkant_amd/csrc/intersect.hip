@@ -7,6 +7,7 @@
 // its best hit into a per-ray 64-bit key (ordered t << 32 | prim) with one atomic min, so ties on t
 // resolve to the lowest triangle index no matter which chunk finishes first.
 #include <cfloat>
+#include <mutex>
 
 #include "context.hpp"
 #include "kernels.hpp"
@@ -442,6 +443,37 @@ void brute_any(vkrt_ctx *ctx, const float *pre, uint32_t n_tri, const float4 *ra
     k_brute_any<<<grid, 256, 0, s>>>(rays, n_ray, pre, n_tri, chunk, occ);
 }
 
+// ---------------------------------------------------------------------------------------------
+// allocation-free brute force: its scratch (the (v0, e1, e2) triangles, then the 64-bit hit keys) is
+// a file-scope device array, so a call allocates nothing. Each GPU holds its own copy of the array;
+// the host reaches it only through the address hipGetSymbolAddress returns on that device (the
+// array's name on the host is a stand-in, not a device address).
+// ---------------------------------------------------------------------------------------------
+constexpr size_t NOALLOC_TRI_BYTES = (size_t)VKRT_NOALLOC_MAX_TRI * 9 * sizeof(float);
+constexpr size_t NOALLOC_BYTES = NOALLOC_TRI_BYTES + (size_t)VKRT_NOALLOC_MAX_RAY * sizeof(uint64_t);
+static_assert(NOALLOC_TRI_BYTES % 256 == 0, "the key area must stay 256-B aligned");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+//! resolves `scratch` on the context's device (current when this runs, from vkrt_create), once per device
+void noalloc_scratch_init(vkrt_ctx *ctx)
+{
+    if(ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return;// the entry point then reports it
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if(!g_scratch_ptr[ctx->device])
+    {
+        void *p = nullptr;
+        VKRT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        VKRT_REQUIRE(p, "hipGetSymbolAddress returned no address for the allocation-free scratch");
+        g_scratch_ptr[ctx->device] = p;
+    }
+    ctx->noalloc_scratch = g_scratch_ptr[ctx->device];
+}
+
 }// namespace vkrt
 
 // =============================================================================================
@@ -599,6 +631,31 @@ int vkrt_intersect_brute_closest(vkrt_ctx *ctx, const float *tris, uint32_t n_tr
                       dh);
         VKRT_HIP(hipGetLastError());
         finish_output(ctx, 2, out, n_ray, flags);
+        finish_call(ctx, flags);
+    });
+}
+
+int vkrt_intersect_brute_closest_noalloc(vkrt_ctx *ctx, const float *tris, uint32_t n_tri, const vkrt_ray *rays,
+                                         uint32_t n_ray, vkrt_hit *out, int flags)
+{
+    return guarded(ctx, [&] {
+        VKRT_REQUIRE(flags & VKRT_DEVICE_PTRS,
+                     "vkrt_intersect_brute_closest_noalloc takes device pointers only (VKRT_DEVICE_PTRS)");
+        VKRT_REQUIRE((n_tri == 0 || tris) && (n_ray == 0 || (rays && out)), "null array argument");
+        VKRT_REQUIRE(n_tri <= VKRT_NOALLOC_MAX_TRI,
+                     "vkrt_intersect_brute_closest_noalloc: n_tri exceeds VKRT_NOALLOC_MAX_TRI (262144)");
+        VKRT_REQUIRE(n_ray <= VKRT_NOALLOC_MAX_RAY,
+                     "vkrt_intersect_brute_closest_noalloc: n_ray exceeds VKRT_NOALLOC_MAX_RAY (786432)");
+        VKRT_REQUIRE(ctx->noalloc_scratch, "no allocation-free scratch on this device");
+        if(n_ray == 0) return;
+        unsigned char *base = static_cast<unsigned char *>(ctx->noalloc_scratch);
+        float *pre = reinterpret_cast<float *>(base);
+        uint64_t *keys = reinterpret_cast<uint64_t *>(base + NOALLOC_TRI_BYTES);
+        // every word brute_closest reads is written first in the same call: the triangles by the prep
+        // kernel, the keys by a fill kernel, so nothing left by an earlier call is read
+        launch_tri_prep(ctx->stream, tris, n_tri, pre);
+        brute_closest(ctx, pre, n_tri, reinterpret_cast<const float4 *>(rays), n_ray, keys, out);
+        VKRT_HIP(hipGetLastError());
         finish_call(ctx, flags);
     });
 }

@@ -50,6 +50,9 @@ void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t value);
 //! brute-force closest hit over pre-processed triangles (v0, e1, e2); keys: n_ray scratch words
 void brute_closest(vkrt_ctx *ctx, const float *pre, uint32_t n_tri, const float4 *rays, uint32_t n_ray,
                    uint64_t *keys, vkrt_hit *hits);
+//! resolves the allocation-free scratch array on the context's device (once per device)
+void noalloc_scratch_init(vkrt_ctx *ctx);
+
 void brute_any(vkrt_ctx *ctx, const float *pre, uint32_t n_tri, const float4 *rays, uint32_t n_ray, uint8_t *occ);
 
 }// namespace vkrt
