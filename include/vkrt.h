@@ -630,6 +630,21 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *scene);
  * transmittance shadow ray (ray/direct_lighting.slang:32-366), and light bodies met by BSDF rays add their
  * MIS-weighted radiance (raypipeline.slang:201-222, :728-761). At most 65536 lights; n = 0 clears the list. */
 int vkrt_pt_set_lights(vkrt_pt *pt, const vkrt_light *lights, uint32_t n);
+/* Launch shape of the path tracer's kernels (no counterpart in the reference, whose RT pipeline schedules
+ * itself). mode FULL_GRID (the default): one workgroup per tile of the pixel count at every depth, the tiles
+ * past the live queue returning at once; PERSISTENT: the per-depth trace and shadow kernels are fixed grids of
+ * the workgroups the GPU holds at once that claim the live queue's tiles with atomics. tail_depth (>= 1;
+ * default VKRT_PT_TAIL_DEPTH_DEFAULT): outside the radiance-cache mode the paths alive at that depth finish in
+ * one lane each, every remaining depth in one kernel; VKRT_PT_NO_TAIL keeps one launch per depth to the end.
+ * Every combination gives the same image bit for bit. */
+enum
+{
+    VKRT_PT_LAUNCH_PERSISTENT = 0,
+    VKRT_PT_LAUNCH_FULL_GRID = 1
+};
+#define VKRT_PT_TAIL_DEPTH_DEFAULT 4u
+#define VKRT_PT_NO_TAIL 0xffffffffu
+int vkrt_pt_set_launch_mode(vkrt_pt *pt, int mode, uint32_t tail_depth);
 /* One render_scene call: traces one batch of settings.num_samples samples per pixel into the running mean
  * unless max_num_batches is reached with suspend_trace_when_done set; then advances the batch index as
  * min(batch + 1, max_num_batches). A resolution change resets the accumulator. */

@@ -57,8 +57,58 @@ def test_pt_matches_cpu_integrator(ctx, scene3):
     diff = np.abs(g - c).max(axis=2)
     close = np.mean(diff <= 1e-4 * np.maximum(1.0, c.max(axis=2)))
     rmse = float(np.sqrt(np.mean((g - c) ** 2)))
+    print(f"GPU vs integrator: {close:.4f} of pixels within 1e-4, relative rmse {rmse / float(c.mean()):.5f}")
     assert close > 0.97, f"only {close:.4f} of pixels within 1e-4"
     assert rmse < 0.02 * float(c.mean()) + 1e-6, f"rmse {rmse} vs mean {c.mean()}"
+
+
+def _alpha_room(scene3):
+    """the room scene with every third material alpha-blended at 0.5 and every third (offset 1) alpha-masked
+    at alpha 0.3, cutoff 0.5 (cut out): the trace and shadow kernels then take the any-hit filter"""
+    sc = dict(scene3)
+    mats = scene3["materials"].copy()
+    mats["alpha_mode"][0::3] = vk.ALPHA_BLEND
+    mats["color"][0::3, 3] = 0.5
+    mats["alpha_mode"][1::3] = vk.ALPHA_MASK
+    mats["color"][1::3, 3] = 0.3
+    mats["alpha_cutoff"][1::3] = 0.5
+    sc["materials"] = mats
+    return sc
+
+
+@pytest.mark.parametrize("alpha", [False, True])
+@pytest.mark.parametrize("lit", [False, True])
+def test_launch_shapes_give_the_same_image(ctx, scene3, lit, alpha):
+    """per-depth launches as persistent grids over the live queue or as one lane per pixel at every depth, and
+    the deep-bounce tail finished in one lane per path from depth 1, 2, 3 or 4 or never: the same image bit for
+    bit and the same segment count (room scene, 960x540 so that the persistent grids are smaller than the full
+    ones, 4 spp, depth 8; lit: the six analytic lights, next-event shadow rays included; alpha: alpha-blended
+    and alpha-masked materials, so the any-hit filter runs in every kernel)"""
+    sc = _alpha_room(scene3) if alpha else scene3
+    shapes = [(vk.PT_LAUNCH_FULL_GRID, vk.PT_NO_TAIL), (vk.PT_LAUNCH_PERSISTENT, vk.PT_NO_TAIL),
+              (vk.PT_LAUNCH_PERSISTENT, 1), (vk.PT_LAUNCH_FULL_GRID, 2), (vk.PT_LAUNCH_PERSISTENT, 3),
+              (vk.PT_LAUNCH_FULL_GRID, vk.PT_TAIL_DEPTH_DEFAULT)]
+    imgs = []
+    for mode, tail in shapes:
+        pt = vk.PathTracer(ctx, settings(960, 540, 4), seed=5)
+        pt.set_scene(sc)
+        pt.set_launch_mode(mode, tail)
+        if lit:
+            pt.set_lights(W.room_lights())
+        pt.render(W.room_camera())
+        imgs.append((pt.pixels(), pt.stats().segments))
+        pt.close()
+    ref = imgs[0]
+    assert ref[1] > 960 * 540 * 4
+    for (mode, tail), (px, seg) in zip(shapes[1:], imgs[1:]):
+        assert px.tobytes() == ref[0].tobytes(), f"mode {mode}, tail {tail}"
+        assert seg == ref[1], f"mode {mode}, tail {tail}: {seg} segments against {ref[1]}"
+    bad = vk.PathTracer(ctx, settings(8, 8), seed=0)
+    with pytest.raises(vk.VkrtError):
+        bad.set_launch_mode(7)
+    with pytest.raises(vk.VkrtError):
+        bad.set_launch_mode(vk.PT_LAUNCH_PERSISTENT, 0)
+    bad.close()
 
 
 def test_pt_sky_only_bit_exact(ctx):
@@ -222,11 +272,14 @@ def test_transmittance_matches_oracle(ctx, scene3):
     pt.close()
 
 
-def test_transmittance_many_layers_in_order(ctx):
-    """20 transmissive plates along the ray: more than one 8-candidate pass, product in ascending t"""
+@pytest.mark.parametrize("coplanar", [False, True])
+def test_transmittance_many_layers_in_order(ctx, coplanar):
+    """20 transmissive plates along the ray: more than one 8-candidate pass, product in ascending (t, triangle);
+    coplanar: all 20 at one height, so every candidate ties in t, the passes' running bound sits exactly on
+    the plates and the order comes from the triangle index alone"""
     import oracle_binding as orc2
     from test_oracle import _quad_scene, _down_rays
-    zs = [0.1 * k for k in range(20)]
+    zs = [0.5] * 20 if coplanar else [0.1 * k for k in range(20)]
     rng = np.random.default_rng(5)
     sc = _quad_scene(zs, list(rng.uniform(0.6, 1.0, 20)), (0.9, 0.7, 0.95))
     mats = sc["materials"].copy()

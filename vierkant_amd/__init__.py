@@ -135,6 +135,8 @@ LIGHT_DTYPE = np.dtype([("position", "<f4", 3), ("type", "<u4"), ("color", "<f4"
 assert LIGHT_DTYPE.itemsize == 80
 LIGHT_OMNI, LIGHT_SPOT, LIGHT_DIRECTIONAL, LIGHT_AREA, LIGHT_RECT, LIGHT_SPHERE, LIGHT_TUBE, LIGHT_DISK = range(8)
 MIS_FULL, MIS_NEE_ONLY, MIS_BSDF_ONLY = 0, 1, 2
+PT_LAUNCH_PERSISTENT, PT_LAUNCH_FULL_GRID = 0, 1
+PT_TAIL_DEPTH_DEFAULT, PT_NO_TAIL = 4, 0xFFFFFFFF
 
 
 def light(type, position=(0.0, 0.0, 0.0), direction=(0.0, 0.0, -1.0), color=(1.0, 1.0, 1.0), intensity=1.0,
@@ -253,6 +255,7 @@ _SIGNATURES = {
     "vkrt_pt_get_settings": (_i32, [_vp, ctypes.POINTER(PtSettings)]),
     "vkrt_pt_set_scene": (_i32, [_vp, ctypes.POINTER(PtScene)]),
     "vkrt_pt_set_lights": (_i32, [_vp, _vp, _u32]),
+    "vkrt_pt_set_launch_mode": (_i32, [_vp, _i32, _u32]),
     "vkrt_model_load": (_i32, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "vkrt_model_error": (ctypes.c_char_p, []),
     "vkrt_model_get": (_i32, [_vp, ctypes.POINTER(ModelView)]),
@@ -770,6 +773,11 @@ class PathTracer:
                      uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
                      tg.ctypes.data if tg is not None else None)
         self.ctx.check(lib().vkrt_pt_set_scene(self._h, ctypes.byref(sc)))
+
+    def set_launch_mode(self, mode, tail_depth=PT_TAIL_DEPTH_DEFAULT):
+        """PT_LAUNCH_FULL_GRID (default) or PT_LAUNCH_PERSISTENT per-depth launches; paths alive at tail_depth
+        finish in one lane each (PT_NO_TAIL: per-depth launches to the end). Every choice gives the same image"""
+        self.ctx.check(lib().vkrt_pt_set_launch_mode(self._h, mode, tail_depth))
 
     def set_lights(self, lights):
         """the frame's light list (LIGHT_DTYPE records; empty clears it)"""

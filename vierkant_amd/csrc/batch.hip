@@ -37,34 +37,14 @@ struct batch_item_dev
     uint32_t pad[3];
 };
 
-constexpr uint32_t XCDS = 8;           // MI355X: workgroup b is dispatched to XCD b % 8
-constexpr uint32_t COUNTER_STRIDE = 16;// one 64-B line per queue counter
-constexpr uint32_t N_COUNTERS = XCDS * COUNTER_STRIDE;
+constexpr uint32_t N_COUNTERS = CLAIM_COUNTERS;
 
 __global__ void k_batch_reset(uint32_t *counter) { counter[threadIdx.x] = 0u; }
 
-//! Tile claims. chunk == 0: one queue, tiles in order. chunk == C > 0: the tile sequence is cut into chunks of
-//! C consecutive tiles dealt round robin to the 8 XCDs; XCD x's i-th claim is tile ((i / C) 8 + x) C + i % C,
-//! which grows with i, so a queue whose claim passes the tile count is exhausted for good. A block first
-//! drains its own XCD's queue (neighbouring tiles of one scene then share that XCD's L2), then the others in
-//! turn, so no XCD idles while tiles remain; every block leaves after at most one failed claim per queue.
-__device__ __forceinline__ uint32_t claim_tile(uint32_t *__restrict__ counters, uint32_t chunk, uint32_t home,
-                                               uint32_t total, uint32_t &queue)
-{
-    const uint32_t n_queues = chunk ? XCDS : 1u;
-    while(queue < n_queues)
-    {
-        const uint32_t x = (home + queue) % n_queues;
-        const uint64_t i = atomicAdd(&counters[x * COUNTER_STRIDE], 1u);
-        const uint64_t t = chunk ? ((i / chunk) * XCDS + x) * chunk + i % chunk : i;
-        if(t < total) return static_cast<uint32_t>(t);
-        ++queue;
-    }
-    return total;
-}
-
+//! (8 waves per SIMD where the LDS stack allows it: the traversal fits 64 VGPRs with a spill or two of the
+//! loop state, against 72 unconstrained, which holds 7)
 template<int BLOCK, int LDS>
-__global__ void __launch_bounds__(BLOCK) k_batch_trace(const batch_item_dev *__restrict__ items, uint32_t n_items,
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(8))) k_batch_trace(const batch_item_dev *__restrict__ items, uint32_t n_items,
                                                        const uint32_t *__restrict__ tile_prefix, uint32_t tile,
                                                        uint32_t *__restrict__ counters, uint32_t chunk)
 {

@@ -43,6 +43,44 @@ inline uint32_t xcd_grid(uint32_t blocks, uint32_t chunk)
     return chunk ? static_cast<uint32_t>((blocks + 8ull * chunk - 1) / (8ull * chunk) * 8ull * chunk) : blocks;
 }
 
+constexpr uint32_t XCDS = 8;           // MI355X: workgroup b is dispatched to XCD b % 8
+constexpr uint32_t COUNTER_STRIDE = 16;// one 64-B line per queue counter
+constexpr uint32_t CLAIM_COUNTERS = XCDS * COUNTER_STRIDE;// words of one set of claim counters
+
+//! Tile claims of a persistent kernel from `counters` (CLAIM_COUNTERS words, zeroed before the launch).
+//! chunk == 0: one queue, tiles in order. chunk == C > 0: the tile sequence is cut into chunks of C
+//! consecutive tiles dealt round robin to the 8 XCDs; XCD x's i-th claim is tile ((i / C) 8 + x) C + i % C,
+//! which grows with i, so a queue whose claim passes the tile count is exhausted for good. A caller first
+//! drains its own XCD's queue (`home`; neighbouring tiles then share that XCD's L2), then the others in turn,
+//! so no XCD idles while tiles remain, and leaves after at most one failed claim per queue. `queue` is the
+//! caller's position in that order (starts at 0). Returns `total` when every queue is exhausted.
+//! A queue is read before it is claimed from: counters only grow, so a value whose tile is past the end means
+//! the queue is exhausted for good, and the caller moves on without an atomic (thousands of workgroups
+//! leaving a persistent launch would otherwise queue up failed atomics on eight addresses).
+__device__ __forceinline__ uint64_t claim_slot_tile(uint64_t i, uint32_t x, uint32_t chunk)
+{
+    return chunk ? ((i / chunk) * XCDS + x) * chunk + i % chunk : i;
+}
+
+__device__ __forceinline__ uint32_t claim_tile(uint32_t *__restrict__ counters, uint32_t chunk, uint32_t home,
+                                               uint32_t total, uint32_t &queue)
+{
+    const uint32_t n_queues = chunk ? XCDS : 1u;
+    while(queue < n_queues)
+    {
+        const uint32_t x = (home + queue) % n_queues;
+        uint32_t *c = &counters[x * COUNTER_STRIDE];
+        const uint32_t seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if(claim_slot_tile(seen, x, chunk) < total)
+        {
+            const uint64_t t = claim_slot_tile(atomicAdd(c, 1u), x, chunk);
+            if(t < total) return static_cast<uint32_t>(t);
+        }
+        ++queue;
+    }
+    return total;
+}
+
 void launch_tri_prep(hipStream_t s, const float *tris, uint32_t n, float *pre);
 void launch_fill_u64(hipStream_t s, uint64_t *p, uint32_t n, uint64_t value);
 void launch_fill_u8(hipStream_t s, uint8_t *p, uint32_t n, uint8_t value);

@@ -16,6 +16,7 @@
 //
 // All per-path state is SoA in HBM (float4 / uint4 per path) so every kernel reads and writes it
 // coalesced; queues hold pixel indices.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -52,6 +53,15 @@ constexpr int PT_LDS = 16;
 // workgroups one XCD receives take PT_TRACE_CHUNK consecutive queue blocks in turn)
 constexpr int PT_TRACE_BLOCK = 64;
 constexpr uint32_t PT_TRACE_CHUNK = 8;
+// Per-depth launches cover every pixel (a workgroup per tile, those past the live queue return at once). The
+// persistent alternative (VKRT_PT_LAUNCH_PERSISTENT) launches the workgroups the chip holds at once, the trace
+// kernel claiming tiles of PT_TRACE_BLOCK paths from per-XCD queues at counter + PT_CLAIMS and the shadow
+// kernel striding over its records; measured on the room scene (1920x1080, 64 spp, depth 8) it is slower,
+// 230 ms against 192 ms a frame: the hardware dispatcher hands out tiles faster than the claim atomics.
+constexpr uint32_t PT_CLAIMS = 64;            // word offset of the trace kernel's claim counters
+constexpr uint32_t PT_COUNTER_WORDS = PT_CLAIMS + CLAIM_COUNTERS;
+constexpr uint32_t PT_TRACE_WAVES_PER_SIMD = 6;// the trace kernel's register cap (amdgpu_waves_per_eu): 80 VGPRs
+constexpr uint32_t PT_TAIL_DEPTH = VKRT_PT_TAIL_DEPTH_DEFAULT;// survivors at this depth finish in one lane each
 
 //! f32tof16 / f16tof32 round trip (round to nearest even) of the media stack's packed ior
 __host__ __device__ inline float round_half(float x)
@@ -342,21 +352,26 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
 
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_zero(uint32_t *c) { c[threadIdx.x] = 0u; }
 
-//! before a depth: the next queue and the shadow-ray count start at zero
+//! before a depth: the next queue, the shadow-ray count and the trace kernel's claim counters start at zero
+//! (one thread per claim counter)
 __global__ void k_pt_reset_depth(uint32_t *c, int next)
 {
-    c[next] = 0u;
-    c[6] = 0u;
+    if(threadIdx.x == 0)
+    {
+        c[next] = 0u;
+        c[6] = 0u;
+    }
+    c[PT_CLAIMS + threadIdx.x * COUNTER_STRIDE] = 0u;
 }
 
 //! the closest hit of path p's current ray at this depth; back faces are culled while the path is in air.
 //! BLEND candidates compare against rnd() of a copy of payload.rng_state = xxhash32(rng_state, depth)
 //! (raypipeline.slang:150-152, :693, :716)
-template<int BLOCK = PT_BLOCK>
+template<int BLOCK = PT_BLOCK, class F = alpha_filter>
 __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                            int32_t root, bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
                                            int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem,
-                                           alpha_filter flt, uint32_t depth)
+                                           F flt, uint32_t depth)
 {
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
@@ -365,7 +380,7 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
         ray_setup r = make_ray_setup(b.ray_o[p], b.ray_d[p]);
         const uint4 st = b.state[p];
         const bool in_air = (st.y & F_MEDIA_MASK) == 0u;
-        flt.xi = rnd_peek(xxhash32(st.x, depth));
+        if constexpr(F::active) flt.xi = rnd_peek(xxhash32(st.x, depth));
         // the shade kernel recovers HitKind (back face: negative determinant) from the vertices
         if(wide)
             traverse4<false, BLOCK, PT_LDS>(nodes4, tris, root4, r, smem, t, u, v, prim, in_air, flt);
@@ -375,17 +390,38 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
     b.hit[p] = make_uint4(__float_as_uint(t), __float_as_uint(u), __float_as_uint(v), prim);
 }
 
-__global__ void __launch_bounds__(PT_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(6))) k_pt_trace(const vkrt_bvh_node *__restrict__ nodes,
-                                                             const float4 *__restrict__ tris, int32_t root, bool empty,
-                                                             const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
-                                                             bool wide, pt_buffers b, int in, alpha_filter flt,
-                                                             uint32_t depth)
+//! F is alpha_filter where some material is alpha-tested, keep_all elsewhere (the any-hit test and its texture
+//! fetch compiled out)
+template<class F>
+__global__ void __launch_bounds__(PT_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRACE_WAVES_PER_SIMD)))
+k_pt_trace(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris, int32_t root, bool empty,
+           const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide, pt_buffers b, int in, F flt,
+           uint32_t depth)
 {
     __shared__ int32_t smem[PT_LDS * PT_TRACE_BLOCK];
-    const uint32_t q = xcd_block(blockIdx.x, PT_TRACE_CHUNK) * PT_TRACE_BLOCK + threadIdx.x;
-    if(q >= b.counter[in]) return;
-    trace_path<PT_TRACE_BLOCK>(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q], smem + threadIdx.x,
-                               flt, depth);
+    const uint32_t count = b.counter[in];
+    const uint32_t tiles = (count + PT_TRACE_BLOCK - 1) / PT_TRACE_BLOCK;
+    const uint32_t home = blockIdx.x % XCDS;
+    uint32_t queue = 0;
+    // one wave per workgroup: lane 0 claims the next tile of the queue and the wave reads it back; a grid of
+    // one workgroup per tile or more (the full-grid launch) takes tile blockIdx.x and leaves
+    const bool one_tile = gridDim.x >= tiles;
+    bool done = false;
+    while(!done)
+    {
+        uint32_t t = blockIdx.x;
+        if(one_tile) done = true;
+        else
+        {
+            if(threadIdx.x == 0) t = claim_tile(b.counter + PT_CLAIMS, PT_TRACE_CHUNK, home, tiles, queue);
+            t = __builtin_amdgcn_readfirstlane(t);
+        }
+        if(t >= tiles) break;
+        const uint32_t q = t * PT_TRACE_BLOCK + threadIdx.x;
+        if(q < count)
+            trace_path<PT_TRACE_BLOCK, F>(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q],
+                                       smem + threadIdx.x, flt, depth);
+    }
 }
 
 //! eval_distant_lights (raypipeline.slang:728-744): sun discs seen by a ray that left the scene
@@ -759,14 +795,12 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
     }
 }
 
-//! one depth of shading over a compacted queue; survivors, cache queries and shadow rays are appended with
-//! one ballot and one atomic per wave each
+//! queue entry q of a depth (inactive past the count): shade_path, then survivors, cache queries and shadow
+//! rays are appended with one ballot and one atomic per wave each
 template<bool LIGHTS>
-__global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b, pt_scene_dev s, uint32_t depth,
-                                                       int in)
+__device__ __forceinline__ void shade_queue_entry(const pt_frame &f, const pt_buffers &b, const pt_scene_dev &s,
+                                                  uint32_t depth, int in, uint32_t q, uint32_t count)
 {
-    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
-    const uint32_t count = b.counter[in];
     const bool active = q < count;
     bool alive = false;
     bool to_cache = false;
@@ -823,6 +857,19 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b,
             for(uint32_t k = 0; k < SHADOW_F4; ++k) dst[k] = shadow_rec[k];
         }
     }
+}
+
+//! one depth of shading over a compacted queue; survivors, cache queries and shadow rays are appended with
+//! one ballot and one atomic per wave each
+//! (one tile per workgroup over a grid that covers every pixel: a grid-striding loop here keeps more values
+//! live across iterations, 190 VGPRs against 161, and drops the kernel from 3 waves per SIMD to 2)
+template<bool LIGHTS>
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b, pt_scene_dev s, uint32_t depth,
+                                                       int in)
+{
+    const uint32_t count = b.counter[in];
+    const uint32_t first = blockIdx.x * PT_BLOCK;
+    if(first < count) shade_queue_entry<LIGHTS>(f, b, s, depth, in, first + threadIdx.x, count);
 }
 
 //! training targets: L_d = e_d + w_d * (g_{d+1} + L_{d+1}) from the last vertex back, where e_d holds the
@@ -899,10 +946,11 @@ constexpr int TR_STACK = 64;
 
 //! the transmittance of one ray through the scene: zero when an opaque triangle lies in (tmin, tmax),
 //! else the product of transmission * color over the transmissive hits in ascending (t, triangle) order
+template<class F = alpha_filter>
 __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                      const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide, bool empty,
                                      const float4 *__restrict__ tris, const gpu_material *__restrict__ mats,
-                                     const ray_setup &r, bool cull, int32_t *lds, const alpha_filter &flt)
+                                     const ray_setup &r, bool cull, int32_t *lds, const F &flt)
 {
     int32_t spill[TR_STACK - PT_LDS];
     lane_stack<PT_BLOCK, PT_LDS> st{lds, spill};
@@ -923,6 +971,12 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
         }
         int cnt = 0;
         more = false;
+        // the running bound of the pass: once TK candidates are kept, boxes beyond the farthest of them can add
+        // nothing to this pass (whatever lies there, an opaque blocker included, is the next pass's); a later
+        // pass skips boxes that end before the previous pass's last candidate (an opaque blocker there would
+        // have ended the walk already)
+        float bound = r.tmax;
+        const float lo = first_pass ? r.tmin : fmaxf(r.tmin, lb_t);
         int sp = 0;
         int32_t ref = wide ? root4 : root2;
         while(true)
@@ -931,15 +985,15 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
             {
                 int32_t c[4];
                 int nc;
-                if(wide) nc = visit4(nodes4, ref, r, r.tmax, c);
+                if(wide) nc = visit4(nodes4, ref, r, bound, c, lo);
                 else
                 {
                     const vkrt_bvh_node nd = nodes2[ref];
                     float tn0, tf0, tn1, tf1;
                     slab_interval(r, nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y, tn0, tf0);
                     slab_interval(r, nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w, tn1, tf1);
-                    const bool h0 = tn0 <= tf0 && tf0 >= r.tmin && tn0 <= r.tmax;
-                    const bool h1 = tn1 <= tf1 && tf1 >= r.tmin && tn1 <= r.tmax;
+                    const bool h0 = tn0 <= tf0 && tf0 >= lo && tn0 <= bound;
+                    const bool h1 = tn1 <= tf1 && tf1 >= lo && tn1 <= bound;
                     nc = 0;
                     if(h0) c[nc++] = nd.d.x;
                     if(h1) c[nc++] = nd.d.y;
@@ -999,6 +1053,11 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
                         cm = less ? tm : cm;
                     }
                     cnt = cnt < TK ? cnt + 1 : TK;
+                    if(cnt == TK)
+                    {
+                        bound = kt[TK - 1];
+                        more = true;
+                    }
                 }
                 if(blocked) break;
             }
@@ -1042,16 +1101,20 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_no
 }
 
 //! trace one shadow record and add its next-event estimate to the pixel (and to its training vertex)
+template<class F = alpha_filter>
 __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                             const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide,
                                             bool empty, const float4 *__restrict__ tris,
                                             const gpu_material *__restrict__ mats, const pt_frame &f,
                                             const pt_buffers &b, const float4 (&rec)[SHADOW_F4], int32_t *smem,
-                                            alpha_filter flt)
+                                            F flt)
 {
     const float4 o = rec[0], d = rec[1];
-    flt.per_candidate = true;
-    flt.seed = __float_as_uint(rec[5].z);
+    if constexpr(F::active)
+    {
+        flt.per_candidate = true;
+        flt.seed = __float_as_uint(rec[5].z);
+    }
     const float4 T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d),
                                         false, smem, flt);
     if(!(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)) return;
@@ -1077,19 +1140,22 @@ __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ no
 //! the shadow half of nee_surface (direct_lighting.slang:311-333): the unculled transmittance toward the
 //! sampled light, then transmittance * weight * n_lights * radiance * F * cos / pdf scaled by the vertex's
 //! throughput joins the pixel's sample sum; a training vertex also records the unscaled estimate
+template<class F>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                                         const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
                                                         bool wide, bool empty, const float4 *__restrict__ tris,
                                                         const gpu_material *__restrict__ mats, pt_frame f,
-                                                        pt_buffers b, alpha_filter flt)
+                                                        pt_buffers b, F flt)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
-    const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
-    if(i >= b.counter[6]) return;
-    const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
-    float4 r[SHADOW_F4];
-    for(uint32_t k = 0; k < SHADOW_F4; ++k) r[k] = rec[k];
-    shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x, flt);
+    const uint32_t count = b.counter[6];
+    for(uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x; i < count; i += gridDim.x * PT_BLOCK)
+    {
+        const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
+        float4 r[SHADOW_F4];
+        for(uint32_t k = 0; k < SHADOW_F4; ++k) r[k] = rec[k];
+        shadow_path<F>(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x, flt);
+    }
 }
 
 //! the cache mode's training paths, one lane per path through every depth: trace, shade and the shadow ray
@@ -1124,6 +1190,50 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node
         if(!alive) break;
     }
     atomicAdd(reinterpret_cast<unsigned long long *>(&b.counter[2]), (unsigned long long)segments);
+}
+
+//! the deep-bounce tail: the paths alive at depth d0 (queue `in`) finish in one lane each, trace, shade and
+//! shadow ray at every remaining depth with no kernel boundary between depths. Past the first few depths the
+//! live queue fills a fraction of the GPU and each per-depth launch lasts as long as its slowest wave's
+//! traversal (a chain of dependent node loads); here a wave's lanes run their depths back to back and the
+//! waves overlap. Each path performs the operations of the per-depth kernels in the same order (trace, shade,
+//! shadow ray, next depth), so the image is bit for bit the same.
+template<bool LIGHTS, class F>
+__global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(const vkrt_bvh_node *__restrict__ nodes2,
+                                                        const float4 *__restrict__ tris, int32_t root2, bool empty,
+                                                        const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
+                                                        bool wide, pt_frame f, pt_buffers b, pt_scene_dev s, F flt,
+                                                        F shadow_flt, uint32_t d0, int in)
+{
+    __shared__ int32_t smem[PT_LDS * PT_BLOCK];
+    __shared__ unsigned long long s_segments;
+    if(threadIdx.x == 0) s_segments = 0ull;
+    __syncthreads();
+    const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
+    const uint32_t count = b.counter[in];
+    uint32_t segments = 0;
+    if(q < count)
+    {
+        const uint32_t p = b.queue[in][q];
+        for(uint32_t depth = d0; depth < f.max_depth; ++depth)
+        {
+            trace_path<PT_BLOCK, F>(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x, flt,
+                                    depth);
+            bool alive = false, to_cache = false, need_shadow = false;
+            float4 shadow_rec[SHADOW_F4];
+            float4 qfeat[QUERY_F4];
+            shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
+            ++segments;
+            if(LIGHTS && need_shadow)
+                shadow_path<F>(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec,
+                               smem + threadIdx.x, shadow_flt);
+            if(!alive) break;
+        }
+    }
+    if(segments) atomicAdd(&s_segments, (unsigned long long)segments);
+    __syncthreads();
+    if(threadIdx.x == 0 && s_segments)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&b.counter[2]), s_segments);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1224,6 +1334,9 @@ struct vkrt_pt
 {
     vkrt_ctx *ctx = nullptr;
     vkrt_pt_settings settings{};
+    int launch_mode = VKRT_PT_LAUNCH_FULL_GRID;// per-depth launch shape (vkrt_pt_set_launch_mode)
+    uint32_t tail_depth = vkrt::PT_TAIL_DEPTH;  // survivors at this depth finish in one lane (k_pt_finish)
+    bool any_alpha = false;                    // some material is alpha-tested (MASK or BLEND)
     std::minstd_rand0 random_engine;// std::default_random_engine of libstdc++ (PBRPathTracer.cpp:32)
     uint32_t batch_index = 0;
     vkrt_bvh *bvh = nullptr;
@@ -1299,7 +1412,7 @@ static void resize_storage(vkrt_pt *pt, uint32_t w, uint32_t h)
     pt->info.reserve(n * 16);
     pt->queue0.reserve(n * 4);
     pt->queue1.reserve(n * 4);
-    pt->counter.reserve(256);
+    pt->counter.reserve(PT_COUNTER_WORDS * sizeof(uint32_t));
     pt->pixels.reserve(n * sizeof(vkrt_pixel));
     VKRT_HIP(hipMemsetAsync(pt->pixels.ptr, 0, n * sizeof(vkrt_pixel), pt->ctx->stream));
     pt->width = w;
@@ -1470,18 +1583,44 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     const vkrt_bvh4_node *nodes4 = bvh->nodes4.as<vkrt_bvh4_node>();
     const float4 *tris = bvh->tris.as<float4>();
     // one depth of a queue of paths: trace, shade, and the shadow rays of next-event estimation
-    auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t grid, uint32_t depth, int in) {
-        k_pt_reset_depth<<<1, 1, 0, st>>>(bb.counter, in ^ 1);
-        k_pt_trace<<<xcd_grid(div_up((uint64_t)grid * PT_BLOCK, PT_TRACE_BLOCK), PT_TRACE_CHUNK), PT_TRACE_BLOCK, 0,
-                     st>>>(nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, bb, in, flt, depth);
+    // scenes with no alpha-tested material take the trace and shadow kernels without the any-hit filter
+    const bool alpha = pt->any_alpha;
+    // persistent grids of the trace and shadow kernels: the workgroups resident at once (the runtime's
+    // occupancy figure for the kernel's registers and LDS), at most one per tile of a full queue
+    // (VKRT_PT_LAUNCH_FULL_GRID launches one workgroup per tile of every pixel at every depth instead: the same
+    // kernels, each workgroup then taking at most one tile)
+    const bool full = pt->launch_mode == VKRT_PT_LAUNCH_FULL_GRID;
+    const uint32_t cus = static_cast<uint32_t>(pt->ctx->cu_count);
+    auto resident = [&](auto kernel, int block, uint32_t cap) {
+        int per_cu = 0;
+        VKRT_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0));
+        return full ? cap : std::min<uint32_t>(cus * static_cast<uint32_t>(std::max(per_cu, 1)), cap);
+    };
+    const uint32_t trace_full = div_up(f.n_pix, PT_TRACE_BLOCK);
+    const uint32_t trace_grid = alpha ? resident(k_pt_trace<alpha_filter>, PT_TRACE_BLOCK, trace_full)
+                                      : resident(k_pt_trace<keep_all>, PT_TRACE_BLOCK, trace_full);
+    const uint32_t shadow_grid = alpha ? resident(k_pt_shadow<alpha_filter>, PT_BLOCK, blocks)
+                                       : resident(k_pt_shadow<keep_all>, PT_BLOCK, blocks);
+    auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t depth, int in) {
+        k_pt_reset_depth<<<1, XCDS, 0, st>>>(bb.counter, in ^ 1);
+        if(alpha)
+            k_pt_trace<alpha_filter><<<trace_grid, PT_TRACE_BLOCK, 0, st>>>(
+                nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, bb, in, flt, depth);
+        else
+            k_pt_trace<keep_all><<<trace_grid, PT_TRACE_BLOCK, 0, st>>>(
+                nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, bb, in, keep_all{}, depth);
         if(f.n_lights)
         {
-            k_pt_shade<true><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
-            k_pt_shadow<<<grid, PT_BLOCK, 0, st>>>(nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty,
-                                                   tris, sc.mats, f, bb, shadow_flt);
+            k_pt_shade<true><<<blocks, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+            if(alpha)
+                k_pt_shadow<alpha_filter><<<shadow_grid, PT_BLOCK, 0, st>>>(
+                    nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty, tris, sc.mats, f, bb, shadow_flt);
+            else
+                k_pt_shadow<keep_all><<<shadow_grid, PT_BLOCK, 0, st>>>(
+                    nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty, tris, sc.mats, f, bb, keep_all{});
         }
         else
-            k_pt_shade<false><<<grid, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+            k_pt_shade<false><<<blocks, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
     };
     VKRT_HIP(hipEventRecord(pt->ev0, s));
     k_pt_zero<<<1, 2 * TRAIN_COUNTERS, 0, s>>>(b.counter);
@@ -1517,7 +1656,30 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
             VKRT_HIP(hipGetLastError());
             VKRT_HIP(hipEventRecord(pt->ev_join, pt->train_stream));
         }
-        for(uint32_t depth = 0; depth < main_depths; ++depth) depth_step(s, b, blocks, depth, static_cast<int>(depth & 1u));
+        // the deep-bounce tail (not in the cache mode, whose main paths end at the cache depth and queue cache
+        // queries): the survivors at tail_depth finish in one lane each
+        const uint32_t per_depth = f.cache_on ? main_depths : std::min(main_depths, pt->tail_depth);
+        for(uint32_t depth = 0; depth < per_depth; ++depth) depth_step(s, b, depth, static_cast<int>(depth & 1u));
+        if(per_depth < main_depths && per_depth > 0)
+        {
+            const int in = static_cast<int>(per_depth & 1u);
+            if(alpha && f.n_lights)
+                k_pt_finish<true, alpha_filter><<<blocks, PT_BLOCK, 0, s>>>(
+                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, flt, shadow_flt,
+                    per_depth, in);
+            else if(alpha)
+                k_pt_finish<false, alpha_filter><<<blocks, PT_BLOCK, 0, s>>>(
+                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, flt, shadow_flt,
+                    per_depth, in);
+            else if(f.n_lights)
+                k_pt_finish<true, keep_all><<<blocks, PT_BLOCK, 0, s>>>(
+                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, keep_all{},
+                    keep_all{}, per_depth, in);
+            else
+                k_pt_finish<false, keep_all><<<blocks, PT_BLOCK, 0, s>>>(
+                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, keep_all{},
+                    keep_all{}, per_depth, in);
+        }
         if(f.cache_on)
         {
             rc::query(pt->rc, b.qrec, QUERY_F4, f.n_pix, nullptr, b.acc, b.counter + 4);
@@ -1608,6 +1770,18 @@ int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out)
 }
 
 static bool finite3(const float *v) { return std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]); }
+
+int vkrt_pt_set_launch_mode(vkrt_pt *pt, int mode, uint32_t tail_depth)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(mode == VKRT_PT_LAUNCH_PERSISTENT || mode == VKRT_PT_LAUNCH_FULL_GRID,
+                     "launch mode must be VKRT_PT_LAUNCH_PERSISTENT or VKRT_PT_LAUNCH_FULL_GRID");
+        VKRT_REQUIRE(tail_depth >= 1, "tail_depth must be at least 1 (VKRT_PT_NO_TAIL: no tail)");
+        pt->launch_mode = mode;
+        pt->tail_depth = tail_depth;
+    });
+}
 
 int vkrt_pt_set_lights(vkrt_pt *pt, const vkrt_light *lights, uint32_t n)
 {
@@ -1777,6 +1951,8 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         upload(pt->object_material, sc->object_material, (size_t)sc->n_objects * 4);
         upload(pt->object_tmax, tmax.data(), tmax.size() * 4);
         upload(pt->mats, mats.data(), mats.size() * sizeof(gpu_material));
+        pt->any_alpha = std::any_of(mats.begin(), mats.end(),
+                                    [](const gpu_material &m) { return m.tex.y != VKRT_ALPHA_OPAQUE; });
         upload(pt->uv, uv.data(), uv.size() * 4);
         upload(pt->tan, tan.data(), tan.size() * 4);
         upload(pt->tex_desc_buf, descs.data(), descs.size() * sizeof(tex_desc));

@@ -23,8 +23,7 @@ struct ray_setup
 {
     vec3 o, d, inv;
     float tmin, tmax;
-    vec3 clo, chi;// slab offsets of the lo and hi planes (see make_ray_setup)
-    vec3 cn, cf;  // the same offsets for each axis's near plane (met first) and far plane
+    vec3 cn, cf;// slab offsets of each axis's near plane (met first) and far plane (see make_ray_setup)
     // byte offsets in a 4-wide node of each axis's near-plane float4 (lo when the direction component is
     // >= 0, else hi); the far plane's float4 is at the offset ^ 16
     uint32_t nx, ny, nz;
@@ -42,25 +41,19 @@ __device__ __forceinline__ ray_setup make_ray_setup(float4 a, float4 b)
     auto safe = [](float x) { return fabsf(x) < 1e-20f ? copysignf(1e-20f, x) : x; };
     r.inv = vec3{1.0f / safe(r.d.x), 1.0f / safe(r.d.y), 1.0f / safe(r.d.z)};
     // a plane b gives t = fmaf(b, inv, c): c = -o inv shifted by A = 2^-20 |o inv| toward smaller t for the
-    // plane the ray meets first on that axis (lo when inv > 0) and toward larger t for the other. fl(o inv)
-    // and the rounding of c each miss o inv by at most 2^-24 |o inv|, far inside A, so the shifted value
-    // bounds (b - o) inv from the conservative side on every axis; the relative error left (inv's and the
-    // fma's rounding, 2 x 2^-24 |t|) is the one slab_interval's T_PAD covers.
+    // plane the ray meets first on that axis (the near plane: lo when inv >= 0, else hi) and toward larger t
+    // for the far one. fl(o inv) and the rounding of c each miss o inv by at most 2^-24 |o inv|, far inside A,
+    // so the shifted value bounds (b - o) inv from the conservative side on every axis; the relative error
+    // left (inv's and the fma's rounding, 2 x 2^-24 |t|) is the one the interval pad T_PAD covers.
     // (an infinite o inv takes no shift: both planes then sit at the same infinite t, which is exact)
-    auto offsets = [](float o, float inv, float &clo, float &chi) {
+    auto offsets = [](float o, float inv, float &cn, float &cf) {
         const float p = o * inv, a = fabsf(p) < __builtin_huge_valf() ? fabsf(p) * 0x1p-20f : 0.0f;
-        clo = inv >= 0.0f ? -p - a : -p + a;
-        chi = inv >= 0.0f ? -p + a : -p - a;
+        cn = -p - a;
+        cf = -p + a;
     };
-    offsets(r.o.x, r.inv.x, r.clo.x, r.chi.x);
-    offsets(r.o.y, r.inv.y, r.clo.y, r.chi.y);
-    offsets(r.o.z, r.inv.z, r.clo.z, r.chi.z);
-    // the near plane is lo where inv >= 0 (its offset is clo), hi elsewhere (its offset is chi): either way
-    // the smaller of the two offsets, the one shifted toward smaller t
-    r.cn = vec3{r.inv.x >= 0.0f ? r.clo.x : r.chi.x, r.inv.y >= 0.0f ? r.clo.y : r.chi.y,
-                r.inv.z >= 0.0f ? r.clo.z : r.chi.z};
-    r.cf = vec3{r.inv.x >= 0.0f ? r.chi.x : r.clo.x, r.inv.y >= 0.0f ? r.chi.y : r.clo.y,
-                r.inv.z >= 0.0f ? r.chi.z : r.clo.z};
+    offsets(r.o.x, r.inv.x, r.cn.x, r.cf.x);
+    offsets(r.o.y, r.inv.y, r.cn.y, r.cf.y);
+    offsets(r.o.z, r.inv.z, r.cn.z, r.cf.z);
     r.nx = r.inv.x >= 0.0f ? 0u : 16u;
     r.ny = r.inv.y >= 0.0f ? 32u : 48u;
     r.nz = r.inv.z >= 0.0f ? 64u : 80u;
@@ -80,9 +73,11 @@ __device__ __forceinline__ ray_setup setup_ray(const float4 *__restrict__ rays, 
 __device__ __forceinline__ void slab_interval(const ray_setup &r, float lox, float hix, float loy, float hiy, float loz,
                                               float hiz, float &tn, float &tf)
 {
-    float x0 = fmaf(lox, r.inv.x, r.clo.x), x1 = fmaf(hix, r.inv.x, r.chi.x);
-    float y0 = fmaf(loy, r.inv.y, r.clo.y), y1 = fmaf(hiy, r.inv.y, r.chi.y);
-    float z0 = fmaf(loz, r.inv.z, r.clo.z), z1 = fmaf(hiz, r.inv.z, r.chi.z);
+    // (the binary-tree walk: each axis's lo plane takes the near offset where inv >= 0, the far one elsewhere)
+    const bool px = r.inv.x >= 0.0f, py = r.inv.y >= 0.0f, pz = r.inv.z >= 0.0f;
+    float x0 = fmaf(lox, r.inv.x, px ? r.cn.x : r.cf.x), x1 = fmaf(hix, r.inv.x, px ? r.cf.x : r.cn.x);
+    float y0 = fmaf(loy, r.inv.y, py ? r.cn.y : r.cf.y), y1 = fmaf(hiy, r.inv.y, py ? r.cf.y : r.cn.y);
+    float z0 = fmaf(loz, r.inv.z, pz ? r.cn.z : r.cf.z), z1 = fmaf(hiz, r.inv.z, pz ? r.cf.z : r.cn.z);
     float n_ = fmaxf(fmaxf(fminf(x0, x1), fminf(y0, y1)), fminf(z0, z1));
     float f_ = fminf(fminf(fmaxf(x0, x1), fmaxf(y0, y1)), fmaxf(z0, z1));
     // one rounding per end (a fused multiply-add): at most half an ulp against the 2^-16 |t| of the pad
@@ -98,19 +93,34 @@ static_assert(sizeof(vkrt_bvh4_node) == 128, "4-wide node layout: lo/hi float4 p
 //! value is the smaller of the two on every axis, the same value slab_interval's per-axis min picks. For a
 //! box with lo > hi (the empty box of non-finite triangles) the interval comes out empty. Node offsets are
 //! 32-bit: the build keeps the 4-wide layout only while the nodes fit 4 GiB.
+//! BVH nodes and triangle records always live in device global memory; saying so lets a pointer that a kernel
+//! read from memory (a batch item, a scene record) take global loads with a scalar base and a 32-bit lane
+//! offset instead of flat loads with a 64-bit address per lane (two VGPRs per load in flight)
+template<class T>
+using global_ptr = const __attribute__((address_space(1))) T *;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float4 load_global_f4(const void *base, size_t byte_offset)
+{
+    const f32x4 v = *(global_ptr<f32x4>)((global_ptr<char>)base + byte_offset);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ int4 load_global_i4(const void *base, size_t byte_offset)
+{
+    const i32x4 v = *(global_ptr<i32x4>)((global_ptr<char>)base + byte_offset);
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
 __device__ __forceinline__ void node4_intervals(const vkrt_bvh4_node *__restrict__ nodes, int32_t ref,
                                                 const ray_setup &r, float tn[4], float tf[4], int4 &child)
 {
-    const char *base = reinterpret_cast<const char *>(nodes);
     const uint32_t nb = static_cast<uint32_t>(ref) * static_cast<uint32_t>(sizeof(vkrt_bvh4_node));
     const uint32_t ax = nb + r.nx, ay = nb + r.ny, az = nb + r.nz;
-    const float4 nx = *reinterpret_cast<const float4 *>(base + ax);
-    const float4 fx = *reinterpret_cast<const float4 *>(base + (ax ^ 16u));
-    const float4 ny = *reinterpret_cast<const float4 *>(base + ay);
-    const float4 fy = *reinterpret_cast<const float4 *>(base + (ay ^ 16u));
-    const float4 nz = *reinterpret_cast<const float4 *>(base + az);
-    const float4 fz = *reinterpret_cast<const float4 *>(base + (az ^ 16u));
-    child = *reinterpret_cast<const int4 *>(base + nb + 96u);
+    const float4 nx = load_global_f4(nodes, ax), fx = load_global_f4(nodes, ax ^ 16u);
+    const float4 ny = load_global_f4(nodes, ay), fy = load_global_f4(nodes, ay ^ 16u);
+    const float4 nz = load_global_f4(nodes, az), fz = load_global_f4(nodes, az ^ 16u);
+    child = load_global_i4(nodes, nb + 96u);
     const float n0[4] = {nx.x, nx.y, nx.z, nx.w}, f0[4] = {fx.x, fx.y, fx.z, fx.w};
     const float n1[4] = {ny.x, ny.y, ny.z, ny.w}, f1[4] = {fy.x, fy.y, fy.z, fy.w};
     const float n2[4] = {nz.x, nz.y, nz.z, nz.w}, f2[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -218,8 +228,9 @@ __device__ __forceinline__ bool traverse(const vkrt_bvh_node *__restrict__ nodes
             leaf_decode(ref, first, count);
             for(uint32_t k = 0; k < count; ++k)
             {
-                const float4 *tr = tris + 3 * (size_t)(first + k);
-                float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+                const size_t tb = 48 * (size_t)(first + k);
+                float4 p0 = load_global_f4(tris, tb), p1 = load_global_f4(tris, tb + 16),
+                       p2 = load_global_f4(tris, tb + 32);
                 float t, u, v;
                 const bool cull = cull_back && !(__float_as_uint(p2.w) & TRI_TWO_SIDED);
                 if(vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
@@ -256,8 +267,9 @@ __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32
     leaf_decode(ref, first, count);
     for(uint32_t k = 0; k < count; ++k)
     {
-        const float4 *tr = tris + 3 * (size_t)(first + k);
-        float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
+        // (a 64-bit offset: triangle records may pass 4 GiB, where a 4-wide node array may not)
+        const size_t tb = 48 * (size_t)(first + k);
+        float4 p0 = load_global_f4(tris, tb), p1 = load_global_f4(tris, tb + 16), p2 = load_global_f4(tris, tb + 32);
         float t, u, v;
         const bool cull = cull_back && !(__float_as_uint(p2.w) & TRI_TWO_SIDED);
         if(vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y}, vec3{p1.z, p1.w, p2.x}, t,
@@ -377,18 +389,19 @@ __device__ __forceinline__ bool step4(const vkrt_bvh4_node *__restrict__ nodes, 
     return false;
 }
 
-//! children of 4-wide node `ref` whose padded interval meets [tmin, best_t], nearest first; returns the count
+//! children of 4-wide node `ref` whose padded interval meets [lo, best_t] (lo: the ray's tmin unless given),
+//! nearest first; returns the count
 __device__ __forceinline__ int visit4(const vkrt_bvh4_node *__restrict__ nodes, int32_t ref, const ray_setup &r,
-                                      float best_t, int32_t c[4])
+                                      float best_t, int32_t c[4], float lo)
 {
     float k[4], tf[4];
     int4 cr;
     node4_intervals(nodes, ref, r, k, tf, cr);
     bool v[4];
-    v[0] = child_hit(cr.x, k[0], tf[0], r.tmin, best_t);
-    v[1] = child_hit(cr.y, k[1], tf[1], r.tmin, best_t);
-    v[2] = child_hit(cr.z, k[2], tf[2], r.tmin, best_t);
-    v[3] = child_hit(cr.w, k[3], tf[3], r.tmin, best_t);
+    v[0] = child_hit(cr.x, k[0], tf[0], lo, best_t);
+    v[1] = child_hit(cr.y, k[1], tf[1], lo, best_t);
+    v[2] = child_hit(cr.z, k[2], tf[2], lo, best_t);
+    v[3] = child_hit(cr.w, k[3], tf[3], lo, best_t);
     c[0] = cr.x;
     c[1] = cr.y;
     c[2] = cr.z;
@@ -401,6 +414,12 @@ __device__ __forceinline__ int visit4(const vkrt_bvh4_node *__restrict__ nodes, 
     compare_exchange(q[1], c[1], q[2], c[2]);
     return (q[0] < KEY_MISSED ? 1 : 0) + (q[1] < KEY_MISSED ? 1 : 0) + (q[2] < KEY_MISSED ? 1 : 0) +
            (q[3] < KEY_MISSED ? 1 : 0);
+}
+
+__device__ __forceinline__ int visit4(const vkrt_bvh4_node *__restrict__ nodes, int32_t ref, const ray_setup &r,
+                                      float best_t, int32_t c[4])
+{
+    return visit4(nodes, ref, r, best_t, c, r.tmin);
 }
 
 template<int BLOCK, int LDS_DEPTH>
