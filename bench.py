@@ -210,7 +210,7 @@ def pt_vs_integrator(ctx, scene, cam, w=480, h=270, spp=16):
             "cpu_integrator_s": round(cpu_s, 2), "cpu_threads": host_threads()[0]}
 
 
-def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=64, max_train_records=16384):
+def radiance_cache_run(ctx, ref_spp, max_frames, cache_depth=2, train_stride=512, max_train_records=4096):
     """time to equal RMSE on the room scene at 1920x1080, max_trace_depth 8. The target is plain path
     tracing at 64 spp, timed as one 64-spp batch (its fastest form). The reference image is plain path
     tracing at ref_spp. Cache frames trace 1 spp: paths that are not training paths stop at hit depth

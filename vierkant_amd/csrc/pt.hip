@@ -1637,15 +1637,20 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         }
         k_pt_begin<<<blocks, PT_BLOCK, 0, s>>>(f, b, sample);
         const bool fork = f.cache_on && f.n_train_paths;
+        // the deep-bounce tail (not in the cache mode, whose main paths end at the cache depth and queue cache
+        // queries): the survivors at tail_depth finish in one lane each
+        const uint32_t per_depth = f.cache_on ? main_depths : std::min(main_depths, pt->tail_depth);
+        if(fork) VKRT_HIP(hipEventRecord(pt->ev_fork, s));
+        // the first depth is queued before the training stream's launches, so the main stream does not wait
+        // for the host to queue them
+        if(per_depth > 0) depth_step(s, b, 0, 0);
         if(fork)
         {
             // the training paths' long, narrow tail is latency-bound: it runs on the second stream while the
-            // main stream traces the full-width depths and queries the cache
-            VKRT_HIP(hipEventRecord(pt->ev_fork, s));
+            // main stream traces the full-width depths, followed there by the training step (targets, gradients,
+            // Adam), which no main-path kernel reads before the cache query; the query waits for it, so the
+            // main paths terminate into the cache as trained on this sample's training paths
             VKRT_HIP(hipStreamWaitEvent(pt->train_stream, pt->ev_fork, 0));
-        }
-        if(fork)
-        {
             const uint32_t tgrid = div_up(f.n_train_paths, PT_BLOCK);
             if(f.n_lights)
                 k_pt_train_paths<true><<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(
@@ -1653,13 +1658,15 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
             else
                 k_pt_train_paths<false><<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(
                         nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, bt, sc);
+            if(f.train_max)
+            {
+                k_pt_train_targets<<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(f, b);
+                rc::train(pt->rc, b.trec, 4, b.ttgt, f.train_max, false, b.counter + 5, pt->train_stream);
+            }
             VKRT_HIP(hipGetLastError());
             VKRT_HIP(hipEventRecord(pt->ev_join, pt->train_stream));
         }
-        // the deep-bounce tail (not in the cache mode, whose main paths end at the cache depth and queue cache
-        // queries): the survivors at tail_depth finish in one lane each
-        const uint32_t per_depth = f.cache_on ? main_depths : std::min(main_depths, pt->tail_depth);
-        for(uint32_t depth = 0; depth < per_depth; ++depth) depth_step(s, b, depth, static_cast<int>(depth & 1u));
+        for(uint32_t depth = 1; depth < per_depth; ++depth) depth_step(s, b, depth, static_cast<int>(depth & 1u));
         if(per_depth < main_depths && per_depth > 0)
         {
             const int in = static_cast<int>(per_depth & 1u);
@@ -1682,13 +1689,8 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         }
         if(f.cache_on)
         {
-            rc::query(pt->rc, b.qrec, QUERY_F4, f.n_pix, nullptr, b.acc, b.counter + 4);
             if(fork) VKRT_HIP(hipStreamWaitEvent(s, pt->ev_join, 0));
-            if(f.n_train_paths && f.train_max)
-            {
-                k_pt_train_targets<<<div_up(f.n_train_paths, PT_BLOCK), PT_BLOCK, 0, s>>>(f, b);
-                rc::train(pt->rc, b.trec, 4, b.ttgt, f.train_max, false, b.counter + 5);
-            }
+            rc::query(pt->rc, b.qrec, QUERY_F4, f.n_pix, nullptr, b.acc, b.counter + 4);
         }
     }
     k_pt_resolve<<<blocks, PT_BLOCK, 0, s>>>(f, b);

@@ -334,13 +334,18 @@ __global__ void k_rc_publish_mlp(const float *__restrict__ master, __bf16 *__res
     if(k < N_MLP) publish_weight(k, master[k], w_fwd, w_bwd);
 }
 
+//! (the gradient entries it reads as non-zero are set back to zero, so the next step starts from a clear buffer
+//! without a fill)
 __global__ void k_rc_adam_grid(float *__restrict__ master, float *__restrict__ mom, float *__restrict__ vel,
-                               const _Float16 *__restrict__ grad, float inv_scale, __half *__restrict__ stored,
+                               _Float16 *__restrict__ grad, float inv_scale, __half *__restrict__ stored,
                                uint64_t n, adam_consts a)
 {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if(k >= n) return;
-    const float g = static_cast<float>(grad[k]) * inv_scale;
+    const _Float16 raw = grad[k];
+    if(raw == static_cast<_Float16>(0.0f)) return;
+    grad[k] = static_cast<_Float16>(0.0f);
+    const float g = static_cast<float>(raw) * inv_scale;
     // untouched entries keep their parameters and moments; a non-finite (overflowed) gradient is dropped
     if(g == 0.0f || !isfinite(g)) return;
     float p = master[k], m = mom[k], v = vel[k];
@@ -403,13 +408,14 @@ __global__ void k_rc_grid_grad_f32(const _Float16 *__restrict__ g, float inv_sca
 }
 
 void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only,
-           const uint32_t *n_dev)
+           const uint32_t *n_dev, hipStream_t s)
 {
-    hipStream_t s = c->ctx->stream;
     const uint32_t nb = train_blocks(c, n);
     c->partials.reserve((size_t)nb * N_MLP * 4);
     c->grad_scale = grid_grad_scale(n);
-    fill(s, c->grid_grad.as<float>(), c->n_grid / 2, 0.0f);// n_grid fp16 values
+    if(!c->grid_grad_clear) fill(s, c->grid_grad.as<float>(), c->n_grid / 2, 0.0f);// n_grid fp16 values
+    // a gradient-only step leaves its gradients for reading; a full step's Adam pass clears them
+    c->grid_grad_clear = !grad_only;
     fill(s, c->loss.as<float>(), 1, 0.0f);
     if(n)
         k_rc_train<<<nb, TRAIN_BLOCK, 0, s>>>(device_view(c), records, stride_f4, targets, n, n_dev,
@@ -634,7 +640,7 @@ int vkrt_rc_train(vkrt_rc *c, const float *records, const float *targets, uint32
         VKRT_REQUIRE(n == 0 || (records && targets), "null records or targets");
         const float4 *rec = reinterpret_cast<const float4 *>(input(ctx, 0, records, (size_t)n * 16, flags));
         const float4 *tg = reinterpret_cast<const float4 *>(input(ctx, 1, targets, (size_t)n * 4, flags));
-        rc::train(c, rec, RECORD_F4, tg, n, (flags & VKRT_RC_GRAD_ONLY) != 0, nullptr);
+        rc::train(c, rec, RECORD_F4, tg, n, (flags & VKRT_RC_GRAD_ONLY) != 0, nullptr, c->ctx->stream);
         if(loss)
         {
             float sum = 0.f;

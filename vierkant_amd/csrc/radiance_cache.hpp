@@ -280,6 +280,8 @@ struct vkrt_rc
     uint32_t step = 0;            // Adam steps taken
     float grad_scale = 1.0f;      // loss scale of the last step's fp16 grid gradient
     vkrt::device_buffer grid_master, grid_half, grid_grad, grid_grad_f32, grid_m, grid_v;
+    // grid_grad holds zeros: the last training step's Adam pass cleared every entry it read as non-zero
+    bool grid_grad_clear = false;
     vkrt::device_buffer mlp_master, mlp_grad, mlp_m, mlp_v, w_fwd, w_bwd, partials, loss;
 };
 
@@ -293,9 +295,9 @@ inline cache_dev device_view(const vkrt_rc *c)
 
 //! one training step over `n` device records (RECORD_F4 float4 each, `stride_f4` apart) with targets
 //! (float4 rgb, -): gradients, then Adam unless grad_only. With n_dev the count is min(*n_dev, n), read on
-//! the device (no host synchronization).
+//! the device (no host synchronization). Launched on `stream` (the caller orders it against queries).
 void train(vkrt_rc *c, const float4 *records, uint32_t stride_f4, const float4 *targets, uint32_t n, bool grad_only,
-           const uint32_t *n_dev);
+           const uint32_t *n_dev, hipStream_t stream);
 
 //! predictions max(y_rgb, 0): written to out[i] (float4), or, when pixel_acc is set, added as
 //! beta * prediction to pixel_acc[pixel] with pixel and beta taken from the record's fifth float4
