@@ -181,6 +181,49 @@ def test_load_errors_carry_a_message(tmp_path):
         vk.load_model(write(tmp_path / "x.fbx", ""))
 
 
+@pytest.mark.parametrize("name, edit, message", [
+    ("negative count", lambda d: d["accessors"][0].update(count=-1), "invalid accessor count"),
+    ("fractional count", lambda d: d["accessors"][0].update(count=2.5), "invalid accessor count"),
+    ("count past the view", lambda d: d["accessors"][0].update(count=4), "past its bufferView"),
+    ("huge count", lambda d: d["accessors"][0].update(count=2 ** 40), "count above"),
+    ("offset past the view", lambda d: d["accessors"][0].update(byteOffset=28), "past its bufferView"),
+    ("negative offset", lambda d: d["accessors"][0].update(byteOffset=-12), "invalid accessor byteOffset"),
+    ("stride below the element", lambda d: d["bufferViews"][0].update(byteStride=8), "byteStride below"),
+    ("stride overflowing", lambda d: d["bufferViews"][0].update(byteStride=2 ** 39), "past its bufferView"),
+    ("view past the buffer", lambda d: d["bufferViews"][2].update(byteLength=4096), "past its buffer"),
+    ("view offset past the buffer", lambda d: d["bufferViews"][1].update(byteOffset=2 ** 36), "past its buffer"),
+    ("negative view length", lambda d: d["bufferViews"][1].update(byteLength=-36),
+     "invalid bufferView byteLength"),
+    ("missing buffer", lambda d: d["bufferViews"][0].update(buffer=3), "buffer out of range"),
+])
+def test_gltf_malformed_accessors_are_rejected(tmp_path, name, edit, message):
+    """accessor and bufferView bounds: negative, fractional and oversized sizes, strides below the element,
+    ranges past the bufferView or the buffer (checked without overflow) fail with a message, never read out
+    of bounds"""
+    data = gltf_bin()
+    (tmp_path / "s.bin").write_bytes(data)
+    doc = gltf_doc("s.bin", len(data))
+    edit(doc)
+    with pytest.raises(vk.VkrtError, match=message):
+        vk.load_model(write(tmp_path / "s.gltf", json.dumps(doc)))
+
+
+def test_gltf_strided_view_within_bounds(tmp_path):
+    """a bufferView holding positions and normals interleaved (byteStride 24) reads both correctly, and an
+    accessor ending exactly at the view's end is accepted"""
+    inter = np.concatenate([TRI, NRM], axis=1).astype(np.float32)
+    data = inter.tobytes() + np.uint16([0, 1, 2]).tobytes()
+    data += b"\0" * (-len(data) % 4)
+    (tmp_path / "s.bin").write_bytes(data)
+    doc = gltf_doc("s.bin", len(data))
+    doc["bufferViews"] = [{"buffer": 0, "byteOffset": 0, "byteLength": 72, "byteStride": 24},
+                          {"buffer": 0, "byteOffset": 72, "byteLength": 6}]
+    doc["accessors"][0].update(bufferView=0, byteOffset=0)
+    doc["accessors"][1].update(bufferView=0, byteOffset=12)
+    doc["accessors"][2].update(bufferView=1)
+    check_gltf_scene(vk.load_model(write(tmp_path / "s.gltf", json.dumps(doc))))
+
+
 def test_obj_round_trip_of_the_room_scene(tmp_path):
     """the room scene written as OBJ (shortest float repr, per-vertex normals, one shape and material per
     object) loads back with the same triangles, normals and object grouping"""

@@ -934,6 +934,16 @@ struct gltf_file
     std::vector<std::vector<uint8_t>> buffers;
 };
 
+//! a JSON size field: absent gives `def`; negative, fractional, non-finite or above 2^40 values are rejected
+size_t json_size(const json &v, size_t def, const char *what)
+{
+    if(v.kind != json::number_t) return def;
+    const double x = v.num;
+    if(!(x >= 0.0) || x > 1099511627776.0 || x != std::floor(x))
+        throw load_error(std::string("glTF: invalid ") + what);
+    return static_cast<size_t>(x);
+}
+
 gltf_file read_gltf(const std::string &path)
 {
     const std::string bytes = read_file(path), base = directory_of(path);
@@ -970,7 +980,7 @@ gltf_file read_gltf(const std::string &path)
     for(size_t i = 0; i < bufs.size(); ++i)
     {
         const json &b = bufs[i];
-        const size_t len = static_cast<size_t>(b["byteLength"].number(0));
+        const size_t len = json_size(b["byteLength"], 0, "buffer byteLength");
         std::vector<uint8_t> data;
         if(!b.has("uri"))
         {
@@ -999,6 +1009,25 @@ gltf_file read_gltf(const std::string &path)
     return g;
 }
 
+//! [offset, offset + length) of a bufferView inside its buffer, checked without overflow
+struct view_range
+{
+    const std::vector<uint8_t> *buf;
+    size_t offset, length;
+};
+
+view_range buffer_view(const gltf_file &g, const json &bv)
+{
+    if(bv.kind != json::object_t) throw load_error("glTF: bufferView out of range");
+    const int bi = bv["buffer"].integer(-1);
+    if(bi < 0 || bi >= static_cast<int>(g.buffers.size())) throw load_error("glTF: buffer out of range");
+    const std::vector<uint8_t> &buf = g.buffers[bi];
+    const size_t off = json_size(bv["byteOffset"], 0, "bufferView byteOffset");
+    const size_t len = json_size(bv["byteLength"], 0, "bufferView byteLength");
+    if(off > buf.size() || len > buf.size() - off) throw load_error("glTF: bufferView past its buffer");
+    return {&buf, off, len};
+}
+
 //! one accessor as doubles (components per element in `comps`); float, and u8/u16/u32 for indices
 std::vector<double> read_accessor(const gltf_file &g, int index, int &comps)
 {
@@ -1010,22 +1039,26 @@ std::vector<double> read_accessor(const gltf_file &g, int index, int &comps)
     const int ct = acc["componentType"].integer(0);
     const size_t csize = ct == 5126 || ct == 5125 ? 4 : ct == 5123 || ct == 5122 ? 2 : ct == 5121 || ct == 5120 ? 1 : 0;
     if(!csize) throw load_error("glTF: unsupported component type");
-    const size_t count = static_cast<size_t>(acc["count"].number(0));
-    std::vector<double> out(count * comps, 0.0);
-    if(!acc.has("bufferView")) return out;// all zeros (sparse accessors are not supported)
-    const json &bv = g.doc["bufferViews"][acc["bufferView"].integer(-1)];
-    if(bv.kind != json::object_t) throw load_error("glTF: bufferView out of range");
-    const int bi = bv["buffer"].integer(-1);
-    if(bi < 0 || bi >= static_cast<int>(g.buffers.size())) throw load_error("glTF: buffer out of range");
-    const std::vector<uint8_t> &buf = g.buffers[bi];
+    const size_t count = json_size(acc["count"], 0, "accessor count");
+    if(count > (size_t(1) << 28)) throw load_error("glTF: accessor count above 2^28");
+    if(!acc.has("bufferView")) return std::vector<double>(count * comps, 0.0);// sparse accessors are not supported
+    const view_range view = buffer_view(g, g.doc["bufferViews"][acc["bufferView"].integer(-1)]);
     const size_t elem = csize * comps;
-    const size_t stride = bv.has("byteStride") ? static_cast<size_t>(bv["byteStride"].number(0)) : elem;
-    const size_t start = static_cast<size_t>(bv["byteOffset"].number(0)) + static_cast<size_t>(acc["byteOffset"].number(0));
-    if(count && start + (count - 1) * stride + elem > buf.size()) throw load_error("glTF: accessor past its buffer");
+    const size_t stride = json_size(g.doc["bufferViews"][acc["bufferView"].integer(-1)]["byteStride"], elem,
+                                    "bufferView byteStride");
+    if(stride < elem) throw load_error("glTF: byteStride below the element size");
+    const size_t acc_off = json_size(acc["byteOffset"], 0, "accessor byteOffset");
+    // the last element ends at acc_off + (count - 1) stride + elem, which must lie inside the view; checked
+    // as a division, so that no product can overflow
+    if(count && (acc_off > view.length || elem > view.length - acc_off ||
+                 (count - 1) > (view.length - acc_off - elem) / stride))
+        throw load_error("glTF: accessor past its bufferView");
+    std::vector<double> out(count * comps, 0.0);
+    const uint8_t *start = view.buf->data() + view.offset + acc_off;
     for(size_t i = 0; i < count; ++i)
         for(int c = 0; c < comps; ++c)
         {
-            const uint8_t *q = buf.data() + start + i * stride + c * csize;
+            const uint8_t *q = start + i * stride + c * csize;
             double x = 0;
             switch(ct)
             {
@@ -1207,12 +1240,17 @@ std::unique_ptr<vkrt_model> load_gltf(const std::string &path)
         uint32_t id = 0;
         if(im.has("bufferView"))
         {
-            const json &bv = doc["bufferViews"][static_cast<size_t>(im["bufferView"].integer(0))];
-            const int bi = bv["buffer"].integer(-1);
-            const size_t o = static_cast<size_t>(bv["byteOffset"].number(0)), n = static_cast<size_t>(bv["byteLength"].number(0));
-            if(bi >= 0 && bi < static_cast<int>(g.buffers.size()) && o + n <= g.buffers[bi].size())
-                id = model->add_image(g.buffers[bi].data() + o, n);
-            else ++model->n_skipped_textures;
+            // an image whose bufferView is malformed is skipped like an undecodable one
+            try
+            {
+                const view_range view =
+                        buffer_view(g, doc["bufferViews"][static_cast<size_t>(im["bufferView"].integer(0))]);
+                id = model->add_image(view.buf->data() + view.offset, view.length);
+            }
+            catch(const load_error &)
+            {
+                ++model->n_skipped_textures;
+            }
         }
         else if(im.has("uri"))
         {

@@ -42,32 +42,47 @@ struct blas_dev
     uint32_t empty;
 };
 
-template<bool ANY, int BLOCK, int LDS_DEPTH>
-__global__ void __launch_bounds__(BLOCK, 6) k_scene_trace(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
-                                                       uint32_t empty_scene, const uint32_t *__restrict__ order,
-                                                       const inst_dev *__restrict__ insts,
-                                                       const blas_dev *__restrict__ blases,
-                                                       const float4 *__restrict__ rays, uint32_t n,
-                                                       void *__restrict__ out, uint32_t chunk)
+//! object-space setup of world ray (ra, rb) for instance I: each row ((w0 x + w1 y) + w2 z) + w3, the direction
+//! keeping its length (so t is shared by both spaces)
+__device__ __forceinline__ ray_setup object_ray(const inst_dev &I, const float4 &ra, const float4 &rb)
 {
-    __shared__ int32_t smem[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+    const float ox = ((I.w0.x * ra.x + I.w0.y * ra.y) + I.w0.z * ra.z) + I.w0.w;
+    const float oy = ((I.w1.x * ra.x + I.w1.y * ra.y) + I.w1.z * ra.z) + I.w1.w;
+    const float oz = ((I.w2.x * ra.x + I.w2.y * ra.y) + I.w2.z * ra.z) + I.w2.w;
+    const float dx = (I.w0.x * rb.x + I.w0.y * rb.y) + I.w0.z * rb.z;
+    const float dy = (I.w1.x * rb.x + I.w1.y * rb.y) + I.w1.z * rb.z;
+    const float dz = (I.w2.x * rb.x + I.w2.y * rb.y) + I.w2.z * rb.z;
+    return make_ray_setup(make_float4(ox, oy, oz, ra.w), make_float4(dx, dy, dz, rb.w));
+}
+
+//! closest (or, ANY, first) hit of world ray (ra, rb) in a two-level scene, in one loop over both levels:
+//! top-level and mesh nodes share the 4-wide format, so a lane visits a node of whichever level it is in and
+//! the wave stays converged across instance boundaries. Entering an instance leaf switches the node array,
+//! triangles and ray to that instance's mesh and object space and remembers the stack depth; when the walk
+//! pops back to that depth the mesh is done and the world ray is set up again. Top-level leaves hold one
+//! instance each. Ties: smallest t, then the lowest instance, then the lowest triangle.
+template<bool ANY, int BLOCK, int LDS_DEPTH>
+__device__ __forceinline__ void scene_walk(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
+                                           const uint32_t *__restrict__ order, const inst_dev *__restrict__ insts,
+                                           const blas_dev *__restrict__ blases, const float4 &ra, const float4 &rb,
+                                           int32_t *lds, float &best_t, float &best_u, float &best_v,
+                                           uint32_t &best_prim, uint32_t &best_inst)
+{
     int32_t spill[SCENE_STACK - LDS_DEPTH];
-    const uint32_t i = xcd_block(blockIdx.x, chunk) * BLOCK + threadIdx.x;
-    if(i >= n) return;
-    lane_stack<BLOCK, LDS_DEPTH> st{smem + threadIdx.x, spill};
-    const float4 ra = rays[2 * (size_t)i], rb = rays[2 * (size_t)i + 1];
-    const ray_setup rw = make_ray_setup(ra, rb);
-    float best_t = rw.tmax, best_u = 0.f, best_v = 0.f;
-    uint32_t best_prim = VKRT_INVALID_PRIM, best_inst = VKRT_INVALID_PRIM;
-    bool found_any = false;
+    lane_stack<BLOCK, LDS_DEPTH> st{lds, spill};
+    ray_setup r = make_ray_setup(ra, rb);
+    const vkrt_bvh4_node *nodes = tlas;
+    const float4 *tris = nullptr;
+    uint32_t inst = VKRT_INVALID_PRIM;
+    int base = -1;// stack depth at which the current mesh walk began (-1: at the top level)
     int sp = 0;
     int32_t ref = troot;
-    while(!empty_scene)
+    while(true)
     {
         if(ref >= 0)
         {
             int32_t c[4];
-            const int cnt = visit4(tlas, ref, rw, best_t, c);
+            const int cnt = visit4(nodes, ref, r, best_t, c);
             if(cnt)
             {
                 for(int j = cnt - 1; j >= 1; --j) st.put(sp++, c[j]);
@@ -75,80 +90,83 @@ __global__ void __launch_bounds__(BLOCK, 6) k_scene_trace(const vkrt_bvh4_node *
                 continue;
             }
         }
-        else
+        else if(base < 0)
         {
             uint32_t first, count;
             leaf_decode(ref, first, count);
-            for(uint32_t q = 0; q < count && !found_any; ++q)
+            inst = order[first];
+            const inst_dev I = insts[inst];
+            const blas_dev B = blases[I.blas];
+            if(!B.empty)
             {
-                const uint32_t inst = order[first + q];
-                const inst_dev I = insts[inst];
-                const blas_dev B = blases[I.blas];
-                if(B.empty) continue;
-                // object-space ray: each row ((w0 x + w1 y) + w2 z) + w3; the direction keeps its length
-                const float ox = ((I.w0.x * ra.x + I.w0.y * ra.y) + I.w0.z * ra.z) + I.w0.w;
-                const float oy = ((I.w1.x * ra.x + I.w1.y * ra.y) + I.w1.z * ra.z) + I.w1.w;
-                const float oz = ((I.w2.x * ra.x + I.w2.y * ra.y) + I.w2.z * ra.z) + I.w2.w;
-                const float dx = (I.w0.x * rb.x + I.w0.y * rb.y) + I.w0.z * rb.z;
-                const float dy = (I.w1.x * rb.x + I.w1.y * rb.y) + I.w1.z * rb.z;
-                const float dz = (I.w2.x * rb.x + I.w2.y * rb.y) + I.w2.z * rb.z;
-                const ray_setup ro = make_ray_setup(make_float4(ox, oy, oz, ra.w), make_float4(dx, dy, dz, rb.w));
-                const int base = sp;
-                int bsp = sp;
-                int32_t bref = B.root;
-                while(true)
+                r = object_ray(I, ra, rb);
+                nodes = B.nodes;
+                tris = B.tris;
+                base = sp;
+                ref = B.root;
+                continue;
+            }
+        }
+        else
+        {
+            uint32_t bf, bc;
+            leaf_decode(ref, bf, bc);
+            bool found = false;
+            for(uint32_t k = 0; k < bc; ++k)
+            {
+                const size_t tb = 48 * (size_t)(bf + k);
+                const float4 p0 = load_global_f4(tris, tb), p1 = load_global_f4(tris, tb + 16),
+                             p2 = load_global_f4(tris, tb + 32);
+                float t, u, v;
+                if(vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
+                                         vec3{p1.z, p1.w, p2.x}, t, u, v) &&
+                   t > r.tmin)
                 {
-                    if(bref >= 0)
+                    const uint32_t prim = __float_as_uint(p2.y);
+                    if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM &&
+                                      (inst < best_inst || (inst == best_inst && prim < best_prim))))
                     {
-                        int32_t c[4];
-                        const int cnt = visit4(B.nodes, bref, ro, best_t, c);
-                        if(cnt)
-                        {
-                            for(int j = cnt - 1; j >= 1; --j) st.put(bsp++, c[j]);
-                            bref = c[0];
-                            continue;
-                        }
+                        best_t = t;
+                        best_u = u;
+                        best_v = v;
+                        best_prim = prim;
+                        best_inst = inst;
+                        found = true;
                     }
-                    else
-                    {
-                        uint32_t bf, bc;
-                        leaf_decode(bref, bf, bc);
-                        for(uint32_t k = 0; k < bc; ++k)
-                        {
-                            const float4 *tr = B.tris + 3 * (size_t)(bf + k);
-                            const float4 p0 = tr[0], p1 = tr[1], p2 = tr[2];
-                            float t, u, v;
-                            if(vierkant::rt::mt_fast(ro.o, ro.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
-                                                     vec3{p1.z, p1.w, p2.x}, t, u, v) &&
-                               t > ro.tmin)
-                            {
-                                const uint32_t prim = __float_as_uint(p2.y);
-                                if(t < best_t ||
-                                   (t == best_t && best_prim != VKRT_INVALID_PRIM &&
-                                    (inst < best_inst || (inst == best_inst && prim < best_prim))))
-                                {
-                                    best_t = t;
-                                    best_u = u;
-                                    best_v = v;
-                                    best_prim = prim;
-                                    best_inst = inst;
-                                    if(ANY) found_any = true;
-                                }
-                            }
-                        }
-                        if(ANY && found_any) break;
-                    }
-                    if(bsp == base) break;
-                    --bsp;
-                    bref = st.get(bsp);
                 }
             }
-            if(ANY && found_any) break;
+            if(ANY && found) return;
         }
-        if(sp == 0) break;
+        if(sp == base)
+        {
+            // the mesh walk is done: back to the top level
+            r = make_ray_setup(ra, rb);
+            nodes = tlas;
+            base = -1;
+        }
+        if(sp == 0) return;
         --sp;
         ref = st.get(sp);
     }
+}
+
+template<bool ANY, int BLOCK, int LDS_DEPTH>
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_scene_trace(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
+                                                       uint32_t empty_scene, const uint32_t *__restrict__ order,
+                                                       const inst_dev *__restrict__ insts,
+                                                       const blas_dev *__restrict__ blases,
+                                                       const float4 *__restrict__ rays, uint32_t n,
+                                                       void *__restrict__ out, uint32_t chunk)
+{
+    __shared__ int32_t smem[LDS_DEPTH > 0 ? LDS_DEPTH * BLOCK : 1];
+    const uint32_t i = xcd_block(blockIdx.x, chunk) * BLOCK + threadIdx.x;
+    if(i >= n) return;
+    const float4 ra = rays[2 * (size_t)i], rb = rays[2 * (size_t)i + 1];
+    float best_t = rb.w, best_u = 0.f, best_v = 0.f;
+    uint32_t best_prim = VKRT_INVALID_PRIM, best_inst = VKRT_INVALID_PRIM;
+    if(!empty_scene)
+        scene_walk<ANY, BLOCK, LDS_DEPTH>(tlas, troot, order, insts, blases, ra, rb, smem + threadIdx.x, best_t,
+                                          best_u, best_v, best_prim, best_inst);
     const bool hit = best_prim != VKRT_INVALID_PRIM;
     if(ANY) static_cast<uint8_t *>(out)[i] = hit ? 1 : 0;
     else

@@ -294,7 +294,9 @@ __device__ __forceinline__ bool test_leaf(const float4 *__restrict__ tris, int32
 //! sort key of a 4-wide node's child: +inf when the ray misses it, else its entry distance clamped below
 //! +inf (so a hit child never sorts with the missed ones)
 constexpr float KEY_MISSED = __builtin_huge_valf();
-__device__ __forceinline__ float sort_key(bool hit, float k) { return hit ? fminf(k, 3.40282347e38f) : KEY_MISSED; }
+//! a child that meets the interval has an entry below +inf (an entry n = +inf pads to fma(-inf, T_PAD, inf) =
+//! NaN, which fails every comparison of child_hit), so its own entry is its key
+__device__ __forceinline__ float sort_key(bool hit, float k) { return hit ? k : KEY_MISSED; }
 
 //! does the ray's padded interval [tn, tf] of child c meet [tmin, best_t]? The terms are joined with '&',
 //! not '&&': a short-circuit puts the far-side reduction in a branch of its own per child, where the
