@@ -24,16 +24,27 @@ MAX_LEAF = 2                 # triangles per BVH leaf (the library default; fast
 ANY_BYTES_PER_RAY = 33       # 32 B ray in + 1 B flag out
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (profiles/k_trace_heightfield_spheres.json, each counter scaled by its plain-copy calibration), or None"""
+# wave64 VALU instructions per second: 1,024 SIMDs of 32 FP32 lanes (157.3 TFLOP/s = 256 CUs x 4 SIMDs x 32
+# lanes x 2 flop x 2.4 GHz), 2 cycles per instruction; the brute-force kernel sustains 8.1e11/s, above what
+# 16-lane SIMDs (4 cycles, 6.1e11/s) could issue
+MI355X_VALU_ISSUE = 1024 * 2.4e9 / 2
+
+
+def profiled(kernel, field):
+    """`field` of `kernel` from the committed rocprofv3 summary (profiles/k_trace_heightfield_spheres.json:
+    HBM bytes per launch from FETCH_SIZE / WRITE_SIZE, each scaled by its plain-copy calibration; VALU
+    instructions per launch from SQ_INSTS_VALU), or None"""
     path = os.path.join(ROOT, "profiles", "k_trace_heightfield_spheres.json")
     try:
         with open(path) as f:
             data = json.load(f)
-        return data["kernels"][kernel]["bytes_per_launch"], data["source"]
+        return data["kernels"][kernel][field], data["source"]
     except (OSError, KeyError, ValueError):
         return None, None
+
+
+def profiled_traffic(kernel):
+    return profiled(kernel, "bytes_per_launch")
 
 
 def rays_per_step_of(primary, bounce):
@@ -718,6 +729,7 @@ def main():
 
     launch_ms = float(np.mean(prim_ms + bounce_ms))
     traffic, traffic_source = profiled_traffic("k_trace_closest")
+    valu, _ = profiled("k_trace_closest", "valu_insts_per_launch")
     achieved = CLOSEST_BYTES_PER_RAY * len(primary) / (launch_ms * 1e-3) / 1e9
     result = {
         "metric": "Grays/s closest-hit (1 GPU); time-to-equal-RMSE: radiance-cache vs plain PT",
@@ -761,6 +773,11 @@ def main():
             "traffic_source": traffic_source,
             "algorithmic_bytes_per_launch": CLOSEST_BYTES_PER_RAY * len(primary),
             "avg_launch_ms": round(launch_ms, 4),
+            # the second bound: VALU issue (the profile's instruction count per launch over this run's
+            # mean launch time, against 1,024 SIMDs issuing one wave64 instruction per 2 cycles)
+            "valu_issue": None if valu is None else {
+                "insts_per_launch": round(valu), "issue_peak_per_s": MI355X_VALU_ISSUE,
+                "frac": round(valu / (launch_ms * 1e-3) / MI355X_VALU_ISSUE, 4)},
         },
         "cpu_baseline": None,
     }
