@@ -624,6 +624,31 @@ int vkrt_pt_set_settings(vkrt_pt *pt, const vkrt_pt_settings *settings);
 int vkrt_pt_get_settings(const vkrt_pt *pt, vkrt_pt_settings *out);
 /* Upload the scene and build its BVH (the RayBuilder step of the reference's render_scene). */
 int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *scene);
+
+/* An instance of a mesh of an instanced scene: entry_t's transform (include/vierkant/RayBuilder.hpp:24-38,
+ * shaders/slang/ray/types.slang:17-32; a vierkant::transform_t, include/vierkant/transform.hpp): points map
+ * to rotate(rotation, scale * p) + translation, normals and tangents turn by the rotation
+ * (raypipeline.slang:336-358). rotation is a unit quaternion (x, y, z, w). */
+typedef struct vkrt_pt_instance
+{
+    float translation[3];
+    float rotation[4];
+    float scale[3];
+    uint32_t mesh; /* index into the scene's meshes */
+    uint32_t reserved[3];
+} vkrt_pt_instance;
+
+/* An instanced scene (the two-level build of RayBuilder, src/RayBuilder.cpp:386-781, and the per-entry
+ * transforms PBRPathTracer hands it, src/PBRPathTracer.cpp:655-675): `scene` holds every mesh's geometry in its
+ * own object space, mesh m being triangles [mesh_first[m], mesh_first[m + 1]) (n_meshes + 1 offsets, from 0 to
+ * scene->n_tri); each mesh gets its own BVH, and `instances` place meshes in the world (a mesh may have any
+ * number of instances, or none). Materials, objects and textures are the scene's. The instances are walked
+ * through a top-level BVH; a hit's instance transforms its shading position, normal and tangent, its
+ * triangle area for texture level of detail, and the scale of the object box for the in-volume tmax
+ * (raypipeline.slang:370). Ties go to the smallest t, then the lowest instance, then the lowest triangle.
+ * Replaces the scene and BVH of vkrt_pt_set_scene (and vice versa). */
+int vkrt_pt_set_instanced_scene(vkrt_pt *pt, const vkrt_pt_scene *scene, const uint32_t *mesh_first,
+                                uint32_t n_meshes, const vkrt_pt_instance *instances, uint32_t n_instances);
 /* The frame's light list (PBRPathTracer::update_trace_descriptors, src/PBRPathTracer.cpp:607-641): lights of
  * positive intensity, other than VKRT_LIGHT_AREA, are kept in the given order (a sun first, as the reference
  * folds it in). With n > 0 after that filter every surface vertex adds next-event estimation with a

@@ -690,22 +690,28 @@ static int box_hit(const cnode *nd, const double o[3], const double inv[3], doub
     return t0 <= t1 && t1 >= tmin && t0 <= tmax;
 }
 
-/* closest hit with tmin < t < tmax, lowest index on ties, back faces culled when `cull`; masked candidates
- * whose alpha falls below their cutoff are ignored (any-hit IgnoreHit) */
-static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int cull, int disable_material,
-                        float blend_xi, float *bt, float *bu, float *bv)
+/* the closest record so far: t (tmax before any hit), barycentrics, triangle and instance */
+typedef struct
+{
+    float t, u, v;
+    uint32_t prim, inst;
+} chit_t;
+
+/* closest hit with tmin < t < h->t over one BVH whose triangles belong to instance `inst` (0 for a flat
+ * scene): the smallest t wins, then the lowest instance, then the lowest triangle; back faces culled when
+ * `cull`; masked candidates whose alpha falls below their cutoff are ignored (any-hit IgnoreHit) */
+static void closest(const cbvh *b, v3 o, v3 d, float tmin, uint32_t inst, int cull, int disable_material,
+                    float blend_xi, chit_t *h)
 {
     double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
-    uint32_t best = 0xFFFFFFFFu;
-    float best_t = tmax;
     uint32_t stack[128];
     int sp = 0;
-    if(b->n_nodes == 0) return best;
+    if(b->n_nodes == 0) return;
     stack[sp++] = 0;
     while(sp)
     {
         const cnode *nd = &b->nodes[stack[--sp]];
-        if(!box_hit(nd, od, inv, tmin, best_t)) continue;
+        if(!box_hit(nd, od, inv, tmin, h->t)) continue;
         if(nd->left < 0)
         {
             for(uint32_t k = nd->first; k < nd->first + nd->count; ++k)
@@ -721,13 +727,15 @@ static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int c
                     if(dot(e1, cross(d, e2)) < 0.f) continue;
                 }
                 if(!(t > tmin)) continue;
-                if(t < best_t || (t == best_t && best != 0xFFFFFFFFu && prim < best))
+                if(t < h->t ||
+                   (t == h->t && h->prim != 0xFFFFFFFFu && (inst < h->inst || (inst == h->inst && prim < h->prim))))
                 {
                     if(!alpha_keep(b->ac, prim, u, v, disable_material, blend_xi, 0u, 0)) continue;
-                    best_t = t;
-                    *bu = u;
-                    *bv = v;
-                    best = prim;
+                    h->t = t;
+                    h->u = u;
+                    h->v = v;
+                    h->prim = prim;
+                    h->inst = inst;
                 }
             }
         }
@@ -737,26 +745,27 @@ static uint32_t closest(const cbvh *b, v3 o, v3 d, float tmin, float tmax, int c
             stack[sp++] = (uint32_t)nd->right;
         }
     }
-    *bt = best_t;
-    return best;
 }
 
-/* transmittance_test (visibility_test.slang:94-165) without culling: 0 when an opaque triangle (not cut out
- * by its alpha mask) lies in (tmin, tmax), else the product of transmission * color over the transmissive hits in ascending
- * (t, triangle) order; mat_of_prim maps a triangle to its material */
+/* transmittance_test (visibility_test.slang:94-165): 0 when an opaque triangle (not cut out by its alpha
+ * mask) lies in (tmin, tmax), else the product of transmission * color over the transmissive hits in
+ * ascending (t, instance, triangle) order; mat_of_prim maps a triangle to its material. With `cull`, back
+ * faces (det < 0 in the triangle's own space) of one-sided triangles are skipped. transmit_collect gathers
+ * one BVH's hits (instance `inst`) and returns 1 when an opaque hit blocks the ray; transmit_product sorts
+ * and multiplies. */
 #define ORC_MAX_TRANSMISSIVE 4096
 typedef struct
 {
     float t;
-    uint32_t prim;
+    uint32_t prim, inst;
 } thit_t;
 
-static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim, v3 o, v3 d, float tmin, float tmax,
-                   uint32_t blend_seed)
+static int transmit_collect(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim, v3 o, v3 d, float tmin,
+                            float tmax, uint32_t blend_seed, int per_candidate, int cull, uint32_t inst, thit_t *hits,
+                            uint32_t *n_hits)
 {
     double od[3] = {o.x, o.y, o.z}, inv[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
-    static __thread thit_t hits[ORC_MAX_TRANSMISSIVE];
-    uint32_t nh = 0, stack[128];
+    uint32_t nh = *n_hits, stack[128];
     int sp = 0;
     if(b->n_nodes) stack[sp++] = 0;
     while(sp)
@@ -772,26 +781,46 @@ static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim
         for(uint32_t k = nd->first; k < nd->first + nd->count; ++k)
         {
             uint32_t prim = b->order[k];
+            const float *tr = b->tris + 9 * (size_t)prim;
             float t, u, v;
-            if(!orc_intersect_tri(b->tris + 9 * (size_t)prim, &o.x, &d.x, &t, &u, &v)) continue;
+            if(!orc_intersect_tri(tr, &o.x, &d.x, &t, &u, &v)) continue;
+            if(cull && !b->two_sided[prim])
+            {
+                v3 e1 = V3(tr[3] - tr[0], tr[4] - tr[1], tr[5] - tr[2]);
+                v3 e2 = V3(tr[6] - tr[0], tr[7] - tr[1], tr[8] - tr[2]);
+                if(dot(e1, cross(d, e2)) < 0.f) continue;
+            }
             if(!(t > tmin && t < tmax)) continue;
             if(!(mats[mat_of_prim[prim]].trans > 0.0f))
             {
                 /* alpha-cutout surfaces let the shadow ray through (visibility_test.slang:145-150) */
-                if(!alpha_keep(b->ac, prim, u, v, 0, 0.0f, blend_seed, 1)) continue;
-                return V3(0, 0, 0);
+                if(!alpha_keep(b->ac, prim, u, v, 0, 0.0f, blend_seed, per_candidate)) continue;
+                *n_hits = nh;
+                return 1;
             }
             if(nh == ORC_MAX_TRANSMISSIVE) abort();
             hits[nh].t = t;
             hits[nh].prim = prim;
+            hits[nh].inst = inst;
             ++nh;
         }
     }
+    *n_hits = nh;
+    return 0;
+}
+
+static int thit_before(const thit_t *a, const thit_t *b)
+{
+    return a->t < b->t || (a->t == b->t && (a->inst < b->inst || (a->inst == b->inst && a->prim < b->prim)));
+}
+
+static v3 transmit_product(const mat_t *mats, const uint32_t *mat_of_prim, thit_t *hits, uint32_t nh)
+{
     for(uint32_t a = 1; a < nh; ++a)
     {
         thit_t x = hits[a];
         uint32_t c = a;
-        while(c > 0 && (hits[c - 1].t > x.t || (hits[c - 1].t == x.t && hits[c - 1].prim > x.prim)))
+        while(c > 0 && thit_before(&x, &hits[c - 1]))
         {
             hits[c] = hits[c - 1];
             --c;
@@ -805,6 +834,90 @@ static v3 transmit(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim
         T = V3(T.x * (m->trans * m->color.x), T.y * (m->trans * m->color.y), T.z * (m->trans * m->color.z));
     }
     return T;
+}
+
+/* ---- two-level scenes (the instance level of RayBuilder's top-level structure, src/RayBuilder.cpp:386-781):
+ * one BVH per mesh over its triangles in object space; instance k places mesh[k] with the world-to-object
+ * rows w2o[12k..] (entry_t::inv_transform) and the transform xf[10k..] = rotation xyzw, translation xyz,
+ * scale xyz (entry_t::transform) */
+typedef struct
+{
+    uint32_t n_inst;
+    const uint32_t *mesh;
+    const float *w2o;
+    const float *xf;
+    const cbvh *blas;
+} oscene_t;
+
+/* the object-space ray: o' = W (o, 1), d' = W (d, 0), each row ((w0 x + w1 y) + w2 z) + w3; d' is not
+ * normalized, so t is the same parameter in both spaces */
+static void object_ray(const float *w, v3 o, v3 d, v3 *oo, v3 *od)
+{
+    const float ov[3] = {o.x, o.y, o.z}, dv[3] = {d.x, d.y, d.z};
+    float ro[3], rd[3];
+    for(int a = 0; a < 3; ++a)
+    {
+        const float *r = w + 4 * a;
+        ro[a] = ((r[0] * ov[0] + r[1] * ov[1]) + r[2] * ov[2]) + r[3];
+        rd[a] = (r[0] * dv[0] + r[1] * dv[1]) + r[2] * dv[2];
+    }
+    *oo = V3(ro[0], ro[1], ro[2]);
+    *od = V3(rd[0], rd[1], rd[2]);
+}
+
+/* rotate_quat (utils/transform.slang:17): v + 2 cross(q.xyz, cross(q.xyz, v) + w v) */
+static inline v3 rotq(const float *q, v3 v)
+{
+    const v3 qv = V3(q[0], q[1], q[2]);
+    return add(v, scl(cross(qv, add(cross(qv, v), scl(v, q[3]))), 2.0f));
+}
+
+/* apply_transform (utils/transform.slang:19-24): rotate_quat(q, v * scale) + translation */
+static inline v3 apply_xf(const float *xf, v3 p)
+{
+    const v3 r = rotq(xf, V3(p.x * xf[7], p.y * xf[8], p.z * xf[9]));
+    return V3(r.x + xf[4], r.y + xf[5], r.z + xf[6]);
+}
+
+/* the closest record of a flat scene (is == NULL) or of every instance of a two-level one */
+static chit_t scene_closest(const cbvh *flat, const oscene_t *is, v3 o, v3 d, float tmin, float tmax, int cull,
+                            int disable_material, float blend_xi)
+{
+    chit_t h = {tmax, 0.f, 0.f, 0xFFFFFFFFu, 0xFFFFFFFFu};
+    if(!is)
+    {
+        closest(flat, o, d, tmin, 0u, cull, disable_material, blend_xi, &h);
+        return h;
+    }
+    for(uint32_t k = 0; k < is->n_inst; ++k)
+    {
+        v3 oo, od;
+        object_ray(is->w2o + 12 * (size_t)k, o, d, &oo, &od);
+        closest(&is->blas[is->mesh[k]], oo, od, tmin, k, cull, disable_material, blend_xi, &h);
+    }
+    return h;
+}
+
+static v3 scene_transmit(const cbvh *flat, const oscene_t *is, const mat_t *mats, const uint32_t *mat_of_prim, v3 o,
+                         v3 d, float tmin, float tmax, uint32_t blend_seed, int per_candidate, int cull)
+{
+    static __thread thit_t hits[ORC_MAX_TRANSMISSIVE];
+    uint32_t nh = 0;
+    if(!is)
+    {
+        if(transmit_collect(flat, mats, mat_of_prim, o, d, tmin, tmax, blend_seed, per_candidate, cull, 0u, hits, &nh))
+            return V3(0, 0, 0);
+    }
+    else
+        for(uint32_t k = 0; k < is->n_inst; ++k)
+        {
+            v3 oo, od;
+            object_ray(is->w2o + 12 * (size_t)k, o, d, &oo, &od);
+            if(transmit_collect(&is->blas[is->mesh[k]], mats, mat_of_prim, oo, od, tmin, tmax, blend_seed,
+                                per_candidate, cull, k, hits, &nh))
+                return V3(0, 0, 0);
+        }
+    return transmit_product(mats, mat_of_prim, hits, nh);
 }
 
 /* eval_disney (bsdf_disney.slang:298-381), iridescence strength 0 */
@@ -1132,6 +1245,8 @@ typedef struct
     const float *uv;  /* 2 per vertex */
     const float *tan; /* 3 per vertex */
     float *obj_tmax;
+    const float *obj_he;   /* per object: the half extents of its box (3 floats) */
+    const oscene_t *inst;  /* NULL for a flat scene */
     float *out; /* 8 floats per pixel: r g b alpha depth normal(bits) object(bits) albedo(bits) */
     uint32_t y0, y1;
 } job_t;
@@ -1191,8 +1306,12 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
             /* BLEND candidates compare against rnd() of a copy of payload.rng_state (raypipeline.slang:693) */
             uint32_t peek = rng;
             const float blend_xi = rnd(&peek);
-            uint32_t prim = closest(&j->bvh, org, dir, EPS, tmax, media_index == 0, (int)pp->disable_material,
-                                    blend_xi, &t, &bu, &bv);
+            const chit_t ch = scene_closest(&j->bvh, j->inst, org, dir, EPS, tmax, media_index == 0,
+                                            (int)pp->disable_material, blend_xi);
+            const uint32_t prim = ch.prim;
+            t = ch.t;
+            bu = ch.u;
+            bv = ch.v;
             int stop = 0, media_op = 0;
             float next_ior = 1.f;
             uint32_t obj = 0xFFFFu;
@@ -1225,15 +1344,37 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                 v3 v1 = V3(P[3 * ix[1]], P[3 * ix[1] + 1], P[3 * ix[1] + 2]);
                 v3 v2 = V3(P[3 * ix[2]], P[3 * ix[2] + 1], P[3 * ix[2] + 2]);
                 v3 e1 = sub(v1, v0), e2 = sub(v2, v0);
-                int backface = dot(e1, cross(dir, e2)) < 0.f;
+                /* HitKind from the determinant in the triangle's own space; an instance's shading point and
+                 * normal go through its transform (interpolate_vertex, raypipeline.slang:336-358) */
+                const float *xf = j->inst ? j->inst->xf + 10 * (size_t)ch.inst : NULL;
+                v3 dir_o = dir;
+                if(j->inst)
+                {
+                    v3 unused_o;
+                    object_ray(j->inst->w2o + 12 * (size_t)ch.inst, V3(0, 0, 0), dir, &unused_o, &dir_o);
+                }
+                int backface = dot(e1, cross(dir_o, e2)) < 0.f;
                 float bx = 1.0f - bu - bv, by = bu, bz = bv;
                 pos = add(add(scl(v0, bx), scl(v1, by)), scl(v2, bz));
                 v3 n = add(add(scl(V3(N[3 * ix[0]], N[3 * ix[0] + 1], N[3 * ix[0] + 2]), bx),
                                scl(V3(N[3 * ix[1]], N[3 * ix[1] + 1], N[3 * ix[1] + 2]), by)),
                            scl(V3(N[3 * ix[2]], N[3 * ix[2] + 1], N[3 * ix[2] + 2]), bz));
                 nrm = nrmz(n);
+                if(xf)
+                {
+                    pos = apply_xf(xf, pos);
+                    nrm = rotq(xf, nrm);
+                }
                 if(backface) nrm = neg(nrm);
                 obj = sc->tri_object[prim];
+                /* tmax_inside_volume = 2 * length(scale * aabb.half_extents()) (raypipeline.slang:370) */
+                float tmax_inside = j->obj_tmax[obj];
+                if(xf)
+                {
+                    const float *he = j->obj_he + 3 * (size_t)obj;
+                    const float hx = xf[7] * he[0], hy = xf[8] * he[1], hz = xf[9] * he[2];
+                    tmax_inside = 2.f * sqrtf((hx * hx + hy * hy) + hz * hz);
+                }
                 mat_t m = j->mats[sc->object_material[obj]];
                 v3 Vw = neg(dir);
                 const int media_boundary = m.trans > 0.0f; /* before the transmission texture (raypipeline.slang:517-522) */
@@ -1245,7 +1386,10 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                     const float *T0 = j->uv + 2 * (size_t)ix[0], *T1 = j->uv + 2 * (size_t)ix[1],
                                 *T2 = j->uv + 2 * (size_t)ix[2];
                     float tu = (T0[0] * bx + T1[0] * by) + T2[0] * bz, tv = (T0[1] * bx + T1[1] * by) + T2[1] * bz;
-                    v3 pc = cross(sub(v1, v0), sub(v2, v0));
+                    /* lod_constant: the triangle's area in world space (raypipeline.slang:299-309) */
+                    const v3 w0 = xf ? apply_xf(xf, v0) : v0, w1 = xf ? apply_xf(xf, v1) : v1,
+                             w2 = xf ? apply_xf(xf, v2) : v2;
+                    v3 pc = cross(sub(w1, w0), sub(w2, w0));
                     float p_a = sqrtf(dot(pc, pc));
                     float t_a = fabsf((T1[0] - T0[0]) * (T2[1] - T0[1]) - (T2[0] - T0[0]) * (T1[1] - T0[1]));
                     const float lod_cone = 0.5f * tlog2(t_a / p_a) + tlog2(fabsf(cone_width));
@@ -1278,6 +1422,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                         v3 tg = nrmz(add(add(scl(V3(G[3 * ix[0]], G[3 * ix[0] + 1], G[3 * ix[0] + 2]), bx),
                                              scl(V3(G[3 * ix[1]], G[3 * ix[1] + 1], G[3 * ix[1] + 2]), by)),
                                          scl(V3(G[3 * ix[2]], G[3 * ix[2] + 1], G[3 * ix[2] + 2]), bz)));
+                        if(xf) tg = rotq(xf, tg);
                         v3 bt = nrmz(cross(nrm, tg));
                         if(backface) bt = neg(bt);
                         v3 nm = nrmz(add(add(scl(tg, nt.x), scl(bt, nt.y)), scl(nrm, nt.z)));
@@ -1343,7 +1488,8 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                     if(ls.radiance.x > 0.0f || ls.radiance.y > 0.0f || ls.radiance.z > 0.0f)
                     {
                         v3 so = add(pos, scl(nrm, EPS));
-                        v3 T = transmit(&j->bvh, j->mats, j->mat_of_prim, so, ls.dir, 0.0f, ls.dist, lr);
+                        v3 T = scene_transmit(&j->bvh, j->inst, j->mats, j->mat_of_prim, so, ls.dir, 0.0f, ls.dist,
+                                              lr, 1, 0);
                         if(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)
                         {
                             float bpdf = 0.0f;
@@ -1376,7 +1522,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                         thr = V3(fminf(thr.x, pp->max_path_beta), fminf(thr.y, pp->max_path_beta),
                                  fminf(thr.z, pp->max_path_beta));
                     beta = mul(beta, thr);
-                    if(b.transmission) tmax = media_op == 1 ? j->obj_tmax[obj] : DEFAULT_TMAX;
+                    if(b.transmission) tmax = media_op == 1 ? tmax_inside : DEFAULT_TMAX;
                     media_op = b.transmission ? media_op : 0;
                     if(!(pp->test_flags & ORC_PT_NO_ROULETTE) && max3(beta) <= 0.05f * (float)i)
                     {
@@ -1478,36 +1624,72 @@ void orc_eval_disney_n(const float *mat36, const float *N, const float *V, float
     }
 }
 
-void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads)
+/* everything a render or a transmittance query reads, built once per call from an orc_pt_scene */
+typedef struct
 {
-    g_ulp_bias = (pp->test_flags & ORC_PT_ULP_BIAS) != 0;
+    cbvh flat;    /* the flat scene's BVH (n_instances == 0) */
+    cbvh *blas;   /* one per mesh (n_instances > 0) */
+    oscene_t is;
+    float *tris;
+    uint8_t *two;
+    mat_t *mats;
+    uint32_t *mat_of_prim;
+    float *obj_tmax, *obj_he;
+    float *uv, *tan;
+    otex_t *tex;
+    alpha_ctx ac;
+} built_t;
+
+static void bvh_over(cbvh *b, const float *tris, const uint8_t *two, const double *cent, uint32_t first, uint32_t n)
+{
+    memset(b, 0, sizeof(*b));
+    b->tris = tris;
+    b->two_sided = two;
+    b->n = n;
+    b->order = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for(uint32_t t = 0; t < n; ++t) b->order[t] = first + t;
+    b->nodes = (cnode *)malloc(sizeof(cnode) * (2 * (size_t)(n ? n : 1)));
+    if(n) build_rec(b, 0, n, cent);
+}
+
+static void scene_build(const orc_pt_scene *sc, built_t *s)
+{
+    memset(s, 0, sizeof(*s));
     const uint32_t n = sc->n_tri;
-    cbvh b;
-    memset(&b, 0, sizeof(b));
-    float *tris = (float *)malloc(sizeof(float) * 9 * (size_t)(n ? n : 1));
-    uint8_t *two = (uint8_t *)calloc(n ? n : 1, 1);
+    s->tris = (float *)malloc(sizeof(float) * 9 * (size_t)(n ? n : 1));
+    s->two = (uint8_t *)calloc(n ? n : 1, 1);
     double *cent = (double *)malloc(sizeof(double) * 3 * (size_t)(n ? n : 1));
-    mat_t *mats = (mat_t *)calloc(sc->n_materials ? sc->n_materials : 1, sizeof(mat_t));
-    for(uint32_t m = 0; m < sc->n_materials; ++m) mats[m] = unpack_material(sc->materials + 36 * (size_t)m, sc->n_textures);
+    s->mats = (mat_t *)calloc(sc->n_materials ? sc->n_materials : 1, sizeof(mat_t));
+    for(uint32_t m = 0; m < sc->n_materials; ++m)
+        s->mats[m] = unpack_material(sc->materials + 36 * (size_t)m, sc->n_textures);
     for(uint32_t t = 0; t < n; ++t)
     {
         for(int k = 0; k < 3; ++k)
-            for(int a = 0; a < 3; ++a) tris[9 * (size_t)t + 3 * k + a] = sc->positions[3 * (size_t)sc->indices[3 * (size_t)t + k] + a];
-        two[t] = (uint8_t)mats[sc->object_material[sc->tri_object[t]]].two_sided;
-        for(int a = 0; a < 3; ++a)
-            cent[3 * (size_t)t + a] = ((double)tris[9 * t + a] + tris[9 * t + 3 + a] + tris[9 * t + 6 + a]) / 3.0;
+            for(int a = 0; a < 3; ++a)
+                s->tris[9 * (size_t)t + 3 * k + a] = sc->positions[3 * (size_t)sc->indices[3 * (size_t)t + k] + a];
+        s->two[t] = (uint8_t)s->mats[sc->object_material[sc->tri_object[t]]].two_sided;
+        const float *tr = s->tris + 9 * (size_t)t;
+        for(int a = 0; a < 3; ++a) cent[3 * (size_t)t + a] = ((double)tr[a] + tr[3 + a] + tr[6 + a]) / 3.0;
     }
-    b.tris = tris;
-    b.two_sided = two;
-    b.n = n;
-    b.order = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
-    for(uint32_t t = 0; t < n; ++t) b.order[t] = t;
-    b.nodes = (cnode *)malloc(sizeof(cnode) * (2 * (size_t)(n ? n : 1)));
-    if(n) build_rec(&b, 0, n, cent);
-    /* per-object tmax inside a volume: 2 * length(aabb half extents) */
-    float *obj_tmax = (float *)calloc(sc->n_objects ? sc->n_objects : 1, sizeof(float));
-    float *lo = (float *)malloc(sizeof(float) * 3 * (sc->n_objects ? sc->n_objects : 1));
-    float *hi = (float *)malloc(sizeof(float) * 3 * (sc->n_objects ? sc->n_objects : 1));
+    if(sc->n_instances)
+    {
+        s->blas = (cbvh *)calloc(sc->n_meshes ? sc->n_meshes : 1, sizeof(cbvh));
+        for(uint32_t m = 0; m < sc->n_meshes; ++m)
+            bvh_over(&s->blas[m], s->tris, s->two, cent, sc->mesh_first[m], sc->mesh_first[m + 1] - sc->mesh_first[m]);
+        s->is.n_inst = sc->n_instances;
+        s->is.mesh = sc->inst_mesh;
+        s->is.w2o = sc->inst_w2o;
+        s->is.xf = sc->inst_xf;
+        s->is.blas = s->blas;
+    }
+    else
+        bvh_over(&s->flat, s->tris, s->two, cent, 0, n);
+    free(cent);
+    /* per object: the half extents of its box and tmax inside a volume, 2 * length(half extents) */
+    const uint32_t no = sc->n_objects ? sc->n_objects : 1;
+    s->obj_tmax = (float *)calloc(no, sizeof(float));
+    s->obj_he = (float *)calloc(3 * (size_t)no, sizeof(float));
+    float *lo = (float *)malloc(sizeof(float) * 3 * no), *hi = (float *)malloc(sizeof(float) * 3 * no);
     for(uint32_t o = 0; o < 3 * sc->n_objects; ++o)
     {
         lo[o] = FLT_MAX;
@@ -1519,27 +1701,60 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
         for(int k = 0; k < 3; ++k)
             for(int a = 0; a < 3; ++a)
             {
-                float c = tris[9 * (size_t)t + 3 * k + a];
+                float c = s->tris[9 * (size_t)t + 3 * k + a];
                 lo[3 * o + a] = c < lo[3 * o + a] ? c : lo[3 * o + a];
                 hi[3 * o + a] = hi[3 * o + a] < c ? c : hi[3 * o + a];
             }
     }
     for(uint32_t o = 0; o < sc->n_objects; ++o)
     {
-        float he0 = (hi[3 * o] - lo[3 * o]) / 2.f, he1 = (hi[3 * o + 1] - lo[3 * o + 1]) / 2.f,
-              he2 = (hi[3 * o + 2] - lo[3 * o + 2]) / 2.f;
-        obj_tmax[o] = hi[3 * o] < lo[3 * o] ? 0.f : 2.f * sqrtf((he0 * he0 + he1 * he1) + he2 * he2);
+        float *he = s->obj_he + 3 * (size_t)o;
+        for(int a = 0; a < 3; ++a) he[a] = hi[3 * o] < lo[3 * o] ? 0.f : (hi[3 * o + a] - lo[3 * o + a]) / 2.f;
+        s->obj_tmax[o] = 2.f * sqrtf((he[0] * he[0] + he[1] * he[1]) + he[2] * he[2]);
     }
-    uint32_t *mat_of_prim = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
-    for(uint32_t t = 0; t < n; ++t) mat_of_prim[t] = sc->object_material[sc->tri_object[t]];
-    float *uv = (float *)calloc(2 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
-    if(sc->texcoords) memcpy(uv, sc->texcoords, sizeof(float) * 2 * (size_t)sc->n_vert);
-    float *tan = (float *)calloc(3 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
-    if(sc->tangents) memcpy(tan, sc->tangents, sizeof(float) * 3 * (size_t)sc->n_vert);
-    otex_t *tex = (otex_t *)calloc(sc->n_textures ? sc->n_textures : 1, sizeof(otex_t));
-    for(uint32_t k = 0; k < sc->n_textures; ++k) otex_build(&tex[k], &sc->textures[k]);
-    alpha_ctx ac = {mats, mat_of_prim, sc->indices, uv, tex};
-    b.ac = &ac;
+    free(lo);
+    free(hi);
+    s->mat_of_prim = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    for(uint32_t t = 0; t < n; ++t) s->mat_of_prim[t] = sc->object_material[sc->tri_object[t]];
+    s->uv = (float *)calloc(2 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
+    if(sc->texcoords) memcpy(s->uv, sc->texcoords, sizeof(float) * 2 * (size_t)sc->n_vert);
+    s->tan = (float *)calloc(3 * (size_t)(sc->n_vert ? sc->n_vert : 1), sizeof(float));
+    if(sc->tangents) memcpy(s->tan, sc->tangents, sizeof(float) * 3 * (size_t)sc->n_vert);
+    s->tex = (otex_t *)calloc(sc->n_textures ? sc->n_textures : 1, sizeof(otex_t));
+    for(uint32_t k = 0; k < sc->n_textures; ++k) otex_build(&s->tex[k], &sc->textures[k]);
+    s->ac = (alpha_ctx){s->mats, s->mat_of_prim, sc->indices, s->uv, s->tex};
+    /* the BVHs point at ac: scene_build's caller keeps `s` in place until scene_free */
+    s->flat.ac = &s->ac;
+    for(uint32_t m = 0; m < sc->n_meshes && s->blas; ++m) s->blas[m].ac = &s->ac;
+}
+
+static void scene_free(const orc_pt_scene *sc, built_t *s)
+{
+    for(uint32_t k = 0; k < sc->n_textures; ++k) otex_free(&s->tex[k]);
+    free(s->tex);
+    free(s->uv);
+    free(s->tan);
+    free(s->mat_of_prim);
+    free(s->obj_tmax);
+    free(s->obj_he);
+    free(s->flat.nodes);
+    free(s->flat.order);
+    for(uint32_t m = 0; m < sc->n_meshes && s->blas; ++m)
+    {
+        free(s->blas[m].nodes);
+        free(s->blas[m].order);
+    }
+    free(s->blas);
+    free(s->mats);
+    free(s->two);
+    free(s->tris);
+}
+
+void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads)
+{
+    g_ulp_bias = (pp->test_flags & ORC_PT_ULP_BIAS) != 0;
+    built_t s;
+    scene_build(sc, &s);
     light_t *lights = (light_t *)calloc(pp->n_lights ? pp->n_lights : 1, sizeof(light_t));
     for(uint32_t li = 0; li < pp->n_lights; ++li) lights[li] = read_light(pp->lights + 20 * (size_t)li);
     if(nthreads < 1) nthreads = 1;
@@ -1549,14 +1764,16 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     {
         jobs[k].sc = sc;
         jobs[k].pp = pp;
-        jobs[k].bvh = b;
-        jobs[k].mats = mats;
-        jobs[k].mat_of_prim = mat_of_prim;
+        jobs[k].bvh = s.flat;
+        jobs[k].mats = s.mats;
+        jobs[k].mat_of_prim = s.mat_of_prim;
         jobs[k].lights = lights;
-        jobs[k].tex = tex;
-        jobs[k].uv = uv;
-        jobs[k].tan = tan;
-        jobs[k].obj_tmax = obj_tmax;
+        jobs[k].tex = s.tex;
+        jobs[k].uv = s.uv;
+        jobs[k].tan = s.tan;
+        jobs[k].obj_tmax = s.obj_tmax;
+        jobs[k].obj_he = s.obj_he;
+        jobs[k].inst = sc->n_instances ? &s.is : NULL;
         jobs[k].out = out8;
         jobs[k].y0 = (uint32_t)((uint64_t)pp->height * k / nthreads);
         jobs[k].y1 = (uint32_t)((uint64_t)pp->height * (k + 1) / nthreads);
@@ -1566,21 +1783,29 @@ void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8,
     free(th);
     free(jobs);
     free(lights);
-    for(uint32_t k = 0; k < sc->n_textures; ++k) otex_free(&tex[k]);
-    free(tex);
-    free(uv);
-    free(tan);
-    free(mat_of_prim);
-    free(lo);
-    free(hi);
-    free(obj_tmax);
-    free(b.nodes);
-    free(b.order);
-    free(mats);
-    free(cent);
-    free(two);
-    free(tris);
+    scene_free(sc, &s);
     g_ulp_bias = 0;
+}
+
+/* vkrt_pt_trace_transmittance's query on the same scene: per ray (8 floats: origin, tmin, direction, tmax)
+ * the transmittance_test product, with BLEND candidates kept (threshold 0) and MASK ones cut at their cutoff,
+ * back faces of one-sided triangles skipped when `cull`. out4: rgb, 0 per ray. */
+void orc_pt_transmittance(const orc_pt_scene *sc, const float *rays8, uint32_t n, int cull, float *out4)
+{
+    built_t s;
+    scene_build(sc, &s);
+    const oscene_t *is = sc->n_instances ? &s.is : NULL;
+    for(uint32_t r = 0; r < n; ++r)
+    {
+        const float *ray = rays8 + 8 * (size_t)r;
+        const v3 T = scene_transmit(&s.flat, is, s.mats, s.mat_of_prim, V3(ray[0], ray[1], ray[2]),
+                                    V3(ray[4], ray[5], ray[6]), ray[3], ray[7], 0u, 0, cull);
+        out4[4 * (size_t)r] = T.x;
+        out4[4 * (size_t)r + 1] = T.y;
+        out4[4 * (size_t)r + 2] = T.z;
+        out4[4 * (size_t)r + 3] = 0.f;
+    }
+    scene_free(sc, &s);
 }
 
 /* ------------------------------------------------------------------------------------------------

@@ -37,6 +37,15 @@ typedef struct orc_pt_scene
     const orc_texture *textures;
     uint32_t n_textures;
     const float *tangents; /* 3 per vertex, or NULL (all zero) */
+    /* two-level scenes (n_instances > 0): the triangles split into meshes at mesh_first (n_meshes + 1
+     * ascending offsets), instance k places mesh inst_mesh[k] with the world-to-object rows
+     * inst_w2o[12k..] and the transform inst_xf[10k..] = rotation xyzw, translation xyz, scale xyz */
+    uint32_t n_meshes;
+    const uint32_t *mesh_first;
+    uint32_t n_instances;
+    const uint32_t *inst_mesh;
+    const float *inst_w2o;
+    const float *inst_xf;
 } orc_pt_scene;
 
 typedef struct orc_pt_params
@@ -62,6 +71,8 @@ enum
 /* One batch: out8 = 8 floats per pixel (radiance rgb / num_samples, coverage, linear depth, -, object id
  * bits, -), row-major from the top-left. */
 void orc_pt_render(const orc_pt_scene *sc, const orc_pt_params *pp, float *out8, int nthreads);
+/* vkrt_pt_trace_transmittance on the same scene: rays8 = origin, tmin, direction, tmax; out4 = rgb, 0 */
+void orc_pt_transmittance(const orc_pt_scene *sc, const float *rays8, uint32_t n, int cull, float *out4);
 void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4);
 /* test hook: only the first `passes` (1..5) of the five a-trous passes (steps 1, 2, 4, 8, 16) */
 void orc_denoise_passes(const uint32_t *px8, uint32_t w, uint32_t h, int passes, float *out4);

@@ -306,7 +306,84 @@ class _PtScene(ctypes.Structure):
                 ("object_material", ctypes.c_void_p), ("n_objects", ctypes.c_uint32),
                 ("materials", ctypes.c_void_p), ("n_materials", ctypes.c_uint32), ("environment", ctypes.c_float),
                 ("texcoords", ctypes.c_void_p), ("textures", ctypes.c_void_p), ("n_textures", ctypes.c_uint32),
-                ("tangents", ctypes.c_void_p)]
+                ("tangents", ctypes.c_void_p), ("n_meshes", ctypes.c_uint32), ("mesh_first", ctypes.c_void_p),
+                ("n_instances", ctypes.c_uint32), ("inst_mesh", ctypes.c_void_p), ("inst_w2o", ctypes.c_void_p),
+                ("inst_xf", ctypes.c_void_p)]
+
+
+def pt_instance_matrix(rec):
+    """the 3x4 object-to-world matrix of one vkrt_pt_instance: column j = R(q) (scale_j e_j) as
+    v + 2 q.xyz x (q.xyz x v + w v) in double, the last column the translation, rounded once to float"""
+    qx, qy, qz, qw = (float(x) for x in rec["rotation"])
+    m = np.zeros((3, 4), np.float32)
+    for j in range(3):
+        v = [0.0, 0.0, 0.0]
+        v[j] = float(rec["scale"][j])
+        c1 = [qy * v[2] - qz * v[1] + qw * v[0], qz * v[0] - qx * v[2] + qw * v[1], qx * v[1] - qy * v[0] + qw * v[2]]
+        c2 = [qy * c1[2] - qz * c1[1], qz * c1[0] - qx * c1[2], qx * c1[1] - qy * c1[0]]
+        for a in range(3):
+            m[a, j] = np.float32(v[a] + 2.0 * c2[a])
+    m[:, 3] = rec["translation"]
+    return m
+
+
+def _pt_instanced(instanced):
+    """(mesh_first u32, world-to-object rows f32 [n,12], mesh u32, transform f32 [n,10]) of (mesh_first,
+    PT_INSTANCE_DTYPE records); the world-to-object rows are vkrt_instance_set_transform's inverse of
+    pt_instance_matrix, the matrices the tracer builds its instances from"""
+    import vierkant_amd as vk
+    mesh_first, inst = instanced
+    mf = np.ascontiguousarray(mesh_first, np.uint32)
+    inst = np.asarray(inst, vk.PT_INSTANCE_DTYPE).reshape(-1)
+    w2o = np.zeros((max(1, len(inst)), 12), np.float32)
+    for k, rec in enumerate(inst):
+        w2o[k] = vk.instance(pt_instance_matrix(rec), int(rec["mesh"]))["world_to_object"]
+    xf = np.zeros((max(1, len(inst)), 10), np.float32)
+    if len(inst):
+        xf[:, 0:4], xf[:, 4:7], xf[:, 7:10] = inst["rotation"], inst["translation"], inst["scale"]
+    return mf, w2o, np.ascontiguousarray(inst["mesh"], np.uint32), xf
+
+
+def _pt_scene(scene, instanced=None):
+    """an orc_pt_scene over a scene dict (and, for a two-level scene, (mesh_first, instances)), with the
+    arrays it points into"""
+    arr = {k: np.ascontiguousarray(scene[k]) for k in
+           ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
+    uv = scene.get("texcoords")
+    uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
+    tg = scene.get("tangents")
+    tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
+    imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
+    smp = list(scene.get("texture_samplers") or [(0, 0, 0, 0)] * len(imgs))
+    tex = (_Texture * max(1, len(imgs)))(*[_Texture(t.shape[1], t.shape[0], t.ctypes.data, *s)
+                                           for t, s in zip(imgs, smp)])
+    ins = _pt_instanced(instanced) if instanced is not None else None
+    sc = _PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
+                  arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
+                  arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,
+                  len(arr["materials"]), float(scene.get("environment", 1.0)),
+                  uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
+                  tg.ctypes.data if tg is not None else None)
+    if ins is not None:
+        mf, w2o, mesh, xf = ins
+        sc.n_meshes, sc.mesh_first = len(mf) - 1, mf.ctypes.data
+        sc.n_instances, sc.inst_mesh = len(mesh), mesh.ctypes.data
+        sc.inst_w2o, sc.inst_xf = w2o.ctypes.data, xf.ctypes.data
+    return sc, (arr, uv, tg, imgs, tex, ins)
+
+
+def pt_transmittance(scene, rays, cull=False, instanced=None):
+    """vkrt_pt_trace_transmittance's product per ray (rgb, 0) through the oracle's scene, flat or two-level"""
+    h = lib()
+    h.orc_pt_transmittance.restype = None
+    h.orc_pt_transmittance.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int,
+                                       ctypes.c_void_p]
+    sc, keep = _pt_scene(scene, instanced)
+    r = np.ascontiguousarray(rays)
+    out = np.zeros((len(r), 4), np.float32)
+    h.orc_pt_transmittance(ctypes.byref(sc), r.ctypes.data, len(r), int(cull), out.ctypes.data)
+    del keep
+    return out
 
 
 class _Texture(ctypes.Structure):
@@ -373,29 +450,15 @@ def texture_sample(rgba, uv, lam, sampler=(0, 0, 0, 0)):
 
 
 def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_path_beta=0.0, disable_material=0,
-              nthreads=None, lights=None, mis_mode=0, suppress_refractive_caustics=0, test_flags=0):
-    """one batch of the CPU reference integrator: (H, W, 8) float32 (rgb, coverage, depth, -, object bits, -)"""
+              nthreads=None, lights=None, mis_mode=0, suppress_refractive_caustics=0, test_flags=0, instanced=None):
+    """one batch of the CPU reference integrator: (H, W, 8) float32 (rgb, coverage, depth, -, object bits, -);
+    instanced = (mesh_first, PT_INSTANCE_DTYPE records) renders the two-level scene"""
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
     h = lib()
     h.orc_pt_render.restype = None
     h.orc_pt_render.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
-    arr = {k: np.ascontiguousarray(scene[k]) for k in
-           ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
-    uv = scene.get("texcoords")
-    uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
-    tg = scene.get("tangents")
-    tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
-    imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
-    smp = list(scene.get("texture_samplers") or [(0, 0, 0, 0)] * len(imgs))
-    tex = (_Texture * max(1, len(imgs)))(*[_Texture(t.shape[1], t.shape[0], t.ctypes.data, *s)
-                                           for t, s in zip(imgs, smp)])
-    sc = _PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
-                  arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
-                  arr["object_material"].ctypes.data, len(arr["object_material"]), arr["materials"].ctypes.data,
-                  len(arr["materials"]), float(scene.get("environment", 1.0)),
-                  uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
-                  tg.ctypes.data if tg is not None else None)
+    sc, keep = _pt_scene(scene, instanced)
     eye, fw, rt, up, thx, thy = W.camera_basis(camera, width, height)
     # the tracer's gather: positive intensity, no Area lights, order kept
     la = np.zeros(0, vk.LIGHT_DTYPE) if lights is None else np.asarray(lights, vk.LIGHT_DTYPE).reshape(-1)
@@ -408,4 +471,5 @@ def pt_render(scene, camera, width, height, num_samples, max_depth, seed, max_pa
                    float(camera.fov_y), test_flags)
     out = np.zeros((height, width, 8), np.float32)
     h.orc_pt_render(ctypes.byref(sc), ctypes.byref(pp), out.ctypes.data, nthreads or threads())
+    del keep
     return out

@@ -80,6 +80,34 @@ def test_scene_matches_brute_force_over_instances(ctx, meshes):
         b.close()
 
 
+@pytest.mark.parametrize("n_tri", [1, 2, 8])
+def test_single_leaf_meshes(ctx, meshes, n_tri):
+    """meshes of at most 8 triangles are one leaf with no node: their bounds (which place the instance in the
+    top level) are the padded box of the leaf's triangles, and the scene finds them like any other mesh"""
+    p, i = meshes[2]
+    small_i = np.ascontiguousarray(i[:3 * n_tri])
+    b = ctx.bvh_build(p, small_i, max_leaf_size=max(2, n_tri))
+    used = p[small_i].astype(np.float32)
+    lo, hi = used.min(axis=0), used.max(axis=0)
+    pad = np.float32(2.0 ** -16)
+    info = b.info()
+    assert info.n_leaves == 1
+    assert np.array_equal(np.asarray(info.bounds[:3], np.float32), lo - np.abs(lo) * pad)
+    assert np.array_equal(np.asarray(info.bounds[3:], np.float32), hi + np.abs(hi) * pad)
+    big = ctx.bvh_build(*meshes[0])
+    inst = random_instances(40, 2, seed=11, spread=3.0)
+    sc = vk.Scene(ctx, [big, b], inst)
+    rays = scene_rays(4000, 12, spread=5.0)
+    got = sc.trace_closest(rays)
+    want = orc.scene_closest([W.flat_triangles(*meshes[0]), W.flat_triangles(p, small_i)], inst, rays)
+    small_hits = (want["prim"] != 0xFFFFFFFF) & (inst["blas"][np.minimum(want["instance"], 39)] == 1)
+    assert small_hits.sum() > 5
+    assert_scene_hits_equal(got, want, f"{n_tri}-triangle mesh")
+    sc.close()
+    b.close()
+    big.close()
+
+
 def test_scene_overlapping_copies_tie_to_lower_instance(ctx, meshes):
     p, i = meshes[0]
     b = ctx.bvh_build(p, i)

@@ -86,6 +86,23 @@ class PtScene(ctypes.Structure):
                 ("tangents", ctypes.c_void_p)]
 
 
+#: vkrt_pt_instance: object to world = translation + rotate(rotation (x, y, z, w), scale * p)
+PT_INSTANCE_DTYPE = np.dtype([("translation", "<f4", 3), ("rotation", "<f4", 4), ("scale", "<f4", 3),
+                              ("mesh", "<u4"), ("reserved", "<u4", 3)])
+
+
+def pt_instances(meshes, translations=None, rotations=None, scales=None):
+    """PT_INSTANCE_DTYPE records: one per mesh index, identity parts where an argument is None"""
+    meshes = np.asarray(meshes, np.uint32).reshape(-1)
+    out = np.zeros(len(meshes), PT_INSTANCE_DTYPE)
+    out["mesh"] = meshes
+    out["rotation"] = (0.0, 0.0, 0.0, 1.0) if rotations is None else np.asarray(rotations, np.float32)
+    out["scale"] = 1.0 if scales is None else np.asarray(scales, np.float32)
+    if translations is not None:
+        out["translation"] = np.asarray(translations, np.float32)
+    return out
+
+
 class Camera(ctypes.Structure):
     _fields_ = [("eye", ctypes.c_float * 3), ("fov_y", ctypes.c_float), ("forward", ctypes.c_float * 3),
                 ("aperture", ctypes.c_float), ("up", ctypes.c_float * 3), ("focal_distance", ctypes.c_float)]
@@ -256,6 +273,7 @@ _SIGNATURES = {
     "vkrt_pt_set_scene": (_i32, [_vp, ctypes.POINTER(PtScene)]),
     "vkrt_pt_set_lights": (_i32, [_vp, _vp, _u32]),
     "vkrt_pt_set_launch_mode": (_i32, [_vp, _i32, _u32]),
+    "vkrt_pt_set_instanced_scene": (_i32, [_vp, ctypes.POINTER(PtScene), _vp, _u32, _vp, _u32]),
     "vkrt_model_load": (_i32, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "vkrt_model_error": (ctypes.c_char_p, []),
     "vkrt_model_get": (_i32, [_vp, ctypes.POINTER(ModelView)]),
@@ -728,6 +746,27 @@ def make_camera(eye, forward, up=(0.0, 1.0, 0.0), fov_y=0.7853981633974483, aper
                   (ctypes.c_float * 3)(*up), focal_distance)
 
 
+def _pt_scene_view(scene):
+    """a vkrt_pt_scene over contiguous copies of a scene dict, with the arrays it points into"""
+    arr = {k: np.ascontiguousarray(scene[k]) for k in
+           ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
+    uv = scene.get("texcoords")
+    uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
+    tg = scene.get("tangents")
+    tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
+    imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
+    smp = list(scene.get("texture_samplers") or [(0, 0, 0, 0)] * len(imgs))
+    tex = (Texture * max(1, len(imgs)))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data, *s)
+                                          for t, s in zip(imgs, smp)])
+    sc = PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
+                 arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
+                 arr["object_material"].ctypes.data, len(arr["object_material"]),
+                 arr["materials"].ctypes.data, len(arr["materials"]), float(scene.get("environment", 1.0)),
+                 uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
+                 tg.ctypes.data if tg is not None else None)
+    return sc, (arr, uv, tg, imgs, tex)
+
+
 class PathTracer:
     """PBRPathTracer on the GPU: `settings` (settings_t subset), render(camera) = render_scene,
     current_batch(), reset_accumulator(), pixels() = the pixel_data_t buffer."""
@@ -756,23 +795,22 @@ class PathTracer:
         tangents [n,3] f32 (for normal maps) and textures (a list of [h,w,4] uint8 RGBA images, referenced by the
         materials' texture slots as index + 1) with texture_samplers (one (address_mode_u, address_mode_v,
         min_filter, mag_filter) per texture; default repeat and linear)"""
-        arr = {k: np.ascontiguousarray(scene[k]) for k in
-               ("positions", "normals", "indices", "tri_object", "object_material", "materials")}
-        uv = scene.get("texcoords")
-        uv = np.ascontiguousarray(uv, np.float32) if uv is not None and len(uv) else None
-        tg = scene.get("tangents")
-        tg = np.ascontiguousarray(tg, np.float32) if tg is not None and len(tg) else None
-        imgs = [np.ascontiguousarray(t, np.uint8) for t in scene.get("textures", [])]
-        smp = list(scene.get("texture_samplers") or [(0, 0, 0, 0)] * len(imgs))
-        tex = (Texture * max(1, len(imgs)))(*[Texture(t.shape[1], t.shape[0], t.ctypes.data, *s)
-                                              for t, s in zip(imgs, smp)])
-        sc = PtScene(arr["positions"].ctypes.data, arr["normals"].ctypes.data, len(arr["positions"]),
-                     arr["indices"].ctypes.data, len(arr["indices"]) // 3, arr["tri_object"].ctypes.data,
-                     arr["object_material"].ctypes.data, len(arr["object_material"]),
-                     arr["materials"].ctypes.data, len(arr["materials"]), float(scene.get("environment", 1.0)),
-                     uv.ctypes.data if uv is not None else None, ctypes.addressof(tex) if imgs else None, len(imgs),
-                     tg.ctypes.data if tg is not None else None)
+        sc, keep = _pt_scene_view(scene)
         self.ctx.check(lib().vkrt_pt_set_scene(self._h, ctypes.byref(sc)))
+        del keep
+
+    def set_instanced_scene(self, scene, mesh_first, instances):
+        """a two-level scene: `scene` as for set_scene, its triangles split into meshes at mesh_first
+        (n_meshes + 1 ascending triangle offsets, 0 first and the triangle count last), each mesh a bottom
+        level BVH, placed by `instances` (PT_INSTANCE_DTYPE records, see pt_instances)"""
+        sc, keep = _pt_scene_view(scene)
+        mf = np.ascontiguousarray(mesh_first, np.uint32).reshape(-1)
+        ins = np.ascontiguousarray(instances, PT_INSTANCE_DTYPE).reshape(-1)
+        if len(mf) < 2:
+            raise ValueError("mesh_first needs n_meshes + 1 offsets")
+        self.ctx.check(lib().vkrt_pt_set_instanced_scene(self._h, ctypes.byref(sc), mf.ctypes.data, len(mf) - 1,
+                                                         ins.ctypes.data if len(ins) else None, len(ins)))
+        del keep
 
     def set_launch_mode(self, mode, tail_depth=PT_TAIL_DEPTH_DEFAULT):
         """PT_LAUNCH_FULL_GRID (default) or PT_LAUNCH_PERSISTENT per-depth launches; paths alive at tail_depth

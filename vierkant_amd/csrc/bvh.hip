@@ -288,7 +288,30 @@ static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
     info.depth = 1;
     info.sah_cost = 0.f;
     info.n_leaves = bvh->empty ? 0 : 1;
-    if(bvh->empty || bvh->root_ref < 0) return;
+    if(bvh->empty) return;
+    if(bvh->root_ref < 0)
+    {
+        // a single leaf (at most 8 triangles): the union of its triangles' boxes (tbox, in leaf order),
+        // padded like every node box
+        const uint32_t v = static_cast<uint32_t>(~bvh->root_ref), first = v >> 3, count = (v & 7u) + 1;
+        std::vector<float> box(6 * (size_t)count);
+        VKRT_HIP(hipMemcpyAsync(box.data(), bvh->tbox.as<float>() + 6 * (size_t)first, box.size() * 4,
+                                hipMemcpyDeviceToHost, s));
+        VKRT_HIP(hipStreamSynchronize(s));
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for(uint32_t k = 0; k < count; ++k)
+            for(int a = 0; a < 3; ++a)
+            {
+                lo[a] = std::min(lo[a], box[6 * (size_t)k + a]);
+                hi[a] = std::max(hi[a], box[6 * (size_t)k + 3 + a]);
+            }
+        for(int a = 0; a < 3; ++a)
+        {
+            info.bounds[a] = lo[a] - std::fabs(lo[a]) * BOX_PAD;
+            info.bounds[3 + a] = hi[a] + std::fabs(hi[a]) * BOX_PAD;
+        }
+        return;
+    }
     std::vector<vkrt_bvh_node> nodes(bvh->n_nodes);
     VKRT_HIP(hipMemcpyAsync(nodes.data(), bvh->nodes.ptr, nodes.size() * sizeof(vkrt_bvh_node), hipMemcpyDeviceToHost,
                             s));

@@ -27,6 +27,7 @@
 #include "bvh_types.hpp"
 #include "context.hpp"
 #include "radiance_cache.hpp"
+#include "scene_walk.hpp"
 #include "trace.hpp"
 #include "traverse.hpp"
 #include "vierkant/pt_math.hpp"
@@ -145,6 +146,7 @@ struct pt_buffers
     float4 *trec, *ttgt;    // training records (4 float4) and targets
     float4 *shadow;         // next-event shadow rays of one depth (SHADOW_F4 float4 each), count in counter[6]
     float2 *cone;           // ray cone per path: spread angle, width
+    uint32_t *hit_inst;     // instanced scenes: the instance of each path's hit
 };
 
 struct pt_scene_dev
@@ -158,7 +160,39 @@ struct pt_scene_dev
     const tex_desc *tex;      // texture descriptors
     const uint32_t *texels;   // RGBA8 mip levels of all textures
     const float *tan;         // tangents per vertex (normal maps)
+    // instanced scenes: per instance the shading transform (transform_t: rotation quaternion, translation and
+    // x scale, y and z scale) and the world-to-object rows; per object its object-space box half extents
+    const float4 *inst_xf;
+    const inst_dev *insts;
+    const float *object_he;
 };
+
+//! the instanced scene's top level for the trace kernels (unused by the flat-scene instances)
+struct pt_tlas
+{
+    const vkrt_bvh4_node *tlas;
+    int32_t root;
+    uint32_t empty;
+    const uint32_t *order;
+    const inst_dev *insts;
+    const blas_dev *blases;
+};
+
+//! transform_t applied to a point and to a direction (utils/transform.slang: rotate_quat, apply_transform)
+__device__ __forceinline__ vec3 rotate_quat(const float4 &q, const vec3 &v)
+{
+    const vec3 qv{q.x, q.y, q.z};
+    const vec3 inner = vierkant::rt::cross(qv, v) + v * q.w;
+    return v + vierkant::rt::cross(qv, inner) * 2.0f;
+}
+
+__device__ __forceinline__ vec3 apply_instance(const float4 *xf, const vec3 &p)
+{
+    const float4 q = xf[0], ts = xf[1], s2 = xf[2];
+    const vec3 scaled{p.x * ts.w, p.y * s2.x, p.z * s2.y};
+    const vec3 r = rotate_quat(q, scaled);
+    return vec3{r.x + ts.x, r.y + ts.y, r.z + ts.z};
+}
 
 __device__ __forceinline__ vec3 ld3f(const float *p, uint32_t i) { return vec3{p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 
@@ -367,15 +401,32 @@ __global__ void k_pt_reset_depth(uint32_t *c, int next)
 //! the closest hit of path p's current ray at this depth; back faces are culled while the path is in air.
 //! BLEND candidates compare against rnd() of a copy of payload.rng_state = xxhash32(rng_state, depth)
 //! (raypipeline.slang:150-152, :693, :716)
-template<int BLOCK = PT_BLOCK, class F = alpha_filter>
+//! INST: an instanced scene, walked through its top level `tl` (the flat arguments are unused); the hit's
+//! instance goes to b.hit_inst
+template<int BLOCK = PT_BLOCK, class F = alpha_filter, bool INST = false>
 __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris,
                                            int32_t root, bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
                                            int32_t root4, bool wide, const pt_buffers &b, uint32_t p, int32_t *smem,
-                                           F flt, uint32_t depth)
+                                           F flt, uint32_t depth, const pt_tlas &tl = pt_tlas{})
 {
     float t = 0.f, u = 0.f, v = 0.f;
     uint32_t prim = VKRT_INVALID_PRIM;
-    if(!empty)
+    if constexpr(INST)
+    {
+        uint32_t inst = VKRT_INVALID_PRIM;
+        if(!tl.empty)
+        {
+            const float4 ra = b.ray_o[p], rb = b.ray_d[p];
+            const uint4 st = b.state[p];
+            const bool in_air = (st.y & F_MEDIA_MASK) == 0u;
+            if constexpr(F::active) flt.xi = rnd_peek(xxhash32(st.x, depth));
+            t = rb.w;
+            scene_walk<false, BLOCK, PT_LDS, F>(tl.tlas, tl.root, tl.order, tl.insts, tl.blases, ra, rb, smem, t, u, v,
+                                                prim, inst, in_air, flt);
+        }
+        b.hit_inst[p] = inst;
+    }
+    else if(!empty)
     {
         ray_setup r = make_ray_setup(b.ray_o[p], b.ray_d[p]);
         const uint4 st = b.state[p];
@@ -392,11 +443,11 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
 
 //! F is alpha_filter where some material is alpha-tested, keep_all elsewhere (the any-hit test and its texture
 //! fetch compiled out)
-template<class F>
+template<class F, bool INST>
 __global__ void __launch_bounds__(PT_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(PT_TRACE_WAVES_PER_SIMD)))
 k_pt_trace(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris, int32_t root, bool empty,
            const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide, pt_buffers b, int in, F flt,
-           uint32_t depth)
+           uint32_t depth, pt_tlas tl)
 {
     __shared__ int32_t smem[PT_LDS * PT_TRACE_BLOCK];
     const uint32_t count = b.counter[in];
@@ -419,8 +470,8 @@ k_pt_trace(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ t
         if(t >= tiles) break;
         const uint32_t q = t * PT_TRACE_BLOCK + threadIdx.x;
         if(q < count)
-            trace_path<PT_TRACE_BLOCK, F>(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q],
-                                       smem + threadIdx.x, flt, depth);
+            trace_path<PT_TRACE_BLOCK, F, INST>(nodes, tris, root, empty, nodes4, root4, wide, b, b.queue[in][q],
+                                                smem + threadIdx.x, flt, depth, tl);
     }
 }
 
@@ -467,7 +518,7 @@ __device__ vec3 ghost_lights(const vkrt_light *__restrict__ lights, const pt_fra
 //! roulette, light bodies met on the segment; writes the path state back and reports what the path needs
 //! next. LIGHTS (the frame has analytic lights) compiles next-event estimation and the light-body passes in,
 //! so the unlit kernels keep their smaller register footprint
-template<bool LIGHTS>
+template<bool LIGHTS, bool INST = false>
 __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &b, const pt_scene_dev &s,
                                            uint32_t depth, uint32_t p, bool &alive, bool &to_cache, bool &need_shadow,
                                            float4 (&shadow_rec)[SHADOW_F4], float4 (&qfeat)[QUERY_F4])
@@ -527,16 +578,45 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         const uint32_t i0 = s.idx[3 * prim], i1 = s.idx[3 * prim + 1], i2 = s.idx[3 * prim + 2];
         const vec3 v0 = ld3f(s.pos, i0), v1 = ld3f(s.pos, i1), v2 = ld3f(s.pos, i2);
         const vec3 e1 = v1 - v0, e2 = v2 - v0;
-        const bool backface = dot(e1, vierkant::rt::cross(dir, e2)) < 0.f;
-        // interpolate_vertex (raypipeline.slang:336-358), identity instance transform
+        // HitKind: the determinant's sign in the triangle's own space (an instance's ray direction turned by
+        // its world-to-object rows, as the walk turned it)
+        vec3 dir_o = dir;
+        const float4 *xf = nullptr;
+        if constexpr(INST)
+        {
+            const uint32_t inst = b.hit_inst[p];
+            xf = s.inst_xf + 3 * (size_t)inst;
+            const inst_dev I = s.insts[inst];
+            dir_o = vec3{(I.w0.x * dir.x + I.w0.y * dir.y) + I.w0.z * dir.z,
+                         (I.w1.x * dir.x + I.w1.y * dir.y) + I.w1.z * dir.z,
+                         (I.w2.x * dir.x + I.w2.y * dir.y) + I.w2.z * dir.z};
+        }
+        const bool backface = dot(e1, vierkant::rt::cross(dir_o, e2)) < 0.f;
+        // interpolate_vertex (raypipeline.slang:336-358): object-space interpolation, then the instance's
+        // transform on the position and its rotation on the normal (identity for a flat scene)
         const float bx = 1.0f - u - v, by = u, bz = v;
         position = (v0 * bx + v1 * by) + v2 * bz;
         vec3 n = (ld3f(s.nrm, i0) * bx + ld3f(s.nrm, i1) * by) + ld3f(s.nrm, i2) * bz;
         normal = normalize3(n);
+        if constexpr(INST)
+        {
+            position = apply_instance(xf, position);
+            normal = rotate_quat(xf[0], normal);
+        }
         if(backface) normal = -normal;
         object = s.tri_object[prim];
         const gpu_material gm = s.mats[s.object_material[object]];
-        const float tmax_inside = s.object_tmax[object];
+        // tmax_inside_volume = 2 * length(scale * aabb.half_extents()) (raypipeline.slang:370)
+        float tmax_inside;
+        if constexpr(INST)
+        {
+            const vec3 he = ld3f(s.object_he, object);
+            const float4 ts = xf[1], s2 = xf[2];
+            const float hx = ts.w * he.x, hy = s2.x * he.y, hz = s2.y * he.z;
+            tmax_inside = 2.f * sqrtf((hx * hx + hy * hy) + hz * hz);
+        }
+        else
+            tmax_inside = s.object_tmax[object];
         const vec3 V = -dir;
         ro = make_float4(position.x, position.y, position.z, ro.w);
 
@@ -562,7 +642,15 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
             // width from before this segment's propagation
             const float2 t0 = s.uv[i0], t1 = s.uv[i1], t2 = s.uv[i2];
             const float tu = (t0.x * bx + t1.x * by) + t2.x * bz, tv = (t0.y * bx + t1.y * by) + t2.y * bz;
-            const float p_a = length3(vierkant::rt::cross(v1 - v0, v2 - v0));
+            vec3 w0 = v0, w1 = v1, w2 = v2;
+            if constexpr(INST)
+            {
+                // lod_constant: the triangle's area in world space
+                w0 = apply_instance(xf, v0);
+                w1 = apply_instance(xf, v1);
+                w2 = apply_instance(xf, v2);
+            }
+            const float p_a = length3(vierkant::rt::cross(w1 - w0, w2 - w0));
             const float t_a = fabsf((t1.x - t0.x) * (t2.y - t0.y) - (t2.x - t0.x) * (t1.y - t0.y));
             const float lod_cone = 0.5f * log2f(t_a / p_a) + log2f(fabsf(cone.y));
             const float lod_nov = log2f(fabsf(dot(V, normal)));
@@ -587,7 +675,8 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
                 nt.x = 2.0f * (c.x - 0.5f);
                 nt.y = 2.0f * (c.y - 0.5f);
                 nt.z = sqrtf(fmaxf(0.0f, 1.0f - nt.x * nt.x - nt.y * nt.y));
-                const vec3 tg = normalize3((ld3f(s.tan, i0) * bx + ld3f(s.tan, i1) * by) + ld3f(s.tan, i2) * bz);
+                vec3 tg = normalize3((ld3f(s.tan, i0) * bx + ld3f(s.tan, i1) * by) + ld3f(s.tan, i2) * bz);
+                if constexpr(INST) tg = rotate_quat(xf[0], tg);
                 vec3 bt = normalize3(vierkant::rt::cross(normal, tg));
                 if(backface) bt = -bt;
                 const vec3 nm = normalize3((tg * nt.x + bt * nt.y) + normal * nt.z);
@@ -797,7 +886,7 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
 
 //! queue entry q of a depth (inactive past the count): shade_path, then survivors, cache queries and shadow
 //! rays are appended with one ballot and one atomic per wave each
-template<bool LIGHTS>
+template<bool LIGHTS, bool INST>
 __device__ __forceinline__ void shade_queue_entry(const pt_frame &f, const pt_buffers &b, const pt_scene_dev &s,
                                                   uint32_t depth, int in, uint32_t q, uint32_t count)
 {
@@ -811,7 +900,7 @@ __device__ __forceinline__ void shade_queue_entry(const pt_frame &f, const pt_bu
     if(active)
     {
         p = b.queue[in][q];
-        shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
+        shade_path<LIGHTS, INST>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
     }
     // append survivors to the next queue: one ballot and one atomic per wave
     const uint64_t mask = __ballot(alive);
@@ -863,13 +952,13 @@ __device__ __forceinline__ void shade_queue_entry(const pt_frame &f, const pt_bu
 //! one ballot and one atomic per wave each
 //! (one tile per workgroup over a grid that covers every pixel: a grid-striding loop here keeps more values
 //! live across iterations, 190 VGPRs against 161, and drops the kernel from 3 waves per SIMD to 2)
-template<bool LIGHTS>
+template<bool LIGHTS, bool INST>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shade(pt_frame f, pt_buffers b, pt_scene_dev s, uint32_t depth,
                                                        int in)
 {
     const uint32_t count = b.counter[in];
     const uint32_t first = blockIdx.x * PT_BLOCK;
-    if(first < count) shade_queue_entry<LIGHTS>(f, b, s, depth, in, first + threadIdx.x, count);
+    if(first < count) shade_queue_entry<LIGHTS, INST>(f, b, s, depth, in, first + threadIdx.x, count);
 }
 
 //! training targets: L_d = e_d + w_d * (g_{d+1} + L_{d+1}) from the last vertex back, where e_d holds the
@@ -1085,29 +1174,186 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
     return blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
 }
 
+//! transmittance_walk over an instanced scene: the same passes, with the top level and the meshes walked in
+//! one loop (scene_walk) and the candidates ordered by (t, instance, triangle); ra, rb = world ray
+template<class F = alpha_filter>
+__device__ float4 transmittance_walk_scene(const pt_tlas &tl, const gpu_material *__restrict__ mats, const float4 &ra,
+                                           const float4 &rb, bool cull, int32_t *lds, const F &flt)
+{
+    int32_t spill[SCENE_STACK - PT_LDS];
+    lane_stack<PT_BLOCK, PT_LDS> st{lds, spill};
+    const ray_setup rw = make_ray_setup(ra, rb);
+    float T0 = 1.f, T1 = 1.f, T2 = 1.f;
+    bool blocked = false, more = !tl.empty, first_pass = true;
+    float lb_t = 0.f;
+    uint64_t lb_id = 0;
+    while(more && !blocked)
+    {
+        float kt[TK];
+        uint64_t kid[TK];
+        uint32_t km[TK];
+#pragma unroll
+        for(int j = 0; j < TK; ++j)
+        {
+            kt[j] = __int_as_float(0x7f800000);
+            kid[j] = ~0ull;
+            km[j] = 0u;
+        }
+        int cnt = 0;
+        more = false;
+        float bound = rw.tmax;
+        const float lo = first_pass ? rw.tmin : fmaxf(rw.tmin, lb_t);
+        ray_setup r = rw;
+        const vkrt_bvh4_node *nodes = tl.tlas;
+        const float4 *tris = nullptr;
+        uint32_t inst = 0;
+        int base = -1;
+        int sp = 0;
+        int32_t ref = tl.root;
+        while(true)
+        {
+            if(ref >= 0)
+            {
+                int32_t c[4];
+                const int nc = visit4(nodes, ref, r, bound, c, lo);
+                if(nc)
+                {
+                    for(int j = nc - 1; j >= 1; --j) st.put(sp++, c[j]);
+                    ref = c[0];
+                    continue;
+                }
+            }
+            else if(base < 0)
+            {
+                uint32_t first, count;
+                leaf_decode(ref, first, count);
+                inst = tl.order[first];
+                const inst_dev I = tl.insts[inst];
+                const blas_dev B = tl.blases[I.blas];
+                if(!B.empty)
+                {
+                    r = object_ray(I, ra, rb);
+                    nodes = B.nodes;
+                    tris = B.tris;
+                    base = sp;
+                    ref = B.root;
+                    continue;
+                }
+            }
+            else
+            {
+                uint32_t first, count;
+                leaf_decode(ref, first, count);
+                for(uint32_t k = 0; k < count; ++k)
+                {
+                    const size_t tb = 48 * (size_t)(first + k);
+                    const float4 p0 = load_global_f4(tris, tb), p1 = load_global_f4(tris, tb + 16),
+                                 p2 = load_global_f4(tris, tb + 32);
+                    const bool two_sided = (__float_as_uint(p2.w) & TRI_TWO_SIDED) != 0u;
+                    float t, u, v;
+                    if(!vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
+                                              vec3{p1.z, p1.w, p2.x}, t, u, v, cull && !two_sided) ||
+                       !(t > r.tmin && t < r.tmax))
+                        continue;
+                    const uint32_t mid = __float_as_uint(p2.z);
+                    const gpu_material m = mats[mid];
+                    if(!(m.mrti.z > 0.0f))
+                    {
+                        if(filtered_out(flt, p2, u, v)) continue;
+                        blocked = true;
+                        break;
+                    }
+                    const uint64_t id = ((uint64_t)inst << 32) | __float_as_uint(p2.y);
+                    if(!first_pass && (t < lb_t || (t == lb_t && id <= lb_id))) continue;
+                    if(cnt == TK && (t > kt[TK - 1] || (t == kt[TK - 1] && id > kid[TK - 1])))
+                    {
+                        more = true;
+                        continue;
+                    }
+                    if(cnt == TK) more = true;
+                    float ct = t;
+                    uint64_t cid = id;
+                    uint32_t cm = mid;
+#pragma unroll
+                    for(int j = 0; j < TK; ++j)
+                    {
+                        const bool less = ct < kt[j] || (ct == kt[j] && cid < kid[j]);
+                        const float tt = kt[j];
+                        const uint64_t ti = kid[j];
+                        const uint32_t tm = km[j];
+                        kt[j] = less ? ct : tt;
+                        kid[j] = less ? cid : ti;
+                        km[j] = less ? cm : tm;
+                        ct = less ? tt : ct;
+                        cid = less ? ti : cid;
+                        cm = less ? tm : cm;
+                    }
+                    cnt = cnt < TK ? cnt + 1 : TK;
+                    if(cnt == TK)
+                    {
+                        bound = kt[TK - 1];
+                        more = true;
+                    }
+                }
+                if(blocked) break;
+            }
+            if(sp == base)
+            {
+                r = rw;
+                nodes = tl.tlas;
+                base = -1;
+            }
+            if(sp == 0) break;
+            --sp;
+            ref = st.get(sp);
+        }
+        if(blocked) break;
+#pragma unroll
+        for(int j = 0; j < TK; ++j)
+        {
+            if(j < cnt)
+            {
+                const gpu_material m = mats[km[j]];
+                T0 = T0 * (m.mrti.z * m.color.x);
+                T1 = T1 * (m.mrti.z * m.color.y);
+                T2 = T2 * (m.mrti.z * m.color.z);
+                lb_t = kt[j];
+                lb_id = kid[j];
+            }
+        }
+        first_pass = false;
+    }
+    return blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
+}
+
+template<bool INST>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_transmittance(const vkrt_bvh_node *__restrict__ nodes2,
                                                                int32_t root2, const vkrt_bvh4_node *__restrict__ nodes4,
                                                                int32_t root4, bool wide, bool empty,
                                                                const float4 *__restrict__ tris,
                                                                const gpu_material *__restrict__ mats,
                                                                const float4 *__restrict__ rays, uint32_t n, bool cull,
-                                                               float4 *__restrict__ out, alpha_filter flt)
+                                                               float4 *__restrict__ out, alpha_filter flt, pt_tlas tl)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t i = blockIdx.x * PT_BLOCK + threadIdx.x;
     if(i >= n) return;
-    out[i] = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, setup_ray(rays, i), cull,
-                                smem + threadIdx.x, flt);
+    if constexpr(INST)
+        out[i] = transmittance_walk_scene(tl, mats, rays[2 * (size_t)i], rays[2 * (size_t)i + 1], cull,
+                                          smem + threadIdx.x, flt);
+    else
+        out[i] = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, setup_ray(rays, i), cull,
+                                    smem + threadIdx.x, flt);
 }
 
 //! trace one shadow record and add its next-event estimate to the pixel (and to its training vertex)
-template<class F = alpha_filter>
+template<class F = alpha_filter, bool INST = false>
 __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                             const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide,
                                             bool empty, const float4 *__restrict__ tris,
                                             const gpu_material *__restrict__ mats, const pt_frame &f,
                                             const pt_buffers &b, const float4 (&rec)[SHADOW_F4], int32_t *smem,
-                                            F flt)
+                                            F flt, const pt_tlas &tl = pt_tlas{})
 {
     const float4 o = rec[0], d = rec[1];
     if constexpr(F::active)
@@ -1115,8 +1361,10 @@ __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ no
         flt.per_candidate = true;
         flt.seed = __float_as_uint(rec[5].z);
     }
-    const float4 T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d),
-                                        false, smem, flt);
+    float4 T;
+    if constexpr(INST) T = transmittance_walk_scene(tl, mats, o, d, false, smem, flt);
+    else T = transmittance_walk(nodes2, root2, nodes4, root4, wide, empty, tris, mats, make_ray_setup(o, d), false, smem,
+                                flt);
     if(!(T.x > 0.0f || T.y > 0.0f || T.z > 0.0f)) return;
     const float4 lr = rec[2], fc = rec[3], bp = rec[4], id = rec[5];
     vec3 nee = vec3{T.x, T.y, T.z} * lr.w;
@@ -1140,12 +1388,12 @@ __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ no
 //! the shadow half of nee_surface (direct_lighting.slang:311-333): the unculled transmittance toward the
 //! sampled light, then transmittance * weight * n_lights * radiance * F * cos / pdf scaled by the vertex's
 //! throughput joins the pixel's sample sum; a training vertex also records the unscaled estimate
-template<class F>
+template<class F, bool INST>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                                         const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
                                                         bool wide, bool empty, const float4 *__restrict__ tris,
                                                         const gpu_material *__restrict__ mats, pt_frame f,
-                                                        pt_buffers b, F flt)
+                                                        pt_buffers b, F flt, pt_tlas tl)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t count = b.counter[6];
@@ -1154,7 +1402,8 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__r
         const float4 *rec = b.shadow + (size_t)i * SHADOW_F4;
         float4 r[SHADOW_F4];
         for(uint32_t k = 0; k < SHADOW_F4; ++k) r[k] = rec[k];
-        shadow_path<F>(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x, flt);
+        shadow_path<F, INST>(nodes2, root2, nodes4, root4, wide, empty, tris, mats, f, b, r, smem + threadIdx.x, flt,
+                             tl);
     }
 }
 
@@ -1162,12 +1411,12 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__r
 //! of the next-event estimate, with no kernel boundary between depths. Launched on the second stream at the
 //! start of a sample, its few waves stay resident beside the main paths' full-width launches, so the chain of
 //! dependent traversals does not wait for free compute units at each depth
-template<bool LIGHTS>
+template<bool LIGHTS, bool INST>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node *__restrict__ nodes2,
                                                              const float4 *__restrict__ tris, int32_t root2,
                                                              bool empty, const vkrt_bvh4_node *__restrict__ nodes4,
                                                              int32_t root4, bool wide, pt_frame f, pt_buffers b,
-                                                             pt_scene_dev s)
+                                                             pt_scene_dev s, pt_tlas tl)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     const uint32_t q = blockIdx.x * PT_BLOCK + threadIdx.x;
@@ -1178,15 +1427,16 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node
     uint32_t segments = 0;
     for(uint32_t depth = 0; depth < f.max_depth; ++depth)
     {
-        trace_path(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x, flt, depth);
+        trace_path<PT_BLOCK, alpha_filter, INST>(nodes2, tris, root2, empty, nodes4, root4, wide, b, p,
+                                                 smem + threadIdx.x, flt, depth, tl);
         bool alive = false, to_cache = false, need_shadow = false;
         float4 shadow_rec[SHADOW_F4];
         float4 qfeat[QUERY_F4];
-        shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
+        shade_path<LIGHTS, INST>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
         ++segments;
         if(LIGHTS && need_shadow)
-            shadow_path(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec, smem + threadIdx.x,
-                        shadow_flt);
+            shadow_path<alpha_filter, INST>(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec,
+                                            smem + threadIdx.x, shadow_flt, tl);
         if(!alive) break;
     }
     atomicAdd(reinterpret_cast<unsigned long long *>(&b.counter[2]), (unsigned long long)segments);
@@ -1198,12 +1448,12 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_train_paths(const vkrt_bvh_node
 //! traversal (a chain of dependent node loads); here a wave's lanes run their depths back to back and the
 //! waves overlap. Each path performs the operations of the per-depth kernels in the same order (trace, shade,
 //! shadow ray, next depth), so the image is bit for bit the same.
-template<bool LIGHTS, class F>
+template<bool LIGHTS, class F, bool INST>
 __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(const vkrt_bvh_node *__restrict__ nodes2,
                                                         const float4 *__restrict__ tris, int32_t root2, bool empty,
                                                         const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
                                                         bool wide, pt_frame f, pt_buffers b, pt_scene_dev s, F flt,
-                                                        F shadow_flt, uint32_t d0, int in)
+                                                        F shadow_flt, uint32_t d0, int in, pt_tlas tl)
 {
     __shared__ int32_t smem[PT_LDS * PT_BLOCK];
     __shared__ unsigned long long s_segments;
@@ -1217,16 +1467,16 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_finish(const vkrt_bvh_node *__r
         const uint32_t p = b.queue[in][q];
         for(uint32_t depth = d0; depth < f.max_depth; ++depth)
         {
-            trace_path<PT_BLOCK, F>(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x, flt,
-                                    depth);
+            trace_path<PT_BLOCK, F, INST>(nodes2, tris, root2, empty, nodes4, root4, wide, b, p, smem + threadIdx.x,
+                                          flt, depth, tl);
             bool alive = false, to_cache = false, need_shadow = false;
             float4 shadow_rec[SHADOW_F4];
             float4 qfeat[QUERY_F4];
-            shade_path<LIGHTS>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
+            shade_path<LIGHTS, INST>(f, b, s, depth, p, alive, to_cache, need_shadow, shadow_rec, qfeat);
             ++segments;
             if(LIGHTS && need_shadow)
-                shadow_path<F>(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec,
-                               smem + threadIdx.x, shadow_flt);
+                shadow_path<F, INST>(nodes2, root2, nodes4, root4, wide, empty, tris, s.mats, f, b, shadow_rec,
+                                     smem + threadIdx.x, shadow_flt, tl);
             if(!alive) break;
         }
     }
@@ -1311,13 +1561,17 @@ static float4 *run_denoise(hipStream_t s, const vkrt_pixel *px, uint32_t w, uint
     return bufs[(passes - 1) % 2];
 }
 
+//! the leaf records' material and flags; prim_offset turns a mesh's own triangle numbers into the scene's
+//! (an instanced scene's meshes are triangle ranges of one list)
 __global__ void k_pt_tag(float4 *__restrict__ tris, uint32_t n, const uint32_t *__restrict__ tri_object,
-                         const uint32_t *__restrict__ object_material, const gpu_material *__restrict__ mats)
+                         const uint32_t *__restrict__ object_material, const gpu_material *__restrict__ mats,
+                         uint32_t prim_offset)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if(j >= n) return;
     float4 r = tris[3 * (size_t)j + 2];
-    const uint32_t prim = __float_as_uint(r.y);
+    const uint32_t prim = __float_as_uint(r.y) + prim_offset;
+    r.y = __uint_as_float(prim);
     const uint32_t mat = object_material[tri_object[prim]];
     r.z = __uint_as_float(mat);
     r.w = __uint_as_float((mats[mat].sheen.w != 0.f ? TRI_TWO_SIDED : 0u) |
@@ -1337,6 +1591,15 @@ struct vkrt_pt
     int launch_mode = VKRT_PT_LAUNCH_FULL_GRID;// per-depth launch shape (vkrt_pt_set_launch_mode)
     uint32_t tail_depth = vkrt::PT_TAIL_DEPTH;  // survivors at this depth finish in one lane (k_pt_finish)
     bool any_alpha = false;                    // some material is alpha-tested (MASK or BLEND)
+    // instanced scenes (vkrt_pt_set_instanced_scene): one BVH per mesh, the top level over the instances, per
+    // instance its shading transform, per object its object-space half extents; `bounds` is the scene's world
+    // box either way (the cache grid's extent)
+    bool instanced = false;
+    std::vector<vkrt_bvh *> mesh_bvhs;
+    vkrt_scene *scene = nullptr;
+    vkrt::pt_tlas tlas{};
+    vkrt::device_buffer inst_xf, object_he, hit_inst;
+    float bounds[6] = {0.f, 0.f, 0.f, -1.f, -1.f, -1.f};
     std::minstd_rand0 random_engine;// std::default_random_engine of libstdc++ (PBRPathTracer.cpp:32)
     uint32_t batch_index = 0;
     vkrt_bvh *bvh = nullptr;
@@ -1403,6 +1666,7 @@ static void resize_storage(vkrt_pt *pt, uint32_t w, uint32_t h)
     pt->ray_o.reserve(n * 16);
     pt->ray_d.reserve(n * 16);
     pt->hit.reserve(n * 16);
+    pt->hit_inst.reserve(n * 4);
     pt->beta.reserve(n * 16);
     pt->rad.reserve(n * 16);
     pt->state.reserve(n * 16);
@@ -1439,6 +1703,7 @@ static pt_buffers buffers(vkrt_pt *pt)
     b.pixels = pt->pixels.as<vkrt_pixel>();
     b.qrec = b.tvtx = b.trec = b.ttgt = b.shadow = nullptr;
     b.train_queue = nullptr;
+    b.hit_inst = pt->hit_inst.as<uint32_t>();
     return b;
 }
 
@@ -1448,7 +1713,8 @@ static pt_scene_dev scene_dev(vkrt_pt *pt)
     return pt_scene_dev{pt->pos.as<float>(), pt->nrm.as<float>(), pt->idx.as<uint32_t>(),
                         pt->tri_object.as<uint32_t>(), pt->object_material.as<uint32_t>(), pt->object_tmax.as<float>(),
                         pt->mats.as<gpu_material>(), pt->lights.as<vkrt_light>(), pt->uv.as<float2>(),
-                        pt->tex_desc_buf.as<tex_desc>(), pt->texels.as<uint32_t>(), pt->tan.as<float>()};
+                        pt->tex_desc_buf.as<tex_desc>(), pt->texels.as<uint32_t>(), pt->tan.as<float>(),
+                        pt->inst_xf.as<float4>(), pt->tlas.insts, pt->object_he.as<float>()};
 }
 
 //! f32tof16 on the host: round to nearest even at the binade's spacing of halves (2^-24 below 2^-14),
@@ -1520,8 +1786,8 @@ static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
 //! the cache's grid covers the scene bounds, padded by 1% per side
 static void fit_cache_bounds(vkrt_pt *pt)
 {
-    if(!pt->rc || !pt->bvh) return;
-    const float *bb = pt->bvh->info.bounds;
+    if(!pt->rc || !(pt->bvh || pt->instanced)) return;
+    const float *bb = pt->bounds;
     float lo[3], hi[3];
     for(int a = 0; a < 3; ++a)
     {
@@ -1533,6 +1799,14 @@ static void fit_cache_bounds(vkrt_pt *pt)
     }
     const int rc = vkrt_rc_set_bounds(pt->rc, lo, hi);
     if(rc != VKRT_OK) throw error(rc, "radiance cache bounds: " + pt->ctx->last_error);
+}
+
+//! the filter object of instance F: the alpha filter, or keep_all
+template<class F>
+static F filter_of(const alpha_filter &a)
+{
+    if constexpr(std::is_same_v<F, alpha_filter>) return a;
+    else return F{};
 }
 
 static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
@@ -1579,9 +1853,12 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
             bt.shadow = b.shadow + (size_t)f.n_pix * SHADOW_F4;
         }
     }
-    const vkrt_bvh_node *nodes2 = bvh->nodes.as<vkrt_bvh_node>();
-    const vkrt_bvh4_node *nodes4 = bvh->nodes4.as<vkrt_bvh4_node>();
-    const float4 *tris = bvh->tris.as<float4>();
+    // the flat scene's BVH (an instanced scene has none: its kernels walk pt->tlas)
+    const vkrt_bvh_node *nodes2 = bvh ? bvh->nodes.as<vkrt_bvh_node>() : nullptr;
+    const vkrt_bvh4_node *nodes4 = bvh ? bvh->nodes4.as<vkrt_bvh4_node>() : nullptr;
+    const float4 *tris = bvh ? bvh->tris.as<float4>() : nullptr;
+    const int32_t root_ref = bvh ? bvh->root_ref : 0, root4 = bvh ? bvh->root4 : 0;
+    const bool flat_empty = bvh ? bvh->empty : true, wide = bvh ? bvh->wide : false;
     // one depth of a queue of paths: trace, shade, and the shadow rays of next-event estimation
     // scenes with no alpha-tested material take the trace and shadow kernels without the any-hit filter
     const bool alpha = pt->any_alpha;
@@ -1597,30 +1874,41 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         return full ? cap : std::min<uint32_t>(cus * static_cast<uint32_t>(std::max(per_cu, 1)), cap);
     };
     const uint32_t trace_full = div_up(f.n_pix, PT_TRACE_BLOCK);
-    const uint32_t trace_grid = alpha ? resident(k_pt_trace<alpha_filter>, PT_TRACE_BLOCK, trace_full)
-                                      : resident(k_pt_trace<keep_all>, PT_TRACE_BLOCK, trace_full);
-    const uint32_t shadow_grid = alpha ? resident(k_pt_shadow<alpha_filter>, PT_BLOCK, blocks)
-                                       : resident(k_pt_shadow<keep_all>, PT_BLOCK, blocks);
+    // compile-time variants: any-hit filter or none, flat or instanced scene
+    const bool inst = pt->instanced;
+    const pt_tlas tl = pt->tlas;
+    auto variant = [&](auto fn) {
+        auto by_inst = [&](auto a) {
+            if(inst) fn(a, std::true_type{});
+            else fn(a, std::false_type{});
+        };
+        if(alpha) by_inst(std::true_type{});
+        else by_inst(std::false_type{});
+    };
+    uint32_t trace_grid = 0, shadow_grid = 0;
+    variant([&](auto a, auto in_) {
+        using F = std::conditional_t<decltype(a)::value, alpha_filter, keep_all>;
+        trace_grid = resident(k_pt_trace<F, decltype(in_)::value>, PT_TRACE_BLOCK, trace_full);
+        shadow_grid = resident(k_pt_shadow<F, decltype(in_)::value>, PT_BLOCK, blocks);
+    });
     auto depth_step = [&](hipStream_t st, const pt_buffers &bb, uint32_t depth, int in) {
         k_pt_reset_depth<<<1, XCDS, 0, st>>>(bb.counter, in ^ 1);
-        if(alpha)
-            k_pt_trace<alpha_filter><<<trace_grid, PT_TRACE_BLOCK, 0, st>>>(
-                nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, bb, in, flt, depth);
-        else
-            k_pt_trace<keep_all><<<trace_grid, PT_TRACE_BLOCK, 0, st>>>(
-                nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, bb, in, keep_all{}, depth);
-        if(f.n_lights)
-        {
-            k_pt_shade<true><<<blocks, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
-            if(alpha)
-                k_pt_shadow<alpha_filter><<<shadow_grid, PT_BLOCK, 0, st>>>(
-                    nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty, tris, sc.mats, f, bb, shadow_flt);
+        variant([&](auto a, auto in_) {
+            using F = std::conditional_t<decltype(a)::value, alpha_filter, keep_all>;
+            constexpr bool I = decltype(in_)::value;
+            k_pt_trace<F, I><<<trace_grid, PT_TRACE_BLOCK, 0, st>>>(nodes2, tris, root_ref, flat_empty, nodes4,
+                                                                    root4, wide, bb, in, filter_of<F>(flt),
+                                                                    depth, tl);
+            if(f.n_lights)
+            {
+                k_pt_shade<true, I><<<blocks, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+                k_pt_shadow<F, I><<<shadow_grid, PT_BLOCK, 0, st>>>(nodes2, root_ref, nodes4, root4,
+                                                                    wide, flat_empty, tris, sc.mats, f, bb,
+                                                                    filter_of<F>(shadow_flt), tl);
+            }
             else
-                k_pt_shadow<keep_all><<<shadow_grid, PT_BLOCK, 0, st>>>(
-                    nodes2, bvh->root_ref, nodes4, bvh->root4, bvh->wide, bvh->empty, tris, sc.mats, f, bb, keep_all{});
-        }
-        else
-            k_pt_shade<false><<<blocks, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+                k_pt_shade<false, I><<<blocks, PT_BLOCK, 0, st>>>(f, bb, sc, depth, in);
+        });
     };
     VKRT_HIP(hipEventRecord(pt->ev0, s));
     k_pt_zero<<<1, 2 * TRAIN_COUNTERS, 0, s>>>(b.counter);
@@ -1652,12 +1940,15 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
             // main paths terminate into the cache as trained on this sample's training paths
             VKRT_HIP(hipStreamWaitEvent(pt->train_stream, pt->ev_fork, 0));
             const uint32_t tgrid = div_up(f.n_train_paths, PT_BLOCK);
-            if(f.n_lights)
-                k_pt_train_paths<true><<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(
-                        nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, bt, sc);
-            else
-                k_pt_train_paths<false><<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(
-                        nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, bt, sc);
+            auto train = [&](auto lights, auto in_) {
+                k_pt_train_paths<decltype(lights)::value, decltype(in_)::value>
+                        <<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(nodes2, tris, root_ref, flat_empty, nodes4,
+                                                                   root4, wide, f, bt, sc, tl);
+            };
+            if(f.n_lights && inst) train(std::true_type{}, std::true_type{});
+            else if(f.n_lights) train(std::true_type{}, std::false_type{});
+            else if(inst) train(std::false_type{}, std::true_type{});
+            else train(std::false_type{}, std::false_type{});
             if(f.train_max)
             {
                 k_pt_train_targets<<<tgrid, PT_BLOCK, 0, pt->train_stream>>>(f, b);
@@ -1670,22 +1961,18 @@ static void trace_batch(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
         if(per_depth < main_depths && per_depth > 0)
         {
             const int in = static_cast<int>(per_depth & 1u);
-            if(alpha && f.n_lights)
-                k_pt_finish<true, alpha_filter><<<blocks, PT_BLOCK, 0, s>>>(
-                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, flt, shadow_flt,
-                    per_depth, in);
-            else if(alpha)
-                k_pt_finish<false, alpha_filter><<<blocks, PT_BLOCK, 0, s>>>(
-                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, flt, shadow_flt,
-                    per_depth, in);
-            else if(f.n_lights)
-                k_pt_finish<true, keep_all><<<blocks, PT_BLOCK, 0, s>>>(
-                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, keep_all{},
-                    keep_all{}, per_depth, in);
-            else
-                k_pt_finish<false, keep_all><<<blocks, PT_BLOCK, 0, s>>>(
-                    nodes2, tris, bvh->root_ref, bvh->empty, nodes4, bvh->root4, bvh->wide, f, b, sc, keep_all{},
-                    keep_all{}, per_depth, in);
+            variant([&](auto a, auto in_) {
+                using F = std::conditional_t<decltype(a)::value, alpha_filter, keep_all>;
+                constexpr bool I = decltype(in_)::value;
+                if(f.n_lights)
+                    k_pt_finish<true, F, I><<<blocks, PT_BLOCK, 0, s>>>(
+                        nodes2, tris, root_ref, flat_empty, nodes4, root4, wide, f, b, sc,
+                        filter_of<F>(flt), filter_of<F>(shadow_flt), per_depth, in, tl);
+                else
+                    k_pt_finish<false, F, I><<<blocks, PT_BLOCK, 0, s>>>(
+                        nodes2, tris, root_ref, flat_empty, nodes4, root4, wide, f, b, sc,
+                        filter_of<F>(flt), filter_of<F>(shadow_flt), per_depth, in, tl);
+            });
         }
         if(f.cache_on)
         {
@@ -1739,13 +2026,18 @@ int vkrt_pt_create(vkrt_ctx *ctx, const vkrt_pt_settings *settings, uint32_t see
     });
 }
 
+namespace vkrt
+{
+static void release_scene_accel(vkrt_pt *pt);
+}
+
 void vkrt_pt_destroy(vkrt_pt *pt)
 {
     if(!pt) return;
     (void)hipSetDevice(pt->ctx->device);
     (void)hipStreamSynchronize(pt->ctx->stream);
     if(pt->train_stream) (void)hipStreamSynchronize(pt->train_stream);
-    if(pt->bvh) vkrt_bvh_destroy(pt->bvh);
+    release_scene_accel(pt);
     if(pt->ev0) (void)hipEventDestroy(pt->ev0);
     if(pt->ev1) (void)hipEventDestroy(pt->ev1);
     if(pt->ev_fork) (void)hipEventDestroy(pt->ev_fork);
@@ -1824,10 +2116,12 @@ int vkrt_pt_set_lights(vkrt_pt *pt, const vkrt_light *lights, uint32_t n)
     });
 }
 
-int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
+namespace vkrt
 {
-    if(!pt) return VKRT_E_INVALID_ARG;
-    return guarded(pt->ctx, [&] {
+
+//! the scene's arrays, materials and textures on the device (the part of set_scene both scene kinds share)
+static void upload_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
+{
         VKRT_REQUIRE(sc, "null scene");
         VKRT_REQUIRE(sc->n_tri == 0 || (sc->positions && sc->normals && sc->indices && sc->tri_object),
                      "null scene arrays");
@@ -1854,12 +2148,14 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
             }
         }
         // tmax_inside_volume = 2 * length(scale * aabb.half_extents()) (raypipeline.slang:370), unit scale
-        std::vector<float> tmax(sc->n_objects);
+        // half extents kept per object: an instance scales them before the length
+        std::vector<float> tmax(sc->n_objects), half(3 * (size_t)sc->n_objects);
         for(uint32_t o = 0; o < sc->n_objects; ++o)
         {
             float he[3];
-            for(int a = 0; a < 3; ++a) he[a] = (hi[3 * o + a] - lo[3 * o + a]) / 2.f;
-            tmax[o] = hi[3 * o] < lo[3 * o] ? 0.f : 2.f * std::sqrt((he[0] * he[0] + he[1] * he[1]) + he[2] * he[2]);
+            for(int a = 0; a < 3; ++a) he[a] = hi[3 * o] < lo[3 * o] ? 0.f : (hi[3 * o + a] - lo[3 * o + a]) / 2.f;
+            for(int a = 0; a < 3; ++a) half[3 * (size_t)o + a] = he[a];
+            tmax[o] = 2.f * std::sqrt((he[0] * he[0] + he[1] * he[1]) + he[2] * he[2]);
         }
         std::vector<gpu_material> mats(sc->n_materials);
         for(uint32_t m = 0; m < sc->n_materials; ++m)
@@ -1952,6 +2248,7 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         upload(pt->tri_object, sc->tri_object, (size_t)sc->n_tri * 4);
         upload(pt->object_material, sc->object_material, (size_t)sc->n_objects * 4);
         upload(pt->object_tmax, tmax.data(), tmax.size() * 4);
+        upload(pt->object_he, half.data(), half.size() * 4);
         upload(pt->mats, mats.data(), mats.size() * sizeof(gpu_material));
         pt->any_alpha = std::any_of(mats.begin(), mats.end(),
                                     [](const gpu_material &m) { return m.tex.y != VKRT_ALPHA_OPAQUE; });
@@ -1960,30 +2257,153 @@ int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         upload(pt->tex_desc_buf, descs.data(), descs.size() * sizeof(tex_desc));
         upload(pt->texels, texels.data(), texels.size() * 4);
         VKRT_HIP(hipStreamSynchronize(s));
-        if(pt->bvh)
-        {
-            vkrt_bvh_destroy(pt->bvh);
-            pt->bvh = nullptr;
-        }
-        vkrt_bvh_params params{};
-        params.builder = VKRT_BUILDER_PLOC;// lower SAH cost than the LBVH, 10-35 % faster traversal
-        params.max_leaf_size = 2;
-        vkrt_bvh *bvh = nullptr;
-        int rc = vkrt_bvh_build(pt->ctx, pt->pos.as<float>(), sc->n_vert, 12, pt->idx.as<uint32_t>(), sc->n_tri,
-                                &params, VKRT_DEVICE_PTRS, &bvh);
-        if(rc != VKRT_OK) throw error(rc, "BVH build failed: " + pt->ctx->last_error);
-        pt->bvh = bvh;
-        if(sc->n_tri)
-            k_pt_tag<<<div_up(sc->n_tri, 256), 256, 0, s>>>(bvh->tris.as<float4>(), sc->n_tri,
-                                                              pt->tri_object.as<uint32_t>(),
-                                                              pt->object_material.as<uint32_t>(),
-                                                              pt->mats.as<gpu_material>());
-        VKRT_HIP(hipStreamSynchronize(s));
         pt->n_tri = sc->n_tri;
         pt->n_vert = sc->n_vert;
         pt->n_objects = sc->n_objects;
         pt->n_materials = sc->n_materials;
         pt->environment = sc->environment;
+}
+
+//! the 3 x 4 object-to-world matrix of an instance: columns R(q) (S e_j), the last column T, the map
+//! apply_transform computes (transform.hpp: rotate(q, p * S) + T), evaluated in double and rounded once
+static void instance_matrix(const vkrt_pt_instance &in, float m[12])
+{
+    const double qx = in.rotation[0], qy = in.rotation[1], qz = in.rotation[2], qw = in.rotation[3];
+    for(int j = 0; j < 3; ++j)
+    {
+        double v[3] = {0.0, 0.0, 0.0};
+        v[j] = in.scale[j];
+        // v + 2 q.xyz x (q.xyz x v + w v)
+        const double c1[3] = {qy * v[2] - qz * v[1] + qw * v[0], qz * v[0] - qx * v[2] + qw * v[1],
+                              qx * v[1] - qy * v[0] + qw * v[2]};
+        const double c2[3] = {qy * c1[2] - qz * c1[1], qz * c1[0] - qx * c1[2], qx * c1[1] - qy * c1[0]};
+        for(int a = 0; a < 3; ++a) m[4 * a + j] = static_cast<float>(v[a] + 2.0 * c2[a]);
+    }
+    for(int a = 0; a < 3; ++a) m[4 * a + 3] = in.translation[a];
+}
+
+//! frees the acceleration structures of the current scene (either kind)
+static void release_scene_accel(vkrt_pt *pt)
+{
+    if(pt->scene) vkrt_scene_destroy(pt->scene);
+    pt->scene = nullptr;
+    for(vkrt_bvh *b: pt->mesh_bvhs) vkrt_bvh_destroy(b);
+    pt->mesh_bvhs.clear();
+    if(pt->bvh) vkrt_bvh_destroy(pt->bvh);
+    pt->bvh = nullptr;
+    pt->instanced = false;
+    pt->tlas = pt_tlas{};
+}
+
+//! a PLOC-built BVH (lower SAH cost than the LBVH, 10-35 % faster traversal) over triangles [first, first + n)
+//! of the uploaded scene, its leaf records tagged with materials, flags and the scene's triangle numbers
+static vkrt_bvh *build_tagged_bvh(vkrt_pt *pt, uint32_t first, uint32_t n)
+{
+    vkrt_bvh_params params{};
+    params.builder = VKRT_BUILDER_PLOC;
+    params.max_leaf_size = 2;
+    vkrt_bvh *bvh = nullptr;
+    const int rc = vkrt_bvh_build(pt->ctx, pt->pos.as<float>(), pt->n_vert, 12, pt->idx.as<uint32_t>() + 3 * (size_t)first,
+                                  n, &params, VKRT_DEVICE_PTRS, &bvh);
+    if(rc != VKRT_OK) throw error(rc, "BVH build failed: " + pt->ctx->last_error);
+    if(n)
+        k_pt_tag<<<div_up(n, 256), 256, 0, pt->ctx->stream>>>(bvh->tris.as<float4>(), n,
+                                                               pt->tri_object.as<uint32_t>(),
+                                                               pt->object_material.as<uint32_t>(),
+                                                               pt->mats.as<gpu_material>(), first);
+    VKRT_HIP(hipGetLastError());
+    return bvh;
+}
+
+}// namespace vkrt
+
+int vkrt_pt_set_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        upload_scene(pt, sc);
+        release_scene_accel(pt);
+        pt->bvh = build_tagged_bvh(pt, 0, sc->n_tri);
+        VKRT_HIP(hipStreamSynchronize(pt->ctx->stream));
+        for(int k = 0; k < 6; ++k) pt->bounds[k] = pt->bvh->info.bounds[k];
+        fit_cache_bounds(pt);
+    });
+}
+
+int vkrt_pt_set_instanced_scene(vkrt_pt *pt, const vkrt_pt_scene *sc, const uint32_t *mesh_first, uint32_t n_meshes,
+                                const vkrt_pt_instance *instances, uint32_t n_instances)
+{
+    if(!pt) return VKRT_E_INVALID_ARG;
+    return guarded(pt->ctx, [&] {
+        VKRT_REQUIRE(sc, "null scene");
+        VKRT_REQUIRE(n_meshes > 0 && mesh_first, "an instanced scene needs meshes");
+        VKRT_REQUIRE(n_instances == 0 || instances, "null instances");
+        VKRT_REQUIRE(mesh_first[0] == 0 && mesh_first[n_meshes] == sc->n_tri, "mesh ranges must cover the triangles");
+        for(uint32_t m = 0; m < n_meshes; ++m)
+            VKRT_REQUIRE(mesh_first[m] <= mesh_first[m + 1], "mesh ranges must not decrease");
+        for(uint32_t i = 0; i < n_instances; ++i)
+        {
+            const vkrt_pt_instance &in = instances[i];
+            VKRT_REQUIRE(in.mesh < n_meshes, "instance mesh index out of range");
+            bool finite = true;
+            for(int k = 0; k < 3; ++k)
+                finite = finite && std::isfinite(in.translation[k]) && std::isfinite(in.scale[k]) && in.scale[k] != 0.f;
+            for(int k = 0; k < 4; ++k) finite = finite && std::isfinite(in.rotation[k]);
+            VKRT_REQUIRE(finite, "instance transforms must be finite with non-zero scale");
+        }
+        upload_scene(pt, sc);
+        release_scene_accel(pt);
+        hipStream_t s = pt->ctx->stream;
+        for(uint32_t m = 0; m < n_meshes; ++m)
+            pt->mesh_bvhs.push_back(build_tagged_bvh(pt, mesh_first[m], mesh_first[m + 1] - mesh_first[m]));
+        VKRT_HIP(hipStreamSynchronize(s));
+        // the instances' matrices (object to world = T R S, the map apply_transform computes), their shading
+        // transforms, and the world box
+        std::vector<vkrt_instance> insts(n_instances);
+        std::vector<float4> xf(3 * (size_t)n_instances);
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for(uint32_t i = 0; i < n_instances; ++i)
+        {
+            const vkrt_pt_instance &in = instances[i];
+            float m12[12];
+            instance_matrix(in, m12);
+            insts[i] = vkrt_instance{};
+            vkrt_instance_set_transform(&insts[i], m12);
+            insts[i].blas = in.mesh;
+            insts[i].custom_index = i;
+            xf[3 * i] = make_float4(in.rotation[0], in.rotation[1], in.rotation[2], in.rotation[3]);
+            xf[3 * i + 1] = make_float4(in.translation[0], in.translation[1], in.translation[2], in.scale[0]);
+            xf[3 * i + 2] = make_float4(in.scale[1], in.scale[2], 0.f, 0.f);
+            const vkrt_bvh *b = pt->mesh_bvhs[in.mesh];
+            if(b->empty) continue;
+            const float *bb = b->info.bounds;
+            for(int c = 0; c < 8; ++c)
+            {
+                const double q[3] = {(c & 1) ? bb[3] : bb[0], (c & 2) ? bb[4] : bb[1], (c & 4) ? bb[5] : bb[2]};
+                for(int a = 0; a < 3; ++a)
+                {
+                    const double w = (double)m12[4 * a] * q[0] + (double)m12[4 * a + 1] * q[1] +
+                                     (double)m12[4 * a + 2] * q[2] + (double)m12[4 * a + 3];
+                    lo[a] = std::min(lo[a], w);
+                    hi[a] = std::max(hi[a], w);
+                }
+            }
+        }
+        const int rc = vkrt_scene_create(pt->ctx, pt->mesh_bvhs.data(), n_meshes, insts.data(), n_instances,
+                                         &pt->scene);
+        if(rc != VKRT_OK) throw error(rc, "instanced scene: " + pt->ctx->last_error);
+        const scene_view v = scene_device_view(pt->scene);
+        pt->tlas = pt_tlas{v.tlas, v.root, v.empty, v.order, v.insts, v.blases};
+        pt->inst_xf.reserve(std::max<size_t>(1, xf.size()) * 16);
+        if(!xf.empty())
+            VKRT_HIP(hipMemcpyAsync(pt->inst_xf.ptr, xf.data(), xf.size() * 16, hipMemcpyHostToDevice, s));
+        VKRT_HIP(hipStreamSynchronize(s));
+        for(int a = 0; a < 3; ++a)
+        {
+            pt->bounds[a] = static_cast<float>(lo[a]);
+            pt->bounds[a + 3] = static_cast<float>(hi[a]);
+        }
+        pt->instanced = true;
         fit_cache_bounds(pt);
     });
 }
@@ -1993,18 +2413,26 @@ int vkrt_pt_trace_transmittance(vkrt_pt *pt, const vkrt_ray *rays, uint32_t n, f
     if(!pt) return VKRT_E_INVALID_ARG;
     vkrt_ctx *ctx = pt->ctx;
     return guarded(ctx, [&] {
-        VKRT_REQUIRE(pt->bvh, "vkrt_pt_set_scene first");
+        VKRT_REQUIRE(pt->bvh || pt->instanced, "vkrt_pt_set_scene first");
         VKRT_REQUIRE(n == 0 || (rays && out4), "null array argument");
         if(!n) return;
         const vkrt_ray *dr = input(ctx, 1, rays, n, flags);
         float4 *dout = reinterpret_cast<float4 *>(output(ctx, 2, out4, (size_t)n * 4, flags));
-        const vkrt_bvh *b = pt->bvh;
         const pt_scene_dev sc = scene_dev(pt);
         const alpha_filter flt{sc.idx, sc.uv, sc.mats, sc.tex, sc.texels, false};
-        k_pt_transmittance<<<div_up(n, PT_BLOCK), PT_BLOCK, 0, ctx->stream>>>(
-                b->nodes.as<vkrt_bvh_node>(), b->root_ref, b->nodes4.as<vkrt_bvh4_node>(), b->root4, b->wide, b->empty,
-                b->tris.as<float4>(), pt->mats.as<gpu_material>(), reinterpret_cast<const float4 *>(dr), n,
-                (flags & VKRT_CULL_BACK) != 0, dout, flt);
+        const bool cull = (flags & VKRT_CULL_BACK) != 0;
+        const float4 *r4 = reinterpret_cast<const float4 *>(dr);
+        if(pt->instanced)
+            k_pt_transmittance<true><<<div_up(n, PT_BLOCK), PT_BLOCK, 0, ctx->stream>>>(
+                    nullptr, 0, nullptr, 0, false, true, nullptr, pt->mats.as<gpu_material>(), r4, n, cull, dout, flt,
+                    pt->tlas);
+        else
+        {
+            const vkrt_bvh *b = pt->bvh;
+            k_pt_transmittance<false><<<div_up(n, PT_BLOCK), PT_BLOCK, 0, ctx->stream>>>(
+                    b->nodes.as<vkrt_bvh_node>(), b->root_ref, b->nodes4.as<vkrt_bvh4_node>(), b->root4, b->wide,
+                    b->empty, b->tris.as<float4>(), pt->mats.as<gpu_material>(), r4, n, cull, dout, flt, pt->tlas);
+        }
         VKRT_HIP(hipGetLastError());
         finish_output(ctx, 2, out4, (size_t)n * 4, flags);
         finish_call(ctx, flags);
@@ -2098,7 +2526,7 @@ int vkrt_pt_render(vkrt_pt *pt, const vkrt_camera *camera)
     if(!pt) return VKRT_E_INVALID_ARG;
     return guarded(pt->ctx, [&] {
         VKRT_REQUIRE(camera, "null camera");
-        VKRT_REQUIRE(pt->bvh, "vkrt_pt_set_scene first");
+        VKRT_REQUIRE(pt->bvh || pt->instanced, "vkrt_pt_set_scene first");
         const vkrt_pt_settings settings = pt->settings;// settings are copied per frame (PBRPathTracer.cpp:198)
         resize_storage(pt, settings.resolution[0], settings.resolution[1]);
         pt->stats.traced = 0;

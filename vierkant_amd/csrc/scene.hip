@@ -19,136 +19,13 @@
 #include "bvh_types.hpp"
 #include "context.hpp"
 #include "kernels.hpp"
-#include "traverse.hpp"
+#include "scene_walk.hpp"
 
 namespace vkrt
 {
 
-constexpr int SCENE_STACK = 96;  // top-level + BLAS entries per ray; checked at build time
 constexpr int SCENE_BLOCK = 64;   // with the context's XCD chunk: 3.02 Grays/s vs 2.96 at 128 and 2.85 at 256
 constexpr int SCENE_LDS = 16;
-
-struct inst_dev
-{
-    float4 w0, w1, w2;// world-to-object rows
-    uint32_t blas, custom, pad0, pad1;
-};
-
-struct blas_dev
-{
-    const vkrt_bvh4_node *nodes;
-    const float4 *tris;
-    int32_t root;
-    uint32_t empty;
-};
-
-//! object-space setup of world ray (ra, rb) for instance I: each row ((w0 x + w1 y) + w2 z) + w3, the direction
-//! keeping its length (so t is shared by both spaces)
-__device__ __forceinline__ ray_setup object_ray(const inst_dev &I, const float4 &ra, const float4 &rb)
-{
-    const float ox = ((I.w0.x * ra.x + I.w0.y * ra.y) + I.w0.z * ra.z) + I.w0.w;
-    const float oy = ((I.w1.x * ra.x + I.w1.y * ra.y) + I.w1.z * ra.z) + I.w1.w;
-    const float oz = ((I.w2.x * ra.x + I.w2.y * ra.y) + I.w2.z * ra.z) + I.w2.w;
-    const float dx = (I.w0.x * rb.x + I.w0.y * rb.y) + I.w0.z * rb.z;
-    const float dy = (I.w1.x * rb.x + I.w1.y * rb.y) + I.w1.z * rb.z;
-    const float dz = (I.w2.x * rb.x + I.w2.y * rb.y) + I.w2.z * rb.z;
-    return make_ray_setup(make_float4(ox, oy, oz, ra.w), make_float4(dx, dy, dz, rb.w));
-}
-
-//! closest (or, ANY, first) hit of world ray (ra, rb) in a two-level scene, in one loop over both levels:
-//! top-level and mesh nodes share the 4-wide format, so a lane visits a node of whichever level it is in and
-//! the wave stays converged across instance boundaries. Entering an instance leaf switches the node array,
-//! triangles and ray to that instance's mesh and object space and remembers the stack depth; when the walk
-//! pops back to that depth the mesh is done and the world ray is set up again. Top-level leaves hold one
-//! instance each. Ties: smallest t, then the lowest instance, then the lowest triangle.
-template<bool ANY, int BLOCK, int LDS_DEPTH>
-__device__ __forceinline__ void scene_walk(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
-                                           const uint32_t *__restrict__ order, const inst_dev *__restrict__ insts,
-                                           const blas_dev *__restrict__ blases, const float4 &ra, const float4 &rb,
-                                           int32_t *lds, float &best_t, float &best_u, float &best_v,
-                                           uint32_t &best_prim, uint32_t &best_inst)
-{
-    int32_t spill[SCENE_STACK - LDS_DEPTH];
-    lane_stack<BLOCK, LDS_DEPTH> st{lds, spill};
-    ray_setup r = make_ray_setup(ra, rb);
-    const vkrt_bvh4_node *nodes = tlas;
-    const float4 *tris = nullptr;
-    uint32_t inst = VKRT_INVALID_PRIM;
-    int base = -1;// stack depth at which the current mesh walk began (-1: at the top level)
-    int sp = 0;
-    int32_t ref = troot;
-    while(true)
-    {
-        if(ref >= 0)
-        {
-            int32_t c[4];
-            const int cnt = visit4(nodes, ref, r, best_t, c);
-            if(cnt)
-            {
-                for(int j = cnt - 1; j >= 1; --j) st.put(sp++, c[j]);
-                ref = c[0];
-                continue;
-            }
-        }
-        else if(base < 0)
-        {
-            uint32_t first, count;
-            leaf_decode(ref, first, count);
-            inst = order[first];
-            const inst_dev I = insts[inst];
-            const blas_dev B = blases[I.blas];
-            if(!B.empty)
-            {
-                r = object_ray(I, ra, rb);
-                nodes = B.nodes;
-                tris = B.tris;
-                base = sp;
-                ref = B.root;
-                continue;
-            }
-        }
-        else
-        {
-            uint32_t bf, bc;
-            leaf_decode(ref, bf, bc);
-            bool found = false;
-            for(uint32_t k = 0; k < bc; ++k)
-            {
-                const size_t tb = 48 * (size_t)(bf + k);
-                const float4 p0 = load_global_f4(tris, tb), p1 = load_global_f4(tris, tb + 16),
-                             p2 = load_global_f4(tris, tb + 32);
-                float t, u, v;
-                if(vierkant::rt::mt_fast(r.o, r.d, vec3{p0.x, p0.y, p0.z}, vec3{p0.w, p1.x, p1.y},
-                                         vec3{p1.z, p1.w, p2.x}, t, u, v) &&
-                   t > r.tmin)
-                {
-                    const uint32_t prim = __float_as_uint(p2.y);
-                    if(t < best_t || (t == best_t && best_prim != VKRT_INVALID_PRIM &&
-                                      (inst < best_inst || (inst == best_inst && prim < best_prim))))
-                    {
-                        best_t = t;
-                        best_u = u;
-                        best_v = v;
-                        best_prim = prim;
-                        best_inst = inst;
-                        found = true;
-                    }
-                }
-            }
-            if(ANY && found) return;
-        }
-        if(sp == base)
-        {
-            // the mesh walk is done: back to the top level
-            r = make_ray_setup(ra, rb);
-            nodes = tlas;
-            base = -1;
-        }
-        if(sp == 0) return;
-        --sp;
-        ref = st.get(sp);
-    }
-}
 
 template<bool ANY, int BLOCK, int LDS_DEPTH>
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) k_scene_trace(const vkrt_bvh4_node *__restrict__ tlas, int32_t troot,
@@ -407,6 +284,12 @@ static void check_instances(const vkrt_scene *sc, const vkrt_instance *inst, uin
             VKRT_REQUIRE(std::isfinite(inst[k].object_to_world[j]) && std::isfinite(inst[k].world_to_object[j]),
                          "instance transforms must be finite");
     }
+}
+
+scene_view scene_device_view(const vkrt_scene *sc)
+{
+    return scene_view{sc->nodes.as<vkrt_bvh4_node>(), sc->root, sc->empty ? 1u : 0u, sc->order.as<uint32_t>(),
+                      sc->insts.as<inst_dev>(), sc->blases.as<blas_dev>()};
 }
 
 template<bool ANY>

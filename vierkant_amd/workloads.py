@@ -381,6 +381,75 @@ def room_lights():
     ], dtype=vk.LIGHT_DTYPE)
 
 
+def object_meshes(scene):
+    """a scene's triangles split into meshes at every change of object (each object's triangles are
+    contiguous), with one identity instance per mesh in order: (mesh_first, PT_INSTANCE_DTYPE records) for
+    PathTracer.set_instanced_scene, which then renders the flat scene's image"""
+    import vierkant_amd as vk
+    to = np.asarray(scene["tri_object"])
+    cuts = np.flatnonzero(np.diff(to)) + 1 if len(to) else np.zeros(0, np.int64)
+    mf = np.concatenate([[0], cuts, [len(to)]]).astype(np.uint32)
+    return mf, vk.pt_instances(np.arange(len(mf) - 1))
+
+
+def _unit_quaternions(u):
+    """uniform random rotations (x, y, z, w) from 3 uniforms per row (Shoemake's subgroup method)"""
+    u = np.asarray(u, np.float64).reshape(-1, 3)
+    a, b = np.sqrt(1.0 - u[:, 0]), np.sqrt(u[:, 0])
+    t1, t2 = 2.0 * np.pi * u[:, 1], 2.0 * np.pi * u[:, 2]
+    return np.column_stack([a * np.sin(t1), a * np.cos(t1), b * np.sin(t2), b * np.cos(t2)]).astype(np.float32)
+
+
+def room_instanced(n_spheres=200, n_sphere_meshes=8, seed=0):
+    """The room as a two-level scene. Mesh 0: the room's quads, emitter and box (one identity instance).
+    Meshes 1..n_sphere_meshes: a unit IcoSphere(1, 3), each with its own material, shared by n_spheres
+    instances (sphere s uses mesh 1 + s % n_sphere_meshes) with uniform scale r ~ U[0.05, 0.3], a uniformly
+    random rotation and a centre inside the room. Last mesh: the jittered subdiv-5 icosphere around the
+    origin, placed at (-1.2, 1.2, 0.3) with a random rotation and non-uniform scale (1.0, 0.8, 1.1).
+    Uniform draws: stream seed + 5, 7 per sphere (centre, radius, rotation), then 3 + 3 x 6 material draws.
+    Returns (scene, mesh_first, instances)."""
+    import vierkant_amd as vk
+    b = _SceneBuilder()
+    u = uniform(seed + 5, 0, 7 * n_spheres + 3 + 6 * (n_sphere_meshes + 2)).reshape(-1).astype(np.float64)
+    white = b.material(vk.material((0.8, 0.8, 0.8), roughness=1.0))
+    red = b.material(vk.material((0.8, 0.1, 0.1), roughness=1.0))
+    green = b.material(vk.material((0.1, 0.8, 0.1), roughness=1.0))
+    light = b.material(vk.material((0.0, 0.0, 0.0), emission=(1.0, 0.9, 0.8), emission_strength=8.0))
+    room = _box_quads((-3, 0, -3), (3, 6, 3), inward=True)
+    b.quads([room[0], room[1], room[2]], white)
+    b.quads([room[4]], red)
+    b.quads([room[5]], green)
+    b.quads([([(-1, 5.99, -1), (1, 5.99, -1), (1, 5.99, 1), (-1, 5.99, 1)], (0, -1, 0))], light)
+    m0 = 7 * n_spheres + 3
+    b.quads(_box_quads((0.4, 0.0, -1.8), (1.8, 1.4, -0.4), inward=False),
+            b.material(_random_material(u[m0:m0 + 6])))
+    first = [0, sum(len(i) // 3 for i in b.idx)]
+    p, i = vk.icosphere(1.0, 3)
+    n = p.astype(np.float64)
+    n = (n / np.linalg.norm(n, axis=1, keepdims=True)).astype(np.float32)
+    for k in range(n_sphere_meshes):
+        mo = m0 + 6 * (k + 1)
+        b.add(p, n, i, b.material(_random_material(u[mo:mo + 6])))
+        first.append(first[-1] + len(i) // 3)
+    pb, ib = vk.icosphere(1.0, 5)
+    nb = pb.astype(np.float64)
+    nb = (nb / np.linalg.norm(nb, axis=1, keepdims=True)).astype(np.float32)
+    pj, _ = jittered_icosphere(5, seed=seed)
+    mo = m0 + 6 * (n_sphere_meshes + 1)
+    b.add(pj, nb, ib, b.material(_random_material(u[mo:mo + 6])))
+    first.append(first[-1] + len(ib) // 3)
+    x = u[:7 * n_spheres].reshape(n_spheres, 7)
+    centres = np.column_stack([-2.7 + 5.4 * x[:, 0], 0.3 + 4.2 * x[:, 1], -2.7 + 5.4 * x[:, 2]])
+    radii = 0.05 + 0.25 * x[:, 3]
+    meshes = np.concatenate([[0], 1 + np.arange(n_spheres) % n_sphere_meshes, [n_sphere_meshes + 1]])
+    trans = np.vstack([np.zeros((1, 3)), centres, [[-1.2, 1.2, 0.3]]])
+    rots = np.vstack([[[0.0, 0.0, 0.0, 1.0]], _unit_quaternions(x[:, 4:7]),
+                      _unit_quaternions(u[7 * n_spheres:7 * n_spheres + 3])])
+    scales = np.vstack([np.ones((1, 3)), np.repeat(radii[:, None], 3, axis=1), [[1.0, 0.8, 1.1]]])
+    inst = vk.pt_instances(meshes, trans, rots, scales)
+    return b.scene(environment=1.0), np.asarray(first, np.uint32), inst
+
+
 def camera_basis(camera, width, height):
     """the tracer's camera basis, computed like the library does (double precision, rounded once):
     returns (eye, forward, right, up, tan_half_x, tan_half_y) as float32"""
