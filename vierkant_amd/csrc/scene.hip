@@ -101,6 +101,7 @@ struct tlas_builder
     const std::vector<host_box> &box;
     std::vector<uint32_t> &order;
     std::vector<vkrt_bvh4_node> &nodes;
+    bool sah = true;
     uint32_t max_stack = 0;
 
     static host_box merge(const host_box &a, const host_box &b)
@@ -121,9 +122,21 @@ struct tlas_builder
         return r;
     }
 
-    //! split [lo, hi) at its median along the widest centroid axis
+    static double half_area(const host_box &b)
+    {
+        const double dx = std::max(0.0, b.hi[0] - b.lo[0]), dy = std::max(0.0, b.hi[1] - b.lo[1]),
+                     dz = std::max(0.0, b.hi[2] - b.lo[2]);
+        return dx * dy + dy * dz + dz * dx;
+    }
+
+    //! split [lo, hi) into two non-empty parts by the surface area heuristic: for every axis the instances are
+    //! put in SAH_BINS bins by box centre and the cut between bins with the least
+    //! area(left) * count(left) + area(right) * count(right) wins; the range is then partitioned at that cut.
+    //! Instances whose centres coincide on every axis (or a cut that would leave a side empty) fall back to
+    //! the median of the widest centre axis
     uint32_t split(uint32_t lo, uint32_t hi)
     {
+        constexpr int SAH_BINS = 32;
         double cl[3] = {INFINITY, INFINITY, INFINITY}, ch[3] = {-INFINITY, -INFINITY, -INFINITY};
         for(uint32_t j = lo; j < hi; ++j)
             for(int k = 0; k < 3; ++k)
@@ -132,6 +145,72 @@ struct tlas_builder
                 cl[k] = std::min(cl[k], c);
                 ch[k] = std::max(ch[k], c);
             }
+        auto bin_of = [&](uint32_t id, int k) {
+            const double c = 0.5 * (box[id].lo[k] + box[id].hi[k]);
+            const int b = static_cast<int>((c - cl[k]) / (ch[k] - cl[k]) * SAH_BINS);
+            return std::min(std::max(b, 0), SAH_BINS - 1);
+        };
+        double best_cost = INFINITY;
+        int best_axis = -1, best_cut = 0;
+        for(int k = 0; k < 3; ++k)
+        {
+            if(!(ch[k] > cl[k])) continue;
+            host_box bb[SAH_BINS];
+            uint32_t bc[SAH_BINS] = {};
+            for(auto &b: bb)
+                for(int a = 0; a < 3; ++a)
+                {
+                    b.lo[a] = INFINITY;
+                    b.hi[a] = -INFINITY;
+                }
+            for(uint32_t j = lo; j < hi; ++j)
+            {
+                const int b = bin_of(order[j], k);
+                bb[b] = bc[b] ? merge(bb[b], box[order[j]]) : box[order[j]];
+                ++bc[b];
+            }
+            // suffix areas and counts, then a sweep from the left
+            double right_area[SAH_BINS];
+            uint32_t right_count[SAH_BINS];
+            host_box acc = bb[0];
+            uint32_t cnt = 0;
+            for(int b = SAH_BINS - 1; b >= 0; --b)
+            {
+                if(bc[b])
+                {
+                    acc = cnt ? merge(acc, bb[b]) : bb[b];
+                    cnt += bc[b];
+                }
+                right_area[b] = cnt ? half_area(acc) : 0.0;
+                right_count[b] = cnt;
+            }
+            host_box left = bb[0];
+            uint32_t lcnt = 0;
+            for(int b = 0; b + 1 < SAH_BINS; ++b)
+            {
+                if(bc[b])
+                {
+                    left = lcnt ? merge(left, bb[b]) : bb[b];
+                    lcnt += bc[b];
+                }
+                if(!lcnt || !right_count[b + 1]) continue;
+                const double cost = half_area(left) * lcnt + right_area[b + 1] * right_count[b + 1];
+                if(cost < best_cost)
+                {
+                    best_cost = cost;
+                    best_axis = k;
+                    best_cut = b + 1;
+                }
+            }
+        }
+        if(sah && best_axis >= 0)
+        {
+            const auto first_right = std::partition(order.begin() + lo, order.begin() + hi, [&](uint32_t id) {
+                return bin_of(id, best_axis) < best_cut;
+            });
+            const uint32_t mid = static_cast<uint32_t>(first_right - order.begin());
+            if(mid > lo && mid < hi) return mid;
+        }
         int axis = 0;
         for(int k = 1; k < 3; ++k)
             if(ch[k] - cl[k] > ch[axis] - cl[axis]) axis = k;
@@ -246,6 +325,15 @@ static void build_tlas(vkrt_scene *sc)
         nodes.reserve(order.size());
         sc->root = tb.build(0, static_cast<uint32_t>(order.size()), 0);
         sc->tlas_stack = tb.max_stack;
+        if(sc->tlas_stack + sc->max_blas_stack > (uint32_t)SCENE_STACK)
+        {
+            // an SAH tree too deep for the stack: the median split's balanced tree instead
+            nodes.clear();
+            tlas_builder mb{box, order, nodes};
+            mb.sah = false;
+            sc->root = mb.build(0, static_cast<uint32_t>(order.size()), 0);
+            sc->tlas_stack = mb.max_stack;
+        }
     }
     VKRT_REQUIRE(sc->tlas_stack + sc->max_blas_stack <= (uint32_t)SCENE_STACK,
                  "scene too deep for the traversal stack (top level + deepest mesh)");
