@@ -154,18 +154,25 @@ def brute_force_leg(ctx, reps=10):
             "gpu_over_cpu": round(cpu_s / (ms * 1e-3), 1)}
 
 
-def path_tracer_frame(ctx, spp, lit=False):
+def path_tracer_frame(ctx, spp, lit=False, instanced=False):
     """one PBRPathTracer batch of the Cornell-style room (12 quads + 200 spheres + the jittered icosphere,
     276,504 triangles) at 1920x1080, max_trace_depth 8, `spp` samples per pixel; `lit` adds the room's six
-    analytic lights (next-event estimation with a transmittance shadow ray per surface vertex, MIS)"""
+    analytic lights (next-event estimation with a transmittance shadow ray per surface vertex, MIS);
+    `instanced` traces the room as a two-level scene (202 placements of 10 meshes: the room's quads, 8 shared
+    unit spheres under random rotations and scales, the jittered icosphere rotated and scaled), and also
+    times the same placed triangles as a flat scene"""
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
-    scene = W.room_scene()
     s = vk.default_settings()
     s.resolution[0], s.resolution[1] = 1920, 1080
     s.num_samples, s.max_trace_depth = spp, 8
     pt = vk.PathTracer(ctx, s, seed=0)
-    pt.set_scene(scene)
+    if instanced:
+        scene, mesh_first, inst = W.room_instanced()
+        pt.set_instanced_scene(scene, mesh_first, inst)
+    else:
+        scene = W.room_scene()
+        pt.set_scene(scene)
     if lit:
         pt.set_lights(W.room_lights())
     cam = W.room_camera()
@@ -181,8 +188,10 @@ def path_tracer_frame(ctx, spp, lit=False):
     px = pt.pixels()
     seg_bytes = 332      # per path segment: SoA ray, hit and path state in/out, 3 indices + 3 vertices, material
     gbps = seg_bytes * st.segments / (st.raytrace_ms * 1e-3) / 1e9
-    out = {"workload": "1920x1080, max_trace_depth 8, %d spp, 276,504 triangles%s"
-                       % (spp, ", 6 analytic lights (NEE + MIS)" if lit else ""),
+    what = ("%d placements of %d meshes (%d mesh triangles, 276,504 placed)"
+            % (len(inst), len(mesh_first) - 1, len(scene["indices"]) // 3)) if instanced else "276,504 triangles"
+    out = {"workload": "1920x1080, max_trace_depth 8, %d spp, %s%s"
+                       % (spp, what, ", 6 analytic lights (NEE + MIS)" if lit else ""),
            "frame_ms": round(st.raytrace_ms, 3), "samples": int(st.samples), "segments": int(st.segments),
            "samples_per_s": round(st.samples / (st.raytrace_ms * 1e-3), 1),
            "segments_per_s": round(st.segments / (st.raytrace_ms * 1e-3), 1),
@@ -191,12 +200,28 @@ def path_tracer_frame(ctx, spp, lit=False):
                         "frac": round(gbps / MI355X_HBM_GBPS, 5), "bytes_per_segment": seg_bytes},
            "mean_radiance": float(px["radiance"].mean())}
     pt.close()
-    if not lit:
+    if instanced:
+        # the same placed triangles as one flat scene, same settings and seed
+        flat = vk.PathTracer(ctx, s_warm, seed=0)
+        flat.set_scene(W.flatten_instances(scene, mesh_first, inst))
+        if lit:
+            flat.set_lights(W.room_lights())
+        flat.render(cam)
+        flat.reset_accumulator()
+        flat.settings = s
+        flat.render(cam)
+        fst = flat.stats()
+        out["flat_equivalent"] = {"frame_ms": round(fst.raytrace_ms, 3), "segments": int(fst.segments),
+                                  "instanced_over_flat": round(st.raytrace_ms / fst.raytrace_ms, 3)}
+        flat.close()
+        out["vs_cpu_integrator"] = pt_vs_integrator(ctx, scene, cam, w=240, h=135, spp=8,
+                                                    instanced=(mesh_first, inst))
+    elif not lit:
         out["vs_cpu_integrator"] = pt_vs_integrator(ctx, scene, cam)
     return out
 
 
-def pt_vs_integrator(ctx, scene, cam, w=480, h=270, spp=16):
+def pt_vs_integrator(ctx, scene, cam, w=480, h=270, spp=16, instanced=None):
     """the GPU tracer and the CPU reference integrator (oracle, same RNG streams and arithmetic) on the same
     scene, camera and seed at equal samples per pixel, at a reduced size: RMSE relative to the mean and the
     share of pixels within 1e-4 (the integrator's one-ulp sensitivity bounds what two math libraries give)"""
@@ -208,12 +233,16 @@ def pt_vs_integrator(ctx, scene, cam, w=480, h=270, spp=16):
     s.resolution[0], s.resolution[1] = w, h
     s.num_samples, s.max_trace_depth = spp, 8
     pt = vk.PathTracer(ctx, s, seed=0)
-    pt.set_scene(scene)
+    if instanced is not None:
+        pt.set_instanced_scene(scene, *instanced)
+    else:
+        pt.set_scene(scene)
     pt.render(cam)
     g = pt.pixels()["radiance"].astype(np.float64)
     pt.close()
     t0 = time.perf_counter()
-    c = orc.pt_render(scene, cam, w, h, spp, 8, W.minstd_rand0(0, 1)[0])[..., :3].astype(np.float64)
+    c = orc.pt_render(scene, cam, w, h, spp, 8, W.minstd_rand0(0, 1)[0],
+                      instanced=instanced)[..., :3].astype(np.float64)
     cpu_s = time.perf_counter() - t0
     close = float(np.mean(np.abs(g - c).max(axis=2) <= 1e-4 * np.maximum(1.0, c.max(axis=2))))
     return {"size": [w, h], "spp": spp, "rmse": float(np.sqrt(np.mean((g - c) ** 2))),
@@ -574,6 +603,8 @@ def main():
     ap.add_argument("--ploc-radius", type=int, default=0, help="PLOC search radius (0: the library default, 16)")
     ap.add_argument("--pt-lit-spp", type=int, default=16,
                     help="samples per pixel of the lit path-tracer frame (0: skip)")
+    ap.add_argument("--pt-inst-spp", type=int, default=16,
+                    help="samples per pixel of the path-tracer frame over the instanced room (0: skip)")
     ap.add_argument("--batch-scenes", type=int, default=256, help="scenes of the batched launch (0: skip)")
     ap.add_argument("--brute-force", type=int, default=1,
                     help="time the brute-force leg (65,536 rays x 20,480 triangles) and its full CPU run (0: skip)")
@@ -794,6 +825,9 @@ def main():
     if args.pt_lit_spp:
         log("lit path tracer frame")
         result["config"]["path_tracer_lit"] = path_tracer_frame(ctx, args.pt_lit_spp, lit=True)
+    if args.pt_inst_spp:
+        log("instanced path tracer frame")
+        result["config"]["path_tracer_instanced"] = path_tracer_frame(ctx, args.pt_inst_spp, instanced=True)
     if args.brute_force:
         log("brute force 65,536 x 20,480")
         result["config"]["brute_force"] = brute_force_leg(ctx)

@@ -450,6 +450,33 @@ def room_instanced(n_spheres=200, n_sphere_meshes=8, seed=0):
     return b.scene(environment=1.0), np.asarray(first, np.uint32), inst
 
 
+def flatten_instances(scene, mesh_first, instances):
+    """the two-level scene with every placement's triangles moved into world space (translation +
+    rotate(q, scale * p), evaluated in double; normals turned by the rotation only, as the shading turns
+    them), one object per placed mesh object: the same geometry as a flat scene"""
+    pos, nrm, idx, tob, omat = [], [], [], [], []
+    nv = 0
+    for rec in instances:
+        a, b = int(mesh_first[rec["mesh"]]), int(mesh_first[rec["mesh"] + 1])
+        tri = np.asarray(scene["indices"]).reshape(-1, 3)[a:b]
+        used, inv = np.unique(tri.reshape(-1), return_inverse=True)
+        x, y, z, w = (float(c) for c in rec["rotation"])
+        r = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+        p = np.asarray(scene["positions"])[used].astype(np.float64) * rec["scale"].astype(np.float64)
+        pos.append((p @ r.T + rec["translation"].astype(np.float64)).astype(np.float32))
+        nrm.append((np.asarray(scene["normals"])[used].astype(np.float64) @ r.T).astype(np.float32))
+        idx.append(inv.astype(np.uint32) + nv)
+        nv += len(used)
+        uo, oinv = np.unique(np.asarray(scene["tri_object"])[a:b], return_inverse=True)
+        tob.append((oinv + len(omat)).astype(np.uint32))
+        omat.extend(np.asarray(scene["object_material"])[uo])
+    return {"positions": np.concatenate(pos), "normals": np.concatenate(nrm), "indices": np.concatenate(idx),
+            "tri_object": np.concatenate(tob), "object_material": np.asarray(omat, np.uint32),
+            "materials": scene["materials"], "environment": scene.get("environment", 1.0)}
+
+
 def camera_basis(camera, width, height):
     """the tracer's camera basis, computed like the library does (double precision, rounded once):
     returns (eye, forward, right, up, tan_half_x, tan_half_y) as float32"""
