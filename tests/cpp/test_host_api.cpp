@@ -207,6 +207,22 @@ int main()
     const double want_lit = 0.52 / 3.14159265358979 * 2.0;
     if(std::fabs(mean / want_lit - 1.0) > 0.01) return fail("sun-lit plane");
 
+    // the same plane as an instanced scene: a quarter-size quad placed once with scale 2 (x and z) and a
+    // half turn about y
+    float hpos[12];
+    for(int k = 0; k < 12; ++k) hpos[k] = qpos[k] * 0.5f;
+    PBRPathTracer::scene_t half_plane{hpos, qnrm, 4, qidx, 2, qobj, &obj_mat, 1, &pm, 1, 0.0f};
+    PBRPathTracer::instance_t place{};
+    place.rotation[1] = 1.f;
+    place.scale[0] = place.scale[1] = place.scale[2] = 2.f;
+    auto placed = PBRPathTracer::create(ctx, lit_info);
+    placed->set_scene(half_plane, {0u, 2u}, {place});
+    placed->render_scene(down);
+    double pmean = 0.0;
+    for(const auto &p: placed->pixels()) pmean += p.radiance[1];
+    pmean /= 64.0;
+    if(std::fabs(pmean / want_lit - 1.0) > 0.01) return fail("sun-lit plane as an instance");
+
     // the same plane from an OBJ + MTL file through vierkant::model::load_model renders the same
     const auto dir = std::filesystem::temp_directory_path();
     std::ofstream(dir / "vkrt_plane.mtl") << "newmtl grey\nKd 0.5 0.5 0.5\nNs 0\nNi 1.5\n";

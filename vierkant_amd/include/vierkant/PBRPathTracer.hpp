@@ -111,6 +111,19 @@ public:
     //! upload a scene and build its acceleration structure
     void set_scene(const scene_t &scene) { m_ctx->check(vkrt_pt_set_scene(m_pt, &scene)); }
 
+    using instance_t = vkrt_pt_instance;
+
+    //! an instanced scene: the scene's triangles split into meshes at mesh_first (n_meshes + 1 offsets),
+    //! one BVH per mesh, placed by `instances` (the top level RayBuilder builds over its entries)
+    void set_scene(const scene_t &scene, const std::vector<uint32_t> &mesh_first,
+                   const std::vector<instance_t> &instances)
+    {
+        if(mesh_first.size() < 2) throw std::runtime_error("vkrt: mesh_first needs n_meshes + 1 offsets");
+        m_ctx->check(vkrt_pt_set_instanced_scene(m_pt, &scene, mesh_first.data(),
+                                                 static_cast<uint32_t>(mesh_first.size() - 1), instances.data(),
+                                                 static_cast<uint32_t>(instances.size())));
+    }
+
     //! the scene's light sources (the reference gathers them from lightsource components each frame)
     void set_lights(std::vector<light_t> lights) { m_lights = std::move(lights); }
 
