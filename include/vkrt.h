@@ -485,6 +485,12 @@ typedef struct vkrt_material
     uint32_t transmission_texture;  /* r scales transmission */
     uint32_t diffuse_transmission_texture;       /* a scales diffuse_transmission */
     uint32_t diffuse_transmission_color_texture; /* rgb tints diffuse_transmission_color */
+    /* KHR_materials_volume (include/vierkant/Material.hpp:104-108): an interior medium absorbing as
+     * exp(-sigma_t d) per channel, sigma_t = -log(attenuation_color) / attenuation_distance
+     * (raypipeline.slang:380-415, :493, visibility_test.slang:110-129). attenuation_distance 0 stands for the
+     * reference's default +inf: no interior medium */
+    float attenuation_color[3];
+    float attenuation_distance;
     uint32_t reserved[2];
 } vkrt_material;
 

@@ -183,6 +183,8 @@ typedef struct
     uint32_t tex_normal, tex_emission, tex_arm, tex_transmission, tex_dt, tex_dtc; /* further slots, same code */
     int alpha_mode;     /* 0 opaque, 1 MASK (cutoff), 2 BLEND (uniform draw): the any-hit test */
     float alpha_cutoff;
+    int vol;            /* an interior medium (attenuation_distance > 0) */
+    v3 sigma;           /* its extinction -log(attenuation_color) / attenuation_distance (raypipeline.slang:493) */
 } mat_t;
 
 typedef struct
@@ -359,7 +361,7 @@ static bs_t sample_disney(mat_t m, v3 N, v3 V, float eta, uint32_t *rng)
     return ret;
 }
 
-/* a vkrt_material record (36 words) as the integrator's material */
+/* a vkrt_material record (40 words) as the integrator's material */
 static mat_t unpack_material(const float *x, uint32_t n_textures)
 {
     mat_t mt;
@@ -395,6 +397,11 @@ static mat_t unpack_material(const float *x, uint32_t n_textures)
     mt.tex_transmission = slot[3];
     mt.tex_dt = slot[4];
     mt.tex_dtc = slot[5];
+    if(x[37] > 0.0f)
+    {
+        mt.vol = 1;
+        mt.sigma = V3(-logf(x[34]) / x[37], -logf(x[35]) / x[37], -logf(x[36]) / x[37]);
+    }
     return mt;
 }
 
@@ -758,6 +765,7 @@ typedef struct
 {
     float t;
     uint32_t prim, inst;
+    int front; /* the ray enters the triangle's volume here (det > 0 in the triangle's own space) */
 } thit_t;
 
 static int transmit_collect(const cbvh *b, const mat_t *mats, const uint32_t *mat_of_prim, v3 o, v3 d, float tmin,
@@ -799,6 +807,11 @@ static int transmit_collect(const cbvh *b, const mat_t *mats, const uint32_t *ma
                 return 1;
             }
             if(nh == ORC_MAX_TRANSMISSIVE) abort();
+            {
+                v3 e1 = V3(tr[3] - tr[0], tr[4] - tr[1], tr[5] - tr[2]);
+                v3 e2 = V3(tr[6] - tr[0], tr[7] - tr[1], tr[8] - tr[2]);
+                hits[nh].front = dot(e1, cross(d, e2)) > 0.f;
+            }
             hits[nh].t = t;
             hits[nh].prim = prim;
             hits[nh].inst = inst;
@@ -827,13 +840,22 @@ static v3 transmit_product(const mat_t *mats, const uint32_t *mat_of_prim, thit_
         }
         hits[c] = x;
     }
-    v3 T = V3(1, 1, 1);
+    /* Beer-Lambert optical depth over the transmissive hits with an interior medium, a signed sum (entry -,
+     * exit +; visibility_test.slang:123-129) taken in the same order, clamped at 0 */
+    v3 T = V3(1, 1, 1), D = V3(0, 0, 0);
     for(uint32_t a = 0; a < nh; ++a)
     {
         const mat_t *m = &mats[mat_of_prim[hits[a].prim]];
         T = V3(T.x * (m->trans * m->color.x), T.y * (m->trans * m->color.y), T.z * (m->trans * m->color.z));
+        if(m->vol)
+        {
+            const float t = hits[a].t;
+            const int f = hits[a].front;
+            D = V3(D.x + (f ? -m->sigma.x * t : m->sigma.x * t), D.y + (f ? -m->sigma.y * t : m->sigma.y * t),
+                   D.z + (f ? -m->sigma.z * t : m->sigma.z * t));
+        }
     }
-    return T;
+    return V3(T.x * expf(-fmaxf(D.x, 0.f)), T.y * expf(-fmaxf(D.y, 0.f)), T.z * expf(-fmaxf(D.z, 0.f)));
 }
 
 /* ---- two-level scenes (the instance level of RayBuilder's top-level structure, src/RayBuilder.cpp:386-781):
@@ -1251,6 +1273,23 @@ typedef struct
     uint32_t y0, y1;
 } job_t;
 
+/* the segment's medium, absorption only (raypipeline.slang:380-415 with sigma_s = 0): a channel picked, a
+ * collision distance drawn against its extinction; a collision before t_hit absorbs the path (returns 1),
+ * else the throughput takes exp(-sigma t_hit) over the mean of the three channels' transmittances */
+static int medium_collision(const mat_t *m, float t_hit, v3 *beta, uint32_t *rng)
+{
+    const v3 sg = m->sigma;
+    if(!(sg.x > 0.f || sg.y > 0.f || sg.z > 0.f)) return 0;
+    const int channel = (int)fminf(rnd(rng) * 3.0f, 2.0f);
+    const float sc = channel == 0 ? sg.x : (channel == 1 ? sg.y : sg.z);
+    const float t = fminf(-logf(1.0f - rnd(rng)) / fmaxf(sc, EPS), t_hit);
+    if(t < t_hit) return 1;
+    const v3 tr = V3(expf(-sg.x * t), expf(-sg.y * t), expf(-sg.z * t));
+    const float pdf = fmaxf((tr.x + tr.y + tr.z) / 3.0f, PDF_EPS);
+    *beta = V3(beta->x * (tr.x / pdf), beta->y * (tr.y / pdf), beta->z * (tr.z / pdf));
+    return 0;
+}
+
 static void render_pixel(job_t *j, uint32_t x, uint32_t y)
 {
     const orc_pt_scene *sc = j->sc;
@@ -1290,6 +1329,8 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
         float tmax = DEFAULT_TMAX;
         uint32_t media_index = 0;
         float iors[4] = {1.f, 1.f, 1.f, 1.f};
+        /* the material whose interior each stack level is (level 0: vacuum) */
+        uint32_t media_mat[4] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
         int regularize = 0, caustic = 0, first_miss = 1;
         v3 beta = V3(1, 1, 1), rad = V3(0, 0, 0);
         float prev_pdf = 0.0f; /* payload.prev_bsdf_pdf */
@@ -1336,6 +1377,13 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                     rad = add(rad, mul(beta, dl));
                 }
             }
+            else if(media_mat[media_index] != 0xFFFFFFFFu && medium_collision(&j->mats[media_mat[media_index]], t,
+                                                                            &beta, &lr))
+            {
+                /* absorbed in the medium before the surface (raypipeline.slang:380-415 with a single-scattering
+                 * albedo of 0: the throughput becomes 0 there, so the path gathers nothing more) */
+                stop = 1;
+            }
             else
             {
                 const uint32_t *ix = sc->indices + 3 * (size_t)prim;
@@ -1377,7 +1425,8 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
                 }
                 mat_t m = j->mats[sc->object_material[obj]];
                 v3 Vw = neg(dir);
-                const int media_boundary = m.trans > 0.0f; /* before the transmission texture (raypipeline.slang:517-522) */
+                /* before the transmission texture, or an interior medium (raypipeline.slang:517-522) */
+                const int media_boundary = m.trans > 0.0f || m.vol;
                 if(m.tex | m.tex_normal | m.tex_emission | m.tex_arm | m.tex_transmission | m.tex_dt | m.tex_dtc)
                 {
                     /* every texture at the ray cone's level of detail (raypipeline.slang:300-321, 368-375,
@@ -1540,6 +1589,7 @@ static void render_pixel(job_t *j, uint32_t x, uint32_t y)
             {
                 media_index = media_index + 1 < 3 ? media_index + 1 : 3;
                 iors[media_index] = round_half(next_ior);
+                media_mat[media_index] = sc->object_material[obj];
             }
             else if(media_op == 2)
                 media_index = media_index > 0 ? media_index - 1 : 0;
@@ -1661,7 +1711,7 @@ static void scene_build(const orc_pt_scene *sc, built_t *s)
     double *cent = (double *)malloc(sizeof(double) * 3 * (size_t)(n ? n : 1));
     s->mats = (mat_t *)calloc(sc->n_materials ? sc->n_materials : 1, sizeof(mat_t));
     for(uint32_t m = 0; m < sc->n_materials; ++m)
-        s->mats[m] = unpack_material(sc->materials + 36 * (size_t)m, sc->n_textures);
+        s->mats[m] = unpack_material(sc->materials + 40 * (size_t)m, sc->n_textures);
     for(uint32_t t = 0; t < n; ++t)
     {
         for(int k = 0; k < 3; ++k)

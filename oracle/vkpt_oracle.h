@@ -30,7 +30,7 @@ typedef struct orc_pt_scene
     const uint32_t *tri_object;
     const uint32_t *object_material;
     uint32_t n_objects;
-    const float *materials; /* vkrt_material records, 36 words each */
+    const float *materials; /* vkrt_material records, 40 words each */
     uint32_t n_materials;
     float environment;
     const float *texcoords; /* 2 per vertex, or NULL (all zero) */
@@ -76,7 +76,7 @@ void orc_pt_transmittance(const orc_pt_scene *sc, const float *rays8, uint32_t n
 void orc_denoise(const uint32_t *px8, uint32_t w, uint32_t h, int denoise, float *out4);
 /* test hook: only the first `passes` (1..5) of the five a-trous passes (steps 1, 2, 4, 8, 16) */
 void orc_denoise_passes(const uint32_t *px8, uint32_t w, uint32_t h, int passes, float *out4);
-/* Test hooks: sample_disney (bsdf_disney.slang:162-296) for n draws on one material record (36 words) at
+/* Test hooks: sample_disney (bsdf_disney.slang:162-296) for n draws on one material record (40 words) at
  * normal N, view V, eta; draw i starts from the RNG state xxhash32(seed, i). out8 per draw: direction xyz,
  * pdf, F rgb, transmission flag. eval_disney (:298-381) for n directions L: out4 = F rgb, pdf. */
 void orc_sample_disney_n(const float *mat36, const float *N, const float *V, float eta, uint32_t seed, uint32_t n,

@@ -102,7 +102,10 @@ def gltf_doc(buffer_uri=None, byte_length=0):
                                       "KHR_materials_clearcoat": {"clearcoatFactor": 0.3,
                                                                   "clearcoatRoughnessFactor": 0.2},
                                       "KHR_materials_sheen": {"sheenColorFactor": [0.1, 0.2, 0.3],
-                                                              "sheenRoughnessFactor": 0.4}}}],
+                                                              "sheenRoughnessFactor": 0.4},
+                                      "KHR_materials_volume": {"thicknessFactor": 0.5,
+                                                               "attenuationColor": [0.25, 0.0, 1.0],
+                                                               "attenuationDistance": 2.5}}}],
         "buffers": [dict({"byteLength": byte_length}, **({"uri": buffer_uri} if buffer_uri else {}))],
         "bufferViews": [{"buffer": 0, "byteOffset": 0, "byteLength": 36},
                         {"buffer": 0, "byteOffset": 36, "byteLength": 36},
@@ -145,6 +148,10 @@ def check_gltf_scene(m):
     assert np.array_equal(glass["sheen_color"][:3], np.float32([0.1, 0.2, 0.3]))
     assert glass["sheen_roughness"] == np.float32(0.4) and glass["two_sided"] == 1
     assert default["metalness"] == 0.0 and default["ior"] == np.float32(1.5)
+    # KHR_materials_volume: the attenuation, a zero channel raised to 1e-6; no extension: no medium
+    assert glass["attenuation_distance"] == np.float32(2.5)
+    assert np.array_equal(glass["attenuation_color"], np.float32([0.25, 1e-6, 1.0]))
+    assert plain["attenuation_distance"] == 0.0 and default["attenuation_distance"] == 0.0
 
 
 def test_gltf_base64_buffer(tmp_path):

@@ -121,8 +121,9 @@ MATERIAL_DTYPE = np.dtype([("color", "<f4", 4), ("emission", "<f4", 4), ("metaln
                            ("alpha_mode", "<u4"), ("alpha_cutoff", "<f4"), ("normal_texture", "<u4"),
                            ("emission_texture", "<u4"), ("ao_rough_metal_texture", "<u4"),
                            ("transmission_texture", "<u4"), ("diffuse_transmission_texture", "<u4"),
-                           ("diffuse_transmission_color_texture", "<u4"), ("reserved", "<u4", 2)])
-assert MATERIAL_DTYPE.itemsize == 144
+                           ("diffuse_transmission_color_texture", "<u4"), ("attenuation_color", "<f4", 3),
+                           ("attenuation_distance", "<f4"), ("reserved", "<u4", 2)])
+assert MATERIAL_DTYPE.itemsize == 160
 ALPHA_OPAQUE, ALPHA_MASK, ALPHA_BLEND = 0, 1, 2
 PIXEL_DTYPE = np.dtype([("radiance", "<f4", 3), ("alpha", "<f4"), ("depth", "<f4"), ("normal", "<u4"),
                         ("object_id", "<u4"), ("albedo", "<u4")])
@@ -130,10 +131,12 @@ PIXEL_DTYPE = np.dtype([("radiance", "<f4", 3), ("alpha", "<f4"), ("depth", "<f4
 
 def material(color=(0.8, 0.8, 0.8), emission=(0.0, 0.0, 0.0), emission_strength=0.0, metalness=0.0, roughness=1.0,
              transmission=0.0, ior=1.5, two_sided=False, alpha=1.0, color_texture=0, alpha_mode=0,
-             alpha_cutoff=0.5, **texture_slots):
+             alpha_cutoff=0.5, attenuation_color=(1.0, 1.0, 1.0), attenuation_distance=0.0, **texture_slots):
     """one vkrt_material record (material_t defaults: alpha 1, no clearcoat/sheen/diffuse transmission);
-    color_texture is 1 + the index of the scene's base color texture (0: none)"""
+    color_texture is 1 + the index of the scene's base color texture (0: none); attenuation_distance > 0
+    gives the material an interior medium absorbing as attenuation_color ** (d / attenuation_distance)"""
     m = np.zeros((), dtype=MATERIAL_DTYPE)
+    m["attenuation_color"], m["attenuation_distance"] = attenuation_color, attenuation_distance
     m["color"] = (*color, alpha)
     m["color_texture"], m["alpha_mode"], m["alpha_cutoff"] = color_texture, alpha_mode, alpha_cutoff
     for k, v in texture_slots.items():  # normal_texture, emission_texture, ao_rough_metal_texture, ...

@@ -14,7 +14,8 @@
 //   glTF  indices u8/u16/u32 or 0..n-1 (create_geometry, gltf.cpp:198-312); POSITION, NORMAL, TEXCOORD_0 as
 //         float; missing normals computed as above; materials as convert_material (:313-660): base color,
 //         emissive factor, metallic and roughness factors, doubleSided, alphaMode/alphaCutoff, and the
-//         KHR_materials_emissive_strength / transmission / ior / clearcoat / sheen / diffuse_transmission
+//         KHR_materials_emissive_strength / transmission / ior / clearcoat / sheen / diffuse_transmission /
+//         volume (attenuation)
 //         factors. Node transforms (matrix, or translation * rotation * scale, :136-160) are composed from the
 //         scene's roots and baked into world-space vertices in double precision, rounded to float once.
 // Base color textures are decoded (PNG, via zlib's inflate) into RGBA8; other images are counted as skipped.
@@ -1197,6 +1198,21 @@ vkrt_material convert_material(const json &jm, F &&texture_of)
         m.diffuse_transmission = static_cast<float>(dt["diffuseTransmissionFactor"].number(0.0));
         for(int k = 0; k < 3; ++k)
             m.diffuse_transmission_color[k] = static_cast<float>(dt["diffuseTransmissionColorFactor"][k].number(1.0));
+    }
+    if(ext.has("KHR_materials_volume"))
+    {
+        // attenuationColor and attenuationDistance as gltf.cpp:476-496 reads them (thickness is not used by the
+        // tracer); the default distance +inf, no interior medium, is kept as 0, and a colour channel of 0 is
+        // raised to 1e-6 (an extinction of 13.8 per attenuation distance) so that -log stays finite
+        const json &vol = ext["KHR_materials_volume"];
+        const double d = vol["attenuationDistance"].number(0.0);
+        if(d > 0.0 && d < 3.4e38)
+        {
+            m.attenuation_distance = static_cast<float>(d);
+            for(int k = 0; k < 3; ++k)
+                m.attenuation_color[k] = static_cast<float>(
+                        std::min(1.0, std::max(1e-6, vol["attenuationColor"][k].number(1.0))));
+        }
     }
     // the other texture slots (gltf.cpp:366-378, :470-476, :622-632); occlusion only fills the red channel
     // of the rough/metal image, which the tracer does not read

@@ -101,7 +101,20 @@ struct gpu_material
     uint4 tex;       // 1 + color texture (0: none), alpha mode, alpha cutoff bits, -
     uint4 tex2;      // 1 + normal, emission, ao/rough/metal, transmission texture
     uint4 tex3;      // 1 + diffuse transmission, diffuse transmission color texture, -, -
+    float4 vol;      // interior medium: sigma_t rgb = -log(attenuation color) / attenuation distance, 1 if any
 };
+
+//! a transmissive candidate's contribution to a shadow ray's Beer-Lambert optical depth
+//! (visibility_test.slang:123-129): -sigma_t t where the ray enters the material's volume (front face), +sigma_t t
+//! where it leaves; km carries the front-face flag in its top bit
+constexpr uint32_t KM_FRONT = 0x80000000u;
+__device__ __forceinline__ uint32_t candidate_tag(uint32_t mid, const gpu_material &m, const vec3 &d, const float4 &p0,
+                                                  const float4 &p1, const float4 &p2)
+{
+    if(!(m.vol.w > 0.0f)) return mid;
+    const vec3 e1{p0.w, p1.x, p1.y}, e2{p1.z, p1.w, p2.x};
+    return dot(e1, vierkant::rt::cross(d, e2)) > 0.0f ? (mid | KM_FRONT) : mid;
+}
 
 constexpr uint32_t MAX_TEX_LEVELS = 16;
 
@@ -124,6 +137,7 @@ struct pt_frame
     uint32_t cache_on, cache_depth, train_stride, train_offset, n_train_paths, train_max;
     // analytic lights: next-event estimation and the light bodies BSDF rays meet
     uint32_t n_lights, mis_mode, suppress_caustics;
+    uint32_t any_volume;// some material has an interior medium: paths keep the media stack's materials
     float cone_spread;// atan(2 tan(fov / 2) / height) as a half (raypipeline.slang:102)
     vec3 eye, right, up, forward;
     float tan_half_x, tan_half_y, aperture, focal_distance;
@@ -136,6 +150,7 @@ struct pt_buffers
     float4 *beta, *rad;     // throughput, path radiance
     uint4 *state;           // raygen rng, flags, sample offset bits, -
     float4 *iors;           // media stack iors
+    uint2 *media;           // media stack: the material of each level's interior, 16 bits each (0xffff: none)
     float4 *acc;            // sum of the batch's sample radiance (rgb), coverage
     uint4 *info;            // first-hit depth bits, normal, object id, albedo
     uint32_t *queue[2];
@@ -377,6 +392,7 @@ __global__ void __launch_bounds__(PT_BLOCK) k_pt_begin(pt_frame f, pt_buffers b,
     b.beta[p] = make_float4(1.f, 1.f, 1.f, 0.f);
     b.rad[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     b.iors[p] = make_float4(1.f, 1.f, 1.f, 1.f);
+    if(f.any_volume) b.media[p] = make_uint2(0xffffffffu, 0xffffffffu);
     b.cone[p] = make_float2(f.cone_spread, 0.f);
     if(n_train == 0u) b.queue[0][p] = p;
     else if(st.y & F_TRAIN) b.train_queue[(p - f.train_offset) / f.train_stride] = p;
@@ -517,6 +533,37 @@ __device__ vec3 ghost_lights(const vkrt_light *__restrict__ lights, const pt_fra
     return radiance;
 }
 
+constexpr uint32_t NO_MEDIUM = 0xffffu;
+
+__device__ __forceinline__ uint32_t media_level(const uint2 &m, uint32_t level)
+{
+    return ((level < 2u ? m.x : m.y) >> (16u * (level & 1u))) & 0xffffu;
+}
+
+__device__ __forceinline__ void set_media_level(uint2 &m, uint32_t level, uint32_t mid)
+{
+    uint32_t &w = level < 2u ? m.x : m.y;
+    const uint32_t sh = 16u * (level & 1u);
+    w = (w & ~(0xffffu << sh)) | ((mid & 0xffffu) << sh);
+}
+
+//! the participating medium of the segment that ended at a hit (raypipeline.slang:380-415), absorption
+//! only (a medium of single-scattering albedo 0): a channel is picked, a collision distance drawn against its
+//! extinction; a collision before the surface absorbs the path (the reference's throughput becomes 0 there),
+//! else the throughput takes the segment's transmittance over the pdf of passing. Returns true on a collision
+__device__ __forceinline__ bool medium_collision(const float4 &sigma, float t_hit, vec3 &beta, uint32_t &rng)
+{
+    if(!(sigma.x > 0.f || sigma.y > 0.f || sigma.z > 0.f)) return false;
+    const int channel = static_cast<int>(fminf(rnd(rng) * 3.0f, 2.0f));
+    const float sc = channel == 0 ? sigma.x : (channel == 1 ? sigma.y : sigma.z);
+    const float t = fminf(-logf(1.0f - rnd(rng)) / fmaxf(sc, EPS), t_hit);
+    if(t < t_hit) return true;
+    const vec3 tr{expf(-sigma.x * t), expf(-sigma.y * t), expf(-sigma.z * t)};
+    const float pdf = fmaxf((tr.x + tr.y + tr.z) / 3.0f, PDF_EPS);
+    beta = vec3{beta.x * (tr.x / pdf), beta.y * (tr.y / pdf), beta.z * (tr.z / pdf)};
+    return false;
+}
+
 //! one path's vertex at `depth` (closest hit or miss): emission, next-event sample, BSDF sample, Russian
 //! roulette, light bodies met on the segment; writes the path state back and reports what the path needs
 //! next. LIGHTS (the frame has analytic lights) compiles next-event estimation and the light-body passes in,
@@ -546,6 +593,7 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
     float4 iors = b.iors[p];
     float ior_stack[MEDIA_STACK] = {iors.x, iors.y, iors.z, iors.w};
     uint32_t media_index = flags & F_MEDIA_MASK;
+    uint2 media = f.any_volume ? b.media[p] : make_uint2(0xffffffffu, 0xffffffffu);
     uint4 h = b.hit[p];
     bool stop = false;
     to_cache = false;
@@ -573,6 +621,15 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
             radiance = radiance + mul3(beta, dl);
             v_e = v_e + dl;
         }
+    }
+    else if(f.any_volume &&
+            medium_collision(media_level(media, media_index) == NO_MEDIUM
+                                     ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                     : s.mats[media_level(media, media_index)].vol,
+                             __uint_as_float(h.x), beta, rng))
+    {
+        // absorbed in the medium before the surface: the path ends with what it has gathered
+        stop = true;
     }
     else
     {
@@ -734,8 +791,9 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         float eta = backface ? m.ior / last_ior : media_ior / m.ior;
         eta += EPS;
         next_ior = m.ior;
-        // decided on the material's own transmission, before its texture (raypipeline.slang:517-522)
-        const bool media_boundary = gm.mrti.z > 0.0f;
+        // decided on the material's own transmission (before its texture) or its interior medium
+        // (raypipeline.slang:517-522)
+        const bool media_boundary = gm.mrti.z > 0.0f || gm.vol.w > 0.0f;
         media_op = !media_boundary ? 0u : (backface ? 2u : 1u);
         if(depth == 0) albedo = m.color;
         const vec3 sn = normal;
@@ -853,6 +911,7 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
     {
         media_index = min(media_index + 1u, MEDIA_STACK - 1u);
         ior_stack[media_index] = round_half(next_ior);
+        if(f.any_volume) set_media_level(media, media_index, s.object_material[object]);
     }
     else if(media_op == 2u)
         media_index = media_index > 0u ? media_index - 1u : 0u;
@@ -877,6 +936,7 @@ __device__ __forceinline__ void shade_path(const pt_frame &f, const pt_buffers &
         b.rad[p] = make_float4(radiance.x, radiance.y, radiance.z, 0.f);
         b.cone[p] = cone;
         b.iors[p] = make_float4(ior_stack[0], ior_stack[1], ior_stack[2], ior_stack[3]);
+        if(f.any_volume) b.media[p] = media;
     }
     else
     {
@@ -1046,7 +1106,7 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
 {
     int32_t spill[TR_STACK - PT_LDS];
     lane_stack<PT_BLOCK, PT_LDS> st{lds, spill};
-    float T0 = 1.f, T1 = 1.f, T2 = 1.f;
+    float T0 = 1.f, T1 = 1.f, T2 = 1.f, D0 = 0.f, D1 = 0.f, D2 = 0.f;
     bool blocked = false, more = !empty, first_pass = true;
     float lb_t = 0.f;
     uint32_t lb_p = 0;
@@ -1121,6 +1181,7 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
                         break;
                     }
                     const uint32_t prim = __float_as_uint(p2.y);
+                    const uint32_t tag = candidate_tag(mid, m, r.d, p0, p1, p2);
                     if(!first_pass && (t < lb_t || (t == lb_t && prim <= lb_p))) continue;
                     if(cnt == TK && (t > kt[TK - 1] || (t == kt[TK - 1] && prim > kp[TK - 1])))
                     {
@@ -1130,7 +1191,7 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
                     if(cnt == TK) more = true;// the largest kept candidate is pushed out
                     // insert into the sorted list (ascending t, then triangle)
                     float ct = t;
-                    uint32_t cp = prim, cm = mid;
+                    uint32_t cp = prim, cm = tag;
 #pragma unroll
                     for(int j = 0; j < TK; ++j)
                     {
@@ -1164,17 +1225,27 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
         {
             if(j < cnt)
             {
-                const gpu_material m = mats[km[j]];
+                const gpu_material m = mats[km[j] & ~KM_FRONT];
                 T0 = T0 * (m.mrti.z * m.color.x);
                 T1 = T1 * (m.mrti.z * m.color.y);
                 T2 = T2 * (m.mrti.z * m.color.z);
+                if(m.vol.w > 0.0f)
+                {
+                    const bool front = (km[j] & KM_FRONT) != 0u;
+                    D0 += front ? -m.vol.x * kt[j] : m.vol.x * kt[j];
+                    D1 += front ? -m.vol.y * kt[j] : m.vol.y * kt[j];
+                    D2 += front ? -m.vol.z * kt[j] : m.vol.z * kt[j];
+                }
                 lb_t = kt[j];
                 lb_p = kp[j];
             }
         }
         first_pass = false;
     }
-    return blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
+    // unpaired entries (the light inside a volume) must not amplify: the depth is clamped at 0
+    return blocked ? make_float4(0.f, 0.f, 0.f, 0.f)
+                   : make_float4(T0 * expf(-fmaxf(D0, 0.f)), T1 * expf(-fmaxf(D1, 0.f)), T2 * expf(-fmaxf(D2, 0.f)),
+                                 0.f);
 }
 
 //! transmittance_walk over an instanced scene: the same passes, with the top level and the meshes walked in
@@ -1186,7 +1257,7 @@ __device__ float4 transmittance_walk_scene(const pt_tlas &tl, const gpu_material
     int32_t spill[SCENE_STACK - PT_LDS];
     lane_stack<PT_BLOCK, PT_LDS> st{lds, spill};
     const ray_setup rw = make_ray_setup(ra, rb);
-    float T0 = 1.f, T1 = 1.f, T2 = 1.f;
+    float T0 = 1.f, T1 = 1.f, T2 = 1.f, D0 = 0.f, D1 = 0.f, D2 = 0.f;
     bool blocked = false, more = !tl.empty, first_pass = true;
     float lb_t = 0.f;
     uint64_t lb_id = 0;
@@ -1267,6 +1338,7 @@ __device__ float4 transmittance_walk_scene(const pt_tlas &tl, const gpu_material
                         break;
                     }
                     const uint64_t id = ((uint64_t)inst << 32) | __float_as_uint(p2.y);
+                    const uint32_t tag = candidate_tag(mid, m, r.d, p0, p1, p2);
                     if(!first_pass && (t < lb_t || (t == lb_t && id <= lb_id))) continue;
                     if(cnt == TK && (t > kt[TK - 1] || (t == kt[TK - 1] && id > kid[TK - 1])))
                     {
@@ -1276,7 +1348,7 @@ __device__ float4 transmittance_walk_scene(const pt_tlas &tl, const gpu_material
                     if(cnt == TK) more = true;
                     float ct = t;
                     uint64_t cid = id;
-                    uint32_t cm = mid;
+                    uint32_t cm = tag;
 #pragma unroll
                     for(int j = 0; j < TK; ++j)
                     {
@@ -1316,17 +1388,27 @@ __device__ float4 transmittance_walk_scene(const pt_tlas &tl, const gpu_material
         {
             if(j < cnt)
             {
-                const gpu_material m = mats[km[j]];
+                const gpu_material m = mats[km[j] & ~KM_FRONT];
                 T0 = T0 * (m.mrti.z * m.color.x);
                 T1 = T1 * (m.mrti.z * m.color.y);
                 T2 = T2 * (m.mrti.z * m.color.z);
+                if(m.vol.w > 0.0f)
+                {
+                    const bool front = (km[j] & KM_FRONT) != 0u;
+                    D0 += front ? -m.vol.x * kt[j] : m.vol.x * kt[j];
+                    D1 += front ? -m.vol.y * kt[j] : m.vol.y * kt[j];
+                    D2 += front ? -m.vol.z * kt[j] : m.vol.z * kt[j];
+                }
                 lb_t = kt[j];
                 lb_id = kid[j];
             }
         }
         first_pass = false;
     }
-    return blocked ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(T0, T1, T2, 0.f);
+    // unpaired entries (the light inside a volume) must not amplify: the depth is clamped at 0
+    return blocked ? make_float4(0.f, 0.f, 0.f, 0.f)
+                   : make_float4(T0 * expf(-fmaxf(D0, 0.f)), T1 * expf(-fmaxf(D1, 0.f)), T2 * expf(-fmaxf(D2, 0.f)),
+                                 0.f);
 }
 
 template<bool INST>
@@ -1594,6 +1676,7 @@ struct vkrt_pt
     int launch_mode = VKRT_PT_LAUNCH_FULL_GRID;// per-depth launch shape (vkrt_pt_set_launch_mode)
     uint32_t tail_depth = vkrt::PT_TAIL_DEPTH;  // survivors at this depth finish in one lane (k_pt_finish)
     bool any_alpha = false;                    // some material is alpha-tested (MASK or BLEND)
+    bool any_volume = false;                   // some material has an interior medium
     // instanced scenes (vkrt_pt_set_instanced_scene): one BVH per mesh, the top level over the instances, per
     // instance its shading transform, per object its object-space half extents; `bounds` is the scene's world
     // box either way (the cache grid's extent)
@@ -1612,7 +1695,7 @@ struct vkrt_pt
     // texture coordinates, texture descriptors and their texels (all mip levels)
     vkrt::device_buffer uv, tan, tex_desc_buf, texels;
     uint32_t width = 0, height = 0;
-    vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, acc, info, queue0, queue1, counter, pixels, cone;
+    vkrt::device_buffer ray_o, ray_d, hit, beta, rad, state, iors, media, acc, info, queue0, queue1, counter, pixels, cone;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // the cache mode's training paths: their own queues, traced on a second stream beside the main paths
     hipStream_t train_stream = nullptr;
@@ -1674,6 +1757,7 @@ static void resize_storage(vkrt_pt *pt, uint32_t w, uint32_t h)
     pt->rad.reserve(n * 16);
     pt->state.reserve(n * 16);
     pt->iors.reserve(n * 16);
+    pt->media.reserve(n * 8);
     pt->cone.reserve(n * 8);
     pt->acc.reserve(n * 16);
     pt->info.reserve(n * 16);
@@ -1697,6 +1781,7 @@ static pt_buffers buffers(vkrt_pt *pt)
     b.rad = pt->rad.as<float4>();
     b.state = pt->state.as<uint4>();
     b.iors = pt->iors.as<float4>();
+    b.media = pt->media.as<uint2>();
     b.cone = pt->cone.as<float2>();
     b.acc = pt->acc.as<float4>();
     b.info = pt->info.as<uint4>();
@@ -1783,6 +1868,7 @@ static pt_frame make_frame(vkrt_pt *pt, const vkrt_camera &cam, uint32_t seed)
     f.n_lights = pt->n_lights;
     f.mis_mode = pt->settings.mis_mode;
     f.suppress_caustics = pt->settings.suppress_refractive_caustics ? 1u : 0u;
+    f.any_volume = pt->any_volume ? 1u : 0u;
     return f;
 }
 
@@ -2183,6 +2269,21 @@ static void upload_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
             for(uint32_t k: slots) VKRT_REQUIRE(k <= sc->n_textures, "material texture index out of range");
             mats[m].tex2 = make_uint4(slots[0], slots[1], slots[2], slots[3]);
             mats[m].tex3 = make_uint4(slots[4], slots[5], 0u, 0u);
+            // KHR_materials_volume: sigma_t = -log(attenuation color) / attenuation distance (raypipeline.slang:493)
+            const float ad = x.attenuation_distance;
+            VKRT_REQUIRE(std::isfinite(ad) && ad >= 0.f, "attenuation_distance must be finite and >= 0 (0: no volume)");
+            mats[m].vol = make_float4(0.f, 0.f, 0.f, 0.f);
+            if(ad > 0.f)
+            {
+                float sg[3];
+                for(int k = 0; k < 3; ++k)
+                {
+                    const float c = x.attenuation_color[k];
+                    VKRT_REQUIRE(c > 0.f && c <= 1.f, "attenuation_color must lie in (0, 1]");
+                    sg[k] = -std::log(c) / ad;
+                }
+                mats[m].vol = make_float4(sg[0], sg[1], sg[2], 1.f);
+            }
         }
         // texture coordinates (zeros when the scene has none) and the textures' mip chains: each level halves
         // the previous one (at least 1 texel) with a 2 x 2 box filter, coordinates clamped at odd edges,
@@ -2253,6 +2354,9 @@ static void upload_scene(vkrt_pt *pt, const vkrt_pt_scene *sc)
         upload(pt->object_tmax, tmax.data(), tmax.size() * 4);
         upload(pt->object_he, half.data(), half.size() * 4);
         upload(pt->mats, mats.data(), mats.size() * sizeof(gpu_material));
+        pt->any_volume = std::any_of(mats.begin(), mats.end(), [](const gpu_material &m) { return m.vol.w > 0.f; });
+        VKRT_REQUIRE(!pt->any_volume || sc->n_materials < NO_MEDIUM,
+                     "scenes with interior media take at most 65,534 materials");
         pt->any_alpha = std::any_of(mats.begin(), mats.end(),
                                     [](const gpu_material &m) { return m.tex.y != VKRT_ALPHA_OPAQUE; });
         upload(pt->uv, uv.data(), uv.size() * 4);
