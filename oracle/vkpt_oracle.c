@@ -2,9 +2,10 @@
  * vkpt_oracle.c - CPU reference integrator for the path tracer (TEST INFRASTRUCTURE ONLY).
  *
  * A plain-C restatement, pixel by pixel, of the reference's ray-tracing programs for the scenes the
- * tracer supports (no textures, no participating media, no iridescence/dispersion):
+ * tracer supports (absorbing interior media but no scattering or emissive media, no iridescence/dispersion):
  *   raygen trace() + main, ghost-light pass    shaders/slang/raypipeline.slang:58-270 (:201-222)
  *   closest_hit (surface subset), NEE call     shaders/slang/raypipeline.slang:360-680 (:615-619)
+ *   homogeneous absorbing medium step          shaders/slang/raypipeline.slang:380-415, :493
  *   miss_default, eval_distant_lights          shaders/slang/raypipeline.slang:728-761
  *   sample_disney, eval_disney + lobes         shaders/slang/ray/bsdf_disney.slang:44-381
  *   analytic lights, MIS, nee_surface          shaders/slang/ray/direct_lighting.slang:32-336
