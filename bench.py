@@ -403,9 +403,11 @@ def cache_kernel_rooflines(ctx, cache, n_query=1 << 20, n_train=1 << 16, reps=10
             "peak_tflops": peak}
 
 
-def instanced_frame(ctx, primary, bounce, verify_rays, reps=5):
+def instanced_frame(ctx, primary, bounce, verify_rays, reps=5, flat_mesh=None):
     """the heightfield + spheres scene as a two-level scene (1 heightfield instance + 1,000 instances of
-    one unit IcoSphere(1, 3)): the same primary and bounce rays through the instance-level walk"""
+    one unit IcoSphere(1, 3)): the same primary and bounce rays through the instance-level walk; with
+    flat_mesh (positions, indices) also the flat scene's BVH walked as one identity instance, which separates
+    the cost of the walk's level switching from that of the two-level structure"""
     import numpy as np
     import vierkant_amd as vk
     from vierkant_amd import workloads as W
@@ -430,6 +432,27 @@ def instanced_frame(ctx, primary, bounce, verify_rays, reps=5):
         ev.sync(2)
         ts.append((ev.ms(0, 1), ev.ms(1, 2)))
     p_ms, b_ms = float(np.median([t[0] for t in ts])), float(np.median([t[1] for t in ts]))
+    one = None
+    if flat_mesh is not None:
+        fb = ctx.bvh_build(*flat_mesh, builder=vk.BUILDER_PLOC, max_leaf_size=MAX_LEAF)
+        ident = np.array([vk.instance(np.hstack([np.eye(3), np.zeros((3, 1))]), 0)], dtype=vk.INSTANCE_DTYPE)
+        sc1 = vk.Scene(ctx, [fb], ident)
+        sc1.trace_closest(dp, out=hp)
+        t1 = []
+        for _ in range(reps):
+            ev.record(0, ctx.stream)
+            sc1.trace_closest(dp, out=hp, flags=vk.ASYNC)
+            ev.record(1, ctx.stream)
+            sc1.trace_closest(db, out=hb, flags=vk.ASYNC)
+            ev.record(2, ctx.stream)
+            ev.sync(2)
+            t1.append((ev.ms(0, 1), ev.ms(1, 2)))
+        one = {"primary_ms": round(float(np.median([t[0] for t in t1])), 4),
+               "bounce_ms": round(float(np.median([t[1] for t in t1])), 4)}
+        sc1.close()
+        fb.close()
+        sc.trace_closest(dp, out=hp)
+        sc.trace_closest(db, out=hb)
     mismatches = None
     if verify_rays:
         import oracle_binding as orc
@@ -451,7 +474,7 @@ def instanced_frame(ctx, primary, bounce, verify_rays, reps=5):
                         f"({sum(len(i) // 3 for _, i in meshes)} mesh triangles); same primary and bounce rays",
             "grays_per_s": round((len(primary) + len(bounce)) / ((p_ms + b_ms) * 1e-3) / 1e9, 4),
             "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4), "top_level_nodes": n_nodes,
-            "stack_bound": stack,
+            "stack_bound": stack, "flat_bvh_as_one_instance": one,
             "verified_vs_instance_brute_force": None if mismatches is None else
             {"rays": 2 * verify_rays, "mismatches": mismatches}}
 
@@ -818,7 +841,8 @@ def main():
         a.free()
     if args.instanced:
         log("two-level scene")
-        result["config"]["instanced"] = instanced_frame(ctx, primary, bounce, 128 if args.verify_rays else 0)
+        result["config"]["instanced"] = instanced_frame(ctx, primary, bounce, 128 if args.verify_rays else 0,
+                                                        flat_mesh=(pos, idx))
     if args.pt_spp:
         log("path tracer frame")
         result["config"]["path_tracer"] = path_tracer_frame(ctx, args.pt_spp)
