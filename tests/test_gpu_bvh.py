@@ -117,6 +117,23 @@ def test_bvh_refit(ctx, icosphere_case):
     assert_hits_equal(ctx.trace_closest(bvh, rays[:8192]), want2, "refit")
 
 
+def test_refit_keeps_the_4wide_shape(ctx, icosphere_case):
+    """a refit refreshes the boxes of the 4-wide collapse and keeps its shape: the node count, the stack bound
+    and the 4-wide flag stay what the build chose (scenes and batches keep them from creation), also under a
+    deformation that skews the boxes (half the sphere stretched 50 times along x); traces stay exact"""
+    pos, idx, tris, rays, want = icosphere_case
+    bvh = ctx.bvh_build(pos, idx)
+    before = bvh.info()
+    skewed = pos.copy()
+    skewed[skewed[:, 0] > 0, 0] *= np.float32(50.0)
+    bvh.refit(skewed)
+    after = bvh.info()
+    assert (after.n_wide_nodes, after.wide_stack) == (before.n_wide_nodes, before.wide_stack)
+    assert after.n_wide_nodes > 0
+    want2 = orc.brute_closest(W.flat_triangles(skewed, idx), rays[:8192])
+    assert_hits_equal(ctx.trace_closest(bvh, rays[:8192]), want2, "skewed refit")
+
+
 @pytest.fixture(scope="module")
 def heightfield_case():
     pos, idx = W.heightfield_spheres_scene()
