@@ -1243,9 +1243,10 @@ __device__ float4 transmittance_walk(const vkrt_bvh_node *__restrict__ nodes2, i
         first_pass = false;
     }
     // unpaired entries (the light inside a volume) must not amplify: the depth is clamped at 0
+    // (exp(-0) = 1 exactly: scenes without volumes skip the exp)
     return blocked ? make_float4(0.f, 0.f, 0.f, 0.f)
-                   : make_float4(T0 * expf(-fmaxf(D0, 0.f)), T1 * expf(-fmaxf(D1, 0.f)), T2 * expf(-fmaxf(D2, 0.f)),
-                                 0.f);
+                   : make_float4(D0 > 0.f ? T0 * expf(-D0) : T0, D1 > 0.f ? T1 * expf(-D1) : T1,
+                                 D2 > 0.f ? T2 * expf(-D2) : T2, 0.f);
 }
 
 //! transmittance_walk over an instanced scene: the same passes, with the top level and the meshes walked in
@@ -1406,9 +1407,10 @@ __device__ float4 transmittance_walk_scene(const pt_tlas &tl, const gpu_material
         first_pass = false;
     }
     // unpaired entries (the light inside a volume) must not amplify: the depth is clamped at 0
+    // (exp(-0) = 1 exactly: scenes without volumes skip the exp)
     return blocked ? make_float4(0.f, 0.f, 0.f, 0.f)
-                   : make_float4(T0 * expf(-fmaxf(D0, 0.f)), T1 * expf(-fmaxf(D1, 0.f)), T2 * expf(-fmaxf(D2, 0.f)),
-                                 0.f);
+                   : make_float4(D0 > 0.f ? T0 * expf(-D0) : T0, D1 > 0.f ? T1 * expf(-D1) : T1,
+                                 D2 > 0.f ? T2 * expf(-D2) : T2, 0.f);
 }
 
 template<bool INST>
