@@ -403,7 +403,7 @@ def cache_kernel_rooflines(ctx, cache, n_query=1 << 20, n_train=1 << 16, reps=10
             "peak_tflops": peak}
 
 
-def instanced_frame(ctx, primary, bounce, verify_rays, reps=5, flat_mesh=None):
+def instanced_frame(ctx, primary, bounce, verify_rays, reps=5, flat_mesh=None, builder=None):
     """the heightfield + spheres scene as a two-level scene (1 heightfield instance + 1,000 instances of
     one unit IcoSphere(1, 3)): the same primary and bounce rays through the instance-level walk; with
     flat_mesh (positions, indices) also the flat scene's BVH walked as one identity instance, which separates
@@ -413,7 +413,8 @@ def instanced_frame(ctx, primary, bounce, verify_rays, reps=5, flat_mesh=None):
     from vierkant_amd import workloads as W
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     meshes, mats, mesh_of = W.heightfield_spheres_instanced()
-    bvhs = [ctx.bvh_build(p, i, max_leaf_size=MAX_LEAF) for p, i in meshes]
+    builder = vk.BUILDER_PLOC if builder is None else builder   # the headline's builder
+    bvhs = [ctx.bvh_build(p, i, builder=builder, max_leaf_size=MAX_LEAF) for p, i in meshes]
     inst = np.array([vk.instance(m, k, custom_index=j) for j, (m, k) in enumerate(zip(mats, mesh_of))],
                     dtype=vk.INSTANCE_DTYPE)
     sc = vk.Scene(ctx, bvhs, inst)
@@ -434,7 +435,7 @@ def instanced_frame(ctx, primary, bounce, verify_rays, reps=5, flat_mesh=None):
     p_ms, b_ms = float(np.median([t[0] for t in ts])), float(np.median([t[1] for t in ts]))
     one = None
     if flat_mesh is not None:
-        fb = ctx.bvh_build(*flat_mesh, builder=vk.BUILDER_PLOC, max_leaf_size=MAX_LEAF)
+        fb = ctx.bvh_build(*flat_mesh, builder=builder, max_leaf_size=MAX_LEAF)
         ident = np.array([vk.instance(np.hstack([np.eye(3), np.zeros((3, 1))]), 0)], dtype=vk.INSTANCE_DTYPE)
         sc1 = vk.Scene(ctx, [fb], ident)
         sc1.trace_closest(dp, out=hp)
@@ -471,7 +472,8 @@ def instanced_frame(ctx, primary, bounce, verify_rays, reps=5, flat_mesh=None):
     for b in bvhs:
         b.close()
     return {"workload": f"heightfield + 1,000 spheres as {n_inst} instances of 2 meshes "
-                        f"({sum(len(i) // 3 for _, i in meshes)} mesh triangles); same primary and bounce rays",
+                        f"({sum(len(i) // 3 for _, i in meshes)} mesh triangles, "
+                        f"{'PLOC' if builder == vk.BUILDER_PLOC else 'LBVH'} mesh BVHs); same primary and bounce rays",
             "grays_per_s": round((len(primary) + len(bounce)) / ((p_ms + b_ms) * 1e-3) / 1e9, 4),
             "primary_ms": round(p_ms, 4), "bounce_ms": round(b_ms, 4), "top_level_nodes": n_nodes,
             "stack_bound": stack, "flat_bvh_as_one_instance": one,
@@ -842,7 +844,7 @@ def main():
     if args.instanced:
         log("two-level scene")
         result["config"]["instanced"] = instanced_frame(ctx, primary, bounce, 128 if args.verify_rays else 0,
-                                                        flat_mesh=(pos, idx))
+                                                        flat_mesh=(pos, idx), builder=builder)
     if args.pt_spp:
         log("path tracer frame")
         result["config"]["path_tracer"] = path_tracer_frame(ctx, args.pt_spp)

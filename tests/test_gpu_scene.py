@@ -108,6 +108,43 @@ def test_single_leaf_meshes(ctx, meshes, n_tri):
     big.close()
 
 
+def test_rays_through_many_instances(ctx, meshes):
+    """a line of 48 overlapping spheres (instances of two meshes, every fourth one a duplicate of its
+    neighbour's placement) and rays along and across it: each ray enters many instance boxes, so the walk
+    takes several top-level passes of four kept instances; the records equal the instance brute force, the
+    any-hit flags agree, and duplicates tie to the lower instance"""
+    bvhs = [ctx.bvh_build(*meshes[0]), ctx.bvh_build(*meshes[1])]
+    tris = [W.flat_triangles(*meshes[0]), W.flat_triangles(*meshes[1])]
+    inst = []
+    for k in range(48):
+        j = k - (k % 4 == 3)  # every fourth placement repeats the previous one, mesh included
+        m = np.hstack([np.eye(3) * 0.5, np.array([[0.6 * j], [0.0], [0.0]])])
+        inst.append(vk.instance(m, j % 2, custom_index=k))
+    inst = np.array(inst, dtype=vk.INSTANCE_DTYPE)
+    sc = vk.Scene(ctx, bvhs, inst)
+    rng = np.random.default_rng(8)
+    n = 3000
+    o = np.column_stack([np.full(n, -3.0), rng.uniform(-0.4, 0.4, n), rng.uniform(-0.4, 0.4, n)]).astype(np.float32)
+    tgt = np.column_stack([np.full(n, 40.0), rng.uniform(-0.4, 0.4, n), rng.uniform(-0.4, 0.4, n)]).astype(np.float32)
+    side = np.column_stack([rng.uniform(-1, 29, n), np.full(n, 3.0), rng.uniform(-0.3, 0.3, n)]).astype(np.float32)
+    stgt = side + np.float32([0.3, -6.0, 0.0])
+    rays = np.concatenate([W.make_rays(o, W.glm_normalize(tgt - o)), W.make_rays(side, W.glm_normalize(stgt - side))])
+    # rays that start inside the line: the nearest boxes are behind or around the origin
+    inner = W.make_rays(np.column_stack([rng.uniform(0, 28, n), np.zeros(n), np.zeros(n)]).astype(np.float32),
+                        W.glm_normalize(rng.normal(size=(n, 3)).astype(np.float32)), tmin=1e-3)
+    rays = np.concatenate([rays, inner])
+    got = sc.trace_closest(rays)
+    want = orc.scene_closest(tris, inst, rays)
+    assert (want["prim"] != 0xFFFFFFFF).sum() > 2 * n
+    assert_scene_hits_equal(got, want, "many instances")
+    assert np.array_equal(sc.trace_any(rays), (want["prim"] != 0xFFFFFFFF).astype(np.uint8))
+    hit = got["prim"] != 0xFFFFFFFF
+    assert not np.any(hit & (got["instance"] % 4 == 3))  # a repeated placement never beats its original
+    sc.close()
+    for b in bvhs:
+        b.close()
+
+
 def test_scene_overlapping_copies_tie_to_lower_instance(ctx, meshes):
     p, i = meshes[0]
     b = ctx.bvh_build(p, i)
