@@ -158,6 +158,10 @@ __global__ void k_tri_prep(const float *__restrict__ tris, uint32_t n, float *__
 // ---------------------------------------------------------------------------------------------
 // brute force
 // ---------------------------------------------------------------------------------------------
+// 65,536 rays x 20,480 triangles: 1.18 ms per launch against 1.23 one triangle at a time; eight at a time
+// (98 VGPRs) is no faster
+constexpr int BRUTE_UNROLL = 4;
+
 __global__ void __launch_bounds__(256) k_brute_closest(const float4 *__restrict__ rays, uint32_t n_ray,
                                                        const float *__restrict__ pre, uint32_t n_tri,
                                                        uint32_t chunk, uint64_t *__restrict__ keys)
@@ -170,11 +174,33 @@ __global__ void __launch_bounds__(256) k_brute_closest(const float4 *__restrict_
     if(r < n_ray) load_ray(rays, r, o, tmin, d, tmax);
     float best = tmax;
     uint32_t best_i = VKRT_INVALID_PRIM;
-    for(uint32_t i = t0; i < t1; ++i)
+    // the triangle index is wave-uniform, so records come through scalar loads, four at a time: one wider
+    // request and one loop step per four tests (the loop's scalar work competes with the tests for issue)
+    uint32_t i = t0;
+    for(; i + BRUTE_UNROLL <= t1; i += BRUTE_UNROLL)
+    {
+        const float *p = pre + 9 * (size_t)i;
+        float rec[9 * BRUTE_UNROLL];
+#pragma unroll
+        for(int k = 0; k < 9 * BRUTE_UNROLL; ++k) rec[k] = p[k];
+#pragma unroll
+        for(int k = 0; k < BRUTE_UNROLL; ++k)
+        {
+            const float *q = rec + 9 * k;
+            float t, u, v;
+            if(mt_fast_window(o, d, vec3{q[0], q[1], q[2]}, vec3{q[3], q[4], q[5]}, vec3{q[6], q[7], q[8]}, tmin,
+                              best, t, u, v))
+            {
+                best = t;
+                best_i = i + k;
+            }
+        }
+    }
+    for(; i < t1; ++i)
     {
         const float *p = pre + 9 * (size_t)i;
         float t, u, v;
-        if(mt_fast(o, d, ld3(p), ld3(p + 3), ld3(p + 6), t, u, v) && t > tmin && t < best)
+        if(mt_fast_window(o, d, ld3(p), ld3(p + 3), ld3(p + 6), tmin, best, t, u, v))
         {
             best = t;
             best_i = i;

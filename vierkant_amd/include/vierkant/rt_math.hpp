@@ -83,8 +83,11 @@ VK_HD bool mt_reference(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3
 //! intermediates fail every bound and fall through to the reference sequence.
 //! With `cull_back` set, a triangle whose det is negative (its counter-clockwise normal faces away from
 //! the ray) is rejected as well; that is the back-face culling of a ray traced in air.
-VK_HD bool mt_fast(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1, const vec3 &e2, float &t, float &u,
-                   float &v, bool cull_back = false)
+//! With WINDOW set, a hit is reported only for t_above < t < t_below (a closest-hit loop's window), a test that
+//! then joins the tail's last branch.
+template<bool WINDOW>
+VK_HD bool mt_fast_impl(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1, const vec3 &e2, float &t,
+                        float &u, float &v, bool cull_back, float t_above, float t_below)
 {
     // every division-free test is evaluated and joined with '|' into one branch: on a GPU the early-outs
     // cost more in divergent branching than the few products they skip
@@ -111,10 +114,25 @@ VK_HD bool mt_fast(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1,
     if(uu < 0.0f || uu > 1.0f) return false;
     float vv = b * inv_det;
     if(vv < 0.0f || (uu + vv) > 1.0f) return false;
-    t = dot(e2, qvec) * inv_det;
+    const float tt = dot(e2, qvec) * inv_det;
+    if(WINDOW && !(tt > t_above && tt < t_below)) return false;
+    t = tt;
     u = uu;
     v = vv;
     return true;
+}
+
+VK_HD bool mt_fast(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1, const vec3 &e2, float &t, float &u,
+                   float &v, bool cull_back = false)
+{
+    return mt_fast_impl<false>(o, d, v0, e1, e2, t, u, v, cull_back, 0.0f, 0.0f);
+}
+
+//! mt_fast's hit restricted to t_above < t < t_below
+VK_HD bool mt_fast_window(const vec3 &o, const vec3 &d, const vec3 &v0, const vec3 &e1, const vec3 &e2, float t_above,
+                          float t_below, float &t, float &u, float &v)
+{
+    return mt_fast_impl<true>(o, d, v0, e1, e2, t, u, v, false, t_above, t_below);
 }
 
 //! intersect(const OBB&, const Ray&), src/intersection.cpp:152-184. axis[i] are the OBB's columns.
