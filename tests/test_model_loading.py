@@ -202,6 +202,8 @@ def test_load_errors_carry_a_message(tmp_path):
     ("negative view length", lambda d: d["bufferViews"][1].update(byteLength=-36),
      "invalid bufferView byteLength"),
     ("missing buffer", lambda d: d["bufferViews"][0].update(buffer=3), "buffer out of range"),
+    ("fractional buffer index", lambda d: d["bufferViews"][0].update(buffer=0.5), "buffer out of range"),
+    ("huge view index", lambda d: d["accessors"][0].update(bufferView=1e300), "bufferView out of range"),
 ])
 def test_gltf_malformed_accessors_are_rejected(tmp_path, name, edit, message):
     """accessor and bufferView bounds: negative, fractional and oversized sizes, strides below the element,

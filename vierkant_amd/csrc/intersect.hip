@@ -241,12 +241,31 @@ __global__ void __launch_bounds__(256) k_brute_any(const float4 *__restrict__ ra
     float tmin = 0.f, tmax = -1.f;
     if(r < n_ray) load_ray(rays, r, o, tmin, d, tmax);
     int occ = (r < n_ray) ? 0 : 1;
-    for(uint32_t i = t0; i < t1; ++i)
+    // four records per loop step, as in k_brute_closest; the wave leaves once every lane is occluded
+    uint32_t i = t0;
+    for(; i + BRUTE_UNROLL <= t1; i += BRUTE_UNROLL)
+    {
+        if(__all(occ)) break;
+        const float *p = pre + 9 * (size_t)i;
+        float rec[9 * BRUTE_UNROLL];
+#pragma unroll
+        for(int k = 0; k < 9 * BRUTE_UNROLL; ++k) rec[k] = p[k];
+#pragma unroll
+        for(int k = 0; k < BRUTE_UNROLL; ++k)
+        {
+            const float *q = rec + 9 * k;
+            float t, u, v;
+            if(!occ && mt_fast_window(o, d, vec3{q[0], q[1], q[2]}, vec3{q[3], q[4], q[5]}, vec3{q[6], q[7], q[8]},
+                                      tmin, tmax, t, u, v))
+                occ = 1;
+        }
+    }
+    for(; i < t1; ++i)
     {
         if(__all(occ)) break;
         const float *p = pre + 9 * (size_t)i;
         float t, u, v;
-        if(!occ && mt_fast(o, d, ld3(p), ld3(p + 3), ld3(p + 6), t, u, v) && t > tmin && t < tmax) occ = 1;
+        if(!occ && mt_fast_window(o, d, ld3(p), ld3(p + 3), ld3(p + 6), tmin, tmax, t, u, v)) occ = 1;
     }
     if(r < n_ray && occ) occluded[r] = 1;
 }

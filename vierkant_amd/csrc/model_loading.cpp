@@ -77,7 +77,13 @@ struct json
     }
     size_t size() const { return kind == array_t ? arr.size() : 0; }
     double number(double def) const { return kind == number_t ? num : def; }
-    int integer(int def) const { return kind == number_t ? static_cast<int>(num) : def; }
+    //! the number when it is an integer that fits an int, else `def` (NaN, fractions and out-of-range values
+    //! never reach the conversion)
+    int integer(int def) const
+    {
+        if(kind != number_t || !(num >= -2147483648.0 && num <= 2147483647.0) || num != std::floor(num)) return def;
+        return static_cast<int>(num);
+    }
 };
 
 struct json_parser
