@@ -141,6 +141,21 @@ def test_brute_closest_ties_resolve_to_lowest_index(ctx):
     assert_hits_equal(got, want, "ties")
 
 
+def test_brute_adjacent_duplicates_and_remainders(ctx):
+    """each triangle twice in a row, so ties fall inside one four-record loop step, and 602 triangles, so the
+    triangle chunks leave records after the last full step: the lower index wins; any-hit agrees"""
+    rng = np.random.default_rng(21)
+    tris = np.repeat(rng.uniform(-1, 1, (301, 9)).astype(np.float32), 2, axis=0)
+    o = rng.uniform(-3, 3, (3000, 3)).astype(np.float32)
+    d = W.glm_normalize(rng.normal(size=(3000, 3)).astype(np.float32))
+    rays = W.make_rays(o, d)
+    want = orc.brute_closest(tris, rays)
+    hit = want["prim"] != 0xFFFFFFFF
+    assert hit.sum() > 100 and np.all(want["prim"][hit] % 2 == 0)
+    assert_hits_equal(ctx.brute_closest(tris, rays), want, "adjacent duplicates")
+    assert np.array_equal(ctx.brute_any(tris, rays), orc.brute_any(tris, rays))
+
+
 @pytest.mark.parametrize("n_tri,n_ray", [(0, 10), (5, 0), (1, 1), (3, 300), (257, 1000)])
 def test_brute_sizes(ctx, n_tri, n_ray):
     rng = np.random.default_rng(n_tri * 7 + n_ray)
