@@ -62,6 +62,12 @@ constexpr uint32_t PT_TRACE_CHUNK = 8;
 constexpr uint32_t PT_CLAIMS = 64;            // word offset of the trace kernel's claim counters
 constexpr uint32_t PT_COUNTER_WORDS = PT_CLAIMS + CLAIM_COUNTERS;
 constexpr uint32_t PT_TRACE_WAVES_PER_SIMD = 6;// the trace kernel's register cap (amdgpu_waves_per_eu): 80 VGPRs
+// the filter-free trace kernel fits 64 VGPRs with no spill: 8 waves per SIMD instead of 7 (room, 64 spp at
+// 1080p: 170.9 ms against 174.2)
+constexpr uint32_t PT_TRACE_PLAIN_WAVES_PER_SIMD = 8;
+// the filter-free flat shadow kernel at 5 waves per SIMD: 96 VGPRs and 21 spilled instead of 121 at 4 (room
+// with six lights, 16 spp at 1080p: 90.7 ms against 93.4; at 6 waves 70 registers spill)
+constexpr uint32_t PT_SHADOW_PLAIN_WAVES_PER_SIMD = 5;
 // the two-level walk's cap: 96 VGPRs and 5 spilled instead of 80 and 14 (instanced room, 16 spp at 1080p:
 // 68.2 ms against 69.9 at 6 waves and 70.5 at 4)
 constexpr uint32_t PT_TRACE_INST_WAVES_PER_SIMD = 5;
@@ -463,7 +469,8 @@ __device__ __forceinline__ void trace_path(const vkrt_bvh_node *__restrict__ nod
 //! F is alpha_filter where some material is alpha-tested, keep_all elsewhere (the any-hit test and its texture
 //! fetch compiled out)
 template<class F, bool INST>
-__global__ void __launch_bounds__(PT_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(INST ? PT_TRACE_INST_WAVES_PER_SIMD : PT_TRACE_WAVES_PER_SIMD)))
+__global__ void __launch_bounds__(PT_TRACE_BLOCK) __attribute__((amdgpu_waves_per_eu(
+        INST ? PT_TRACE_INST_WAVES_PER_SIMD : (F::active ? PT_TRACE_WAVES_PER_SIMD : PT_TRACE_PLAIN_WAVES_PER_SIMD))))
 k_pt_trace(const vkrt_bvh_node *__restrict__ nodes, const float4 *__restrict__ tris, int32_t root, bool empty,
            const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4, bool wide, pt_buffers b, int in, F flt,
            uint32_t depth, pt_tlas tl)
@@ -1476,7 +1483,8 @@ __device__ __forceinline__ void shadow_path(const vkrt_bvh_node *__restrict__ no
 //! sampled light, then transmittance * weight * n_lights * radiance * F * cos / pdf scaled by the vertex's
 //! throughput joins the pixel's sample sum; a training vertex also records the unscaled estimate
 template<class F, bool INST>
-__global__ void __launch_bounds__(PT_BLOCK) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
+__global__ void __launch_bounds__(PT_BLOCK)
+__attribute__((amdgpu_waves_per_eu(INST || F::active ? 1 : PT_SHADOW_PLAIN_WAVES_PER_SIMD))) k_pt_shadow(const vkrt_bvh_node *__restrict__ nodes2, int32_t root2,
                                                         const vkrt_bvh4_node *__restrict__ nodes4, int32_t root4,
                                                         bool wide, bool empty, const float4 *__restrict__ tris,
                                                         const gpu_material *__restrict__ mats, pt_frame f,
