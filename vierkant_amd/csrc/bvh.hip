@@ -10,7 +10,8 @@
 //   radix sort  (code, triangle) pairs, project-written (radix_sort.hip)
 //   k_karras    binary radix tree over the sorted codes (Karras 2012), ties broken by position; or, with
 //               VKRT_BUILDER_PLOC, agglomerative clustering over the sorted leaves (k_ploc_*)
-//   k_refit     bottom-up AABB union, one thread per leaf, second arrival at a node continues
+//   k_box_*     node boxes as unions over each node's leaf run (segment prefix/suffix unions and a sparse
+//               table of segment unions)
 //   k_layout    BVH2 nodes holding both children's boxes (64 B), subtrees of <= max_leaf_size
 //               triangles collapsed into leaves over the contiguous sorted range
 //   k_reorder   triangles in leaf order as 48-B records (v0, e1, e2, original index)
@@ -61,37 +62,54 @@ __global__ void k_gather(const uint8_t *__restrict__ pos, uint32_t stride, const
                          uint32_t n_tri, uint32_t n_vert, float *__restrict__ pre, float *__restrict__ box,
                          uint32_t *__restrict__ cbounds, uint32_t *__restrict__ err)
 {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if(j >= n_tri) return;
-    uint32_t i0 = idx[3 * (size_t)j], i1 = idx[3 * (size_t)j + 1], i2 = idx[3 * (size_t)j + 2];
-    if(i0 >= n_vert || i1 >= n_vert || i2 >= n_vert)
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    // the centroid bounds are reduced across the wave first and reach memory as one atomic per bound and wave
+    // (one per triangle serialized 1.8 M triangles' updates on six words: about 1.9 ms); lanes past the end
+    // stay in the wave for the reduction
+    float cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if(j < n_tri)
     {
-        atomicOr(err, 1u);
-        i0 = i1 = i2 = 0;
+        uint32_t i0 = idx[3 * (size_t)j], i1 = idx[3 * (size_t)j + 1], i2 = idx[3 * (size_t)j + 2];
+        if(i0 >= n_vert || i1 >= n_vert || i2 >= n_vert)
+        {
+            atomicOr(err, 1u);
+            i0 = i1 = i2 = 0;
+        }
+        vec3 v0 = ld3(reinterpret_cast<const float *>(pos + (size_t)i0 * stride));
+        vec3 v1 = ld3(reinterpret_cast<const float *>(pos + (size_t)i1 * stride));
+        vec3 v2 = ld3(reinterpret_cast<const float *>(pos + (size_t)i2 * stride));
+        vec3 e1 = v1 - v0, e2 = v2 - v0;
+        float *p = pre + 9 * (size_t)j;
+        p[0] = v0.x, p[1] = v0.y, p[2] = v0.z, p[3] = e1.x, p[4] = e1.y, p[5] = e1.z, p[6] = e2.x, p[7] = e2.y,
+        p[8] = e2.z;
+        float *b = box + 6 * (size_t)j;
+        if(finite3(v0) && finite3(v1) && finite3(v2))
+        {
+            vec3 lo{fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y), fminf(fminf(v0.z, v1.z), v2.z)};
+            vec3 hi{fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y), fmaxf(fmaxf(v0.z, v1.z), v2.z)};
+            b[0] = lo.x, b[1] = lo.y, b[2] = lo.z, b[3] = hi.x, b[4] = hi.y, b[5] = hi.z;
+            cmin[0] = cmax[0] = 0.5f * lo.x + 0.5f * hi.x;
+            cmin[1] = cmax[1] = 0.5f * lo.y + 0.5f * hi.y;
+            cmin[2] = cmax[2] = 0.5f * lo.z + 0.5f * hi.z;
+        }
+        else
+        {
+            // a triangle with a non-finite vertex never yields a finite t (the reference test then produces
+            // NaN), so it gets an empty box and is never visited
+            b[0] = b[1] = b[2] = FLT_MAX;
+            b[3] = b[4] = b[5] = -FLT_MAX;
+        }
     }
-    vec3 v0 = ld3(reinterpret_cast<const float *>(pos + (size_t)i0 * stride));
-    vec3 v1 = ld3(reinterpret_cast<const float *>(pos + (size_t)i1 * stride));
-    vec3 v2 = ld3(reinterpret_cast<const float *>(pos + (size_t)i2 * stride));
-    vec3 e1 = v1 - v0, e2 = v2 - v0;
-    float *p = pre + 9 * (size_t)j;
-    p[0] = v0.x, p[1] = v0.y, p[2] = v0.z, p[3] = e1.x, p[4] = e1.y, p[5] = e1.z, p[6] = e2.x, p[7] = e2.y,
-    p[8] = e2.z;
-    float *b = box + 6 * (size_t)j;
-    if(finite3(v0) && finite3(v1) && finite3(v2))
+    for(int off = 32; off > 0; off >>= 1)
+        for(int k = 0; k < 3; ++k)
+        {
+            cmin[k] = fminf(cmin[k], __shfl_xor(cmin[k], off));
+            cmax[k] = fmaxf(cmax[k], __shfl_xor(cmax[k], off));
+        }
+    if((threadIdx.x & 63u) == 0 && cmin[0] <= cmax[0])
     {
-        vec3 lo{fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y), fminf(fminf(v0.z, v1.z), v2.z)};
-        vec3 hi{fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y), fmaxf(fmaxf(v0.z, v1.z), v2.z)};
-        b[0] = lo.x, b[1] = lo.y, b[2] = lo.z, b[3] = hi.x, b[4] = hi.y, b[5] = hi.z;
-        float cx = 0.5f * lo.x + 0.5f * hi.x, cy = 0.5f * lo.y + 0.5f * hi.y, cz = 0.5f * lo.z + 0.5f * hi.z;
-        atomic_min_f(&cbounds[0], cx), atomic_min_f(&cbounds[1], cy), atomic_min_f(&cbounds[2], cz);
-        atomic_max_f(&cbounds[3], cx), atomic_max_f(&cbounds[4], cy), atomic_max_f(&cbounds[5], cz);
-    }
-    else
-    {
-        // a triangle with a non-finite vertex never yields a finite t (the reference test then produces
-        // NaN), so it gets an empty box and is never visited
-        b[0] = b[1] = b[2] = FLT_MAX;
-        b[3] = b[4] = b[5] = -FLT_MAX;
+        atomic_min_f(&cbounds[0], cmin[0]), atomic_min_f(&cbounds[1], cmin[1]), atomic_min_f(&cbounds[2], cmin[2]);
+        atomic_max_f(&cbounds[3], cmax[0]), atomic_max_f(&cbounds[4], cmax[1]), atomic_max_f(&cbounds[5], cmax[2]);
     }
 }
 
@@ -188,41 +206,134 @@ __device__ __forceinline__ void load_box(const float *p, float b[6])
     for(int a = 0; a < 6; ++a) b[a] = p[a];
 }
 
-//! one arrival at node p of the bottom-up refit: the atomic counts the arrivals, and the second one (the
-//! return value 1) owns the node, its children's boxes being published by then. Release side: the box
-//! store of the previous step must be written back to memory before the flag increment is visible on
-//! another XCD (each XCD has its own L2). The acq_rel atomic emits that write-back (buffer_wbl2 sc1); the
-//! wait for the store and the write-back to complete is spelled out here, because hipcc leaves it out when
-//! the store and the atomic meet across a loop back edge, and without it another XCD could read a stale
-//! child box and union a too-small parent box.
-__device__ __forceinline__ uint32_t refit_arrive(uint32_t *flags, int32_t p)
+//! Node boxes without a bottom-up hand-off. Every internal node covers a contiguous run [first, last] of the
+//! leaf order (`range`), so its box is the union of those leaves' boxes. The leaves are cut into segments of
+//! BOX_SEG: k_box_scan writes, per leaf, the union from its segment's start (pre) and to its segment's end
+//! (suf), and each segment's union (level 0 of a sparse table, levels 1.. by k_box_sparse: level k holds the
+//! union of 2^k segments from each start). A run that crosses segments is suf[first], pre[last] and at most two
+//! overlapping table entries; a run inside one segment is pre or suf when it touches the segment's start or
+//! end, else a loop over its (at most BOX_SEG - 2) leaves. Every pass reads what earlier kernels wrote, so no
+//! box crosses between workgroups, or XCDs, inside a kernel. A min or max is exact whatever the grouping, so
+//! the boxes are the ones the pairwise union up the tree gives; only leaves' boxes are combined (the padding
+//! past the last leaf is never read into a union).
+constexpr uint32_t BOX_SEG = 64;
+
+__device__ __forceinline__ void box_union(float a[6], const float b[6])
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-    const uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    return old;
+    for(int k = 0; k < 3; ++k) a[k] = fminf(a[k], b[k]);
+    for(int k = 3; k < 6; ++k) a[k] = fmaxf(a[k], b[k]);
 }
 
-__global__ void k_refit(uint32_t n, const int2 *__restrict__ child, const int32_t *__restrict__ parent_int,
-                        const int32_t *__restrict__ parent_leaf, const float *__restrict__ tbox,
-                        float *__restrict__ nbox, uint32_t *__restrict__ flags)
+//! one segment of BOX_SEG leaves per workgroup (one wave): inclusive prefix and suffix unions in LDS
+__global__ void __launch_bounds__(BOX_SEG) k_box_scan(uint32_t n, const float *__restrict__ tbox,
+                                                      float *__restrict__ pre, float *__restrict__ suf,
+                                                      float *__restrict__ seg)
 {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if(j >= n) return;
-    int32_t p = parent_leaf[j];
-    if(p < 0 || refit_arrive(flags, p) == 0) return;// the sibling subtree is not finished; its thread continues
-    while(true)
+    __shared__ float sp[6][BOX_SEG], ss[6][BOX_SEG];
+    const uint32_t t = threadIdx.x, j = blockIdx.x * BOX_SEG + t;
+    const uint32_t cnt = min(BOX_SEG, n - blockIdx.x * BOX_SEG);// leaves in this segment
+    float a[6], b[6];
+    if(t < cnt) load_box(tbox + 6 * (size_t)j, a);
+    else
+        for(int k = 0; k < 6; ++k) a[k] = 0.f;// past the last leaf: never combined into a leaf's union
+    for(int k = 0; k < 6; ++k) b[k] = a[k];
+    for(uint32_t off = 1; off < BOX_SEG; off <<= 1)
     {
-        int2 c = child[p];
-        float a[6], b[6];
-        load_box(c.x < 0 ? tbox + 6 * (size_t)(~c.x) : nbox + 6 * (size_t)c.x, a);
-        load_box(c.y < 0 ? tbox + 6 * (size_t)(~c.y) : nbox + 6 * (size_t)c.y, b);
-        float *o = nbox + 6 * (size_t)p;
-        for(int k = 0; k < 3; ++k) o[k] = fminf(a[k], b[k]);
-        for(int k = 3; k < 6; ++k) o[k] = fmaxf(a[k], b[k]);
-        p = parent_int[p];
-        if(p < 0 || refit_arrive(flags, p) == 0) return;
+        for(int k = 0; k < 6; ++k)
+        {
+            sp[k][t] = a[k];
+            ss[k][t] = b[k];
+        }
+        __syncthreads();
+        if(t >= off)
+        {
+            float o[6];
+            for(int k = 0; k < 6; ++k) o[k] = sp[k][t - off];
+            box_union(a, o);
+        }
+        if(t + off < cnt)
+        {
+            float o[6];
+            for(int k = 0; k < 6; ++k) o[k] = ss[k][t + off];
+            box_union(b, o);
+        }
+        __syncthreads();
     }
+    if(t < cnt)
+    {
+        for(int k = 0; k < 6; ++k)
+        {
+            pre[6 * (size_t)j + k] = a[k];
+            suf[6 * (size_t)j + k] = b[k];
+        }
+    }
+    if(t == 0)
+        for(int k = 0; k < 6; ++k) seg[6 * (size_t)blockIdx.x + k] = b[k];// leaf 0's suffix: the segment
+}
+
+//! sparse-table level k from level k - 1: entry i = union of entries i and i + 2^(k-1), for i + 2^k <= nseg
+__global__ void k_box_sparse(uint32_t nseg, uint32_t k, float *__restrict__ table)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t half = 1u << (k - 1);
+    if((size_t)i + 2 * half > nseg) return;
+    const float *lo = table + 6 * ((size_t)(k - 1) * nseg + i);
+    const float *hi = table + 6 * ((size_t)(k - 1) * nseg + i + half);
+    float a[6], b[6];
+    load_box(lo, a);
+    load_box(hi, b);
+    box_union(a, b);
+    for(int c = 0; c < 6; ++c) table[6 * ((size_t)k * nseg + i) + c] = a[c];
+}
+
+__global__ void k_box_nodes(uint32_t ni, uint32_t n, uint32_t nseg, const uint2 *__restrict__ range,
+                            const float *__restrict__ tbox, const float *__restrict__ pre,
+                            const float *__restrict__ suf, const float *__restrict__ table, float *__restrict__ nbox)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if(p >= ni) return;
+    const uint2 r = range[p];
+    const uint32_t first = r.x, last = r.y, sf = first / BOX_SEG, sl = last / BOX_SEG;
+    float o[6], x[6];
+    if(sf == sl)
+    {
+        const bool at_end = last % BOX_SEG == BOX_SEG - 1 || last == n - 1;
+        if(first % BOX_SEG == 0) load_box(pre + 6 * (size_t)last, o);
+        else if(at_end) load_box(suf + 6 * (size_t)first, o);
+        else
+        {
+            load_box(tbox + 6 * (size_t)first, o);
+            for(uint32_t j = first + 1; j <= last; ++j)
+            {
+                load_box(tbox + 6 * (size_t)j, x);
+                box_union(o, x);
+            }
+        }
+    }
+    else
+    {
+        load_box(suf + 6 * (size_t)first, o);
+        load_box(pre + 6 * (size_t)last, x);
+        box_union(o, x);
+        if(sl > sf + 1)
+        {
+            const uint32_t lo = sf + 1, count = sl - 1 - lo + 1;
+            const uint32_t k = 31u - __builtin_clz(count);
+            load_box(table + 6 * ((size_t)k * nseg + lo), x);
+            box_union(o, x);
+            load_box(table + 6 * ((size_t)k * nseg + (sl - 1) - (1u << k) + 1), x);
+            box_union(o, x);
+        }
+    }
+    for(int c = 0; c < 6; ++c) nbox[6 * (size_t)p + c] = o[c];
+}
+
+//! the sparse table's depth for nseg segments: levels 0 .. floor(log2(nseg))
+static uint32_t box_levels(uint32_t nseg)
+{
+    uint32_t l = 1;
+    while((2ull << (l - 1)) <= nseg) ++l;
+    return l;
 }
 
 __device__ __forceinline__ float pad_lo(float x) { return x - fabsf(x) * BOX_PAD; }
@@ -403,7 +514,6 @@ static void build_topology(vkrt_ctx *ctx, vkrt_bvh *bvh, const uint8_t *pos, uin
         bvh->range.reserve(ni * sizeof(uint2));
         bvh->parent.reserve(((size_t)ni + n) * sizeof(int32_t));
         bvh->nbox.reserve((size_t)ni * 6 * sizeof(float));
-        bvh->flags.reserve(ni * sizeof(uint32_t));
         int32_t *parent_int = bvh->parent.as<int32_t>(), *parent_leaf = parent_int + ni;
         if(bvh->builder == VKRT_BUILDER_PLOC) build_topology_ploc(ctx, bvh);
         else
@@ -792,11 +902,16 @@ static void build_bounds(vkrt_ctx *ctx, vkrt_bvh *bvh)
     if(n >= 2)
     {
         const uint32_t ni = n - 1;
-        int32_t *parent_int = bvh->parent.as<int32_t>(), *parent_leaf = parent_int + ni;
-        VKRT_HIP(hipMemsetAsync(bvh->flags.ptr, 0, ni * sizeof(uint32_t), s));
-        k_refit<<<div_up(n, 256), 256, 0, s>>>(n, bvh->child.as<int2>(), parent_int, parent_leaf,
-                                               bvh->tbox.as<float>(), bvh->nbox.as<float>(),
-                                               bvh->flags.as<uint32_t>());
+        // scratch: per-leaf prefix and suffix unions, then the segments' sparse table
+        const uint32_t nseg = div_up(n, BOX_SEG), levels = box_levels(nseg);
+        device_buffer &w = ctx->scratch[3];
+        w.reserve(((size_t)2 * n + (size_t)levels * nseg) * 6 * sizeof(float));
+        float *pre = w.as<float>(), *suf = pre + 6 * (size_t)n, *table = suf + 6 * (size_t)n;
+        k_box_scan<<<nseg, BOX_SEG, 0, s>>>(n, bvh->tbox.as<float>(), pre, suf, table);
+        for(uint32_t k = 1; k < levels; ++k)
+            k_box_sparse<<<div_up(nseg, 256), 256, 0, s>>>(nseg, k, table);
+        k_box_nodes<<<div_up(ni, 256), 256, 0, s>>>(ni, n, nseg, bvh->range.as<uint2>(), bvh->tbox.as<float>(), pre,
+                                                     suf, table, bvh->nbox.as<float>());
         bvh->n_nodes = ni;
         bvh->nodes.reserve((size_t)ni * sizeof(vkrt_bvh_node));
         k_layout<<<div_up(ni, 256), 256, 0, s>>>(n, bvh->child.as<int2>(), bvh->range.as<uint2>(),

@@ -39,7 +39,7 @@ struct vkrt_bvh
     int32_t root4 = 0;
     uint32_t n_nodes4 = 0, stack4 = 0;
     bool wide = false, allow_wide = true;
-    vkrt::device_buffer indices, order, child, range, parent, nbox, tbox, flags, pre, err;
+    vkrt::device_buffer indices, order, child, range, parent, nbox, tbox, pre, err;
     vkrt_bvh_info info{};
 };
 
