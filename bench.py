@@ -787,6 +787,16 @@ def main():
         any_ms.append(ev.ms(3, 4))
     any_ms = float(np.mean(any_ms))
 
+    # refit (the update path of a deforming mesh) with the same device-resident positions, after the timed
+    # steps: boxes, node layout and the 4-wide boxes recomputed, the topology kept
+    d_pos = vk.DeviceArray.from_host(ctx, pos)
+    bvh.refit(d_pos)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        bvh.refit(d_pos)
+    refit_ms = (time.perf_counter() - t0) * 1e3 / 5
+    d_pos.free()
+
     launch_ms = float(np.mean(prim_ms + bounce_ms))
     traffic, traffic_source = profiled_traffic("k_trace_closest")
     valu, _ = profiled("k_trace_closest", "valu_insts_per_launch")
@@ -812,7 +822,7 @@ def main():
             "bvh": {"builder": args.builder, "ploc_radius": args.ploc_radius or 16, "max_leaf": MAX_LEAF, "nodes": int(info.n_nodes), "depth": int(info.depth),
                     "wide_nodes": int(info.n_wide_nodes), "wide_stack_bound": int(info.wide_stack),
                     "sah_cost": round(float(info.sah_cost), 2), "build_ms": round(float(info.build_ms), 2),
-                    "build_wall_ms": round(build_wall_ms, 2)},
+                    "build_wall_ms": round(build_wall_ms, 2), "refit_ms": round(refit_ms, 3)},
             "primary_ms": round(float(np.mean(prim_ms)), 4),
             "bounce_ms": round(float(np.mean(bounce_ms)), 4),
             "primary_grays_per_s": round(len(primary) / (np.mean(prim_ms) * 1e-3) / 1e9, 4),

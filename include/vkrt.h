@@ -215,6 +215,8 @@ typedef struct vkrt_bvh_params
     uint32_t reserved[4];
 } vkrt_bvh_params;
 
+/* depth, n_leaves and sah_cost are computed when the BVH is built; a refit keeps the topology and updates
+   bounds (and build_ms stays the build's time) */
 typedef struct vkrt_bvh_info
 {
     uint32_t n_tri;

@@ -130,6 +130,13 @@ def test_refit_keeps_the_4wide_shape(ctx, icosphere_case):
     after = bvh.info()
     assert (after.n_wide_nodes, after.wide_stack) == (before.n_wide_nodes, before.wide_stack)
     assert after.n_wide_nodes > 0
+    # the refit's bounds hold the deformed mesh, padded by at most 2^-16 relative; depth and SAH cost stay
+    # the build's statistics
+    b = np.array(after.bounds[:6], np.float64)
+    lo, hi = skewed.min(axis=0).astype(np.float64), skewed.max(axis=0).astype(np.float64)
+    assert np.all(b[:3] <= lo) and np.all(b[3:] >= hi)
+    assert np.allclose(b, np.concatenate([lo, hi]), rtol=2 ** -15, atol=0)
+    assert (after.depth, after.sah_cost) == (before.depth, before.sah_cost)
     want2 = orc.brute_closest(W.flat_triangles(skewed, idx), rays[:8192])
     assert_hits_equal(ctx.trace_closest(bvh, rays[:8192]), want2, "skewed refit")
 

@@ -392,6 +392,19 @@ __global__ void k_reorder(const uint32_t *__restrict__ order, uint32_t n, const 
 // ---------------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------------
+//! the root's box from its two child boxes: rb as (lo, hi) pairs per axis, bounds as lo xyz then hi xyz
+static void root_bounds(const vkrt_bvh_node &root, float rb[6], float bounds[6])
+{
+    const float r[6] = {std::min(root.a.x, root.b.x), std::max(root.a.y, root.b.y), std::min(root.a.z, root.b.z),
+                        std::max(root.a.w, root.b.w), std::min(root.c.x, root.c.z), std::max(root.c.y, root.c.w)};
+    for(int a = 0; a < 6; ++a) rb[a] = r[a];
+    for(int a = 0; a < 3; ++a)
+    {
+        bounds[a] = r[2 * a];
+        bounds[3 + a] = r[2 * a + 1];
+    }
+}
+
 static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
 {
     // depth and SAH cost from the final nodes (build-time statistics only)
@@ -432,14 +445,8 @@ static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
                dz = std::max(0.0, (double)hz - lz);
         return 2.0 * (dx * dy + dy * dz + dz * dx);
     };
-    const vkrt_bvh_node &root = nodes[0];
-    float rb[6] = {std::min(root.a.x, root.b.x), std::max(root.a.y, root.b.y), std::min(root.a.z, root.b.z),
-                   std::max(root.a.w, root.b.w), std::min(root.c.x, root.c.z), std::max(root.c.y, root.c.w)};
-    for(int a = 0; a < 3; ++a)
-    {
-        info.bounds[a] = rb[2 * a];
-        info.bounds[3 + a] = rb[2 * a + 1];
-    }
+    float rb[6];
+    root_bounds(nodes[0], rb, info.bounds);
     double root_area = std::max(1e-30, area(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5]));
     double sah = 1.0;// root traversal
     uint32_t leaves = 0, max_depth = 1;
@@ -1122,14 +1129,25 @@ static void build_wide(vkrt_ctx *ctx, vkrt_bvh *bvh)
                 (uint64_t)bvh->n_nodes4 * sizeof(vkrt_bvh4_node) <= 0xffffffffull;
 }
 
-static void refresh_info(vkrt_bvh *bvh, hipStream_t s)
+//! the info block after a build (stats: depth, leaf count and SAH cost from all nodes, read back to the host)
+//! or after a refit, which keeps the topology: only the bounds change, read from the root node
+static void refresh_info(vkrt_bvh *bvh, hipStream_t s, bool stats)
 {
     bvh->info.n_tri = bvh->n_tri;
     bvh->info.n_nodes = bvh->n_nodes;
     bvh->info.device_bytes = bvh->nodes.bytes + bvh->tris.bytes + bvh->nodes4.bytes;
     bvh->info.n_wide_nodes = bvh->wide ? bvh->n_nodes4 : 0;
     bvh->info.wide_stack = bvh->stack4;
-    compute_stats(bvh, s);
+    if(stats || bvh->empty || bvh->root_ref < 0)
+    {
+        compute_stats(bvh, s);// (a single leaf or an empty BVH reads no node array)
+        return;
+    }
+    vkrt_bvh_node root;
+    VKRT_HIP(hipMemcpyAsync(&root, bvh->nodes.ptr, sizeof root, hipMemcpyDeviceToHost, s));
+    VKRT_HIP(hipStreamSynchronize(s));
+    float rb[6];
+    root_bounds(root, rb, bvh->info.bounds);
 }
 
 }// namespace vkrt
@@ -1179,7 +1197,7 @@ int vkrt_bvh_build(vkrt_ctx *ctx, const float *positions, uint32_t n_vert, uint3
             VKRT_HIP(hipStreamSynchronize(ctx->stream));
         }
         bvh->info.build_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-        refresh_info(bvh.get(), ctx->stream);
+        refresh_info(bvh.get(), ctx->stream, true);
         *out = bvh.release();
     });
 }
@@ -1206,7 +1224,7 @@ int vkrt_bvh_refit(vkrt_ctx *ctx, vkrt_bvh *bvh, const float *positions, uint32_
         build_bounds(ctx, bvh);
         refit_wide(ctx, bvh);
         VKRT_HIP(hipStreamSynchronize(s));
-        refresh_info(bvh, s);
+        refresh_info(bvh, s, false);
     });
 }
 
