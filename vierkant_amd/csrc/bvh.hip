@@ -392,6 +392,70 @@ __global__ void k_reorder(const uint32_t *__restrict__ order, uint32_t n, const 
 // ---------------------------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------------------------
+//! build statistics over the final BVH2 nodes. A node stays internal in the layout when its run holds more
+//! than max_leaf triangles (every ancestor's run then does too), so those nodes and the root are the ones a
+//! traversal reaches. Each counts its depth up the parent chain (the root is 1), and adds its children's SAH
+//! terms (box area; a leaf's times its triangle count, divided by the root's area on the host), its leaf
+//! children and their depth. Sums go through a wave reduction and one atomic per wave.
+struct bvh_stats_dev
+{
+    double sah;
+    uint32_t leaves, depth;
+};
+
+__device__ __forceinline__ double child_area(float lx, float hx, float ly, float hy, float lz, float hz)
+{
+    const double dx = fmax(0.0, (double)hx - lx), dy = fmax(0.0, (double)hy - ly), dz = fmax(0.0, (double)hz - lz);
+    return 2.0 * (dx * dy + dy * dz + dz * dx);
+}
+
+__global__ void k_stats(uint32_t ni, uint32_t max_leaf, const vkrt_bvh_node *__restrict__ nodes,
+                        const uint2 *__restrict__ range, const int32_t *__restrict__ parent_int,
+                        bvh_stats_dev *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    double sah = 0.0;
+    uint32_t leaves = 0, depth = 0;
+    if(i < ni)
+    {
+        const uint2 r = range[i];
+        if(i == 0 || r.y - r.x + 1 > max_leaf)
+        {
+            // (past STACK_SIZE + 1 the walk stops: the host rejects any depth above STACK_SIZE)
+            uint32_t d = 1;
+            for(int32_t p = static_cast<int32_t>(i); p > 0 && d <= STACK_SIZE + 1; ++d) p = parent_int[p];
+            const vkrt_bvh_node nd = nodes[i];
+            const int32_t refs[2] = {nd.d.x, nd.d.y};
+            const double ar[2] = {child_area(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y),
+                                  child_area(nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w)};
+            depth = d;
+            for(int k = 0; k < 2; ++k)
+            {
+                if(refs[k] >= 0) sah += ar[k];
+                else
+                {
+                    const uint32_t v = static_cast<uint32_t>(~refs[k]);
+                    sah += ((v & 7u) + 1) * ar[k];
+                    ++leaves;
+                    depth = d + 1;
+                }
+            }
+        }
+    }
+    for(int off = 32; off > 0; off >>= 1)
+    {
+        sah += __shfl_xor(sah, off);
+        leaves += __shfl_xor(leaves, off);
+        depth = max(depth, (uint32_t)__shfl_xor(depth, off));
+    }
+    if((threadIdx.x & 63u) == 0 && (leaves || depth))
+    {
+        atomicAdd(&out->sah, sah);
+        atomicAdd(&out->leaves, leaves);
+        atomicMax(&out->depth, depth);
+    }
+}
+
 //! the root's box from its two child boxes: rb as (lo, hi) pairs per axis, bounds as lo xyz then hi xyz
 static void root_bounds(const vkrt_bvh_node &root, float rb[6], float bounds[6])
 {
@@ -436,51 +500,26 @@ static void compute_stats(vkrt_bvh *bvh, hipStream_t s)
         }
         return;
     }
-    std::vector<vkrt_bvh_node> nodes(bvh->n_nodes);
-    VKRT_HIP(hipMemcpyAsync(nodes.data(), bvh->nodes.ptr, nodes.size() * sizeof(vkrt_bvh_node), hipMemcpyDeviceToHost,
-                            s));
+    // the sums on the GPU (a wave reduction and one atomic per wave), the root node and the sums read back
+    bvh_stats_dev *dst = reinterpret_cast<bvh_stats_dev *>(bvh->err.as<uint8_t>() + 32);
+    VKRT_HIP(hipMemsetAsync(dst, 0, sizeof(bvh_stats_dev), s));
+    k_stats<<<div_up(bvh->n_nodes, 256), 256, 0, s>>>(bvh->n_nodes, bvh->max_leaf, bvh->nodes.as<vkrt_bvh_node>(),
+                                                      bvh->range.as<uint2>(), bvh->parent.as<int32_t>(), dst);
+    VKRT_HIP(hipGetLastError());
+    bvh_stats_dev st;
+    vkrt_bvh_node root;
+    VKRT_HIP(hipMemcpyAsync(&st, dst, sizeof st, hipMemcpyDeviceToHost, s));
+    VKRT_HIP(hipMemcpyAsync(&root, bvh->nodes.ptr, sizeof root, hipMemcpyDeviceToHost, s));
     VKRT_HIP(hipStreamSynchronize(s));
-    auto area = [](float lx, float hx, float ly, float hy, float lz, float hz) {
-        double dx = std::max(0.0, (double)hx - lx), dy = std::max(0.0, (double)hy - ly),
-               dz = std::max(0.0, (double)hz - lz);
-        return 2.0 * (dx * dy + dy * dz + dz * dx);
-    };
     float rb[6];
-    root_bounds(nodes[0], rb, info.bounds);
-    double root_area = std::max(1e-30, area(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5]));
-    double sah = 1.0;// root traversal
-    uint32_t leaves = 0, max_depth = 1;
-    std::vector<std::pair<int32_t, uint32_t>> todo{{0, 1}};
-    while(!todo.empty())
-    {
-        auto [ref, depth] = todo.back();
-        todo.pop_back();
-        max_depth = std::max(max_depth, depth);
-        if(depth > STACK_SIZE) throw error(VKRT_E_INTERNAL, "BVH deeper than the traversal stack");
-        const vkrt_bvh_node &nd = nodes[ref];
-        const int refs[2] = {nd.d.x, nd.d.y};
-        const double ar[2] = {area(nd.a.x, nd.a.y, nd.a.z, nd.a.w, nd.c.x, nd.c.y),
-                              area(nd.b.x, nd.b.y, nd.b.z, nd.b.w, nd.c.z, nd.c.w)};
-        for(int k = 0; k < 2; ++k)
-        {
-            if(refs[k] >= 0)
-            {
-                sah += 1.0 * ar[k] / root_area;
-                todo.push_back({refs[k], depth + 1});
-            }
-            else
-            {
-                uint32_t v = static_cast<uint32_t>(~refs[k]);
-                sah += 1.0 * ((v & 7u) + 1) * ar[k] / root_area;
-                ++leaves;
-                max_depth = std::max(max_depth, depth + 1);
-            }
-        }
-    }
-    info.depth = max_depth;
-    info.sah_cost = static_cast<float>(sah);
-    info.n_leaves = leaves;
-    if(max_depth > STACK_SIZE) throw error(VKRT_E_INTERNAL, "BVH deeper than the traversal stack");
+    root_bounds(root, rb, info.bounds);
+    const double dx = std::max(0.0, (double)rb[1] - rb[0]), dy = std::max(0.0, (double)rb[3] - rb[2]),
+                 dz = std::max(0.0, (double)rb[5] - rb[4]);
+    const double root_area = std::max(1e-30, 2.0 * (dx * dy + dy * dz + dz * dx));
+    info.depth = std::max(1u, st.depth);
+    info.sah_cost = static_cast<float>(1.0 + st.sah / root_area);// 1: the root's own visit
+    info.n_leaves = st.leaves;
+    if(info.depth > (uint32_t)STACK_SIZE) throw error(VKRT_E_INTERNAL, "BVH deeper than the traversal stack");
 }
 
 static void build_topology_ploc(vkrt_ctx *ctx, vkrt_bvh *bvh);
